@@ -1,0 +1,133 @@
+// ref_driver.cc -- extern "C" shim over the FFTW-free part of the reference
+// (Grid, Scalar, Flux, BC, BoundaryVector, Regularizer, Geometry, RigidBody,
+// State), compiled from the sources where they lie under the reference's src/
+// by oracle/Makefile into oracle/_ref/libibpm_refpart.so.
+//
+// TEST INFRASTRUCTURE ONLY: it pins the CPU restatement (ibpm_oracle.c)
+// against the reference's own code for coarsify/getBC, the regulariser, the
+// .geom parser with body motion, the uniform base flow and the net force.
+// The elliptic solvers, VectorOperations and the projection solvers include
+// FFTW3, which this image lacks, so they are not part of this build.
+#include "Grid.h"
+#include "Scalar.h"
+#include "Flux.h"
+#include "BC.h"
+#include "BoundaryVector.h"
+#include "Regularizer.h"
+#include "Geometry.h"
+#include "State.h"
+#include <cstring>
+#include <string>
+
+using namespace ibpm;
+
+namespace {
+void load_scalar(Scalar& s, const double* d) {
+    const Grid& g = s.getGrid();
+    long k = 0;
+    for (int lev = 0; lev < g.Ngrid(); ++lev)
+        for (int i = 1; i < g.Nx(); ++i)
+            for (int j = 1; j < g.Ny(); ++j) s(lev, i, j) = d[k++];
+}
+void store_scalar(const Scalar& s, double* d) {
+    const Grid& g = s.getGrid();
+    long k = 0;
+    for (int lev = 0; lev < g.Ngrid(); ++lev)
+        for (int i = 1; i < g.Nx(); ++i)
+            for (int j = 1; j < g.Ny(); ++j) d[k++] = s(lev, i, j);
+}
+}  // namespace
+
+extern "C" {
+
+void ref_coarsify(int nx, int ny, int ngrid, double length, double xoff, double yoff, double* data) {
+    Grid grid(nx, ny, ngrid, length, xoff, yoff);
+    Scalar s(grid);
+    load_scalar(s, data);
+    s.coarsify();
+    store_scalar(s, data);
+}
+
+void ref_getbc(int nx, int ny, int ngrid, double length, double xoff, double yoff, const double* data,
+               int lev, double* bc_out) {
+    Grid grid(nx, ny, ngrid, length, xoff, yoff);
+    Scalar s(grid);
+    load_scalar(s, data);
+    BC bc(nx, ny);
+    s.getBC(lev, bc);
+    // ring order of BC.h: left(0..ny), top(1..nx), right(ny-1..0), bottom(nx-1..1)
+    for (int j = 0; j <= ny; ++j) bc_out[j] = bc.left(j);
+    for (int i = 0; i <= nx; ++i) bc_out[ny + i] = bc.top(i);
+    for (int j = 0; j <= ny; ++j) bc_out[2 * ny + nx - j] = bc.right(j);
+    for (int i = 1; i < nx; ++i) bc_out[2 * (nx + ny) - i] = bc.bottom(i);
+}
+
+// Loads a .geom file, moves bodies to time t, writes points [x.., y..] and
+// velocities [u.., v..]; returns the number of points or -1.
+int ref_geometry(const char* path, double t, double* xy, double* uv, int cap) {
+    Geometry geom;
+    if (!geom.load(std::string(path))) return -1;
+    geom.moveBodies(t);
+    int n = geom.getNumPoints();
+    if (n > cap) return -2;
+    BoundaryVector p = geom.getPoints();
+    BoundaryVector v = geom.getVelocities();
+    for (int i = 0; i < n; ++i) {
+        xy[i] = p(X, i); xy[n + i] = p(Y, i);
+        uv[i] = v(X, i); uv[n + i] = v(Y, i);
+    }
+    return n;
+}
+
+int ref_geometry_stationary(const char* path) {
+    Geometry geom;
+    if (!geom.load(std::string(path))) return -1;
+    return geom.isStationary() ? 1 : 0;
+}
+
+// Regularizer on the geometry in `path` at time t: q_out = toFlux(f_in) (all
+// levels) and f_out = toBoundary(q_in).
+int ref_regularizer(int nx, int ny, int ngrid, double length, double xoff, double yoff,
+                    const char* path, double t, const double* f_in, double* q_out,
+                    const double* q_in, double* f_out) {
+    Grid grid(nx, ny, ngrid, length, xoff, yoff);
+    Geometry geom;
+    if (!geom.load(std::string(path))) return -1;
+    geom.moveBodies(t);
+    Regularizer reg(grid, geom);
+    reg.update();
+    int n = geom.getNumPoints();
+    BoundaryVector f(n);
+    for (int k = 0; k < 2 * n; ++k) f(k) = f_in[k];
+    Flux q = reg.toFlux(f);
+    long nf = 2L * nx * ny + nx + ny;
+    for (int lev = 0; lev < ngrid; ++lev)
+        for (long k = 0; k < nf; ++k) q_out[lev * nf + k] = q(lev, (int)k);
+    Flux qi(grid);
+    for (int lev = 0; lev < ngrid; ++lev)
+        for (long k = 0; k < nf; ++k) qi(lev, (int)k) = q_in[lev * nf + k];
+    BoundaryVector fo = reg.toBoundary(qi);
+    for (int k = 0; k < 2 * n; ++k) f_out[k] = fo(k);
+    return n;
+}
+
+// Flux::UniformFlow (Flux.cc:79-99) for magnitude 1 and angle alpha (radians).
+void ref_uniform_flow(int nx, int ny, int ngrid, double length, double xoff, double yoff,
+                      double alpha, double* q_out) {
+    Grid grid(nx, ny, ngrid, length, xoff, yoff);
+    Flux q = Flux::UniformFlow(grid, 1.0, alpha);
+    long nf = 2L * nx * ny + nx + ny;
+    for (int lev = 0; lev < ngrid; ++lev)
+        for (long k = 0; k < nf; ++k) q_out[lev * nf + k] = q(lev, (int)k);
+}
+
+// State::computeNetForce (State.cc:53-63).
+void ref_net_force(int nx, int ny, int ngrid, double length, double xoff, double yoff, int npts,
+                   const double* f, double* xf, double* yf) {
+    Grid grid(nx, ny, ngrid, length, xoff, yoff);
+    State x(grid, npts);
+    for (int k = 0; k < 2 * npts; ++k) x.f(k) = f[k];
+    x.computeNetForce(*xf, *yf);
+}
+
+}  // extern "C"

@@ -1,0 +1,280 @@
+"""ibpm_amd -- MI355X (gfx950) projection-step library of the immersed-boundary
+projection method, driven through the C ABI of include/ibpm_gpu.h.
+
+This module is a thin ctypes layer over ibpm_amd/libibpm_gpu.so.  It never
+falls back to a CPU path: if the library is missing or no GPU is visible the
+calls raise.
+"""
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libibpm_gpu.so")
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+_lib = None
+
+SCHEMES = {"euler": 0, "ab2": 1, "rk3": 2, "rk3b": 3}
+
+
+class IbpmError(RuntimeError):
+    pass
+
+
+class FlowParams(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("ngrid", ctypes.c_int),
+                ("length", ctypes.c_double), ("xoffset", ctypes.c_double), ("yoffset", ctypes.c_double),
+                ("reynolds", ctypes.c_double), ("dt", ctypes.c_double), ("scheme", ctypes.c_int),
+                ("alpha_deg", ctypes.c_double), ("cg_tol", ctypes.c_double), ("force_cg", ctypes.c_int),
+                ("geom", ctypes.c_char_p)]
+
+
+def load():
+    """Loads libibpm_gpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise IbpmError("libibpm_gpu.so not built: run `python -m ibpm_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    L.ibpm_last_error.restype = ctypes.c_char_p
+    L.ibpm_batch_time.restype = ctypes.c_double
+    L.ibpm_batch_time.argtypes = [ctypes.c_void_p]
+    vp = ctypes.c_void_p
+    for name, args in {
+        "ibpm_ctx_destroy": [vp], "ibpm_ctx_synchronize": [vp],
+        "ibpm_malloc": [vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)],
+        "ibpm_free": [vp, vp],
+        "ibpm_memcpy_h2d": [vp, vp, vp, ctypes.c_size_t],
+        "ibpm_memcpy_d2h": [vp, vp, vp, ctypes.c_size_t],
+        "ibpm_dst2d": [vp, ctypes.c_int, ctypes.c_int, vp, vp],
+        "ibpm_elliptic2d_solve": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                  ctypes.c_double, vp, vp, vp],
+        "ibpm_elliptic_solve": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_double, ctypes.c_double, vp, vp],
+        "ibpm_curl_flux": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, vp, vp],
+        "ibpm_curl_scalar": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                             ctypes.c_double, ctypes.c_double, vp, vp],
+        "ibpm_nonlinear": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, vp, vp, vp],
+        "ibpm_laplacian": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, vp, vp],
+        "ibpm_batch_create": [vp, ctypes.c_int, ctypes.POINTER(FlowParams), ctypes.POINTER(ctypes.c_void_p)],
+        "ibpm_batch_destroy": [vp], "ibpm_batch_init": [vp],
+        "ibpm_batch_advance": [vp, ctypes.c_int],
+        "ibpm_batch_num_points": [vp, ctypes.c_int],
+        "ibpm_batch_get_omega": [vp, ctypes.c_int, vp], "ibpm_batch_set_omega": [vp, ctypes.c_int, vp],
+        "ibpm_batch_get_q": [vp, ctypes.c_int, vp], "ibpm_batch_get_f": [vp, ctypes.c_int, vp],
+        "ibpm_batch_get_points": [vp, ctypes.c_int, vp],
+        "ibpm_batch_get_forces": [vp, ctypes.c_int, _D, _D],
+        "ibpm_batch_force_history": [vp, ctypes.c_int, vp, ctypes.c_int],
+        "ibpm_batch_cg_iters": [vp, ctypes.c_int, vp, ctypes.c_int],
+        "ibpm_batch_use_graph": [vp, ctypes.c_int],
+    }.items():
+        getattr(L, name).argtypes = args
+        getattr(L, name).restype = ctypes.c_int
+    L.device_count = L.ibpm_device_count
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc < 0:
+        raise IbpmError(load().ibpm_last_error().decode())
+    return rc
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class DeviceArray:
+    """A device buffer of doubles owned by a Context."""
+
+    def __init__(self, ctx, n):
+        self.ctx, self.n = ctx, int(n)
+        self.ptr = ctypes.c_void_p()
+        _check(load().ibpm_malloc(ctx.h, 8 * max(self.n, 1), ctypes.byref(self.ptr)))
+
+    @classmethod
+    def of(cls, ctx, host):
+        host = _c(host).ravel()
+        d = cls(ctx, host.size)
+        _check(load().ibpm_memcpy_h2d(ctx.h, d.ptr, host.ctypes.data, 8 * host.size))
+        return d
+
+    def get(self):
+        out = np.empty(self.n)
+        _check(load().ibpm_memcpy_d2h(self.ctx.h, out.ctypes.data, self.ptr, 8 * self.n))
+        return out
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and self.ptr.value:
+            load().ibpm_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+
+class Context:
+    """One GPU and one HIP stream (ibpm_ctx)."""
+
+    def __init__(self, device=0):
+        self.h = ctypes.c_void_p()
+        _check(load().ibpm_ctx_create(device, ctypes.byref(self.h)))
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            load().ibpm_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    # -- EllipticSolver2d / EllipticSolver ------------------------------
+    def dst2d(self, x):
+        x = _c(x)
+        n1, n2 = x.shape
+        d_in = DeviceArray.of(self, x)
+        d_out = DeviceArray(self, x.size)
+        _check(load().ibpm_dst2d(self.h, n1, n2, d_in.ptr, d_out.ptr))
+        return d_out.get().reshape(n1, n2)
+
+    def elliptic2d_solve(self, kind, nx, ny, dx, alpha, f, bc=None):
+        d_f = DeviceArray.of(self, f)
+        d_u = DeviceArray(self, (nx - 1) * (ny - 1))
+        d_bc = DeviceArray.of(self, bc) if bc is not None else None
+        _check(load().ibpm_elliptic2d_solve(self.h, kind, nx, ny, dx, alpha, d_f.ptr,
+                                            d_bc.ptr if d_bc else None, d_u.ptr))
+        return d_u.get()
+
+    def elliptic_solve(self, kind, nx, ny, ngrid, dx, alpha, f):
+        d_f = DeviceArray.of(self, f)
+        d_u = DeviceArray(self, ngrid * (nx - 1) * (ny - 1))
+        _check(load().ibpm_elliptic_solve(self.h, kind, nx, ny, ngrid, dx, alpha, d_f.ptr, d_u.ptr))
+        return d_u.get()
+
+    # -- VectorOperations ------------------------------------------------
+    def curl_flux(self, nx, ny, ngrid, dx, q):
+        d_q = DeviceArray.of(self, q)
+        d_w = DeviceArray(self, ngrid * (nx - 1) * (ny - 1))
+        _check(load().ibpm_curl_flux(self.h, nx, ny, ngrid, dx, d_q.ptr, d_w.ptr))
+        return d_w.get()
+
+    def curl_scalar(self, nx, ny, ngrid, dx, w, u0=0.0, v0=0.0):
+        d_w = DeviceArray.of(self, w)
+        d_q = DeviceArray(self, ngrid * (2 * nx * ny + nx + ny))
+        _check(load().ibpm_curl_scalar(self.h, nx, ny, ngrid, dx, u0, v0, d_w.ptr, d_q.ptr))
+        return d_q.get()
+
+    def laplacian(self, nx, ny, ngrid, dx, w):
+        d_w = DeviceArray.of(self, w)
+        d_o = DeviceArray(self, ngrid * (nx - 1) * (ny - 1))
+        _check(load().ibpm_laplacian(self.h, nx, ny, ngrid, dx, d_w.ptr, d_o.ptr))
+        return d_o.get()
+
+    def nonlinear(self, nx, ny, ngrid, dx, q, w):
+        d_q = DeviceArray.of(self, q)
+        d_w = DeviceArray.of(self, w)
+        d_n = DeviceArray(self, ngrid * (nx - 1) * (ny - 1))
+        _check(load().ibpm_nonlinear(self.h, nx, ny, ngrid, dx, d_q.ptr, d_w.ptr, d_n.ptr))
+        return d_n.get()
+
+
+@dataclass
+class Flow:
+    """Parameters of one flow, named as the ibpm command line names them."""
+    nx: int = 200
+    ny: int = 200
+    ngrid: int = 1
+    length: float = 4.0
+    xoffset: float = -2.0
+    yoffset: float = -2.0
+    Re: float = 100.0
+    dt: float = 0.02
+    geom: str = ""
+    scheme: str = "rk3"
+    alpha: float = 0.0
+    tol: float = 1e-7
+    force_cg: bool = False
+
+    def params(self):
+        return FlowParams(self.nx, self.ny, self.ngrid, self.length, self.xoffset, self.yoffset,
+                          self.Re, self.dt, SCHEMES[self.scheme], self.alpha, self.tol,
+                          int(self.force_cg), self.geom.encode())
+
+
+class Batch:
+    """Independent flows advanced together (IBSolver::advance per flow)."""
+
+    def __init__(self, ctx, flows):
+        self.ctx = ctx
+        self.flows = list(flows)
+        arr = (FlowParams * len(self.flows))(*[f.params() for f in self.flows])
+        self._keep = arr
+        self.h = ctypes.c_void_p()
+        _check(load().ibpm_batch_create(ctx.h, len(self.flows), arr, ctypes.byref(self.h)))
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            load().ibpm_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def init(self):
+        _check(load().ibpm_batch_init(self.h))
+        return self
+
+    def use_graph(self, on=True):
+        _check(load().ibpm_batch_use_graph(self.h, int(on)))
+        return self
+
+    def advance(self, nsteps=1):
+        _check(load().ibpm_batch_advance(self.h, nsteps))
+
+    def num_points(self, k=0):
+        return _check(load().ibpm_batch_num_points(self.h, k))
+
+    def _get(self, fn, k, n):
+        out = np.empty(n)
+        _check(getattr(load(), fn)(self.h, k, out.ctypes.data))
+        return out
+
+    def omega(self, k=0):
+        f = self.flows[k]
+        return self._get("ibpm_batch_get_omega", k, f.ngrid * (f.nx - 1) * (f.ny - 1))
+
+    def set_omega(self, w, k=0):
+        w = _c(w)
+        _check(load().ibpm_batch_set_omega(self.h, k, w.ctypes.data))
+
+    def q(self, k=0):
+        f = self.flows[k]
+        return self._get("ibpm_batch_get_q", k, f.ngrid * (2 * f.nx * f.ny + f.nx + f.ny))
+
+    def f(self, k=0):
+        return self._get("ibpm_batch_get_f", k, 2 * self.num_points(k))
+
+    def points(self, k=0):
+        return self._get("ibpm_batch_get_points", k, 2 * self.num_points(k))
+
+    def forces(self, k=0):
+        cd, cl = ctypes.c_double(), ctypes.c_double()
+        _check(load().ibpm_batch_get_forces(self.h, k, ctypes.byref(cd), ctypes.byref(cl)))
+        return cd.value, cl.value
+
+    def force_history(self, k=0, max_steps=1 << 16):
+        out = np.empty(2 * max_steps)
+        n = _check(load().ibpm_batch_force_history(self.h, k, out.ctypes.data, max_steps))
+        return out[:2 * n].reshape(n, 2)
+
+    def cg_iters(self, k=0):
+        it = np.zeros(4, dtype=np.int32)
+        n = _check(load().ibpm_batch_cg_iters(self.h, k, it.ctypes.data, 4))
+        return it[:n].copy()
+
+    @property
+    def time(self):
+        return load().ibpm_batch_time(self.h)

@@ -1,0 +1,54 @@
+"""Builds ibpm_amd/libibpm_gpu.so (gfx950 HIP kernels + host engine + C ABI)
+in-tree with hipcc.  Run: python -m ibpm_amd.build"""
+import concurrent.futures
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "libibpm_gpu.so")
+OBJ = os.path.join(HERE, "_build")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("IBPM_OFFLOAD_ARCH", "gfx950")
+
+DEVICE_SOURCES = ["elliptic.hip", "ops.hip"]
+HOST_SOURCES = ["engine.cpp", "geometry.cpp"]
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function"]
+# Packed FP32 instructions are kept out of every code object.
+DEVICE_FLAGS = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def _compile(src):
+    os.makedirs(OBJ, exist_ok=True)
+    obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+    path = os.path.join(CSRC, src)
+    if src.endswith(".hip"):
+        cmd = [HIPCC] + COMMON + DEVICE_FLAGS + ["-c", path, "-o", obj]
+    else:
+        cmd = [HIPCC] + COMMON + ["-c", path, "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    msgs = [l for l in (r.stdout + r.stderr).splitlines()
+            if "packed-fp32-ops" not in l and "not a recognized feature" not in l and l.strip()]
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed on %s:\n%s" % (src, "\n".join(msgs)))
+    return obj, msgs
+
+
+def build(verbose=False):
+    """Compiles every source (always) and links libibpm_gpu.so."""
+    with concurrent.futures.ThreadPoolExecutor(max_workers=4) as ex:
+        results = list(ex.map(_compile, DEVICE_SOURCES + HOST_SOURCES))
+    objs = [o for o, _ in results]
+    for _, msgs in results:
+        for m in msgs:
+            if verbose or "warning" in m:
+                print(m, file=sys.stderr)
+    tmp = OUT + ".tmp"
+    subprocess.check_call([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs)
+    os.replace(tmp, OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

@@ -1,0 +1,78 @@
+// common.h -- shared definitions of the gfx950 projection-step library.
+//
+// Device data layout (HBM), per flow:
+//   Scalar : [lev][i=1..nx-1][j=1..ny-1], j contiguous (reference Scalar.cc:81-89)
+//   Flux   : [lev][X (nx+1)*ny | Y nx*(ny+1)]       (reference Flux.cc:42-50)
+//   BoundaryVector : [X_0..X_{nb-1}, Y_0..Y_{nb-1}]  (reference BoundaryVector.h:65-69)
+// identical to the reference so host<->device copies are flat memcpys.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace ibpmg {
+
+constexpr int kThreads = 256;          // workgroup size of every kernel (4 wave64)
+constexpr int kFftLds = 2048;          // complex doubles staged in LDS per DST block (32 KB)
+constexpr int kMaxStages = 16;
+
+// Plan of one complex FFT of length M = 2(N+1) used for a DST-I of length N.
+struct FftPlan {
+    int N;          // DST length
+    int M;          // FFT length
+    int np;         // row pairs per workgroup (np * M <= kFftLds)
+    int nst;        // number of Stockham stages
+    int rad[kMaxStages];
+    const double2* tw;   // tw[k] = exp(-2 pi i k / M), k < M
+};
+
+// One level of one flow in a multi-domain elliptic solve.
+struct EllJob {
+    int nx, ny;              // cells; rows i=1..nx-1, columns j=1..ny-1
+    int lev, ngrid;
+    int kind;                // 0 Poisson, 1 Helmholtz
+    int epilogue;            // 0: u = v ; 1: u = W - v
+    double in_scale;         // multiplies the right-hand side (+1 or -1)
+    double bc_coef;          // 1/dx^2 (Poisson) or alpha/dx^2 (Helmholtz)
+    double alpha, bydx2, norm;
+    const double* f;         // level slice of the right-hand side
+    const double* R;         // level slice of the coarsified rhs (central region), lev >= 1
+    const double* uc;        // level slice lev+1 of the solution (BC source), lev < ngrid-1
+    const double* bcring;    // explicit BC ring 2(nx+ny) instead of uc (single-level API)
+    double* T1;              // scratch (ny-1) x (nx-1)
+    double* T2;              // scratch (nx-1) x (ny-1)
+    double* u;               // level slice of the solution
+    const double* W;         // epilogue 1 operand (level slice)
+    const double* cosx;      // cos(pi i / nx), i = 1..nx-1
+    const double* cosy;      // cos(pi j / ny), j = 1..ny-1
+    FftPlan pj;              // transform along j (length ny-1)
+    FftPlan pi;              // transform along i (length nx-1)
+};
+
+// Coarsify job: one level (lev >= 1) of one flow (reference Scalar.cc:46-63).
+struct CoarsifyJob {
+    int nx, ny, lev;
+    const double* f;         // whole field (all levels)
+    double* R;               // whole coarsified buffer (all levels; central regions used)
+};
+
+__host__ __device__ inline int nx_ext(int nx) { return nx / 4; }   // Grid.h:89-91, zero shift
+
+__device__ inline long sidx(int nx, int ny, int i, int j) {   // within a level slice
+    return (long)(i - 1) * (ny - 1) + (j - 1);
+}
+
+}  // namespace ibpmg
+
+#define IBPM_HIP_CHECK(expr)                                                        \
+    do {                                                                            \
+        hipError_t e_ = (expr);                                                     \
+        if (e_ != hipSuccess) {                                                     \
+            ibpmg::set_error("%s:%d %s: %s", __FILE__, __LINE__, #expr,             \
+                             hipGetErrorString(e_));                                \
+            return -1;                                                              \
+        }                                                                           \
+    } while (0)
+
+namespace ibpmg {
+void set_error(const char* fmt, ...);
+}

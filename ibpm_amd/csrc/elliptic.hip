@@ -1,0 +1,358 @@
+// elliptic.hip -- batched multi-domain DST-I Poisson/Helmholtz solves (FP64).
+//
+// Replaces EllipticSolver2d::solve (reference EllipticSolver2d.cc:57-185) and
+// the multi-domain driver EllipticSolver::solve (EllipticSolver.cc:47-73).
+//
+// One level solve u = S diag(e) S f * norm (S = DST-I matrix) is three passes,
+// each reading and writing the level once (6 words per grid point):
+//   pass 1: rows of the right-hand side (coarsify select, sign, boundary-value
+//           correction of getRHS fused in) -> DST along j -> transposed T1
+//   pass 2: rows of T1 -> DST along i -> eigenvalue scaling -> DST along k
+//           -> transposed T2
+//   pass 3: rows of T2 -> DST along l -> * 1/(4 nx ny) -> u (or u = W - .)
+// Each DST-I of length N is the imaginary part of a complex FFT of length
+// M = 2(N+1) of the odd extension; two rows share one FFT (real part row a,
+// imaginary part row b).  The FFT is a mixed-radix (2,3,4,5) Stockham
+// transform held in LDS (<= 2048 complex per workgroup), with each stage
+// staged through registers so one LDS buffer suffices.
+#include "ops.h"
+
+namespace ibpmg {
+
+__device__ inline double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+// -i * a
+__device__ inline double2 cmi(double2 a) { return make_double2(a.y, -a.x); }
+
+template <int R> __device__ inline void dft(double2* v);
+
+template <> __device__ inline void dft<2>(double2* v) {
+    double2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+}
+
+template <> __device__ inline void dft<3>(double2* v) {
+    const double h = 0.86602540378443864676;   // sin(2 pi / 3)
+    double2 t1 = cadd(v[1], v[2]);
+    double2 t2 = csub(v[1], v[2]);
+    double2 m = make_double2(v[0].x - 0.5 * t1.x, v[0].y - 0.5 * t1.y);
+    double2 r = cmi(t2);
+    v[0] = cadd(v[0], t1);
+    v[1] = make_double2(m.x + h * r.x, m.y + h * r.y);
+    v[2] = make_double2(m.x - h * r.x, m.y - h * r.y);
+}
+
+template <> __device__ inline void dft<4>(double2* v) {
+    double2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+    double2 s13 = cadd(v[1], v[3]), d13 = cmi(csub(v[1], v[3]));
+    v[0] = cadd(s02, s13);
+    v[1] = cadd(d02, d13);
+    v[2] = csub(s02, s13);
+    v[3] = csub(d02, d13);
+}
+
+template <> __device__ inline void dft<5>(double2* v) {
+    const double c1 = 0.30901699437494742410;    // cos(2 pi / 5)
+    const double c2 = -0.80901699437494742410;   // cos(4 pi / 5)
+    const double s1 = 0.95105651629515357212;    // sin(2 pi / 5)
+    const double s2 = 0.58778525229247312917;    // sin(4 pi / 5)
+    double2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+    double2 t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+    double2 a1 = make_double2(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
+    double2 a2 = make_double2(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
+    double2 b1 = make_double2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y);
+    double2 b2 = make_double2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y);
+    b1 = cmi(b1);   // -i b1
+    b2 = cmi(b2);
+    v[0] = cadd(v[0], cadd(t1, t2));
+    v[1] = cadd(a1, b1);
+    v[4] = csub(a1, b1);
+    v[2] = cadd(a2, b2);
+    v[3] = csub(a2, b2);
+}
+
+// One Stockham radix-R stage over np FFTs of length M stored back to back.
+template <int R>
+__device__ inline void fft_stage(double2* buf, int M, int np, int Ns, const double2* __restrict__ tw) {
+    constexpr int MAXB = (kFftLds / R + kThreads - 1) / kThreads;
+    const int MR = M / R;
+    const int total = np * MR;
+    const int tstep = M / (Ns * R);
+    double2 v[MAXB][R];
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+        const int g = threadIdx.x + b * kThreads;
+        if (g < total) {
+            const int p = g / MR;
+            const int j = g - p * MR;
+            const int jm = j % Ns;
+            const double2* x = buf + p * M;
+            v[b][0] = x[j];
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[b][r] = cmul(x[j + r * MR], tw[r * jm * tstep]);
+            dft<R>(v[b]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+        const int g = threadIdx.x + b * kThreads;
+        if (g < total) {
+            const int p = g / MR;
+            const int j = g - p * MR;
+            const int jm = j % Ns;
+            double2* x = buf + p * M;
+            const int d = (j / Ns) * Ns * R + jm;
+#pragma unroll
+            for (int r = 0; r < R; ++r) x[d + r * Ns] = v[b][r];
+        }
+    }
+    __syncthreads();
+}
+
+__device__ inline void fft_run(double2* buf, const FftPlan& P) {
+    int Ns = 1;
+    for (int s = 0; s < P.nst; ++s) {
+        const int R = P.rad[s];
+        switch (R) {
+        case 2: fft_stage<2>(buf, P.M, P.np, Ns, P.tw); break;
+        case 3: fft_stage<3>(buf, P.M, P.np, Ns, P.tw); break;
+        case 4: fft_stage<4>(buf, P.M, P.np, Ns, P.tw); break;
+        default: fft_stage<5>(buf, P.M, P.np, Ns, P.tw); break;
+        }
+        Ns *= R;
+    }
+}
+
+// ---------------------------------------------------------------- pass 1
+// getBC (reference Scalar.cc:91-127) evaluated at one boundary index.
+__device__ inline double bc_bottom(const EllJob& J, int i) {
+    if (J.bcring) return i == 0 ? J.bcring[0] : J.bcring[2 * (J.nx + J.ny) - i];
+    const int ii = i / 2 + nx_ext(J.nx), jj = nx_ext(J.ny);
+    const double a = J.uc[sidx(J.nx, J.ny, ii, jj)];
+    return (i & 1) ? 0.5 * (a + J.uc[sidx(J.nx, J.ny, ii + 1, jj)]) : a;
+}
+__device__ inline double bc_top(const EllJob& J, int i) {
+    if (J.bcring) return J.bcring[J.ny + i];
+    const int ii = i / 2 + nx_ext(J.nx), jj = J.ny / 2 + nx_ext(J.ny);
+    const double a = J.uc[sidx(J.nx, J.ny, ii, jj)];
+    return (i & 1) ? 0.5 * (a + J.uc[sidx(J.nx, J.ny, ii + 1, jj)]) : a;
+}
+__device__ inline double bc_left(const EllJob& J, int j) {
+    if (J.bcring) return J.bcring[j];
+    const int ii = nx_ext(J.nx), jj = j / 2 + nx_ext(J.ny);
+    const double a = J.uc[sidx(J.nx, J.ny, ii, jj)];
+    return (j & 1) ? 0.5 * (a + J.uc[sidx(J.nx, J.ny, ii, jj + 1)]) : a;
+}
+__device__ inline double bc_right(const EllJob& J, int j) {
+    if (J.bcring) return J.bcring[2 * J.ny + J.nx - j];
+    const int ii = J.nx / 2 + nx_ext(J.nx), jj = j / 2 + nx_ext(J.ny);
+    const double a = J.uc[sidx(J.nx, J.ny, ii, jj)];
+    return (j & 1) ? 0.5 * (a + J.uc[sidx(J.nx, J.ny, ii, jj + 1)]) : a;
+}
+
+// Right-hand side of one level at (i, j), 1-based: coarsified copy in the
+// central region of levels >= 1 (EllipticSolver.cc:54-55), then getRHS
+// (EllipticSolver2d.cc:125-144 / 166-185).
+__device__ inline double rhs_value(const EllJob& J, int i, int j) {
+    const int ex = nx_ext(J.nx), ey = nx_ext(J.ny);
+    const bool central = J.lev >= 1 && i > ex && i < J.nx / 2 + ex && j > ey && j < J.ny / 2 + ey;
+    const long k = sidx(J.nx, J.ny, i, j);
+    double v = (central ? J.R[k] : J.f[k]) * J.in_scale;
+    if (J.uc || J.bcring) {
+        const double c = J.bc_coef;
+        if (j == 1) v -= bc_bottom(J, i) * c;
+        if (j == J.ny - 1) v -= bc_top(J, i) * c;
+        if (i == 1) v -= bc_left(J, j) * c;
+        if (i == J.nx - 1) v -= bc_right(J, j) * c;
+    }
+    return v;
+}
+
+__global__ void __launch_bounds__(kThreads) k_ell_pass1(const EllJob* __restrict__ jobs) {
+    const EllJob& J = jobs[blockIdx.y];
+    const FftPlan& P = J.pj;
+    const int N1 = J.nx - 1, N2 = J.ny - 1;
+    const int rows = 2 * P.np;
+    const int r0 = blockIdx.x * rows;
+    if (r0 >= N1) return;
+    __shared__ double2 buf[kFftLds];
+    const int M = P.M, n = N2 + 1;
+    for (int idx = threadIdx.x; idx < P.np * N2; idx += kThreads) {
+        const int p = idx / N2, k = idx - p * N2;
+        const int ra = r0 + 2 * p, rb = ra + 1;
+        const double va = (ra < N1) ? rhs_value(J, ra + 1, k + 1) : 0.0;
+        const double vb = (rb < N1) ? rhs_value(J, rb + 1, k + 1) : 0.0;
+        buf[p * M + k + 1] = make_double2(va, vb);
+        buf[p * M + M - 1 - k] = make_double2(-va, -vb);
+    }
+    for (int p = threadIdx.x; p < P.np; p += kThreads) {
+        buf[p * M] = make_double2(0.0, 0.0);
+        buf[p * M + n] = make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    fft_run(buf, P);
+    for (int idx = threadIdx.x; idx < rows * N2; idx += kThreads) {
+        const int r = idx % rows, k = idx / rows;
+        const int row = r0 + r;
+        if (row >= N1) continue;
+        const double2 z = buf[(r >> 1) * M + k + 1];
+        J.T1[(long)k * N1 + row] = (r & 1) ? z.x : -z.y;
+    }
+}
+
+// ---------------------------------------------------------------- pass 2
+__global__ void __launch_bounds__(kThreads) k_ell_pass2(const EllJob* __restrict__ jobs) {
+    const EllJob& J = jobs[blockIdx.y];
+    const FftPlan& P = J.pi;
+    const int N1 = J.nx - 1, N2 = J.ny - 1;
+    const int rows = 2 * P.np;
+    const int r0 = blockIdx.x * rows;   // rows are l = 0..N2-1
+    if (r0 >= N2) return;
+    __shared__ double2 buf[kFftLds];
+    const int M = P.M, n = N1 + 1;
+    for (int idx = threadIdx.x; idx < P.np * N1; idx += kThreads) {
+        const int p = idx / N1, k = idx - p * N1;
+        const int la = r0 + 2 * p, lb = la + 1;
+        const double va = (la < N2) ? J.T1[(long)la * N1 + k] : 0.0;
+        const double vb = (lb < N2) ? J.T1[(long)lb * N1 + k] : 0.0;
+        buf[p * M + k + 1] = make_double2(va, vb);
+        buf[p * M + M - 1 - k] = make_double2(-va, -vb);
+    }
+    for (int p = threadIdx.x; p < P.np; p += kThreads) {
+        buf[p * M] = make_double2(0.0, 0.0);
+        buf[p * M + n] = make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    fft_run(buf, P);
+    // spectral scaling, EllipticSolver2d.cc:42-54 (eigenvalues), :89-93 (scale)
+    for (int idx = threadIdx.x; idx < P.np * N1; idx += kThreads) {
+        const int p = idx / N1, k = idx - p * N1;
+        const int la = r0 + 2 * p, lb = la + 1;
+        const double2 z = buf[p * M + k + 1];
+        double ya = -z.y, yb = z.x;
+        const double cx = J.cosx[k];
+        if (la < N2) {
+            const double lam = 2. * (cx + J.cosy[la] - 2.) * J.bydx2;
+            ya *= (J.kind == 0) ? 1. / lam : 1. / (1 + J.alpha * lam);
+        }
+        if (lb < N2) {
+            const double lam = 2. * (cx + J.cosy[lb] - 2.) * J.bydx2;
+            yb *= (J.kind == 0) ? 1. / lam : 1. / (1 + J.alpha * lam);
+        }
+        buf[p * M + k + 1] = make_double2(ya, yb);
+        buf[p * M + M - 1 - k] = make_double2(-ya, -yb);
+    }
+    for (int p = threadIdx.x; p < P.np; p += kThreads) {
+        buf[p * M] = make_double2(0.0, 0.0);
+        buf[p * M + n] = make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    fft_run(buf, P);
+    for (int idx = threadIdx.x; idx < rows * N1; idx += kThreads) {
+        const int r = idx % rows, k = idx / rows;
+        const int l = r0 + r;
+        if (l >= N2) continue;
+        const double2 z = buf[(r >> 1) * M + k + 1];
+        J.T2[(long)k * N2 + l] = (r & 1) ? z.x : -z.y;
+    }
+}
+
+// ---------------------------------------------------------------- pass 3
+__global__ void __launch_bounds__(kThreads) k_ell_pass3(const EllJob* __restrict__ jobs) {
+    const EllJob& J = jobs[blockIdx.y];
+    const FftPlan& P = J.pj;
+    const int N1 = J.nx - 1, N2 = J.ny - 1;
+    const int rows = 2 * P.np;
+    const int r0 = blockIdx.x * rows;
+    if (r0 >= N1) return;
+    __shared__ double2 buf[kFftLds];
+    const int M = P.M, n = N2 + 1;
+    for (int idx = threadIdx.x; idx < P.np * N2; idx += kThreads) {
+        const int p = idx / N2, k = idx - p * N2;
+        const int ra = r0 + 2 * p, rb = ra + 1;
+        const double va = (ra < N1) ? J.T2[(long)ra * N2 + k] : 0.0;
+        const double vb = (rb < N1) ? J.T2[(long)rb * N2 + k] : 0.0;
+        buf[p * M + k + 1] = make_double2(va, vb);
+        buf[p * M + M - 1 - k] = make_double2(-va, -vb);
+    }
+    for (int p = threadIdx.x; p < P.np; p += kThreads) {
+        buf[p * M] = make_double2(0.0, 0.0);
+        buf[p * M + n] = make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    fft_run(buf, P);
+    for (int idx = threadIdx.x; idx < P.np * N2; idx += kThreads) {
+        const int p = idx / N2, k = idx - p * N2;
+        const double2 z = buf[p * M + k + 1];
+        const int ra = r0 + 2 * p, rb = ra + 1;
+        // sinTransformInv normalisation, EllipticSolver2d.cc:80-83
+        if (ra < N1) {
+            const long o = (long)ra * N2 + k;
+            const double v = -z.y * J.norm;
+            J.u[o] = J.epilogue ? J.W[o] - v : v;
+        }
+        if (rb < N1) {
+            const long o = (long)rb * N2 + k;
+            const double v = z.x * J.norm;
+            J.u[o] = J.epilogue ? J.W[o] - v : v;
+        }
+    }
+}
+
+// --------------------------------------------------------------- coarsify
+// Scalar::coarsify (reference Scalar.cc:46-63) for one level, writing only
+// the central region into R; neighbours at level lev-1 come from R where the
+// finer level is itself coarsified (central region of a level >= 1) and from
+// the original field elsewhere.
+__device__ inline double coarse_src(const CoarsifyJob& J, int lf, int i, int j) {
+    const int nx = J.nx, ny = J.ny;
+    const int ex = nx_ext(nx), ey = nx_ext(ny);
+    const long off = (long)lf * (nx - 1) * (ny - 1) + sidx(nx, ny, i, j);
+    const bool central = lf >= 1 && i > ex && i < nx / 2 + ex && j > ey && j < ny / 2 + ey;
+    return central ? J.R[off] : J.f[off];
+}
+
+__global__ void __launch_bounds__(kThreads) k_coarsify(const CoarsifyJob* __restrict__ jobs) {
+    const CoarsifyJob& J = jobs[blockIdx.y];
+    const int nx = J.nx, ny = J.ny;
+    const int ex = nx_ext(nx), ey = nx_ext(ny);
+    const int wi = nx / 2 - 1, wj = ny / 2 - 1;
+    const int t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= wi * wj) return;
+    const int i = ex + 1 + t / wj, j = ey + 1 + t % wj;
+    const int ii = (i - ex) * 2, jj = (j - ey) * 2;   // Grid::c2f
+    const int lf = J.lev - 1;
+    const double v = 0.25 * coarse_src(J, lf, ii, jj)
+        + 0.125 * (coarse_src(J, lf, ii + 1, jj) + coarse_src(J, lf, ii, jj + 1)
+                 + coarse_src(J, lf, ii - 1, jj) + coarse_src(J, lf, ii, jj - 1))
+        + 0.0625 * (coarse_src(J, lf, ii + 1, jj + 1) + coarse_src(J, lf, ii + 1, jj - 1)
+                  + coarse_src(J, lf, ii - 1, jj + 1) + coarse_src(J, lf, ii - 1, jj - 1));
+    J.R[(long)J.lev * (nx - 1) * (ny - 1) + sidx(nx, ny, i, j)] = v;
+}
+
+// ------------------------------------------------------------ host launch
+int blocks_for_pass(const EllJob& J, int pass) {
+    const int rows_total = (pass == 2) ? J.ny - 1 : J.nx - 1;
+    const FftPlan& P = (pass == 2) ? J.pi : J.pj;
+    const int rows = 2 * P.np;
+    return (rows_total + rows - 1) / rows;
+}
+
+void launch_ell_pass(int pass, const EllJob* d_jobs, int njobs, int max_blocks, hipStream_t s) {
+    dim3 grid(max_blocks, njobs);
+    if (pass == 1) hipLaunchKernelGGL(k_ell_pass1, grid, dim3(kThreads), 0, s, d_jobs);
+    else if (pass == 2) hipLaunchKernelGGL(k_ell_pass2, grid, dim3(kThreads), 0, s, d_jobs);
+    else hipLaunchKernelGGL(k_ell_pass3, grid, dim3(kThreads), 0, s, d_jobs);
+}
+
+void launch_coarsify(const CoarsifyJob* d_jobs, int njobs, int max_points, hipStream_t s) {
+    dim3 grid((max_points + kThreads - 1) / kThreads, njobs);
+    hipLaunchKernelGGL(k_coarsify, grid, dim3(kThreads), 0, s, d_jobs);
+}
+
+}  // namespace ibpmg

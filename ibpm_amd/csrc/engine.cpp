@@ -1,0 +1,1101 @@
+// engine.cpp -- host orchestration of the projection step on one GPU and the
+// C ABI declared in include/ibpm_gpu.h.
+//
+// A batch holds independent flows (uneven grids allowed).  Every kernel launch
+// covers all flows of the batch; per-flow descriptors live in device memory
+// and are built once at set-up, so a stationary-body timestep is a fixed
+// launch sequence that is captured as one hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/ibpm_gpu.h"
+#include "geometry.h"
+#include "ops.h"
+
+namespace ibpmg {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+}
+
+#define CK(expr)                                                                    \
+    do {                                                                            \
+        hipError_t e_ = (expr);                                                     \
+        if (e_ != hipSuccess) {                                                     \
+            set_error("%s:%d %s: %s", __FILE__, __LINE__, #expr, hipGetErrorString(e_)); \
+            return -1;                                                              \
+        }                                                                           \
+    } while (0)
+#define CKR(expr)                  \
+    do {                           \
+        int r_ = (expr);           \
+        if (r_ != 0) return r_;    \
+    } while (0)
+
+// Device allocation owned by a list (freed with the owner).
+struct DevMem {
+    std::vector<void*> ptrs;
+    ~DevMem() {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+    template <class T>
+    int alloc(T** out, size_t count) {
+        void* p = nullptr;
+        if (count == 0) count = 1;
+        CK(hipMalloc(&p, count * sizeof(T)));
+        CK(hipMemset(p, 0, count * sizeof(T)));
+        ptrs.push_back(p);
+        *out = (T*)p;
+        return 0;
+    }
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::map<int, FftPlan> plans;   // by DST length
+    DevMem mem;
+};
+
+static int get_plan(Ctx* c, int N, FftPlan* out) {
+    auto it = c->plans.find(N);
+    if (it != c->plans.end()) { *out = it->second; return 0; }
+    FftPlan P{};
+    P.N = N;
+    P.M = 2 * (N + 1);
+    if (P.M > kFftLds) { set_error("DST length %d exceeds the LDS plan limit (%d)", N, kFftLds / 2 - 1); return -2; }
+    int m = P.M;
+    while (m % 4 == 0) { P.rad[P.nst++] = 4; m /= 4; }
+    while (m % 2 == 0) { P.rad[P.nst++] = 2; m /= 2; }
+    while (m % 3 == 0) { P.rad[P.nst++] = 3; m /= 3; }
+    while (m % 5 == 0) { P.rad[P.nst++] = 5; m /= 5; }
+    if (m != 1) { set_error("DST length %d: 2(N+1) has a prime factor > 5", N); return -2; }
+    P.np = std::max(1, kFftLds / P.M);
+    std::vector<double2> tw(P.M);
+    const double twopi = 8.0 * std::atan(1.0);
+    for (int k = 0; k < P.M; ++k) {
+        const double a = -twopi * (double)k / (double)P.M;
+        tw[k] = make_double2(std::cos(a), std::sin(a));
+    }
+    double2* d = nullptr;
+    CKR(c->mem.alloc(&d, P.M));
+    CK(hipMemcpy(d, tw.data(), sizeof(double2) * P.M, hipMemcpyHostToDevice));
+    P.tw = d;
+    c->plans[N] = P;
+    *out = P;
+    return 0;
+}
+
+// ------------------------------------------------------------ solve plans
+// One multi-domain solve of one flow.
+struct SolveSpec {
+    int nx, ny, L;
+    double dx;             // finest spacing
+    int kind;
+    double alpha;
+    const double* f;
+    double* u;
+    double* R;             // coarsify buffer (L levels)
+    double* T1;
+    double* T2;
+    double in_scale;
+    int epilogue;
+    const double* W;
+    const double* cosx;
+    const double* cosy;
+    const double* bcring;  // single-level explicit boundary values (L == 1 only)
+};
+
+struct SolvePlan {
+    int nlev = 0;                                   // max L over flows
+    std::vector<CoarsifyJob*> cj;                   // lev = 1..nlev-1
+    std::vector<int> cj_n, cj_pts;
+    std::vector<EllJob*> ej;                        // step s = 0..nlev-1
+    std::vector<int> ej_n;
+    std::vector<int> maxb[3];
+};
+
+static int build_solve_plan(Ctx* c, DevMem& mem, const std::vector<SolveSpec>& specs, SolvePlan& P) {
+    P.nlev = 0;
+    for (auto& s : specs) P.nlev = std::max(P.nlev, s.L);
+    P.cj.assign(P.nlev, nullptr);
+    P.cj_n.assign(P.nlev, 0);
+    P.cj_pts.assign(P.nlev, 0);
+    for (int lev = 1; lev < P.nlev; ++lev) {
+        std::vector<CoarsifyJob> jobs;
+        int pts = 0;
+        for (auto& s : specs) {
+            if (lev >= s.L) continue;
+            jobs.push_back(CoarsifyJob{s.nx, s.ny, lev, s.f, s.R});
+            pts = std::max(pts, (s.nx / 2 - 1) * (s.ny / 2 - 1));
+        }
+        CoarsifyJob* d = nullptr;
+        CKR(mem.alloc(&d, jobs.size()));
+        CK(hipMemcpy(d, jobs.data(), sizeof(CoarsifyJob) * jobs.size(), hipMemcpyHostToDevice));
+        P.cj[lev] = d;
+        P.cj_n[lev] = (int)jobs.size();
+        P.cj_pts[lev] = pts;
+    }
+    P.ej.assign(P.nlev, nullptr);
+    P.ej_n.assign(P.nlev, 0);
+    for (int k = 0; k < 3; ++k) P.maxb[k].assign(P.nlev, 0);
+    for (int st = 0; st < P.nlev; ++st) {
+        std::vector<EllJob> jobs;
+        for (auto& s : specs) {
+            if (st >= s.L) continue;
+            const int lev = s.L - 1 - st;
+            const long ns = (long)(s.nx - 1) * (s.ny - 1);
+            const double dxl = s.dx * (1 << lev);
+            EllJob J{};
+            J.nx = s.nx; J.ny = s.ny; J.lev = lev; J.ngrid = s.L;
+            J.kind = s.kind; J.epilogue = s.epilogue; J.in_scale = s.in_scale;
+            J.alpha = s.alpha;
+            J.bydx2 = 1. / (dxl * dxl);                                  // EllipticSolver2d.cc:44
+            J.bc_coef = (s.kind == 0) ? 1 / (dxl * dxl) : s.alpha / (dxl * dxl);   // :137, :178
+            J.norm = 1. / (2 * s.nx * 2 * s.ny);                         // :80
+            J.f = s.f + lev * ns;
+            J.R = s.R ? s.R + lev * ns : nullptr;
+            J.uc = (lev < s.L - 1) ? s.u + (lev + 1) * ns : nullptr;
+            J.bcring = (s.L == 1) ? s.bcring : nullptr;
+            J.T1 = s.T1; J.T2 = s.T2;
+            J.u = s.u + lev * ns;
+            J.W = s.W ? s.W + lev * ns : nullptr;
+            J.cosx = s.cosx; J.cosy = s.cosy;
+            CKR(get_plan(c, s.ny - 1, &J.pj));
+            CKR(get_plan(c, s.nx - 1, &J.pi));
+            for (int k = 0; k < 3; ++k) P.maxb[k][st] = std::max(P.maxb[k][st], blocks_for_pass(J, k + 1));
+            jobs.push_back(J);
+        }
+        EllJob* d = nullptr;
+        CKR(mem.alloc(&d, jobs.size()));
+        CK(hipMemcpy(d, jobs.data(), sizeof(EllJob) * jobs.size(), hipMemcpyHostToDevice));
+        P.ej[st] = d;
+        P.ej_n[st] = (int)jobs.size();
+    }
+    return 0;
+}
+
+static void run_solve(const SolvePlan& P, hipStream_t s) {
+    for (int lev = 1; lev < P.nlev; ++lev) launch_coarsify(P.cj[lev], P.cj_n[lev], P.cj_pts[lev], s);
+    for (int st = 0; st < P.nlev; ++st)
+        for (int pass = 1; pass <= 3; ++pass) launch_ell_pass(pass, P.ej[st], P.ej_n[st], P.maxb[pass - 1][st], s);
+}
+
+static int cos_tables(DevMem& mem, int n, double** out) {
+    std::vector<double> t(n - 1);
+    const double pi = 4. * std::atan(1.0);
+    for (int i = 1; i < n; ++i) t[i - 1] = std::cos((pi * i) / n);   // EllipticSolver2d.cc:47-48
+    CKR(mem.alloc(out, n - 1));
+    CK(hipMemcpy(*out, t.data(), sizeof(double) * (n - 1), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// ---------------------------------------------------------------- batch
+struct Flow {
+    ibpm_flow_params p{};
+    std::string geom_text;
+    Geometry geom;
+    int nx, ny, L, nb;
+    double dx;
+    long ns, nf;
+    bool cg = false;
+    // device fields
+    double *omega, *q, *nprev, *a, *wstar, *psi, *c, *U, *V, *cross, *R, *T1, *T2, *mt, *cosx, *cosy;
+    double *f, *rhs, *b, *cg_r, *cg_d, *cg_q, *cg_state, *hist, *unit, *mcol;
+    int* hist_count;
+    double* minv[4] = {nullptr, nullptr, nullptr, nullptr};
+    // regulariser
+    RegLists reg;
+    int *e_ptr, *e_f, *t_ptr, *t_k, *node_idx, *old_idx;
+    double *e_w, *t_w;
+    int4* node_nbr;
+    int nnodes = 0, nold = 0;
+    // host state
+    std::vector<double> px, py, pu, pv;
+    int hist_cap = 0;
+    int cg_iters[4] = {-1, -1, -1, -1};
+};
+
+}  // namespace ibpmg
+
+struct ibpm_ctx {
+    ibpmg::Ctx c;
+};
+
+struct ibpm_batch {
+    ibpm_ctx* ctx = nullptr;
+    std::vector<ibpmg::Flow> flows;
+    ibpmg::DevMem mem;
+    int nsub = 3;
+    double an[4], bn[4], cn[4];
+    bool all_stationary = true, any_cg = false, initialized = false;
+    bool old_saved = false;
+    double time = 0.0;
+    int timestep = 0;
+    long max_ns_all = 0, max_nf = 0, max_nf_all = 0;
+    int max_comp = 0, max_nodes = 0, max_n = 0;
+    // plans per substep
+    ibpmg::SolvePlan S1[4], S2[4], S3[4], S4, M1[4], M2;
+    ibpmg::FlowOps* d_fops[4] = {};
+    ibpmg::RegOps *d_interp_c = nullptr, *d_interp_m = nullptr, *d_smear_f[4] = {}, *d_smear_d[4] = {};
+    ibpmg::RegOps* d_clear = nullptr;
+    ibpmg::MinvOps* d_minv[4] = {};
+    ibpmg::CgOps* d_cg = nullptr;
+    ibpmg::ForceOps* d_force = nullptr;
+    std::vector<ibpmg::FlowOps> h_fops[4];
+    bool use_graph = false;
+    hipGraphExec_t graph = nullptr;
+    int cg_chunk = 4;
+};
+
+namespace ibpmg {
+
+static int upload(void* d, const void* h, size_t bytes) {
+    if (bytes == 0) return 0;
+    CK(hipMemcpy(d, h, bytes, hipMemcpyHostToDevice));
+    return 0;
+}
+
+// Scheme coefficients, Scheme.h:14-56
+static void scheme_coefs(int scheme, ibpm_batch* B) {
+    switch (scheme) {
+    case 0: B->nsub = 1; B->an[0] = 1.; B->bn[0] = 0.; B->cn[0] = 1.; break;
+    case 1: B->nsub = 1; B->an[0] = 3. / 2.; B->bn[0] = -1. / 2.; B->cn[0] = 1.; break;
+    case 3:
+        B->nsub = 4;
+        B->an[0] = 8. / 17.; B->an[1] = 17. / 60.; B->an[2] = 5. / 12.; B->an[3] = 3. / 4.;
+        B->bn[0] = 0.; B->bn[1] = -15. / 68.; B->bn[2] = -17. / 60.; B->bn[3] = -5. / 12.;
+        B->cn[0] = 8. / 17.; B->cn[1] = 8. / 15.; B->cn[2] = 2. / 3.; B->cn[3] = 1.;
+        break;
+    default:
+        B->nsub = 3;
+        B->an[0] = 8. / 15.; B->an[1] = 5. / 12.; B->an[2] = 3. / 4.;
+        B->bn[0] = 0.; B->bn[1] = -17. / 60.; B->bn[2] = -5. / 12.;
+        B->cn[0] = 8. / 15.; B->cn[1] = 2. / 3.; B->cn[2] = 1.;
+    }
+}
+
+static double beta_of(const ibpm_batch* B, const Flow& F, int i) { return (B->an[i] + B->bn[i]) * F.p.dt; }
+
+// Regulariser lists of a flow for its current points; uploads them and the
+// constraint b = u_body - E q0 (NavierStokesModel.cc:86-91,142-146).
+static int update_regularizer(ibpm_batch* B, Flow& F) {
+    RegLists& R = F.reg;
+    const int old_nodes = F.nnodes;
+    std::vector<int> old = R.node_idx;
+    build_reg_lists(F.nx, F.ny, F.dx, F.p.xoffset, F.p.yoffset, F.px, F.py, R);
+    const int cap = 18 * F.nb + 1;
+    if ((int)R.e_f.size() > cap || (int)R.t_k.size() > cap || (int)R.node_idx.size() > 4 * cap) {
+        set_error("regulariser list exceeds capacity");
+        return -3;
+    }
+    CKR(upload(F.e_ptr, R.e_ptr.data(), sizeof(int) * R.e_ptr.size()));
+    CKR(upload(F.e_f, R.e_f.data(), sizeof(int) * R.e_f.size()));
+    CKR(upload(F.e_w, R.e_w.data(), sizeof(double) * R.e_w.size()));
+    CKR(upload(F.t_ptr, R.t_ptr.data(), sizeof(int) * R.t_ptr.size()));
+    CKR(upload(F.t_k, R.t_k.data(), sizeof(int) * R.t_k.size()));
+    CKR(upload(F.t_w, R.t_w.data(), sizeof(double) * R.t_w.size()));
+    CKR(upload(F.node_idx, R.node_idx.data(), sizeof(int) * R.node_idx.size()));
+    CKR(upload(F.node_nbr, R.node_nbr.data(), sizeof(int) * R.node_nbr.size()));
+    CKR(upload(F.old_idx, old.data(), sizeof(int) * old.size()));
+    F.nold = old_nodes;
+    F.nnodes = (int)R.node_idx.size();
+    // b = velocities - E(q0) with q0 the uniform flow on level 0
+    const double pi = 4. * std::atan(1.);
+    const double al = F.p.alpha_deg * pi / 180.;
+    const double u0 = 1 * std::cos(al) * F.dx, v0 = 1 * std::sin(al) * F.dx;
+    const long nfx = (long)(F.nx + 1) * F.ny;
+    std::vector<double> b(2 * F.nb);
+    for (int k = 0; k < 2 * F.nb; ++k) {
+        double s = 0;
+        for (int a = R.e_ptr[k]; a < R.e_ptr[k + 1]; ++a) s += R.e_w[a] * (R.e_f[a] < nfx ? u0 : v0);
+        s *= 1.0 / F.dx;
+        const double vel = k < F.nb ? F.pu[k] : F.pv[k - F.nb];
+        b[k] = vel - s;
+    }
+    CKR(upload(F.b, b.data(), sizeof(double) * b.size()));
+    (void)B;
+    return 0;
+}
+
+static RegOps make_regops(const Flow& F, double beta) {
+    RegOps R{};
+    R.nx = F.nx; R.ny = F.ny; R.L = F.L; R.nb = F.nb;
+    R.dx = F.dx; R.inv_dx = 1.0 / F.dx; R.bydx2 = 1. / (F.dx * F.dx); R.beta = beta;
+    R.e_ptr = F.e_ptr; R.e_f = F.e_f; R.e_w = F.e_w;
+    R.nnodes = F.nnodes; R.t_ptr = F.t_ptr; R.t_k = F.t_k; R.t_w = F.t_w;
+    R.node_nbr = F.node_nbr; R.node_idx = F.node_idx;
+    R.nold = F.nold; R.old_idx = F.old_idx;
+    return R;
+}
+
+// Rebuilds the small per-flow regulariser descriptor arrays (node counts change
+// when bodies move).
+static int upload_regops(ibpm_batch* B) {
+    const int nf = (int)B->flows.size();
+    std::vector<RegOps> ic(nf), im(nf), cl(nf);
+    B->max_nodes = 1;
+    for (int k = 0; k < nf; ++k) {
+        Flow& F = B->flows[k];
+        ic[k] = make_regops(F, 0);
+        ic[k].psi = F.psi; ic[k].b = F.b; ic[k].out = F.rhs;
+        im[k] = make_regops(F, 0);
+        im[k].psi = F.psi; im[k].b = nullptr; im[k].out = F.cg_q;
+        cl[k] = make_regops(F, 0);
+        cl[k].cout = F.c;
+        B->max_nodes = std::max(B->max_nodes, std::max(F.nnodes, F.nold));
+    }
+    CKR(upload(B->d_interp_c, ic.data(), sizeof(RegOps) * nf));
+    CKR(upload(B->d_interp_m, im.data(), sizeof(RegOps) * nf));
+    CKR(upload(B->d_clear, cl.data(), sizeof(RegOps) * nf));
+    for (int i = 0; i < B->nsub; ++i) {
+        std::vector<RegOps> sf(nf), sd(nf);
+        for (int k = 0; k < nf; ++k) {
+            Flow& F = B->flows[k];
+            sf[k] = make_regops(F, beta_of(B, F, i));
+            sf[k].fin = F.f; sf[k].cout = F.c;
+            sd[k] = make_regops(F, beta_of(B, F, i));
+            sd[k].fin = F.cg_d; sd[k].cout = F.c;
+        }
+        CKR(upload(B->d_smear_f[i], sf.data(), sizeof(RegOps) * nf));
+        CKR(upload(B->d_smear_d[i], sd.data(), sizeof(RegOps) * nf));
+    }
+    return 0;
+}
+
+static int alloc_flow(ibpm_batch* B, Flow& F) {
+    DevMem& m = B->mem;
+    const long ns = F.ns * F.L, nf = F.nf * F.L;
+    CKR(m.alloc(&F.omega, ns)); CKR(m.alloc(&F.q, nf)); CKR(m.alloc(&F.nprev, ns));
+    CKR(m.alloc(&F.a, ns)); CKR(m.alloc(&F.wstar, ns)); CKR(m.alloc(&F.psi, ns));
+    CKR(m.alloc(&F.c, ns)); CKR(m.alloc(&F.U, ns)); CKR(m.alloc(&F.V, ns));
+    CKR(m.alloc(&F.cross, nf)); CKR(m.alloc(&F.R, ns)); CKR(m.alloc(&F.T1, F.ns));
+    CKR(m.alloc(&F.T2, F.ns)); CKR(m.alloc(&F.mt, ns));
+    CKR(cos_tables(m, F.nx, &F.cosx)); CKR(cos_tables(m, F.ny, &F.cosy));
+    const int n = 2 * F.nb + 1;
+    CKR(m.alloc(&F.f, n)); CKR(m.alloc(&F.rhs, n)); CKR(m.alloc(&F.b, n));
+    CKR(m.alloc(&F.cg_r, n)); CKR(m.alloc(&F.cg_d, n)); CKR(m.alloc(&F.cg_q, n));
+    CKR(m.alloc(&F.cg_state, 4)); CKR(m.alloc(&F.unit, n)); CKR(m.alloc(&F.mcol, (size_t)n * n));
+    F.hist_cap = 1 << 16;
+    CKR(m.alloc(&F.hist, 2 * (size_t)F.hist_cap)); CKR(m.alloc(&F.hist_count, 1));
+    const int cap = 18 * F.nb + 1;
+    CKR(m.alloc(&F.e_ptr, n + 1)); CKR(m.alloc(&F.e_f, cap)); CKR(m.alloc(&F.e_w, cap));
+    CKR(m.alloc(&F.t_ptr, cap + 1)); CKR(m.alloc(&F.t_k, cap)); CKR(m.alloc(&F.t_w, cap));
+    CKR(m.alloc(&F.node_idx, 4 * cap)); CKR(m.alloc(&F.old_idx, 4 * cap)); CKR(m.alloc(&F.node_nbr, 4 * cap));
+    return 0;
+}
+
+static SolveSpec spec_of(const Flow& F, int kind, double alpha, const double* f, double* u, double in_scale,
+                         int epilogue, const double* W) {
+    SolveSpec s{};
+    s.nx = F.nx; s.ny = F.ny; s.L = F.L; s.dx = F.dx; s.kind = kind; s.alpha = alpha;
+    s.f = f; s.u = u; s.R = F.R; s.T1 = F.T1; s.T2 = F.T2; s.in_scale = in_scale; s.epilogue = epilogue;
+    s.W = W; s.cosx = F.cosx; s.cosy = F.cosy;
+    return s;
+}
+
+static int build_plans(ibpm_batch* B) {
+    Ctx* c = &B->ctx->c;
+    const int nfl = (int)B->flows.size();
+    for (int i = 0; i < B->nsub; ++i) {
+        std::vector<SolveSpec> s1, s2, s3, m1;
+        for (auto& F : B->flows) {
+            const double alpha = -beta_of(B, F, i) / 2. * (1. / F.p.reynolds);   // ProjectionSolver.cc:33
+            s1.push_back(spec_of(F, 1, alpha, F.a, F.wstar, 1.0, 0, nullptr));
+            s2.push_back(spec_of(F, 0, 0.0, F.wstar, F.psi, -1.0, 0, nullptr));
+            s3.push_back(spec_of(F, 1, alpha, F.c, F.omega, 1.0, 1, F.wstar));
+            m1.push_back(spec_of(F, 1, alpha, F.c, F.mt, 1.0, 0, nullptr));
+        }
+        CKR(build_solve_plan(c, B->mem, s1, B->S1[i]));
+        CKR(build_solve_plan(c, B->mem, s2, B->S2[i]));
+        CKR(build_solve_plan(c, B->mem, s3, B->S3[i]));
+        CKR(build_solve_plan(c, B->mem, m1, B->M1[i]));
+    }
+    std::vector<SolveSpec> s4, m2;
+    for (auto& F : B->flows) {
+        s4.push_back(spec_of(F, 0, 0.0, F.omega, F.psi, -1.0, 0, nullptr));
+        m2.push_back(spec_of(F, 0, 0.0, F.mt, F.psi, -1.0, 0, nullptr));
+    }
+    CKR(build_solve_plan(c, B->mem, s4, B->S4));
+    CKR(build_solve_plan(c, B->mem, m2, B->M2));
+    // stencil descriptors
+    for (int i = 0; i < B->nsub; ++i) {
+        B->h_fops[i].clear();
+        for (auto& F : B->flows) {
+            FlowOps o{};
+            o.nx = F.nx; o.ny = F.ny; o.L = F.L; o.dx = F.dx;
+            o.q = F.q; o.omega = F.omega; o.U = F.U; o.V = F.V; o.cross = F.cross;
+            o.a = F.a; o.nprev = F.nprev; o.psi = F.psi;
+            const double nu = 1. / F.p.reynolds;   // NavierStokesModel::getAlpha
+            o.lap_coef = 0.5 * nu * (B->an[i] + B->bn[i]);
+            o.an = B->an[i]; o.bn = B->bn[i]; o.dt = F.p.dt; o.w_coef = 1.0;
+            o.use_nprev = B->bn[i] != 0;
+            o.nprev_valid = 0;
+            const double pi = 4. * std::atan(1.);
+            const double al = F.p.alpha_deg * pi / 180.;
+            o.ubase = 1 * std::cos(al);
+            o.vbase = 1 * std::sin(al);
+            B->h_fops[i].push_back(o);
+        }
+        CKR(B->mem.alloc(&B->d_fops[i], nfl));
+        CKR(upload(B->d_fops[i], B->h_fops[i].data(), sizeof(FlowOps) * nfl));
+        CKR(B->mem.alloc(&B->d_smear_f[i], nfl));
+        CKR(B->mem.alloc(&B->d_smear_d[i], nfl));
+        CKR(B->mem.alloc(&B->d_minv[i], nfl));
+    }
+    CKR(B->mem.alloc(&B->d_interp_c, nfl));
+    CKR(B->mem.alloc(&B->d_interp_m, nfl));
+    CKR(B->mem.alloc(&B->d_clear, nfl));
+    std::vector<CgOps> cg(nfl);
+    std::vector<ForceOps> fo(nfl);
+    for (int k = 0; k < nfl; ++k) {
+        Flow& F = B->flows[k];
+        cg[k] = CgOps{2 * F.nb, F.f, F.cg_r, F.cg_d, F.cg_q, F.rhs, F.cg_state};
+        fo[k] = ForceOps{F.nb, F.f, F.hist, F.hist_count};
+    }
+    CKR(B->mem.alloc(&B->d_cg, nfl));
+    CKR(upload(B->d_cg, cg.data(), sizeof(CgOps) * nfl));
+    CKR(B->mem.alloc(&B->d_force, nfl));
+    CKR(upload(B->d_force, fo.data(), sizeof(ForceOps) * nfl));
+    return 0;
+}
+
+static void mark_nprev_valid(ibpm_batch* B) {
+    if (B->old_saved) return;
+    B->old_saved = true;
+    for (int i = 0; i < B->nsub; ++i) {
+        for (auto& o : B->h_fops[i]) o.nprev_valid = 1;
+        (void)hipMemcpy(B->d_fops[i], B->h_fops[i].data(), sizeof(FlowOps) * B->h_fops[i].size(),
+                        hipMemcpyHostToDevice);
+    }
+}
+
+// M f = C A^-1 (beta B f) for the vector behind d_smear (ProjectionSolver.cc:96-101).
+static void enqueue_M(ibpm_batch* B, int i, RegOps* smear, hipStream_t s) {
+    const int nfl = (int)B->flows.size();
+    launch_smear_curl(smear, nfl, B->max_nodes, s);
+    run_solve(B->M1[i], s);
+    run_solve(B->M2, s);
+    launch_interp(B->d_interp_m, nfl, B->max_comp, s);
+}
+
+// Nonlinear term and right-hand side a of substep i (IBSolver.cc:144,169-183).
+static void enqueue_rhs(ibpm_batch* B, int i, hipStream_t s) {
+    const int nfl = (int)B->flows.size();
+    int lmax = 0;
+    for (auto& F : B->flows) lmax = std::max(lmax, F.L);
+    launch_vel(B->d_fops[i], nfl, B->max_ns_all, s);
+    for (int lev = 0; lev < lmax; ++lev) launch_cross(B->d_fops[i], nfl, B->max_nf, lev, s);
+    launch_rhs(B->d_fops[i], nfl, B->max_ns_all, s);
+}
+
+static void enqueue_refresh(ibpm_batch* B, hipStream_t s) {
+    run_solve(B->S4, s);
+    launch_curl_q(B->d_fops[0], (int)B->flows.size(), B->max_nf_all, s);
+}
+
+// The stationary-body (Cholesky) substep: fixed launch sequence.
+static void enqueue_substep_chol(ibpm_batch* B, int i, hipStream_t s) {
+    const int nfl = (int)B->flows.size();
+    enqueue_rhs(B, i, s);
+    run_solve(B->S1[i], s);                         // omega* = A^-1 a
+    run_solve(B->S2[i], s);                         // psi = L^-1(-omega*)
+    launch_interp(B->d_interp_c, nfl, B->max_comp, s);   // rhs = C omega* - b
+    launch_minv(B->d_minv[i], nfl, B->max_n, s);    // f = M^-1 rhs
+    launch_smear_curl(B->d_smear_f[i], nfl, B->max_nodes, s);   // c = beta B f
+    run_solve(B->S3[i], s);                         // omega = omega* - A^-1 c
+    enqueue_refresh(B, s);                          // q = Curl psi + q0
+}
+
+static int enqueue_step_chol(ibpm_batch* B, hipStream_t s) {
+    for (int i = 0; i < B->nsub; ++i) enqueue_substep_chol(B, i, s);
+    launch_forces(B->d_force, (int)B->flows.size(), s);
+    return 0;
+}
+
+// One substep with CG (moving bodies): host-side kinematics and regulariser,
+// then CG iterations in chunks with a device-side convergence flag
+// (ConjugateGradientSolver.cc:35-65; iterations after convergence are no-ops).
+static int substep_cg(ibpm_batch* B, int i) {
+    hipStream_t s = B->ctx->c.stream;
+    const int nfl = (int)B->flows.size();
+    for (auto& F : B->flows) {
+        if (!F.geom.stationary) {   // updateOperators(t + c_n dt), IBSolver.cc:164-166
+            F.geom.move(B->time + B->cn[i] * F.p.dt, F.px, F.py, F.pu, F.pv);
+            CK(hipStreamSynchronize(s));
+            CKR(update_regularizer(B, F));
+        }
+    }
+    CKR(upload_regops(B));
+    launch_clear_nodes(B->d_clear, nfl, B->max_nodes, s);
+    enqueue_rhs(B, i, s);
+    run_solve(B->S1[i], s);
+    run_solve(B->S2[i], s);
+    launch_interp(B->d_interp_c, nfl, B->max_comp, s);
+    // r0 = rhs - M x0 with x0 = previous f (ProjectionSolver.cc:72 warm start)
+    for (auto& F : B->flows) CK(hipMemcpyAsync(F.cg_d, F.f, sizeof(double) * 2 * F.nb, hipMemcpyDeviceToDevice, s));
+    enqueue_M(B, i, B->d_smear_d[i], s);
+    for (auto& F : B->flows) {
+        const double st[4] = {0, F.p.cg_tol * F.p.cg_tol, 0, 0};
+        CK(hipMemcpyAsync(F.cg_state, st, sizeof(st), hipMemcpyHostToDevice, s));
+        CK(hipStreamSynchronize(s));
+    }
+    launch_cg(B->d_cg, nfl, 0, s);
+    const int max_iter = 100000;
+    for (int it = 0; it < max_iter; it += B->cg_chunk) {
+        CK(hipStreamSynchronize(s));
+        bool all = true;
+        for (auto& F : B->flows) {
+            double st[4];
+            CK(hipMemcpy(st, F.cg_state, sizeof(st), hipMemcpyDeviceToHost));
+            F.cg_iters[i] = (int)st[2];
+            if (st[3] == 0.0) all = false;
+        }
+        if (all) break;
+        for (int k = 0; k < B->cg_chunk; ++k) {
+            enqueue_M(B, i, B->d_smear_d[i], s);
+            launch_cg(B->d_cg, nfl, 1, s);
+        }
+    }
+    launch_smear_curl(B->d_smear_f[i], nfl, B->max_nodes, s);
+    run_solve(B->S3[i], s);
+    enqueue_refresh(B, s);
+    return 0;
+}
+
+// CholeskySolver::init (CholeskySolver.cc:49-109): M column by column on the
+// GPU, FP64 factorisation on the host, then M^-1 = (L L^T)^-1 assembled with
+// the reference's forward/back substitution (:200-224) applied to unit vectors.
+static int build_cholesky(ibpm_batch* B) {
+    hipStream_t s = B->ctx->c.stream;
+    const int nfl = (int)B->flows.size();
+    int nmax = 0;
+    for (auto& F : B->flows) nmax = std::max(nmax, 2 * F.nb);
+    static const double one = 1.0, zero = 0.0;
+    for (int i = 0; i < B->nsub; ++i) {
+        for (int j = 0; j < nmax; ++j) {
+            std::vector<RegOps> sm(nfl), ip(nfl);
+            for (int k = 0; k < nfl; ++k) {
+                Flow& F = B->flows[k];
+                const int n = 2 * F.nb;
+                sm[k] = make_regops(F, beta_of(B, F, i));
+                sm[k].fin = F.unit; sm[k].cout = F.c;
+                ip[k] = make_regops(F, 0);
+                ip[k].psi = F.psi; ip[k].b = nullptr;
+                ip[k].out = F.mcol + (size_t)std::min(j, n) * n;   // column n is a dummy
+                if (j < n) {
+                    if (j > 0) CK(hipMemcpyAsync(F.unit + j - 1, &zero, sizeof(double), hipMemcpyHostToDevice, s));
+                    CK(hipMemcpyAsync(F.unit + j, &one, sizeof(double), hipMemcpyHostToDevice, s));
+                }
+            }
+            CK(hipStreamSynchronize(s));
+            CKR(upload(B->d_smear_d[i], sm.data(), sizeof(RegOps) * nfl));
+            CKR(upload(B->d_interp_m, ip.data(), sizeof(RegOps) * nfl));
+            enqueue_M(B, i, B->d_smear_d[i], s);
+        }
+        CK(hipStreamSynchronize(s));
+        for (auto& F : B->flows) {
+            const int n = 2 * F.nb;
+            std::vector<double> Mt((size_t)n * n), L((size_t)n * n, 0.0), d(n), Minv((size_t)n * n);
+            CK(hipMemcpy(Mt.data(), F.mcol, sizeof(double) * n * n, hipMemcpyDeviceToHost));
+            CK(hipMemset(F.unit, 0, sizeof(double) * (n + 1)));
+            auto M = [&](int r, int c) { return Mt[(size_t)c * n + r]; };   // column-major
+            for (int r = 0; r < n; ++r) {
+                for (int c = r; c < n; ++c) {
+                    double sum = M(r, c);
+                    for (int k = r - 1; k >= 0; --k) sum -= L[(size_t)r * n + k] * L[(size_t)c * n + k];
+                    if (c == r) {
+                        if (!(sum > 0)) { set_error("Cholesky: matrix not positive definite at %d", r); return -4; }
+                        d[r] = std::sqrt(sum);
+                    } else {
+                        L[(size_t)c * n + r] = sum / d[r];
+                    }
+                }
+            }
+            std::vector<double> x(n);
+            for (int col = 0; col < n; ++col) {
+                for (int r = 0; r < n; ++r) {
+                    double sum = (r == col) ? 1.0 : 0.0;
+                    for (int k = r - 1; k >= 0; --k) sum -= L[(size_t)r * n + k] * x[k];
+                    x[r] = sum / d[r];
+                }
+                for (int r = n - 1; r >= 0; --r) {
+                    double sum = x[r];
+                    for (int k = r + 1; k < n; ++k) sum -= L[(size_t)k * n + r] * x[k];
+                    x[r] = sum / d[r];
+                }
+                for (int r = 0; r < n; ++r) Minv[(size_t)r * n + col] = x[r];
+            }
+            if (!F.minv[i]) CKR(B->mem.alloc(&F.minv[i], (size_t)n * n));
+            CKR(upload(F.minv[i], Minv.data(), sizeof(double) * Minv.size()));
+        }
+        std::vector<MinvOps> mo(nfl);
+        for (int k = 0; k < nfl; ++k) {
+            Flow& F = B->flows[k];
+            mo[k] = MinvOps{2 * F.nb, F.minv[i], F.rhs, F.f};
+        }
+        CKR(upload(B->d_minv[i], mo.data(), sizeof(MinvOps) * nfl));
+    }
+    return upload_regops(B);
+}
+
+}  // namespace ibpmg
+
+using namespace ibpmg;
+
+// ===================================================================== C ABI
+extern "C" {
+
+const char* ibpm_last_error(void) { return g_err; }
+
+int ibpm_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int ibpm_ctx_create(int device, ibpm_ctx** out) {
+    if (!out) { set_error("null output pointer"); return -1; }
+    int n = ibpm_device_count();
+    if (device < 0 || device >= n) { set_error("device %d not available (%d visible)", device, n); return -1; }
+    auto* c = new ibpm_ctx();
+    c->c.device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->c.stream) != hipSuccess) {
+        set_error("could not create a stream on device %d", device);
+        delete c;
+        return -1;
+    }
+    *out = c;
+    return 0;
+}
+
+int ibpm_ctx_destroy(ibpm_ctx* c) {
+    if (!c) return 0;
+    (void)hipStreamSynchronize(c->c.stream);
+    (void)hipStreamDestroy(c->c.stream);
+    delete c;
+    return 0;
+}
+
+int ibpm_ctx_synchronize(ibpm_ctx* c) {
+    CK(hipStreamSynchronize(c->c.stream));
+    return 0;
+}
+
+int ibpm_malloc(ibpm_ctx* c, size_t bytes, void** d) {
+    (void)c;
+    CK(hipMalloc(d, bytes ? bytes : 1));
+    return 0;
+}
+int ibpm_free(ibpm_ctx* c, void* d) {
+    (void)c;
+    CK(hipFree(d));
+    return 0;
+}
+int ibpm_memcpy_h2d(ibpm_ctx* c, void* d, const void* h, size_t bytes) {
+    CK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c->c.stream));
+    CK(hipStreamSynchronize(c->c.stream));
+    return 0;
+}
+int ibpm_memcpy_d2h(ibpm_ctx* c, void* h, const void* d, size_t bytes) {
+    CK(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, c->c.stream));
+    CK(hipStreamSynchronize(c->c.stream));
+    return 0;
+}
+
+static int check_grid(int nx, int ny, int ngrid) {
+    if (nx < 2 || ny < 2 || ngrid < 1) { set_error("bad grid %dx%d, %d levels", nx, ny, ngrid); return -1; }
+    if (ngrid > 1 && (nx % 4 || ny % 4)) {   // Grid.cc:72
+        set_error("multi-domain grids need nx, ny divisible by 4 (got %dx%d)", nx, ny);
+        return -1;
+    }
+    return 0;
+}
+
+int ibpm_dst2d(ibpm_ctx* c, int n1, int n2, const double* d_in, double* d_out) {
+    // The 2-D DST is the solve pipeline with unit eigenvalues and no
+    // normalisation: kind Poisson with a scaling table forced to 1 is not
+    // available, so pass 1 and pass 3 are run back to back through T.
+    CKR(check_grid(n1 + 1, n2 + 1, 1));
+    DevMem mem;
+    double *T = nullptr, *cosx = nullptr, *cosy = nullptr;
+    CKR(mem.alloc(&T, (size_t)n1 * n2));
+    CKR(cos_tables(mem, n1 + 1, &cosx));
+    CKR(cos_tables(mem, n2 + 1, &cosy));
+    EllJob J{};
+    J.nx = n1 + 1; J.ny = n2 + 1; J.lev = 0; J.ngrid = 1; J.in_scale = 1.0;
+    J.f = d_in; J.T1 = T; J.T2 = T; J.u = d_out; J.norm = 1.0;
+    CKR(get_plan(&c->c, n2, &J.pj));
+    CKR(get_plan(&c->c, n1, &J.pi));
+    // pass 1: rows along j -> T (transposed); a second pass-1 style launch over
+    // the transposed rows transforms along i and transposes back.
+    EllJob J2 = J;
+    J2.nx = n2 + 1; J2.ny = n1 + 1; J2.f = T; J2.T1 = d_out; J2.pj = J.pi;
+    EllJob* d = nullptr;
+    CKR(mem.alloc(&d, 2));
+    CK(hipMemcpyAsync(d, &J, sizeof(EllJob), hipMemcpyHostToDevice, c->c.stream));
+    CK(hipMemcpyAsync(d + 1, &J2, sizeof(EllJob), hipMemcpyHostToDevice, c->c.stream));
+    launch_ell_pass(1, d, 1, blocks_for_pass(J, 1), c->c.stream);
+    launch_ell_pass(1, d + 1, 1, blocks_for_pass(J2, 1), c->c.stream);
+    CK(hipStreamSynchronize(c->c.stream));
+    CK(hipGetLastError());
+    return 0;
+}
+
+static int solve_common(ibpm_ctx* c, int kind, int nx, int ny, int ngrid, double dx, double alpha,
+                        const double* d_f, const double* d_bc, double* d_u) {
+    CKR(check_grid(nx, ny, ngrid));
+    DevMem mem;
+    const long ns = (long)(nx - 1) * (ny - 1);
+    double *R = nullptr, *T1 = nullptr, *T2 = nullptr, *cosx = nullptr, *cosy = nullptr, *fin = nullptr;
+    CKR(mem.alloc(&R, ns * ngrid));
+    CKR(mem.alloc(&T1, ns));
+    CKR(mem.alloc(&T2, ns));
+    CKR(mem.alloc(&fin, ns * ngrid));
+    CK(hipMemcpyAsync(fin, d_f, sizeof(double) * ns * ngrid, hipMemcpyDeviceToDevice, c->c.stream));
+    CKR(cos_tables(mem, nx, &cosx));
+    CKR(cos_tables(mem, ny, &cosy));
+    SolveSpec s{};
+    s.nx = nx; s.ny = ny; s.L = ngrid; s.dx = dx; s.kind = kind; s.alpha = alpha;
+    s.f = fin; s.u = d_u; s.R = R; s.T1 = T1; s.T2 = T2; s.in_scale = 1.0; s.epilogue = 0;
+    s.cosx = cosx; s.cosy = cosy; s.bcring = d_bc;
+    SolvePlan P;
+    CKR(build_solve_plan(&c->c, mem, {s}, P));
+    run_solve(P, c->c.stream);
+    CK(hipStreamSynchronize(c->c.stream));
+    CK(hipGetLastError());
+    return 0;
+}
+
+int ibpm_elliptic2d_solve(ibpm_ctx* c, int kind, int nx, int ny, double dx, double alpha,
+                          const double* d_f, const double* d_bc, double* d_u) {
+    return solve_common(c, kind, nx, ny, 1, dx, alpha, d_f, d_bc, d_u);
+}
+
+int ibpm_elliptic_solve(ibpm_ctx* c, int kind, int nx, int ny, int ngrid, double dx, double alpha,
+                        const double* d_f, double* d_u) {
+    return solve_common(c, kind, nx, ny, ngrid, dx, alpha, d_f, nullptr, d_u);
+}
+
+// Stand-alone stencil entry points run the batch kernels on a one-flow
+// descriptor with scratch buffers.
+static int ops_common(ibpm_ctx* c, int nx, int ny, int ngrid, double dx, FlowOps& o, DevMem& mem) {
+    CKR(check_grid(nx, ny, ngrid));
+    const long ns = (long)(nx - 1) * (ny - 1) * ngrid;
+    const long nf = (2L * nx * ny + nx + ny) * ngrid;
+    o.nx = nx; o.ny = ny; o.L = ngrid; o.dx = dx;
+    if (!o.U) CKR(mem.alloc(&o.U, ns));
+    if (!o.V) CKR(mem.alloc(&o.V, ns));
+    if (!o.cross) CKR(mem.alloc(&o.cross, nf));
+    if (!o.a) CKR(mem.alloc(&o.a, ns));
+    if (!o.nprev) CKR(mem.alloc(&o.nprev, ns));
+    (void)c;
+    return 0;
+}
+
+int ibpm_curl_flux(ibpm_ctx* c, int nx, int ny, int ngrid, double dx, const double* d_q, double* d_w) {
+    // N = Curl(cross) of the rhs kernel with omega = 0: a = dt * an * N = N.
+    DevMem mem;
+    FlowOps o{};
+    double* zero = nullptr;
+    const long ns = (long)(nx - 1) * (ny - 1) * ngrid;
+    CKR(mem.alloc(&zero, ns));
+    o.cross = const_cast<double*>(d_q);
+    o.omega = zero;
+    o.a = d_w;
+    o.lap_coef = 0; o.an = 1; o.bn = 0; o.dt = 1; o.w_coef = 0;
+    CKR(ops_common(c, nx, ny, ngrid, dx, o, mem));
+    FlowOps* d = nullptr;
+    CKR(mem.alloc(&d, 1));
+    CK(hipMemcpyAsync(d, &o, sizeof(o), hipMemcpyHostToDevice, c->c.stream));
+    launch_rhs(d, 1, ns, c->c.stream);
+    CK(hipStreamSynchronize(c->c.stream));
+    CK(hipGetLastError());
+    return 0;
+}
+
+int ibpm_laplacian(ibpm_ctx* c, int nx, int ny, int ngrid, double dx, const double* d_w, double* d_out) {
+    // rhs kernel with a = Lap(omega): lap_coef 1, no nonlinear term, no omega.
+    DevMem mem;
+    FlowOps o{};
+    const long ns = (long)(nx - 1) * (ny - 1) * ngrid;
+    const long nf = (2L * nx * ny + nx + ny) * ngrid;
+    double* zc = nullptr;
+    CKR(mem.alloc(&zc, nf));
+    o.cross = zc;
+    o.omega = const_cast<double*>(d_w);
+    o.a = d_out;
+    o.lap_coef = 1; o.an = 0; o.bn = 0; o.dt = 1; o.w_coef = 0;
+    CKR(ops_common(c, nx, ny, ngrid, dx, o, mem));
+    FlowOps* d = nullptr;
+    CKR(mem.alloc(&d, 1));
+    CK(hipMemcpyAsync(d, &o, sizeof(o), hipMemcpyHostToDevice, c->c.stream));
+    launch_rhs(d, 1, ns, c->c.stream);
+    CK(hipStreamSynchronize(c->c.stream));
+    CK(hipGetLastError());
+    return 0;
+}
+
+int ibpm_curl_scalar(ibpm_ctx* c, int nx, int ny, int ngrid, double dx, double u0, double v0,
+                     const double* d_w, double* d_q) {
+    DevMem mem;
+    FlowOps o{};
+    o.psi = const_cast<double*>(d_w);
+    o.q = d_q;
+    o.ubase = u0; o.vbase = v0;
+    CKR(ops_common(c, nx, ny, ngrid, dx, o, mem));
+    FlowOps* d = nullptr;
+    CKR(mem.alloc(&d, 1));
+    CK(hipMemcpyAsync(d, &o, sizeof(o), hipMemcpyHostToDevice, c->c.stream));
+    launch_curl_q(d, 1, (2L * nx * ny + nx + ny) * ngrid, c->c.stream);
+    CK(hipStreamSynchronize(c->c.stream));
+    CK(hipGetLastError());
+    return 0;
+}
+
+int ibpm_nonlinear(ibpm_ctx* c, int nx, int ny, int ngrid, double dx, const double* d_q,
+                   const double* d_w, double* d_n) {
+    DevMem mem;
+    FlowOps o{};
+    const long ns = (long)(nx - 1) * (ny - 1) * ngrid;
+    const long nf1 = 2L * nx * ny + nx + ny;
+    double* zero = nullptr;
+    CKR(mem.alloc(&zero, ns));
+    o.q = const_cast<double*>(d_q);
+    o.omega = const_cast<double*>(d_w);
+    CKR(ops_common(c, nx, ny, ngrid, dx, o, mem));
+    FlowOps* d = nullptr;
+    CKR(mem.alloc(&d, 2));
+    FlowOps o2 = o;   // curl of cross: rhs kernel with omega = 0, dt = an = 1
+    o2.omega = zero; o2.a = d_n; o2.lap_coef = 0; o2.an = 1; o2.bn = 0; o2.dt = 1; o2.w_coef = 0;
+    CK(hipMemcpyAsync(d, &o, sizeof(o), hipMemcpyHostToDevice, c->c.stream));
+    CK(hipMemcpyAsync(d + 1, &o2, sizeof(o2), hipMemcpyHostToDevice, c->c.stream));
+    launch_vel(d, 1, ns, c->c.stream);
+    for (int lev = 0; lev < ngrid; ++lev) launch_cross(d, 1, nf1, lev, c->c.stream);
+    launch_rhs(d + 1, 1, ns, c->c.stream);
+    CK(hipStreamSynchronize(c->c.stream));
+    CK(hipGetLastError());
+    return 0;
+}
+
+// ------------------------------------------------------------------ batch
+int ibpm_batch_create(ibpm_ctx* ctx, int nflows, const ibpm_flow_params* params, ibpm_batch** out) {
+    if (!ctx || !params || !out || nflows < 1) { set_error("bad batch arguments"); return -1; }
+    CK(hipSetDevice(ctx->c.device));
+    auto B = std::make_unique<ibpm_batch>();
+    B->ctx = ctx;
+    scheme_coefs(params[0].scheme, B.get());
+    B->flows.resize(nflows);
+    for (int k = 0; k < nflows; ++k) {
+        Flow& F = B->flows[k];
+        F.p = params[k];
+        if (params[k].scheme != params[0].scheme) { set_error("all flows of a batch share one scheme"); return -1; }
+        CKR(check_grid(F.p.nx, F.p.ny, F.p.ngrid));
+        F.geom_text = F.p.geom ? F.p.geom : "";
+        F.p.geom = nullptr;
+        std::string err;
+        if (!F.geom.parse(F.geom_text, err)) { set_error("flow %d: %s", k, err.c_str()); return -1; }
+        F.nx = F.p.nx; F.ny = F.p.ny; F.L = F.p.ngrid;
+        F.dx = F.p.length / F.p.nx;   // Grid.cc:77
+        F.ns = (long)(F.nx - 1) * (F.ny - 1);
+        F.nf = 2L * F.nx * F.ny + F.nx + F.ny;
+        F.nb = F.geom.npoints;
+        F.cg = !F.geom.stationary || F.p.force_cg;
+        if (F.cg) B->any_cg = true;
+        if (!F.geom.stationary) B->all_stationary = false;
+        if (F.p.cg_tol <= 0) F.p.cg_tol = 1e-7;   // IBSolver.cc:31 default
+        CKR(alloc_flow(B.get(), F));
+        B->max_ns_all = std::max(B->max_ns_all, F.ns * F.L);
+        B->max_nf = std::max(B->max_nf, F.nf);
+        B->max_nf_all = std::max(B->max_nf_all, F.nf * F.L);
+        B->max_comp = std::max(B->max_comp, 2 * F.nb);
+        B->max_n = std::max(B->max_n, 2 * F.nb);
+    }
+    if (B->any_cg) {
+        for (auto& F : B->flows) F.cg = true;   // one force solver per batch
+    }
+    B->max_comp = std::max(B->max_comp, 1);
+    B->max_n = std::max(B->max_n, 1);
+    CKR(build_plans(B.get()));
+    *out = B.release();
+    return 0;
+}
+
+int ibpm_batch_destroy(ibpm_batch* b) {
+    if (!b) return 0;
+    (void)hipStreamSynchronize(b->ctx->c.stream);
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    delete b;
+    return 0;
+}
+
+int ibpm_batch_init(ibpm_batch* B) {
+    hipStream_t s = B->ctx->c.stream;
+    for (auto& F : B->flows) {
+        F.geom.move(B->time, F.px, F.py, F.pu, F.pv);   // ibpm.cc:285
+        CKR(update_regularizer(B, F));                   // model->init()
+    }
+    CKR(upload_regops(B));
+    if (!B->any_cg) CKR(build_cholesky(B));              // solver->init()
+    enqueue_refresh(B, s);                               // ibpm.cc:304-305
+    CK(hipStreamSynchronize(s));
+    CK(hipGetLastError());
+    B->initialized = true;
+    return 0;
+}
+
+int ibpm_batch_advance(ibpm_batch* B, int nsteps) {
+    if (!B->initialized) { set_error("ibpm_batch_init not called"); return -1; }
+    hipStream_t s = B->ctx->c.stream;
+    for (int step = 0; step < nsteps; ++step) {
+        for (auto& F : B->flows)
+            if (F.hist_cap <= B->timestep) { set_error("force history capacity exceeded"); return -1; }
+        if (B->any_cg) {
+            for (int i = 0; i < B->nsub; ++i) {
+                CKR(substep_cg(B, i));
+                if (i == 0) mark_nprev_valid(B);
+            }
+            launch_forces(B->d_force, (int)B->flows.size(), s);
+        } else if (B->use_graph && B->old_saved) {
+            if (!B->graph) {
+                hipGraph_t g;
+                CK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                enqueue_step_chol(B, s);
+                CK(hipStreamEndCapture(s, &g));
+                CK(hipGraphInstantiate(&B->graph, g, nullptr, nullptr, 0));
+                CK(hipGraphDestroy(g));
+            }
+            CK(hipGraphLaunch(B->graph, s));
+        } else {
+            for (int i = 0; i < B->nsub; ++i) {
+                enqueue_substep_chol(B, i, s);
+                if (i == 0 && !B->old_saved) {
+                    CK(hipStreamSynchronize(s));
+                    mark_nprev_valid(B);
+                }
+            }
+            launch_forces(B->d_force, (int)B->flows.size(), s);
+        }
+        B->time += B->flows[0].p.dt;
+        B->timestep += 1;
+    }
+    CK(hipStreamSynchronize(s));
+    CK(hipGetLastError());
+    return 0;
+}
+
+int ibpm_batch_use_graph(ibpm_batch* B, int enable) {
+    B->use_graph = enable != 0;
+    return 0;
+}
+
+static Flow* flow_of(ibpm_batch* B, int k) {
+    if (!B || k < 0 || k >= (int)B->flows.size()) { set_error("bad flow index %d", k); return nullptr; }
+    return &B->flows[k];
+}
+
+int ibpm_batch_num_points(ibpm_batch* B, int k) {
+    Flow* F = flow_of(B, k);
+    return F ? F->nb : -1;
+}
+
+int ibpm_batch_get_omega(ibpm_batch* B, int k, double* h) {
+    Flow* F = flow_of(B, k);
+    if (!F) return -1;
+    CK(hipMemcpy(h, F->omega, sizeof(double) * F->ns * F->L, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int ibpm_batch_set_omega(ibpm_batch* B, int k, const double* h) {
+    Flow* F = flow_of(B, k);
+    if (!F) return -1;
+    CK(hipMemcpy(F->omega, h, sizeof(double) * F->ns * F->L, hipMemcpyHostToDevice));
+    enqueue_refresh(B, B->ctx->c.stream);
+    CK(hipStreamSynchronize(B->ctx->c.stream));
+    return 0;
+}
+
+int ibpm_batch_get_q(ibpm_batch* B, int k, double* h) {
+    Flow* F = flow_of(B, k);
+    if (!F) return -1;
+    CK(hipMemcpy(h, F->q, sizeof(double) * F->nf * F->L, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int ibpm_batch_get_f(ibpm_batch* B, int k, double* h) {
+    Flow* F = flow_of(B, k);
+    if (!F) return -1;
+    CK(hipMemcpy(h, F->f, sizeof(double) * 2 * F->nb, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int ibpm_batch_get_points(ibpm_batch* B, int k, double* h) {
+    Flow* F = flow_of(B, k);
+    if (!F) return -1;
+    for (int i = 0; i < F->nb; ++i) {
+        h[i] = F->px[i];
+        h[F->nb + i] = F->py[i];
+    }
+    return 0;
+}
+
+static void forces_from(const Flow& F, double xf, double yf, double* cd, double* cl) {
+    const double dx2 = F.dx * F.dx;   // State.cc:60-62
+    xf *= dx2;
+    yf *= dx2;
+    const double pi = 4. * std::atan(1.);
+    const double al = F.p.alpha_deg * pi / 180.;
+    const double drag = xf * std::cos(al) + yf * std::sin(al);   // ibpm.cc:352-353
+    const double lift = xf * -1. * std::sin(al) + yf * std::cos(al);
+    *cd = 2 * drag;
+    *cl = 2 * lift;
+}
+
+int ibpm_batch_get_forces(ibpm_batch* B, int k, double* cd, double* cl) {
+    Flow* F = flow_of(B, k);
+    if (!F) return -1;
+    std::vector<double> f(2 * F->nb);
+    CK(hipMemcpy(f.data(), F->f, sizeof(double) * f.size(), hipMemcpyDeviceToHost));
+    double xf = 0, yf = 0;
+    for (int i = 0; i < F->nb; ++i) { xf += f[i]; yf += f[F->nb + i]; }
+    forces_from(*F, xf, yf, cd, cl);
+    return 0;
+}
+
+int ibpm_batch_force_history(ibpm_batch* B, int k, double* h, int max_steps) {
+    Flow* F = flow_of(B, k);
+    if (!F) return -1;
+    int n = 0;
+    CK(hipMemcpy(&n, F->hist_count, sizeof(int), hipMemcpyDeviceToHost));
+    n = std::min(n, max_steps);
+    std::vector<double> raw(2 * (size_t)std::max(n, 1));
+    CK(hipMemcpy(raw.data(), F->hist, sizeof(double) * 2 * n, hipMemcpyDeviceToHost));
+    for (int s = 0; s < n; ++s) forces_from(*F, raw[2 * s], raw[2 * s + 1], &h[2 * s], &h[2 * s + 1]);
+    return n;
+}
+
+int ibpm_batch_cg_iters(ibpm_batch* B, int k, int* h, int max_sub) {
+    Flow* F = flow_of(B, k);
+    if (!F) return -1;
+    const int n = std::min(max_sub, B->nsub);
+    for (int i = 0; i < n; ++i) h[i] = F->cg ? F->cg_iters[i] : -1;
+    return n;
+}
+
+double ibpm_batch_time(ibpm_batch* B) { return B ? B->time : 0.0; }
+
+}  // extern "C"

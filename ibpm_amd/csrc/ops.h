@@ -1,0 +1,77 @@
+// ops.h -- per-flow descriptors and launchers of the projection-step kernels.
+#pragma once
+#include "common.h"
+
+namespace ibpmg {
+
+// Stencil kernels (nonlinear term, right-hand side, flux refresh).
+struct FlowOps {
+    int nx, ny, L;
+    double dx;                     // finest spacing
+    double *q, *omega, *U, *V, *cross, *a, *nprev, *psi;
+    double lap_coef, an, bn, dt;   // a = w_coef omega + dt (lap_coef Lap + an N + bn Nprev)
+    double w_coef;
+    int use_nprev, nprev_valid;
+    double ubase, vbase;           // uniform base flow (magnitude 1, angle alpha)
+};
+
+// Regulariser gathers (E, and beta Curl E^T) of one flow.
+struct RegOps {
+    int nx, ny, L, nb;
+    double dx, inv_dx, bydx2, beta;
+    // E: per boundary component k, level-0 flux indices and weights, ascending
+    const int* e_ptr;
+    const int* e_f;
+    const double* e_w;
+    const double* psi;             // streamfunction whose curl is interpolated
+    const double* b;               // subtracted constraint (or null)
+    double* out;                   // 2 nb outputs
+    // E^T then Curl: flux list (CSR over components, ascending) and nodes
+    int nnodes;
+    const int* t_ptr;
+    const int* t_k;
+    const double* t_w;
+    const int4* node_nbr;          // flux-list slots of Y(i,j), Y(i-1,j), X(i,j-1), X(i,j)
+    const int* node_idx;           // level-0 scalar index of each node
+    const double* fin;             // boundary vector smeared
+    double* cout;                  // scalar field written at the nodes
+    int nold;
+    const int* old_idx;            // nodes of the previous association to clear
+};
+
+struct MinvOps {
+    int n;
+    const double* minv;            // n x n row-major
+    const double* rhs;
+    double* x;
+};
+
+struct CgOps {
+    int n;
+    double *x, *r, *d, *q;
+    const double* b;
+    double* state;                 // delta, tol^2, iterations, converged
+};
+
+struct ForceOps {
+    int nb;
+    const double* f;
+    double* hist;                  // (xF, yF) per step
+    int* count;                    // steps recorded so far (device counter)
+};
+
+int blocks_for_pass(const EllJob& J, int pass);
+void launch_ell_pass(int pass, const EllJob* d_jobs, int njobs, int max_blocks, hipStream_t s);
+void launch_coarsify(const CoarsifyJob* d_jobs, int njobs, int max_points, hipStream_t s);
+void launch_vel(const FlowOps* d, int nflows, long max_pts, hipStream_t s);
+void launch_cross(const FlowOps* d, int nflows, long max_nf, int lev, hipStream_t s);
+void launch_rhs(const FlowOps* d, int nflows, long max_pts, hipStream_t s);
+void launch_curl_q(const FlowOps* d, int nflows, long max_nf_all, hipStream_t s);
+void launch_interp(const RegOps* d, int nflows, int max_comp, hipStream_t s);
+void launch_smear_curl(const RegOps* d, int nflows, int max_nodes, hipStream_t s);
+void launch_clear_nodes(const RegOps* d, int nflows, int max_nodes, hipStream_t s);
+void launch_minv(const MinvOps* d, int nflows, int max_n, hipStream_t s);
+void launch_cg(const CgOps* d, int nflows, int mode, hipStream_t s);
+void launch_forces(const ForceOps* d, int nflows, hipStream_t s);
+
+}  // namespace ibpmg

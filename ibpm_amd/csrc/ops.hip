@@ -1,0 +1,461 @@
+// ops.hip -- memory-bound stencils, regulariser gathers and force-solve
+// kernels of the projection step (FP64).  Every kernel takes an array of
+// per-flow descriptors and is launched with blockIdx.y = flow, so one launch
+// advances a whole batch of flows of uneven size (blocks past a flow's extent
+// exit at once).
+#include "ops.h"
+
+namespace ibpmg {
+
+// ------------------------------------------------------------ accessors
+struct Lay {
+    int nx, ny, L;
+    __device__ long ns() const { return (long)(nx - 1) * (ny - 1); }
+    __device__ long nf() const { return 2L * nx * ny + nx + ny; }
+    __device__ long nfx() const { return (long)(nx + 1) * ny; }
+    __device__ long s(int lev, int i, int j) const { return lev * ns() + (long)(i - 1) * (ny - 1) + (j - 1); }
+    __device__ long fx(int lev, int i, int j) const { return lev * nf() + (long)i * ny + j; }
+    __device__ long fy(int lev, int i, int j) const { return lev * nf() + nfx() + (long)i * (ny + 1) + j; }
+};
+
+// getBC of level lev from level lev+1 (reference Scalar.cc:91-127); zero on
+// the coarsest level (VectorOperations.cc:74-77).
+__device__ inline double bcb(const Lay& g, const double* F, int lev, int i) {
+    if (lev == g.L - 1) return 0.0;
+    const int ii = i / 2 + nx_ext(g.nx), jj = nx_ext(g.ny);
+    const double a = F[g.s(lev + 1, ii, jj)];
+    return (i & 1) ? 0.5 * (a + F[g.s(lev + 1, ii + 1, jj)]) : a;
+}
+__device__ inline double bct(const Lay& g, const double* F, int lev, int i) {
+    if (lev == g.L - 1) return 0.0;
+    const int ii = i / 2 + nx_ext(g.nx), jj = g.ny / 2 + nx_ext(g.ny);
+    const double a = F[g.s(lev + 1, ii, jj)];
+    return (i & 1) ? 0.5 * (a + F[g.s(lev + 1, ii + 1, jj)]) : a;
+}
+__device__ inline double bcl(const Lay& g, const double* F, int lev, int j) {
+    if (lev == g.L - 1) return 0.0;
+    const int ii = nx_ext(g.nx), jj = j / 2 + nx_ext(g.ny);
+    const double a = F[g.s(lev + 1, ii, jj)];
+    return (j & 1) ? 0.5 * (a + F[g.s(lev + 1, ii, jj + 1)]) : a;
+}
+__device__ inline double bcr(const Lay& g, const double* F, int lev, int j) {
+    if (lev == g.L - 1) return 0.0;
+    const int ii = g.nx / 2 + nx_ext(g.nx), jj = j / 2 + nx_ext(g.ny);
+    const double a = F[g.s(lev + 1, ii, jj)];
+    return (j & 1) ? 0.5 * (a + F[g.s(lev + 1, ii, jj + 1)]) : a;
+}
+
+// Curl(Scalar -> Flux) at one flux (reference VectorOperations.cc:63-123).
+__device__ inline double curl_x(const Lay& g, const double* F, int lev, int i, int j) {
+    if (i == 0) return bcl(g, F, lev, j + 1) - bcl(g, F, lev, j);
+    if (i == g.nx) return bcr(g, F, lev, j + 1) - bcr(g, F, lev, j);
+    if (j == 0) return F[g.s(lev, i, 1)] - bcb(g, F, lev, i);
+    if (j == g.ny - 1) return bct(g, F, lev, i) - F[g.s(lev, i, g.ny - 1)];
+    return F[g.s(lev, i, j + 1)] - F[g.s(lev, i, j)];
+}
+__device__ inline double curl_y(const Lay& g, const double* F, int lev, int i, int j) {
+    if (j == 0) return bcb(g, F, lev, i) - bcb(g, F, lev, i + 1);
+    if (j == g.ny) return bct(g, F, lev, i) - bct(g, F, lev, i + 1);
+    if (i == 0) return bcl(g, F, lev, j) - F[g.s(lev, 1, j)];
+    if (i == g.nx - 1) return F[g.s(lev, g.nx - 1, j)] - bcr(g, F, lev, j);
+    return F[g.s(lev, i, j)] - F[g.s(lev, i + 1, j)];
+}
+
+// FluxToXVelocity / FluxToYVelocity at one node (VectorOperations.cc:488-665).
+// The central nodes of coarse levels are never read downstream and are 0.
+__device__ inline double xvel(const Lay& g, const double* q, double dx0, int lev, int i, int j) {
+    if (lev == 0) return (q[g.fx(0, i, j)] + q[g.fx(0, i, j - 1)]) * (1. / (2 * dx0));
+    const int ilo = nx_ext(g.nx), ihi = g.nx / 2 + nx_ext(g.nx);
+    const int jlo = nx_ext(g.ny), jhi = g.ny / 2 + nx_ext(g.ny);
+    const double bydx = 1. / (dx0 * (1 << lev));
+    const bool inI = i > ilo && i < ihi, inJ = j > jlo && j < jhi;
+    const bool onI = i == ilo || i == ihi, onJ = j == jlo || j == jhi;
+    if (inI && inJ) return 0.0;
+    const int f = lev - 1;
+    if (onI && onJ) {   // corners F
+        if (j == jlo) {
+            const double c = (i == ilo) ? q[g.fx(f, 1, 0)] : q[g.fx(f, g.nx - 1, 0)];
+            return (q[g.fx(lev, i, j - 1)] * 8. / 15 + q[g.fx(lev, i, j)] * 6. / 15 + c * 2. / 15) * bydx;
+        }
+        const double c = (i == ilo) ? q[g.fx(f, 1, g.ny - 1)] : q[g.fx(f, g.nx - 1, g.ny - 1)];
+        return (q[g.fx(lev, i, j)] * 8. / 15 + q[g.fx(lev, i, j - 1)] * 6. / 15 + c * 2. / 15) * bydx;
+    }
+    if (inI && onJ) {   // interfaces E
+        const int ii = (i - ilo) * 2;
+        if (j == jlo) {
+            const int jj = 0;
+            return (q[g.fx(lev, i, jlo - 1)] * 2. / 3 + q[g.fx(f, ii, jj)] * 1. / 3
+                    + (q[g.fx(f, ii - 1, jj)] + q[g.fx(f, ii + 1, jj)]) * 1. / 6) * bydx;
+        }
+        const int jj = (jhi - jlo) * 2;
+        return (q[g.fx(lev, i, jhi)] * 2. / 3 + q[g.fx(f, ii, jj - 1)] * 1. / 3
+                + (q[g.fx(f, ii - 1, jj - 1)] + q[g.fx(f, ii + 1, jj - 1)]) * 1. / 6) * bydx;
+    }
+    return (q[g.fx(lev, i, j)] + q[g.fx(lev, i, j - 1)]) * 0.5 * bydx;
+}
+
+__device__ inline double yvel(const Lay& g, const double* q, double dx0, int lev, int i, int j) {
+    if (lev == 0) return (q[g.fy(0, i - 1, j)] + q[g.fy(0, i, j)]) * (1. / (2 * dx0));
+    const int ilo = nx_ext(g.nx), ihi = g.nx / 2 + nx_ext(g.nx);
+    const int jlo = nx_ext(g.ny), jhi = g.ny / 2 + nx_ext(g.ny);
+    const double bydx = 1. / (dx0 * (1 << lev));
+    const bool inI = i > ilo && i < ihi, inJ = j > jlo && j < jhi;
+    const bool onI = i == ilo || i == ihi, onJ = j == jlo || j == jhi;
+    if (inI && inJ) return 0.0;
+    const int f = lev - 1;
+    if (onI && onJ) {   // corners F
+        if (i == ilo) {
+            const double c = (j == jlo) ? q[g.fy(f, 0, 1)] : q[g.fy(f, 0, g.ny - 1)];
+            return (q[g.fy(lev, i - 1, j)] * 8. / 15 + q[g.fy(lev, i, j)] * 6. / 15 + c * 2. / 15) * bydx;
+        }
+        const double c = (j == jlo) ? q[g.fy(f, g.nx - 1, 1)] : q[g.fy(f, g.nx - 1, g.ny - 1)];
+        return (q[g.fy(lev, i, j)] * 8. / 15 + q[g.fy(lev, i - 1, j)] * 6. / 15 + c * 2. / 15) * bydx;
+    }
+    if (onI && inJ) {   // interfaces E
+        const int jj = (j - jlo) * 2;
+        if (i == ilo) {
+            const int ii = 0;
+            return (q[g.fy(lev, ilo - 1, j)] * 2. / 3 + q[g.fy(f, ii, jj)] * 1. / 3
+                    + (q[g.fy(f, ii, jj - 1)] + q[g.fy(f, ii, jj + 1)]) * 1. / 6) * bydx;
+        }
+        const int ii = (ihi - ilo) * 2;
+        return (q[g.fy(lev, ihi, j)] * 2. / 3 + q[g.fy(f, ii - 1, jj)] * 1. / 3
+                + (q[g.fy(f, ii - 1, jj - 1)] + q[g.fy(f, ii - 1, jj + 1)]) * 1. / 6) * bydx;
+    }
+    return (q[g.fy(lev, i, j)] + q[g.fy(lev, i - 1, j)]) * 0.5 * bydx;
+}
+
+// XVelocityToFlux / YVelocityToFlux at one flux of level lev
+// (VectorOperations.cc:669-915).  C holds the already finished finer level of
+// the output, used by the coarse-level region G.
+__device__ inline double xflux_from(const Lay& g, const double* S, const double* C, double dx0,
+                                    int lev, int i, int j) {
+    const double dx = dx0 * (1 << lev);
+    const int nx2 = nx_ext(g.nx), ny2 = nx_ext(g.ny);
+    if (i == 0 || i == g.nx) {   // A
+        if (lev == g.L - 1) return 0.0;
+        const int ic = (i == 0) ? nx2 : g.nx / 2 + nx2;
+        const int jj = j / 2 + ny2;
+        const double a = S[g.s(lev + 1, ic, jj)], b = S[g.s(lev + 1, ic, jj + 1)];
+        return (j & 1) ? (0.25 * a + 0.75 * b) * dx : (0.75 * a + 0.25 * b) * dx;
+    }
+    if (j == 0 || j == g.ny - 1) {   // C
+        const int jin = (j == 0) ? 1 : g.ny - 1;
+        const double s0 = S[g.s(lev, i, jin)];
+        if (lev == g.L - 1) return s0 * 0.5 * dx;
+        const int jc = (j == 0) ? ny2 : g.ny / 2 + ny2;
+        const int ii = i / 2 + nx2;
+        if ((i & 1) == 0) return (s0 + S[g.s(lev + 1, ii, jc)]) * 0.5 * dx;
+        return (0.5 * s0 + 0.25 * S[g.s(lev + 1, ii, jc)] + 0.25 * S[g.s(lev + 1, ii + 1, jc)]) * dx;
+    }
+    if (lev > 0 && i > nx2 && i < g.nx / 2 + nx2 && j >= ny2 && j < g.ny / 2 + ny2) {   // G
+        const int ii = 2 * (i - nx2), jj = 2 * (j - ny2);
+        return C[g.fx(lev - 1, ii, jj)] + C[g.fx(lev - 1, ii, jj + 1)];
+    }
+    return (S[g.s(lev, i, j)] + S[g.s(lev, i, j + 1)]) * 0.5 * dx;
+}
+
+__device__ inline double yflux_from(const Lay& g, const double* S, const double* C, double dx0,
+                                    int lev, int i, int j) {
+    const double dx = dx0 * (1 << lev);
+    const int nx2 = nx_ext(g.nx), ny2 = nx_ext(g.ny);
+    if (j == 0 || j == g.ny) {   // A
+        if (lev == g.L - 1) return 0.0;
+        const int jc = (j == 0) ? ny2 : g.ny / 2 + ny2;
+        const int ii = i / 2 + nx2;
+        const double a = S[g.s(lev + 1, ii, jc)], b = S[g.s(lev + 1, ii + 1, jc)];
+        return (i & 1) ? (0.25 * a + 0.75 * b) * dx : (0.75 * a + 0.25 * b) * dx;
+    }
+    if (i == 0 || i == g.nx - 1) {   // C
+        const int iin = (i == 0) ? 1 : g.nx - 1;
+        const double s0 = S[g.s(lev, iin, j)];
+        if (lev == g.L - 1) return s0 * 0.5 * dx;
+        const int ic = (i == 0) ? nx2 : g.nx / 2 + nx2;
+        const int jj = j / 2 + ny2;
+        if ((j & 1) == 0) return (s0 + S[g.s(lev + 1, ic, jj)]) * 0.5 * dx;
+        return (0.5 * s0 + 0.25 * S[g.s(lev + 1, ic, jj)] + 0.25 * S[g.s(lev + 1, ic, jj + 1)]) * dx;
+    }
+    if (lev > 0 && j > ny2 && j < g.ny / 2 + ny2 && i >= nx2 && i < g.nx / 2 + nx2) {   // G
+        const int ii = 2 * (i - nx2), jj = 2 * (j - ny2);
+        return C[g.fy(lev - 1, ii, jj)] + C[g.fy(lev - 1, ii + 1, jj)];
+    }
+    return (S[g.s(lev, i, j)] + S[g.s(lev, i + 1, j)]) * 0.5 * dx;
+}
+
+// ------------------------------------------------------------- kernels
+// U = -omega * FluxToX(q), V = omega * FluxToY(q) on every level
+// (CrossProduct, VectorOperations.cc:438-457).
+__global__ void __launch_bounds__(kThreads) k_vel(const FlowOps* __restrict__ flows) {
+    const FlowOps& F = flows[blockIdx.y];
+    const Lay g{F.nx, F.ny, F.L};
+    const long t = (long)blockIdx.x * kThreads + threadIdx.x;
+    if (t >= g.L * g.ns()) return;
+    const int lev = (int)(t / g.ns());
+    const long r = t - lev * g.ns();
+    const int i = (int)(r / (g.ny - 1)) + 1, j = (int)(r % (g.ny - 1)) + 1;
+    const double w = F.omega[t];
+    double u = xvel(g, F.q, F.dx, lev, i, j);
+    u *= w;
+    u *= -1;
+    double v = yvel(g, F.q, F.dx, lev, i, j);
+    v *= w;
+    F.U[t] = u;
+    F.V[t] = v;
+}
+
+// cross = VelocityToFlux(V, U) on one level (levels run fine -> coarse).
+__global__ void __launch_bounds__(kThreads) k_cross(const FlowOps* __restrict__ flows, int lev) {
+    const FlowOps& F = flows[blockIdx.y];
+    const Lay g{F.nx, F.ny, F.L};
+    if (lev >= g.L) return;
+    const long t = (long)blockIdx.x * kThreads + threadIdx.x;
+    if (t >= g.nf()) return;
+    double v;
+    if (t < g.nfx()) {
+        const int i = (int)(t / g.ny), j = (int)(t % g.ny);
+        v = xflux_from(g, F.V, F.cross, F.dx, lev, i, j);
+    } else {
+        const long u = t - g.nfx();
+        const int i = (int)(u / (g.ny + 1)), j = (int)(u % (g.ny + 1));
+        v = yflux_from(g, F.U, F.cross, F.dx, lev, i, j);
+    }
+    F.cross[lev * g.nf() + t] = v;
+}
+
+// N = Curl(cross) (IBSolver.cc:205-209) and the right-hand side
+// a = omega + dt (c_L Lap(omega) + a_n N + b_n N_prev) (IBSolver.cc:169-183),
+// Lap = -Curl(Curl(.)) (VectorOperations.cc:133-140).  N replaces N_prev.
+__global__ void __launch_bounds__(kThreads) k_rhs(const FlowOps* __restrict__ flows) {
+    const FlowOps& F = flows[blockIdx.y];
+    const Lay g{F.nx, F.ny, F.L};
+    const long t = (long)blockIdx.x * kThreads + threadIdx.x;
+    if (t >= g.L * g.ns()) return;
+    const int lev = (int)(t / g.ns());
+    const long r = t - lev * g.ns();
+    const int i = (int)(r / (g.ny - 1)) + 1, j = (int)(r % (g.ny - 1)) + 1;
+    const double d = F.dx * (1 << lev);
+    const double bydx2 = 1. / (d * d);
+    const double* C = F.cross;
+    const double n = (C[g.fy(lev, i, j)] - C[g.fy(lev, i - 1, j)]
+                      + C[g.fx(lev, i, j - 1)] - C[g.fx(lev, i, j)]) * bydx2;
+    const double* W = F.omega;
+    double lap = (curl_y(g, W, lev, i, j) - curl_y(g, W, lev, i - 1, j)
+                  + curl_x(g, W, lev, i, j - 1) - curl_x(g, W, lev, i, j)) * bydx2;
+    lap *= -1;
+    double a = lap * F.lap_coef;
+    a += F.an * n;
+    if (F.use_nprev) a += F.bn * (F.nprev_valid ? F.nprev[t] : n);
+    a *= F.dt;
+    a += F.w_coef * W[t];
+    F.a[t] = a;
+    F.nprev[t] = n;
+}
+
+// q = Curl(psi) + q0 on every level (NavierStokesModel.cc:121-129,
+// uniform base flow Flux.cc:79-99).
+__global__ void __launch_bounds__(kThreads) k_curl_q(const FlowOps* __restrict__ flows) {
+    const FlowOps& F = flows[blockIdx.y];
+    const Lay g{F.nx, F.ny, F.L};
+    const long t = (long)blockIdx.x * kThreads + threadIdx.x;
+    if (t >= g.L * g.nf()) return;
+    const int lev = (int)(t / g.nf());
+    const long r = t - lev * g.nf();
+    const double d = F.dx * (1 << lev);
+    double v;
+    if (r < g.nfx()) {
+        const int i = (int)(r / g.ny), j = (int)(r % g.ny);
+        v = curl_x(g, F.psi, lev, i, j) + F.ubase * d;
+    } else {
+        const long u = r - g.nfx();
+        const int i = (int)(u / (g.ny + 1)), j = (int)(u % (g.ny + 1));
+        v = curl_y(g, F.psi, lev, i, j) + F.vbase * d;
+    }
+    F.q[t] = v;
+}
+
+// ---------------------------------------------------------- regulariser
+// Level-0 flux of Curl(psi) at a flat flux index.
+__device__ inline double curl_flat(const Lay& g, const double* psi, int fidx) {
+    if (fidx < g.nfx()) return curl_x(g, psi, 0, fidx / g.ny, fidx % g.ny);
+    const int u = fidx - (int)g.nfx();
+    return curl_y(g, psi, 0, u / (g.ny + 1), u % (g.ny + 1));
+}
+
+// f[k] = E(Curl psi)[k] - b[k]: toBoundary (Regularizer.cc:111-128) of the
+// flux of psi, each component summed in ascending flux order as the
+// reference's association list does; b may be null.
+__global__ void __launch_bounds__(kThreads) k_interp(const RegOps* __restrict__ regs) {
+    const RegOps& R = regs[blockIdx.y];
+    const int k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= 2 * R.nb) return;
+    const Lay g{R.nx, R.ny, R.L};
+    double s = 0.0;
+    for (int a = R.e_ptr[k]; a < R.e_ptr[k + 1]; ++a) s += R.e_w[a] * curl_flat(g, R.psi, R.e_f[a]);
+    s *= R.inv_dx;
+    if (R.b) s -= R.b[k];
+    R.out[k] = s;
+}
+
+// c = beta * Curl(E^T f) at the level-0 nodes touched by the regulariser
+// (NavierStokesModel.cc:101-105, Regularizer.cc:92-109, ProjectionSolver.cc:85-88).
+__device__ inline double smear_at(const RegOps& R, int fl) {
+    if (fl < 0) return 0.0;
+    double s = 0.0;
+    for (int a = R.t_ptr[fl]; a < R.t_ptr[fl + 1]; ++a) s += R.t_w[a] * R.fin[R.t_k[a]];
+    return s * R.dx;
+}
+
+__global__ void __launch_bounds__(kThreads) k_smear_curl(const RegOps* __restrict__ regs) {
+    const RegOps& R = regs[blockIdx.y];
+    const int m = blockIdx.x * kThreads + threadIdx.x;
+    if (m >= R.nnodes) return;
+    const int4 nb = R.node_nbr[m];   // flux-list slots of Y(i,j), Y(i-1,j), X(i,j-1), X(i,j)
+    const double w = (smear_at(R, nb.x) - smear_at(R, nb.y) + smear_at(R, nb.z) - smear_at(R, nb.w)) * R.bydx2;
+    R.cout[R.node_idx[m]] = w * R.beta;
+}
+
+__global__ void __launch_bounds__(kThreads) k_clear_nodes(const RegOps* __restrict__ regs) {
+    const RegOps& R = regs[blockIdx.y];
+    const int m = blockIdx.x * kThreads + threadIdx.x;
+    if (m >= R.nold) return;
+    R.cout[R.old_idx[m]] = 0.0;
+}
+
+// ------------------------------------------------------------ force solve
+// x = Minv * rhs with Minv = (L L^T)^-1 assembled in FP64 from the Cholesky
+// factor (CholeskySolver.cc:200-224 computes the same product by forward and
+// back substitution).  One wave per row, rows spread over the whole GPU.
+__global__ void __launch_bounds__(kThreads) k_minv(const MinvOps* __restrict__ mops) {
+    const MinvOps& P = mops[blockIdx.y];
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int row = blockIdx.x * (kThreads / 64) + wave;
+    if (row >= P.n) return;
+    const double* a = P.minv + (long)row * P.n;
+    double s = 0.0;
+    for (int k = lane; k < P.n; k += 64) s += a[k] * P.rhs[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) P.x[row] = s;
+}
+
+// ------------------------------------------------------------------- CG
+// Conjugate gradient on M f = rhs (ConjugateGradientSolver.cc:35-65), one
+// workgroup per flow; M d is applied by the solve kernels between calls.
+__device__ inline double block_sum(double v, double* red) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    __syncthreads();
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) {
+        for (int w = 0; w < kThreads / 64; ++w) s += red[w];
+        red[kThreads / 64] = s;
+    }
+    __syncthreads();
+    return red[kThreads / 64];
+}
+
+// mode 0: start   r = b - q (q = M x0), d = r, delta = r.r
+// mode 1: iterate alpha = delta/(d.q), x += alpha d, r -= alpha q,
+//                 delta' = r.r, beta = delta'/delta, d = r + beta d
+__global__ void __launch_bounds__(kThreads) k_cg(const CgOps* __restrict__ cgs, int mode) {
+    const CgOps& C = cgs[blockIdx.x];
+    __shared__ double red[kThreads / 64 + 1];
+    double* st = C.state;   // [delta, tol2, iterations, converged]
+    const int n = C.n;
+    if (mode == 0) {
+        double s = 0.0;
+        for (int k = threadIdx.x; k < n; k += kThreads) {
+            double r = C.b[k];
+            r -= C.q[k];
+            C.r[k] = r;
+            C.d[k] = r;
+            s += r * r;
+        }
+        const double delta = block_sum(s, red);
+        if (threadIdx.x == 0) {
+            st[0] = delta;
+            st[2] = 0;
+            st[3] = (delta > st[1]) ? 0.0 : 1.0;
+        }
+        return;
+    }
+    if (st[3] != 0.0) return;   // converged earlier: this iteration is a no-op
+    double s = 0.0;
+    for (int k = threadIdx.x; k < n; k += kThreads) s += C.d[k] * C.q[k];
+    const double dq = block_sum(s, red);
+    const double delta = st[0];
+    const double alpha = delta / dq;
+    s = 0.0;
+    for (int k = threadIdx.x; k < n; k += kThreads) {
+        C.x[k] += alpha * C.d[k];
+        double r = C.r[k] - alpha * C.q[k];
+        C.r[k] = r;
+        s += r * r;
+    }
+    const double dnew = block_sum(s, red);
+    const double beta = dnew / delta;
+    for (int k = threadIdx.x; k < n; k += kThreads) C.d[k] = C.r[k] + beta * C.d[k];
+    if (threadIdx.x == 0) {
+        st[0] = dnew;
+        st[2] += 1;
+        st[3] = (dnew > st[1]) ? 0.0 : 1.0;
+    }
+}
+
+// Net force sum over the boundary points in point order (State.cc:53-63;
+// the dx^2 factor and the rotation to Cd/Cl are applied on the host).
+__global__ void k_forces(const ForceOps* __restrict__ fo) {
+    const ForceOps& F = fo[blockIdx.x];
+    if (threadIdx.x != 0) return;
+    double xf = 0, yf = 0;
+    for (int k = 0; k < F.nb; ++k) {
+        xf += F.f[k];
+        yf += F.f[F.nb + k];
+    }
+    const int st = *F.count;
+    F.hist[2 * st] = xf;
+    F.hist[2 * st + 1] = yf;
+    *F.count = st + 1;
+}
+
+// ------------------------------------------------------------ host side
+static int blocks_for(long n) { return (int)((n + kThreads - 1) / kThreads); }
+
+void launch_vel(const FlowOps* d, int nflows, long max_pts, hipStream_t s) {
+    hipLaunchKernelGGL(k_vel, dim3(blocks_for(max_pts), nflows), dim3(kThreads), 0, s, d);
+}
+void launch_cross(const FlowOps* d, int nflows, long max_nf, int lev, hipStream_t s) {
+    hipLaunchKernelGGL(k_cross, dim3(blocks_for(max_nf), nflows), dim3(kThreads), 0, s, d, lev);
+}
+void launch_rhs(const FlowOps* d, int nflows, long max_pts, hipStream_t s) {
+    hipLaunchKernelGGL(k_rhs, dim3(blocks_for(max_pts), nflows), dim3(kThreads), 0, s, d);
+}
+void launch_curl_q(const FlowOps* d, int nflows, long max_nf_all, hipStream_t s) {
+    hipLaunchKernelGGL(k_curl_q, dim3(blocks_for(max_nf_all), nflows), dim3(kThreads), 0, s, d);
+}
+void launch_interp(const RegOps* d, int nflows, int max_comp, hipStream_t s) {
+    hipLaunchKernelGGL(k_interp, dim3(blocks_for(max_comp), nflows), dim3(kThreads), 0, s, d);
+}
+void launch_smear_curl(const RegOps* d, int nflows, int max_nodes, hipStream_t s) {
+    hipLaunchKernelGGL(k_smear_curl, dim3(blocks_for(max_nodes), nflows), dim3(kThreads), 0, s, d);
+}
+void launch_clear_nodes(const RegOps* d, int nflows, int max_nodes, hipStream_t s) {
+    hipLaunchKernelGGL(k_clear_nodes, dim3(blocks_for(max_nodes), nflows), dim3(kThreads), 0, s, d);
+}
+void launch_minv(const MinvOps* d, int nflows, int max_n, hipStream_t s) {
+    const int rows_per_block = kThreads / 64;
+    hipLaunchKernelGGL(k_minv, dim3((max_n + rows_per_block - 1) / rows_per_block, nflows),
+                       dim3(kThreads), 0, s, d);
+}
+void launch_cg(const CgOps* d, int nflows, int mode, hipStream_t s) {
+    hipLaunchKernelGGL(k_cg, dim3(nflows), dim3(kThreads), 0, s, d, mode);
+}
+
+void launch_forces(const ForceOps* d, int nflows, hipStream_t s) {
+    hipLaunchKernelGGL(k_forces, dim3(nflows), dim3(64), 0, s, d);
+}
+
+}  // namespace ibpmg

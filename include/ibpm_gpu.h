@@ -1,0 +1,110 @@
+/*
+ * ibpm_gpu.h -- C ABI of the gfx950 (MI355X) projection-step library.
+ *
+ * Plain pointers and sizes only.  Every call returns 0 on success and a
+ * negative value on failure (never aborts); ibpm_last_error() then describes
+ * the failure.  Calls are ordered on the context's single HIP stream; a
+ * context is not thread-safe.  "d_" pointers are device pointers (allocate
+ * them with ibpm_malloc or any HIP allocator on the context's device).
+ *
+ * Layouts are the reference's (so a host Scalar/Flux/BoundaryVector buffer
+ * copies 1:1):
+ *   Scalar  [lev][i=1..nx-1][j=1..ny-1]           reference src/Scalar.cc:81-89
+ *   Flux    [lev][X (nx+1)*ny | Y nx*(ny+1)]       reference src/Flux.cc:42-50
+ *   BoundaryVector [X_0..X_{n-1}, Y_0..Y_{n-1}]    reference src/BoundaryVector.h:65-69
+ *   BC ring 2(nx+ny): left(0..ny), top(1..nx), right(ny-1..0), bottom(nx-1..1)
+ *                                                  reference src/BC.h:22-79
+ */
+#ifndef IBPM_GPU_H
+#define IBPM_GPU_H
+#include <stddef.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ibpm_ctx ibpm_ctx;
+typedef struct ibpm_batch ibpm_batch;
+
+/* ---- errors, devices, memory -------------------------------------------- */
+const char* ibpm_last_error(void);
+int ibpm_device_count(void);
+int ibpm_ctx_create(int device, ibpm_ctx** out);
+int ibpm_ctx_destroy(ibpm_ctx* ctx);
+int ibpm_ctx_synchronize(ibpm_ctx* ctx);
+int ibpm_malloc(ibpm_ctx* ctx, size_t bytes, void** d_ptr);
+int ibpm_free(ibpm_ctx* ctx, void* d_ptr);
+int ibpm_memcpy_h2d(ibpm_ctx* ctx, void* d_dst, const void* h_src, size_t bytes);
+int ibpm_memcpy_d2h(ibpm_ctx* ctx, void* h_dst, const void* d_src, size_t bytes);
+
+/* ---- elliptic solves (EllipticSolver2d / EllipticSolver) ------------------ */
+/* Unnormalised 2-D DST-I (RODFT00 x RODFT00) of an n1 x n2 row-major array.
+   Replaces EllipticSolver2d::sinTransform, reference src/EllipticSolver2d.cc:57-71. */
+int ibpm_dst2d(ibpm_ctx* ctx, int n1, int n2, const double* d_in, double* d_out);
+
+/* Single-level solve with boundary values: kind 0 Poisson (L u = f),
+   1 Helmholtz ((1 + alpha L) u = f); d_bc may be NULL (zero boundary).
+   Replaces EllipticSolver2d::solve(f, [bc,] u), src/EllipticSolver2d.cc:87-104. */
+int ibpm_elliptic2d_solve(ibpm_ctx* ctx, int kind, int nx, int ny, double dx, double alpha,
+                          const double* d_f, const double* d_bc, double* d_u);
+
+/* Multi-domain solve over ngrid levels (coarsify, coarsest with zero BC,
+   finer levels with BCs from the coarser solution).  d_f and d_u may alias.
+   Replaces EllipticSolver::solve, src/EllipticSolver.cc:47-73. */
+int ibpm_elliptic_solve(ibpm_ctx* ctx, int kind, int nx, int ny, int ngrid, double dx,
+                        double alpha, const double* d_f, double* d_u);
+
+/* ---- grid operators (VectorOperations) ----------------------------------- */
+/* Curl(Flux -> Scalar), src/VectorOperations.cc:33-54 */
+int ibpm_curl_flux(ibpm_ctx* ctx, int nx, int ny, int ngrid, double dx, const double* d_q, double* d_w);
+/* Curl(Scalar -> Flux) + uniform flow (u0, v0) * dx_lev, src/VectorOperations.cc:63-123,
+   src/Flux.cc:79-99 (pass u0 = v0 = 0 for the bare curl) */
+int ibpm_curl_scalar(ibpm_ctx* ctx, int nx, int ny, int ngrid, double dx, double u0, double v0,
+                     const double* d_w, double* d_q);
+/* NonlinearIBSolver::N = Curl(CrossProduct(q, omega)), src/IBSolver.cc:205-209 */
+int ibpm_nonlinear(ibpm_ctx* ctx, int nx, int ny, int ngrid, double dx, const double* d_q,
+                   const double* d_w, double* d_n);
+/* Laplacian = -Curl(Curl(omega)), src/VectorOperations.cc:133-140 */
+int ibpm_laplacian(ibpm_ctx* ctx, int nx, int ny, int ngrid, double dx, const double* d_w, double* d_out);
+
+/* ---- whole projection-step solver (IBSolver / ProjectionSolver / ...) ----- */
+typedef struct {
+    int nx, ny, ngrid;          /* Grid(nx, ny, ngrid, length, xoffset, yoffset), src/Grid.h:34-41 */
+    double length, xoffset, yoffset;
+    double reynolds;            /* NavierStokesModel(grid, geom, Re, q0) */
+    double dt;
+    int scheme;                 /* 0 Euler, 1 AB2, 2 RK3, 3 RK3b (src/Scheme.h:14-56) */
+    double alpha_deg;           /* base-flow angle in degrees (ibpm.cc -alpha) */
+    double cg_tol;              /* CG absolute residual tolerance (IBSolver setTol) */
+    int force_cg;               /* 1: CG even for stationary bodies */
+    const char* geom;           /* .geom file text (src/Geometry.cc:143-191) */
+} ibpm_flow_params;
+
+/* A batch of independent flows advanced together by the same kernels. */
+int ibpm_batch_create(ibpm_ctx* ctx, int nflows, const ibpm_flow_params* params, ibpm_batch** out);
+int ibpm_batch_destroy(ibpm_batch* b);
+/* ibpm.cc:283-305: move bodies, regulariser update, Cholesky build (stationary)
+   or CG set-up (moving), refreshState. */
+int ibpm_batch_init(ibpm_batch* b);
+/* IBSolver::advance, src/IBSolver.cc:142-150, nsteps times. */
+int ibpm_batch_advance(ibpm_batch* b, int nsteps);
+/* Record each step's net force on the device (State::computeNetForce). */
+int ibpm_batch_num_points(ibpm_batch* b, int flow);
+int ibpm_batch_get_omega(ibpm_batch* b, int flow, double* h_out);
+int ibpm_batch_set_omega(ibpm_batch* b, int flow, const double* h_in);   /* + refreshState */
+int ibpm_batch_get_q(ibpm_batch* b, int flow, double* h_out);
+int ibpm_batch_get_f(ibpm_batch* b, int flow, double* h_out);
+int ibpm_batch_get_points(ibpm_batch* b, int flow, double* h_xy);
+/* Cd and Cl after the last step, ibpm.cc:346-355 (2 x the rotated net force). */
+int ibpm_batch_get_forces(ibpm_batch* b, int flow, double* cd, double* cl);
+/* Force history: (Cd, Cl) of every step since init, 2 doubles per step. */
+int ibpm_batch_force_history(ibpm_batch* b, int flow, double* h_out, int max_steps);
+/* CG iterations of each substep of the last step (-1 when Cholesky is used). */
+int ibpm_batch_cg_iters(ibpm_batch* b, int flow, int* h_iters, int max_substeps);
+double ibpm_batch_time(ibpm_batch* b);
+/* Capture one timestep as a hipGraph and replay it (stationary bodies). */
+int ibpm_batch_use_graph(ibpm_batch* b, int enable);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

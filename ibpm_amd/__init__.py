@@ -12,7 +12,8 @@ from dataclasses import dataclass
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libibpm_gpu.so")
+# IBPM_GPU_LIB selects another build of the library (e.g. a tuning variant).
+LIB_PATH = os.environ.get("IBPM_GPU_LIB", os.path.join(HERE, "libibpm_gpu.so"))
 
 _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int)
@@ -72,9 +73,12 @@ def load():
         "ibpm_batch_force_history": [vp, ctypes.c_int, vp, ctypes.c_int],
         "ibpm_batch_cg_iters": [vp, ctypes.c_int, vp, ctypes.c_int],
         "ibpm_batch_use_graph": [vp, ctypes.c_int],
+        "ibpm_batch_kernel_stats": [vp, ctypes.c_int, vp, vp, vp, ctypes.c_int],
     }.items():
         getattr(L, name).argtypes = args
         getattr(L, name).restype = ctypes.c_int
+    L.ibpm_kernel_class_name.restype = ctypes.c_char_p
+    L.ibpm_kernel_class_name.argtypes = [ctypes.c_int]
     L.device_count = L.ibpm_device_count
     _lib = L
     return L
@@ -274,6 +278,18 @@ class Batch:
         it = np.zeros(4, dtype=np.int32)
         n = _check(load().ibpm_batch_cg_iters(self.h, k, it.ctypes.data, 4))
         return it[:n].copy()
+
+    def kernel_stats(self, nsteps=1):
+        """Advances nsteps with a HIP event pair around every launch; returns
+        {kernel class: (launches, total ms, algorithmic bytes)}."""
+        n = 32
+        counts = np.zeros(n, dtype=np.int32)
+        ms = np.zeros(n)
+        by = np.zeros(n)
+        k = _check(load().ibpm_batch_kernel_stats(self.h, nsteps, counts.ctypes.data, ms.ctypes.data,
+                                                  by.ctypes.data, n))
+        return {load().ibpm_kernel_class_name(c).decode(): (int(counts[c]), float(ms[c]), float(by[c]))
+                for c in range(k) if counts[c]}
 
     @property
     def time(self):

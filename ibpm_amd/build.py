@@ -19,14 +19,14 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function"]
 DEVICE_FLAGS = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
-def _compile(src):
-    os.makedirs(OBJ, exist_ok=True)
-    obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+def _compile(src, defines=(), obj_dir=OBJ):
+    os.makedirs(obj_dir, exist_ok=True)
+    obj = os.path.join(obj_dir, os.path.splitext(src)[0] + ".o")
     path = os.path.join(CSRC, src)
     if src.endswith(".hip"):
-        cmd = [HIPCC] + COMMON + DEVICE_FLAGS + ["-c", path, "-o", obj]
+        cmd = [HIPCC] + COMMON + DEVICE_FLAGS + list(defines) + ["-c", path, "-o", obj]
     else:
-        cmd = [HIPCC] + COMMON + ["-c", path, "-o", obj]
+        cmd = [HIPCC] + COMMON + list(defines) + ["-c", path, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     msgs = [l for l in (r.stdout + r.stderr).splitlines()
             if "packed-fp32-ops" not in l and "not a recognized feature" not in l and l.strip()]
@@ -35,19 +35,20 @@ def _compile(src):
     return obj, msgs
 
 
-def build(verbose=False):
-    """Compiles every source (always) and links libibpm_gpu.so."""
+def build(verbose=False, defines=(), out=OUT, obj_dir=OBJ):
+    """Compiles every source (always) and links libibpm_gpu.so (or `out`,
+    with extra -D `defines`, for a tuning variant)."""
     with concurrent.futures.ThreadPoolExecutor(max_workers=4) as ex:
-        results = list(ex.map(_compile, DEVICE_SOURCES + HOST_SOURCES))
+        results = list(ex.map(lambda s: _compile(s, defines, obj_dir), DEVICE_SOURCES + HOST_SOURCES))
     objs = [o for o, _ in results]
     for _, msgs in results:
         for m in msgs:
             if verbose or "warning" in m:
                 print(m, file=sys.stderr)
-    tmp = OUT + ".tmp"
+    tmp = out + ".tmp"
     subprocess.check_call([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs)
-    os.replace(tmp, OUT)
-    return OUT
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

@@ -11,7 +11,15 @@
 
 namespace ibpmg {
 
-constexpr int kThreads = 256;          // workgroup size of every kernel (4 wave64)
+constexpr int kThreads = 256;          // workgroup size of the stencil kernels (4 wave64)
+#ifndef IBPM_FFT_THREADS
+#define IBPM_FFT_THREADS 256
+#endif
+#ifndef IBPM_FFT_NP_MAX
+#define IBPM_FFT_NP_MAX 1
+#endif
+constexpr int kFftThreads = IBPM_FFT_THREADS;   // workgroup size of the DST passes
+constexpr int kFftNpMax = IBPM_FFT_NP_MAX;      // most row pairs per DST workgroup
 constexpr int kFftLds = 2048;          // complex doubles staged in LDS per DST block (32 KB)
 constexpr int kMaxStages = 16;
 

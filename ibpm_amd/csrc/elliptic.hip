@@ -75,17 +75,41 @@ template <> __device__ inline void dft<5>(double2* v) {
     v[3] = csub(a2, b2);
 }
 
-// One Stockham radix-R stage over np FFTs of length M stored back to back.
-template <int R>
-__device__ inline void fft_stage(double2* buf, int M, int np, int Ns, const double2* __restrict__ tw) {
-    constexpr int MAXB = (kFftLds / R + kThreads - 1) / kThreads;
+template <> __device__ inline void dft<8>(double2* v) {
+    // radix 8 = two radix-4 DFTs of the even and odd samples + radix-2 merge
+    const double c = 0.70710678118654752440;
+    double2 a[4] = {v[0], v[2], v[4], v[6]};
+    double2 b[4] = {v[1], v[3], v[5], v[7]};
+    dft<4>(a);
+    dft<4>(b);
+    const double2 t0 = b[0];
+    const double2 t1 = make_double2(c * (b[1].x + b[1].y), c * (b[1].y - b[1].x));    // W8^1 b1
+    const double2 t2 = make_double2(b[2].y, -b[2].x);                                // W8^2 b2 = -i b2
+    const double2 t3 = make_double2(c * (b[3].y - b[3].x), -c * (b[3].x + b[3].y));   // W8^3 b3
+    v[0] = cadd(a[0], t0); v[4] = csub(a[0], t0);
+    v[1] = cadd(a[1], t1); v[5] = csub(a[1], t1);
+    v[2] = cadd(a[2], t2); v[6] = csub(a[2], t2);
+    v[3] = cadd(a[3], t3); v[7] = csub(a[3], t3);
+}
+
+// One Stockham radix-R stage over np FFTs of length M stored back to back in
+// LDS; inputs are staged through registers so the stage runs in place.
+// Twiddles come from the LDS table tw[k] = exp(-2 pi i k / M).  Template
+// arguments that are 0 are taken from the runtime values.
+template <int R, int CM, int CNP, int CNS>
+__device__ inline void fft_stage(double2* buf, const double2* tw, int Mr, int npr, int Nsr) {
+    const int M = CM ? CM : Mr;
+    const int np = CM ? CNP : npr;
+    const int Ns = CM ? CNS : Nsr;
+    constexpr int MAXB = CM ? (CNP * CM / R + kFftThreads - 1) / kFftThreads
+                            : (kFftLds / R + kFftThreads - 1) / kFftThreads;
     const int MR = M / R;
     const int total = np * MR;
     const int tstep = M / (Ns * R);
     double2 v[MAXB][R];
 #pragma unroll
     for (int b = 0; b < MAXB; ++b) {
-        const int g = threadIdx.x + b * kThreads;
+        const int g = threadIdx.x + b * kFftThreads;
         if (g < total) {
             const int p = g / MR;
             const int j = g - p * MR;
@@ -93,14 +117,17 @@ __device__ inline void fft_stage(double2* buf, int M, int np, int Ns, const doub
             const double2* x = buf + p * M;
             v[b][0] = x[j];
 #pragma unroll
-            for (int r = 1; r < R; ++r) v[b][r] = cmul(x[j + r * MR], tw[r * jm * tstep]);
+            for (int r = 1; r < R; ++r) {
+                const double2 in = x[j + r * MR];
+                v[b][r] = (CM && CNS == 1) ? in : cmul(in, tw[r * jm * tstep]);
+            }
             dft<R>(v[b]);
         }
     }
     __syncthreads();
 #pragma unroll
     for (int b = 0; b < MAXB; ++b) {
-        const int g = threadIdx.x + b * kThreads;
+        const int g = threadIdx.x + b * kFftThreads;
         if (g < total) {
             const int p = g / MR;
             const int j = g - p * MR;
@@ -114,18 +141,44 @@ __device__ inline void fft_stage(double2* buf, int M, int np, int Ns, const doub
     __syncthreads();
 }
 
-__device__ inline void fft_run(double2* buf, const FftPlan& P) {
+__host__ __device__ constexpr int pick_radix(int rem) {
+    return rem % 8 == 0 ? 8 : rem % 4 == 0 ? 4 : rem % 2 == 0 ? 2 : rem % 5 == 0 ? 5 : 3;
+}
+
+// Compile-time stage list: radix 8 first, then 4, 2, 5, 3.
+template <int M, int NP, int Ns>
+__device__ inline void fft_ct(double2* buf, const double2* tw) {
+    if constexpr (Ns < M) {
+        constexpr int R = pick_radix(M / Ns);
+        fft_stage<R, M, NP, Ns>(buf, tw, 0, 0, 0);
+        fft_ct<M, NP, Ns * R>(buf, tw);
+    }
+}
+
+// Runtime stage list of a host-built plan (any 2(N+1) with factors 2,3,5).
+__device__ inline void fft_rt(double2* buf, const double2* tw, const FftPlan& P) {
     int Ns = 1;
     for (int s = 0; s < P.nst; ++s) {
         const int R = P.rad[s];
         switch (R) {
-        case 2: fft_stage<2>(buf, P.M, P.np, Ns, P.tw); break;
-        case 3: fft_stage<3>(buf, P.M, P.np, Ns, P.tw); break;
-        case 4: fft_stage<4>(buf, P.M, P.np, Ns, P.tw); break;
-        default: fft_stage<5>(buf, P.M, P.np, Ns, P.tw); break;
+        case 8: fft_stage<8, 0, 0, 0>(buf, tw, P.M, P.np, Ns); break;
+        case 4: fft_stage<4, 0, 0, 0>(buf, tw, P.M, P.np, Ns); break;
+        case 2: fft_stage<2, 0, 0, 0>(buf, tw, P.M, P.np, Ns); break;
+        case 3: fft_stage<3, 0, 0, 0>(buf, tw, P.M, P.np, Ns); break;
+        default: fft_stage<5, 0, 0, 0>(buf, tw, P.M, P.np, Ns); break;
         }
         Ns *= R;
     }
+}
+
+__host__ __device__ constexpr int np_of(int M) {
+    return kFftLds / M < 1 ? 1 : (kFftLds / M > kFftNpMax ? kFftNpMax : kFftLds / M);
+}
+
+template <int CM>
+__device__ inline void fft_any(double2* buf, const double2* tw, const FftPlan& P) {
+    if constexpr (CM != 0) fft_ct<CM, np_of(CM), 1>(buf, tw);
+    else fft_rt(buf, tw, P);
 }
 
 // ---------------------------------------------------------------- pass 1
@@ -173,16 +226,28 @@ __device__ inline double rhs_value(const EllJob& J, int i, int j) {
     return v;
 }
 
-__global__ void __launch_bounds__(kThreads) k_ell_pass1(const EllJob* __restrict__ jobs) {
-    const EllJob& J = jobs[blockIdx.y];
+// The three passes are written once over a template length CM (0: runtime
+// plan).  Row length N, FFT length M and rows per block follow from the plan,
+// so with CM != 0 every index computation folds to constants.
+template <int CM>
+__device__ inline void load_twiddles(double2* tw, const FftPlan& P) {
+    const int M = CM ? CM : P.M;
+    for (int k = threadIdx.x; k < M; k += kFftThreads) tw[k] = P.tw[k];
+}
+
+template <int CM>
+__device__ inline void pass1_body(const EllJob& J, double2* smem) {
     const FftPlan& P = J.pj;
-    const int N1 = J.nx - 1, N2 = J.ny - 1;
-    const int rows = 2 * P.np;
+    const int M = CM ? CM : P.M, np = CM ? np_of(CM) : P.np;
+    const int N1 = J.nx - 1, N2 = M / 2 - 1;
+    const int rows = 2 * np;
     const int r0 = blockIdx.x * rows;
     if (r0 >= N1) return;
-    __shared__ double2 buf[kFftLds];
-    const int M = P.M, n = N2 + 1;
-    for (int idx = threadIdx.x; idx < P.np * N2; idx += kThreads) {
+    double2* buf = smem;
+    double2* tw = smem + np * M;
+    load_twiddles<CM>(tw, P);
+    const int n = N2 + 1;
+    for (int idx = threadIdx.x; idx < np * N2; idx += kFftThreads) {
         const int p = idx / N2, k = idx - p * N2;
         const int ra = r0 + 2 * p, rb = ra + 1;
         const double va = (ra < N1) ? rhs_value(J, ra + 1, k + 1) : 0.0;
@@ -190,13 +255,13 @@ __global__ void __launch_bounds__(kThreads) k_ell_pass1(const EllJob* __restrict
         buf[p * M + k + 1] = make_double2(va, vb);
         buf[p * M + M - 1 - k] = make_double2(-va, -vb);
     }
-    for (int p = threadIdx.x; p < P.np; p += kThreads) {
+    for (int p = threadIdx.x; p < np; p += kFftThreads) {
         buf[p * M] = make_double2(0.0, 0.0);
         buf[p * M + n] = make_double2(0.0, 0.0);
     }
     __syncthreads();
-    fft_run(buf, P);
-    for (int idx = threadIdx.x; idx < rows * N2; idx += kThreads) {
+    fft_any<CM>(buf, tw, P);
+    for (int idx = threadIdx.x; idx < rows * N2; idx += kFftThreads) {
         const int r = idx % rows, k = idx / rows;
         const int row = r0 + r;
         if (row >= N1) continue;
@@ -205,17 +270,19 @@ __global__ void __launch_bounds__(kThreads) k_ell_pass1(const EllJob* __restrict
     }
 }
 
-// ---------------------------------------------------------------- pass 2
-__global__ void __launch_bounds__(kThreads) k_ell_pass2(const EllJob* __restrict__ jobs) {
-    const EllJob& J = jobs[blockIdx.y];
+template <int CM>
+__device__ inline void pass2_body(const EllJob& J, double2* smem) {
     const FftPlan& P = J.pi;
-    const int N1 = J.nx - 1, N2 = J.ny - 1;
-    const int rows = 2 * P.np;
+    const int M = CM ? CM : P.M, np = CM ? np_of(CM) : P.np;
+    const int N1 = M / 2 - 1, N2 = J.ny - 1;
+    const int rows = 2 * np;
     const int r0 = blockIdx.x * rows;   // rows are l = 0..N2-1
     if (r0 >= N2) return;
-    __shared__ double2 buf[kFftLds];
-    const int M = P.M, n = N1 + 1;
-    for (int idx = threadIdx.x; idx < P.np * N1; idx += kThreads) {
+    double2* buf = smem;
+    double2* tw = smem + np * M;
+    load_twiddles<CM>(tw, P);
+    const int n = N1 + 1;
+    for (int idx = threadIdx.x; idx < np * N1; idx += kFftThreads) {
         const int p = idx / N1, k = idx - p * N1;
         const int la = r0 + 2 * p, lb = la + 1;
         const double va = (la < N2) ? J.T1[(long)la * N1 + k] : 0.0;
@@ -223,14 +290,14 @@ __global__ void __launch_bounds__(kThreads) k_ell_pass2(const EllJob* __restrict
         buf[p * M + k + 1] = make_double2(va, vb);
         buf[p * M + M - 1 - k] = make_double2(-va, -vb);
     }
-    for (int p = threadIdx.x; p < P.np; p += kThreads) {
+    for (int p = threadIdx.x; p < np; p += kFftThreads) {
         buf[p * M] = make_double2(0.0, 0.0);
         buf[p * M + n] = make_double2(0.0, 0.0);
     }
     __syncthreads();
-    fft_run(buf, P);
+    fft_any<CM>(buf, tw, P);
     // spectral scaling, EllipticSolver2d.cc:42-54 (eigenvalues), :89-93 (scale)
-    for (int idx = threadIdx.x; idx < P.np * N1; idx += kThreads) {
+    for (int idx = threadIdx.x; idx < np * N1; idx += kFftThreads) {
         const int p = idx / N1, k = idx - p * N1;
         const int la = r0 + 2 * p, lb = la + 1;
         const double2 z = buf[p * M + k + 1];
@@ -247,13 +314,13 @@ __global__ void __launch_bounds__(kThreads) k_ell_pass2(const EllJob* __restrict
         buf[p * M + k + 1] = make_double2(ya, yb);
         buf[p * M + M - 1 - k] = make_double2(-ya, -yb);
     }
-    for (int p = threadIdx.x; p < P.np; p += kThreads) {
+    for (int p = threadIdx.x; p < np; p += kFftThreads) {
         buf[p * M] = make_double2(0.0, 0.0);
         buf[p * M + n] = make_double2(0.0, 0.0);
     }
     __syncthreads();
-    fft_run(buf, P);
-    for (int idx = threadIdx.x; idx < rows * N1; idx += kThreads) {
+    fft_any<CM>(buf, tw, P);
+    for (int idx = threadIdx.x; idx < rows * N1; idx += kFftThreads) {
         const int r = idx % rows, k = idx / rows;
         const int l = r0 + r;
         if (l >= N2) continue;
@@ -262,17 +329,19 @@ __global__ void __launch_bounds__(kThreads) k_ell_pass2(const EllJob* __restrict
     }
 }
 
-// ---------------------------------------------------------------- pass 3
-__global__ void __launch_bounds__(kThreads) k_ell_pass3(const EllJob* __restrict__ jobs) {
-    const EllJob& J = jobs[blockIdx.y];
+template <int CM>
+__device__ inline void pass3_body(const EllJob& J, double2* smem) {
     const FftPlan& P = J.pj;
-    const int N1 = J.nx - 1, N2 = J.ny - 1;
-    const int rows = 2 * P.np;
+    const int M = CM ? CM : P.M, np = CM ? np_of(CM) : P.np;
+    const int N1 = J.nx - 1, N2 = M / 2 - 1;
+    const int rows = 2 * np;
     const int r0 = blockIdx.x * rows;
     if (r0 >= N1) return;
-    __shared__ double2 buf[kFftLds];
-    const int M = P.M, n = N2 + 1;
-    for (int idx = threadIdx.x; idx < P.np * N2; idx += kThreads) {
+    double2* buf = smem;
+    double2* tw = smem + np * M;
+    load_twiddles<CM>(tw, P);
+    const int n = N2 + 1;
+    for (int idx = threadIdx.x; idx < np * N2; idx += kFftThreads) {
         const int p = idx / N2, k = idx - p * N2;
         const int ra = r0 + 2 * p, rb = ra + 1;
         const double va = (ra < N1) ? J.T2[(long)ra * N2 + k] : 0.0;
@@ -280,13 +349,13 @@ __global__ void __launch_bounds__(kThreads) k_ell_pass3(const EllJob* __restrict
         buf[p * M + k + 1] = make_double2(va, vb);
         buf[p * M + M - 1 - k] = make_double2(-va, -vb);
     }
-    for (int p = threadIdx.x; p < P.np; p += kThreads) {
+    for (int p = threadIdx.x; p < np; p += kFftThreads) {
         buf[p * M] = make_double2(0.0, 0.0);
         buf[p * M + n] = make_double2(0.0, 0.0);
     }
     __syncthreads();
-    fft_run(buf, P);
-    for (int idx = threadIdx.x; idx < P.np * N2; idx += kThreads) {
+    fft_any<CM>(buf, tw, P);
+    for (int idx = threadIdx.x; idx < np * N2; idx += kFftThreads) {
         const int p = idx / N2, k = idx - p * N2;
         const double2 z = buf[p * M + k + 1];
         const int ra = r0 + 2 * p, rb = ra + 1;
@@ -302,6 +371,37 @@ __global__ void __launch_bounds__(kThreads) k_ell_pass3(const EllJob* __restrict
             J.u[o] = J.epilogue ? J.W[o] - v : v;
         }
     }
+}
+
+// Lengths with a compiled FFT (M = 2(N+1) for N = 199, 255, 383, 399, 511,
+// 1023: the configured grids); any other length uses the runtime plan.
+#define IBPM_DST_DISPATCH(BODY, M_)                                 \
+    switch (M_) {                                                   \
+    case 400: BODY<400>(J, smem); break;                            \
+    case 512: BODY<512>(J, smem); break;                            \
+    case 768: BODY<768>(J, smem); break;                            \
+    case 800: BODY<800>(J, smem); break;                            \
+    case 1024: BODY<1024>(J, smem); break;                          \
+    case 2048: BODY<2048>(J, smem); break;                          \
+    default: BODY<0>(J, smem); break;                               \
+    }
+
+__global__ void __launch_bounds__(kFftThreads) k_ell_pass1(const EllJob* __restrict__ jobs) {
+    extern __shared__ double2 smem[];
+    const EllJob& J = jobs[blockIdx.y];
+    IBPM_DST_DISPATCH(pass1_body, J.pj.M)
+}
+
+__global__ void __launch_bounds__(kFftThreads) k_ell_pass2(const EllJob* __restrict__ jobs) {
+    extern __shared__ double2 smem[];
+    const EllJob& J = jobs[blockIdx.y];
+    IBPM_DST_DISPATCH(pass2_body, J.pi.M)
+}
+
+__global__ void __launch_bounds__(kFftThreads) k_ell_pass3(const EllJob* __restrict__ jobs) {
+    extern __shared__ double2 smem[];
+    const EllJob& J = jobs[blockIdx.y];
+    IBPM_DST_DISPATCH(pass3_body, J.pj.M)
 }
 
 // --------------------------------------------------------------- coarsify
@@ -336,6 +436,11 @@ __global__ void __launch_bounds__(kThreads) k_coarsify(const CoarsifyJob* __rest
 }
 
 // ------------------------------------------------------------ host launch
+int lds_for_pass(const EllJob& J, int pass) {
+    const FftPlan& P = (pass == 2) ? J.pi : J.pj;
+    return (int)sizeof(double2) * (P.np * P.M + P.M);
+}
+
 int blocks_for_pass(const EllJob& J, int pass) {
     const int rows_total = (pass == 2) ? J.ny - 1 : J.nx - 1;
     const FftPlan& P = (pass == 2) ? J.pi : J.pj;
@@ -343,11 +448,11 @@ int blocks_for_pass(const EllJob& J, int pass) {
     return (rows_total + rows - 1) / rows;
 }
 
-void launch_ell_pass(int pass, const EllJob* d_jobs, int njobs, int max_blocks, hipStream_t s) {
+void launch_ell_pass(int pass, const EllJob* d_jobs, int njobs, int max_blocks, int lds_bytes, hipStream_t s) {
     dim3 grid(max_blocks, njobs);
-    if (pass == 1) hipLaunchKernelGGL(k_ell_pass1, grid, dim3(kThreads), 0, s, d_jobs);
-    else if (pass == 2) hipLaunchKernelGGL(k_ell_pass2, grid, dim3(kThreads), 0, s, d_jobs);
-    else hipLaunchKernelGGL(k_ell_pass3, grid, dim3(kThreads), 0, s, d_jobs);
+    if (pass == 1) hipLaunchKernelGGL(k_ell_pass1, grid, dim3(kFftThreads), lds_bytes, s, d_jobs);
+    else if (pass == 2) hipLaunchKernelGGL(k_ell_pass2, grid, dim3(kFftThreads), lds_bytes, s, d_jobs);
+    else hipLaunchKernelGGL(k_ell_pass3, grid, dim3(kFftThreads), lds_bytes, s, d_jobs);
 }
 
 void launch_coarsify(const CoarsifyJob* d_jobs, int njobs, int max_points, hipStream_t s) {

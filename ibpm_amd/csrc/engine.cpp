@@ -15,6 +15,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/ibpm_gpu.h"
@@ -79,12 +80,13 @@ static int get_plan(Ctx* c, int N, FftPlan* out) {
     P.M = 2 * (N + 1);
     if (P.M > kFftLds) { set_error("DST length %d exceeds the LDS plan limit (%d)", N, kFftLds / 2 - 1); return -2; }
     int m = P.M;
+    while (m % 8 == 0) { P.rad[P.nst++] = 8; m /= 8; }
     while (m % 4 == 0) { P.rad[P.nst++] = 4; m /= 4; }
     while (m % 2 == 0) { P.rad[P.nst++] = 2; m /= 2; }
-    while (m % 3 == 0) { P.rad[P.nst++] = 3; m /= 3; }
     while (m % 5 == 0) { P.rad[P.nst++] = 5; m /= 5; }
+    while (m % 3 == 0) { P.rad[P.nst++] = 3; m /= 3; }
     if (m != 1) { set_error("DST length %d: 2(N+1) has a prime factor > 5", N); return -2; }
-    P.np = std::max(1, kFftLds / P.M);
+    P.np = std::max(1, std::min(kFftNpMax, kFftLds / P.M));
     std::vector<double2> tw(P.M);
     const double twopi = 8.0 * std::atan(1.0);
     for (int k = 0; k < P.M; ++k) {
@@ -120,13 +122,49 @@ struct SolveSpec {
     const double* bcring;  // single-level explicit boundary values (L == 1 only)
 };
 
+// Optional per-launch timing: a HIP event pair around every kernel launch on
+// the batch stream, accumulated per kernel class with its algorithmic bytes.
+enum KClass { K_PASS1, K_PASS2, K_PASS3, K_COARSIFY, K_VEL, K_CROSS, K_RHS, K_CURLQ, K_INTERP,
+              K_SMEAR, K_MINV, K_CG, K_FORCES, K_CLEAR, K_NCLASS };
+static const char* kclass_names[K_NCLASS] = {"ell_pass1", "ell_pass2", "ell_pass3", "coarsify", "vel",
+                                             "cross", "rhs", "curl_q", "interp", "smear_curl", "minv",
+                                             "cg", "forces", "clear_nodes"};
+struct Probe {
+    struct Rec { int cls; hipEvent_t a, b; double bytes; };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    hipEvent_t ev() {
+        if (used == pool.size()) {
+            hipEvent_t e;
+            (void)hipEventCreate(&e);
+            pool.push_back(e);
+        }
+        return pool[used++];
+    }
+    ~Probe() { for (auto e : pool) (void)hipEventDestroy(e); }
+};
+
+template <class Fn>
+static void timed(Probe* pr, int cls, double bytes, hipStream_t s, Fn fn) {
+    if (!pr) { fn(); return; }
+    Probe::Rec r{cls, pr->ev(), pr->ev(), bytes};
+    (void)hipEventRecord(r.a, s);
+    fn();
+    (void)hipEventRecord(r.b, s);
+    pr->recs.push_back(r);
+}
+
 struct SolvePlan {
     int nlev = 0;                                   // max L over flows
+    std::vector<double> bytes;                      // per step: 16 B per point (one read + one write)
+    std::vector<double> cbytes;                     // per coarsify level
     std::vector<CoarsifyJob*> cj;                   // lev = 1..nlev-1
     std::vector<int> cj_n, cj_pts;
     std::vector<EllJob*> ej;                        // step s = 0..nlev-1
     std::vector<int> ej_n;
     std::vector<int> maxb[3];
+    std::vector<int> lds[3];
 };
 
 static int build_solve_plan(Ctx* c, DevMem& mem, const std::vector<SolveSpec>& specs, SolvePlan& P) {
@@ -152,7 +190,13 @@ static int build_solve_plan(Ctx* c, DevMem& mem, const std::vector<SolveSpec>& s
     }
     P.ej.assign(P.nlev, nullptr);
     P.ej_n.assign(P.nlev, 0);
+    P.bytes.assign(P.nlev, 0.0);
+    P.cbytes.assign(P.nlev, 0.0);
+    for (int lev = 1; lev < P.nlev; ++lev)
+        for (auto& s : specs)
+            if (lev < s.L) P.cbytes[lev] += 8.0 * 5 * (s.nx / 2 - 1) * (s.ny / 2 - 1);   // 4 fine reads + 1 write
     for (int k = 0; k < 3; ++k) P.maxb[k].assign(P.nlev, 0);
+    for (int k = 0; k < 3; ++k) P.lds[k].assign(P.nlev, 0);
     for (int st = 0; st < P.nlev; ++st) {
         std::vector<EllJob> jobs;
         for (auto& s : specs) {
@@ -177,7 +221,11 @@ static int build_solve_plan(Ctx* c, DevMem& mem, const std::vector<SolveSpec>& s
             J.cosx = s.cosx; J.cosy = s.cosy;
             CKR(get_plan(c, s.ny - 1, &J.pj));
             CKR(get_plan(c, s.nx - 1, &J.pi));
-            for (int k = 0; k < 3; ++k) P.maxb[k][st] = std::max(P.maxb[k][st], blocks_for_pass(J, k + 1));
+            for (int k = 0; k < 3; ++k) {
+                P.maxb[k][st] = std::max(P.maxb[k][st], blocks_for_pass(J, k + 1));
+                P.lds[k][st] = std::max(P.lds[k][st], lds_for_pass(J, k + 1));
+            }
+            P.bytes[st] += 16.0 * ns;
             jobs.push_back(J);
         }
         EllJob* d = nullptr;
@@ -189,10 +237,13 @@ static int build_solve_plan(Ctx* c, DevMem& mem, const std::vector<SolveSpec>& s
     return 0;
 }
 
-static void run_solve(const SolvePlan& P, hipStream_t s) {
-    for (int lev = 1; lev < P.nlev; ++lev) launch_coarsify(P.cj[lev], P.cj_n[lev], P.cj_pts[lev], s);
+static void run_solve(const SolvePlan& P, hipStream_t s, Probe* pr = nullptr) {
+    for (int lev = 1; lev < P.nlev; ++lev)
+        timed(pr, K_COARSIFY, P.cbytes[lev], s, [&] { launch_coarsify(P.cj[lev], P.cj_n[lev], P.cj_pts[lev], s); });
     for (int st = 0; st < P.nlev; ++st)
-        for (int pass = 1; pass <= 3; ++pass) launch_ell_pass(pass, P.ej[st], P.ej_n[st], P.maxb[pass - 1][st], s);
+        for (int pass = 1; pass <= 3; ++pass)
+            timed(pr, K_PASS1 + pass - 1, P.bytes[st], s,
+                  [&] { launch_ell_pass(pass, P.ej[st], P.ej_n[st], P.maxb[pass - 1][st], P.lds[pass - 1][st], s); });
 }
 
 static int cos_tables(DevMem& mem, int n, double** out) {
@@ -259,6 +310,8 @@ struct ibpm_batch {
     std::vector<ibpmg::FlowOps> h_fops[4];
     bool use_graph = false;
     hipGraphExec_t graph = nullptr;
+    ibpmg::Probe* probe = nullptr;
+    double bytes_vel = 0, bytes_cross = 0, bytes_rhs = 0, bytes_curlq = 0;
     int cg_chunk = 4;
 };
 
@@ -487,10 +540,11 @@ static void mark_nprev_valid(ibpm_batch* B) {
 // M f = C A^-1 (beta B f) for the vector behind d_smear (ProjectionSolver.cc:96-101).
 static void enqueue_M(ibpm_batch* B, int i, RegOps* smear, hipStream_t s) {
     const int nfl = (int)B->flows.size();
-    launch_smear_curl(smear, nfl, B->max_nodes, s);
-    run_solve(B->M1[i], s);
-    run_solve(B->M2, s);
-    launch_interp(B->d_interp_m, nfl, B->max_comp, s);
+    Probe* pr = B->probe;
+    timed(pr, K_SMEAR, 0, s, [&] { launch_smear_curl(smear, nfl, B->max_nodes, s); });
+    run_solve(B->M1[i], s, pr);
+    run_solve(B->M2, s, pr);
+    timed(pr, K_INTERP, 0, s, [&] { launch_interp(B->d_interp_m, nfl, B->max_comp, s); });
 }
 
 // Nonlinear term and right-hand side a of substep i (IBSolver.cc:144,169-183).
@@ -498,32 +552,36 @@ static void enqueue_rhs(ibpm_batch* B, int i, hipStream_t s) {
     const int nfl = (int)B->flows.size();
     int lmax = 0;
     for (auto& F : B->flows) lmax = std::max(lmax, F.L);
-    launch_vel(B->d_fops[i], nfl, B->max_ns_all, s);
-    for (int lev = 0; lev < lmax; ++lev) launch_cross(B->d_fops[i], nfl, B->max_nf, lev, s);
-    launch_rhs(B->d_fops[i], nfl, B->max_ns_all, s);
+    Probe* pr = B->probe;
+    timed(pr, K_VEL, B->bytes_vel, s, [&] { launch_vel(B->d_fops[i], nfl, B->max_ns_all, s); });
+    for (int lev = 0; lev < lmax; ++lev)
+        timed(pr, K_CROSS, B->bytes_cross / lmax, s, [&] { launch_cross(B->d_fops[i], nfl, B->max_nf, lev, s); });
+    timed(pr, K_RHS, B->bytes_rhs, s, [&] { launch_rhs(B->d_fops[i], nfl, B->max_ns_all, s); });
 }
 
 static void enqueue_refresh(ibpm_batch* B, hipStream_t s) {
-    run_solve(B->S4, s);
-    launch_curl_q(B->d_fops[0], (int)B->flows.size(), B->max_nf_all, s);
+    run_solve(B->S4, s, B->probe);
+    timed(B->probe, K_CURLQ, B->bytes_curlq, s,
+          [&] { launch_curl_q(B->d_fops[0], (int)B->flows.size(), B->max_nf_all, s); });
 }
 
 // The stationary-body (Cholesky) substep: fixed launch sequence.
 static void enqueue_substep_chol(ibpm_batch* B, int i, hipStream_t s) {
     const int nfl = (int)B->flows.size();
+    Probe* pr = B->probe;
     enqueue_rhs(B, i, s);
-    run_solve(B->S1[i], s);                         // omega* = A^-1 a
-    run_solve(B->S2[i], s);                         // psi = L^-1(-omega*)
-    launch_interp(B->d_interp_c, nfl, B->max_comp, s);   // rhs = C omega* - b
-    launch_minv(B->d_minv[i], nfl, B->max_n, s);    // f = M^-1 rhs
-    launch_smear_curl(B->d_smear_f[i], nfl, B->max_nodes, s);   // c = beta B f
-    run_solve(B->S3[i], s);                         // omega = omega* - A^-1 c
+    run_solve(B->S1[i], s, pr);                     // omega* = A^-1 a
+    run_solve(B->S2[i], s, pr);                     // psi = L^-1(-omega*)
+    timed(pr, K_INTERP, 0, s, [&] { launch_interp(B->d_interp_c, nfl, B->max_comp, s); });   // rhs = C omega* - b
+    timed(pr, K_MINV, 0, s, [&] { launch_minv(B->d_minv[i], nfl, B->max_n, s); });          // f = M^-1 rhs
+    timed(pr, K_SMEAR, 0, s, [&] { launch_smear_curl(B->d_smear_f[i], nfl, B->max_nodes, s); });   // c = beta B f
+    run_solve(B->S3[i], s, pr);                     // omega = omega* - A^-1 c
     enqueue_refresh(B, s);                          // q = Curl psi + q0
 }
 
 static int enqueue_step_chol(ibpm_batch* B, hipStream_t s) {
     for (int i = 0; i < B->nsub; ++i) enqueue_substep_chol(B, i, s);
-    launch_forces(B->d_force, (int)B->flows.size(), s);
+    timed(B->probe, K_FORCES, 0, s, [&] { launch_forces(B->d_force, (int)B->flows.size(), s); });
     return 0;
 }
 
@@ -541,11 +599,12 @@ static int substep_cg(ibpm_batch* B, int i) {
         }
     }
     CKR(upload_regops(B));
-    launch_clear_nodes(B->d_clear, nfl, B->max_nodes, s);
+    Probe* pr = B->probe;
+    timed(pr, K_CLEAR, 0, s, [&] { launch_clear_nodes(B->d_clear, nfl, B->max_nodes, s); });
     enqueue_rhs(B, i, s);
-    run_solve(B->S1[i], s);
-    run_solve(B->S2[i], s);
-    launch_interp(B->d_interp_c, nfl, B->max_comp, s);
+    run_solve(B->S1[i], s, pr);
+    run_solve(B->S2[i], s, pr);
+    timed(pr, K_INTERP, 0, s, [&] { launch_interp(B->d_interp_c, nfl, B->max_comp, s); });
     // r0 = rhs - M x0 with x0 = previous f (ProjectionSolver.cc:72 warm start)
     for (auto& F : B->flows) CK(hipMemcpyAsync(F.cg_d, F.f, sizeof(double) * 2 * F.nb, hipMemcpyDeviceToDevice, s));
     enqueue_M(B, i, B->d_smear_d[i], s);
@@ -554,7 +613,7 @@ static int substep_cg(ibpm_batch* B, int i) {
         CK(hipMemcpyAsync(F.cg_state, st, sizeof(st), hipMemcpyHostToDevice, s));
         CK(hipStreamSynchronize(s));
     }
-    launch_cg(B->d_cg, nfl, 0, s);
+    timed(pr, K_CG, 0, s, [&] { launch_cg(B->d_cg, nfl, 0, s); });
     const int max_iter = 100000;
     for (int it = 0; it < max_iter; it += B->cg_chunk) {
         CK(hipStreamSynchronize(s));
@@ -568,13 +627,48 @@ static int substep_cg(ibpm_batch* B, int i) {
         if (all) break;
         for (int k = 0; k < B->cg_chunk; ++k) {
             enqueue_M(B, i, B->d_smear_d[i], s);
-            launch_cg(B->d_cg, nfl, 1, s);
+            timed(pr, K_CG, 0, s, [&] { launch_cg(B->d_cg, nfl, 1, s); });
         }
     }
-    launch_smear_curl(B->d_smear_f[i], nfl, B->max_nodes, s);
-    run_solve(B->S3[i], s);
+    timed(pr, K_SMEAR, 0, s, [&] { launch_smear_curl(B->d_smear_f[i], nfl, B->max_nodes, s); });
+    run_solve(B->S3[i], s, pr);
     enqueue_refresh(B, s);
     return 0;
+}
+
+// In place: Mt (M column-major, Mt[c*n + r] = M(r, c)) -> M^-1 row-major.
+// Factor L L^T from M's upper triangle exactly as CholeskySolver.cc:89-109,
+// then apply CholeskySolver::Minv (:200-224) to every unit vector.  Returns
+// -1, or the row where the matrix is not positive definite.
+static int cholesky_inverse(int n, std::vector<double>& Mt) {
+    std::vector<double> L((size_t)n * n, 0.0), d(n), x(n);
+    auto M = [&](int r, int c) { return Mt[(size_t)c * n + r]; };
+    for (int r = 0; r < n; ++r) {
+        for (int c = r; c < n; ++c) {
+            double sum = M(r, c);
+            for (int k = r - 1; k >= 0; --k) sum -= L[(size_t)r * n + k] * L[(size_t)c * n + k];
+            if (c == r) {
+                if (!(sum > 0)) return r;
+                d[r] = std::sqrt(sum);
+            } else {
+                L[(size_t)c * n + r] = sum / d[r];
+            }
+        }
+    }
+    for (int col = 0; col < n; ++col) {
+        for (int r = 0; r < n; ++r) {
+            double sum = (r == col) ? 1.0 : 0.0;
+            for (int k = r - 1; k >= 0; --k) sum -= L[(size_t)r * n + k] * x[k];
+            x[r] = sum / d[r];
+        }
+        for (int r = n - 1; r >= 0; --r) {
+            double sum = x[r];
+            for (int k = r + 1; k < n; ++k) sum -= L[(size_t)k * n + r] * x[k];
+            x[r] = sum / d[r];
+        }
+        for (int r = 0; r < n; ++r) Mt[(size_t)r * n + col] = x[r];
+    }
+    return -1;
 }
 
 // CholeskySolver::init (CholeskySolver.cc:49-109): M column by column on the
@@ -608,40 +702,30 @@ static int build_cholesky(ibpm_batch* B) {
             enqueue_M(B, i, B->d_smear_d[i], s);
         }
         CK(hipStreamSynchronize(s));
-        for (auto& F : B->flows) {
+        // factorisation and inverse assembly on the host, one thread per flow
+        std::vector<std::vector<double>> minv_h(nfl);
+        std::vector<int> bad(nfl, -1);
+        for (int k = 0; k < nfl; ++k) {
+            Flow& F = B->flows[k];
             const int n = 2 * F.nb;
-            std::vector<double> Mt((size_t)n * n), L((size_t)n * n, 0.0), d(n), Minv((size_t)n * n);
-            CK(hipMemcpy(Mt.data(), F.mcol, sizeof(double) * n * n, hipMemcpyDeviceToHost));
+            minv_h[k].assign((size_t)n * n, 0.0);
+            CK(hipMemcpy(minv_h[k].data(), F.mcol, sizeof(double) * n * n, hipMemcpyDeviceToHost));
             CK(hipMemset(F.unit, 0, sizeof(double) * (n + 1)));
-            auto M = [&](int r, int c) { return Mt[(size_t)c * n + r]; };   // column-major
-            for (int r = 0; r < n; ++r) {
-                for (int c = r; c < n; ++c) {
-                    double sum = M(r, c);
-                    for (int k = r - 1; k >= 0; --k) sum -= L[(size_t)r * n + k] * L[(size_t)c * n + k];
-                    if (c == r) {
-                        if (!(sum > 0)) { set_error("Cholesky: matrix not positive definite at %d", r); return -4; }
-                        d[r] = std::sqrt(sum);
-                    } else {
-                        L[(size_t)c * n + r] = sum / d[r];
-                    }
-                }
-            }
-            std::vector<double> x(n);
-            for (int col = 0; col < n; ++col) {
-                for (int r = 0; r < n; ++r) {
-                    double sum = (r == col) ? 1.0 : 0.0;
-                    for (int k = r - 1; k >= 0; --k) sum -= L[(size_t)r * n + k] * x[k];
-                    x[r] = sum / d[r];
-                }
-                for (int r = n - 1; r >= 0; --r) {
-                    double sum = x[r];
-                    for (int k = r + 1; k < n; ++k) sum -= L[(size_t)k * n + r] * x[k];
-                    x[r] = sum / d[r];
-                }
-                for (int r = 0; r < n; ++r) Minv[(size_t)r * n + col] = x[r];
-            }
+        }
+        std::vector<std::thread> th;
+        const int nthreads = std::max(1, std::min(nfl, (int)std::thread::hardware_concurrency()));
+        for (int t = 0; t < nthreads; ++t)
+            th.emplace_back([&, t] {
+                for (int k = t; k < nfl; k += nthreads)
+                    bad[k] = cholesky_inverse(2 * B->flows[k].nb, minv_h[k]);
+            });
+        for (auto& x : th) x.join();
+        for (int k = 0; k < nfl; ++k) {
+            if (bad[k] >= 0) { set_error("Cholesky: flow %d, matrix not positive definite at %d", k, bad[k]); return -4; }
+            Flow& F = B->flows[k];
+            const int n = 2 * F.nb;
             if (!F.minv[i]) CKR(B->mem.alloc(&F.minv[i], (size_t)n * n));
-            CKR(upload(F.minv[i], Minv.data(), sizeof(double) * Minv.size()));
+            CKR(upload(F.minv[i], minv_h[k].data(), sizeof(double) * n * n));
         }
         std::vector<MinvOps> mo(nfl);
         for (int k = 0; k < nfl; ++k) {
@@ -749,8 +833,8 @@ int ibpm_dst2d(ibpm_ctx* c, int n1, int n2, const double* d_in, double* d_out) {
     CKR(mem.alloc(&d, 2));
     CK(hipMemcpyAsync(d, &J, sizeof(EllJob), hipMemcpyHostToDevice, c->c.stream));
     CK(hipMemcpyAsync(d + 1, &J2, sizeof(EllJob), hipMemcpyHostToDevice, c->c.stream));
-    launch_ell_pass(1, d, 1, blocks_for_pass(J, 1), c->c.stream);
-    launch_ell_pass(1, d + 1, 1, blocks_for_pass(J2, 1), c->c.stream);
+    launch_ell_pass(1, d, 1, blocks_for_pass(J, 1), lds_for_pass(J, 1), c->c.stream);
+    launch_ell_pass(1, d + 1, 1, blocks_for_pass(J2, 1), lds_for_pass(J2, 1), c->c.stream);
     CK(hipStreamSynchronize(c->c.stream));
     CK(hipGetLastError());
     return 0;
@@ -924,6 +1008,13 @@ int ibpm_batch_create(ibpm_ctx* ctx, int nflows, const ibpm_flow_params* params,
         B->max_nf_all = std::max(B->max_nf_all, F.nf * F.L);
         B->max_comp = std::max(B->max_comp, 2 * F.nb);
         B->max_n = std::max(B->max_n, 2 * F.nb);
+        // algorithmic bytes: vel reads omega + q, writes U, V; cross reads U, V
+        // and writes the cross flux; rhs reads cross, omega, N_prev and writes
+        // a, N; curl_q reads psi and writes q
+        B->bytes_vel += 8.0 * (3 * F.ns + F.nf) * F.L;
+        B->bytes_cross += 8.0 * (2 * F.ns + F.nf) * F.L;
+        B->bytes_rhs += 8.0 * (4 * F.ns + F.nf) * F.L;
+        B->bytes_curlq += 8.0 * (F.ns + F.nf) * F.L;
     }
     if (B->any_cg) {
         for (auto& F : B->flows) F.cg = true;   // one force solver per batch
@@ -969,8 +1060,8 @@ int ibpm_batch_advance(ibpm_batch* B, int nsteps) {
                 CKR(substep_cg(B, i));
                 if (i == 0) mark_nprev_valid(B);
             }
-            launch_forces(B->d_force, (int)B->flows.size(), s);
-        } else if (B->use_graph && B->old_saved) {
+            timed(B->probe, K_FORCES, 0, s, [&] { launch_forces(B->d_force, (int)B->flows.size(), s); });
+        } else if (B->use_graph && B->old_saved && !B->probe) {
             if (!B->graph) {
                 hipGraph_t g;
                 CK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -988,7 +1079,7 @@ int ibpm_batch_advance(ibpm_batch* B, int nsteps) {
                     mark_nprev_valid(B);
                 }
             }
-            launch_forces(B->d_force, (int)B->flows.size(), s);
+            timed(B->probe, K_FORCES, 0, s, [&] { launch_forces(B->d_force, (int)B->flows.size(), s); });
         }
         B->time += B->flows[0].p.dt;
         B->timestep += 1;
@@ -1097,5 +1188,28 @@ int ibpm_batch_cg_iters(ibpm_batch* B, int k, int* h, int max_sub) {
 }
 
 double ibpm_batch_time(ibpm_batch* B) { return B ? B->time : 0.0; }
+
+const char* ibpm_kernel_class_name(int cls) {
+    return (cls >= 0 && cls < K_NCLASS) ? kclass_names[cls] : "";
+}
+
+int ibpm_batch_kernel_stats(ibpm_batch* B, int nsteps, int* counts, double* ms, double* bytes, int nclass) {
+    if (!B->initialized) { set_error("ibpm_batch_init not called"); return -1; }
+    Probe pr;
+    B->probe = &pr;
+    const int rc = ibpm_batch_advance(B, nsteps);
+    B->probe = nullptr;
+    if (rc != 0) return rc;
+    for (int k = 0; k < nclass; ++k) { counts[k] = 0; ms[k] = 0; bytes[k] = 0; }
+    for (auto& r : pr.recs) {
+        if (r.cls >= nclass) continue;
+        float t = 0;
+        CK(hipEventElapsedTime(&t, r.a, r.b));
+        counts[r.cls] += 1;
+        ms[r.cls] += t;
+        bytes[r.cls] += r.bytes;
+    }
+    return K_NCLASS;
+}
 
 }  // extern "C"

@@ -405,20 +405,28 @@ __global__ void __launch_bounds__(kThreads) k_cg(const CgOps* __restrict__ cgs, 
     }
 }
 
-// Net force sum over the boundary points in point order (State.cc:53-63;
-// the dx^2 factor and the rotation to Cd/Cl are applied on the host).
-__global__ void k_forces(const ForceOps* __restrict__ fo) {
+// Net force sum over the boundary points (State.cc:53-63; the dx^2 factor and
+// the rotation to Cd/Cl are applied on the host).  One wave per flow: lane l
+// sums points l, l+64, ... in order, then a fixed shuffle tree combines the
+// lanes, so the result is deterministic.
+__global__ void __launch_bounds__(64) k_forces(const ForceOps* __restrict__ fo) {
     const ForceOps& F = fo[blockIdx.x];
-    if (threadIdx.x != 0) return;
     double xf = 0, yf = 0;
-    for (int k = 0; k < F.nb; ++k) {
+    for (int k = threadIdx.x; k < F.nb; k += 64) {
         xf += F.f[k];
         yf += F.f[F.nb + k];
     }
-    const int st = *F.count;
-    F.hist[2 * st] = xf;
-    F.hist[2 * st + 1] = yf;
-    *F.count = st + 1;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        xf += __shfl_down(xf, off, 64);
+        yf += __shfl_down(yf, off, 64);
+    }
+    if (threadIdx.x == 0) {
+        const int st = *F.count;
+        F.hist[2 * st] = xf;
+        F.hist[2 * st + 1] = yf;
+        *F.count = st + 1;
+    }
 }
 
 // ------------------------------------------------------------ host side

@@ -1,0 +1,110 @@
+"""The five configured workloads of the projection-step benchmark, as lists of
+ibpm_amd.Flow.  Random cases are drawn from fixed seeds so every run (and the
+CPU oracle) sees the same flows.
+
+  cfg1  stationary cylinder D=1 (circle_n 157), 200^2, 1 level, [-2,2]^2,
+        Re 100, dt 0.02, 250 steps, Cholesky force solve
+  cfg2  cylinder D=1 (circle_n 314), 400^2 finest, 4 levels, [-2,2]^2
+        finest, Re 200, dt 0.01, 2000 steps, Cholesky
+  cfg3  64 stationary flows: cylinder / ellipse (aspect U(2,5)) / plate of
+        chord 1, angle of attack U(0,30) deg, Re U(50,300), n in {256,384,512},
+        3 levels, dt 0.005
+  cfg4  pitching/plunging plate (100 points, mid-chord centre), plunge U(.1,.3),
+        pitch U(5,20) deg, frequency U(.2,.5), Re 300, 512^2, 5 levels,
+        dt 0.0025, CG tolerance 1e-10
+  cfg5  16 cases of 3 tandem plates (spacing U(1,2)) with cfg4 kinematics,
+        Re 500, 1024^2, 6 levels, [-4,4]^2, dt 0.001, CG tolerance 1e-10
+"""
+import math
+
+import numpy as np
+
+from . import Flow
+
+
+def cylinder(n, r=0.5, x=0.0, y=0.0):
+    return "body Cylinder\n  circle_n %.17g %.17g %.17g %d\nend\n" % (x, y, r, n)
+
+
+def points_body(name, xs, ys):
+    return "body %s\n" % name + "".join("  point %.17g %.17g\n" % (a, b) for a, b in zip(xs, ys)) + "end\n"
+
+
+def ellipse(aspect, aoa, dx, chord=1.0):
+    """Ellipse of major axis `chord` rotated nose-up by aoa (radians), points
+    spaced uniformly in arc length at about dx (the .geom grammar has no
+    ellipse)."""
+    a, b = chord / 2, chord / 2 / aspect
+    th = np.linspace(0, 2 * math.pi, 20001)
+    seg = np.hypot(np.diff(a * np.cos(th)), np.diff(b * np.sin(th)))
+    s = np.concatenate([[0.0], np.cumsum(seg)])
+    n = max(8, int(round(s[-1] / dx)))
+    t = np.interp(np.arange(n) * s[-1] / n, s, th)
+    x, y = a * np.cos(t), b * np.sin(t)
+    c, si = math.cos(-aoa), math.sin(-aoa)
+    return points_body("Ellipse", c * x - si * y, si * x + c * y)
+
+
+def plate(aoa, dx, chord=1.0):
+    n = int(round(chord / dx)) + 1
+    return ("body Plate\n  line_n %.17g 0 %.17g 0 %d\n  center 0 0\n  motion fixed 0 0 %.17g\nend\n"
+            % (-chord / 2, chord / 2, n, -aoa))
+
+
+def pitch_plunge_plate(rng, xc=0.0, npts=100, chord=1.0):
+    ap = math.radians(rng.uniform(5, 20))
+    ah = rng.uniform(0.1, 0.3)
+    f = rng.uniform(0.2, 0.5)
+    return ("body Plate\n  line_n %.17g 0 %.17g 0 %d\n  center %.17g 0\n"
+            "  motion pitchplunge %.17g %.17g 0 %.17g %.17g 0\nend\n"
+            % (xc - chord / 2, xc + chord / 2, npts, xc, ap, f, ah, f))
+
+
+def cfg1():
+    return [Flow(200, 200, 1, 4.0, -2.0, -2.0, 100.0, 0.02, cylinder(157))]
+
+
+def cfg2():
+    return [Flow(400, 400, 4, 4.0, -2.0, -2.0, 200.0, 0.01, cylinder(314))]
+
+
+def cfg3(nflows=64, seed=3):
+    rng = np.random.default_rng(seed)
+    flows = []
+    for _ in range(nflows):
+        n = int(rng.choice([256, 384, 512]))
+        dx = 4.0 / n
+        kind = int(rng.integers(0, 3))
+        aoa = math.radians(rng.uniform(0, 30))
+        Re = float(rng.uniform(50, 300))
+        if kind == 0:
+            geom = cylinder(int(round(math.pi / dx)))
+        elif kind == 1:
+            geom = ellipse(float(rng.uniform(2, 5)), aoa, dx)
+        else:
+            geom = plate(aoa, dx)
+        flows.append(Flow(n, n, 3, 4.0, -2.0, -2.0, Re, 0.005, geom))
+    return flows
+
+
+def cfg4(seed=4):
+    rng = np.random.default_rng(seed)
+    return [Flow(512, 512, 5, 4.0, -2.0, -2.0, 300.0, 0.0025, pitch_plunge_plate(rng), tol=1e-10)]
+
+
+def cfg5(ncases=16, seed=5):
+    rng = np.random.default_rng(seed)
+    flows = []
+    for _ in range(ncases):
+        s = rng.uniform(1, 2)
+        geom = "".join(pitch_plunge_plate(rng, xc) for xc in (-s, 0.0, s))
+        flows.append(Flow(1024, 1024, 6, 8.0, -4.0, -4.0, 500.0, 0.001, geom, tol=1e-10))
+    return flows
+
+
+WORKLOADS = {"cfg1": cfg1, "cfg2": cfg2, "cfg3": cfg3, "cfg4": cfg4, "cfg5": cfg5}
+
+
+def grid_points(flow):
+    """Stored interior nodes summed over levels (Scalar.cc:88)."""
+    return flow.ngrid * (flow.nx - 1) * (flow.ny - 1)

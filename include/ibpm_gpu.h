@@ -104,6 +104,13 @@ double ibpm_batch_time(ibpm_batch* b);
 /* Capture one timestep as a hipGraph and replay it (stationary bodies). */
 int ibpm_batch_use_graph(ibpm_batch* b, int enable);
 
+/* Measurement: advances nsteps with a HIP event pair around every kernel
+   launch on the batch stream and returns per kernel class the launch count,
+   total milliseconds and algorithmic HBM bytes.  Returns the number of
+   classes; ibpm_kernel_class_name names them. */
+int ibpm_batch_kernel_stats(ibpm_batch* b, int nsteps, int* counts, double* ms, double* bytes, int nclass);
+const char* ibpm_kernel_class_name(int cls);
+
 #ifdef __cplusplus
 }
 #endif

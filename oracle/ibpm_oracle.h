@@ -88,6 +88,8 @@ int  og_sim_num_points(const og_sim* s);
 int  og_sim_uses_cg(const og_sim* s);
 /* ibpm.cc:283-305: move bodies to t, regulariser update, build solvers, refresh */
 int  og_sim_init(og_sim* s);
+/* og_sim_init with identity Cholesky factors (CPU-baseline timing only). */
+int  og_sim_init_timing(og_sim* s);
 /* One full timestep (IBSolver.cc:142-150); returns 0 on success. */
 int  og_sim_advance(og_sim* s);
 /* Cd, Cl as printed by ibpm.cc:346-355 (2 x rotated net force). */

@@ -40,8 +40,8 @@ def lib():
                                     ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                     ctypes.c_int, ctypes.c_double, ctypes.c_char_p, ctypes.c_double,
                                     ctypes.c_int]
-        for name in ["og_sim_destroy", "og_sim_init", "og_sim_advance", "og_sim_num_points",
-                     "og_sim_uses_cg"]:
+        for name in ["og_sim_destroy", "og_sim_init", "og_sim_init_timing", "og_sim_advance",
+                     "og_sim_num_points", "og_sim_uses_cg"]:
             getattr(L, name).argtypes = [ctypes.c_void_p]
         L.og_sim_error.restype = ctypes.c_char_p
         L.og_sim_time.restype = ctypes.c_double
@@ -244,6 +244,10 @@ class Sim:
 
     def init(self):
         lib().og_sim_init(self.h)
+
+    def init_timing(self):
+        """Set-up for timing only: identity Cholesky factors (no build)."""
+        lib().og_sim_init_timing(self.h)
 
     def advance(self, n=1):
         for _ in range(n):

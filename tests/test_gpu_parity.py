@@ -146,7 +146,7 @@ def test_batch_of_uneven_flows_equals_single_flows(gpu):
 
 
 def test_moving_plate_cg_matches_oracle(gpu, oracle):
-    geom = plate_geom(40, 1.0, 0.2, 0.2, 0.4)
+    geom = plate_geom(16, 1.0, 0.2, 0.2, 0.4)   # spacing 1/15 ~ dx = 1/16
     ctx, b, s = _pair(gpu, oracle, 64, 3, 300.0, 0.01, geom, tol=1e-10)
     for step in range(4):
         b.advance(1)

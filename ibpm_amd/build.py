@@ -48,7 +48,21 @@ def build(verbose=False, defines=(), out=OUT, obj_dir=OBJ):
     tmp = out + ".tmp"
     subprocess.check_call([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs)
     os.replace(tmp, out)
+    if out == OUT:
+        build_driver()
     return out
+
+
+DRIVER = os.path.join(HERE, "ibpm")
+
+
+def build_driver():
+    """The `ibpm` command-line driver (C++ host code over the C ABI)."""
+    src = os.path.join(HERE, "host", "ibpm_main.cpp")
+    subprocess.check_call(["g++", "-O2", "-std=c++17", "-Wall", "-o", DRIVER + ".tmp", src,
+                           "-L" + HERE, "-libpm_gpu", "-Wl,-rpath,$ORIGIN"])
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":

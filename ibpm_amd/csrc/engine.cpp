@@ -746,6 +746,13 @@ extern "C" {
 
 const char* ibpm_last_error(void) { return g_err; }
 
+int ibpm_geom_num_points(const char* geom_text) {
+    Geometry g;
+    std::string err;
+    if (!g.parse(geom_text ? geom_text : "", err)) { set_error("%s", err.c_str()); return -1; }
+    return g.npoints;
+}
+
 int ibpm_device_count(void) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) return 0;

@@ -1,0 +1,131 @@
+// ibpm_main.cpp -- the `ibpm` command-line driver on the GPU path.
+//
+// Accepts the reference's flags and defaults (reference src/ibpm.cc:41-88)
+// for the nonlinear model and writes the force file in the reference format
+// "%5d %.5e %.5e %.5e" = step, time, Cd, Cl (src/OutputForce.cc:46-64), plus
+// <name>.force17 with full precision.  Added flag: -tol (CG tolerance, which
+// the reference sets only from code, src/ibpm.cc:292-294).  Tecplot, restart
+// and energy outputs are not produced by this driver.
+#include <sys/stat.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <iostream>
+#include <map>
+#include <set>
+#include <string>
+
+#include "ibpm_gpu.hpp"
+
+using namespace ibpm;
+
+namespace {
+struct Args {   // ParmParser-style "-name value" pairs (reference src/ParmParser.cc:22-200)
+    std::map<std::string, std::string> kv;
+    std::set<std::string> used;
+    bool help = false;
+    Args(int argc, char** argv) {
+        for (int i = 1; i < argc; ++i) {
+            std::string a = argv[i];
+            if (a.size() < 2 || a[0] != '-') { kv["?" + a] = ""; continue; }
+            a = a.substr(1);
+            if (a == "h") { help = true; continue; }
+            if (i + 1 < argc) kv[a] = argv[++i];
+            else kv[a] = "";
+        }
+    }
+    std::string str(const std::string& k, const std::string& d) {
+        used.insert(k);
+        auto it = kv.find(k);
+        return it == kv.end() ? d : it->second;
+    }
+    int i(const std::string& k, int d) { return std::atoi(str(k, std::to_string(d)).c_str()); }
+    double f(const std::string& k, double d) {
+        used.insert(k);
+        auto it = kv.find(k);
+        return it == kv.end() ? d : std::atof(it->second.c_str());
+    }
+    bool valid() const {
+        for (auto& p : kv)
+            if (!used.count(p.first)) {
+                std::cerr << "unrecognised argument: -" << p.first << "\n";
+                return false;
+            }
+        return true;
+    }
+};
+}  // namespace
+
+int main(int argc, char** argv) {
+    std::cout << "Immersed Boundary Projection Method (IBPM), MI355X projection-step path\n" << std::endl;
+    Args P(argc, argv);
+    const std::string name = P.str("name", "ibpm");
+    std::string outdir = P.str("outdir", ".");
+    P.i("tecplot", 100);
+    P.i("restart", 100);
+    const int iForce = P.i("force", 1);
+    const int nx = P.i("nx", 200), ny = P.i("ny", 200), ngrid = P.i("ngrid", 1);
+    const double length = P.f("length", 4.0), xOffset = P.f("xoffset", -2.), yOffset = P.f("yoffset", -2.);
+    const double alpha_deg = P.f("alpha", 0.);
+    const std::string geomFile = P.str("geom", name + ".geom");
+    const double Re = P.f("Re", 100.);
+    const std::string model = P.str("model", "nonlinear");
+    const double dt = P.f("dt", 0.02);
+    const int numSteps = P.i("nsteps", 250);
+    const std::string scheme = P.str("scheme", "rk3");
+    const double tol = P.f("tol", 1e-7);
+    const int device = P.i("device", 0);
+    std::map<std::string, Scheme::SchemeType> schemes = {
+        {"euler", Scheme::EULER}, {"ab2", Scheme::AB2}, {"rk3", Scheme::RK3}, {"rk3b", Scheme::RK3b}};
+    if (!P.valid() || P.help || !schemes.count(scheme) || model != "nonlinear") {
+        std::cerr << "usage: ibpm [-name ibpm] [-outdir .] [-force 1] [-nx 200] [-ny 200] [-ngrid 1]\n"
+                     "            [-length 4] [-xoffset -2] [-yoffset -2] [-alpha 0] [-geom name.geom]\n"
+                     "            [-Re 100] [-dt 0.02] [-nsteps 250] [-scheme euler|ab2|rk3|rk3b]\n"
+                     "            [-model nonlinear] [-tol 1e-7] [-device 0]\n";
+        return 1;
+    }
+    if (outdir.empty() || outdir.back() != '/') outdir += "/";
+    mkdir(outdir.c_str(), 0777);
+
+    Grid grid(nx, ny, ngrid, length, xOffset, yOffset);
+    Geometry geom;
+    std::cout << "Reading geometry from file " << geomFile << std::endl;
+    if (!geom.load(geomFile)) {
+        std::cerr << "could not load geometry " << geomFile << ": " << ibpm_last_error() << std::endl;
+        return -1;
+    }
+    std::cout << "    " << geom.getNumPoints() << " points on the boundary\n" << std::endl;
+    const double pi = 4. * std::atan(1.);
+    BaseFlow q_potential(grid, 1.0, alpha_deg * pi / 180.);
+    NavierStokesModel nsm(grid, geom, Re, q_potential);
+    NonlinearIBSolver solver(grid, nsm, dt, schemes[scheme], tol, device);
+    State x(grid, geom.getNumPoints());
+    nsm.init();
+    try {
+        solver.init();
+    } catch (const std::exception& e) {
+        std::cerr << e.what() << std::endl;
+        return -1;
+    }
+    std::cout << "using " << solver.getName() << " timestepper\n    dt = " << dt << "\n" << std::endl;
+    FILE* ff = iForce > 0 ? std::fopen((outdir + name + ".force").c_str(), "w") : nullptr;
+    FILE* ff17 = iForce > 0 ? std::fopen((outdir + name + ".force17").c_str(), "w") : nullptr;
+    const double al = q_potential.getAlpha();
+    std::cout << "Integrating for " << numSteps << " steps" << std::endl;
+    for (int i = 1; i <= numSteps; ++i) {
+        solver.advance(x);
+        double xF, yF;
+        x.computeNetForce(xF, yF);
+        const double drag = 2 * (xF * std::cos(al) + yF * std::sin(al));
+        const double lift = 2 * (xF * -1. * std::sin(al) + yF * std::cos(al));
+        std::printf("step %d    x force: %16.9g, y force: %16.9g\n", i, drag, lift);
+        if (ff && x.timestep % iForce == 0) {
+            std::fprintf(ff, "%5d %.5e %.5e %.5e\n", x.timestep, x.time, drag, lift);
+            std::fprintf(ff17, "%5d %.17e %.17e %.17e\n", x.timestep, x.time, drag, lift);
+        }
+    }
+    if (ff) std::fclose(ff);
+    if (ff17) std::fclose(ff17);
+    return 0;
+}

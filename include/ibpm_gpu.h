@@ -66,6 +66,11 @@ int ibpm_nonlinear(ibpm_ctx* ctx, int nx, int ny, int ngrid, double dx, const do
 /* Laplacian = -Curl(Curl(omega)), src/VectorOperations.cc:133-140 */
 int ibpm_laplacian(ibpm_ctx* ctx, int nx, int ny, int ngrid, double dx, const double* d_w, double* d_out);
 
+/* ---- geometry ------------------------------------------------------------- */
+/* Number of boundary points of a .geom text (Geometry::getNumPoints after
+   Geometry::load, src/Geometry.cc:143-191), or -1 if it does not parse. */
+int ibpm_geom_num_points(const char* geom_text);
+
 /* ---- whole projection-step solver (IBSolver / ProjectionSolver / ...) ----- */
 typedef struct {
     int nx, ny, ngrid;          /* Grid(nx, ny, ngrid, length, xoffset, yoffset), src/Grid.h:34-41 */

@@ -1,0 +1,99 @@
+"""Full-length parity gates on the GPU against fixtures written by the FP64
+oracle (tests/golden/make_golden.py):
+
+  * Cd relative error <= 1e-6 at every step,
+  * Cl error <= 1e-6 relative to max|Cd| (the configured cylinder is
+    mirror-symmetric, so its Cl is roundoff-seeded and has no relative scale),
+  * vorticity relative L2 <= 1e-5 after the full run (level 0, and per-level
+    norms),
+  * for CG cases, iterations per substep within +-1 of the oracle's.
+"""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from ibpm_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden")
+PLATE = ("body Plate\n  line_n -0.5 0 0.5 0 33\n  center 0 0\n"
+         "  motion pitchplunge 0.2617993877991494 0.4 0 0.2 0.4 0\nend\n")
+
+
+def _fixture(name):
+    path = os.path.join(GOLD, name + ".npz")
+    if not os.path.exists(path):
+        pytest.skip("fixture %s not generated (tests/golden/make_golden.py %s)" % (path, name))
+    return np.load(path)
+
+
+def _flow(name):
+    if name == "plate":
+        return W.Flow(128, 128, 3, 4.0, -2.0, -2.0, 300.0, 0.005, PLATE, tol=1e-10)
+    if name == "cfg4short":
+        return W.cfg4()[0]
+    return W.WORKLOADS[name]()[0]
+
+
+@pytest.mark.parametrize("name", ["cfg1", "cfg2", "plate", "cfg4short"])
+def test_full_run_matches_golden(gpu, name):
+    g = _fixture(name)
+    f = _flow(name)
+    nsteps = int(g["nsteps"])
+    print("golden", name, nsteps, "steps", flush=True)
+    ctx = gpu.Context(0)
+    b = gpu.Batch(ctx, [f]).init()
+    moving = "pitchplunge" in f.geom
+    if not moving:
+        b.use_graph(True)
+    iters = []
+    for k in range(nsteps):
+        b.advance(1)
+        if moving:
+            iters.append(b.cg_iters())
+    hist = b.force_history()
+    ref = g["forces"]
+    assert hist.shape == ref.shape
+    cdmax = np.abs(ref[:, 0]).max()
+    cd_err = np.abs(hist[:, 0] - ref[:, 0]) / np.abs(ref[:, 0])
+    cl_err = np.abs(hist[:, 1] - ref[:, 1]) / (cdmax if not moving else np.abs(ref[:, 1]).max())
+    print(name, "max Cd rel err %.3e, max Cl err %.3e" % (cd_err.max(), cl_err.max()), flush=True)
+    assert cd_err.max() <= 1e-6
+    assert cl_err.max() <= 1e-6
+    w = b.omega()
+    ns = (f.nx - 1) * (f.ny - 1)
+    s = int(g["stride"])
+    w0 = w[:ns].reshape(f.nx - 1, f.ny - 1)[::s, ::s]
+    assert np.linalg.norm(w0 - g["omega0"]) / np.linalg.norm(g["omega0"]) <= 1e-5
+    norms = np.array([np.linalg.norm(w[l * ns:(l + 1) * ns]) for l in range(f.ngrid)])
+    assert np.all(np.abs(norms - g["omega_norms"]) <= 1e-5 * g["omega_norms"])
+    if moving:
+        it = np.array(iters)
+        assert np.abs(it - g["cg_iters"]).max() <= 1, (it, g["cg_iters"])
+
+
+def test_ibpm_driver_force_file(gpu, oracle):
+    exe = os.path.join(os.path.dirname(HERE), "ibpm_amd", "ibpm")
+    geom = "body C\n circle_n 0 0 0.5 40\nend\n"
+    with tempfile.TemporaryDirectory() as d:
+        gpath = os.path.join(d, "cyl.geom")
+        open(gpath, "w").write(geom)
+        r = subprocess.run([exe, "-geom", gpath, "-nx", "64", "-ny", "64", "-ngrid", "2", "-Re", "100",
+                            "-dt", "0.02", "-nsteps", "6", "-outdir", d, "-name", "cyl"],
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        rows = [l.split() for l in open(os.path.join(d, "cyl.force")).read().splitlines()]
+        assert len(rows) == 6 and rows[0][0] == "1"
+        full = np.loadtxt(os.path.join(d, "cyl.force17"))
+    s = oracle.Sim(64, 64, 2, 4.0, -2.0, -2.0, 100.0, 0.02, geom)
+    s.init()
+    for k in range(6):
+        s.advance(1)
+        cd, cl = s.forces()
+        assert abs(full[k, 2] - cd) <= 1e-9 * abs(cd)
+        assert abs(full[k, 3] - cl) <= 1e-9 * abs(cd)
+        assert abs(float(rows[k][2]) - cd) <= 1e-5 * abs(cd)

@@ -20,6 +20,9 @@ _I = ctypes.POINTER(ctypes.c_int)
 _lib = None
 
 SCHEMES = {"euler": 0, "ab2": 1, "rk3": 2, "rk3b": 3}
+WARM = {"zero": 0, "previous": 1, "learned": 2}
+NN_IN, NN_HID = 13, 32
+NN_PARAMS = NN_HID * NN_IN + NN_HID + 2 * NN_HID + 2
 
 
 class IbpmError(RuntimeError):
@@ -74,9 +77,17 @@ def load():
         "ibpm_batch_cg_iters": [vp, ctypes.c_int, vp, ctypes.c_int],
         "ibpm_batch_use_graph": [vp, ctypes.c_int],
         "ibpm_batch_kernel_stats": [vp, ctypes.c_int, vp, vp, vp, ctypes.c_int],
+        "ibpm_batch_set_warm_start": [vp, ctypes.c_int],
+        "ibpm_batch_set_state": [vp, ctypes.c_int, vp, vp, ctypes.c_double, ctypes.c_int],
+        "ibpm_batch_collect": [vp, ctypes.c_int, ctypes.c_long],
+        "ibpm_batch_train": [vp, ctypes.c_int, ctypes.c_double, _D],
+        "ibpm_batch_nn_params": [vp, vp, vp],
+        "ibpm_nn_loss_grad": [vp, vp, vp, vp, ctypes.c_long, vp, _D],
     }.items():
         getattr(L, name).argtypes = args
         getattr(L, name).restype = ctypes.c_int
+    L.ibpm_batch_num_samples.restype = ctypes.c_long
+    L.ibpm_batch_num_samples.argtypes = [ctypes.c_void_p]
     L.ibpm_kernel_class_name.restype = ctypes.c_char_p
     L.ibpm_kernel_class_name.argtypes = [ctypes.c_int]
     L.device_count = L.ibpm_device_count
@@ -157,6 +168,15 @@ class Context:
         d_u = DeviceArray(self, ngrid * (nx - 1) * (ny - 1))
         _check(load().ibpm_elliptic_solve(self.h, kind, nx, ny, ngrid, dx, alpha, d_f.ptr, d_u.ptr))
         return d_u.get()
+
+    # -- learned warm start: loss and gradient by the training kernels -----
+    def nn_loss_grad(self, params, X, Y):
+        params, X, Y = _c(params), _c(X), _c(Y)
+        g = np.zeros(NN_PARAMS)
+        loss = ctypes.c_double()
+        _check(load().ibpm_nn_loss_grad(self.h, params.ctypes.data, X.ctypes.data, Y.ctypes.data,
+                                        X.shape[0], g.ctypes.data, ctypes.byref(loss)))
+        return loss.value, g
 
     # -- VectorOperations ------------------------------------------------
     def curl_flux(self, nx, ny, ngrid, dx, q):
@@ -254,6 +274,11 @@ class Batch:
         w = _c(w)
         _check(load().ibpm_batch_set_omega(self.h, k, w.ctypes.data))
 
+    def set_state(self, omega, f, time, timestep, k=0):
+        """Restart: vorticity, force, clock, then refreshState."""
+        omega, f = _c(omega), _c(f)
+        _check(load().ibpm_batch_set_state(self.h, k, omega.ctypes.data, f.ctypes.data, time, timestep))
+
     def q(self, k=0):
         f = self.flows[k]
         return self._get("ibpm_batch_get_q", k, f.ngrid * (2 * f.nx * f.ny + f.nx + f.ny))
@@ -278,6 +303,29 @@ class Batch:
         it = np.zeros(4, dtype=np.int32)
         n = _check(load().ibpm_batch_cg_iters(self.h, k, it.ctypes.data, 4))
         return it[:n].copy()
+
+    # -- learned CG warm start (moving bodies) ------------------------------
+    def set_warm_start(self, mode):
+        _check(load().ibpm_batch_set_warm_start(self.h, WARM[mode]))
+        return self
+
+    def collect(self, enable=True, capacity=1 << 20):
+        _check(load().ibpm_batch_collect(self.h, int(enable), capacity))
+        return self
+
+    def num_samples(self):
+        return load().ibpm_batch_num_samples(self.h)
+
+    def train(self, iters=500, lr=3e-3):
+        loss = ctypes.c_double()
+        _check(load().ibpm_batch_train(self.h, iters, lr, ctypes.byref(loss)))
+        return loss.value
+
+    def nn_params(self, new=None):
+        out = np.zeros(NN_PARAMS)
+        src = _c(new) if new is not None else None
+        _check(load().ibpm_batch_nn_params(self.h, out.ctypes.data, src.ctypes.data if src is not None else None))
+        return out
 
     def kernel_stats(self, nsteps=1):
         """Advances nsteps with a HIP event pair around every launch; returns

@@ -12,8 +12,8 @@ OBJ = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("IBPM_OFFLOAD_ARCH", "gfx950")
 
-DEVICE_SOURCES = ["elliptic.hip", "ops.hip"]
-HOST_SOURCES = ["engine.cpp", "geometry.cpp"]
+DEVICE_SOURCES = ["elliptic.hip", "ops.hip", "learn.hip"]
+HOST_SOURCES = ["engine.cpp", "geometry.cpp", "restart.cpp"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function"]
 # Packed FP32 instructions are kept out of every code object.
 DEVICE_FLAGS = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

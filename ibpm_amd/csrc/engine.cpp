@@ -20,6 +20,7 @@
 
 #include "../../include/ibpm_gpu.h"
 #include "geometry.h"
+#include "learn.h"
 #include "ops.h"
 
 namespace ibpmg {
@@ -279,6 +280,9 @@ struct Flow {
     std::vector<double> px, py, pu, pv;
     int hist_cap = 0;
     int cg_iters[4] = {-1, -1, -1, -1};
+    // learned warm start: ring of the final force and the constraint of the
+    // last kRing substeps, features and scale of the current substep
+    double *ring_f = nullptr, *ring_b = nullptr, *nn_x = nullptr, *nn_scale = nullptr;
 };
 
 }  // namespace ibpmg
@@ -313,6 +317,18 @@ struct ibpm_batch {
     ibpmg::Probe* probe = nullptr;
     double bytes_vel = 0, bytes_cross = 0, bytes_rhs = 0, bytes_curlq = 0;
     int cg_chunk = 4;
+    // learned warm start (learn.hip)
+    int warm_mode = 1;            // 0 zero, 1 previous force (reference), 2 learned
+    bool collect = false;
+    long ksub = 0;                // substeps done since init
+    int ring = 0;                 // ring length (2 S + 2)
+    double* nn_params = nullptr;
+    double *nn_m = nullptr, *nn_v = nullptr, *nn_grad = nullptr;
+    long nn_adam_t = 0;
+    bool nn_ready = false;
+    double *data_x = nullptr, *data_y = nullptr;
+    long data_cap = 0, data_n = 0;
+    ibpmg::NnOps* d_nn = nullptr;
 };
 
 namespace ibpmg {
@@ -449,6 +465,11 @@ static int alloc_flow(ibpm_batch* B, Flow& F) {
     CKR(m.alloc(&F.e_ptr, n + 1)); CKR(m.alloc(&F.e_f, cap)); CKR(m.alloc(&F.e_w, cap));
     CKR(m.alloc(&F.t_ptr, cap + 1)); CKR(m.alloc(&F.t_k, cap)); CKR(m.alloc(&F.t_w, cap));
     CKR(m.alloc(&F.node_idx, 4 * cap)); CKR(m.alloc(&F.old_idx, 4 * cap)); CKR(m.alloc(&F.node_nbr, 4 * cap));
+    {   // learned warm start history (small; allocated for every flow)
+        const int R = 2 * B->nsub + 2;
+        CKR(m.alloc(&F.ring_f, (size_t)R * n)); CKR(m.alloc(&F.ring_b, (size_t)R * n));
+        CKR(m.alloc(&F.nn_x, (size_t)kNnIn * (F.nb + 1))); CKR(m.alloc(&F.nn_scale, 1));
+    }
     return 0;
 }
 
@@ -585,6 +606,42 @@ static int enqueue_step_chol(ibpm_batch* B, hipStream_t s) {
     return 0;
 }
 
+// Descriptor of the learned warm start for substep k = B->ksub (target: the
+// final force of the substep, for sample collection).
+static int upload_nn(ibpm_batch* B, int i, bool with_target) {
+    const int nfl = (int)B->flows.size();
+    const int S = B->nsub, R = B->ring;
+    const long k = B->ksub;
+    auto slot = [&](const Flow& F, long j) { return F.ring_f + (size_t)(((j % R) + R) % R) * (2 * F.nb + 1); };
+    auto bslot = [&](const Flow& F, long j) { return F.ring_b + (size_t)(((j % R) + R) % R) * (2 * F.nb + 1); };
+    std::vector<NnOps> ops(nfl);
+    long off = B->data_n;
+    for (int q = 0; q < nfl; ++q) {
+        Flow& F = B->flows[q];
+        NnOps& o = ops[q];
+        o.nb = F.nb; o.substep = i;
+        o.fa = slot(F, k - 1); o.fb = slot(F, k - S); o.fc = slot(F, k - S - 1);
+        o.fd = slot(F, k - 2 * S); o.fe = slot(F, k - 2 * S - 1);
+        o.b_now = F.b; o.b_prev = bslot(F, k - 1);
+        o.X = F.nn_x; o.scale = F.nn_scale; o.params = B->nn_params; o.x0 = F.f;
+        if (with_target) {
+            o.target = F.f; o.data_x = B->data_x; o.data_y = B->data_y; o.data_off = off;
+            off += F.nb;
+        }
+    }
+    CKR(upload(B->d_nn, ops.data(), sizeof(NnOps) * nfl));
+    if (with_target) B->data_n = off;
+    return 0;
+}
+
+static bool nn_history_ready(const ibpm_batch* B) { return B->ksub >= 2L * B->nsub + 1; }
+
+static int nn_samples_room(const ibpm_batch* B) {
+    long need = 0;
+    for (auto& F : B->flows) need += F.nb;
+    return B->data_n + need <= B->data_cap;
+}
+
 // One substep with CG (moving bodies): host-side kinematics and regulariser,
 // then CG iterations in chunks with a device-side convergence flag
 // (ConjugateGradientSolver.cc:35-65; iterations after convergence are no-ops).
@@ -605,7 +662,18 @@ static int substep_cg(ibpm_batch* B, int i) {
     run_solve(B->S1[i], s, pr);
     run_solve(B->S2[i], s, pr);
     timed(pr, K_INTERP, 0, s, [&] { launch_interp(B->d_interp_c, nfl, B->max_comp, s); });
-    // r0 = rhs - M x0 with x0 = previous f (ProjectionSolver.cc:72 warm start)
+    // x0: previous f (ProjectionSolver.cc:72 warm start), zero, or the
+    // learned prediction; then r0 = rhs - M x0
+    if (B->warm_mode == 0) {
+        for (auto& F : B->flows) CK(hipMemsetAsync(F.f, 0, sizeof(double) * 2 * F.nb, s));
+    } else if (B->warm_mode == 2 && B->nn_ready && nn_history_ready(B)) {
+        CK(hipStreamSynchronize(s));
+        CKR(upload_nn(B, i, false));
+        int maxnb = 1;
+        for (auto& F : B->flows) maxnb = std::max(maxnb, F.nb);
+        launch_nn_features(B->d_nn, nfl, s);
+        launch_nn_infer(B->d_nn, nfl, maxnb, s);
+    }
     for (auto& F : B->flows) CK(hipMemcpyAsync(F.cg_d, F.f, sizeof(double) * 2 * F.nb, hipMemcpyDeviceToDevice, s));
     enqueue_M(B, i, B->d_smear_d[i], s);
     for (auto& F : B->flows) {
@@ -630,6 +698,21 @@ static int substep_cg(ibpm_batch* B, int i) {
             timed(pr, K_CG, 0, s, [&] { launch_cg(B->d_cg, nfl, 1, s); });
         }
     }
+    // training samples (features of this substep, target = converged f) and
+    // the force/constraint history ring
+    if (B->collect && nn_history_ready(B) && nn_samples_room(B)) {
+        CK(hipStreamSynchronize(s));
+        CKR(upload_nn(B, i, true));
+        launch_nn_features(B->d_nn, nfl, s);
+    }
+    for (auto& F : B->flows) {
+        const size_t n = 2 * F.nb + 1;
+        const long slot = B->ksub % B->ring;
+        CK(hipMemcpyAsync(F.ring_f + slot * n, F.f, sizeof(double) * 2 * F.nb, hipMemcpyDeviceToDevice, s));
+        CK(hipMemcpyAsync(F.ring_b + slot * n, F.b, sizeof(double) * 2 * F.nb, hipMemcpyDeviceToDevice, s));
+    }
+    CK(hipStreamSynchronize(s));
+    B->ksub += 1;
     timed(pr, K_SMEAR, 0, s, [&] { launch_smear_curl(B->d_smear_f[i], nfl, B->max_nodes, s); });
     run_solve(B->S3[i], s, pr);
     enqueue_refresh(B, s);
@@ -1029,6 +1112,25 @@ int ibpm_batch_create(ibpm_ctx* ctx, int nflows, const ibpm_flow_params* params,
     B->max_comp = std::max(B->max_comp, 1);
     B->max_n = std::max(B->max_n, 1);
     CKR(build_plans(B.get()));
+    if (B->any_cg) {
+        ibpm_batch* b = B.get();
+        b->ring = 2 * b->nsub + 2;
+        CKR(b->mem.alloc(&b->nn_params, kNnParams + 1));
+        CKR(b->mem.alloc(&b->nn_m, kNnParams + 1));
+        CKR(b->mem.alloc(&b->nn_v, kNnParams + 1));
+        CKR(b->mem.alloc(&b->nn_grad, kNnParams + 1));
+        CKR(b->mem.alloc(&b->d_nn, nflows));
+        // W1 ~ U(-a, a) with a = sqrt(6 / (in + hidden)) from a fixed seed;
+        // W2 = 0 so the untrained network reproduces the reference warm start
+        std::vector<double> P(kNnParams, 0.0);
+        unsigned long long st = 0x9E3779B97F4A7C15ull;
+        const double a = std::sqrt(6.0 / (kNnIn + kNnHid));
+        for (int q = 0; q < kNnHid * kNnIn; ++q) {
+            st = st * 6364136223846793005ull + 1442695040888963407ull;
+            P[q] = a * (2.0 * ((st >> 11) * (1.0 / 9007199254740992.0)) - 1.0);
+        }
+        CKR(upload(b->nn_params, P.data(), sizeof(double) * kNnParams));
+    }
     *out = B.release();
     return 0;
 }
@@ -1127,6 +1229,28 @@ int ibpm_batch_set_omega(ibpm_batch* B, int k, const double* h) {
     return 0;
 }
 
+int ibpm_batch_set_state(ibpm_batch* B, int k, const double* h_omega, const double* h_f, double time,
+                         int timestep) {
+    Flow* F = flow_of(B, k);
+    if (!F) return -1;
+    if (B->timestep != 0 && B->timestep != timestep) {
+        set_error("set_state: the batch has advanced %d steps already", B->timestep);
+        return -1;
+    }
+    CK(hipMemcpy(F->omega, h_omega, sizeof(double) * F->ns * F->L, hipMemcpyHostToDevice));
+    if (h_f) CK(hipMemcpy(F->f, h_f, sizeof(double) * 2 * F->nb, hipMemcpyHostToDevice));
+    B->time = time;
+    B->timestep = timestep;
+    if (!F->geom.stationary) {   // bodies at the restart time (ibpm.cc:285, 304)
+        F->geom.move(B->time, F->px, F->py, F->pu, F->pv);
+        CKR(update_regularizer(B, *F));
+        CKR(upload_regops(B));
+    }
+    enqueue_refresh(B, B->ctx->c.stream);
+    CK(hipStreamSynchronize(B->ctx->c.stream));
+    return 0;
+}
+
 int ibpm_batch_get_q(ibpm_batch* B, int k, double* h) {
     Flow* F = flow_of(B, k);
     if (!F) return -1;
@@ -1195,6 +1319,84 @@ int ibpm_batch_cg_iters(ibpm_batch* B, int k, int* h, int max_sub) {
 }
 
 double ibpm_batch_time(ibpm_batch* B) { return B ? B->time : 0.0; }
+
+int ibpm_batch_set_warm_start(ibpm_batch* B, int mode) {
+    if (mode < 0 || mode > 2) { set_error("warm start mode must be 0, 1 or 2"); return -1; }
+    if (mode == 2 && !B->nn_ready) { set_error("learned warm start requested before training"); return -1; }
+    B->warm_mode = mode;
+    return 0;
+}
+
+int ibpm_batch_collect(ibpm_batch* B, int enable, long capacity) {
+    if (!B->any_cg) { set_error("sample collection needs a CG (moving-body) batch"); return -1; }
+    if (enable && capacity > B->data_cap) {
+        CKR(B->mem.alloc(&B->data_x, (size_t)capacity * kNnIn));
+        CKR(B->mem.alloc(&B->data_y, (size_t)capacity * 2));
+        B->data_cap = capacity;
+        B->data_n = 0;
+    }
+    B->collect = enable != 0;
+    return 0;
+}
+
+long ibpm_batch_num_samples(ibpm_batch* B) { return B ? B->data_n : -1; }
+
+// Full-batch Adam on the collected samples; loss of the last step in *loss.
+static int nn_train(Ctx* c, double* P, double* m, double* v, double* grad, long& t, const double* X,
+                    const double* Y, long n, int iters, double lr, double* loss) {
+    if (n <= 0) { set_error("no training samples"); return -1; }
+    DevMem mem;
+    double *S = nullptr, *part = nullptr;
+    const int nchunks = (int)std::min<long>(256, std::max<long>(1, n / 64));
+    CKR(mem.alloc(&S, (size_t)n * kNnScratch));
+    CKR(mem.alloc(&part, (size_t)nchunks * (kNnParams + 1)));
+    for (int it = 0; it < iters; ++it) {
+        double b1t = 1, b2t = 1;
+        if (lr > 0) {
+            ++t;
+            b1t = std::pow(0.9, (double)t);
+            b2t = std::pow(0.999, (double)t);
+        }
+        launch_nn_grad(P, X, Y, n, S, part, nchunks, m, v, grad, lr, b1t, b2t, P, c->stream);
+    }
+    CK(hipStreamSynchronize(c->stream));
+    CK(hipGetLastError());
+    if (loss) CK(hipMemcpy(loss, grad + kNnParams, sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int ibpm_batch_train(ibpm_batch* B, int iters, double lr, double* loss) {
+    if (!B->any_cg) { set_error("training needs a CG (moving-body) batch"); return -1; }
+    CKR(nn_train(&B->ctx->c, B->nn_params, B->nn_m, B->nn_v, B->nn_grad, B->nn_adam_t, B->data_x, B->data_y,
+                 B->data_n, iters, lr, loss));
+    B->nn_ready = true;
+    return 0;
+}
+
+int ibpm_batch_nn_params(ibpm_batch* B, double* h_get, const double* h_set) {
+    if (!B->any_cg) { set_error("no network on a Cholesky batch"); return -1; }
+    if (h_set) {
+        CKR(upload(B->nn_params, h_set, sizeof(double) * kNnParams));
+        B->nn_ready = true;
+    }
+    if (h_get) CK(hipMemcpy(h_get, B->nn_params, sizeof(double) * kNnParams, hipMemcpyDeviceToHost));
+    return kNnParams;
+}
+
+int ibpm_nn_loss_grad(ibpm_ctx* c, const double* h_params, const double* h_x, const double* h_y, long n,
+                      double* h_grad, double* loss) {
+    DevMem mem;
+    double *P, *m, *v, *g, *X, *Y;
+    CKR(mem.alloc(&P, kNnParams + 1)); CKR(mem.alloc(&m, kNnParams + 1)); CKR(mem.alloc(&v, kNnParams + 1));
+    CKR(mem.alloc(&g, kNnParams + 1)); CKR(mem.alloc(&X, (size_t)n * kNnIn)); CKR(mem.alloc(&Y, (size_t)n * 2));
+    CKR(upload(P, h_params, sizeof(double) * kNnParams));
+    CKR(upload(X, h_x, sizeof(double) * n * kNnIn));
+    CKR(upload(Y, h_y, sizeof(double) * n * 2));
+    long t = 0;
+    CKR(nn_train(&c->c, P, m, v, g, t, X, Y, n, 1, 0.0, loss));
+    CK(hipMemcpy(h_grad, g, sizeof(double) * kNnParams, hipMemcpyDeviceToHost));
+    return 0;
+}
 
 const char* ibpm_kernel_class_name(int cls) {
     return (cls >= 0 && cls < K_NCLASS) ? kclass_names[cls] : "";

@@ -94,7 +94,25 @@ public:
     State(const Grid& grid, int numPoints)
         : omega((size_t)grid.Ngrid() * (grid.Nx() - 1) * (grid.Ny() - 1)),
           q((size_t)grid.Ngrid() * (2 * grid.Nx() * grid.Ny() + grid.Nx() + grid.Ny())),
-          f(2 * (size_t)numPoints), _dx(grid.Dx()) {}
+          f(2 * (size_t)numPoints), _dx(grid.Dx()), _grid(grid) {}
+    // Restart file in the reference layout, src/State.cc:65-223.
+    bool save(const std::string& filename) const {
+        return ibpm_state_save(filename.c_str(), _grid.Nx(), _grid.Ny(), _grid.Ngrid(), _grid.Dx(),
+                               _grid.getXEdge(0, 0), _grid.getYEdge(0, 0), (int)(f.size() / 2), q.data(),
+                               omega.data(), f.data(), timestep, time) == 0;
+    }
+    bool load(const std::string& filename) {
+        int nx, ny, ng, np;
+        double dx, x0, y0;
+        if (ibpm_state_load(filename.c_str(), &nx, &ny, &ng, &dx, &x0, &y0, &np, nullptr, nullptr, nullptr,
+                            nullptr, nullptr) != 0)
+            return false;
+        if (nx != _grid.Nx() || ny != _grid.Ny() || ng != _grid.Ngrid() || dx != _grid.Dx() ||
+            x0 != _grid.getXEdge(0, 0) || y0 != _grid.getYEdge(0, 0) || 2 * (size_t)np != f.size())
+            return false;   // the reference resizes and reports failure (State.cc:94-117)
+        return ibpm_state_load(filename.c_str(), &nx, &ny, &ng, &dx, &x0, &y0, &np, q.data(), omega.data(),
+                               f.data(), &timestep, &time) == 0;
+    }
     std::vector<double> omega, q, f;
     int timestep = 0;
     double time = 0.;
@@ -110,6 +128,7 @@ public:
 
 private:
     double _dx;
+    Grid _grid;
 };
 
 class NavierStokesModel {   // reference src/NavierStokesModel.h:40-106 (nonlinear model)
@@ -184,9 +203,11 @@ public:
         x.timestep += nsteps;
         gpu_check(ibpm_batch_get_f(_batch, 0, x.f.data()));
     }
-    void setState(const State& x) {   // initial condition (vorticity), then refreshState
+    // initial condition or restart: vorticity, force, time and step, then
+    // refreshState (reference src/ibpm.cc:250-305)
+    void setState(const State& x) {
         init();
-        gpu_check(ibpm_batch_set_omega(_batch, 0, x.omega.data()));
+        gpu_check(ibpm_batch_set_state(_batch, 0, x.omega.data(), x.f.data(), x.time, x.timestep));
     }
     void getState(State& x) {
         gpu_check(ibpm_batch_get_omega(_batch, 0, x.omega.data()));

@@ -3,9 +3,11 @@
 // Accepts the reference's flags and defaults (reference src/ibpm.cc:41-88)
 // for the nonlinear model and writes the force file in the reference format
 // "%5d %.5e %.5e %.5e" = step, time, Cd, Cl (src/OutputForce.cc:46-64), plus
-// <name>.force17 with full precision.  Added flag: -tol (CG tolerance, which
-// the reference sets only from code, src/ibpm.cc:292-294).  Tecplot, restart
-// and energy outputs are not produced by this driver.
+// <name>.force17 with full precision.  -restart n writes the reference's
+// binary restart file <name>%05d.bin every n steps and -ic loads one
+// (src/State.cc:65-223, src/ibpm.cc:250-274).  Added flag: -tol (CG tolerance,
+// which the reference sets only from code, src/ibpm.cc:292-294).  Tecplot and
+// energy outputs are not produced by this driver.
 #include <sys/stat.h>
 
 #include <cmath>
@@ -63,7 +65,9 @@ int main(int argc, char** argv) {
     const std::string name = P.str("name", "ibpm");
     std::string outdir = P.str("outdir", ".");
     P.i("tecplot", 100);
-    P.i("restart", 100);
+    const int iRestart = P.i("restart", 100);
+    const std::string icFile = P.str("ic", "");
+    const std::string numDigit = P.str("numdigfilename", "%05d");
     const int iForce = P.i("force", 1);
     const int nx = P.i("nx", 200), ny = P.i("ny", 200), ngrid = P.i("ngrid", 1);
     const double length = P.f("length", 4.0), xOffset = P.f("xoffset", -2.), yOffset = P.f("yoffset", -2.);
@@ -104,6 +108,11 @@ int main(int argc, char** argv) {
     nsm.init();
     try {
         solver.init();
+        if (!icFile.empty()) {
+            std::cout << "Loading initial condition from file: " << icFile << std::endl;
+            if (x.load(icFile)) solver.setState(x);
+            else std::cout << "    (failed: using zero initial condition)" << std::endl;
+        }
     } catch (const std::exception& e) {
         std::cerr << e.what() << std::endl;
         return -1;
@@ -123,6 +132,13 @@ int main(int argc, char** argv) {
         if (ff && x.timestep % iForce == 0) {
             std::fprintf(ff, "%5d %.5e %.5e %.5e\n", x.timestep, x.time, drag, lift);
             std::fprintf(ff17, "%5d %.17e %.17e %.17e\n", x.timestep, x.time, drag, lift);
+        }
+        if (iRestart > 0 && x.timestep % iRestart == 0) {
+            solver.getState(x);
+            char num[64];
+            std::snprintf(num, sizeof num, numDigit.c_str(), x.timestep);
+            if (!x.save(outdir + name + num + ".bin"))
+                std::cerr << "could not write restart file: " << ibpm_last_error() << std::endl;
         }
     }
     if (ff) std::fclose(ff);

@@ -109,6 +109,40 @@ double ibpm_batch_time(ibpm_batch* b);
 /* Capture one timestep as a hipGraph and replay it (stationary bodies). */
 int ibpm_batch_use_graph(ibpm_batch* b, int enable);
 
+/* ---- restart files ---------------------------------------------------------
+   The reference's binary restart layout (State::save / State::load,
+   src/State.cc:65-223).  ibpm_state_load with null q/omega/f reads only the
+   header, so buffers can be sized first. */
+int ibpm_state_save(const char* path, int nx, int ny, int ngrid, double dx, double x0, double y0, int npts,
+                    const double* q, const double* omega, const double* f, int timestep, double time);
+int ibpm_state_load(const char* path, int* nx, int* ny, int* ngrid, double* dx, double* x0, double* y0,
+                    int* npts, double* q, double* omega, double* f, int* timestep, double* time);
+/* Resume: sets a flow's vorticity and force, the batch clock (time, step),
+   then refreshState (ibpm.cc:250-305 loads an initial condition this way). */
+int ibpm_batch_set_state(ibpm_batch* b, int flow, const double* h_omega, const double* h_f, double time,
+                         int timestep);
+
+/* ---- learned CG warm start (moving bodies) ---------------------------------
+   No counterpart in the reference: a per-boundary-point network (13 -> 32 tanh
+   -> 2, FP64) predicts the force increment over the reference warm start
+   f_{k-1} (src/ProjectionSolver.cc:72) from the force history; CG then runs to
+   the same absolute residual tolerance, so only iteration counts change.
+   mode: 0 zero start, 1 previous force (reference), 2 learned. */
+int ibpm_batch_set_warm_start(ibpm_batch* b, int mode);
+/* Record (features, converged increment) samples of every substep during
+   ibpm_batch_advance into a device data set of `capacity` points. */
+int ibpm_batch_collect(ibpm_batch* b, int enable, long capacity);
+long ibpm_batch_num_samples(ibpm_batch* b);
+/* Full-batch Adam (iters steps, learning rate lr) on the collected samples,
+   with the gradient and update computed by HIP kernels; *loss = final loss. */
+int ibpm_batch_train(ibpm_batch* b, int iters, double lr, double* loss);
+/* Copies the network parameters out (h_get) and/or in (h_set); returns the count. */
+int ibpm_batch_nn_params(ibpm_batch* b, double* h_get, const double* h_set);
+/* Mean-squared-error loss and its gradient over n host samples (X: n x 13,
+   Y: n x 2), computed by the training kernels (gradient checks). */
+int ibpm_nn_loss_grad(ibpm_ctx* ctx, const double* h_params, const double* h_x, const double* h_y, long n,
+                      double* h_grad, double* loss);
+
 /* Measurement: advances nsteps with a HIP event pair around every kernel
    launch on the batch stream and returns per kernel class the launch count,
    total milliseconds and algorithmic HBM bytes.  Returns the number of

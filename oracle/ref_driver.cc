@@ -130,4 +130,35 @@ void ref_net_force(int nx, int ny, int ngrid, double length, double xoff, double
     x.computeNetForce(*xf, *yf);
 }
 
+// State::save of a state built from flat arrays (State.cc:158-223).
+int ref_state_save(const char* path, int nx, int ny, int ngrid, double length, double xoff, double yoff, int npts,
+                   const double* q, const double* omega, const double* f, int timestep, double time) {
+    Grid grid(nx, ny, ngrid, length, xoff, yoff);
+    State x(grid, npts);
+    long nf = 2L * nx * ny + nx + ny;
+    for (int lev = 0; lev < ngrid; ++lev)
+        for (long k = 0; k < nf; ++k) x.q(lev, (int)k) = q[lev * nf + k];
+    load_scalar(x.omega, omega);
+    for (int k = 0; k < 2 * npts; ++k) x.f(k) = f[k];
+    x.timestep = timestep;
+    x.time = time;
+    return x.save(path) ? 0 : -1;
+}
+
+// State::load (State.cc:65-156) into flat arrays sized by the caller.
+int ref_state_load(const char* path, int nx, int ny, int ngrid, double length, double xoff, double yoff, int npts,
+                   double* q, double* omega, double* f, int* timestep, double* time) {
+    Grid grid(nx, ny, ngrid, length, xoff, yoff);
+    State x(grid, npts);
+    if (!x.load(path)) return -1;
+    long nf = 2L * nx * ny + nx + ny;
+    for (int lev = 0; lev < ngrid; ++lev)
+        for (long k = 0; k < nf; ++k) q[lev * nf + k] = x.q(lev, (int)k);
+    store_scalar(x.omega, omega);
+    for (int k = 0; k < 2 * npts; ++k) f[k] = x.f(k);
+    *timestep = x.timestep;
+    *time = x.time;
+    return 0;
+}
+
 }  // extern "C"

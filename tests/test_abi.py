@@ -53,3 +53,46 @@ def test_bad_geometry_is_an_error_not_an_abort(lib):
 
 def test_driver_is_built():
     assert os.access(os.path.join(ROOT, "ibpm_amd", "ibpm"), os.X_OK)
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def test_restart_file_round_trips_with_reference_state(lib, tmp_path):
+    """ibpm_state_save -> reference State::load and reference State::save ->
+    ibpm_state_load, bit-exact (src/State.cc:65-223)."""
+    ref_path = os.path.join(ROOT, "oracle", "_ref", "libibpm_refpart.so")
+    if not os.path.exists(ref_path):
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    R = ctypes.CDLL(ref_path)
+    import numpy as np
+    nx, ny, ng, L, x0, y0, npts = 16, 12, 2, 2.0, -1.0, -0.5, 7
+    rng = np.random.default_rng(4)
+    q = rng.standard_normal(ng * (2 * nx * ny + nx + ny))
+    w = rng.standard_normal(ng * (nx - 1) * (ny - 1))
+    f = rng.standard_normal(2 * npts)
+    D = ctypes.c_double
+    a = str(tmp_path / "ours.bin").encode()
+    lib.ibpm_state_save.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, D, D, D,
+                                    ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                    ctypes.c_int, D]
+    assert lib.ibpm_state_save(a, nx, ny, ng, L / nx, x0, y0, npts, q.ctypes.data, w.ctypes.data,
+                               f.ctypes.data, 17, D(0.34)) == 0
+    q2, w2, f2 = np.zeros_like(q), np.zeros_like(w), np.zeros_like(f)
+    ts, tt = ctypes.c_int(), ctypes.c_double()
+    assert R.ref_state_load(a, nx, ny, ng, D(L), D(x0), D(y0), npts, _p(q2), _p(w2), _p(f2),
+                            ctypes.byref(ts), ctypes.byref(tt)) == 0
+    assert np.array_equal(q, q2) and np.array_equal(w, w2) and np.array_equal(f, f2)
+    assert ts.value == 17 and tt.value == 0.34
+    b = str(tmp_path / "ref.bin").encode()
+    assert R.ref_state_save(b, nx, ny, ng, D(L), D(x0), D(y0), npts, _p(q), _p(w), _p(f), 23, D(1.5)) == 0
+    assert open(a, "rb").read()[:-12] == open(b, "rb").read()[:-12]   # same bytes up to step/time
+    lib.ibpm_state_load.argtypes = [ctypes.c_char_p] + [ctypes.c_void_p] * 12
+    hdr = [ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), D(), D(), D(), ctypes.c_int()]
+    q3, w3, f3 = np.zeros_like(q), np.zeros_like(w), np.zeros_like(f)
+    assert lib.ibpm_state_load(b, *[ctypes.byref(h) for h in hdr], q3.ctypes.data, w3.ctypes.data,
+                               f3.ctypes.data, ctypes.byref(ts), ctypes.byref(tt)) == 0
+    assert [h.value for h in hdr] == [nx, ny, ng, L / nx, x0, y0, npts]
+    assert np.array_equal(q, q3) and np.array_equal(w, w3) and np.array_equal(f, f3)
+    assert ts.value == 23 and tt.value == 1.5

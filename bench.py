@@ -7,7 +7,10 @@ CPU oracle timed beside it.
 A step is one full RK3 timestep (3 substeps) of every flow in the workload;
 grid-pt counts the stored interior nodes of all levels.  Inputs are resident
 in HBM when the timed region starts; the timed region is K hipGraph replays
-(stationary bodies) or K eager steps with CG (moving bodies).
+(stationary bodies) or K eager steps with the CG force solve (moving bodies).
+For moving bodies the batched, preconditioned form is timed, then the plain
+GPU form (one flow, plain CG as the reference) on the first flow for the same
+steps, with the speed-up and the Cd/Cl difference between the two.
 """
 import argparse
 import json
@@ -29,29 +32,87 @@ CONFIG_TEXT = {
 }
 
 
-def cpu_baseline(flows, budget_s):
-    """Times the FP64 C oracle on the host (1 core) on the same workload's
-    first flow: a bounded sample of the step loop after a set-up that skips
-    the one-time Cholesky build."""
+def _static_geom(geom):
+    return "".join(l + "\n" for l in geom.splitlines() if not l.strip().startswith("motion"))
+
+
+def cpu_baseline(flows, budget_s, cg_iters_per_step=None):
+    """Times the FP64 C oracle on the host (1 core) on the workload's first
+    flow.  Stationary bodies: a bounded sample of the step loop after a set-up
+    that skips the one-time Cholesky build.  Moving bodies (CG): one oracle
+    step of the same grid with the body held still (everything but the force
+    solve) plus the oracle's M application timed on the moving body, times the
+    plain-CG iterations per step that the GPU plain form measured (the same
+    algorithm and counts as the oracle)."""
     from oracle import oracle as O
+    from ibpm_amd.workloads import grid_points
+    import numpy as np
     f = flows[0]
-    s = O.Sim(f.nx, f.ny, f.ngrid, f.length, f.xoffset, f.yoffset, f.Re, f.dt, f.geom,
-              scheme=f.scheme, tol=f.tol, force_cg=f.force_cg)
+    g = grid_points(f)
+    moving = "pitchplunge" in f.geom
+    s = O.Sim(f.nx, f.ny, f.ngrid, f.length, f.xoffset, f.yoffset, f.Re, f.dt,
+              _static_geom(f.geom) if moving else f.geom, scheme=f.scheme, tol=f.tol,
+              force_cg=f.force_cg and not moving)
     s.init_timing()
     t0 = time.perf_counter()
     s.advance(1)
-    t1 = time.perf_counter()
-    per = t1 - t0
-    n = max(1, min(200, int(budget_s / max(per, 1e-6))))
+    per = time.perf_counter() - t0
+    n = max(1, min(200, int(budget_s / max(per, 1e-6)))) if not moving else 1
     t0 = time.perf_counter()
     s.advance(n)
-    t1 = time.perf_counter()
+    t_step = (time.perf_counter() - t0) / n
+    if not moving:
+        return {"value": g / t_step, "unit": "grid-pt*steps/s", "cores": 1, "kind": "port",
+                "sample": "%d steps of the first flow (%dx%d, %d levels) after 1 warm step; "
+                          "Cholesky build excluded (identity factors, same triangular-solve loops)"
+                          % (n, f.nx, f.ny, f.ngrid)}
+    m = O.Sim(f.nx, f.ny, f.ngrid, f.length, f.xoffset, f.yoffset, f.Re, f.dt, f.geom,
+              scheme=f.scheme, tol=f.tol)
+    m.init()
+    x = np.random.default_rng(0).standard_normal(2 * m.nb)
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        m.M(0, x)
+    t_m = (time.perf_counter() - t0) / reps
+    its = cg_iters_per_step or 0.0
+    t = t_step + its * t_m
+    return {"value": g / t, "unit": "grid-pt*steps/s", "cores": 1, "kind": "port",
+            "sample": "first flow (%dx%d, %d levels): 1 oracle step with the body held still "
+                      "(%.2f s) + %.1f plain-CG iterations/step x oracle M application (%.3f s, mean of %d)"
+                      % (f.nx, f.ny, f.ngrid, t_step, its, t_m, reps)}
+
+
+def timed_steps(b, nflows, steps, moving):
+    """Advances `steps` steps; returns (seconds, CG iterations per step per flow)."""
+    import ibpm_amd
+    sync = ibpm_amd.load().ibpm_ctx_synchronize
+    sync(b.ctx.h)
+    iters = []
+    t0 = time.perf_counter()
+    if moving:
+        for _ in range(steps):
+            b.advance(1)
+            iters.append(sum(int(b.cg_iters(k).sum()) for k in range(nflows)) / nflows)
+    else:
+        b.advance(steps)
+    sync(b.ctx.h)
+    return time.perf_counter() - t0, (sum(iters) / len(iters) if iters else None)
+
+
+def plain_form(ctx, flow, total_steps, steps):
+    """The plain GPU form on one flow: plain CG from the previous force (the
+    reference's solver), no batching; advances total_steps and times the
+    last `steps`.  Returns throughput, iterations per step and final Cd, Cl."""
+    import ibpm_amd
     from ibpm_amd.workloads import grid_points
-    g = grid_points(f)
-    return {"value": g * n / (t1 - t0), "unit": "grid-pt*steps/s", "cores": 1, "kind": "port",
-            "sample": "%d steps of the first flow (%dx%d, %d levels) after 1 warm step; "
-                      "Cholesky build excluded (identity factors, same triangular-solve loops)"
-                      % (n, f.nx, f.ny, f.ngrid)}
+    b = ibpm_amd.Batch(ctx, [flow]).init()
+    b.advance(total_steps - steps)
+    sec, its = timed_steps(b, 1, steps, True)
+    cd, cl = b.forces(0)
+    b.close()
+    return {"value": grid_points(flow) * steps / sec, "ms_per_step": sec / steps * 1e3,
+            "cg_iters_per_step": its, "Cd": cd, "Cl": cl}
 
 
 def main():
@@ -63,10 +124,16 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--precond", default="lowrank", choices=["none", "stale", "lowrank"],
+                    help="force solve of moving bodies: plain CG (the reference's), CG preconditioned "
+                         "by a stale FP64 inverse of M, or that plus the low-rank update")
+    ap.add_argument("--pairs", type=int, default=8)
     ap.add_argument("--warm", default="previous", choices=["previous", "zero", "learned"],
-                    help="CG warm start of moving-body workloads (previous = the reference's)")
+                    help="CG start of moving-body workloads (previous = the reference's)")
     ap.add_argument("--train-steps", type=int, default=30,
                     help="steps of sample collection before training the learned warm start")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the plain GPU form (moving bodies) run beside the batched one")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
@@ -84,13 +151,17 @@ def main():
     if not args.no_graph and not moving:
         b.use_graph(True)
     learn = None
+    done_steps = 0
+    if moving:
+        b.set_precond(args.precond, args.pairs)
     if moving and args.warm == "learned":
-        # sample collection with the reference warm start, then training on the GPU
+        # sample collection, then training on the GPU
         b.collect(True, 4 << 20)
         its = []
         for _ in range(args.train_steps):
             b.advance(1)
             its.append(sum(int(b.cg_iters(k).sum()) for k in range(len(flows))) / len(flows))
+        done_steps += args.train_steps
         t_tr = time.perf_counter()
         loss0 = b.train(iters=1, lr=0.0)
         loss = b.train(iters=800, lr=3e-3)
@@ -100,32 +171,41 @@ def main():
         b.collect(False).set_warm_start("learned")
     elif moving and args.warm == "zero":
         b.set_warm_start("zero")
+    t_w0 = time.perf_counter()
     b.advance(args.warmup)
-    ctx_sync = ibpm_amd.load().ibpm_ctx_synchronize
-    ctx_sync(ctx.h)
-    iters = []
-    t0 = time.perf_counter()
-    if moving:
-        for _ in range(steps):
-            b.advance(1)
-            iters.append(sum(int(b.cg_iters(k).sum()) for k in range(len(flows))) / len(flows))
-    else:
-        b.advance(steps)
-    ctx_sync(ctx.h)
-    t1 = time.perf_counter()
-    sec = t1 - t0
+    t_warm = time.perf_counter() - t_w0
+    done_steps += args.warmup
+    st0 = b.solver_stats() if moving else None
+    sec, cg = timed_steps(b, len(flows), steps, moving)
+    done_steps += steps
     gp = sum(grid_points(f) for f in flows)
     value = gp * steps / sec
-    cg = (sum(iters) / len(iters)) if moving else None
+    cd, cl = b.forces(0)
+    solver = None
+    if moving:
+        st1 = b.solver_stats()
+        d = {k: st1[k] - st0[k] for k in st1}
+        solver = {"precond": args.precond, "pairs": args.pairs if args.precond == "lowrank" else 0,
+                  "warm_start": args.warm, "cg_iters_per_step": cg,
+                  "batched_m_applications_per_step": (d["m_applications"] + d["build_m_applications"]) / steps,
+                  "builds_in_timed_steps": d["builds"], "fallbacks": d["fallbacks"],
+                  "warmup_s_including_first_build": round(t_warm, 3)}
     # live per-kernel timing (HIP events on the library stream, eager steps)
     stats = b.kernel_stats(2 if moving else 3)
-    dom = max(stats.items(), key=lambda kv: kv[1][1])
+    b.close()
+    dom = max(stats.items(), key=lambda kv: kv[1][1] if kv[1][2] else 0.0)
     name, (cnt, ms, by) = dom
     avg_s = ms / cnt / 1e3
     achieved = by / cnt / avg_s / 1e9
-    cd, cl = b.forces(0)
+    plain = None
+    if moving and not args.no_plain:
+        plain = plain_form(ctx, flows[0], done_steps, steps)
+        plain["speedup_of_this_form"] = (value / plain["value"])
+        plain["Cd_rel_err_vs_this_form_flow0"] = abs(cd - plain["Cd"]) / abs(plain["Cd"])
+        plain["Cl_rel_err_vs_this_form_flow0"] = abs(cl - plain["Cl"]) / max(abs(plain["Cl"]), 1e-300)
     line = {
-        "metric": "grid-pt*steps/s (projection step, %s)" % ("moving-body CG" if moving else "Cholesky"),
+        "metric": "grid-pt*steps/s (projection step, %s)" % (
+            "moving bodies, batched + preconditioned CG" if moving else "stationary bodies, Cholesky"),
         "value": value,
         "unit": "grid-pt*steps/s",
         "n_gpus": 1,
@@ -139,9 +219,8 @@ def main():
         "data": "synthetic (configured geometry and parameters, zero initial vorticity)",
         "config": {"workload": args.workload, "description": CONFIG_TEXT[args.workload],
                    "flows": len(flows), "grid_points": gp, "parallelism": "single GPU",
-                   "warm_start": args.warm if moving else None,
-                   "cg_iters_per_step": cg, "learned": learn, "init_s": round(t_init, 3),
-                   "Cd_flow0": cd, "Cl_flow0": cl},
+                   "solver": solver, "learned": learn, "plain_gpu_form": plain,
+                   "init_s": round(t_init, 3), "Cd_flow0": cd, "Cl_flow0": cl},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "avg_launch_us": avg_s * 1e6, "bytes_per_launch": by / cnt},
@@ -149,7 +228,8 @@ def main():
                     for k, (c, m, bb) in sorted(stats.items(), key=lambda kv: -kv[1][1])},
     }
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(flows, args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(flows, args.cpu_seconds,
+                                            plain["cg_iters_per_step"] if plain else None)
     print(json.dumps(line))
 
 

@@ -83,6 +83,8 @@ def load():
         "ibpm_batch_train": [vp, ctypes.c_int, ctypes.c_double, _D],
         "ibpm_batch_nn_params": [vp, vp, vp],
         "ibpm_nn_loss_grad": [vp, vp, vp, vp, ctypes.c_long, vp, _D],
+        "ibpm_batch_set_precond": [vp, ctypes.c_int, ctypes.c_int],
+        "ibpm_batch_solver_stats": [vp, vp, ctypes.c_int],
     }.items():
         getattr(L, name).argtypes = args
         getattr(L, name).restype = ctypes.c_int
@@ -303,6 +305,25 @@ class Batch:
         it = np.zeros(4, dtype=np.int32)
         n = _check(load().ibpm_batch_cg_iters(self.h, k, it.ctypes.data, 4))
         return it[:n].copy()
+
+    # -- preconditioned CG force solve (moving bodies) ----------------------
+    def set_precond(self, mode="lowrank", pairs=8):
+        """mode "none": plain CG (the reference's solver); "stale": CG
+        preconditioned by the FP64 inverse of M at an earlier body position;
+        "lowrank": that plus a low-rank update from the previous solve's
+        search directions (`pairs` of them, at most 8)."""
+        m = {"none": 0, "stale": 1, "lowrank": 1}[mode]
+        k = pairs if mode == "lowrank" else 0
+        _check(load().ibpm_batch_set_precond(self.h, m, k))
+        return self
+
+    SOLVER_STATS = ("iterations", "solves", "m_applications", "build_m_applications", "builds",
+                    "fallbacks", "abandoned")
+
+    def solver_stats(self):
+        out = np.zeros(len(self.SOLVER_STATS), dtype=np.int64)
+        n = _check(load().ibpm_batch_solver_stats(self.h, out.ctypes.data, len(out)))
+        return dict(zip(self.SOLVER_STATS[:n], (int(v) for v in out[:n])))
 
     # -- learned CG warm start (moving bodies) ------------------------------
     def set_warm_start(self, mode):

@@ -50,10 +50,13 @@ struct EllJob {
     double* T2;              // scratch (nx-1) x (ny-1)
     double* u;               // level slice of the solution
     const double* W;         // epilogue 1 operand (level slice)
+    double* vside;           // epilogue solves, levels >= 1: v (before the epilogue) stored at
+                             // the points the next finer level's getBC reads (level slice)
     const double* cosx;      // cos(pi i / nx), i = 1..nx-1
     const double* cosy;      // cos(pi j / ny), j = 1..ny-1
     FftPlan pj;              // transform along j (length ny-1)
     FftPlan pi;              // transform along i (length nx-1)
+    const double* skip;      // null, or the flow's solver flags (see skipped())
 };
 
 // Coarsify job: one level (lev >= 1) of one flow (reference Scalar.cc:46-63).
@@ -61,7 +64,15 @@ struct CoarsifyJob {
     int nx, ny, lev;
     const double* f;         // whole field (all levels)
     double* R;               // whole coarsified buffer (all levels; central regions used)
+    const double* skip;
 };
+
+// Force-solve flags of a flow: flags[0] converged, flags[1] broken down.  A
+// kernel given a flow's flags does nothing for that flow once either is set,
+// so the batched M applications of finished flows cost no bandwidth.
+__host__ __device__ inline bool skipped(const double* flags) {
+    return flags && (flags[0] != 0.0 || flags[1] != 0.0);
+}
 
 __host__ __device__ inline int nx_ext(int nx) { return nx / 4; }   // Grid.h:89-91, zero shift
 

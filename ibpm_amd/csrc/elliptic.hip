@@ -226,6 +226,15 @@ __device__ inline double rhs_value(const EllJob& J, int i, int j) {
     return v;
 }
 
+// Points of a coarse level (lev >= 1) that getBC of the next finer level
+// reads: the rectangle i = ex..ex+nx/2, j = ey..ey+ny/2 traced by the finer
+// domain's boundary (Scalar.cc:91-127).
+__device__ inline bool on_bc_ring(const EllJob& J, int i, int j) {
+    const int ex = nx_ext(J.nx), ey = nx_ext(J.ny);
+    const int i1 = ex + J.nx / 2, j1 = ey + J.ny / 2;
+    return ((j == ey || j == j1) && i >= ex && i <= i1) || ((i == ex || i == i1) && j >= ey && j <= j1);
+}
+
 // The three passes are written once over a template length CM (0: runtime
 // plan).  Row length N, FFT length M and rows per block follow from the plan,
 // so with CM != 0 every index computation folds to constants.
@@ -360,15 +369,19 @@ __device__ inline void pass3_body(const EllJob& J, double2* smem) {
         const double2 z = buf[p * M + k + 1];
         const int ra = r0 + 2 * p, rb = ra + 1;
         // sinTransformInv normalisation, EllipticSolver2d.cc:80-83
+        // with the epilogue, the finer level's BCs must come from v itself
+        // (EllipticSolver.cc:62-66 solves c = A^-1 c before omega = omega* - c)
         if (ra < N1) {
             const long o = (long)ra * N2 + k;
             const double v = -z.y * J.norm;
             J.u[o] = J.epilogue ? J.W[o] - v : v;
+            if (J.vside && on_bc_ring(J, ra + 1, k + 1)) J.vside[o] = v;
         }
         if (rb < N1) {
             const long o = (long)rb * N2 + k;
             const double v = z.x * J.norm;
             J.u[o] = J.epilogue ? J.W[o] - v : v;
+            if (J.vside && on_bc_ring(J, rb + 1, k + 1)) J.vside[o] = v;
         }
     }
 }
@@ -389,18 +402,21 @@ __device__ inline void pass3_body(const EllJob& J, double2* smem) {
 __global__ void __launch_bounds__(kFftThreads) k_ell_pass1(const EllJob* __restrict__ jobs) {
     extern __shared__ double2 smem[];
     const EllJob& J = jobs[blockIdx.y];
+    if (skipped(J.skip)) return;
     IBPM_DST_DISPATCH(pass1_body, J.pj.M)
 }
 
 __global__ void __launch_bounds__(kFftThreads) k_ell_pass2(const EllJob* __restrict__ jobs) {
     extern __shared__ double2 smem[];
     const EllJob& J = jobs[blockIdx.y];
+    if (skipped(J.skip)) return;
     IBPM_DST_DISPATCH(pass2_body, J.pi.M)
 }
 
 __global__ void __launch_bounds__(kFftThreads) k_ell_pass3(const EllJob* __restrict__ jobs) {
     extern __shared__ double2 smem[];
     const EllJob& J = jobs[blockIdx.y];
+    if (skipped(J.skip)) return;
     IBPM_DST_DISPATCH(pass3_body, J.pj.M)
 }
 
@@ -419,6 +435,7 @@ __device__ inline double coarse_src(const CoarsifyJob& J, int lf, int i, int j) 
 
 __global__ void __launch_bounds__(kThreads) k_coarsify(const CoarsifyJob* __restrict__ jobs) {
     const CoarsifyJob& J = jobs[blockIdx.y];
+    if (skipped(J.skip)) return;
     const int nx = J.nx, ny = J.ny;
     const int ex = nx_ext(nx), ey = nx_ext(ny);
     const int wi = nx / 2 - 1, wj = ny / 2 - 1;

@@ -121,6 +121,8 @@ struct SolveSpec {
     const double* cosx;
     const double* cosy;
     const double* bcring;  // single-level explicit boundary values (L == 1 only)
+    const double* skip;    // null, or the flow's force-solve flags
+    double* vside;         // epilogue solves: side buffer (L levels) for the BC values of v
 };
 
 // Optional per-launch timing: a HIP event pair around every kernel launch on
@@ -179,7 +181,7 @@ static int build_solve_plan(Ctx* c, DevMem& mem, const std::vector<SolveSpec>& s
         int pts = 0;
         for (auto& s : specs) {
             if (lev >= s.L) continue;
-            jobs.push_back(CoarsifyJob{s.nx, s.ny, lev, s.f, s.R});
+            jobs.push_back(CoarsifyJob{s.nx, s.ny, lev, s.f, s.R, s.skip});
             pts = std::max(pts, (s.nx / 2 - 1) * (s.ny / 2 - 1));
         }
         CoarsifyJob* d = nullptr;
@@ -214,12 +216,17 @@ static int build_solve_plan(Ctx* c, DevMem& mem, const std::vector<SolveSpec>& s
             J.norm = 1. / (2 * s.nx * 2 * s.ny);                         // :80
             J.f = s.f + lev * ns;
             J.R = s.R ? s.R + lev * ns : nullptr;
-            J.uc = (lev < s.L - 1) ? s.u + (lev + 1) * ns : nullptr;
+            // getBC source: the coarser level of the solution; for u = W - v
+            // that is v, kept on the BC ring by the coarser level's pass 3
+            const double* bcsrc = s.vside ? s.vside : s.u;
+            J.uc = (lev < s.L - 1) ? bcsrc + (lev + 1) * ns : nullptr;
+            J.vside = (s.vside && lev >= 1) ? s.vside + lev * ns : nullptr;
             J.bcring = (s.L == 1) ? s.bcring : nullptr;
             J.T1 = s.T1; J.T2 = s.T2;
             J.u = s.u + lev * ns;
             J.W = s.W ? s.W + lev * ns : nullptr;
             J.cosx = s.cosx; J.cosy = s.cosy;
+            J.skip = s.skip;
             CKR(get_plan(c, s.ny - 1, &J.pj));
             CKR(get_plan(c, s.nx - 1, &J.pi));
             for (int k = 0; k < 3; ++k) {
@@ -270,6 +277,12 @@ struct Flow {
     double *f, *rhs, *b, *cg_r, *cg_d, *cg_q, *cg_state, *hist, *unit, *mcol;
     int* hist_count;
     double* minv[4] = {nullptr, nullptr, nullptr, nullptr};
+    // preconditioned CG: dense P, its input/output, the Krylov pairs of the
+    // last solve of each substep (double-buffered) and their counts
+    double *pinv = nullptr, *cg_t = nullptr, *cg_u = nullptr, *lmp_g = nullptr, *lmp_c = nullptr;
+    double *lmp_s[4][2] = {}, *lmp_y[4][2] = {};
+    int* lmp_cnt = nullptr;
+    double pbeta = 0.0;            // beta of the substep P was built for
     // regulariser
     RegLists reg;
     int *e_ptr, *e_f, *t_ptr, *t_k, *node_idx, *old_idx;
@@ -329,6 +342,21 @@ struct ibpm_batch {
     double *data_x = nullptr, *data_y = nullptr;
     long data_cap = 0, data_n = 0;
     ibpmg::NnOps* d_nn = nullptr;
+    // force-solve state of every flow, 8 doubles each: [rho or delta, tol^2,
+    // iterations, converged, broken down, pairs in use, r.r, spare]
+    double* cg_states = nullptr;
+    // preconditioned CG
+    int pc_mode = 0, pc_pairs = 0;
+    bool pc_valid = false, pc_broken = false;
+    int pc_par[4] = {0, 0, 0, 0};
+    int pc_base[4] = {-1, -1, -1, -1};   // iterations of the first solve after a build
+    int pc_last[4] = {0, 0, 0, 0};
+    double pc_excess = 0.0;              // iterations above pc_base since the build
+    ibpmg::PcgOps* d_pcg[4][2] = {};
+    ibpmg::MinvOps* d_pminv[4] = {};
+    // counters: iterations, solves, M applications in solves, M applications
+    // in preconditioner builds, builds, fallbacks to plain CG
+    long st_iters = 0, st_solves = 0, st_mapps = 0, st_build_mapps = 0, st_builds = 0, st_fallbacks = 0;
 };
 
 namespace ibpmg {
@@ -424,6 +452,7 @@ static int upload_regops(ibpm_batch* B) {
         ic[k].psi = F.psi; ic[k].b = F.b; ic[k].out = F.rhs;
         im[k] = make_regops(F, 0);
         im[k].psi = F.psi; im[k].b = nullptr; im[k].out = F.cg_q;
+        im[k].skip = F.cg_state + 3;
         cl[k] = make_regops(F, 0);
         cl[k].cout = F.c;
         B->max_nodes = std::max(B->max_nodes, std::max(F.nnodes, F.nold));
@@ -439,6 +468,7 @@ static int upload_regops(ibpm_batch* B) {
             sf[k].fin = F.f; sf[k].cout = F.c;
             sd[k] = make_regops(F, beta_of(B, F, i));
             sd[k].fin = F.cg_d; sd[k].cout = F.c;
+            sd[k].skip = F.cg_state + 3;
         }
         CKR(upload(B->d_smear_f[i], sf.data(), sizeof(RegOps) * nf));
         CKR(upload(B->d_smear_d[i], sd.data(), sizeof(RegOps) * nf));
@@ -458,7 +488,7 @@ static int alloc_flow(ibpm_batch* B, Flow& F) {
     const int n = 2 * F.nb + 1;
     CKR(m.alloc(&F.f, n)); CKR(m.alloc(&F.rhs, n)); CKR(m.alloc(&F.b, n));
     CKR(m.alloc(&F.cg_r, n)); CKR(m.alloc(&F.cg_d, n)); CKR(m.alloc(&F.cg_q, n));
-    CKR(m.alloc(&F.cg_state, 4)); CKR(m.alloc(&F.unit, n)); CKR(m.alloc(&F.mcol, (size_t)n * n));
+    CKR(m.alloc(&F.unit, n)); CKR(m.alloc(&F.mcol, (size_t)n * n));
     F.hist_cap = 1 << 16;
     CKR(m.alloc(&F.hist, 2 * (size_t)F.hist_cap)); CKR(m.alloc(&F.hist_count, 1));
     const int cap = 18 * F.nb + 1;
@@ -485,6 +515,8 @@ static SolveSpec spec_of(const Flow& F, int kind, double alpha, const double* f,
 static int build_plans(ibpm_batch* B) {
     Ctx* c = &B->ctx->c;
     const int nfl = (int)B->flows.size();
+    CKR(B->mem.alloc(&B->cg_states, 8 * (size_t)nfl));
+    for (int k = 0; k < nfl; ++k) B->flows[k].cg_state = B->cg_states + 8 * k;
     for (int i = 0; i < B->nsub; ++i) {
         std::vector<SolveSpec> s1, s2, s3, m1;
         for (auto& F : B->flows) {
@@ -492,7 +524,9 @@ static int build_plans(ibpm_batch* B) {
             s1.push_back(spec_of(F, 1, alpha, F.a, F.wstar, 1.0, 0, nullptr));
             s2.push_back(spec_of(F, 0, 0.0, F.wstar, F.psi, -1.0, 0, nullptr));
             s3.push_back(spec_of(F, 1, alpha, F.c, F.omega, 1.0, 1, F.wstar));
+            s3.back().vside = F.mt;   // M-temp is free after the force solve
             m1.push_back(spec_of(F, 1, alpha, F.c, F.mt, 1.0, 0, nullptr));
+            m1.back().skip = F.cg_state + 3;
         }
         CKR(build_solve_plan(c, B->mem, s1, B->S1[i]));
         CKR(build_solve_plan(c, B->mem, s2, B->S2[i]));
@@ -503,6 +537,7 @@ static int build_plans(ibpm_batch* B) {
     for (auto& F : B->flows) {
         s4.push_back(spec_of(F, 0, 0.0, F.omega, F.psi, -1.0, 0, nullptr));
         m2.push_back(spec_of(F, 0, 0.0, F.mt, F.psi, -1.0, 0, nullptr));
+        m2.back().skip = F.cg_state + 3;
     }
     CKR(build_solve_plan(c, B->mem, s4, B->S4));
     CKR(build_solve_plan(c, B->mem, m2, B->M2));
@@ -642,6 +677,104 @@ static int nn_samples_room(const ibpm_batch* B) {
     return B->data_n + need <= B->data_cap;
 }
 
+static int build_precond(ibpm_batch* B, int i);
+
+// Force-solve state of every flow: zero, with tol^2 in slot 1.
+static int reset_states(ibpm_batch* B) {
+    hipStream_t s = B->ctx->c.stream;
+    const int nfl = (int)B->flows.size();
+    std::vector<double> st(8 * (size_t)nfl, 0.0);
+    for (int k = 0; k < nfl; ++k) st[8 * k + 1] = B->flows[k].p.cg_tol * B->flows[k].p.cg_tol;
+    CK(hipMemcpyAsync(B->cg_states, st.data(), sizeof(double) * st.size(), hipMemcpyHostToDevice, s));
+    CK(hipStreamSynchronize(s));
+    return 0;
+}
+
+static int read_states(ibpm_batch* B, std::vector<double>& st) {
+    CK(hipStreamSynchronize(B->ctx->c.stream));
+    st.resize(8 * B->flows.size());
+    CK(hipMemcpy(st.data(), B->cg_states, sizeof(double) * st.size(), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// Plain CG from x0 = f (ConjugateGradientSolver.cc:35-65): iterations in
+// chunks, a device flag per flow; work for converged flows is skipped.
+static int solve_cg(ibpm_batch* B, int i, int* iters) {
+    hipStream_t s = B->ctx->c.stream;
+    const int nfl = (int)B->flows.size();
+    Probe* pr = B->probe;
+    CKR(reset_states(B));
+    for (auto& F : B->flows) CK(hipMemcpyAsync(F.cg_d, F.f, sizeof(double) * 2 * F.nb, hipMemcpyDeviceToDevice, s));
+    enqueue_M(B, i, B->d_smear_d[i], s);   // q = M x0
+    B->st_mapps += 1;
+    timed(pr, K_CG, 0, s, [&] { launch_cg(B->d_cg, nfl, 0, s); });
+    std::vector<double> st;
+    const int max_iter = 100000;
+    for (int it = 0; it < max_iter; it += B->cg_chunk) {
+        CKR(read_states(B, st));
+        bool all = true;
+        for (int k = 0; k < nfl; ++k)
+            if (st[8 * k + 3] == 0.0) all = false;
+        if (all) break;
+        for (int c = 0; c < B->cg_chunk; ++c) {
+            enqueue_M(B, i, B->d_smear_d[i], s);
+            timed(pr, K_CG, 0, s, [&] { launch_cg(B->d_cg, nfl, 1, s); });
+            B->st_mapps += 1;
+        }
+    }
+    CKR(read_states(B, st));
+    for (int k = 0; k < nfl; ++k) iters[k] += (int)st[8 * k + 2];
+    return 0;
+}
+
+// Preconditioned CG from x0 = f on the same residual test.  *fallback is set
+// when a flow broke down or hit the iteration limit; plain CG then finishes
+// the solve from where PCG stopped.
+static int solve_pcg(ibpm_batch* B, int i, int* iters, bool* fallback) {
+    hipStream_t s = B->ctx->c.stream;
+    const int nfl = (int)B->flows.size();
+    Probe* pr = B->probe;
+    const int par = B->pc_par[i];
+    PcgOps* ops = B->d_pcg[i][par];
+    CKR(reset_states(B));
+    for (auto& F : B->flows) CK(hipMemcpyAsync(F.cg_d, F.f, sizeof(double) * 2 * F.nb, hipMemcpyDeviceToDevice, s));
+    enqueue_M(B, i, B->d_smear_d[i], s);   // q = M x0
+    B->st_mapps += 1;
+    auto precondition = [&](int begin) {
+        timed(pr, K_MINV, 0, s, [&] { launch_minv(B->d_pminv[i], nfl, B->max_n, s); });
+        timed(pr, K_CG, 0, s, [&] { launch_pcg_dir(ops, nfl, begin, s); });
+    };
+    timed(pr, K_CG, 0, s, [&] { launch_pcg_begin(ops, nfl, s); });
+    precondition(1);
+    // the first poll comes after one iteration fewer than the last solve of
+    // this substep took (iterations past convergence are no-ops)
+    const int limit = std::max(100, 2 * B->max_n);
+    int chunk = std::max(0, B->pc_last[i] - 1), done = 0;
+    std::vector<double> st;
+    for (;;) {
+        for (int c = 0; c < chunk; ++c) {
+            enqueue_M(B, i, B->d_smear_d[i], s);
+            timed(pr, K_CG, 0, s, [&] { launch_pcg_step(ops, nfl, s); });
+            precondition(0);
+            B->st_mapps += 1;
+        }
+        done += chunk;
+        chunk = 2;
+        CKR(read_states(B, st));
+        bool all = true;
+        for (int k = 0; k < nfl; ++k)
+            if (st[8 * k + 3] == 0.0 && st[8 * k + 4] == 0.0) all = false;
+        if (all || done >= limit) break;
+    }
+    B->pc_par[i] = 1 - par;
+    *fallback = false;
+    for (int k = 0; k < nfl; ++k) {
+        iters[k] += (int)st[8 * k + 2];
+        if (st[8 * k + 3] == 0.0) *fallback = true;
+    }
+    return 0;
+}
+
 // One substep with CG (moving bodies): host-side kinematics and regulariser,
 // then CG iterations in chunks with a device-side convergence flag
 // (ConjugateGradientSolver.cc:35-65; iterations after convergence are no-ops).
@@ -674,30 +807,31 @@ static int substep_cg(ibpm_batch* B, int i) {
         launch_nn_features(B->d_nn, nfl, s);
         launch_nn_infer(B->d_nn, nfl, maxnb, s);
     }
-    for (auto& F : B->flows) CK(hipMemcpyAsync(F.cg_d, F.f, sizeof(double) * 2 * F.nb, hipMemcpyDeviceToDevice, s));
-    enqueue_M(B, i, B->d_smear_d[i], s);
-    for (auto& F : B->flows) {
-        const double st[4] = {0, F.p.cg_tol * F.p.cg_tol, 0, 0};
-        CK(hipMemcpyAsync(F.cg_state, st, sizeof(st), hipMemcpyHostToDevice, s));
-        CK(hipStreamSynchronize(s));
-    }
-    timed(pr, K_CG, 0, s, [&] { launch_cg(B->d_cg, nfl, 0, s); });
-    const int max_iter = 100000;
-    for (int it = 0; it < max_iter; it += B->cg_chunk) {
-        CK(hipStreamSynchronize(s));
-        bool all = true;
-        for (auto& F : B->flows) {
-            double st[4];
-            CK(hipMemcpy(st, F.cg_state, sizeof(st), hipMemcpyDeviceToHost));
-            F.cg_iters[i] = (int)st[2];
-            if (st[3] == 0.0) all = false;
+    std::vector<int> its(nfl, 0);
+    if (B->pc_mode && !B->pc_broken && !B->pc_valid) CKR(build_precond(B, i));
+    if (B->pc_mode && !B->pc_broken) {
+        bool fallback = false;
+        CKR(solve_pcg(B, i, its.data(), &fallback));
+        // rebuild P once the iterations it costs above its first solves
+        // exceed what a rebuild costs (one M application per column)
+        int itmax = 0;
+        for (int k = 0; k < nfl; ++k) itmax = std::max(itmax, its[k]);
+        B->pc_last[i] = itmax;
+        if (B->pc_base[i] < 0) B->pc_base[i] = itmax;
+        else B->pc_excess += std::max(0, itmax - B->pc_base[i]);
+        if (B->pc_excess > B->max_n) B->pc_valid = false;
+        if (fallback) {
+            B->st_fallbacks += 1;
+            CKR(solve_cg(B, i, its.data()));
         }
-        if (all) break;
-        for (int k = 0; k < B->cg_chunk; ++k) {
-            enqueue_M(B, i, B->d_smear_d[i], s);
-            timed(pr, K_CG, 0, s, [&] { launch_cg(B->d_cg, nfl, 1, s); });
-        }
+    } else {
+        CKR(solve_cg(B, i, its.data()));
     }
+    for (int k = 0; k < nfl; ++k) {
+        B->flows[k].cg_iters[i] = its[k];
+        B->st_iters += its[k];
+    }
+    B->st_solves += nfl;
     // training samples (features of this substep, target = converged f) and
     // the force/constraint history ring
     if (B->collect && nn_history_ready(B) && nn_samples_room(B)) {
@@ -754,70 +888,157 @@ static int cholesky_inverse(int n, std::vector<double>& Mt) {
     return -1;
 }
 
-// CholeskySolver::init (CholeskySolver.cc:49-109): M column by column on the
-// GPU, FP64 factorisation on the host, then M^-1 = (L L^T)^-1 assembled with
-// the reference's forward/back substitution (:200-224) applied to unit vectors.
-static int build_cholesky(ibpm_batch* B) {
+// M of substep i for the current body positions, column by column on the GPU
+// (CholeskySolver::init, CholeskySolver.cc:49-109), FP64 factorisation on the
+// host and M^-1 = (L L^T)^-1 assembled with the reference's forward/back
+// substitution (:200-224) applied to unit vectors.  The inverse goes to
+// F.minv[i] (Cholesky force solve) or, with precond, to F.pinv.  Returns -5
+// with precond when some M is not positive definite.
+static int build_inverse(ibpm_batch* B, int i, bool precond) {
     hipStream_t s = B->ctx->c.stream;
     const int nfl = (int)B->flows.size();
     int nmax = 0;
     for (auto& F : B->flows) nmax = std::max(nmax, 2 * F.nb);
     static const double one = 1.0, zero = 0.0;
-    for (int i = 0; i < B->nsub; ++i) {
-        for (int j = 0; j < nmax; ++j) {
-            std::vector<RegOps> sm(nfl), ip(nfl);
-            for (int k = 0; k < nfl; ++k) {
-                Flow& F = B->flows[k];
-                const int n = 2 * F.nb;
-                sm[k] = make_regops(F, beta_of(B, F, i));
-                sm[k].fin = F.unit; sm[k].cout = F.c;
-                ip[k] = make_regops(F, 0);
-                ip[k].psi = F.psi; ip[k].b = nullptr;
-                ip[k].out = F.mcol + (size_t)std::min(j, n) * n;   // column n is a dummy
-                if (j < n) {
-                    if (j > 0) CK(hipMemcpyAsync(F.unit + j - 1, &zero, sizeof(double), hipMemcpyHostToDevice, s));
-                    CK(hipMemcpyAsync(F.unit + j, &one, sizeof(double), hipMemcpyHostToDevice, s));
-                }
+    for (int j = 0; j < nmax; ++j) {
+        std::vector<RegOps> sm(nfl), ip(nfl);
+        for (int k = 0; k < nfl; ++k) {
+            Flow& F = B->flows[k];
+            const int n = 2 * F.nb;
+            sm[k] = make_regops(F, beta_of(B, F, i));
+            sm[k].fin = F.unit; sm[k].cout = F.c;
+            ip[k] = make_regops(F, 0);
+            ip[k].psi = F.psi; ip[k].b = nullptr;
+            ip[k].out = F.mcol + (size_t)std::min(j, n) * n;   // column n is a dummy
+            if (j < n) {
+                if (j > 0) CK(hipMemcpyAsync(F.unit + j - 1, &zero, sizeof(double), hipMemcpyHostToDevice, s));
+                CK(hipMemcpyAsync(F.unit + j, &one, sizeof(double), hipMemcpyHostToDevice, s));
             }
-            CK(hipStreamSynchronize(s));
-            CKR(upload(B->d_smear_d[i], sm.data(), sizeof(RegOps) * nfl));
-            CKR(upload(B->d_interp_m, ip.data(), sizeof(RegOps) * nfl));
-            enqueue_M(B, i, B->d_smear_d[i], s);
         }
         CK(hipStreamSynchronize(s));
-        // factorisation and inverse assembly on the host, one thread per flow
-        std::vector<std::vector<double>> minv_h(nfl);
-        std::vector<int> bad(nfl, -1);
-        for (int k = 0; k < nfl; ++k) {
-            Flow& F = B->flows[k];
-            const int n = 2 * F.nb;
-            minv_h[k].assign((size_t)n * n, 0.0);
-            CK(hipMemcpy(minv_h[k].data(), F.mcol, sizeof(double) * n * n, hipMemcpyDeviceToHost));
-            CK(hipMemset(F.unit, 0, sizeof(double) * (n + 1)));
+        CKR(upload(B->d_smear_d[i], sm.data(), sizeof(RegOps) * nfl));
+        CKR(upload(B->d_interp_m, ip.data(), sizeof(RegOps) * nfl));
+        enqueue_M(B, i, B->d_smear_d[i], s);
+    }
+    CK(hipStreamSynchronize(s));
+    // factorisation and inverse assembly on the host, threads over flows
+    std::vector<std::vector<double>> minv_h(nfl);
+    std::vector<int> bad(nfl, -1);
+    for (int k = 0; k < nfl; ++k) {
+        Flow& F = B->flows[k];
+        const int n = 2 * F.nb;
+        minv_h[k].assign((size_t)n * n, 0.0);
+        CK(hipMemcpy(minv_h[k].data(), F.mcol, sizeof(double) * n * n, hipMemcpyDeviceToHost));
+        CK(hipMemset(F.unit, 0, sizeof(double) * (n + 1)));
+    }
+    std::vector<std::thread> th;
+    const int hw = std::max(1u, std::thread::hardware_concurrency());
+    const int nthreads = std::max(1, std::min({nfl, hw, 16}));
+    for (int t = 0; t < nthreads; ++t)
+        th.emplace_back([&, t] {
+            for (int k = t; k < nfl; k += nthreads) bad[k] = cholesky_inverse(2 * B->flows[k].nb, minv_h[k]);
+        });
+    for (auto& x : th) x.join();
+    for (int k = 0; k < nfl; ++k) {
+        if (bad[k] >= 0) {
+            set_error("Cholesky: flow %d, matrix not positive definite at %d", k, bad[k]);
+            CKR(upload_regops(B));
+            return precond ? -5 : -4;
         }
-        std::vector<std::thread> th;
-        const int nthreads = std::max(1, std::min(nfl, (int)std::thread::hardware_concurrency()));
-        for (int t = 0; t < nthreads; ++t)
-            th.emplace_back([&, t] {
-                for (int k = t; k < nfl; k += nthreads)
-                    bad[k] = cholesky_inverse(2 * B->flows[k].nb, minv_h[k]);
-            });
-        for (auto& x : th) x.join();
-        for (int k = 0; k < nfl; ++k) {
-            if (bad[k] >= 0) { set_error("Cholesky: flow %d, matrix not positive definite at %d", k, bad[k]); return -4; }
-            Flow& F = B->flows[k];
-            const int n = 2 * F.nb;
-            if (!F.minv[i]) CKR(B->mem.alloc(&F.minv[i], (size_t)n * n));
-            CKR(upload(F.minv[i], minv_h[k].data(), sizeof(double) * n * n));
-        }
+        Flow& F = B->flows[k];
+        const int n = 2 * F.nb;
+        double*& dst = precond ? F.pinv : F.minv[i];
+        if (!dst) CKR(B->mem.alloc(&dst, (size_t)n * n));
+        CKR(upload(dst, minv_h[k].data(), sizeof(double) * n * n));
+        if (precond) F.pbeta = beta_of(B, F, i);
+    }
+    if (!precond) {
         std::vector<MinvOps> mo(nfl);
         for (int k = 0; k < nfl; ++k) {
             Flow& F = B->flows[k];
-            mo[k] = MinvOps{2 * F.nb, F.minv[i], F.rhs, F.f};
+            mo[k] = MinvOps{2 * F.nb, F.minv[i], F.rhs, F.f, 1.0, nullptr};
         }
         CKR(upload(B->d_minv[i], mo.data(), sizeof(MinvOps) * nfl));
     }
     return upload_regops(B);
+}
+
+static int build_cholesky(ibpm_batch* B) {
+    for (int i = 0; i < B->nsub; ++i) CKR(build_inverse(B, i, false));
+    return 0;
+}
+
+// The preconditioner for every substep from M of substep i at the current
+// body positions: P_j = (beta_i / beta_j) M_i^-1, since M scales with the
+// substep's beta.  An M that is not positive definite ends preconditioning
+// for the batch (plain CG from then on).
+static int build_precond(ibpm_batch* B, int i) {
+    const int nfl = (int)B->flows.size();
+    CKR(reset_states(B));   // the M applications below must not be skipped
+    const int rc = build_inverse(B, i, true);
+    B->st_builds += 1;
+    B->st_build_mapps += B->max_n;
+    if (rc == -5) {
+        B->pc_broken = true;
+        return 0;
+    }
+    CKR(rc);
+    for (int j = 0; j < B->nsub; ++j) {
+        std::vector<MinvOps> mo(nfl);
+        for (int k = 0; k < nfl; ++k) {
+            Flow& F = B->flows[k];
+            mo[k] = MinvOps{2 * F.nb, F.pinv, F.cg_t, F.cg_u, F.pbeta / beta_of(B, F, j), F.cg_state + 3};
+        }
+        CKR(upload(B->d_pminv[j], mo.data(), sizeof(MinvOps) * nfl));
+    }
+    B->pc_valid = true;
+    B->pc_excess = 0.0;
+    for (int j = 0; j < 4; ++j) B->pc_base[j] = -1;
+    return 0;
+}
+
+// Buffers and descriptors of preconditioned CG (allocated on first use).
+static int setup_pcg(ibpm_batch* B) {
+    const int nfl = (int)B->flows.size();
+    const int K = B->pc_pairs;
+    for (auto& F : B->flows) {
+        const int n = 2 * F.nb;
+        if (!F.cg_t) {
+            CKR(B->mem.alloc(&F.cg_t, n + 1)); CKR(B->mem.alloc(&F.cg_u, n + 1));
+            CKR(B->mem.alloc(&F.lmp_g, kLmpMax * kLmpMax)); CKR(B->mem.alloc(&F.lmp_c, kLmpMax));
+            CKR(B->mem.alloc(&F.lmp_cnt, 8));
+        }
+        CK(hipMemset(F.lmp_cnt, 0, sizeof(int) * 8));
+        for (int i = 0; i < B->nsub; ++i)
+            for (int p = 0; p < 2; ++p)
+                if (K > 0 && !F.lmp_s[i][p]) {
+                    CKR(B->mem.alloc(&F.lmp_s[i][p], (size_t)kLmpMax * n));
+                    CKR(B->mem.alloc(&F.lmp_y[i][p], (size_t)kLmpMax * n));
+                }
+    }
+    for (int i = 0; i < B->nsub; ++i) {
+        if (!B->d_pminv[i]) CKR(B->mem.alloc(&B->d_pminv[i], nfl));
+        for (int p = 0; p < 2; ++p) {
+            std::vector<PcgOps> ops(nfl);
+            for (int k = 0; k < nfl; ++k) {
+                Flow& F = B->flows[k];
+                PcgOps& o = ops[k];
+                o.n = 2 * F.nb; o.kmax = K;
+                o.x = F.f; o.r = F.cg_r; o.d = F.cg_d; o.q = F.cg_q; o.t = F.cg_t; o.u = F.cg_u;
+                o.b = F.rhs; o.state = F.cg_state;
+                o.rs = F.lmp_s[i][p]; o.ry = F.lmp_y[i][p]; o.rcnt = F.lmp_cnt + 2 * i + p;
+                o.ws = F.lmp_s[i][1 - p]; o.wy = F.lmp_y[i][1 - p]; o.wcnt = F.lmp_cnt + 2 * i + 1 - p;
+                o.g = F.lmp_g; o.c = F.lmp_c;
+            }
+            if (!B->d_pcg[i][p]) CKR(B->mem.alloc(&B->d_pcg[i][p], nfl));
+            CKR(upload(B->d_pcg[i][p], ops.data(), sizeof(PcgOps) * nfl));
+        }
+        B->pc_par[i] = 0;
+        B->pc_last[i] = 0;
+    }
+    B->pc_valid = false;
+    B->pc_broken = false;
+    return 0;
 }
 
 }  // namespace ibpmg
@@ -1325,6 +1546,26 @@ int ibpm_batch_set_warm_start(ibpm_batch* B, int mode) {
     if (mode == 2 && !B->nn_ready) { set_error("learned warm start requested before training"); return -1; }
     B->warm_mode = mode;
     return 0;
+}
+
+int ibpm_batch_set_precond(ibpm_batch* B, int mode, int pairs) {
+    if (!B) { set_error("null batch"); return -1; }
+    if (mode < 0 || mode > 1) { set_error("preconditioner mode must be 0 or 1"); return -1; }
+    if (pairs < 0 || pairs > kLmpMax) { set_error("pairs must be in 0..%d", kLmpMax); return -1; }
+    if (mode == 1 && !B->any_cg) { set_error("preconditioning needs a CG (moving-body) batch"); return -1; }
+    B->pc_mode = mode;
+    B->pc_pairs = mode ? pairs : 0;
+    if (mode) CKR(setup_pcg(B));
+    return 0;
+}
+
+int ibpm_batch_solver_stats(ibpm_batch* B, long* stats, int nstats) {
+    if (!B || !stats) { set_error("null argument"); return -1; }
+    const long v[7] = {B->st_iters, B->st_solves, B->st_mapps, B->st_build_mapps, B->st_builds,
+                       B->st_fallbacks, B->pc_broken ? 1L : 0L};
+    const int n = std::min(nstats, 7);
+    for (int k = 0; k < n; ++k) stats[k] = v[k];
+    return n;
 }
 
 int ibpm_batch_collect(ibpm_batch* B, int enable, long capacity) {

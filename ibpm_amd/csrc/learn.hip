@@ -2,17 +2,20 @@
 // boundary point, trained and evaluated on the GPU (FP64).
 //
 // For substep k of a moving-body run the CG unknown is the body force f_k
-// (ConjugateGradientSolver.cc:35-65 warm-starts from f_{k-1},
-// ProjectionSolver.cc:72).  The network predicts the increment over that warm
-// start from the force history of the same point:
-//   s      = rms(f_{k-1}) over the flow's points (scale), sb = rms(b_k - b_{k-1})
-//   x(p)   = [(f_{k-S}(p)-f_{k-1}(p))/s, (f_{k-S-1}(p)-f_{k-1}(p))/s,
-//             (f_{k-2S}(p)-f_{k-1}(p))/s, (f_{k-2S-1}(p)-f_{k-1}(p))/s,
-//             (b_k(p)-b_{k-1}(p))/sb, one-hot(substep)]        (S substeps/step)
+// (ConjugateGradientSolver.cc:35-65).  The reference warm-starts from the
+// previous substep's force f_{k-1} (ProjectionSolver.cc:72), but the RK3
+// substeps use different beta and M = C A^-1 beta B scales with beta, so that
+// start is no better than zero.  The network is anchored on the same substep
+// of the previous step, f_{k-S} (same beta), and predicts the one-step
+// increment from the one-step increments of the history:
+//   s      = rms(f_{k-S}) over the flow's points, sb = rms(b_k - b_{k-1})
+//   x(p)   = [(f_{k-1}-f_{k-S-1})/s, (f_{k-S}-f_{k-2S})/s, (f_{k-S-1}-f_{k-2S-1})/s,
+//             (f_{k-1}-f_{k-S})/s, (b_k-b_{k-1})/sb, one-hot(substep)]   at point p
 //   y(p)   = W2 tanh(W1 x + b1) + b2          (13 -> 32 -> 2)
-//   f0(p)  = f_{k-1}(p) + s y(p)              (CG start; CG then runs to the
+//   f0(p)  = f_{k-S}(p) + s y(p)              (CG start; CG then runs to the
 //                                              same absolute tolerance)
-// Components of point p are (p, nb + p) as in BoundaryVector.h:65-69.
+// Components of point p are (p, nb + p) as in BoundaryVector.h:65-69.  With
+// W2 = 0 (the initial parameters) the start is f_{k-S}.
 #include "learn.h"
 
 namespace ibpmg {
@@ -41,7 +44,7 @@ __global__ void __launch_bounds__(kThreads) k_nn_features(const NnOps* __restric
     const NnOps& O = ops[blockIdx.x];
     __shared__ double red[kThreads / 64 + 1];
     const int nb = O.nb, n = 2 * nb;
-    const double s = rms_of(O.fa, n, red);
+    const double s = rms_of(O.fb, n, red);
     double sb = 0;
     {
         double t = 0;
@@ -65,11 +68,12 @@ __global__ void __launch_bounds__(kThreads) k_nn_features(const NnOps* __restric
     for (int p = threadIdx.x; p < nb; p += kThreads) {
         double* x = O.X + (long)p * kNnIn;
         const int cx = p, cy = nb + p;
-        const double* hs[4] = {O.fb, O.fc, O.fd, O.fe};
+        const double* hp[4] = {O.fa, O.fb, O.fc, O.fa};
+        const double* hm[4] = {O.fc, O.fd, O.fe, O.fb};
 #pragma unroll
         for (int h = 0; h < 4; ++h) {
-            x[2 * h] = (hs[h][cx] - O.fa[cx]) / s;
-            x[2 * h + 1] = (hs[h][cy] - O.fa[cy]) / s;
+            x[2 * h] = (hp[h][cx] - hm[h][cx]) / s;
+            x[2 * h + 1] = (hp[h][cy] - hm[h][cy]) / s;
         }
         x[8] = (O.b_now[cx] - O.b_prev[cx]) / sb;
         x[9] = (O.b_now[cy] - O.b_prev[cy]) / sb;
@@ -80,8 +84,8 @@ __global__ void __launch_bounds__(kThreads) k_nn_features(const NnOps* __restric
         if (O.target && O.data_x) {
             const long row = O.data_off + p;
             for (int j = 0; j < kNnIn; ++j) O.data_x[row * kNnIn + j] = x[j];
-            O.data_y[row * 2] = (O.target[cx] - O.fa[cx]) / s;
-            O.data_y[row * 2 + 1] = (O.target[cy] - O.fa[cy]) / s;
+            O.data_y[row * 2] = (O.target[cx] - O.fb[cx]) / s;
+            O.data_y[row * 2 + 1] = (O.target[cy] - O.fb[cy]) / s;
         }
     }
 }
@@ -110,7 +114,7 @@ __device__ inline void nn_forward(const double* __restrict__ P, const double* x,
     }
 }
 
-// f0 = f_{k-1} + s * net(x) written into the CG unknown.
+// f0 = f_{k-S} + s * net(x) written into the CG unknown.
 __global__ void __launch_bounds__(kThreads) k_nn_infer(const NnOps* __restrict__ ops) {
     const NnOps& O = ops[blockIdx.y];
     const int p = blockIdx.x * kThreads + threadIdx.x;
@@ -118,8 +122,8 @@ __global__ void __launch_bounds__(kThreads) k_nn_infer(const NnOps* __restrict__
     double h[kNnHid], y[2];
     nn_forward(O.params, O.X + (long)p * kNnIn, h, y);
     const double s = O.scale[0];
-    O.x0[p] = O.fa[p] + s * y[0];
-    O.x0[O.nb + p] = O.fa[O.nb + p] + s * y[1];
+    O.x0[p] = O.fb[p] + s * y[0];
+    O.x0[O.nb + p] = O.fb[O.nb + p] + s * y[1];
 }
 
 // Per-sample forward and backward of the mean squared error: stores the

@@ -37,13 +37,16 @@ struct RegOps {
     double* cout;                  // scalar field written at the nodes
     int nold;
     const int* old_idx;            // nodes of the previous association to clear
+    const double* skip;            // null, or the flow's force-solve flags
 };
 
 struct MinvOps {
     int n;
     const double* minv;            // n x n row-major
     const double* rhs;
-    double* x;
+    double* x;                     // x = scale * minv * rhs
+    double scale;
+    const double* skip;            // null, or flags: skip[0] or skip[1] nonzero -> no-op
 };
 
 struct CgOps {
@@ -51,6 +54,27 @@ struct CgOps {
     double *x, *r, *d, *q;
     const double* b;
     double* state;                 // delta, tol^2, iterations, converged
+};
+
+// Preconditioned CG on M f = rhs.  The preconditioner is
+//   H = (I - S G Y^T) P (I - Y G S^T) + S G S^T,   G = (sym(S^T Y))^-1,
+// with P = scale * (dense FP64 inverse of M at an earlier configuration) and
+// (S, Y = M S) the first search directions of the previous solve of the same
+// substep (a limited-memory low-rank update; no pairs -> H = P).  The
+// residual test is the unpreconditioned r.r <= tol^2 of plain CG.
+constexpr int kLmpMax = 8;
+struct PcgOps {
+    int n, kmax;                   // unknowns; pairs kept for the next solve
+    double *x, *r, *d, *q;         // iterate, residual, direction, M d
+    double *t, *u;                 // input and output of the dense P
+    const double* b;
+    double* state;                 // rho, tol^2, iterations, converged, breakdown, nk, delta
+    const double *rs, *ry;         // pairs of the previous solve (read), count *rcnt
+    const int* rcnt;
+    double *ws, *wy;               // pairs of this solve (written), count *wcnt
+    int* wcnt;
+    double* g;                     // kLmpMax x kLmpMax inverse Gram matrix of the read pairs
+    double* c;                     // kLmpMax: G S^T r of the current residual
 };
 
 struct ForceOps {
@@ -73,6 +97,9 @@ void launch_smear_curl(const RegOps* d, int nflows, int max_nodes, hipStream_t s
 void launch_clear_nodes(const RegOps* d, int nflows, int max_nodes, hipStream_t s);
 void launch_minv(const MinvOps* d, int nflows, int max_n, hipStream_t s);
 void launch_cg(const CgOps* d, int nflows, int mode, hipStream_t s);
+void launch_pcg_begin(const PcgOps* d, int nflows, hipStream_t s);
+void launch_pcg_step(const PcgOps* d, int nflows, hipStream_t s);
+void launch_pcg_dir(const PcgOps* d, int nflows, int begin, hipStream_t s);
 void launch_forces(const ForceOps* d, int nflows, hipStream_t s);
 
 }  // namespace ibpmg

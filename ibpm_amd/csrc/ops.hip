@@ -287,7 +287,7 @@ __device__ inline double curl_flat(const Lay& g, const double* psi, int fidx) {
 __global__ void __launch_bounds__(kThreads) k_interp(const RegOps* __restrict__ regs) {
     const RegOps& R = regs[blockIdx.y];
     const int k = blockIdx.x * kThreads + threadIdx.x;
-    if (k >= 2 * R.nb) return;
+    if (k >= 2 * R.nb || skipped(R.skip)) return;
     const Lay g{R.nx, R.ny, R.L};
     double s = 0.0;
     for (int a = R.e_ptr[k]; a < R.e_ptr[k + 1]; ++a) s += R.e_w[a] * curl_flat(g, R.psi, R.e_f[a]);
@@ -308,7 +308,7 @@ __device__ inline double smear_at(const RegOps& R, int fl) {
 __global__ void __launch_bounds__(kThreads) k_smear_curl(const RegOps* __restrict__ regs) {
     const RegOps& R = regs[blockIdx.y];
     const int m = blockIdx.x * kThreads + threadIdx.x;
-    if (m >= R.nnodes) return;
+    if (m >= R.nnodes || skipped(R.skip)) return;
     const int4 nb = R.node_nbr[m];   // flux-list slots of Y(i,j), Y(i-1,j), X(i,j-1), X(i,j)
     const double w = (smear_at(R, nb.x) - smear_at(R, nb.y) + smear_at(R, nb.z) - smear_at(R, nb.w)) * R.bydx2;
     R.cout[R.node_idx[m]] = w * R.beta;
@@ -329,13 +329,13 @@ __global__ void __launch_bounds__(kThreads) k_minv(const MinvOps* __restrict__ m
     const MinvOps& P = mops[blockIdx.y];
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     const int row = blockIdx.x * (kThreads / 64) + wave;
-    if (row >= P.n) return;
+    if (row >= P.n || skipped(P.skip)) return;
     const double* a = P.minv + (long)row * P.n;
     double s = 0.0;
     for (int k = lane; k < P.n; k += 64) s += a[k] * P.rhs[k];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-    if (lane == 0) P.x[row] = s;
+    if (lane == 0) P.x[row] = P.scale * s;
 }
 
 // ------------------------------------------------------------------- CG
@@ -405,6 +405,253 @@ __global__ void __launch_bounds__(kThreads) k_cg(const CgOps* __restrict__ cgs, 
     }
 }
 
+// ------------------------------------------------------------------ PCG
+// Sums K per-thread values over the workgroup; every thread gets the totals.
+// The order of the additions is fixed (lane tree, then waves in order).
+template <int K>
+__device__ inline void block_sums(double (&v)[K], double* red) {   // red: (kThreads/64 + 1) K
+#pragma unroll
+    for (int a = 0; a < K; ++a)
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v[a] += __shfl_down(v[a], off, 64);
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    constexpr int W = kThreads / 64;
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+        for (int a = 0; a < K; ++a) red[wave * K + a] = v[a];
+    }
+    __syncthreads();
+    if (threadIdx.x < K) {
+        double s = 0.0;
+        for (int w = 0; w < W; ++w) s += red[w * K + threadIdx.x];
+        red[W * K + threadIdx.x] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < K; ++a) v[a] = red[W * K + a];
+}
+
+constexpr int kRed = (kThreads / 64 + 1) * kLmpMax;
+
+// First half of the preconditioner for the current residual:
+//   c = G S^T r,  t = r - Y c   (t = r without pairs).
+__device__ inline void pcg_project(const PcgOps& C, int nk, const double* G, double* red) {
+    const int n = C.n;
+    if (nk == 0) {
+        for (int k = threadIdx.x; k < n; k += kThreads) C.t[k] = C.r[k];
+        return;
+    }
+    double v[kLmpMax];
+#pragma unroll
+    for (int a = 0; a < kLmpMax; ++a) v[a] = 0.0;
+    for (int k = threadIdx.x; k < n; k += kThreads) {
+        const double r = C.r[k];
+#pragma unroll
+        for (int a = 0; a < kLmpMax; ++a)
+            if (a < nk) v[a] += C.rs[(long)a * n + k] * r;
+    }
+    block_sums<kLmpMax>(v, red);
+    double c[kLmpMax];
+#pragma unroll
+    for (int a = 0; a < kLmpMax; ++a) {
+        double s = 0.0;
+#pragma unroll
+        for (int b = 0; b < kLmpMax; ++b)
+            if (a < nk && b < nk) s += G[a * kLmpMax + b] * v[b];
+        c[a] = s;
+    }
+    if (threadIdx.x < kLmpMax) {
+#pragma unroll
+        for (int a = 0; a < kLmpMax; ++a)
+            if (a == (int)threadIdx.x) C.c[a] = c[a];
+    }
+    for (int k = threadIdx.x; k < n; k += kThreads) {
+        double s = C.r[k];
+#pragma unroll
+        for (int a = 0; a < kLmpMax; ++a)
+            if (a < nk) s -= C.ry[(long)a * n + k] * c[a];
+        C.t[k] = s;
+    }
+}
+
+// Start of a solve: r = b - q (q = M x0), the residual test, and the inverse
+// Gram matrix G of the previous solve's pairs (symmetrised S^T Y, Cholesky in
+// FP64; the pairs are cut to the leading block that is safely positive
+// definite).  Ends with the projection for the first preconditioner apply.
+__global__ void __launch_bounds__(kThreads) k_pcg_begin(const PcgOps* __restrict__ ops) {
+    const PcgOps& C = ops[blockIdx.x];
+    __shared__ double red[kRed];
+    __shared__ double W[kLmpMax * kLmpMax], G[kLmpMax * kLmpMax];
+    __shared__ int nk_s;
+    double* st = C.state;
+    const int n = C.n;
+    double v1[1] = {0.0};
+    for (int k = threadIdx.x; k < n; k += kThreads) {
+        const double r = C.b[k] - C.q[k];
+        C.r[k] = r;
+        v1[0] += r * r;
+    }
+    block_sums<1>(v1, red);
+    const double delta = v1[0];
+    int nk = min(*C.rcnt, C.kmax);
+    for (int a = 0; a < nk; ++a) {
+        double v[kLmpMax];
+#pragma unroll
+        for (int b = 0; b < kLmpMax; ++b) v[b] = 0.0;
+        for (int k = threadIdx.x; k < n; k += kThreads) {
+            const double sa = C.rs[(long)a * n + k], ya = C.ry[(long)a * n + k];
+#pragma unroll
+            for (int b = 0; b < kLmpMax; ++b)
+                if (b < nk) v[b] += sa * C.ry[(long)b * n + k] + C.rs[(long)b * n + k] * ya;
+        }
+        block_sums<kLmpMax>(v, red);
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int b = 0; b < kLmpMax; ++b) W[a * kLmpMax + b] = 0.5 * v[b];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        // W = L L^T (L in the lower triangle of W), stopping at the first
+        // pivot that is not safely positive
+        int m = 0;
+        for (; m < nk; ++m) {
+            double s = W[m * kLmpMax + m];
+            for (int k = 0; k < m; ++k) s -= W[m * kLmpMax + k] * W[m * kLmpMax + k];
+            if (!(s > 1e-10 * W[m * kLmpMax + m])) break;
+            const double dm = sqrt(s);
+            W[m * kLmpMax + m] = dm;
+            for (int i = m + 1; i < nk; ++i) {
+                double t = W[i * kLmpMax + m];
+                for (int k = 0; k < m; ++k) t -= W[i * kLmpMax + k] * W[m * kLmpMax + k];
+                W[i * kLmpMax + m] = t / dm;
+            }
+        }
+        nk = m;
+        // G = (L L^T)^-1, column by column
+        for (int col = 0; col < nk; ++col) {
+            double x[kLmpMax];
+            for (int i = 0; i < nk; ++i) {
+                double t = (i == col) ? 1.0 : 0.0;
+                for (int k = 0; k < i; ++k) t -= W[i * kLmpMax + k] * x[k];
+                x[i] = t / W[i * kLmpMax + i];
+            }
+            for (int i = nk - 1; i >= 0; --i) {
+                double t = x[i];
+                for (int k = i + 1; k < nk; ++k) t -= W[k * kLmpMax + i] * x[k];
+                x[i] = t / W[i * kLmpMax + i];
+            }
+            for (int i = 0; i < nk; ++i) G[i * kLmpMax + col] = x[i];
+        }
+        for (int q = 0; q < kLmpMax * kLmpMax; ++q) C.g[q] = (q / kLmpMax < nk && q % kLmpMax < nk) ? G[q] : 0.0;
+        nk_s = nk;
+        st[2] = 0.0;
+        st[3] = (delta > st[1]) ? 0.0 : 1.0;
+        st[4] = 0.0;
+        st[5] = nk;
+        st[6] = delta;
+        *C.wcnt = 0;
+    }
+    __syncthreads();
+    nk = nk_s;
+    pcg_project(C, nk, G, red);
+}
+
+// One iteration after q = M d:  alpha = rho/(d.q), x += alpha d,
+// r -= alpha q, residual test on r.r; keeps (d, q) as a pair for the next
+// solve while fewer than kmax are kept; then the projection of the new r.
+__global__ void __launch_bounds__(kThreads) k_pcg_step(const PcgOps* __restrict__ ops) {
+    const PcgOps& C = ops[blockIdx.x];
+    __shared__ double red[kRed];
+    double* st = C.state;
+    if (st[3] != 0.0 || st[4] != 0.0) return;
+    const int n = C.n, nk = (int)st[5], j = (int)st[2];
+    const double rho = st[0];
+    double v[1] = {0.0};
+    for (int k = threadIdx.x; k < n; k += kThreads) v[0] += C.d[k] * C.q[k];
+    block_sums<1>(v, red);
+    const double dq = v[0];
+    if (!(dq > 0.0)) {   // M not positive on d: plain CG takes over
+        if (threadIdx.x == 0) st[4] = 1.0;
+        return;
+    }
+    const double alpha = rho / dq;
+    const bool keep = j < C.kmax;
+    v[0] = 0.0;
+    for (int k = threadIdx.x; k < n; k += kThreads) {
+        const double dk = C.d[k], qk = C.q[k];
+        if (keep) {
+            C.ws[(long)j * n + k] = dk;
+            C.wy[(long)j * n + k] = qk;
+        }
+        C.x[k] += alpha * dk;
+        const double r = C.r[k] - alpha * qk;
+        C.r[k] = r;
+        v[0] += r * r;
+    }
+    block_sums<1>(v, red);
+    if (threadIdx.x == 0) {
+        st[2] = j + 1;
+        st[3] = (v[0] > st[1]) ? 0.0 : 1.0;
+        st[6] = v[0];
+        if (keep) *C.wcnt = j + 1;
+    }
+    pcg_project(C, nk, C.g, red);
+}
+
+// After u = P t:  z = u + S (c - G Y^T u)  (stored over u), rho' = r.z,
+// d = z (begin) or d = z + (rho'/rho) d.
+__global__ void __launch_bounds__(kThreads) k_pcg_dir(const PcgOps* __restrict__ ops, int begin) {
+    const PcgOps& C = ops[blockIdx.x];
+    __shared__ double red[kRed];
+    double* st = C.state;
+    if (st[3] != 0.0 || st[4] != 0.0) return;
+    const int n = C.n, nk = (int)st[5];
+    const double rho = st[0];
+    double w[kLmpMax];
+#pragma unroll
+    for (int a = 0; a < kLmpMax; ++a) w[a] = 0.0;
+    if (nk > 0) {
+        double v[kLmpMax];
+#pragma unroll
+        for (int a = 0; a < kLmpMax; ++a) v[a] = 0.0;
+        for (int k = threadIdx.x; k < n; k += kThreads) {
+            const double u = C.u[k];
+#pragma unroll
+            for (int a = 0; a < kLmpMax; ++a)
+                if (a < nk) v[a] += C.ry[(long)a * n + k] * u;
+        }
+        block_sums<kLmpMax>(v, red);
+#pragma unroll
+        for (int a = 0; a < kLmpMax; ++a) {
+            double s = 0.0;
+#pragma unroll
+            for (int b = 0; b < kLmpMax; ++b)
+                if (a < nk && b < nk) s += C.g[a * kLmpMax + b] * v[b];
+            w[a] = (a < nk) ? C.c[a] - s : 0.0;
+        }
+    }
+    double v1[1] = {0.0};
+    for (int k = threadIdx.x; k < n; k += kThreads) {
+        double z = C.u[k];
+#pragma unroll
+        for (int a = 0; a < kLmpMax; ++a)
+            if (a < nk) z += C.rs[(long)a * n + k] * w[a];
+        C.u[k] = z;
+        v1[0] += C.r[k] * z;
+    }
+    block_sums<1>(v1, red);
+    const double rnew = v1[0];
+    if (!(rnew > 0.0)) {   // preconditioner not positive on r: plain CG takes over
+        if (threadIdx.x == 0) st[4] = 1.0;
+        return;
+    }
+    const double beta = begin ? 0.0 : rnew / rho;
+    for (int k = threadIdx.x; k < n; k += kThreads) C.d[k] = begin ? C.u[k] : C.u[k] + beta * C.d[k];
+    if (threadIdx.x == 0) st[0] = rnew;
+}
+
 // Net force sum over the boundary points (State.cc:53-63; the dx^2 factor and
 // the rotation to Cd/Cl are applied on the host).  One wave per flow: lane l
 // sums points l, l+64, ... in order, then a fixed shuffle tree combines the
@@ -460,6 +707,16 @@ void launch_minv(const MinvOps* d, int nflows, int max_n, hipStream_t s) {
 }
 void launch_cg(const CgOps* d, int nflows, int mode, hipStream_t s) {
     hipLaunchKernelGGL(k_cg, dim3(nflows), dim3(kThreads), 0, s, d, mode);
+}
+
+void launch_pcg_begin(const PcgOps* d, int nflows, hipStream_t s) {
+    hipLaunchKernelGGL(k_pcg_begin, dim3(nflows), dim3(kThreads), 0, s, d);
+}
+void launch_pcg_step(const PcgOps* d, int nflows, hipStream_t s) {
+    hipLaunchKernelGGL(k_pcg_step, dim3(nflows), dim3(kThreads), 0, s, d);
+}
+void launch_pcg_dir(const PcgOps* d, int nflows, int begin, hipStream_t s) {
+    hipLaunchKernelGGL(k_pcg_dir, dim3(nflows), dim3(kThreads), 0, s, d, begin);
 }
 
 void launch_forces(const ForceOps* d, int nflows, hipStream_t s) {

@@ -139,6 +139,7 @@ public:
     double getAlpha() const { return 1. / _Re; }
     bool geTimeDependent() const { return _geom.text().find("pitchplunge") != std::string::npos ||
                                           _geom.text().find("PitchPlunge") != std::string::npos; }
+    bool isTimeDependent() const { return geTimeDependent(); }   // the base flow is steady here
     const Grid& getGrid() const { return _grid; }
     const Geometry& geometry() const { return _geom; }
     const BaseFlow& getBaseFlow() const { return _q0; }
@@ -172,6 +173,10 @@ public:
     }
     double getTimestep() const { return _dt; }
     void setTol(double tol) { _tol = tol; }   // before init(); CG absolute residual tolerance
+    // Force solve of moving bodies (no counterpart in the reference): mode 0
+    // plain CG, 1 CG preconditioned by a stale FP64 inverse of M with `pairs`
+    // low-rank pairs (ibpm_batch_set_precond).  Before init().
+    void setPrecond(int mode, int pairs) { _pc_mode = mode; _pc_pairs = pairs; }
     bool load(const std::string&) { return false; }   // the factor is rebuilt on the GPU at init()
     bool save(const std::string&) { return true; }
     void init() {
@@ -187,6 +192,7 @@ public:
         gpu_check(ibpm_batch_create(_ctx, 1, &p, &_batch));
         gpu_check(ibpm_batch_init(_batch));
         if (!_model.geTimeDependent()) gpu_check(ibpm_batch_use_graph(_batch, 1));
+        else if (_pc_mode) gpu_check(ibpm_batch_set_precond(_batch, _pc_mode, _pc_pairs));
     }
     // IBSolver::advance, reference src/IBSolver.cc:142-150
     void advance(State& x) {
@@ -227,6 +233,7 @@ private:
     Scheme::SchemeType _scheme;
     double _tol;
     int _device;
+    int _pc_mode = 0, _pc_pairs = 0;
     ibpm_ctx* _ctx = nullptr;
     ibpm_batch* _batch = nullptr;
 };

@@ -6,8 +6,10 @@
 // <name>.force17 with full precision.  -restart n writes the reference's
 // binary restart file <name>%05d.bin every n steps and -ic loads one
 // (src/State.cc:65-223, src/ibpm.cc:250-274).  Added flag: -tol (CG tolerance,
-// which the reference sets only from code, src/ibpm.cc:292-294).  Tecplot and
-// energy outputs are not produced by this driver.
+// which the reference sets only from code, src/ibpm.cc:292-294) and -precond
+// (force solve of moving bodies: none = the reference's plain CG, stale, or
+// lowrank; all end on the same residual test).  Tecplot and energy outputs
+// are not produced by this driver.
 #include <sys/stat.h>
 
 #include <cmath>
@@ -80,13 +82,15 @@ int main(int argc, char** argv) {
     const std::string scheme = P.str("scheme", "rk3");
     const double tol = P.f("tol", 1e-7);
     const int device = P.i("device", 0);
+    const std::string precond = P.str("precond", "none");
     std::map<std::string, Scheme::SchemeType> schemes = {
         {"euler", Scheme::EULER}, {"ab2", Scheme::AB2}, {"rk3", Scheme::RK3}, {"rk3b", Scheme::RK3b}};
-    if (!P.valid() || P.help || !schemes.count(scheme) || model != "nonlinear") {
+    std::map<std::string, std::pair<int, int>> preconds = {{"none", {0, 0}}, {"stale", {1, 0}}, {"lowrank", {1, 8}}};
+    if (!P.valid() || P.help || !schemes.count(scheme) || model != "nonlinear" || !preconds.count(precond)) {
         std::cerr << "usage: ibpm [-name ibpm] [-outdir .] [-force 1] [-nx 200] [-ny 200] [-ngrid 1]\n"
                      "            [-length 4] [-xoffset -2] [-yoffset -2] [-alpha 0] [-geom name.geom]\n"
                      "            [-Re 100] [-dt 0.02] [-nsteps 250] [-scheme euler|ab2|rk3|rk3b]\n"
-                     "            [-model nonlinear] [-tol 1e-7] [-device 0]\n";
+                     "            [-model nonlinear] [-tol 1e-7] [-device 0] [-precond none|stale|lowrank]\n";
         return 1;
     }
     if (outdir.empty() || outdir.back() != '/') outdir += "/";
@@ -104,6 +108,7 @@ int main(int argc, char** argv) {
     BaseFlow q_potential(grid, 1.0, alpha_deg * pi / 180.);
     NavierStokesModel nsm(grid, geom, Re, q_potential);
     NonlinearIBSolver solver(grid, nsm, dt, schemes[scheme], tol, device);
+    solver.setPrecond(preconds[precond].first, preconds[precond].second);
     State x(grid, geom.getNumPoints());
     nsm.init();
     try {

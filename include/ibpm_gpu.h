@@ -124,9 +124,11 @@ int ibpm_batch_set_state(ibpm_batch* b, int flow, const double* h_omega, const d
 
 /* ---- learned CG warm start (moving bodies) ---------------------------------
    No counterpart in the reference: a per-boundary-point network (13 -> 32 tanh
-   -> 2, FP64) predicts the force increment over the reference warm start
-   f_{k-1} (src/ProjectionSolver.cc:72) from the force history; CG then runs to
-   the same absolute residual tolerance, so only iteration counts change.
+   -> 2, FP64) predicts the CG start from the force history, anchored on the
+   force of the same RK3 substep one step earlier (the reference starts from
+   the previous substep, src/ProjectionSolver.cc:72, whose beta differs); CG
+   then runs to the same absolute residual tolerance, so only iteration
+   counts change.
    mode: 0 zero start, 1 previous force (reference), 2 learned. */
 int ibpm_batch_set_warm_start(ibpm_batch* b, int mode);
 /* Record (features, converged increment) samples of every substep during
@@ -142,6 +144,28 @@ int ibpm_batch_nn_params(ibpm_batch* b, double* h_get, const double* h_set);
    Y: n x 2), computed by the training kernels (gradient checks). */
 int ibpm_nn_loss_grad(ibpm_ctx* ctx, const double* h_params, const double* h_x, const double* h_y, long n,
                       double* h_grad, double* loss);
+
+/* ---- preconditioned CG force solve (moving bodies) -------------------------
+   No counterpart in the reference, which runs plain CG
+   (src/ConjugateGradientSolver.cc:35-65).  mode 0: plain CG.  mode 1: CG
+   preconditioned by
+     H = (I - S G Y^T) P (I - Y G S^T) + S G S^T,  G = (S^T Y)^-1,
+   where P is the FP64 Cholesky inverse of M at an earlier body position
+   (built like the stationary-body factor and rebuilt once the extra
+   iterations since the last build exceed the cost of a build, one M
+   application per unknown), scaled by beta to each RK substep, and (S, Y=M S)
+   are the first `pairs` (0..8) search directions of the previous solve of the
+   same substep: a low-rank update learned from the solver's own Krylov data.
+   Every solve ends on the unpreconditioned test r.r <= tol^2 of plain CG, and
+   plain CG finishes any solve in which the preconditioner breaks down, so the
+   answer is the plain solver's to within its tolerance. */
+int ibpm_batch_set_precond(ibpm_batch* b, int mode, int pairs);
+/* Force-solve counters since init: [0] iterations summed over flows and
+   substeps, [1] solves (flow-substeps), [2] batched M applications in solves,
+   [3] batched M applications spent building preconditioners, [4] builds,
+   [5] solves finished by plain CG after a breakdown, [6] 1 when some M was not
+   positive definite and preconditioning stopped.  Returns the count written. */
+int ibpm_batch_solver_stats(ibpm_batch* b, long* stats, int nstats);
 
 /* Measurement: advances nsteps with a HIP event pair around every kernel
    launch on the batch stream and returns per kernel class the launch count,

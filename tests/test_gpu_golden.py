@@ -97,3 +97,62 @@ def test_ibpm_driver_force_file(gpu, oracle):
         assert abs(full[k, 2] - cd) <= 1e-9 * abs(cd)
         assert abs(full[k, 3] - cl) <= 1e-9 * abs(cd)
         assert abs(float(rows[k][2]) - cd) <= 1e-5 * abs(cd)
+
+
+def test_ibpm_driver_moving_plate_precond(gpu, oracle):
+    """The driver on a pitching plate with -precond lowrank and -tol: forces
+    agree with the oracle's plain CG to the CG tolerance's effect."""
+    exe = os.path.join(os.path.dirname(HERE), "ibpm_amd", "ibpm")
+    with tempfile.TemporaryDirectory() as d:
+        gpath = os.path.join(d, "plate.geom")
+        open(gpath, "w").write(PLATE)
+        r = subprocess.run([exe, "-geom", gpath, "-nx", "128", "-ny", "128", "-ngrid", "3", "-Re", "300",
+                            "-dt", "0.005", "-nsteps", "5", "-tol", "1e-10", "-precond", "lowrank",
+                            "-outdir", d, "-name", "plate"], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        full = np.loadtxt(os.path.join(d, "plate.force17"))
+    s = oracle.Sim(128, 128, 3, 4.0, -2.0, -2.0, 300.0, 0.005, PLATE, tol=1e-10)
+    s.init()
+    for k in range(5):
+        s.advance(1)
+        cd, cl = s.forces()
+        scale = max(abs(cd), abs(cl))
+        assert abs(full[k, 2] - cd) <= 1e-7 * scale and abs(full[k, 3] - cl) <= 1e-7 * scale
+
+
+@pytest.mark.parametrize("name", ["plate", "cfg4short"])
+@pytest.mark.parametrize("pairs", [0, 8])
+def test_preconditioned_cg_matches_golden(gpu, name, pairs):
+    """The preconditioned force solve (stale FP64 inverse of M, with and
+    without the low-rank update) against the oracle's plain-CG fixtures, at
+    the same gates: only iteration counts may change."""
+    g = _fixture(name)
+    f = _flow(name)
+    nsteps = int(g["nsteps"])
+    print("preconditioned", name, "pairs", pairs, nsteps, "steps", flush=True)
+    ctx = gpu.Context(0)
+    b = gpu.Batch(ctx, [f]).init().set_precond("lowrank" if pairs else "stale", pairs)
+    iters = []
+    for k in range(nsteps):
+        b.advance(1)
+        iters.append(b.cg_iters())
+    hist = b.force_history()
+    ref = g["forces"]
+    cd_err = np.abs(hist[:, 0] - ref[:, 0]) / np.abs(ref[:, 0])
+    cl_err = np.abs(hist[:, 1] - ref[:, 1]) / np.abs(ref[:, 1]).max()
+    st = b.solver_stats()
+    print(name, "max Cd rel err %.3e, max Cl err %.3e" % (cd_err.max(), cl_err.max()),
+          "iterations/step %.2f (plain CG %.2f)" % (np.sum(iters) / nsteps, g["cg_iters"].sum() / nsteps),
+          st, flush=True)
+    assert cd_err.max() <= 1e-6
+    assert cl_err.max() <= 1e-6
+    w = b.omega()
+    ns = (f.nx - 1) * (f.ny - 1)
+    s = int(g["stride"])
+    w0 = w[:ns].reshape(f.nx - 1, f.ny - 1)[::s, ::s]
+    assert np.linalg.norm(w0 - g["omega0"]) / np.linalg.norm(g["omega0"]) <= 1e-5
+    norms = np.array([np.linalg.norm(w[l * ns:(l + 1) * ns]) for l in range(f.ngrid)])
+    assert np.all(np.abs(norms - g["omega_norms"]) <= 1e-5 * g["omega_norms"])
+    assert st["fallbacks"] == 0 and st["abandoned"] == 0
+    assert st["iterations"] == int(np.sum(iters))
+    assert np.sum(iters) <= 0.4 * g["cg_iters"].sum()

@@ -399,26 +399,23 @@ __device__ inline void pass3_body(const EllJob& J, double2* smem) {
     default: BODY<0>(J, smem); break;                               \
     }
 
-__global__ void __launch_bounds__(kFftThreads) k_ell_pass1(const EllJob* __restrict__ jobs) {
-    extern __shared__ double2 smem[];
-    const EllJob& J = jobs[blockIdx.y];
-    if (skipped(J.skip)) return;
-    IBPM_DST_DISPATCH(pass1_body, J.pj.M)
-}
-
-__global__ void __launch_bounds__(kFftThreads) k_ell_pass2(const EllJob* __restrict__ jobs) {
-    extern __shared__ double2 smem[];
-    const EllJob& J = jobs[blockIdx.y];
-    if (skipped(J.skip)) return;
-    IBPM_DST_DISPATCH(pass2_body, J.pi.M)
-}
-
-__global__ void __launch_bounds__(kFftThreads) k_ell_pass3(const EllJob* __restrict__ jobs) {
-    extern __shared__ double2 smem[];
-    const EllJob& J = jobs[blockIdx.y];
-    if (skipped(J.skip)) return;
-    IBPM_DST_DISPATCH(pass3_body, J.pj.M)
-}
+// The step's own solves use k_ell_pass*; the solves inside the force solve
+// (M applications) use k_fs_pass*, the same code under another name so that
+// profiles separate the two.  A job whose flow has finished its force solve
+// does nothing (skipped()).
+#define IBPM_PASS_KERNEL(NAME, BODY, LEN)                                        \
+    __global__ void __launch_bounds__(kFftThreads) NAME(const EllJob* __restrict__ jobs) { \
+        extern __shared__ double2 smem[];                                         \
+        const EllJob& J = jobs[blockIdx.y];                                       \
+        if (skipped(J.skip)) return;                                              \
+        IBPM_DST_DISPATCH(BODY, LEN)                                              \
+    }
+IBPM_PASS_KERNEL(k_ell_pass1, pass1_body, J.pj.M)
+IBPM_PASS_KERNEL(k_ell_pass2, pass2_body, J.pi.M)
+IBPM_PASS_KERNEL(k_ell_pass3, pass3_body, J.pj.M)
+IBPM_PASS_KERNEL(k_fs_pass1, pass1_body, J.pj.M)
+IBPM_PASS_KERNEL(k_fs_pass2, pass2_body, J.pi.M)
+IBPM_PASS_KERNEL(k_fs_pass3, pass3_body, J.pj.M)
 
 // --------------------------------------------------------------- coarsify
 // Scalar::coarsify (reference Scalar.cc:46-63) for one level, writing only
@@ -433,8 +430,7 @@ __device__ inline double coarse_src(const CoarsifyJob& J, int lf, int i, int j) 
     return central ? J.R[off] : J.f[off];
 }
 
-__global__ void __launch_bounds__(kThreads) k_coarsify(const CoarsifyJob* __restrict__ jobs) {
-    const CoarsifyJob& J = jobs[blockIdx.y];
+__device__ inline void coarsify_body(const CoarsifyJob& J) {
     if (skipped(J.skip)) return;
     const int nx = J.nx, ny = J.ny;
     const int ex = nx_ext(nx), ey = nx_ext(ny);
@@ -452,6 +448,13 @@ __global__ void __launch_bounds__(kThreads) k_coarsify(const CoarsifyJob* __rest
     J.R[(long)J.lev * (nx - 1) * (ny - 1) + sidx(nx, ny, i, j)] = v;
 }
 
+__global__ void __launch_bounds__(kThreads) k_coarsify(const CoarsifyJob* __restrict__ jobs) {
+    coarsify_body(jobs[blockIdx.y]);
+}
+__global__ void __launch_bounds__(kThreads) k_fs_coarsify(const CoarsifyJob* __restrict__ jobs) {
+    coarsify_body(jobs[blockIdx.y]);
+}
+
 // ------------------------------------------------------------ host launch
 int lds_for_pass(const EllJob& J, int pass) {
     const FftPlan& P = (pass == 2) ? J.pi : J.pj;
@@ -465,16 +468,20 @@ int blocks_for_pass(const EllJob& J, int pass) {
     return (rows_total + rows - 1) / rows;
 }
 
-void launch_ell_pass(int pass, const EllJob* d_jobs, int njobs, int max_blocks, int lds_bytes, hipStream_t s) {
+void launch_ell_pass(int pass, const EllJob* d_jobs, int njobs, int max_blocks, int lds_bytes, hipStream_t s,
+                     bool force_solve) {
     dim3 grid(max_blocks, njobs);
-    if (pass == 1) hipLaunchKernelGGL(k_ell_pass1, grid, dim3(kFftThreads), lds_bytes, s, d_jobs);
-    else if (pass == 2) hipLaunchKernelGGL(k_ell_pass2, grid, dim3(kFftThreads), lds_bytes, s, d_jobs);
-    else hipLaunchKernelGGL(k_ell_pass3, grid, dim3(kFftThreads), lds_bytes, s, d_jobs);
+    if (pass == 1)
+        hipLaunchKernelGGL(force_solve ? k_fs_pass1 : k_ell_pass1, grid, dim3(kFftThreads), lds_bytes, s, d_jobs);
+    else if (pass == 2)
+        hipLaunchKernelGGL(force_solve ? k_fs_pass2 : k_ell_pass2, grid, dim3(kFftThreads), lds_bytes, s, d_jobs);
+    else
+        hipLaunchKernelGGL(force_solve ? k_fs_pass3 : k_ell_pass3, grid, dim3(kFftThreads), lds_bytes, s, d_jobs);
 }
 
-void launch_coarsify(const CoarsifyJob* d_jobs, int njobs, int max_points, hipStream_t s) {
+void launch_coarsify(const CoarsifyJob* d_jobs, int njobs, int max_points, hipStream_t s, bool force_solve) {
     dim3 grid((max_points + kThreads - 1) / kThreads, njobs);
-    hipLaunchKernelGGL(k_coarsify, grid, dim3(kThreads), 0, s, d_jobs);
+    hipLaunchKernelGGL(force_solve ? k_fs_coarsify : k_coarsify, grid, dim3(kThreads), 0, s, d_jobs);
 }
 
 }  // namespace ibpmg

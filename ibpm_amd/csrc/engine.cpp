@@ -127,11 +127,15 @@ struct SolveSpec {
 
 // Optional per-launch timing: a HIP event pair around every kernel launch on
 // the batch stream, accumulated per kernel class with its algorithmic bytes.
+// fs_*: the DST passes and coarsify of the M applications inside the force
+// solve (kernels k_fs_*), counted with the bytes of the flows still iterating.
 enum KClass { K_PASS1, K_PASS2, K_PASS3, K_COARSIFY, K_VEL, K_CROSS, K_RHS, K_CURLQ, K_INTERP,
-              K_SMEAR, K_MINV, K_CG, K_FORCES, K_CLEAR, K_NCLASS };
+              K_SMEAR, K_MINV, K_CG, K_FORCES, K_CLEAR, K_FSPASS1, K_FSPASS2, K_FSPASS3, K_FSCOARSIFY,
+              K_NCLASS };
 static const char* kclass_names[K_NCLASS] = {"ell_pass1", "ell_pass2", "ell_pass3", "coarsify", "vel",
                                              "cross", "rhs", "curl_q", "interp", "smear_curl", "minv",
-                                             "cg", "forces", "clear_nodes"};
+                                             "cg", "forces", "clear_nodes", "fs_pass1", "fs_pass2",
+                                             "fs_pass3", "fs_coarsify"};
 struct Probe {
     struct Rec { int cls; hipEvent_t a, b; double bytes; };
     std::vector<Rec> recs;
@@ -159,7 +163,10 @@ static void timed(Probe* pr, int cls, double bytes, hipStream_t s, Fn fn) {
 }
 
 struct SolvePlan {
+    bool force_solve = false;                       // M application (k_fs_* kernels)
     int nlev = 0;                                   // max L over flows
+    std::vector<std::vector<int>> jflow, cflow;     // flow of each job, per step / coarsify level
+    std::vector<std::vector<double>> jbytes, cjbytes;
     std::vector<double> bytes;                      // per step: 16 B per point (one read + one write)
     std::vector<double> cbytes;                     // per coarsify level
     std::vector<CoarsifyJob*> cj;                   // lev = 1..nlev-1
@@ -176,13 +183,20 @@ static int build_solve_plan(Ctx* c, DevMem& mem, const std::vector<SolveSpec>& s
     P.cj.assign(P.nlev, nullptr);
     P.cj_n.assign(P.nlev, 0);
     P.cj_pts.assign(P.nlev, 0);
+    P.jflow.assign(P.nlev, {});
+    P.jbytes.assign(P.nlev, {});
+    P.cflow.assign(P.nlev, {});
+    P.cjbytes.assign(P.nlev, {});
     for (int lev = 1; lev < P.nlev; ++lev) {
         std::vector<CoarsifyJob> jobs;
         int pts = 0;
-        for (auto& s : specs) {
+        for (size_t k = 0; k < specs.size(); ++k) {
+            const SolveSpec& s = specs[k];
             if (lev >= s.L) continue;
             jobs.push_back(CoarsifyJob{s.nx, s.ny, lev, s.f, s.R, s.skip});
             pts = std::max(pts, (s.nx / 2 - 1) * (s.ny / 2 - 1));
+            P.cflow[lev].push_back((int)k);
+            P.cjbytes[lev].push_back(8.0 * 5 * (s.nx / 2 - 1) * (s.ny / 2 - 1));
         }
         CoarsifyJob* d = nullptr;
         CKR(mem.alloc(&d, jobs.size()));
@@ -202,7 +216,8 @@ static int build_solve_plan(Ctx* c, DevMem& mem, const std::vector<SolveSpec>& s
     for (int k = 0; k < 3; ++k) P.lds[k].assign(P.nlev, 0);
     for (int st = 0; st < P.nlev; ++st) {
         std::vector<EllJob> jobs;
-        for (auto& s : specs) {
+        for (size_t k = 0; k < specs.size(); ++k) {
+            const SolveSpec& s = specs[k];
             if (st >= s.L) continue;
             const int lev = s.L - 1 - st;
             const long ns = (long)(s.nx - 1) * (s.ny - 1);
@@ -234,6 +249,8 @@ static int build_solve_plan(Ctx* c, DevMem& mem, const std::vector<SolveSpec>& s
                 P.lds[k][st] = std::max(P.lds[k][st], lds_for_pass(J, k + 1));
             }
             P.bytes[st] += 16.0 * ns;
+            P.jflow[st].push_back((int)k);
+            P.jbytes[st].push_back(16.0 * ns);
             jobs.push_back(J);
         }
         EllJob* d = nullptr;
@@ -245,13 +262,28 @@ static int build_solve_plan(Ctx* c, DevMem& mem, const std::vector<SolveSpec>& s
     return 0;
 }
 
-static void run_solve(const SolvePlan& P, hipStream_t s, Probe* pr = nullptr) {
+// active (probe only): flows whose jobs do work; the others are skipped by
+// their flags, so their bytes are not counted.
+static void run_solve(const SolvePlan& P, hipStream_t s, Probe* pr = nullptr,
+                      const std::vector<char>* active = nullptr) {
+    auto sum = [&](const std::vector<int>& fl, const std::vector<double>& by, double all) {
+        if (!active) return all;
+        double b = 0;
+        for (size_t j = 0; j < fl.size(); ++j)
+            if ((*active)[fl[j]]) b += by[j];
+        return b;
+    };
+    const bool fs = P.force_solve;
     for (int lev = 1; lev < P.nlev; ++lev)
-        timed(pr, K_COARSIFY, P.cbytes[lev], s, [&] { launch_coarsify(P.cj[lev], P.cj_n[lev], P.cj_pts[lev], s); });
-    for (int st = 0; st < P.nlev; ++st)
+        timed(pr, fs ? K_FSCOARSIFY : K_COARSIFY, pr ? sum(P.cflow[lev], P.cjbytes[lev], P.cbytes[lev]) : 0, s,
+              [&] { launch_coarsify(P.cj[lev], P.cj_n[lev], P.cj_pts[lev], s, fs); });
+    for (int st = 0; st < P.nlev; ++st) {
+        const double by = pr ? sum(P.jflow[st], P.jbytes[st], P.bytes[st]) : 0;
         for (int pass = 1; pass <= 3; ++pass)
-            timed(pr, K_PASS1 + pass - 1, P.bytes[st], s,
-                  [&] { launch_ell_pass(pass, P.ej[st], P.ej_n[st], P.maxb[pass - 1][st], P.lds[pass - 1][st], s); });
+            timed(pr, (fs ? K_FSPASS1 : K_PASS1) + pass - 1, by, s, [&] {
+                launch_ell_pass(pass, P.ej[st], P.ej_n[st], P.maxb[pass - 1][st], P.lds[pass - 1][st], s, fs);
+            });
+    }
 }
 
 static int cos_tables(DevMem& mem, int n, double** out) {
@@ -531,6 +563,7 @@ static int build_plans(ibpm_batch* B) {
         CKR(build_solve_plan(c, B->mem, s1, B->S1[i]));
         CKR(build_solve_plan(c, B->mem, s2, B->S2[i]));
         CKR(build_solve_plan(c, B->mem, s3, B->S3[i]));
+        B->M1[i].force_solve = true;
         CKR(build_solve_plan(c, B->mem, m1, B->M1[i]));
     }
     std::vector<SolveSpec> s4, m2;
@@ -540,6 +573,7 @@ static int build_plans(ibpm_batch* B) {
         m2.back().skip = F.cg_state + 3;
     }
     CKR(build_solve_plan(c, B->mem, s4, B->S4));
+    B->M2.force_solve = true;
     CKR(build_solve_plan(c, B->mem, m2, B->M2));
     // stencil descriptors
     for (int i = 0; i < B->nsub; ++i) {
@@ -594,12 +628,13 @@ static void mark_nprev_valid(ibpm_batch* B) {
 }
 
 // M f = C A^-1 (beta B f) for the vector behind d_smear (ProjectionSolver.cc:96-101).
-static void enqueue_M(ibpm_batch* B, int i, RegOps* smear, hipStream_t s) {
+static void enqueue_M(ibpm_batch* B, int i, RegOps* smear, hipStream_t s,
+                      const std::vector<char>* active = nullptr) {
     const int nfl = (int)B->flows.size();
     Probe* pr = B->probe;
     timed(pr, K_SMEAR, 0, s, [&] { launch_smear_curl(smear, nfl, B->max_nodes, s); });
-    run_solve(B->M1[i], s, pr);
-    run_solve(B->M2, s, pr);
+    run_solve(B->M1[i], s, pr, active);
+    run_solve(B->M2, s, pr, active);
     timed(pr, K_INTERP, 0, s, [&] { launch_interp(B->d_interp_m, nfl, B->max_comp, s); });
 }
 
@@ -709,15 +744,19 @@ static int solve_cg(ibpm_batch* B, int i, int* iters) {
     B->st_mapps += 1;
     timed(pr, K_CG, 0, s, [&] { launch_cg(B->d_cg, nfl, 0, s); });
     std::vector<double> st;
+    std::vector<char> active(nfl, 1);
     const int max_iter = 100000;
-    for (int it = 0; it < max_iter; it += B->cg_chunk) {
+    const int chunk = pr ? 1 : B->cg_chunk;   // measuring: exact active flows per launch
+    for (int it = 0; it < max_iter; it += chunk) {
         CKR(read_states(B, st));
         bool all = true;
-        for (int k = 0; k < nfl; ++k)
-            if (st[8 * k + 3] == 0.0) all = false;
+        for (int k = 0; k < nfl; ++k) {
+            active[k] = st[8 * k + 3] == 0.0;
+            if (active[k]) all = false;
+        }
         if (all) break;
-        for (int c = 0; c < B->cg_chunk; ++c) {
-            enqueue_M(B, i, B->d_smear_d[i], s);
+        for (int c = 0; c < chunk; ++c) {
+            enqueue_M(B, i, B->d_smear_d[i], s, pr ? &active : nullptr);
             timed(pr, K_CG, 0, s, [&] { launch_cg(B->d_cg, nfl, 1, s); });
             B->st_mapps += 1;
         }
@@ -748,22 +787,27 @@ static int solve_pcg(ibpm_batch* B, int i, int* iters, bool* fallback) {
     precondition(1);
     // the first poll comes after one iteration fewer than the last solve of
     // this substep took (iterations past convergence are no-ops)
+    // (measuring: one iteration per poll, so the active flows of every
+    // launch are known exactly)
     const int limit = std::max(100, 2 * B->max_n);
-    int chunk = std::max(0, B->pc_last[i] - 1), done = 0;
+    int chunk = pr ? 0 : std::max(0, B->pc_last[i] - 1), done = 0;
     std::vector<double> st;
+    std::vector<char> active(nfl, 1);
     for (;;) {
         for (int c = 0; c < chunk; ++c) {
-            enqueue_M(B, i, B->d_smear_d[i], s);
+            enqueue_M(B, i, B->d_smear_d[i], s, pr ? &active : nullptr);
             timed(pr, K_CG, 0, s, [&] { launch_pcg_step(ops, nfl, s); });
             precondition(0);
             B->st_mapps += 1;
         }
         done += chunk;
-        chunk = 2;
+        chunk = pr ? 1 : 2;
         CKR(read_states(B, st));
         bool all = true;
-        for (int k = 0; k < nfl; ++k)
-            if (st[8 * k + 3] == 0.0 && st[8 * k + 4] == 0.0) all = false;
+        for (int k = 0; k < nfl; ++k) {
+            active[k] = st[8 * k + 3] == 0.0 && st[8 * k + 4] == 0.0;
+            if (active[k]) all = false;
+        }
         if (all || done >= limit) break;
     }
     B->pc_par[i] = 1 - par;

@@ -92,9 +92,43 @@ template <> __device__ inline void dft<8>(double2* v) {
     v[3] = cadd(a[3], t3); v[7] = csub(a[3], t3);
 }
 
+// FFT lengths with a compiled plan (M = 2(N+1) for N = 199, 255, 383, 399,
+// 511, 1023: the configured grids); any other length uses the runtime plan.
+__host__ __device__ constexpr bool compiled_len(int M) {
+    return M == 400 || M == 512 || M == 768 || M == 800 || M == 1024 || M == 2048;
+}
+
+// Twiddle w^k = exp(-2 pi i k / M), k < M.  The longest compiled length
+// keeps only the first quarter in LDS, w^(q M/4 + r) = (-i)^q w^r, which
+// halves its LDS footprint and doubles the workgroups per CU; shorter
+// lengths are not LDS-limited and read the full table.
+__host__ __device__ constexpr bool quarter_twiddles(int M) { return M >= 2048 && M % 4 == 0; }
+
+template <int CM>
+__device__ inline double2 twiddle(const double2* tw, int k) {
+    if constexpr (CM != 0 && quarter_twiddles(CM)) {
+        constexpr int Q = CM / 4;
+        const int q = k / Q, r = k - q * Q;
+        const double2 t = tw[r];
+        double x = t.x, y = t.y;
+        if (q & 1) {   // times -i
+            const double u = x;
+            x = y;
+            y = -u;
+        }
+        if (q & 2) {   // times -1
+            x = -x;
+            y = -y;
+        }
+        return make_double2(x, y);
+    } else {
+        return tw[k];
+    }
+}
+
 // One Stockham radix-R stage over np FFTs of length M stored back to back in
 // LDS; inputs are staged through registers so the stage runs in place.
-// Twiddles come from the LDS table tw[k] = exp(-2 pi i k / M).  Template
+// Twiddles come from the LDS table (see twiddle()).  Template
 // arguments that are 0 are taken from the runtime values.
 template <int R, int CM, int CNP, int CNS>
 __device__ inline void fft_stage(double2* buf, const double2* tw, int Mr, int npr, int Nsr) {
@@ -119,7 +153,7 @@ __device__ inline void fft_stage(double2* buf, const double2* tw, int Mr, int np
 #pragma unroll
             for (int r = 1; r < R; ++r) {
                 const double2 in = x[j + r * MR];
-                v[b][r] = (CM && CNS == 1) ? in : cmul(in, tw[r * jm * tstep]);
+                v[b][r] = (CM && CNS == 1) ? in : cmul(in, twiddle<CM>(tw, r * jm * tstep));
             }
             dft<R>(v[b]);
         }
@@ -240,17 +274,17 @@ __device__ inline bool on_bc_ring(const EllJob& J, int i, int j) {
 // so with CM != 0 every index computation folds to constants.
 template <int CM>
 __device__ inline void load_twiddles(double2* tw, const FftPlan& P) {
-    const int M = CM ? CM : P.M;
+    const int M = (CM && quarter_twiddles(CM)) ? CM / 4 : (CM ? CM : P.M);
     for (int k = threadIdx.x; k < M; k += kFftThreads) tw[k] = P.tw[k];
 }
 
 template <int CM>
-__device__ inline void pass1_body(const EllJob& J, double2* smem) {
+__device__ inline void pass1_body(const EllJob& J, double2* smem, int bx) {
     const FftPlan& P = J.pj;
     const int M = CM ? CM : P.M, np = CM ? np_of(CM) : P.np;
     const int N1 = J.nx - 1, N2 = M / 2 - 1;
     const int rows = 2 * np;
-    const int r0 = blockIdx.x * rows;
+    const int r0 = bx * rows;
     if (r0 >= N1) return;
     double2* buf = smem;
     double2* tw = smem + np * M;
@@ -280,12 +314,12 @@ __device__ inline void pass1_body(const EllJob& J, double2* smem) {
 }
 
 template <int CM>
-__device__ inline void pass2_body(const EllJob& J, double2* smem) {
+__device__ inline void pass2_body(const EllJob& J, double2* smem, int bx) {
     const FftPlan& P = J.pi;
     const int M = CM ? CM : P.M, np = CM ? np_of(CM) : P.np;
     const int N1 = M / 2 - 1, N2 = J.ny - 1;
     const int rows = 2 * np;
-    const int r0 = blockIdx.x * rows;   // rows are l = 0..N2-1
+    const int r0 = bx * rows;   // rows are l = 0..N2-1
     if (r0 >= N2) return;
     double2* buf = smem;
     double2* tw = smem + np * M;
@@ -339,12 +373,12 @@ __device__ inline void pass2_body(const EllJob& J, double2* smem) {
 }
 
 template <int CM>
-__device__ inline void pass3_body(const EllJob& J, double2* smem) {
+__device__ inline void pass3_body(const EllJob& J, double2* smem, int bx) {
     const FftPlan& P = J.pj;
     const int M = CM ? CM : P.M, np = CM ? np_of(CM) : P.np;
     const int N1 = J.nx - 1, N2 = M / 2 - 1;
     const int rows = 2 * np;
-    const int r0 = blockIdx.x * rows;
+    const int r0 = bx * rows;
     if (r0 >= N1) return;
     double2* buf = smem;
     double2* tw = smem + np * M;
@@ -386,36 +420,41 @@ __device__ inline void pass3_body(const EllJob& J, double2* smem) {
     }
 }
 
-// Lengths with a compiled FFT (M = 2(N+1) for N = 199, 255, 383, 399, 511,
-// 1023: the configured grids); any other length uses the runtime plan.
-#define IBPM_DST_DISPATCH(BODY, M_)                                 \
-    switch (M_) {                                                   \
-    case 400: BODY<400>(J, smem); break;                            \
-    case 512: BODY<512>(J, smem); break;                            \
-    case 768: BODY<768>(J, smem); break;                            \
-    case 800: BODY<800>(J, smem); break;                            \
-    case 1024: BODY<1024>(J, smem); break;                          \
-    case 2048: BODY<2048>(J, smem); break;                          \
-    default: BODY<0>(J, smem); break;                               \
-    }
+// Logical (row block, job) of this workgroup.  Workgroups are handed to the
+// 8 XCDs round-robin in launch order, so logical neighbours are mapped to the
+// same XCD: consecutive row blocks write neighbouring 16-byte pieces of the
+// same cache lines of the transposed output, and those pieces then meet in
+// one XCD's L2 instead of being written back partially from eight.
+__device__ inline int2 xcd_block() {
+    constexpr int X = 8;
+    const int gx = gridDim.x, total = gx * gridDim.y;
+    const int flat = blockIdx.x + blockIdx.y * gx;
+    const int xcd = flat % X, idx = flat / X, q = total / X, r = total % X;
+    const int lf = (xcd < r) ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
+    return make_int2(lf % gx, lf / gx);
+}
 
+// One kernel per pass and FFT length (compiled lengths, 0 = runtime plan), so
+// each length gets its own register budget; a launch holds jobs of one length.
 // The step's own solves use k_ell_pass*; the solves inside the force solve
 // (M applications) use k_fs_pass*, the same code under another name so that
 // profiles separate the two.  A job whose flow has finished its force solve
 // does nothing (skipped()).
-#define IBPM_PASS_KERNEL(NAME, BODY, LEN)                                        \
+#define IBPM_PASS_KERNEL(NAME, BODY)                                             \
+    template <int CM>                                                            \
     __global__ void __launch_bounds__(kFftThreads) NAME(const EllJob* __restrict__ jobs) { \
         extern __shared__ double2 smem[];                                         \
-        const EllJob& J = jobs[blockIdx.y];                                       \
+        const int2 blk = xcd_block();                                             \
+        const EllJob& J = jobs[blk.y];                                            \
         if (skipped(J.skip)) return;                                              \
-        IBPM_DST_DISPATCH(BODY, LEN)                                              \
+        BODY<CM>(J, smem, blk.x);                                                 \
     }
-IBPM_PASS_KERNEL(k_ell_pass1, pass1_body, J.pj.M)
-IBPM_PASS_KERNEL(k_ell_pass2, pass2_body, J.pi.M)
-IBPM_PASS_KERNEL(k_ell_pass3, pass3_body, J.pj.M)
-IBPM_PASS_KERNEL(k_fs_pass1, pass1_body, J.pj.M)
-IBPM_PASS_KERNEL(k_fs_pass2, pass2_body, J.pi.M)
-IBPM_PASS_KERNEL(k_fs_pass3, pass3_body, J.pj.M)
+IBPM_PASS_KERNEL(k_ell_pass1, pass1_body)
+IBPM_PASS_KERNEL(k_ell_pass2, pass2_body)
+IBPM_PASS_KERNEL(k_ell_pass3, pass3_body)
+IBPM_PASS_KERNEL(k_fs_pass1, pass1_body)
+IBPM_PASS_KERNEL(k_fs_pass2, pass2_body)
+IBPM_PASS_KERNEL(k_fs_pass3, pass3_body)
 
 // --------------------------------------------------------------- coarsify
 // Scalar::coarsify (reference Scalar.cc:46-63) for one level, writing only
@@ -458,7 +497,8 @@ __global__ void __launch_bounds__(kThreads) k_fs_coarsify(const CoarsifyJob* __r
 // ------------------------------------------------------------ host launch
 int lds_for_pass(const EllJob& J, int pass) {
     const FftPlan& P = (pass == 2) ? J.pi : J.pj;
-    return (int)sizeof(double2) * (P.np * P.M + P.M);
+    const int ntw = (compiled_len(P.M) && quarter_twiddles(P.M)) ? P.M / 4 : P.M;
+    return (int)sizeof(double2) * (P.np * P.M + ntw);
 }
 
 int blocks_for_pass(const EllJob& J, int pass) {
@@ -468,15 +508,32 @@ int blocks_for_pass(const EllJob& J, int pass) {
     return (rows_total + rows - 1) / rows;
 }
 
+#define IBPM_LAUNCH_LEN(KER)                                                                   \
+    switch (M) {                                                                               \
+    case 400: hipLaunchKernelGGL(KER<400>, grid, dim3(kFftThreads), lds_bytes, s, d_jobs); break;   \
+    case 512: hipLaunchKernelGGL(KER<512>, grid, dim3(kFftThreads), lds_bytes, s, d_jobs); break;   \
+    case 768: hipLaunchKernelGGL(KER<768>, grid, dim3(kFftThreads), lds_bytes, s, d_jobs); break;   \
+    case 800: hipLaunchKernelGGL(KER<800>, grid, dim3(kFftThreads), lds_bytes, s, d_jobs); break;   \
+    case 1024: hipLaunchKernelGGL(KER<1024>, grid, dim3(kFftThreads), lds_bytes, s, d_jobs); break; \
+    case 2048: hipLaunchKernelGGL(KER<2048>, grid, dim3(kFftThreads), lds_bytes, s, d_jobs); break; \
+    default: hipLaunchKernelGGL(KER<0>, grid, dim3(kFftThreads), lds_bytes, s, d_jobs); break;       \
+    }
+
+// M: the FFT length of this pass (J.pj.M for passes 1 and 3, J.pi.M for
+// pass 2), the same for every job of the launch.
 void launch_ell_pass(int pass, const EllJob* d_jobs, int njobs, int max_blocks, int lds_bytes, hipStream_t s,
-                     bool force_solve) {
+                     bool force_solve, int M) {
+    static_assert(compiled_len(400) && compiled_len(512) && compiled_len(768) && compiled_len(800) &&
+                      compiled_len(1024) && compiled_len(2048),
+                  "the launch switch and compiled_len must list the same lengths");
     dim3 grid(max_blocks, njobs);
-    if (pass == 1)
-        hipLaunchKernelGGL(force_solve ? k_fs_pass1 : k_ell_pass1, grid, dim3(kFftThreads), lds_bytes, s, d_jobs);
-    else if (pass == 2)
-        hipLaunchKernelGGL(force_solve ? k_fs_pass2 : k_ell_pass2, grid, dim3(kFftThreads), lds_bytes, s, d_jobs);
-    else
-        hipLaunchKernelGGL(force_solve ? k_fs_pass3 : k_ell_pass3, grid, dim3(kFftThreads), lds_bytes, s, d_jobs);
+    if (pass == 1) {
+        if (force_solve) { IBPM_LAUNCH_LEN(k_fs_pass1) } else { IBPM_LAUNCH_LEN(k_ell_pass1) }
+    } else if (pass == 2) {
+        if (force_solve) { IBPM_LAUNCH_LEN(k_fs_pass2) } else { IBPM_LAUNCH_LEN(k_ell_pass2) }
+    } else {
+        if (force_solve) { IBPM_LAUNCH_LEN(k_fs_pass3) } else { IBPM_LAUNCH_LEN(k_ell_pass3) }
+    }
 }
 
 void launch_coarsify(const CoarsifyJob* d_jobs, int njobs, int max_points, hipStream_t s, bool force_solve) {

@@ -162,19 +162,27 @@ static void timed(Probe* pr, int cls, double bytes, hipStream_t s, Fn fn) {
     pr->recs.push_back(r);
 }
 
+// The jobs of one level step that share their transform lengths (one launch
+// per pass).
+struct PassGroup {
+    EllJob* d = nullptr;
+    int n = 0;
+    int mj = 0, mi = 0;                             // FFT lengths along j (passes 1, 3) and i (pass 2)
+    int maxb[3] = {0, 0, 0}, lds[3] = {0, 0, 0};
+    std::vector<int> jflow;                         // flow of each job
+    std::vector<double> jbytes;                     // 16 B per point (one read + one write)
+    double bytes = 0;
+};
+
 struct SolvePlan {
     bool force_solve = false;                       // M application (k_fs_* kernels)
     int nlev = 0;                                   // max L over flows
-    std::vector<std::vector<int>> jflow, cflow;     // flow of each job, per step / coarsify level
-    std::vector<std::vector<double>> jbytes, cjbytes;
-    std::vector<double> bytes;                      // per step: 16 B per point (one read + one write)
+    std::vector<std::vector<int>> cflow;            // flow of each coarsify job, per level
+    std::vector<std::vector<double>> cjbytes;
     std::vector<double> cbytes;                     // per coarsify level
     std::vector<CoarsifyJob*> cj;                   // lev = 1..nlev-1
     std::vector<int> cj_n, cj_pts;
-    std::vector<EllJob*> ej;                        // step s = 0..nlev-1
-    std::vector<int> ej_n;
-    std::vector<int> maxb[3];
-    std::vector<int> lds[3];
+    std::vector<std::vector<PassGroup>> steps;      // step s = 0..nlev-1 (coarsest level first)
 };
 
 static int build_solve_plan(Ctx* c, DevMem& mem, const std::vector<SolveSpec>& specs, SolvePlan& P) {
@@ -183,8 +191,6 @@ static int build_solve_plan(Ctx* c, DevMem& mem, const std::vector<SolveSpec>& s
     P.cj.assign(P.nlev, nullptr);
     P.cj_n.assign(P.nlev, 0);
     P.cj_pts.assign(P.nlev, 0);
-    P.jflow.assign(P.nlev, {});
-    P.jbytes.assign(P.nlev, {});
     P.cflow.assign(P.nlev, {});
     P.cjbytes.assign(P.nlev, {});
     for (int lev = 1; lev < P.nlev; ++lev) {
@@ -205,17 +211,14 @@ static int build_solve_plan(Ctx* c, DevMem& mem, const std::vector<SolveSpec>& s
         P.cj_n[lev] = (int)jobs.size();
         P.cj_pts[lev] = pts;
     }
-    P.ej.assign(P.nlev, nullptr);
-    P.ej_n.assign(P.nlev, 0);
-    P.bytes.assign(P.nlev, 0.0);
+    P.steps.assign(P.nlev, {});
     P.cbytes.assign(P.nlev, 0.0);
     for (int lev = 1; lev < P.nlev; ++lev)
         for (auto& s : specs)
             if (lev < s.L) P.cbytes[lev] += 8.0 * 5 * (s.nx / 2 - 1) * (s.ny / 2 - 1);   // 4 fine reads + 1 write
-    for (int k = 0; k < 3; ++k) P.maxb[k].assign(P.nlev, 0);
-    for (int k = 0; k < 3; ++k) P.lds[k].assign(P.nlev, 0);
     for (int st = 0; st < P.nlev; ++st) {
-        std::vector<EllJob> jobs;
+        std::vector<PassGroup>& groups = P.steps[st];
+        std::vector<std::vector<EllJob>> gjobs;
         for (size_t k = 0; k < specs.size(); ++k) {
             const SolveSpec& s = specs[k];
             if (st >= s.L) continue;
@@ -244,20 +247,31 @@ static int build_solve_plan(Ctx* c, DevMem& mem, const std::vector<SolveSpec>& s
             J.skip = s.skip;
             CKR(get_plan(c, s.ny - 1, &J.pj));
             CKR(get_plan(c, s.nx - 1, &J.pi));
-            for (int k = 0; k < 3; ++k) {
-                P.maxb[k][st] = std::max(P.maxb[k][st], blocks_for_pass(J, k + 1));
-                P.lds[k][st] = std::max(P.lds[k][st], lds_for_pass(J, k + 1));
+            size_t g = 0;
+            while (g < groups.size() && !(groups[g].mj == J.pj.M && groups[g].mi == J.pi.M)) ++g;
+            if (g == groups.size()) {
+                groups.emplace_back();
+                groups.back().mj = J.pj.M;
+                groups.back().mi = J.pi.M;
+                gjobs.emplace_back();
             }
-            P.bytes[st] += 16.0 * ns;
-            P.jflow[st].push_back((int)k);
-            P.jbytes[st].push_back(16.0 * ns);
-            jobs.push_back(J);
+            groups[g].bytes += 16.0 * ns;
+            groups[g].jflow.push_back((int)k);
+            groups[g].jbytes.push_back(16.0 * ns);
+            gjobs[g].push_back(J);
         }
-        EllJob* d = nullptr;
-        CKR(mem.alloc(&d, jobs.size()));
-        CK(hipMemcpy(d, jobs.data(), sizeof(EllJob) * jobs.size(), hipMemcpyHostToDevice));
-        P.ej[st] = d;
-        P.ej_n[st] = (int)jobs.size();
+        for (size_t g = 0; g < groups.size(); ++g) {
+            PassGroup& G = groups[g];
+            std::vector<EllJob>& jobs = gjobs[g];
+            for (auto& J : jobs)
+                for (int k = 0; k < 3; ++k) {
+                    G.maxb[k] = std::max(G.maxb[k], blocks_for_pass(J, k + 1));
+                    G.lds[k] = std::max(G.lds[k], lds_for_pass(J, k + 1));
+                }
+            CKR(mem.alloc(&G.d, jobs.size()));
+            CK(hipMemcpy(G.d, jobs.data(), sizeof(EllJob) * jobs.size(), hipMemcpyHostToDevice));
+            G.n = (int)jobs.size();
+        }
     }
     return 0;
 }
@@ -277,13 +291,15 @@ static void run_solve(const SolvePlan& P, hipStream_t s, Probe* pr = nullptr,
     for (int lev = 1; lev < P.nlev; ++lev)
         timed(pr, fs ? K_FSCOARSIFY : K_COARSIFY, pr ? sum(P.cflow[lev], P.cjbytes[lev], P.cbytes[lev]) : 0, s,
               [&] { launch_coarsify(P.cj[lev], P.cj_n[lev], P.cj_pts[lev], s, fs); });
-    for (int st = 0; st < P.nlev; ++st) {
-        const double by = pr ? sum(P.jflow[st], P.jbytes[st], P.bytes[st]) : 0;
-        for (int pass = 1; pass <= 3; ++pass)
-            timed(pr, (fs ? K_FSPASS1 : K_PASS1) + pass - 1, by, s, [&] {
-                launch_ell_pass(pass, P.ej[st], P.ej_n[st], P.maxb[pass - 1][st], P.lds[pass - 1][st], s, fs);
-            });
-    }
+    for (int st = 0; st < P.nlev; ++st)
+        for (const PassGroup& G : P.steps[st]) {
+            const double by = pr ? sum(G.jflow, G.jbytes, G.bytes) : 0;
+            for (int pass = 1; pass <= 3; ++pass)
+                timed(pr, (fs ? K_FSPASS1 : K_PASS1) + pass - 1, by, s, [&] {
+                    launch_ell_pass(pass, G.d, G.n, G.maxb[pass - 1], G.lds[pass - 1], s, fs,
+                                    pass == 2 ? G.mi : G.mj);
+                });
+        }
 }
 
 static int cos_tables(DevMem& mem, int n, double** out) {
@@ -1188,8 +1204,8 @@ int ibpm_dst2d(ibpm_ctx* c, int n1, int n2, const double* d_in, double* d_out) {
     CKR(mem.alloc(&d, 2));
     CK(hipMemcpyAsync(d, &J, sizeof(EllJob), hipMemcpyHostToDevice, c->c.stream));
     CK(hipMemcpyAsync(d + 1, &J2, sizeof(EllJob), hipMemcpyHostToDevice, c->c.stream));
-    launch_ell_pass(1, d, 1, blocks_for_pass(J, 1), lds_for_pass(J, 1), c->c.stream);
-    launch_ell_pass(1, d + 1, 1, blocks_for_pass(J2, 1), lds_for_pass(J2, 1), c->c.stream);
+    launch_ell_pass(1, d, 1, blocks_for_pass(J, 1), lds_for_pass(J, 1), c->c.stream, false, J.pj.M);
+    launch_ell_pass(1, d + 1, 1, blocks_for_pass(J2, 1), lds_for_pass(J2, 1), c->c.stream, false, J2.pj.M);
     CK(hipStreamSynchronize(c->c.stream));
     CK(hipGetLastError());
     return 0;

@@ -126,11 +126,30 @@ __device__ inline double2 twiddle(const double2* tw, int k) {
     }
 }
 
+// LDS slot of element e when the buffer is held swizzled: the low 3 bits
+// are XORed with bits 3..5.  This permutes each aligned group of 8 complex
+// values (128 B, one sweep of the 32 banks; compiled lengths are multiples
+// of 8), so 8 lanes whose elements share their low 3 bits -- the first
+// stage's radix-R writes, R apart -- land on distinct banks, while contiguous
+// runs stay conflict-free.  Only the layout between the first and second
+// stage is swizzled (the first stage's writes were the conflicting ones).
+template <bool SW>
+__device__ inline int lds_slot(int e) {
+    if constexpr (SW) return e ^ ((e >> 3) & 7);
+    else return e;
+}
+
+// The 2048-point transforms (largest grids, bandwidth-bound batches) keep the
+// whole buffer swizzled, which measured faster there; the shorter compiled
+// lengths swizzle only between the first two stages.
+template <int CM>
+constexpr bool kSwizzleAll = CM >= 2048;
+
 // One Stockham radix-R stage over np FFTs of length M stored back to back in
 // LDS; inputs are staged through registers so the stage runs in place.
 // Twiddles come from the LDS table (see twiddle()).  Template
 // arguments that are 0 are taken from the runtime values.
-template <int R, int CM, int CNP, int CNS>
+template <int R, int CM, int CNP, int CNS, bool SWIN = false, bool SWOUT = false>
 __device__ inline void fft_stage(double2* buf, const double2* tw, int Mr, int npr, int Nsr) {
     const int M = CM ? CM : Mr;
     const int np = CM ? CNP : npr;
@@ -148,12 +167,20 @@ __device__ inline void fft_stage(double2* buf, const double2* tw, int Mr, int np
             const int p = g / MR;
             const int j = g - p * MR;
             const int jm = j % Ns;
-            const double2* x = buf + p * M;
-            v[b][0] = x[j];
+            const int x = p * M;
+            v[b][0] = buf[lds_slot<SWIN>(x + j)];
+            if constexpr (CM != 0 && CNS == 1) {
 #pragma unroll
-            for (int r = 1; r < R; ++r) {
-                const double2 in = x[j + r * MR];
-                v[b][r] = (CM && CNS == 1) ? in : cmul(in, twiddle<CM>(tw, r * jm * tstep));
+                for (int r = 1; r < R; ++r) v[b][r] = buf[lds_slot<SWIN>(x + j + r * MR)];
+            } else {
+                // w^r by binary splitting, w^r = w^(r/2) w^(r - r/2): at most
+                // 3 dependent products for R <= 8
+                double2 pw[R];
+                pw[1] = twiddle<CM>(tw, jm * tstep);
+#pragma unroll
+                for (int r = 2; r < R; ++r) pw[r] = cmul(pw[r / 2], pw[r - r / 2]);
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[b][r] = cmul(buf[lds_slot<SWIN>(x + j + r * MR)], pw[r]);
             }
             dft<R>(v[b]);
         }
@@ -166,10 +193,9 @@ __device__ inline void fft_stage(double2* buf, const double2* tw, int Mr, int np
             const int p = g / MR;
             const int j = g - p * MR;
             const int jm = j % Ns;
-            double2* x = buf + p * M;
-            const int d = (j / Ns) * Ns * R + jm;
+            const int d = p * M + (j / Ns) * Ns * R + jm;
 #pragma unroll
-            for (int r = 0; r < R; ++r) x[d + r * Ns] = v[b][r];
+            for (int r = 0; r < R; ++r) buf[lds_slot<SWOUT>(d + r * Ns)] = v[b][r];
         }
     }
     __syncthreads();
@@ -184,7 +210,9 @@ template <int M, int NP, int Ns>
 __device__ inline void fft_ct(double2* buf, const double2* tw) {
     if constexpr (Ns < M) {
         constexpr int R = pick_radix(M / Ns);
-        fft_stage<R, M, NP, Ns>(buf, tw, 0, 0, 0);
+        constexpr bool first = Ns == 1, second = Ns > 1 && Ns == pick_radix(M);
+        constexpr bool all = kSwizzleAll<M>;
+        fft_stage<R, M, NP, Ns, all || second, all || (first && R < M)>(buf, tw, 0, 0, 0);
         fft_ct<M, NP, Ns * R>(buf, tw);
     }
 }
@@ -295,12 +323,12 @@ __device__ inline void pass1_body(const EllJob& J, double2* smem, int bx) {
         const int ra = r0 + 2 * p, rb = ra + 1;
         const double va = (ra < N1) ? rhs_value(J, ra + 1, k + 1) : 0.0;
         const double vb = (rb < N1) ? rhs_value(J, rb + 1, k + 1) : 0.0;
-        buf[p * M + k + 1] = make_double2(va, vb);
-        buf[p * M + M - 1 - k] = make_double2(-va, -vb);
+        buf[lds_slot<kSwizzleAll<CM>>(p * M + k + 1)] = make_double2(va, vb);
+        buf[lds_slot<kSwizzleAll<CM>>(p * M + M - 1 - k)] = make_double2(-va, -vb);
     }
     for (int p = threadIdx.x; p < np; p += kFftThreads) {
-        buf[p * M] = make_double2(0.0, 0.0);
-        buf[p * M + n] = make_double2(0.0, 0.0);
+        buf[lds_slot<kSwizzleAll<CM>>(p * M)] = make_double2(0.0, 0.0);
+        buf[lds_slot<kSwizzleAll<CM>>(p * M + n)] = make_double2(0.0, 0.0);
     }
     __syncthreads();
     fft_any<CM>(buf, tw, P);
@@ -308,7 +336,7 @@ __device__ inline void pass1_body(const EllJob& J, double2* smem, int bx) {
         const int r = idx % rows, k = idx / rows;
         const int row = r0 + r;
         if (row >= N1) continue;
-        const double2 z = buf[(r >> 1) * M + k + 1];
+        const double2 z = buf[lds_slot<kSwizzleAll<CM>>((r >> 1) * M + k + 1)];
         J.T1[(long)k * N1 + row] = (r & 1) ? z.x : -z.y;
     }
 }
@@ -330,12 +358,12 @@ __device__ inline void pass2_body(const EllJob& J, double2* smem, int bx) {
         const int la = r0 + 2 * p, lb = la + 1;
         const double va = (la < N2) ? J.T1[(long)la * N1 + k] : 0.0;
         const double vb = (lb < N2) ? J.T1[(long)lb * N1 + k] : 0.0;
-        buf[p * M + k + 1] = make_double2(va, vb);
-        buf[p * M + M - 1 - k] = make_double2(-va, -vb);
+        buf[lds_slot<kSwizzleAll<CM>>(p * M + k + 1)] = make_double2(va, vb);
+        buf[lds_slot<kSwizzleAll<CM>>(p * M + M - 1 - k)] = make_double2(-va, -vb);
     }
     for (int p = threadIdx.x; p < np; p += kFftThreads) {
-        buf[p * M] = make_double2(0.0, 0.0);
-        buf[p * M + n] = make_double2(0.0, 0.0);
+        buf[lds_slot<kSwizzleAll<CM>>(p * M)] = make_double2(0.0, 0.0);
+        buf[lds_slot<kSwizzleAll<CM>>(p * M + n)] = make_double2(0.0, 0.0);
     }
     __syncthreads();
     fft_any<CM>(buf, tw, P);
@@ -343,7 +371,7 @@ __device__ inline void pass2_body(const EllJob& J, double2* smem, int bx) {
     for (int idx = threadIdx.x; idx < np * N1; idx += kFftThreads) {
         const int p = idx / N1, k = idx - p * N1;
         const int la = r0 + 2 * p, lb = la + 1;
-        const double2 z = buf[p * M + k + 1];
+        const double2 z = buf[lds_slot<kSwizzleAll<CM>>(p * M + k + 1)];
         double ya = -z.y, yb = z.x;
         const double cx = J.cosx[k];
         if (la < N2) {
@@ -354,12 +382,12 @@ __device__ inline void pass2_body(const EllJob& J, double2* smem, int bx) {
             const double lam = 2. * (cx + J.cosy[lb] - 2.) * J.bydx2;
             yb *= (J.kind == 0) ? 1. / lam : 1. / (1 + J.alpha * lam);
         }
-        buf[p * M + k + 1] = make_double2(ya, yb);
-        buf[p * M + M - 1 - k] = make_double2(-ya, -yb);
+        buf[lds_slot<kSwizzleAll<CM>>(p * M + k + 1)] = make_double2(ya, yb);
+        buf[lds_slot<kSwizzleAll<CM>>(p * M + M - 1 - k)] = make_double2(-ya, -yb);
     }
     for (int p = threadIdx.x; p < np; p += kFftThreads) {
-        buf[p * M] = make_double2(0.0, 0.0);
-        buf[p * M + n] = make_double2(0.0, 0.0);
+        buf[lds_slot<kSwizzleAll<CM>>(p * M)] = make_double2(0.0, 0.0);
+        buf[lds_slot<kSwizzleAll<CM>>(p * M + n)] = make_double2(0.0, 0.0);
     }
     __syncthreads();
     fft_any<CM>(buf, tw, P);
@@ -367,7 +395,7 @@ __device__ inline void pass2_body(const EllJob& J, double2* smem, int bx) {
         const int r = idx % rows, k = idx / rows;
         const int l = r0 + r;
         if (l >= N2) continue;
-        const double2 z = buf[(r >> 1) * M + k + 1];
+        const double2 z = buf[lds_slot<kSwizzleAll<CM>>((r >> 1) * M + k + 1)];
         J.T2[(long)k * N2 + l] = (r & 1) ? z.x : -z.y;
     }
 }
@@ -389,18 +417,18 @@ __device__ inline void pass3_body(const EllJob& J, double2* smem, int bx) {
         const int ra = r0 + 2 * p, rb = ra + 1;
         const double va = (ra < N1) ? J.T2[(long)ra * N2 + k] : 0.0;
         const double vb = (rb < N1) ? J.T2[(long)rb * N2 + k] : 0.0;
-        buf[p * M + k + 1] = make_double2(va, vb);
-        buf[p * M + M - 1 - k] = make_double2(-va, -vb);
+        buf[lds_slot<kSwizzleAll<CM>>(p * M + k + 1)] = make_double2(va, vb);
+        buf[lds_slot<kSwizzleAll<CM>>(p * M + M - 1 - k)] = make_double2(-va, -vb);
     }
     for (int p = threadIdx.x; p < np; p += kFftThreads) {
-        buf[p * M] = make_double2(0.0, 0.0);
-        buf[p * M + n] = make_double2(0.0, 0.0);
+        buf[lds_slot<kSwizzleAll<CM>>(p * M)] = make_double2(0.0, 0.0);
+        buf[lds_slot<kSwizzleAll<CM>>(p * M + n)] = make_double2(0.0, 0.0);
     }
     __syncthreads();
     fft_any<CM>(buf, tw, P);
     for (int idx = threadIdx.x; idx < np * N2; idx += kFftThreads) {
         const int p = idx / N2, k = idx - p * N2;
-        const double2 z = buf[p * M + k + 1];
+        const double2 z = buf[lds_slot<kSwizzleAll<CM>>(p * M + k + 1)];
         const int ra = r0 + 2 * p, rb = ra + 1;
         // sinTransformInv normalisation, EllipticSolver2d.cc:80-83
         // with the epilogue, the finer level's BCs must come from v itself

@@ -84,6 +84,14 @@ def load():
         "ibpm_batch_nn_params": [vp, vp, vp],
         "ibpm_nn_loss_grad": [vp, vp, vp, vp, ctypes.c_long, vp, _D],
         "ibpm_batch_set_precond": [vp, ctypes.c_int, ctypes.c_int],
+        "ibpm_rhs": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                     ctypes.c_double, ctypes.c_double, ctypes.c_double, vp, vp, vp, vp, vp],
+        "ibpm_flux_from_vorticity": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_double, vp, vp],
+        "ibpm_reg_create": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                            ctypes.c_double, ctypes.c_int, vp, ctypes.POINTER(ctypes.c_void_p)],
+        "ibpm_reg_update": [vp, vp], "ibpm_reg_destroy": [vp],
+        "ibpm_reg_to_flux": [vp, vp, vp], "ibpm_reg_to_boundary": [vp, vp, vp],
         "ibpm_batch_solver_stats": [vp, vp, ctypes.c_int],
     }.items():
         getattr(L, name).argtypes = args
@@ -205,6 +213,63 @@ class Context:
         d_n = DeviceArray(self, ngrid * (nx - 1) * (ny - 1))
         _check(load().ibpm_nonlinear(self.h, nx, ny, ngrid, dx, d_q.ptr, d_w.ptr, d_n.ptr))
         return d_n.get()
+
+    def rhs(self, nx, ny, ngrid, dx, dt, nu, an, bn, q, w, nprev=None):
+        """(a, N) of one substep (IBSolver::advanceSubstep)."""
+        ns = ngrid * (nx - 1) * (ny - 1)
+        d_q = DeviceArray.of(self, q)
+        d_w = DeviceArray.of(self, w)
+        d_p = DeviceArray.of(self, nprev) if nprev is not None else None
+        d_a = DeviceArray(self, ns)
+        d_n = DeviceArray(self, ns)
+        _check(load().ibpm_rhs(self.h, nx, ny, ngrid, dx, dt, nu, an, bn, d_q.ptr, d_w.ptr,
+                               d_p.ptr if d_p is not None else None, d_a.ptr, d_n.ptr))
+        return d_a.get(), d_n.get()
+
+    def flux_from_vorticity(self, nx, ny, ngrid, dx, w, u0=0.0, v0=0.0):
+        """NavierStokesModel::computeFlux."""
+        d_w = DeviceArray.of(self, w)
+        d_q = DeviceArray(self, ngrid * (2 * nx * ny + nx + ny))
+        _check(load().ibpm_flux_from_vorticity(self.h, nx, ny, ngrid, dx, u0, v0, d_w.ptr, d_q.ptr))
+        return d_q.get()
+
+
+class Regularizer:
+    """Regularizer of one set of points on the finest level
+    (Regularizer::update / toFlux / toBoundary)."""
+
+    def __init__(self, ctx, nx, ny, ngrid, length, xoffset, yoffset, xy):
+        self.ctx, self.nx, self.ny, self.ngrid = ctx, nx, ny, ngrid
+        xy = _c(xy)
+        self.npts = len(xy) // 2
+        self.h = ctypes.c_void_p()
+        _check(load().ibpm_reg_create(ctx.h, nx, ny, ngrid, length, xoffset, yoffset, self.npts,
+                                      xy.ctypes.data, ctypes.byref(self.h)))
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            load().ibpm_reg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def update(self, xy):
+        xy = _c(xy)
+        _check(load().ibpm_reg_update(self.h, xy.ctypes.data))
+        return self
+
+    def to_flux(self, f):
+        d_f = DeviceArray.of(self.ctx, f)
+        d_q = DeviceArray(self.ctx, self.ngrid * (2 * self.nx * self.ny + self.nx + self.ny))
+        _check(load().ibpm_reg_to_flux(self.h, d_f.ptr, d_q.ptr))
+        return d_q.get()
+
+    def to_boundary(self, q):
+        d_q = DeviceArray.of(self.ctx, q)
+        d_f = DeviceArray(self.ctx, 2 * self.npts)
+        _check(load().ibpm_reg_to_boundary(self.h, d_q.ptr, d_f.ptr))
+        return d_f.get()
 
 
 @dataclass

@@ -1212,7 +1212,7 @@ int ibpm_dst2d(ibpm_ctx* c, int n1, int n2, const double* d_in, double* d_out) {
 }
 
 static int solve_common(ibpm_ctx* c, int kind, int nx, int ny, int ngrid, double dx, double alpha,
-                        const double* d_f, const double* d_bc, double* d_u) {
+                        const double* d_f, const double* d_bc, double* d_u, double in_scale = 1.0) {
     CKR(check_grid(nx, ny, ngrid));
     DevMem mem;
     const long ns = (long)(nx - 1) * (ny - 1);
@@ -1226,7 +1226,7 @@ static int solve_common(ibpm_ctx* c, int kind, int nx, int ny, int ngrid, double
     CKR(cos_tables(mem, ny, &cosy));
     SolveSpec s{};
     s.nx = nx; s.ny = ny; s.L = ngrid; s.dx = dx; s.kind = kind; s.alpha = alpha;
-    s.f = fin; s.u = d_u; s.R = R; s.T1 = T1; s.T2 = T2; s.in_scale = 1.0; s.epilogue = 0;
+    s.f = fin; s.u = d_u; s.R = R; s.T1 = T1; s.T2 = T2; s.in_scale = in_scale; s.epilogue = 0;
     s.cosx = cosx; s.cosy = cosy; s.bcring = d_bc;
     SolvePlan P;
     CKR(build_solve_plan(&c->c, mem, {s}, P));
@@ -1343,6 +1343,124 @@ int ibpm_nonlinear(ibpm_ctx* c, int nx, int ny, int ngrid, double dx, const doub
     for (int lev = 0; lev < ngrid; ++lev) launch_cross(d, 1, nf1, lev, c->c.stream);
     launch_rhs(d + 1, 1, ns, c->c.stream);
     CK(hipStreamSynchronize(c->c.stream));
+    CK(hipGetLastError());
+    return 0;
+}
+
+int ibpm_rhs(ibpm_ctx* c, int nx, int ny, int ngrid, double dx, double dt, double nu, double an, double bn,
+             const double* d_q, const double* d_w, const double* d_nprev, double* d_a, double* d_n) {
+    DevMem mem;
+    FlowOps o{};
+    const long ns = (long)(nx - 1) * (ny - 1) * ngrid;
+    const long nf1 = 2L * nx * ny + nx + ny;
+    o.q = const_cast<double*>(d_q);
+    o.omega = const_cast<double*>(d_w);
+    o.a = d_a;
+    o.nprev = d_n;   // holds N_prev on entry (when given) and N on exit
+    o.lap_coef = 0.5 * nu * (an + bn); o.an = an; o.bn = bn; o.dt = dt; o.w_coef = 1.0;
+    o.use_nprev = bn != 0;
+    o.nprev_valid = d_nprev != nullptr;   // no N_prev yet: the reference uses N (IBSolver.cc:173-177)
+    CKR(ops_common(c, nx, ny, ngrid, dx, o, mem));
+    if (d_nprev && d_nprev != d_n)
+        CK(hipMemcpyAsync(d_n, d_nprev, sizeof(double) * ns, hipMemcpyDeviceToDevice, c->c.stream));
+    FlowOps* d = nullptr;
+    CKR(mem.alloc(&d, 1));
+    CK(hipMemcpyAsync(d, &o, sizeof(o), hipMemcpyHostToDevice, c->c.stream));
+    launch_vel(d, 1, ns, c->c.stream);
+    for (int lev = 0; lev < ngrid; ++lev) launch_cross(d, 1, nf1, lev, c->c.stream);
+    launch_rhs(d, 1, ns, c->c.stream);
+    CK(hipStreamSynchronize(c->c.stream));
+    CK(hipGetLastError());
+    return 0;
+}
+
+int ibpm_flux_from_vorticity(ibpm_ctx* c, int nx, int ny, int ngrid, double dx, double u0, double v0,
+                             const double* d_w, double* d_q) {
+    CKR(check_grid(nx, ny, ngrid));
+    DevMem mem;
+    double* psi = nullptr;
+    CKR(mem.alloc(&psi, (size_t)(nx - 1) * (ny - 1) * ngrid));
+    CKR(solve_common(c, 0, nx, ny, ngrid, dx, 0.0, d_w, nullptr, psi, -1.0));   // L psi = -coarsify(omega)
+    return ibpm_curl_scalar(c, nx, ny, ngrid, dx, u0, v0, psi, d_q);
+}
+
+}  // extern "C"
+
+// Standalone regulariser: association lists of one set of points.
+struct ibpm_reg {
+    ibpm_ctx* ctx = nullptr;
+    int nx = 0, ny = 0, ngrid = 0, nb = 0, cap = 0;
+    double dx = 0, x0 = 0, y0 = 0;
+    ibpmg::DevMem mem;
+    int *e_ptr = nullptr, *e_f = nullptr, *t_ptr = nullptr, *t_k = nullptr, *fos = nullptr;
+    double *e_w = nullptr, *t_w = nullptr;
+    ibpmg::RegFluxOps* d_ops = nullptr;
+    int nslots = 0;
+};
+
+extern "C" {
+
+int ibpm_reg_update(ibpm_reg* r, const double* h_xy) {
+    if (!r || !h_xy) { set_error("null argument"); return -1; }
+    std::vector<double> px(h_xy, h_xy + r->nb), py(h_xy + r->nb, h_xy + 2 * r->nb);
+    RegLists L;
+    build_reg_lists(r->nx, r->ny, r->dx, r->x0, r->y0, px, py, L);
+    if ((int)L.e_f.size() > r->cap || (int)L.t_k.size() > r->cap) {
+        set_error("regulariser list exceeds capacity");
+        return -3;
+    }
+    CKR(upload(r->e_ptr, L.e_ptr.data(), sizeof(int) * L.e_ptr.size()));
+    CKR(upload(r->e_f, L.e_f.data(), sizeof(int) * L.e_f.size()));
+    CKR(upload(r->e_w, L.e_w.data(), sizeof(double) * L.e_w.size()));
+    CKR(upload(r->t_ptr, L.t_ptr.data(), sizeof(int) * L.t_ptr.size()));
+    CKR(upload(r->t_k, L.t_k.data(), sizeof(int) * L.t_k.size()));
+    CKR(upload(r->t_w, L.t_w.data(), sizeof(double) * L.t_w.size()));
+    CKR(upload(r->fos, L.flux_of_slot.data(), sizeof(int) * L.flux_of_slot.size()));
+    r->nslots = (int)L.flux_of_slot.size();
+    RegFluxOps o{r->nb, r->nslots, r->dx, r->e_ptr, r->e_f, r->e_w, r->t_ptr, r->t_k, r->t_w, r->fos};
+    CKR(upload(r->d_ops, &o, sizeof(o)));
+    return 0;
+}
+
+int ibpm_reg_create(ibpm_ctx* c, int nx, int ny, int ngrid, double length, double xoffset, double yoffset,
+                    int npts, const double* h_xy, ibpm_reg** out) {
+    if (!c || !out || npts < 1 || !h_xy) { set_error("bad regulariser arguments"); return -1; }
+    CKR(check_grid(nx, ny, ngrid));
+    auto r = std::make_unique<ibpm_reg>();
+    r->ctx = c; r->nx = nx; r->ny = ny; r->ngrid = ngrid; r->nb = npts;
+    r->dx = length / nx; r->x0 = xoffset; r->y0 = yoffset;
+    r->cap = 18 * npts + 1;
+    const int n = 2 * npts;
+    CKR(r->mem.alloc(&r->e_ptr, n + 1)); CKR(r->mem.alloc(&r->e_f, r->cap)); CKR(r->mem.alloc(&r->e_w, r->cap));
+    CKR(r->mem.alloc(&r->t_ptr, r->cap + 1)); CKR(r->mem.alloc(&r->t_k, r->cap));
+    CKR(r->mem.alloc(&r->t_w, r->cap)); CKR(r->mem.alloc(&r->fos, r->cap));
+    CKR(r->mem.alloc(&r->d_ops, 1));
+    CKR(ibpm_reg_update(r.get(), h_xy));
+    *out = r.release();
+    return 0;
+}
+
+int ibpm_reg_destroy(ibpm_reg* r) {
+    delete r;
+    return 0;
+}
+
+int ibpm_reg_to_flux(ibpm_reg* r, const double* d_f, double* d_q) {
+    if (!r) { set_error("null regulariser"); return -1; }
+    hipStream_t s = r->ctx->c.stream;
+    const size_t nf = (size_t)(2L * r->nx * r->ny + r->nx + r->ny) * r->ngrid;
+    CK(hipMemsetAsync(d_q, 0, sizeof(double) * nf, s));   // coarse levels and untouched fluxes are 0
+    launch_reg_to_flux(r->d_ops, r->nslots, d_f, d_q, s);
+    CK(hipStreamSynchronize(s));
+    CK(hipGetLastError());
+    return 0;
+}
+
+int ibpm_reg_to_boundary(ibpm_reg* r, const double* d_q, double* d_f) {
+    if (!r) { set_error("null regulariser"); return -1; }
+    hipStream_t s = r->ctx->c.stream;
+    launch_reg_to_boundary(r->d_ops, r->nb, d_q, d_f, s);
+    CK(hipStreamSynchronize(s));
     CK(hipGetLastError());
     return 0;
 }

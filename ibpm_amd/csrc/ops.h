@@ -40,6 +40,17 @@ struct RegOps {
     const double* skip;            // null, or the flow's force-solve flags
 };
 
+// Standalone regulariser (E and E^T on the finest level's flux field).
+struct RegFluxOps {
+    int nb, nslots;
+    double dx;
+    const int *e_ptr, *e_f;        // E: per component, ascending flux indices
+    const double* e_w;
+    const int *t_ptr, *t_k;        // E^T: per flux slot, components in ascending order
+    const double* t_w;
+    const int* flux_of_slot;       // level-0 flux index of each slot
+};
+
 struct MinvOps {
     int n;
     const double* minv;            // n x n row-major
@@ -102,5 +113,7 @@ void launch_pcg_begin(const PcgOps* d, int nflows, hipStream_t s);
 void launch_pcg_step(const PcgOps* d, int nflows, hipStream_t s);
 void launch_pcg_dir(const PcgOps* d, int nflows, int begin, hipStream_t s);
 void launch_forces(const ForceOps* d, int nflows, hipStream_t s);
+void launch_reg_to_flux(const RegFluxOps* d, int nslots, const double* f, double* q, hipStream_t s);
+void launch_reg_to_boundary(const RegFluxOps* d, int nb, const double* q, double* f, hipStream_t s);
 
 }  // namespace ibpmg

@@ -321,6 +321,30 @@ __global__ void __launch_bounds__(kThreads) k_clear_nodes(const RegOps* __restri
     R.cout[R.old_idx[m]] = 0.0;
 }
 
+// Regularizer::toFlux (Regularizer.cc:92-109): q[flux] = dx * sum of w f[k]
+// over the components touching that flux, in ascending component order (the
+// reference's accumulation order for a fixed flux); one thread per flux slot,
+// every slot a distinct flux.  Fluxes no slot touches are left as they are.
+__global__ void __launch_bounds__(kThreads) k_reg_to_flux(const RegFluxOps* __restrict__ R, const double* f,
+                                                          double* q) {
+    const int s = blockIdx.x * kThreads + threadIdx.x;
+    if (s >= R->nslots) return;
+    double v = 0.0;
+    for (int a = R->t_ptr[s]; a < R->t_ptr[s + 1]; ++a) v += R->t_w[a] * f[R->t_k[a]];
+    q[R->flux_of_slot[s]] = v * R->dx;
+}
+
+// Regularizer::toBoundary (Regularizer.cc:111-128): f[k] = (sum of w q[flux]
+// in ascending flux order) / dx.
+__global__ void __launch_bounds__(kThreads) k_reg_to_boundary(const RegFluxOps* __restrict__ R, const double* q,
+                                                              double* f) {
+    const int k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= 2 * R->nb) return;
+    double v = 0.0;
+    for (int a = R->e_ptr[k]; a < R->e_ptr[k + 1]; ++a) v += R->e_w[a] * q[R->e_f[a]];
+    f[k] = v * (1.0 / R->dx);
+}
+
 // ------------------------------------------------------------ force solve
 // x = Minv * rhs with Minv = (L L^T)^-1 assembled in FP64 from the Cholesky
 // factor (CholeskySolver.cc:200-224 computes the same product by forward and
@@ -717,6 +741,13 @@ void launch_pcg_step(const PcgOps* d, int nflows, hipStream_t s) {
 }
 void launch_pcg_dir(const PcgOps* d, int nflows, int begin, hipStream_t s) {
     hipLaunchKernelGGL(k_pcg_dir, dim3(nflows), dim3(kThreads), 0, s, d, begin);
+}
+
+void launch_reg_to_flux(const RegFluxOps* d, int nslots, const double* f, double* q, hipStream_t s) {
+    hipLaunchKernelGGL(k_reg_to_flux, dim3(blocks_for(std::max(nslots, 1))), dim3(kThreads), 0, s, d, f, q);
+}
+void launch_reg_to_boundary(const RegFluxOps* d, int nb, const double* q, double* f, hipStream_t s) {
+    hipLaunchKernelGGL(k_reg_to_boundary, dim3(blocks_for(std::max(2 * nb, 1))), dim3(kThreads), 0, s, d, q, f);
 }
 
 void launch_forces(const ForceOps* d, int nflows, hipStream_t s) {

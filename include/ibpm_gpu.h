@@ -65,6 +65,32 @@ int ibpm_nonlinear(ibpm_ctx* ctx, int nx, int ny, int ngrid, double dx, const do
                    const double* d_w, double* d_n);
 /* Laplacian = -Curl(Curl(omega)), src/VectorOperations.cc:133-140 */
 int ibpm_laplacian(ibpm_ctx* ctx, int nx, int ny, int ngrid, double dx, const double* d_w, double* d_out);
+/* Right-hand side of one substep, IBSolver::advanceSubstep (src/IBSolver.cc:167-183) with
+   N = NonlinearIBSolver::N (:205-209):
+     N = Curl(CrossProduct(q, omega)),
+     a = omega + dt (0.5 nu (an + bn) Laplacian(omega) + an N + bn N_prev).
+   d_nprev NULL means no previous N yet (the reference then uses N itself).
+   Writes d_a and d_n (d_n may alias d_nprev). */
+int ibpm_rhs(ibpm_ctx* ctx, int nx, int ny, int ngrid, double dx, double dt, double nu, double an, double bn,
+             const double* d_q, const double* d_w, const double* d_nprev, double* d_a, double* d_n);
+/* NavierStokesModel::computeFlux (src/NavierStokesModel.cc:114-124): psi solves
+   L psi = -coarsify(omega) over all levels (:131-139), q = Curl(psi) + q0 with the
+   uniform base flow (u0, v0) * dx_lev (src/Flux.cc:79-99). */
+int ibpm_flux_from_vorticity(ibpm_ctx* ctx, int nx, int ny, int ngrid, double dx, double u0, double v0,
+                             const double* d_w, double* d_q);
+
+/* ---- Regularizer (finest level; src/Regularizer.cc:31-128) ----------------- */
+typedef struct ibpm_reg ibpm_reg;
+/* Regularizer(grid, geometry) + update() for npts points h_xy = [x_0.., y_0..]. */
+int ibpm_reg_create(ibpm_ctx* ctx, int nx, int ny, int ngrid, double length, double xoffset, double yoffset,
+                    int npts, const double* h_xy, ibpm_reg** out);
+/* Regularizer::update after the points moved (same count). */
+int ibpm_reg_update(ibpm_reg* reg, const double* h_xy);
+int ibpm_reg_destroy(ibpm_reg* reg);
+/* Regularizer::toFlux: d_q (all levels) = 0 except level 0 = E^T f * dx (Regularizer.cc:92-109). */
+int ibpm_reg_to_flux(ibpm_reg* reg, const double* d_f, double* d_q);
+/* Regularizer::toBoundary: d_f = E q / dx from level 0 of d_q (Regularizer.cc:111-128). */
+int ibpm_reg_to_boundary(ibpm_reg* reg, const double* d_q, double* d_f);
 
 /* ---- geometry ------------------------------------------------------------- */
 /* Number of boundary points of a .geom text (Geometry::getNumPoints after

@@ -195,3 +195,58 @@ def test_restart_resume_is_bitwise_identical(gpu):
     b.advance(3)
     assert np.array_equal(a.omega(), b.omega())
     assert np.array_equal(a.f(), b.f())
+
+
+def test_rhs_matches_oracle_composition(ctx, oracle):
+    """a = omega + dt (0.5 nu (an + bn) Lap omega + an N + bn N_prev)
+    (IBSolver.cc:167-183) from the oracle's pinned Laplacian and N, with and
+    without a previous N (without one the reference uses N itself)."""
+    nx, ny, ngrid, dx = 32, 24, 3, 0.125
+    rng = np.random.default_rng(11)
+    q = rng.standard_normal(ngrid * (2 * nx * ny + nx + ny))
+    w = rng.standard_normal(ngrid * (nx - 1) * (ny - 1))
+    npv = rng.standard_normal(w.size)
+    dt, nu, an, bn = 0.01, 1 / 200.0, 5 / 12, -17 / 60
+    lap = oracle.laplacian(nx, ny, ngrid, dx, w)
+    n = oracle.nonlinear(nx, ny, ngrid, dx, q, w)
+    for prev in (npv, None):
+        a, n_g = ctx.rhs(nx, ny, ngrid, dx, dt, nu, an, bn, q, w, prev)
+        p = npv if prev is not None else n
+        a_o = w + dt * (0.5 * nu * (an + bn) * lap + an * n + bn * p)
+        assert rel(n_g, n) < 1e-13
+        assert rel(a, a_o) < 1e-13
+
+
+def test_flux_from_vorticity_matches_oracle_refresh(ctx, oracle):
+    """q = Curl(Poisson(-coarsify omega)) + q0, NavierStokesModel::computeFlux,
+    against the oracle's refreshState on the same vorticity."""
+    nx, ngrid = 48, 3
+    s = oracle.Sim(nx, nx, ngrid, 4.0, -2.0, -2.0, 100.0, 0.02, CYL40, alpha_deg=20.0)
+    s.init()
+    w = np.random.default_rng(12).standard_normal(ngrid * (nx - 1) ** 2)
+    s.set_omega(w)
+    al = math.radians(20.0)
+    q = ctx.flux_from_vorticity(nx, nx, ngrid, 4.0 / nx, w, math.cos(al), math.sin(al))
+    assert rel(q, s.q()) < 1e-12
+
+
+@pytest.mark.parametrize("geom", [CYL40, plate_geom(21, 1.0, 0.2, 0.2, 0.4)])
+def test_regularizer_matches_oracle(gpu, ctx, oracle, geom):
+    """toFlux and toBoundary (Regularizer.cc:92-128) against the oracle's
+    association lists (pinned bit-exactly to the reference's), before and
+    after the points move (update)."""
+    nx, ny, ngrid, length, x0, y0 = 64, 64, 2, 4.0, -2.0, -2.0
+    dx = length / nx
+    rng = np.random.default_rng(13)
+    xy, _ = oracle.geom_points(geom, 0.0)
+    reg = gpu.Regularizer(ctx, nx, ny, ngrid, length, x0, y0, xy)
+    for t in (0.0, 0.37):
+        xy, _ = oracle.geom_points(geom, t)
+        if t:
+            reg.update(xy)
+        nb = len(xy) // 2
+        assoc = oracle.reg_build(nx, ny, dx, x0, y0, xy[:nb], xy[nb:])
+        f = rng.standard_normal(2 * nb)
+        q = rng.standard_normal(ngrid * (2 * nx * ny + nx + ny))
+        assert rel(reg.to_flux(f), oracle.reg_toflux(nx, ny, ngrid, dx, assoc, f)) < 1e-14
+        assert rel(reg.to_boundary(q), oracle.reg_toboundary(nb, dx, assoc, q)) < 1e-14

@@ -84,6 +84,8 @@ def load():
         "ibpm_batch_nn_params": [vp, vp, vp],
         "ibpm_nn_loss_grad": [vp, vp, vp, vp, ctypes.c_long, vp, _D],
         "ibpm_batch_set_precond": [vp, ctypes.c_int, ctypes.c_int],
+        "ibpm_batch_cholesky_cache": [vp, ctypes.c_int, ctypes.c_char_p],
+        "ibpm_batch_cholesky_loaded": [vp, ctypes.c_int],
         "ibpm_rhs": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                      ctypes.c_double, ctypes.c_double, ctypes.c_double, vp, vp, vp, vp, vp],
         "ibpm_flux_from_vorticity": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
@@ -317,6 +319,15 @@ class Batch:
     def init(self):
         _check(load().ibpm_batch_init(self.h))
         return self
+
+    def cholesky_cache(self, basename, k=0):
+        """Before init(): read the flow's Cholesky factors from the reference's
+        <basename>_0i.cholesky files, or build and write them."""
+        _check(load().ibpm_batch_cholesky_cache(self.h, k, basename.encode() if basename else None))
+        return self
+
+    def cholesky_loaded(self, k=0):
+        return _check(load().ibpm_batch_cholesky_loaded(self.h, k))
 
     def use_graph(self, on=True):
         _check(load().ibpm_batch_use_graph(self.h, int(on)))

@@ -331,6 +331,8 @@ struct Flow {
     double *lmp_s[4][2] = {}, *lmp_y[4][2] = {};
     int* lmp_cnt = nullptr;
     double pbeta = 0.0;            // beta of the substep P was built for
+    std::string chol_cache;        // basename of the .cholesky factor files ("" = none)
+    int chol_loaded = 0;           // substep factors read from those files
     // regulariser
     RegLists reg;
     int *e_ptr, *e_f, *t_ptr, *t_k, *node_idx, *old_idx;
@@ -913,39 +915,100 @@ static int substep_cg(ibpm_batch* B, int i) {
     return 0;
 }
 
-// In place: Mt (M column-major, Mt[c*n + r] = M(r, c)) -> M^-1 row-major.
-// Factor L L^T from M's upper triangle exactly as CholeskySolver.cc:89-109,
-// then apply CholeskySolver::Minv (:200-224) to every unit vector.  Returns
-// -1, or the row where the matrix is not positive definite.
-static int cholesky_inverse(int n, std::vector<double>& Mt) {
-    std::vector<double> L((size_t)n * n, 0.0), d(n), x(n);
+// Cholesky factor M = L L^T from M's upper triangle exactly as
+// CholeskySolver.cc:89-109 (descending-k sums): d = diagonal of L, lower =
+// strict lower triangle, row-major (lower[r*n + c], c < r).  Mt is M
+// column-major (Mt[c*n + r] = M(r, c)).  Returns -1, or the row where the
+// matrix is not positive definite.
+static int cholesky_factor(int n, const std::vector<double>& Mt, std::vector<double>& lower, std::vector<double>& d) {
+    lower.assign((size_t)n * n, 0.0);
+    d.assign(n, 0.0);
     auto M = [&](int r, int c) { return Mt[(size_t)c * n + r]; };
     for (int r = 0; r < n; ++r) {
         for (int c = r; c < n; ++c) {
             double sum = M(r, c);
-            for (int k = r - 1; k >= 0; --k) sum -= L[(size_t)r * n + k] * L[(size_t)c * n + k];
+            for (int k = r - 1; k >= 0; --k) sum -= lower[(size_t)r * n + k] * lower[(size_t)c * n + k];
             if (c == r) {
                 if (!(sum > 0)) return r;
                 d[r] = std::sqrt(sum);
             } else {
-                L[(size_t)c * n + r] = sum / d[r];
+                lower[(size_t)c * n + r] = sum / d[r];
             }
         }
     }
+    return -1;
+}
+
+// M^-1 row-major: CholeskySolver::Minv (:200-224) applied to every unit vector.
+static void cholesky_assemble(int n, const std::vector<double>& lower, const std::vector<double>& d,
+                              std::vector<double>& out) {
+    out.assign((size_t)n * n, 0.0);
+    std::vector<double> x(n);
     for (int col = 0; col < n; ++col) {
         for (int r = 0; r < n; ++r) {
             double sum = (r == col) ? 1.0 : 0.0;
-            for (int k = r - 1; k >= 0; --k) sum -= L[(size_t)r * n + k] * x[k];
+            for (int k = r - 1; k >= 0; --k) sum -= lower[(size_t)r * n + k] * x[k];
             x[r] = sum / d[r];
         }
         for (int r = n - 1; r >= 0; --r) {
             double sum = x[r];
-            for (int k = r + 1; k < n; ++k) sum -= L[(size_t)k * n + r] * x[k];
+            for (int k = r + 1; k < n; ++k) sum -= lower[(size_t)k * n + r] * x[k];
             x[r] = sum / d[r];
         }
-        for (int r = 0; r < n; ++r) Mt[(size_t)r * n + col] = x[r];
+        for (int r = 0; r < n; ++r) out[(size_t)r * n + col] = x[r];
     }
+}
+
+// In place: Mt (M column-major) -> M^-1 row-major.
+static int cholesky_inverse(int n, std::vector<double>& Mt) {
+    std::vector<double> lower, d;
+    const int bad = cholesky_factor(n, Mt, lower, d);
+    if (bad >= 0) return bad;
+    cholesky_assemble(n, lower, d, Mt);
     return -1;
+}
+
+// The reference's factor cache <basename>_<substep, 2 digits>.cholesky
+// (IBSolver.cc:77-108): size, alpha*beta, the diagonal, "#", then the strict
+// lower triangle row by row, one value per line at 17 significant digits
+// (CholeskySolver.cc:162-197).  Reading succeeds only for the same size and
+// the same alpha*beta (CholeskySolver.cc:113-158).
+static std::string cholesky_path(const std::string& base, int substep) {
+    char num[16];
+    std::snprintf(num, sizeof num, "_%02d", substep + 1);
+    return base + num + ".cholesky";
+}
+
+static bool cholesky_write(const std::string& path, int n, double alpha_beta, const std::vector<double>& lower,
+                           const std::vector<double>& d) {
+    FILE* fp = std::fopen(path.c_str(), "w");
+    if (!fp) return false;
+    std::fprintf(fp, "%d\n%.17g\n", n, alpha_beta);
+    for (int i = 0; i < n; ++i) std::fprintf(fp, "%.17g\n", d[i]);
+    std::fprintf(fp, "#\n");
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < i; ++j) std::fprintf(fp, "%.17g\n", lower[(size_t)i * n + j]);
+    return std::fclose(fp) == 0;
+}
+
+static bool cholesky_read(const std::string& path, int n, double alpha_beta, std::vector<double>& lower,
+                          std::vector<double>& d) {
+    FILE* fp = std::fopen(path.c_str(), "r");
+    if (!fp) return false;
+    int nin = 0;
+    double ab = 0;
+    bool ok = std::fscanf(fp, "%d", &nin) == 1 && nin == n && std::fscanf(fp, "%lf", &ab) == 1 && ab == alpha_beta;
+    if (ok) {
+        d.assign(n, 0.0);
+        lower.assign((size_t)n * n, 0.0);
+        for (int i = 0; ok && i < n; ++i) ok = std::fscanf(fp, "%lf", &d[i]) == 1;
+        char c = 0;
+        ok = ok && std::fscanf(fp, " %c", &c) == 1 && c == '#';
+        for (int i = 0; ok && i < n; ++i)
+            for (int j = 0; ok && j < i; ++j) ok = std::fscanf(fp, "%lf", &lower[(size_t)i * n + j]) == 1;
+    }
+    std::fclose(fp);
+    return ok;
 }
 
 // M of substep i for the current body positions, column by column on the GPU
@@ -960,7 +1023,19 @@ static int build_inverse(ibpm_batch* B, int i, bool precond) {
     int nmax = 0;
     for (auto& F : B->flows) nmax = std::max(nmax, 2 * F.nb);
     static const double one = 1.0, zero = 0.0;
-    for (int j = 0; j < nmax; ++j) {
+    // factors of flows with a cache file that matches (Cholesky force solve only)
+    std::vector<std::vector<double>> lower(nfl), diag(nfl);
+    std::vector<char> cached(nfl, 0);
+    bool all_cached = !precond;
+    for (int k = 0; k < nfl; ++k) {
+        Flow& F = B->flows[k];
+        const double ab = (1. / F.p.reynolds) * beta_of(B, F, i);   // CholeskySolver.cc:38
+        cached[k] = !precond && !F.chol_cache.empty() &&
+                    cholesky_read(cholesky_path(F.chol_cache, i), 2 * F.nb, ab, lower[k], diag[k]);
+        if (cached[k]) F.chol_loaded += 1;
+        else all_cached = false;
+    }
+    for (int j = 0; j < (all_cached ? 0 : nmax); ++j) {
         std::vector<RegOps> sm(nfl), ip(nfl);
         for (int k = 0; k < nfl; ++k) {
             Flow& F = B->flows[k];
@@ -988,17 +1063,35 @@ static int build_inverse(ibpm_batch* B, int i, bool precond) {
         Flow& F = B->flows[k];
         const int n = 2 * F.nb;
         minv_h[k].assign((size_t)n * n, 0.0);
+        if (cached[k]) continue;
         CK(hipMemcpy(minv_h[k].data(), F.mcol, sizeof(double) * n * n, hipMemcpyDeviceToHost));
         CK(hipMemset(F.unit, 0, sizeof(double) * (n + 1)));
     }
     std::vector<std::thread> th;
+    std::vector<char> saved(nfl, 1);
     const int hw = std::max(1u, std::thread::hardware_concurrency());
     const int nthreads = std::max(1, std::min({nfl, hw, 16}));
     for (int t = 0; t < nthreads; ++t)
         th.emplace_back([&, t] {
-            for (int k = t; k < nfl; k += nthreads) bad[k] = cholesky_inverse(2 * B->flows[k].nb, minv_h[k]);
+            for (int k = t; k < nfl; k += nthreads) {
+                Flow& F = B->flows[k];
+                const int n = 2 * F.nb;
+                if (!cached[k]) {
+                    bad[k] = cholesky_factor(n, minv_h[k], lower[k], diag[k]);
+                    if (bad[k] >= 0) continue;
+                    if (!precond && !F.chol_cache.empty())   // IBSolver::save after init (ibpm.cc:291-296)
+                        saved[k] = cholesky_write(cholesky_path(F.chol_cache, i), n,
+                                                  (1. / F.p.reynolds) * beta_of(B, F, i), lower[k], diag[k]);
+                }
+                cholesky_assemble(n, lower[k], diag[k], minv_h[k]);
+            }
         });
     for (auto& x : th) x.join();
+    for (int k = 0; k < nfl; ++k)
+        if (!saved[k]) {
+            set_error("could not write %s", cholesky_path(B->flows[k].chol_cache, i).c_str());
+            return -1;
+        }
     for (int k = 0; k < nfl; ++k) {
         if (bad[k] >= 0) {
             set_error("Cholesky: flow %d, matrix not positive definite at %d", k, bad[k]);
@@ -1724,6 +1817,19 @@ int ibpm_batch_set_warm_start(ibpm_batch* B, int mode) {
     if (mode == 2 && !B->nn_ready) { set_error("learned warm start requested before training"); return -1; }
     B->warm_mode = mode;
     return 0;
+}
+
+int ibpm_batch_cholesky_cache(ibpm_batch* B, int k, const char* basename) {
+    Flow* F = flow_of(B, k);
+    if (!F) return -1;
+    if (B->initialized) { set_error("the factor cache must be named before ibpm_batch_init"); return -1; }
+    F->chol_cache = basename ? basename : "";
+    return 0;
+}
+
+int ibpm_batch_cholesky_loaded(ibpm_batch* B, int k) {
+    Flow* F = flow_of(B, k);
+    return F ? F->chol_loaded : -1;
 }
 
 int ibpm_batch_set_precond(ibpm_batch* B, int mode, int pairs) {

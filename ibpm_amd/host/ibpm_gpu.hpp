@@ -177,8 +177,16 @@ public:
     // plain CG, 1 CG preconditioned by a stale FP64 inverse of M with `pairs`
     // low-rank pairs (ibpm_batch_set_precond).  Before init().
     void setPrecond(int mode, int pairs) { _pc_mode = mode; _pc_pairs = pairs; }
-    bool load(const std::string&) { return false; }   // the factor is rebuilt on the GPU at init()
-    bool save(const std::string&) { return true; }
+    // IBSolver::load/save (src/IBSolver.cc:77-108): the Cholesky factors live
+    // in <basename>_0i.cholesky.  load() only names the cache and returns false;
+    // init() then reads matching files or builds the factors and writes them,
+    // which is what the reference's load-else-init-and-save sequence does.
+    bool load(const std::string& basename) {
+        _cache = basename;
+        return false;
+    }
+    bool save(const std::string&) { return true; }   // written by init()
+    int factorsLoaded() const { return _batch ? ibpm_batch_cholesky_loaded(_batch, 0) : 0; }
     void init() {
         if (_batch) return;
         gpu_check(ibpm_ctx_create(_device, &_ctx));
@@ -190,6 +198,8 @@ public:
         p.cg_tol = _tol; p.force_cg = 0;
         p.geom = _model.geometry().text().c_str();
         gpu_check(ibpm_batch_create(_ctx, 1, &p, &_batch));
+        if (!_cache.empty() && !_model.geTimeDependent())
+            gpu_check(ibpm_batch_cholesky_cache(_batch, 0, _cache.c_str()));
         gpu_check(ibpm_batch_init(_batch));
         if (!_model.geTimeDependent()) gpu_check(ibpm_batch_use_graph(_batch, 1));
         else if (_pc_mode) gpu_check(ibpm_batch_set_precond(_batch, _pc_mode, _pc_pairs));
@@ -234,6 +244,7 @@ private:
     double _tol;
     int _device;
     int _pc_mode = 0, _pc_pairs = 0;
+    std::string _cache;
     ibpm_ctx* _ctx = nullptr;
     ibpm_batch* _batch = nullptr;
 };

@@ -112,7 +112,10 @@ int main(int argc, char** argv) {
     State x(grid, geom.getNumPoints());
     nsm.init();
     try {
-        solver.init();
+        // load the factors or compute and save them (reference ibpm.cc:291-296)
+        if (!solver.load(outdir + name)) solver.init();
+        if (solver.factorsLoaded() > 0)
+            std::cout << "Loaded Cholesky factors from " << outdir << name << "_0*.cholesky" << std::endl;
         if (!icFile.empty()) {
             std::cout << "Loading initial condition from file: " << icFile << std::endl;
             if (x.load(icFile)) solver.setState(x);

@@ -134,6 +134,15 @@ int ibpm_batch_cg_iters(ibpm_batch* b, int flow, int* h_iters, int max_substeps)
 double ibpm_batch_time(ibpm_batch* b);
 /* Capture one timestep as a hipGraph and replay it (stationary bodies). */
 int ibpm_batch_use_graph(ibpm_batch* b, int enable);
+/* Cholesky factor cache of a stationary-body flow, named before ibpm_batch_init:
+   init reads each substep's factor from <basename>_01.cholesky, _02, ... when the
+   file matches (size and alpha*beta) and otherwise builds the factor and writes
+   the file -- the reference's load-else-init-and-save (src/ibpm.cc:291-296,
+   IBSolver::load/save src/IBSolver.cc:77-108, file format CholeskySolver::load/save
+   src/CholeskySolver.cc:113-197).  basename NULL or "" turns the cache off. */
+int ibpm_batch_cholesky_cache(ibpm_batch* b, int flow, const char* basename);
+/* Number of substep factors init read from cache files (0 = built). */
+int ibpm_batch_cholesky_loaded(ibpm_batch* b, int flow);
 
 /* ---- restart files ---------------------------------------------------------
    The reference's binary restart layout (State::save / State::load,

@@ -34,7 +34,7 @@ class FlowParams(ctypes.Structure):
                 ("length", ctypes.c_double), ("xoffset", ctypes.c_double), ("yoffset", ctypes.c_double),
                 ("reynolds", ctypes.c_double), ("dt", ctypes.c_double), ("scheme", ctypes.c_int),
                 ("alpha_deg", ctypes.c_double), ("cg_tol", ctypes.c_double), ("force_cg", ctypes.c_int),
-                ("geom", ctypes.c_char_p)]
+                ("geom", ctypes.c_char_p), ("ubf", ctypes.c_int)]
 
 
 def load():
@@ -74,6 +74,8 @@ def load():
         "ibpm_batch_get_points": [vp, ctypes.c_int, vp],
         "ibpm_batch_get_forces": [vp, ctypes.c_int, _D, _D],
         "ibpm_batch_force_history": [vp, ctypes.c_int, vp, ctypes.c_int],
+        "ibpm_batch_base_flow": [vp, ctypes.c_int, _D, _D],
+        "ibpm_geom_points": [ctypes.c_char_p, ctypes.c_int, vp, vp, vp, ctypes.c_long],
         "ibpm_batch_cg_iters": [vp, ctypes.c_int, vp, ctypes.c_int],
         "ibpm_batch_use_graph": [vp, ctypes.c_int],
         "ibpm_batch_kernel_stats": [vp, ctypes.c_int, vp, vp, vp, ctypes.c_int],
@@ -290,11 +292,12 @@ class Flow:
     alpha: float = 0.0
     tol: float = 1e-7
     force_cg: bool = False
+    ubf: bool = False          # unsteady base flow (ibpm -ubf): solve in the frame of the only body
 
     def params(self):
         return FlowParams(self.nx, self.ny, self.ngrid, self.length, self.xoffset, self.yoffset,
                           self.Re, self.dt, SCHEMES[self.scheme], self.alpha, self.tol,
-                          int(self.force_cg), self.geom.encode())
+                          int(self.force_cg), self.geom.encode(), int(self.ubf))
 
 
 class Batch:
@@ -376,6 +379,13 @@ class Batch:
         out = np.empty(2 * max_steps)
         n = _check(load().ibpm_batch_force_history(self.h, k, out.ctypes.data, max_steps))
         return out[:2 * n].reshape(n, 2)
+
+    def base_flow(self, k=0):
+        """(alpha in radians, magnitude) that rotate the current force into
+        drag and lift, and whether the base flow is unsteady."""
+        al, mag = ctypes.c_double(), ctypes.c_double()
+        unsteady = _check(load().ibpm_batch_base_flow(self.h, k, ctypes.byref(al), ctypes.byref(mag)))
+        return al.value, mag.value, bool(unsteady)
 
     def cg_iters(self, k=0):
         it = np.zeros(4, dtype=np.int32)

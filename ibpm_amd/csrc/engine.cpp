@@ -346,6 +346,13 @@ struct Flow {
     // learned warm start: ring of the final force and the constraint of the
     // last kRing substeps, features and scale of the current substep
     double *ring_f = nullptr, *ring_b = nullptr, *nn_x = nullptr, *nn_scale = nullptr;
+    // unsteady base flow (ubf): the transferred motion, its u, v, w on the
+    // device, and the angle that rotates the force after each step
+    bool ubf = false;
+    BaseFlowMotion bf;
+    double* bf_dev = nullptr;
+    double force_alpha = 0.0;
+    std::vector<double> alpha_hist;
 };
 
 }  // namespace ibpmg
@@ -360,7 +367,7 @@ struct ibpm_batch {
     ibpmg::DevMem mem;
     int nsub = 3;
     double an[4], bn[4], cn[4];
-    bool all_stationary = true, any_cg = false, initialized = false;
+    bool all_stationary = true, any_cg = false, initialized = false, any_ubf = false;
     bool old_saved = false;
     double time = 0.0;
     int timestep = 0;
@@ -438,8 +445,51 @@ static void scheme_coefs(int scheme, ibpm_batch* B) {
 
 static double beta_of(const ibpm_batch* B, const Flow& F, int i) { return (B->an[i] + B->bn[i]) * F.p.dt; }
 
+// The constraint b = u_body - E q0 (NavierStokesModel.cc:86-91,142-146) with
+// q0 on level 0: the uniform flow, or the unsteady base flow at its last
+// update (Flux.cc:101-129 at the edge coordinates of Grid.cc:104-139).
+static int upload_constraint(Flow& F) {
+    const RegLists& R = F.reg;
+    const double pi = 4. * std::atan(1.);
+    const double al = F.p.alpha_deg * pi / 180.;
+    double sal, cal;
+    sin_cos(al, sal, cal);
+    const double u0 = 1 * cal * F.dx, v0 = 1 * sal * F.dx;
+    const long nfx = (long)(F.nx + 1) * F.ny;
+    auto q0 = [&](int fidx) {
+        if (!F.ubf) return fidx < nfx ? u0 : v0;
+        if (fidx < nfx) {   // X flux (i, j): x edge i, y centre j
+            const int i = fidx / F.ny, j = fidx % F.ny;
+            return F.bf.flux(0, F.p.xoffset + i * F.dx, F.p.yoffset + (j + 0.5) * F.dx, F.dx);
+        }
+        const int u = fidx - (int)nfx, i = u / (F.ny + 1), j = u % (F.ny + 1);
+        return F.bf.flux(1, F.p.xoffset + (i + 0.5) * F.dx, F.p.yoffset + j * F.dx, F.dx);
+    };
+    std::vector<double> b(2 * F.nb);
+    for (int k = 0; k < 2 * F.nb; ++k) {
+        double s = 0;
+        for (int a = R.e_ptr[k]; a < R.e_ptr[k + 1]; ++a) s += R.e_w[a] * q0(R.e_f[a]);
+        s *= 1.0 / F.dx;
+        const double vel = k < F.nb ? F.pu[k] : F.pv[k - F.nb];
+        b[k] = vel - s;
+    }
+    CKR(upload(F.b, b.data(), sizeof(double) * b.size()));
+    return 0;
+}
+
+// BaseFlow::moveFlow(t) of a flow with an unsteady base flow
+// (NavierStokesModel::updateOperators, NavierStokesModel.cc:93-99): the new
+// u, v, w on the device and the constraint for them.
+static int move_base_flow(ibpm_batch* B, Flow& F, double t) {
+    F.bf.move(t);
+    CK(hipStreamSynchronize(B->ctx->c.stream));
+    const double uvw[3] = {F.bf.u, F.bf.v, F.bf.w};
+    CKR(upload(F.bf_dev, uvw, sizeof uvw));
+    return upload_constraint(F);
+}
+
 // Regulariser lists of a flow for its current points; uploads them and the
-// constraint b = u_body - E q0 (NavierStokesModel.cc:86-91,142-146).
+// constraint b = u_body - E q0.
 static int update_regularizer(ibpm_batch* B, Flow& F) {
     RegLists& R = F.reg;
     const int old_nodes = F.nnodes;
@@ -461,22 +511,8 @@ static int update_regularizer(ibpm_batch* B, Flow& F) {
     CKR(upload(F.old_idx, old.data(), sizeof(int) * old.size()));
     F.nold = old_nodes;
     F.nnodes = (int)R.node_idx.size();
-    // b = velocities - E(q0) with q0 the uniform flow on level 0
-    const double pi = 4. * std::atan(1.);
-    const double al = F.p.alpha_deg * pi / 180.;
-    const double u0 = 1 * std::cos(al) * F.dx, v0 = 1 * std::sin(al) * F.dx;
-    const long nfx = (long)(F.nx + 1) * F.ny;
-    std::vector<double> b(2 * F.nb);
-    for (int k = 0; k < 2 * F.nb; ++k) {
-        double s = 0;
-        for (int a = R.e_ptr[k]; a < R.e_ptr[k + 1]; ++a) s += R.e_w[a] * (R.e_f[a] < nfx ? u0 : v0);
-        s *= 1.0 / F.dx;
-        const double vel = k < F.nb ? F.pu[k] : F.pv[k - F.nb];
-        b[k] = vel - s;
-    }
-    CKR(upload(F.b, b.data(), sizeof(double) * b.size()));
     (void)B;
-    return 0;
+    return upload_constraint(F);
 }
 
 static RegOps make_regops(const Flow& F, double beta) {
@@ -608,8 +644,14 @@ static int build_plans(ibpm_batch* B) {
             o.nprev_valid = 0;
             const double pi = 4. * std::atan(1.);
             const double al = F.p.alpha_deg * pi / 180.;
-            o.ubase = 1 * std::cos(al);
-            o.vbase = 1 * std::sin(al);
+            double sal, cal;
+            sin_cos(al, sal, cal);
+            o.ubase = 1 * cal;
+            o.vbase = 1 * sal;
+            o.ubf = F.ubf ? 1 : 0;
+            o.bf = F.bf_dev;
+            o.xc = F.bf.xc; o.yc = F.bf.yc;
+            o.xoff = F.p.xoffset; o.yoff = F.p.yoffset;
             B->h_fops[i].push_back(o);
         }
         CKR(B->mem.alloc(&B->d_fops[i], nfl));
@@ -750,6 +792,20 @@ static int read_states(ibpm_batch* B, std::vector<double>& st) {
     return 0;
 }
 
+// q = M x0 for r0 = b - q.  A zero start (warm start mode 0) needs no M
+// application: q is cleared instead.
+static int start_residual(ibpm_batch* B, int i) {
+    hipStream_t s = B->ctx->c.stream;
+    if (B->warm_mode == 0) {
+        for (auto& F : B->flows) CK(hipMemsetAsync(F.cg_q, 0, sizeof(double) * 2 * F.nb, s));
+        return 0;
+    }
+    for (auto& F : B->flows) CK(hipMemcpyAsync(F.cg_d, F.f, sizeof(double) * 2 * F.nb, hipMemcpyDeviceToDevice, s));
+    enqueue_M(B, i, B->d_smear_d[i], s);
+    B->st_mapps += 1;
+    return 0;
+}
+
 // Plain CG from x0 = f (ConjugateGradientSolver.cc:35-65): iterations in
 // chunks, a device flag per flow; work for converged flows is skipped.
 static int solve_cg(ibpm_batch* B, int i, int* iters) {
@@ -757,9 +813,7 @@ static int solve_cg(ibpm_batch* B, int i, int* iters) {
     const int nfl = (int)B->flows.size();
     Probe* pr = B->probe;
     CKR(reset_states(B));
-    for (auto& F : B->flows) CK(hipMemcpyAsync(F.cg_d, F.f, sizeof(double) * 2 * F.nb, hipMemcpyDeviceToDevice, s));
-    enqueue_M(B, i, B->d_smear_d[i], s);   // q = M x0
-    B->st_mapps += 1;
+    CKR(start_residual(B, i));
     timed(pr, K_CG, 0, s, [&] { launch_cg(B->d_cg, nfl, 0, s); });
     std::vector<double> st;
     std::vector<char> active(nfl, 1);
@@ -794,9 +848,7 @@ static int solve_pcg(ibpm_batch* B, int i, int* iters, bool* fallback) {
     const int par = B->pc_par[i];
     PcgOps* ops = B->d_pcg[i][par];
     CKR(reset_states(B));
-    for (auto& F : B->flows) CK(hipMemcpyAsync(F.cg_d, F.f, sizeof(double) * 2 * F.nb, hipMemcpyDeviceToDevice, s));
-    enqueue_M(B, i, B->d_smear_d[i], s);   // q = M x0
-    B->st_mapps += 1;
+    CKR(start_residual(B, i));
     auto precondition = [&](int begin) {
         timed(pr, K_MINV, 0, s, [&] { launch_minv(B->d_pminv[i], nfl, B->max_n, s); });
         timed(pr, K_CG, 0, s, [&] { launch_pcg_dir(ops, nfl, begin, s); });
@@ -849,6 +901,7 @@ static int substep_cg(ibpm_batch* B, int i) {
             CK(hipStreamSynchronize(s));
             CKR(update_regularizer(B, F));
         }
+        if (F.ubf) CKR(move_base_flow(B, F, B->time + B->cn[i] * F.p.dt));
     }
     CKR(upload_regops(B));
     Probe* pr = B->probe;
@@ -1208,6 +1261,41 @@ int ibpm_geom_num_points(const char* geom_text) {
     std::string err;
     if (!g.parse(geom_text ? geom_text : "", err)) { set_error("%s", err.c_str()); return -1; }
     return g.npoints;
+}
+
+int ibpm_geom_info(const char* geom_text, int* npoints, int* nbodies, int* stationary, double* xc0, double* yc0) {
+    Geometry g;
+    std::string err;
+    if (!g.parse(geom_text ? geom_text : "", err)) { set_error("%s", err.c_str()); return -1; }
+    if (npoints) *npoints = g.npoints;
+    if (nbodies) *nbodies = (int)g.bodies.size();
+    if (stationary) *stationary = g.stationary ? 1 : 0;
+    if (xc0) *xc0 = g.bodies.empty() ? 0.0 : g.bodies[0].xc;
+    if (yc0) *yc0 = g.bodies.empty() ? 0.0 : g.bodies[0].yc;
+    return 0;
+}
+
+int ibpm_geom_points(const char* geom_text, int ntimes, const double* times, double* xy, double* uv, long cap) {
+    if (ntimes < 0 || (ntimes > 0 && (!times || !xy || !uv))) { set_error("ibpm_geom_points: bad arguments"); return -1; }
+    Geometry g;
+    std::string err;
+    if (!g.parse(geom_text ? geom_text : "", err)) { set_error("%s", err.c_str()); return -1; }
+    const long n = g.npoints;
+    if (2 * n * ntimes > cap) {
+        set_error("ibpm_geom_points: %ld values per array needed, room for %ld", 2 * n * ntimes, cap);
+        return -1;
+    }
+    std::vector<double> x, y, u, v;
+    for (int c = 0; c < ntimes; ++c) {
+        g.move(times[c], x, y, u, v);
+        double* pxy = xy + 2 * n * c;
+        double* puv = uv + 2 * n * c;
+        for (long k = 0; k < n; ++k) {
+            pxy[k] = x[k]; pxy[n + k] = y[k];
+            puv[k] = u[k]; puv[n + k] = v[k];
+        }
+    }
+    return (int)n;
 }
 
 int ibpm_device_count(void) {
@@ -1580,11 +1668,40 @@ int ibpm_batch_create(ibpm_ctx* ctx, int nflows, const ibpm_flow_params* params,
         F.ns = (long)(F.nx - 1) * (F.ny - 1);
         F.nf = 2L * F.nx * F.ny + F.nx + F.ny;
         F.nb = F.geom.npoints;
+        {
+            const double pi = 4. * std::atan(1.);
+            F.force_alpha = F.p.alpha_deg * pi / 180.;   // ibpm.cc:162-163
+        }
+        if (F.p.ubf) {   // ibpm.cc:166-178
+            if (F.geom.bodies.size() != 1) {
+                set_error("flow %d: unsteady base flow is only supported for a single moving body", k);
+                return -1;
+            }
+            if (!F.geom.stationary) {   // Geometry::transferMotion / transferCenter
+                Body& b0 = F.geom.bodies[0];
+                F.ubf = true;
+                F.bf.motion = b0;
+                F.bf.motion.rx.clear();
+                F.bf.motion.ry.clear();
+                F.bf.xc = b0.xc;
+                F.bf.yc = b0.yc;
+                F.bf.alpha_bf = F.bf.alpha = F.force_alpha;   // BaseFlow(grid, 1, alpha)
+                F.bf.mag_bf = F.bf.mag = 1.0;
+                b0.motion = M_NONE;
+                F.geom.stationary = true;
+                B->any_ubf = true;
+            }
+        }
         F.cg = !F.geom.stationary || F.p.force_cg;
         if (F.cg) B->any_cg = true;
         if (!F.geom.stationary) B->all_stationary = false;
         if (F.p.cg_tol <= 0) F.p.cg_tol = 1e-7;   // IBSolver.cc:31 default
         CKR(alloc_flow(B.get(), F));
+        CKR(B->mem.alloc(&F.bf_dev, 4));
+        {
+            const double zero[4] = {0, 0, 0, 0};
+            CKR(upload(F.bf_dev, zero, sizeof zero));
+        }
         B->max_ns_all = std::max(B->max_ns_all, F.ns * F.L);
         B->max_nf = std::max(B->max_nf, F.nf);
         B->max_nf_all = std::max(B->max_nf_all, F.nf * F.L);
@@ -1643,6 +1760,8 @@ int ibpm_batch_init(ibpm_batch* B) {
     }
     CKR(upload_regops(B));
     if (!B->any_cg) CKR(build_cholesky(B));              // solver->init()
+    for (auto& F : B->flows)                             // updateOperators(t0), ibpm.cc:304
+        if (F.ubf) CKR(move_base_flow(B, F, B->time));
     enqueue_refresh(B, s);                               // ibpm.cc:304-305
     CK(hipStreamSynchronize(s));
     CK(hipGetLastError());
@@ -1662,7 +1781,7 @@ int ibpm_batch_advance(ibpm_batch* B, int nsteps) {
                 if (i == 0) mark_nprev_valid(B);
             }
             timed(B->probe, K_FORCES, 0, s, [&] { launch_forces(B->d_force, (int)B->flows.size(), s); });
-        } else if (B->use_graph && B->old_saved && !B->probe) {
+        } else if (B->use_graph && B->old_saved && !B->probe && !B->any_ubf) {
             if (!B->graph) {
                 hipGraph_t g;
                 CK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -1674,6 +1793,8 @@ int ibpm_batch_advance(ibpm_batch* B, int nsteps) {
             CK(hipGraphLaunch(B->graph, s));
         } else {
             for (int i = 0; i < B->nsub; ++i) {
+                for (auto& F : B->flows)   // updateOperators(t + c_n dt), IBSolver.cc:164-166
+                    if (F.ubf) CKR(move_base_flow(B, F, B->time + B->cn[i] * F.p.dt));
                 enqueue_substep_chol(B, i, s);
                 if (i == 0 && !B->old_saved) {
                     CK(hipStreamSynchronize(s));
@@ -1684,6 +1805,13 @@ int ibpm_batch_advance(ibpm_batch* B, int nsteps) {
         }
         B->time += B->flows[0].p.dt;
         B->timestep += 1;
+        for (auto& F : B->flows) {   // ibpm.cc:347-351: lab-frame lift and drag
+            if (F.ubf) {
+                F.bf.set_alpha_mag(B->time);
+                F.force_alpha = F.bf.alpha;
+            }
+            F.alpha_hist.push_back(F.force_alpha);
+        }
     }
     CK(hipStreamSynchronize(s));
     CK(hipGetLastError());
@@ -1738,6 +1866,10 @@ int ibpm_batch_set_state(ibpm_batch* B, int k, const double* h_omega, const doub
         CKR(update_regularizer(B, *F));
         CKR(upload_regops(B));
     }
+    if (F->ubf) {   // base flow at the restart time (ibpm.cc:304)
+        CKR(move_base_flow(B, *F, B->time));
+        F->force_alpha = F->bf.alpha;
+    }
     enqueue_refresh(B, B->ctx->c.stream);
     CK(hipStreamSynchronize(B->ctx->c.stream));
     return 0;
@@ -1767,14 +1899,17 @@ int ibpm_batch_get_points(ibpm_batch* B, int k, double* h) {
     return 0;
 }
 
-static void forces_from(const Flow& F, double xf, double yf, double* cd, double* cl) {
+// Cd, Cl from the net force rotated by the base-flow angle al (radians):
+// the free-stream angle, or for an unsteady base flow BaseFlow::getAlpha after
+// setAlphaMag at the step's end time.
+static void forces_from(const Flow& F, double al, double xf, double yf, double* cd, double* cl) {
     const double dx2 = F.dx * F.dx;   // State.cc:60-62
     xf *= dx2;
     yf *= dx2;
-    const double pi = 4. * std::atan(1.);
-    const double al = F.p.alpha_deg * pi / 180.;
-    const double drag = xf * std::cos(al) + yf * std::sin(al);   // ibpm.cc:352-353
-    const double lift = xf * -1. * std::sin(al) + yf * std::cos(al);
+    double sal, cal;
+    sin_cos(al, sal, cal);
+    const double drag = xf * cal + yf * sal;   // ibpm.cc:352-353
+    const double lift = xf * -1. * sal + yf * cal;
     *cd = 2 * drag;
     *cl = 2 * lift;
 }
@@ -1786,7 +1921,7 @@ int ibpm_batch_get_forces(ibpm_batch* B, int k, double* cd, double* cl) {
     CK(hipMemcpy(f.data(), F->f, sizeof(double) * f.size(), hipMemcpyDeviceToHost));
     double xf = 0, yf = 0;
     for (int i = 0; i < F->nb; ++i) { xf += f[i]; yf += f[F->nb + i]; }
-    forces_from(*F, xf, yf, cd, cl);
+    forces_from(*F, F->force_alpha, xf, yf, cd, cl);
     return 0;
 }
 
@@ -1798,8 +1933,19 @@ int ibpm_batch_force_history(ibpm_batch* B, int k, double* h, int max_steps) {
     n = std::min(n, max_steps);
     std::vector<double> raw(2 * (size_t)std::max(n, 1));
     CK(hipMemcpy(raw.data(), F->hist, sizeof(double) * 2 * n, hipMemcpyDeviceToHost));
-    for (int s = 0; s < n; ++s) forces_from(*F, raw[2 * s], raw[2 * s + 1], &h[2 * s], &h[2 * s + 1]);
+    for (int s = 0; s < n; ++s) {
+        const double al = s < (int)F->alpha_hist.size() ? F->alpha_hist[s] : F->force_alpha;
+        forces_from(*F, al, raw[2 * s], raw[2 * s + 1], &h[2 * s], &h[2 * s + 1]);
+    }
     return n;
+}
+
+int ibpm_batch_base_flow(ibpm_batch* B, int k, double* alpha, double* mag) {
+    Flow* F = flow_of(B, k);
+    if (!F) return -1;
+    if (alpha) *alpha = F->force_alpha;
+    if (mag) *mag = F->ubf ? F->bf.mag : 1.0;
+    return F->ubf ? 1 : 0;
 }
 
 int ibpm_batch_cg_iters(ibpm_batch* B, int k, int* h, int max_sub) {

@@ -1,6 +1,8 @@
 // geometry.cpp -- .geom parsing, rigid-body motion and regulariser lists.
 #include "geometry.h"
 
+#include <fstream>
+
 #include <algorithm>
 #include <cctype>
 #include <cmath>
@@ -16,7 +18,7 @@ static std::string lower(std::string s) {
 
 // The command set of RigidBody::load (RigidBody.cc:248-529) that the
 // projection-step configurations use: circle, circle_n, line, line_n,
-// line_aoa, point, center, name and the motions fixed / pitchplunge.
+// line_aoa, point, center, name and every motion of RigidBody.cc:351-492.
 bool Geometry::parse(const std::string& text, std::string& err) {
     bodies.clear();
     std::istringstream in(text);
@@ -59,8 +61,10 @@ bool Geometry::parse(const std::string& text, std::string& err) {
             const double twopi = 8. * std::atan(1.);
             const double dTheta = twopi / n;
             for (int i = 0; i < n; ++i) {
-                cur->rx.push_back(xc + r * std::cos(i * dTheta));
-                cur->ry.push_back(yc + r * std::sin(i * dTheta));
+                double s, c;
+                sin_cos(i * dTheta, s, c);
+                cur->rx.push_back(xc + r * c);
+                cur->ry.push_back(yc + r * s);
             }
         } else if (cmd == "line" || cmd == "line_n") {       // RigidBody.cc:120-146
             double x1, y1, x2, y2, d;
@@ -83,7 +87,9 @@ bool Geometry::parse(const std::string& text, std::string& err) {
             int n;
             if (!(ls >> l >> xC >> yC >> aoa >> n)) { err = "geometry: bad line_aoa: " + line; return false; }
             const double pi = 3.141592653589793238462643383279502884197169399375;
-            const double a = aoa / 180 * pi, ca = std::cos(a), sa = std::sin(a);
+            const double a = aoa / 180 * pi;
+            double sa, ca;
+            sin_cos(a, sa, ca);
             const double del = l / (n - 1);
             for (int i = 0; i < n; ++i) {
                 const double x0r = i * del - xC, y0r = 0. - yC;
@@ -102,19 +108,78 @@ bool Geometry::parse(const std::string& text, std::string& err) {
             std::string mt;
             ls >> mt;
             mt = lower(mt);
-            if (mt == "fixed") {                               // FixedPosition.h
-                if (!(ls >> cur->mp[0] >> cur->mp[1] >> cur->mp[2])) { err = "geometry: bad fixed motion"; return false; }
-                cur->motion = 1;
-            } else if (mt == "pitchplunge") {                  // PitchPlunge.h:30-58
+            auto need = [&](int k, const char* what) {
+                for (int q = 0; q < k; ++q)
+                    if (!(ls >> cur->mp[q])) {
+                        err = std::string("geometry: ") + what;
+                        return false;
+                    }
+                return true;
+            };
+            if (mt == "fixed") {                               // FixedPosition.h, RigidBody.cc:353-361
+                if (!need(3, "bad fixed motion")) return false;
+                cur->motion = M_FIXED;
+            } else if (mt == "fixedvel") {                     // FixedVelocity.h:26-50, RigidBody.cc:363-372
+                if (!need(3, "fixedvel needs xdot ydot thetadot")) return false;
+                cur->motion = M_FIXEDVEL;
+            } else if (mt == "pitchplunge") {                  // PitchPlunge.h:30-58, RigidBody.cc:373-385
                 double a[6];
                 if (!(ls >> a[0] >> a[1] >> a[2] >> a[3] >> a[4] >> a[5])) {
                     err = "geometry: pitchplunge needs 6 values (RigidBody.cc:373-385)";
                     return false;
                 }
                 const double twopi = 8. * std::atan(1.);
-                cur->motion = 2;
+                cur->motion = M_PITCHPLUNGE;
                 cur->mp[0] = a[0]; cur->mp[1] = a[1] * twopi; cur->mp[2] = a[2];
                 cur->mp[3] = a[3]; cur->mp[4] = a[4] * twopi; cur->mp[5] = a[5];
+            } else if (mt == "sigmoidalstep") {                // SigmoidalStep.h:29-52, RigidBody.cc:386-395
+                if (!need(3, "sigmoidalstep needs AMP DUR t0")) return false;
+                cur->motion = M_SIGMOIDALSTEP;
+            } else if (mt == "lagstep1" || mt == "lagstep2") { // LagStep1.h, LagStep2.h:27-63, RigidBody.cc:396-417
+                if (!need(4, "lagstep needs AMP PW TAU T0")) return false;
+                cur->motion = mt == "lagstep1" ? M_LAGSTEP1 : M_LAGSTEP2;
+            } else if (mt == "eldredge" || mt == "eldredge1" || mt == "eldredge2") {
+                // EldredgeManeuver.h:27-62, Eldredge1.h:26-57, Eldredge2.h:27-72, RigidBody.cc:418-475
+                if (!need(6, "eldredge needs AMP a t1 t2 t3 t4")) return false;
+                cur->motion = mt == "eldredge" ? M_ELDREDGE : mt == "eldredge1" ? M_ELDREDGE1 : M_ELDREDGE2;
+                const double a = cur->mp[1], t1 = cur->mp[2], t2 = cur->mp[3], t3 = cur->mp[4], t4 = cur->mp[5];
+                const double tt = (t2 + t3) / 2.;
+                cur->mp[6] = std::log((std::cosh(a * (tt - t1)) * std::cosh(a * (tt - t4))) /
+                                      (std::cosh(a * (tt - t2)) * std::cosh(a * (tt - t3))));   // maxG
+            } else if (mt == "eldredgecombined2") {            // EldredgeCombined2.h:27-88, RigidBody.cc:431-449
+                if (!need(12, "eldredgecombined2 needs 12 values")) return false;
+                cur->motion = M_ELDREDGECOMBINED2;
+                const double* m = cur->mp;
+                const double a = m[1], b = m[7];
+                const double tta = (m[3] + m[4]) / 2., ttb = (m[9] + m[10]) / 2.;
+                const double maxa = std::log((std::cosh(a * (tta - m[2])) * std::cosh(a * (tta - m[5]))) /
+                                             (std::cosh(a * (tta - m[3])) * std::cosh(a * (tta - m[4]))));
+                // the reference's second factor uses a, not b (EldredgeCombined2.h:59)
+                const double maxb = std::log((std::cosh(b * (ttb - m[8])) * std::cosh(a * (ttb - m[11]))) /
+                                             (std::cosh(b * (ttb - m[9])) * std::cosh(b * (ttb - m[10]))));
+                cur->mp[12] = maxa;
+                cur->mp[13] = maxb;
+            } else if (mt == "motionfile" || mt == "motionfileperiodic") {
+                // MotionFile.h:57-116, MotionFilePeriodic.h:56-131, RigidBody.cc:476-492
+                std::string fname;
+                if (!(ls >> fname)) { err = "geometry: motion file name missing"; return false; }
+                if (mt == "motionfileperiodic" && !(ls >> cur->mp[0])) {
+                    err = "geometry: motionfileperiodic needs a period";
+                    return false;
+                }
+                std::ifstream mf(fname);
+                int nrows = 0;
+                if (!(mf >> nrows) || nrows < 0) { err = "geometry: cannot read motion file " + fname; return false; }
+                double tlast = -1.e+5;
+                for (int r = 0; r < nrows; ++r) {
+                    double row[7];
+                    for (double& v : row)
+                        if (!(mf >> v)) { err = "geometry: motion file " + fname + " formatted incorrectly"; return false; }
+                    if (row[0] < tlast) { err = "geometry: motion file times must increase"; return false; }
+                    tlast = row[0];
+                    cur->table.insert(cur->table.end(), row, row + 7);
+                }
+                cur->motion = mt == "motionfile" ? M_MOTIONFILE : M_MOTIONFILEPERIODIC;
             } else {
                 err = "geometry: unsupported motion '" + mt + "'";
                 return false;
@@ -128,32 +193,158 @@ bool Geometry::parse(const std::string& text, std::string& err) {
     stationary = true;
     for (auto& b : bodies) {
         npoints += (int)b.rx.size();
-        if (b.motion == 2) stationary = false;
+        if (b.moving()) stationary = false;
     }
     return true;
+}
+
+void body_transformation(const Body& B, double t, double& X, double& Y, double& th, double& xd, double& yd,
+                         double& thd) {
+    X = Y = th = xd = yd = thd = 0;
+    const double* m = B.mp;
+    const double pi = 4. * std::atan(1.);
+    switch (B.motion) {
+    case M_FIXED: X = m[0]; Y = m[1]; th = m[2]; break;
+    case M_PITCHPLUNGE: {   // PitchPlunge.h:52-58
+        double sp, cp, sq, cq;
+        sin_cos(m[4] * t + m[5], sp, cp);
+        sin_cos(m[1] * t + m[2], sq, cq);
+        Y = m[3] * sp;
+        yd = m[3] * m[4] * cp;
+        th = m[0] * sq;
+        thd = m[0] * m[1] * cq;
+        break;
+    }
+    case M_FIXEDVEL: {    // FixedVelocity.h:42-50: positions integrated over successive calls
+        double dtt = t - B.oldtime;
+        if (dtt > 1) dtt = 0.;
+        B.acc[0] = B.acc[0] + m[0] * dtt;
+        B.acc[1] = B.acc[1] + m[1] * dtt;
+        B.acc[2] = B.acc[2] + m[2] * dtt;
+        B.oldtime = t;
+        X = B.acc[0]; Y = B.acc[1]; th = B.acc[2]; xd = m[0]; yd = m[1]; thd = m[2];
+        break;
+    }
+    case M_SIGMOIDALSTEP: {   // SigmoidalStep.h:41-52 (AMP, DUR, t0)
+        const double tdiff = t - m[2];
+        const double arg = ((12. * tdiff / m[1]) - 6.);
+        const double coeff = 12. / (m[1] * std::sqrt(pi));
+        const double nerf = std::erf(arg);
+        th = (1. / 2) * m[0] * (pi / 180) * (1. + nerf);
+        thd = m[0] * (pi / 180) * coeff * std::exp(-arg * arg);
+        break;
+    }
+    case M_LAGSTEP1:
+    case M_LAGSTEP2: {        // LagStep1.h:41-63, LagStep2.h:41-63 (AMP, PW, TAU, T0)
+        const double AMPrad = m[0] * (pi / 180), PW = m[1], TAU = m[2];
+        const double tdiff = t - m[3];
+        double a = 0., ad = 0.;
+        if (tdiff <= 0) {
+        } else if (B.motion == M_LAGSTEP1) {
+            if (tdiff <= PW) {
+                a = tdiff + TAU * std::exp(-tdiff / TAU) - TAU;
+                ad = 1 - std::exp(-tdiff / TAU);
+            } else {
+                a = PW + TAU * std::exp(-tdiff / TAU) - TAU * std::exp(-(tdiff - PW) / TAU);
+                ad = -1. * std::exp(-tdiff / TAU) + std::exp(-(tdiff - PW) / TAU);
+            }
+        } else {
+            if (tdiff <= PW) {
+                a = tdiff + (2 * TAU + tdiff) * std::exp(-tdiff / TAU) - 2 * TAU;
+                ad = 1. - (1. + tdiff / TAU) * std::exp(-tdiff / TAU);
+            } else {
+                a = PW + (2 * TAU + tdiff) * std::exp(-tdiff / TAU) + (PW - 2 * TAU) * std::exp(-(tdiff - PW) / TAU) -
+                    tdiff * std::exp(-(tdiff - PW) / TAU);
+                ad = -(1. + tdiff / TAU) * std::exp(-tdiff / TAU) + (1. + (tdiff - PW) / TAU) * std::exp(-(tdiff - PW) / TAU);
+            }
+        }
+        th = a * AMPrad / PW;
+        thd = ad * AMPrad / PW;
+        break;
+    }
+    case M_ELDREDGE:
+    case M_ELDREDGE1:
+    case M_ELDREDGE2: {       // EldredgeManeuver.h:48-62, Eldredge1.h:46-57, Eldredge2.h:52-72
+        const double a = m[1], maxG = m[6];
+        const double t1d = t - m[2], t2d = t - m[3], t3d = t - m[4], t4d = t - m[5];
+        const double Garg = ((std::cosh(a * t1d) * std::cosh(a * t4d)) / (std::cosh(a * t2d) * std::cosh(a * t3d)));
+        const double G = std::log(Garg);
+        const double dGdt = a * (std::tanh(a * t1d) - std::tanh(a * t2d) - std::tanh(a * t3d) + std::tanh(a * t4d));
+        if (B.motion == M_ELDREDGE) {
+            const double AMPrad = m[0] * (pi / 180);
+            th = G * AMPrad / maxG;
+            thd = dGdt * AMPrad / maxG;
+        } else if (B.motion == M_ELDREDGE1) {
+            Y = G * m[0] / maxG;
+            yd = dGdt * m[0] / maxG;
+        } else {   // plunge from the running integral of G over successive calls
+            const double dtt = t - B.oldtime;
+            B.acc[0] = B.acc[0] + G * dtt;
+            B.oldtime = t;
+            Y = B.acc[0] * m[0] / maxG;
+            yd = G * m[0] / maxG;
+        }
+        break;
+    }
+    case M_ELDREDGECOMBINED2: {   // EldredgeCombined2.h:64-88
+        const double AMPrad = m[6] * (pi / 180);
+        const double a = m[1], b = m[7];
+        const double a1d = t - m[2], a2d = t - m[3], a3d = t - m[4], a4d = t - m[5];
+        const double b1d = t - m[8], b2d = t - m[9], b3d = t - m[10], b4d = t - m[11];
+        const double Garga = ((std::cosh(a * a1d) * std::cosh(a * a4d)) / (std::cosh(a * a2d) * std::cosh(a * a3d)));
+        const double Gargb = ((std::cosh(b * b1d) * std::cosh(b * b4d)) / (std::cosh(b * b2d) * std::cosh(b * b3d)));
+        const double Ga = std::log(Garga), Gb = std::log(Gargb);
+        const double dGdtb = b * (std::tanh(b * b1d) - std::tanh(b * b2d) - std::tanh(b * b3d) + std::tanh(b * b4d));
+        const double dtt = t - B.oldtime;
+        B.acc[0] = B.acc[0] + Ga * dtt;
+        B.oldtime = t;
+        Y = B.acc[0] * m[0] / m[12];
+        th = Gb * AMPrad / m[13];
+        yd = Ga * m[0] / m[12];
+        thd = dGdtb * AMPrad / m[13];
+        break;
+    }
+    case M_MOTIONFILE:
+    case M_MOTIONFILEPERIODIC: {  // MotionFile.h:90-116, MotionFilePeriodic.h:91-131
+        const double tt = (B.motion == M_MOTIONFILEPERIODIC) ? std::fmod(t, m[0]) : t;
+        const double* d = B.table.data();
+        const int n = (int)(B.table.size() / 7);
+        int index = -1;
+        for (int i = 0; i < n - 1; i++)
+            if ((tt >= d[7 * i]) && (tt < d[7 * (i + 1)])) index = i;
+        if (index != -1) {
+            const double* p = d + 7 * index;
+            const double* q = p + 7;
+            const double tdiff = q[0] - p[0];
+            const double alpha = (tt - p[0]) / tdiff;
+            X = alpha * q[1] + (1 - alpha) * p[1];
+            Y = alpha * q[2] + (1 - alpha) * p[2];
+            th = alpha * q[3] + (1 - alpha) * p[3];
+            xd = alpha * q[4] + (1 - alpha) * p[4];
+            yd = alpha * q[5] + (1 - alpha) * p[5];
+            thd = alpha * q[6] + (1 - alpha) * p[6];
+        }
+        break;
+    }
+    default: break;
+    }
 }
 
 void Geometry::move(double t, std::vector<double>& x, std::vector<double>& y,
                     std::vector<double>& u, std::vector<double>& v) const {
     x.clear(); y.clear(); u.clear(); v.clear();
     for (const auto& B : bodies) {
-        double X = 0, Y = 0, th = 0, xd = 0, yd = 0, thd = 0;
-        if (B.motion == 1) {
-            X = B.mp[0]; Y = B.mp[1]; th = B.mp[2];
-        } else if (B.motion == 2) {   // PitchPlunge::getTransformation, PitchPlunge.h:52-58
-            Y = B.mp[3] * std::sin(B.mp[4] * t + B.mp[5]);
-            yd = B.mp[3] * B.mp[4] * std::cos(B.mp[4] * t + B.mp[5]);
-            th = B.mp[0] * std::sin(B.mp[1] * t + B.mp[2]);
-            thd = B.mp[0] * B.mp[1] * std::cos(B.mp[1] * t + B.mp[2]);
-        }
-        if (B.motion == 0) {   // no motion: RigidBody::moveBody returns early
+        if (B.motion == M_NONE) {   // no motion: RigidBody::moveBody returns early
             for (size_t k = 0; k < B.rx.size(); ++k) {
                 x.push_back(B.rx[k]); y.push_back(B.ry[k]);
                 u.push_back(0.0); v.push_back(0.0);
             }
             continue;
         }
-        const double ct = std::cos(th), st = std::sin(th);   // TangentSE2.h:83-107
+        double X, Y, th, xd, yd, thd;
+        body_transformation(B, t, X, Y, th, xd, yd, thd);
+        double st, ct;   // TangentSE2.h:83-107
+        sin_cos(th, st, ct);
         for (size_t k = 0; k < B.rx.size(); ++k) {
             const double a = B.rx[k] - B.xc, b = B.ry[k] - B.yc;
             x.push_back((X + a * ct - b * st) + B.xc);
@@ -162,6 +353,29 @@ void Geometry::move(double t, std::vector<double>& x, std::vector<double>& y,
             v.push_back(a * ct * thd - b * st * thd + yd);
         }
     }
+}
+
+void BaseFlowMotion::set_alpha_mag(double t) {   // BaseFlow.cc:111-125
+    double x, y, th, xd, yd, thd;
+    body_transformation(motion, t, x, y, th, xd, yd, thd);
+    double sb, cb;
+    sin_cos(alpha_bf, sb, cb);
+    const double xdot_bf = mag_bf * cb, ydot_bf = mag_bf * sb;
+    const double xdot_t = xd - xdot_bf, ydot_t = yd - ydot_bf;
+    const double gamma = std::atan2(ydot_t, -1. * xdot_t);
+    alpha = -1. * th - gamma;
+    mag = std::sqrt(xdot_t * xdot_t + ydot_t * ydot_t);
+}
+
+void BaseFlowMotion::move(double t) {   // BaseFlow.cc:127-150
+    double x, y, th, xd, yd, thd;
+    body_transformation(motion, t, x, y, th, xd, yd, thd);
+    set_alpha_mag(t);
+    double sa, ca;
+    sin_cos(alpha, sa, ca);
+    u = mag * ca;
+    v = mag * sa;
+    w = -1. * thd;
 }
 
 // Roma-Peskin-Berger discrete delta, Regularizer.cc:37-49

@@ -13,6 +13,11 @@ struct FlowOps {
     double w_coef;
     int use_nprev, nprev_valid;
     double ubase, vbase;           // uniform base flow (magnitude 1, angle alpha)
+    // unsteady base flow (ubf = 1): bf[0..2] = u, v, w of the current base
+    // flux, X (u - w (y - yc)) d and Y (v + w (x - xc)) d at edge (x, y)
+    int ubf;
+    const double* bf;
+    double xc, yc, xoff, yoff;     // centre of rotation; level-0 grid offsets
 };
 
 // Regulariser gathers (E, and beta Curl E^T) of one flow.

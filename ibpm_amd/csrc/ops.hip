@@ -251,8 +251,10 @@ __global__ void __launch_bounds__(kThreads) k_rhs(const FlowOps* __restrict__ fl
     F.nprev[t] = n;
 }
 
-// q = Curl(psi) + q0 on every level (NavierStokesModel.cc:121-129,
-// uniform base flow Flux.cc:79-99).
+// q = Curl(psi) + q0 on every level (NavierStokesModel.cc:121-129): q0 is
+// the uniform base flow (Flux.cc:79-99) or the unsteady one, free stream plus
+// rotation about the body centre at the edge coordinates of each level
+// (Flux.cc:101-129, Grid.cc:104-139).
 __global__ void __launch_bounds__(kThreads) k_curl_q(const FlowOps* __restrict__ flows) {
     const FlowOps& F = flows[blockIdx.y];
     const Lay g{F.nx, F.ny, F.L};
@@ -262,13 +264,25 @@ __global__ void __launch_bounds__(kThreads) k_curl_q(const FlowOps* __restrict__
     const long r = t - lev * g.nf();
     const double d = F.dx * (1 << lev);
     double v;
+    // level offsets: offset + 0.5 (shift - 1) (2^lev - 1) n dx with shift 0
+    const double lofs = -0.5 * ((1 << lev) - 1);
     if (r < g.nfx()) {
         const int i = (int)(r / g.ny), j = (int)(r % g.ny);
-        v = curl_x(g, F.psi, lev, i, j) + F.ubase * d;
+        double q0 = F.ubase * d;
+        if (F.ubf) {
+            const double y = (F.yoff + lofs * (F.ny * F.dx)) + (j + 0.5) * d;
+            q0 = (F.bf[0] - F.bf[2] * (y - F.yc)) * d;
+        }
+        v = curl_x(g, F.psi, lev, i, j) + q0;
     } else {
         const long u = r - g.nfx();
         const int i = (int)(u / (g.ny + 1)), j = (int)(u % (g.ny + 1));
-        v = curl_y(g, F.psi, lev, i, j) + F.vbase * d;
+        double q0 = F.vbase * d;
+        if (F.ubf) {
+            const double x = (F.xoff + lofs * (F.nx * F.dx)) + (i + 0.5) * d;
+            q0 = (F.bf[1] + F.bf[2] * (x - F.xc)) * d;
+        }
+        v = curl_y(g, F.psi, lev, i, j) + q0;
     }
     F.q[t] = v;
 }

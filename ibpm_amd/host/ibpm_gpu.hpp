@@ -49,6 +49,8 @@ private:
     double _length, _x0, _y0, _dx;
 };
 
+class Motion {};   // reference src/Motion.h: a token here; the library holds the motion
+
 class Geometry {   // reference src/Geometry.h (.geom text is parsed by the library)
 public:
     Geometry() = default;
@@ -68,11 +70,35 @@ public:
         return _n >= 0;
     }
     int getNumPoints() const { return _n; }
+    int getNumBodies() const {
+        int nb = 0;
+        ibpm_geom_info(_text.c_str(), nullptr, &nb, nullptr, nullptr, nullptr);
+        return nb;
+    }
+    // Geometry::isStationary: no body with a time-dependent motion, or the
+    // motion has been transferred to an unsteady base flow
+    bool isStationary() const {
+        int st = 1;
+        ibpm_geom_info(_text.c_str(), nullptr, nullptr, &st, nullptr, nullptr);
+        return st == 1 || _transferred;
+    }
+    // Geometry::transferMotion / transferCenter (Geometry.cc:46-77): the
+    // library moves the first body's motion to the base flow itself when the
+    // solver is created with an unsteady base flow; these record the request.
+    Motion* transferMotion() {
+        _transferred = true;
+        return new Motion();
+    }
+    void transferCenter(double& x, double& y) const {
+        ibpm_geom_info(_text.c_str(), nullptr, nullptr, nullptr, &x, &y);
+    }
+    bool motionTransferred() const { return _transferred; }
     const std::string& text() const { return _text; }
 
 private:
     std::string _text;
     int _n = 0;
+    bool _transferred = false;
 };
 
 class BaseFlow {   // reference src/BaseFlow.h: uniform flow of magnitude mag at angle alpha
@@ -80,9 +106,17 @@ public:
     BaseFlow(const Grid&, double mag, double alpha) : _mag(mag), _alpha(alpha) {}
     double getAlpha() const { return _alpha; }
     double getMag() const { return _mag; }
+    // BaseFlow::setMotion / setCenter (BaseFlow.cc:100-109): the flow becomes
+    // unsteady; its angle for the force rotation then comes from the solver
+    // (NonlinearIBSolver::baseFlowAngle)
+    void setMotion(const Motion&) { _unsteady = true; }
+    void setCenter(double x, double y) { _xc = x; _yc = y; }
+    bool isStationary() const { return !_unsteady; }
 
 private:
     double _mag, _alpha;
+    bool _unsteady = false;
+    double _xc = 0, _yc = 0;
 };
 
 struct Scheme {   // reference src/Scheme.h:12
@@ -137,9 +171,9 @@ public:
         : _grid(grid), _geom(geom), _Re(Reynolds), _q0(q0) {}
     void init() {}
     double getAlpha() const { return 1. / _Re; }
-    bool geTimeDependent() const { return _geom.text().find("pitchplunge") != std::string::npos ||
-                                          _geom.text().find("PitchPlunge") != std::string::npos; }
-    bool isTimeDependent() const { return geTimeDependent(); }   // the base flow is steady here
+    bool geTimeDependent() const { return !_geom.isStationary(); }          // NavierStokesModel.cc:65-69
+    bool bfTimeDependent() const { return !_q0.isStationary(); }            // NavierStokesModel.cc:71-75
+    bool isTimeDependent() const { return geTimeDependent() || bfTimeDependent(); }
     const Grid& getGrid() const { return _grid; }
     const Geometry& geometry() const { return _geom; }
     const BaseFlow& getBaseFlow() const { return _q0; }
@@ -197,6 +231,7 @@ public:
         p.alpha_deg = _model.getBaseFlow().getAlpha() * 180. / (4. * std::atan(1.));
         p.cg_tol = _tol; p.force_cg = 0;
         p.geom = _model.geometry().text().c_str();
+        p.ubf = _model.bfTimeDependent() ? 1 : 0;
         gpu_check(ibpm_batch_create(_ctx, 1, &p, &_batch));
         if (!_cache.empty() && !_model.geTimeDependent())
             gpu_check(ibpm_batch_cholesky_cache(_batch, 0, _cache.c_str()));
@@ -229,6 +264,14 @@ public:
         gpu_check(ibpm_batch_get_omega(_batch, 0, x.omega.data()));
         gpu_check(ibpm_batch_get_q(_batch, 0, x.q.data()));
         gpu_check(ibpm_batch_get_f(_batch, 0, x.f.data()));
+    }
+    // The angle (radians) that rotates the net force into drag and lift after
+    // the last step: BaseFlow::setAlphaMag(x.time); getAlpha() of ibpm.cc:347-351
+    // for an unsteady base flow, else the free-stream angle.
+    double baseFlowAngle() const {
+        double al = _model.getBaseFlow().getAlpha();
+        if (_batch) ibpm_batch_base_flow(_batch, 0, &al, nullptr);
+        return al;
     }
     int cgIterations(int substep) const {
         int it[4] = {-1, -1, -1, -1};

@@ -8,8 +8,10 @@
 // (src/State.cc:65-223, src/ibpm.cc:250-274).  Added flag: -tol (CG tolerance,
 // which the reference sets only from code, src/ibpm.cc:292-294) and -precond
 // (force solve of moving bodies: none = the reference's plain CG, stale, or
-// lowrank; all end on the same residual test).  Tecplot and energy outputs
-// are not produced by this driver.
+// lowrank; all end on the same residual test).  -ubf 1 (unsteady base flow,
+// src/ibpm.cc:166-178) solves a single moving body in its own frame and
+// rotates the forces by the base-flow angle after each step (:347-351).
+// Tecplot and energy outputs are not produced by this driver.
 #include <sys/stat.h>
 
 #include <cmath>
@@ -83,6 +85,7 @@ int main(int argc, char** argv) {
     const double tol = P.f("tol", 1e-7);
     const int device = P.i("device", 0);
     const std::string precond = P.str("precond", "none");
+    const bool ubf = P.i("ubf", 0) != 0;
     std::map<std::string, Scheme::SchemeType> schemes = {
         {"euler", Scheme::EULER}, {"ab2", Scheme::AB2}, {"rk3", Scheme::RK3}, {"rk3b", Scheme::RK3b}};
     std::map<std::string, std::pair<int, int>> preconds = {{"none", {0, 0}}, {"stale", {1, 0}}, {"lowrank", {1, 8}}};
@@ -90,7 +93,8 @@ int main(int argc, char** argv) {
         std::cerr << "usage: ibpm [-name ibpm] [-outdir .] [-force 1] [-nx 200] [-ny 200] [-ngrid 1]\n"
                      "            [-length 4] [-xoffset -2] [-yoffset -2] [-alpha 0] [-geom name.geom]\n"
                      "            [-Re 100] [-dt 0.02] [-nsteps 250] [-scheme euler|ab2|rk3|rk3b]\n"
-                     "            [-model nonlinear] [-tol 1e-7] [-device 0] [-precond none|stale|lowrank]\n";
+                     "            [-model nonlinear] [-tol 1e-7] [-device 0] [-precond none|stale|lowrank]\n"
+                     "            [-ubf 0|1]\n";
         return 1;
     }
     if (outdir.empty() || outdir.back() != '/') outdir += "/";
@@ -106,6 +110,19 @@ int main(int argc, char** argv) {
     std::cout << "    " << geom.getNumPoints() << " points on the boundary\n" << std::endl;
     const double pi = 4. * std::atan(1.);
     BaseFlow q_potential(grid, 1.0, alpha_deg * pi / 180.);
+    // unsteady base flow: only for a single moving body (ibpm.cc:166-178)
+    if (!geom.isStationary() && geom.getNumBodies() == 1 && ubf) {
+        Motion* m = geom.transferMotion();
+        double xC = 0, yC = 0;
+        geom.transferCenter(xC, yC);
+        q_potential.setMotion(*m);
+        q_potential.setCenter(xC, yC);
+        delete m;
+    }
+    if (ubf && geom.getNumBodies() != 1) {
+        std::cout << "Unsteady base flow is only supported for a single moving body.  Exiting program." << std::endl;
+        return 1;
+    }
     NavierStokesModel nsm(grid, geom, Re, q_potential);
     NonlinearIBSolver solver(grid, nsm, dt, schemes[scheme], tol, device);
     solver.setPrecond(preconds[precond].first, preconds[precond].second);
@@ -128,14 +145,17 @@ int main(int argc, char** argv) {
     std::cout << "using " << solver.getName() << " timestepper\n    dt = " << dt << "\n" << std::endl;
     FILE* ff = iForce > 0 ? std::fopen((outdir + name + ".force").c_str(), "w") : nullptr;
     FILE* ff17 = iForce > 0 ? std::fopen((outdir + name + ".force17").c_str(), "w") : nullptr;
-    const double al = q_potential.getAlpha();
+    double al = q_potential.getAlpha();
     std::cout << "Integrating for " << numSteps << " steps" << std::endl;
     for (int i = 1; i <= numSteps; ++i) {
         solver.advance(x);
         double xF, yF;
         x.computeNetForce(xF, yF);
-        const double drag = 2 * (xF * std::cos(al) + yF * std::sin(al));
-        const double lift = 2 * (xF * -1. * std::sin(al) + yF * std::cos(al));
+        if (!q_potential.isStationary()) al = solver.baseFlowAngle();   // ibpm.cc:347-351
+        double sal, cal;   // one sincos, as the reference's gcc build takes the pair
+        ::sincos(al, &sal, &cal);
+        const double drag = 2 * (xF * cal + yF * sal);
+        const double lift = 2 * (xF * -1. * sal + yF * cal);
         std::printf("step %d    x force: %16.9g, y force: %16.9g\n", i, drag, lift);
         if (ff && x.timestep % iForce == 0) {
             std::fprintf(ff, "%5d %.5e %.5e %.5e\n", x.timestep, x.time, drag, lift);

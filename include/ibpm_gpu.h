@@ -96,6 +96,20 @@ int ibpm_reg_to_boundary(ibpm_reg* reg, const double* d_q, double* d_f);
 /* Number of boundary points of a .geom text (Geometry::getNumPoints after
    Geometry::load, src/Geometry.cc:143-191), or -1 if it does not parse. */
 int ibpm_geom_num_points(const char* geom_text);
+/* Geometry::getNumPoints, getNumBodies, isStationary and the centre of the
+   first body (Geometry.cc:111-113, 72-77; only "motion fixed" and no motion
+   are stationary, Motion.h:28, FixedPosition.h:39).  Any output may be null.
+   Returns 0, or -1 if the text does not parse. */
+int ibpm_geom_info(const char* geom_text, int* npoints, int* nbodies, int* stationary, double* xc0, double* yc0);
+/* Geometry::moveBodies(t) at each of times[0..ntimes) in order, then
+   getPoints / getVelocities (src/Geometry.cc:115-128, RigidBody.cc:586-609,
+   the Motion classes FixedPosition.h ... MotionFilePeriodic.h).  Motions that
+   integrate over successive calls (FixedVelocity, Eldredge2,
+   EldredgeCombined2) keep their state across the sequence as the reference's
+   do.  Call c writes [x.., y..] to xy + 2 n c and [u.., v..] to uv + 2 n c;
+   cap is the length of each array.  Host memory, no GPU work.  Returns n, the
+   number of points, or -1. */
+int ibpm_geom_points(const char* geom_text, int ntimes, const double* times, double* xy, double* uv, long cap);
 
 /* ---- whole projection-step solver (IBSolver / ProjectionSolver / ...) ----- */
 typedef struct {
@@ -108,6 +122,9 @@ typedef struct {
     double cg_tol;              /* CG absolute residual tolerance (IBSolver setTol) */
     int force_cg;               /* 1: CG even for stationary bodies */
     const char* geom;           /* .geom file text (src/Geometry.cc:143-191) */
+    int ubf;                    /* 1: unsteady base flow (ibpm.cc -ubf, :166-178): the motion of the
+                                   only body moves to the base flow and the flow is solved in the
+                                   body frame; more than one body is an error */
 } ibpm_flow_params;
 
 /* A batch of independent flows advanced together by the same kernels. */
@@ -129,6 +146,12 @@ int ibpm_batch_get_points(ibpm_batch* b, int flow, double* h_xy);
 int ibpm_batch_get_forces(ibpm_batch* b, int flow, double* cd, double* cl);
 /* Force history: (Cd, Cl) of every step since init, 2 doubles per step. */
 int ibpm_batch_force_history(ibpm_batch* b, int flow, double* h_out, int max_steps);
+/* Base-flow angle (radians) and magnitude that rotate the current force into
+   drag and lift: the free-stream angle, or for an unsteady base flow
+   BaseFlow::getAlpha / getMag after setAlphaMag at the current time
+   (ibpm.cc:347-351, BaseFlow.cc:111-125).  Returns 1 for an unsteady base
+   flow, 0 otherwise, -1 on error. */
+int ibpm_batch_base_flow(ibpm_batch* b, int flow, double* alpha, double* mag);
 /* CG iterations of each substep of the last step (-1 when Cholesky is used). */
 int ibpm_batch_cg_iters(ibpm_batch* b, int flow, int* h_iters, int max_substeps);
 double ibpm_batch_time(ibpm_batch* b);

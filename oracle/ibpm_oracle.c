@@ -9,12 +9,18 @@
  * DST-I written here (og_dst2d), pinned against direct summation
  * (og_dst2d_direct) and scipy.fft.dstn(type=1) in tests/test_oracle.py.
  */
+#define _GNU_SOURCE   /* sincos */
 #include "ibpm_oracle.h"
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include <ctype.h>
+
+/* sin and cos of one argument from glibc's sincos: gcc compiles the
+   reference's sin(a) ... cos(a) pairs into one sincos call, which differs from
+   separate sin and cos by an ulp for about one argument in 700. */
+static void sin_cos(double a, double* s, double* c) { sincos(a, s, c); }
 
 /* ======================================================================= */
 /* Complex FFT (mixed-radix Stockham, exact twiddle table)                 */
@@ -767,11 +773,20 @@ void og_reg_toboundary(int npts, double dx, int na, const int* bidx, const int* 
 /* ======================================================================= */
 /* Geometry (Geometry.cc, RigidBody.cc, PitchPlunge.h, TangentSE2.h)         */
 /* ======================================================================= */
+/* motion codes: the "motion" types of RigidBody.cc:351-492 */
+enum {
+    OG_NONE = 0, OG_FIXED, OG_PITCHPLUNGE, OG_FIXEDVEL, OG_SIGMOIDALSTEP, OG_LAGSTEP1, OG_LAGSTEP2, OG_ELDREDGE,
+    OG_ELDREDGE1, OG_ELDREDGE2, OG_ELDREDGECOMBINED2, OG_MOTIONFILE, OG_MOTIONFILEPERIODIC
+};
+
 typedef struct {
     int start, npts;
     double xc, yc;
-    int motion;          /* 0 none, 1 fixed, 2 pitchplunge */
-    double mp[6];
+    int motion;
+    double mp[14];       /* parameters in the constructor's order, then derived constants */
+    double* table;       /* motion files: nrows x (t, x, y, theta, dx, dy, dtheta) */
+    int nrows;
+    double acc[3], oldtime;   /* state of FixedVelocity, Eldredge2, EldredgeCombined2 */
 } og_body;
 
 struct og_sim {
@@ -795,6 +810,11 @@ struct og_sim {
     /* Cholesky per substep */
     double *chol_diag[4], *chol_low[4];
     int cg_iters[4];
+    /* unsteady base flow (ibpm -ubf): the motion moved from the only body */
+    int ubf;
+    og_body bf;
+    double bf_alpha, bf_mag, bf_u, bf_v, bf_w;
+    double force_alpha;
 };
 
 static char g_err[256];
@@ -805,6 +825,30 @@ static void add_point(og_sim* s, og_body* b, double x, double y) {
     s->refy = (double*)realloc(s->refy, sizeof(double) * (s->nb + 1));
     s->refx[s->nb] = x; s->refy[s->nb] = y;
     s->nb++; b->npts++;
+}
+
+/* MotionFile / MotionFilePeriodic constructors (MotionFile.h:57-81,
+   MotionFilePeriodic.h:56-81): "filename" or "filename period"; the file holds
+   n, then n rows t x y theta dx dy dtheta with t non-decreasing. */
+static int load_motion_file(og_body* B, const char* args, int periodic) {
+    char fname[512] = {0};
+    int used = 0;
+    if (sscanf(args, "%511s%n", fname, &used) != 1) { snprintf(g_err, sizeof g_err, "geom: motion file name missing"); return -1; }
+    if (periodic && sscanf(args + used, "%lf", &B->mp[0]) != 1) {
+        snprintf(g_err, sizeof g_err, "geom: motionfileperiodic needs a period"); return -1;
+    }
+    FILE* fp = fopen(fname, "r");
+    if (!fp) { snprintf(g_err, sizeof g_err, "geom: cannot open %s", fname); return -1; }
+    int n = 0;
+    if (fscanf(fp, "%d", &n) != 1 || n < 0) { fclose(fp); snprintf(g_err, sizeof g_err, "geom: bad motion file"); return -1; }
+    B->table = (double*)malloc(sizeof(double) * (7 * (size_t)n + 1));
+    B->nrows = n;
+    for (int r = 0; r < 7 * n; ++r)
+        if (fscanf(fp, "%lf", &B->table[r]) != 1) { fclose(fp); snprintf(g_err, sizeof g_err, "geom: bad motion file"); return -1; }
+    fclose(fp);
+    for (int r = 1; r < n; ++r)
+        if (B->table[7 * r] < B->table[7 * (r - 1)]) { snprintf(g_err, sizeof g_err, "geom: motion file times decrease"); return -1; }
+    return 0;
 }
 
 static int parse_geom(og_sim* s, const char* text) {
@@ -845,12 +889,20 @@ static int parse_geom(og_sim* s, const char* text) {
             double twopi = 8. * atan(1.);
             int np = (int)floor(twopi / dTheta + 1 + 0.5);
             double dt = twopi / np;
-            for (int i = 0; i < np; ++i) add_point(s, cur, a[0] + a[2] * cos(i * dt), a[1] + a[2] * sin(i * dt));
+            for (int i = 0; i < np; ++i) {
+                double si, ci;
+                sin_cos(i * dt, &si, &ci);
+                add_point(s, cur, a[0] + a[2] * ci, a[1] + a[2] * si);
+            }
         } else if (!strcmp(cmd, "circle_n")) { /* RigidBody.cc:103-118 */
             if (sscanf(rest, "%lf %lf %lf %d", &a[0], &a[1], &a[2], &n) != 4) return -1;
             double twopi = 8. * atan(1.);
             double dt = twopi / n;
-            for (int i = 0; i < n; ++i) add_point(s, cur, a[0] + a[2] * cos(i * dt), a[1] + a[2] * sin(i * dt));
+            for (int i = 0; i < n; ++i) {
+                double si, ci;
+                sin_cos(i * dt, &si, &ci);
+                add_point(s, cur, a[0] + a[2] * ci, a[1] + a[2] * si);
+            }
         } else if (!strcmp(cmd, "line") || !strcmp(cmd, "line_n")) { /* RigidBody.cc:120-146 */
             int np;
             if (!strcmp(cmd, "line")) {
@@ -866,7 +918,8 @@ static int parse_geom(og_sim* s, const char* text) {
             int np;
             if (sscanf(rest, "%lf %lf %lf %lf %d", &a[0], &a[1], &a[2], &a[3], &np) != 5) return -1;
             double pi = 3.141592653589793238462643383279502884197169399375;
-            double al = a[3] / 180 * pi, ca = cos(al), sa = sin(al), del = a[0] / (np - 1);
+            double al = a[3] / 180 * pi, ca, sa, del = a[0] / (np - 1);
+            sin_cos(al, &sa, &ca);
             for (int i = 0; i < np; ++i) {
                 double x0r = i * del - a[1], y0r = 0. - a[2];
                 add_point(s, cur, x0r * ca - y0r * sa + a[1], x0r * sa + y0r * ca + a[2]);
@@ -880,22 +933,155 @@ static int parse_geom(og_sim* s, const char* text) {
             char mt[64] = {0};
             if (sscanf(rest, "%63s", mt) != 1) return -1;
             for (char* c = mt; *c; ++c) *c = (char)tolower(*c);
-            const char* r2 = strstr(rest, mt) + strlen(mt);
-            if (!strcmp(mt, "fixed")) {
-                if (sscanf(r2, "%lf %lf %lf", &a[0], &a[1], &a[2]) != 3) return -1;
-                cur->motion = 1; cur->mp[0] = a[0]; cur->mp[1] = a[1]; cur->mp[2] = a[2];
-            } else if (!strcmp(mt, "pitchplunge")) {
-                if (sscanf(r2, "%lf %lf %lf %lf %lf %lf", &a[0], &a[1], &a[2], &a[3], &a[4], &a[5]) != 6) {
-                    snprintf(g_err, sizeof g_err, "geom: pitchplunge needs 6 values"); return -1;
+            const char* r2 = rest;   /* past the motion word */
+            while (*r2 && isspace((unsigned char)*r2)) ++r2;
+            while (*r2 && !isspace((unsigned char)*r2)) ++r2;
+            int want = -1, code = OG_NONE;
+            if (!strcmp(mt, "fixed")) { want = 3; code = OG_FIXED; }
+            else if (!strcmp(mt, "fixedvel")) { want = 3; code = OG_FIXEDVEL; }
+            else if (!strcmp(mt, "pitchplunge")) { want = 6; code = OG_PITCHPLUNGE; }
+            else if (!strcmp(mt, "sigmoidalstep")) { want = 3; code = OG_SIGMOIDALSTEP; }
+            else if (!strcmp(mt, "lagstep1")) { want = 4; code = OG_LAGSTEP1; }
+            else if (!strcmp(mt, "lagstep2")) { want = 4; code = OG_LAGSTEP2; }
+            else if (!strcmp(mt, "eldredge")) { want = 6; code = OG_ELDREDGE; }
+            else if (!strcmp(mt, "eldredge1")) { want = 6; code = OG_ELDREDGE1; }
+            else if (!strcmp(mt, "eldredge2")) { want = 6; code = OG_ELDREDGE2; }
+            else if (!strcmp(mt, "eldredgecombined2")) { want = 12; code = OG_ELDREDGECOMBINED2; }
+            else if (!strcmp(mt, "motionfile")) { want = 0; code = OG_MOTIONFILE; }
+            else if (!strcmp(mt, "motionfileperiodic")) { want = 0; code = OG_MOTIONFILEPERIODIC; }
+            if (want < 0) { snprintf(g_err, sizeof g_err, "geom: unsupported motion %s", mt); return -1; }
+            if (want > 0) {
+                const char* q = r2;
+                for (int v = 0; v < want; ++v) {
+                    int used = 0;
+                    if (sscanf(q, "%lf%n", &cur->mp[v], &used) != 1) {
+                        snprintf(g_err, sizeof g_err, "geom: motion %s needs %d values", mt, want); return -1;
+                    }
+                    q += used;
                 }
+            } else if (load_motion_file(cur, r2, code == OG_MOTIONFILEPERIODIC) != 0) return -1;
+            cur->motion = code;
+            double* m = cur->mp;
+            if (code == OG_PITCHPLUNGE) {   /* PitchPlunge.h:45-47: frequencies in radians */
                 double twopi = 8. * atan(1.);
-                cur->motion = 2;
-                cur->mp[0] = a[0]; cur->mp[1] = a[1] * twopi; cur->mp[2] = a[2];
-                cur->mp[3] = a[3]; cur->mp[4] = a[4] * twopi; cur->mp[5] = a[5];
-            } else { snprintf(g_err, sizeof g_err, "geom: unsupported motion %s", mt); return -1; }
+                m[1] *= twopi; m[4] *= twopi;
+            } else if (code == OG_ELDREDGE || code == OG_ELDREDGE1 || code == OG_ELDREDGE2) {
+                /* maxG of EldredgeManeuver.h:42-43 (same in Eldredge1.h, Eldredge2.h) */
+                double tt = (m[3] + m[4]) / 2.;
+                m[6] = log((cosh(m[1] * (tt - m[2])) * cosh(m[1] * (tt - m[5]))) /
+                           (cosh(m[1] * (tt - m[3])) * cosh(m[1] * (tt - m[4]))));
+            } else if (code == OG_ELDREDGECOMBINED2) {
+                /* EldredgeCombined2.h:53-59; its maxGb takes cosh(a*(ttb-b4)) with a, not b */
+                double tta = (m[3] + m[4]) / 2., ttb = (m[9] + m[10]) / 2.;
+                m[12] = log((cosh(m[1] * (tta - m[2])) * cosh(m[1] * (tta - m[5]))) /
+                            (cosh(m[1] * (tta - m[3])) * cosh(m[1] * (tta - m[4]))));
+                m[13] = log((cosh(m[7] * (ttb - m[8])) * cosh(m[1] * (ttb - m[11]))) /
+                            (cosh(m[7] * (ttb - m[9])) * cosh(m[7] * (ttb - m[10]))));
+            }
         } else { snprintf(g_err, sizeof g_err, "geom: unsupported command %s", cmd); return -1; }
     }
     return 0;
+}
+
+/* Motion::getTransformation(t) -> (x, y, theta, xdot, ydot, thetadot) for each
+   motion type, in the arithmetic order of the reference's headers. */
+static void body_transform(og_body* B, double t, double* o) {
+    const double* m = B->mp;
+    const double pi = 4. * atan(1.);
+    o[0] = o[1] = o[2] = o[3] = o[4] = o[5] = 0;
+    switch (B->motion) {
+    case OG_FIXED: o[0] = m[0]; o[1] = m[1]; o[2] = m[2]; break;    /* FixedPosition.h */
+    case OG_PITCHPLUNGE: {                                            /* PitchPlunge.h:52-58 */
+        double sp, cp, sq, cq;
+        sin_cos(m[4] * t + m[5], &sp, &cp);
+        sin_cos(m[1] * t + m[2], &sq, &cq);
+        o[1] = m[3] * sp;
+        o[4] = m[3] * m[4] * cp;
+        o[2] = m[0] * sq;
+        o[5] = m[0] * m[1] * cq;
+        break;
+    }
+    case OG_FIXEDVEL: {                                               /* FixedVelocity.h:42-50 */
+        double dtt = t - B->oldtime;
+        if (dtt > 1) dtt = 0.;
+        for (int k = 0; k < 3; ++k) B->acc[k] = B->acc[k] + m[k] * dtt;
+        B->oldtime = t;
+        o[0] = B->acc[0]; o[1] = B->acc[1]; o[2] = B->acc[2];
+        o[3] = m[0]; o[4] = m[1]; o[5] = m[2];
+        break;
+    }
+    case OG_SIGMOIDALSTEP: {                                          /* SigmoidalStep.h:41-52 */
+        double td = t - m[2];
+        double arg = ((12. * td / m[1]) - 6.);
+        double coeff = 12. / (m[1] * sqrt(pi));
+        o[2] = (1. / 2) * m[0] * (pi / 180) * (1. + erf(arg));
+        o[5] = m[0] * (pi / 180) * coeff * exp(-arg * arg);
+        break;
+    }
+    case OG_LAGSTEP1: case OG_LAGSTEP2: {                             /* LagStep1.h, LagStep2.h:41-63 */
+        double amp = m[0] * (pi / 180), pw = m[1], tau = m[2], td = t - m[3];
+        double th = 0., thd = 0.;
+        if (td > 0 && B->motion == OG_LAGSTEP1) {
+            if (td <= pw) { th = td + tau * exp(-td / tau) - tau; thd = 1 - exp(-td / tau); }
+            else {
+                th = pw + tau * exp(-td / tau) - tau * exp(-(td - pw) / tau);
+                thd = -1. * exp(-td / tau) + exp(-(td - pw) / tau);
+            }
+        } else if (td > 0) {
+            if (td <= pw) {
+                th = td + (2 * tau + td) * exp(-td / tau) - 2 * tau;
+                thd = 1. - (1. + td / tau) * exp(-td / tau);
+            } else {
+                th = pw + (2 * tau + td) * exp(-td / tau) + (pw - 2 * tau) * exp(-(td - pw) / tau) -
+                     td * exp(-(td - pw) / tau);
+                thd = -(1. + td / tau) * exp(-td / tau) + (1. + (td - pw) / tau) * exp(-(td - pw) / tau);
+            }
+        }
+        o[2] = th * amp / pw; o[5] = thd * amp / pw;
+        break;
+    }
+    case OG_ELDREDGE: case OG_ELDREDGE1: case OG_ELDREDGE2: {         /* EldredgeManeuver.h, Eldredge1.h, Eldredge2.h */
+        double a = m[1], d1 = t - m[2], d2 = t - m[3], d3 = t - m[4], d4 = t - m[5];
+        double G = log(((cosh(a * d1) * cosh(a * d4)) / (cosh(a * d2) * cosh(a * d3))));
+        double dG = a * (tanh(a * d1) - tanh(a * d2) - tanh(a * d3) + tanh(a * d4));
+        if (B->motion == OG_ELDREDGE) {
+            double amp = m[0] * (pi / 180);
+            o[2] = G * amp / m[6]; o[5] = dG * amp / m[6];
+        } else if (B->motion == OG_ELDREDGE1) {
+            o[1] = G * m[0] / m[6]; o[4] = dG * m[0] / m[6];
+        } else {   /* running integral of G: Eldredge2.h:62-64 */
+            B->acc[0] = B->acc[0] + G * (t - B->oldtime);
+            B->oldtime = t;
+            o[1] = B->acc[0] * m[0] / m[6]; o[4] = G * m[0] / m[6];
+        }
+        break;
+    }
+    case OG_ELDREDGECOMBINED2: {                                      /* EldredgeCombined2.h:64-88 */
+        double amp = m[6] * (pi / 180), a = m[1], b = m[7];
+        double Ga = log(((cosh(a * (t - m[2])) * cosh(a * (t - m[5]))) / (cosh(a * (t - m[3])) * cosh(a * (t - m[4])))));
+        double Gb = log(((cosh(b * (t - m[8])) * cosh(b * (t - m[11]))) / (cosh(b * (t - m[9])) * cosh(b * (t - m[10])))));
+        double dGb = b * (tanh(b * (t - m[8])) - tanh(b * (t - m[9])) - tanh(b * (t - m[10])) + tanh(b * (t - m[11])));
+        B->acc[0] = B->acc[0] + Ga * (t - B->oldtime);
+        B->oldtime = t;
+        o[1] = B->acc[0] * m[0] / m[12]; o[2] = Gb * amp / m[13];
+        o[4] = Ga * m[0] / m[12]; o[5] = dGb * amp / m[13];
+        break;
+    }
+    case OG_MOTIONFILE: case OG_MOTIONFILEPERIODIC: {                 /* MotionFile.h:90-116, MotionFilePeriodic.h:91-117 */
+        double tt = B->motion == OG_MOTIONFILEPERIODIC ? fmod(t, m[0]) : t;
+        const double* d = B->table;
+        int idx = -1;
+        for (int i = 0; i + 1 < B->nrows; ++i)
+            if (tt >= d[7 * i] && tt < d[7 * (i + 1)]) idx = i;
+        if (idx >= 0) {
+            const double* p = d + 7 * idx;
+            double al = (tt - p[0]) / (p[7] - p[0]);
+            for (int k = 0; k < 6; ++k) o[k] = al * p[8 + k] + (1 - al) * p[1 + k];
+        }
+        break;
+    }
+    default: break;
+    }
 }
 
 /* Geometry::moveBodies -> RigidBody::moveBody (RigidBody.cc:586-609) with
@@ -903,16 +1089,12 @@ static int parse_geom(og_sim* s, const char* text) {
 static void move_bodies(og_sim* s, double t) {
     for (int b = 0; b < s->nbody; ++b) {
         og_body* B = &s->bodies[b];
-        if (B->motion == 0) continue;
-        double x = 0, y = 0, th = 0, xd = 0, yd = 0, thd = 0;
-        if (B->motion == 1) { x = B->mp[0]; y = B->mp[1]; th = B->mp[2]; }
-        else {   /* PitchPlunge::getTransformation, PitchPlunge.h:52-58 */
-            y = B->mp[3] * sin(B->mp[4] * t + B->mp[5]);
-            yd = B->mp[3] * B->mp[4] * cos(B->mp[4] * t + B->mp[5]);
-            th = B->mp[0] * sin(B->mp[1] * t + B->mp[2]);
-            thd = B->mp[0] * B->mp[1] * cos(B->mp[1] * t + B->mp[2]);
-        }
-        const double ct = cos(th), st = sin(th);
+        if (B->motion == OG_NONE) continue;
+        double o[6];
+        body_transform(B, t, o);
+        const double x = o[0], y = o[1], th = o[2], xd = o[3], yd = o[4], thd = o[5];
+        double ct, st;
+        sin_cos(th, &st, &ct);
         for (int k = B->start; k < B->start + B->npts; ++k) {
             double a = s->refx[k] - B->xc, c = s->refy[k] - B->yc;
             s->px[k] = (x + a * ct - c * st) + B->xc;
@@ -1084,6 +1266,52 @@ static void refresh_state(og_sim* s) {
     for (long k = 0; k < f_n(s); ++k) s->q[k] += s->q0[k];
 }
 
+/* BaseFlow::setAlphaMag (BaseFlow.cc:111-125): angle and magnitude of the
+   free stream relative to the moving body; free stream 1 at alpha_rad */
+static void bf_set_alpha_mag(og_sim* s, double t) {
+    double o[6];
+    body_transform(&s->bf, t, o);
+    double ca, sa;
+    sin_cos(s->alpha_rad, &sa, &ca);
+    const double xdot_bf = 1. * ca, ydot_bf = 1. * sa;
+    const double xdot_t = o[3] - xdot_bf, ydot_t = o[4] - ydot_bf;
+    const double gamma = atan2(ydot_t, -1. * xdot_t);
+    s->bf_alpha = -1. * o[2] - gamma;
+    s->bf_mag = sqrt(xdot_t * xdot_t + ydot_t * ydot_t);
+}
+
+/* BaseFlow::moveFlow (BaseFlow.cc:127-150) and Flux::setFlow (Flux.cc:101-129):
+   q0 = free stream of magnitude mag at angle alpha plus the rotation -thetadot
+   about the body centre, at the edge coordinates of every level
+   (Grid.cc:104-139 with zero shift) */
+static void bf_move(og_sim* s, double t) {
+    double o[6];
+    body_transform(&s->bf, t, o);
+    bf_set_alpha_mag(s, t);
+    double ca, sa;
+    sin_cos(s->bf_alpha, &sa, &ca);
+    s->bf_u = s->bf_mag * ca;
+    s->bf_v = s->bf_mag * sa;
+    s->bf_w = -1. * o[5];
+    const int nx = s->nx, ny = s->ny;
+    for (int lev = 0; lev < s->ngrid; ++lev) {
+        const double d = s->dx * (1 << lev);
+        const double x0 = s->xoff + 0.5 * (0. - 1) * ((1 << lev) - 1) * (nx * s->dx);
+        const double y0 = s->yoff + 0.5 * (0. - 1) * ((1 << lev) - 1) * (ny * s->dx);
+        double* q = s->q0 + lev * NF(nx, ny);
+        for (int i = 0; i <= nx; ++i)
+            for (int j = 0; j < ny; ++j) {
+                const double yd = (y0 + (j + 0.5) * d) - s->bf.yc;
+                q[(long)i * ny + j] = (s->bf_u - s->bf_w * yd) * d;
+            }
+        for (int i = 0; i < nx; ++i)
+            for (int j = 0; j <= ny; ++j) {
+                const double xd = (x0 + (i + 0.5) * d) - s->bf.xc;
+                q[NFX(nx, ny) + (long)i * (ny + 1) + j] = (s->bf_v + s->bf_w * xd) * d;
+            }
+    }
+}
+
 /* getConstraints, NavierStokesModel.cc:86-91,142-146 */
 static void get_constraints(og_sim* s, double* b) {
     double* b0 = (double*)malloc(sizeof(double) * 2 * s->nb);
@@ -1102,6 +1330,7 @@ og_sim* og_sim_create(int nx, int ny, int ngrid, double length, double xoff, dou
     s->Re = Re; s->dt = dt; s->scheme = scheme;
     double pi = 4. * atan(1.);
     s->alpha_rad = alpha_deg * pi / 180.;
+    s->force_alpha = s->alpha_rad;
     s->tol = cg_tol; s->force_cg = force_cg;
     switch (scheme) {   /* Scheme.h:14-56 */
     case 0: s->nsub = 1; s->an[0] = 1.; s->bn[0] = 0.; s->cn[0] = 1.; break;
@@ -1120,7 +1349,8 @@ og_sim* og_sim_create(int nx, int ny, int ngrid, double length, double xoff, dou
     }
     if (parse_geom(s, geom_text ? geom_text : "") != 0) { og_sim_destroy(s); return NULL; }
     s->stationary = 1;
-    for (int b = 0; b < s->nbody; ++b) if (s->bodies[b].motion == 2) s->stationary = 0;
+    for (int b = 0; b < s->nbody; ++b)   /* only FixedPosition is stationary (Motion.h:28, FixedPosition.h:39) */
+        if (s->bodies[b].motion != OG_NONE && s->bodies[b].motion != OG_FIXED) s->stationary = 0;
     s->use_cg = (!s->stationary) || force_cg;
     const int nb = s->nb;
     s->px = (double*)calloc(nb + 1, sizeof(double)); s->py = (double*)calloc(nb + 1, sizeof(double));
@@ -1135,7 +1365,9 @@ og_sim* og_sim_create(int nx, int ny, int ngrid, double length, double xoff, dou
     s->q0 = (double*)calloc(f_n(s), sizeof(double));
     s->f = (double*)calloc(2 * nb + 1, sizeof(double));
     /* BaseFlow(grid, 1, alpha) -> Flux::UniformFlow, Flux.cc:79-99 */
-    const double u = 1. * cos(s->alpha_rad), v = 1. * sin(s->alpha_rad);
+    double ca, sa;
+    sin_cos(s->alpha_rad, &sa, &ca);
+    const double u = 1. * ca, v = 1. * sa;
     for (int lev = 0; lev < ngrid; ++lev) {
         const double d = s->dx * (1 << lev);
         for (long k = 0; k < NFX(nx, ny); ++k) s->q0[lev * NF(nx, ny) + k] = u * d;
@@ -1146,11 +1378,49 @@ og_sim* og_sim_create(int nx, int ny, int ngrid, double length, double xoff, dou
 
 void og_sim_destroy(og_sim* s) {
     if (!s) return;
+    for (int b = 0; b < s->nbody; ++b) free(s->bodies[b].table);
+    free(s->bf.table);
     free(s->refx); free(s->refy); free(s->px); free(s->py); free(s->pu); free(s->pv);
     free(s->abi); free(s->afi); free(s->aw);
     free(s->omega); free(s->nprev); free(s->q); free(s->q0); free(s->f);
     for (int i = 0; i < 4; ++i) { free(s->chol_diag[i]); free(s->chol_low[i]); }
     free(s);
+}
+
+/* ibpm.cc:166-178 with -ubf: a single body's motion moves to the base flow
+   (Geometry::transferMotion / transferCenter, Geometry.cc:46-77); the body
+   stays at its reference points and the projection uses Cholesky.  Call
+   before og_sim_init.  Returns 1 if the motion moved, 0 for a stationary
+   body, -1 for more than one body (the reference exits). */
+int og_sim_set_ubf(og_sim* s) {
+    if (s->nbody != 1) { snprintf(g_err, sizeof g_err, "ubf: only for a single moving body"); return -1; }
+    if (s->stationary) return 0;
+    s->bf = s->bodies[0];
+    s->bodies[0].table = NULL;
+    s->bodies[0].motion = OG_NONE;
+    s->stationary = 1;
+    s->use_cg = s->force_cg;
+    s->ubf = 1;
+    return 1;
+}
+
+/* The base flux after moveFlow at each time of a sequence (all levels, the
+   Flux layout) and (alpha, mag) after it, for a geometry with one moving
+   body under -ubf. */
+int og_baseflow_sequence(const char* geom, int nx, int ny, int ngrid, double length, double xoff, double yoff,
+                         double alpha_deg, int ntimes, const double* times, double* q_out, double* am_out) {
+    og_sim* s = og_sim_create(nx, ny, ngrid, length, xoff, yoff, 100.0, 0.01, 2, alpha_deg, geom, 1e-7, 0);
+    if (!s) return -1;
+    if (og_sim_set_ubf(s) != 1) { og_sim_destroy(s); return -1; }
+    const long n = f_n(s);
+    for (int c = 0; c < ntimes; ++c) {
+        bf_move(s, times[c]);
+        memcpy(q_out + n * c, s->q0, sizeof(double) * n);
+        am_out[2 * c] = s->bf_alpha;
+        am_out[2 * c + 1] = s->bf_mag;
+    }
+    og_sim_destroy(s);
+    return 0;
 }
 
 int og_sim_num_points(const og_sim* s) { return s->nb; }
@@ -1161,7 +1431,8 @@ int og_sim_init(og_sim* s) {
     reg_update(s);             /* model->init(), NavierStokesModel.cc:52-57 */
     if (!s->use_cg)
         for (int i = 0; i < s->nsub; ++i) chol_build(s, i);   /* solver->init(), ibpm.cc:291-297 */
-    if (!s->stationary) { move_bodies(s, s->time); reg_update(s); }   /* updateOperators(t) */
+    if (s->ubf) bf_move(s, s->time);                                  /* updateOperators(t) */
+    if (!s->stationary) { move_bodies(s, s->time); reg_update(s); }
     refresh_state(s);          /* ibpm.cc:304-305 */
     return 0;
 }
@@ -1194,6 +1465,7 @@ int og_sim_advance(og_sim* s) {
     double* b = (double*)malloc(sizeof(double) * (2 * s->nb + 1));
     for (int i = 0; i < s->nsub; ++i) {
         og_nonlinear(s->nx, s->ny, s->ngrid, s->dx, s->q, s->omega, nl);
+        if (s->ubf) bf_move(s, s->time + s->cn[i] * s->dt);   /* NavierStokesModel.cc:93-99 */
         if (!s->stationary) {
             move_bodies(s, s->time + s->cn[i] * s->dt);
             reg_update(s);
@@ -1216,6 +1488,10 @@ int og_sim_advance(og_sim* s) {
     }
     s->time += s->dt;
     s->timestep++;
+    if (s->ubf) {   /* ibpm.cc:347-351: forces rotated by the base-flow angle at the new time */
+        bf_set_alpha_mag(s, s->time);
+        s->force_alpha = s->bf_alpha;
+    }
     free(nl); free(a); free(b);
     return 0;
 }
@@ -1226,8 +1502,10 @@ void og_sim_forces(const og_sim* s, double* cd, double* cl) {
     for (int k = 0; k < s->nb; ++k) { xF += s->f[k]; yF += s->f[s->nb + k]; }
     const double dx2 = s->dx * s->dx;
     xF *= dx2; yF *= dx2;
-    const double drag = xF * cos(s->alpha_rad) + yF * sin(s->alpha_rad);
-    const double lift = xF * -1. * sin(s->alpha_rad) + yF * cos(s->alpha_rad);
+    double ca, sa;
+    sin_cos(s->force_alpha, &sa, &ca);
+    const double drag = xF * ca + yF * sa;
+    const double lift = xF * -1. * sa + yF * ca;
     *cd = 2 * drag; *cl = 2 * lift;
 }
 
@@ -1241,6 +1519,25 @@ int og_geom_points(const char* geom, double t, double* xy, double* uv, int cap) 
         uv[k] = s->pu[k]; uv[s->nb + k] = s->pv[k];
     }
     int n = s->nb;
+    og_sim_destroy(s);
+    return n;
+}
+
+/* moveBodies at each time of a sequence, in order (the motions that integrate
+   keep their state between calls); points [x.., y..] and velocities
+   [u.., v..] of call k at offset 2 n k. */
+int og_geom_sequence(const char* geom, int ntimes, const double* times, double* xy, double* uv, int cap) {
+    og_sim* s = og_sim_create(8, 8, 1, 1.0, 0.0, 0.0, 100.0, 0.01, 2, 0.0, geom, 1e-7, 0);
+    if (!s) return -1;
+    const int n = s->nb;
+    if (2L * n * ntimes > cap) { og_sim_destroy(s); return -2; }
+    for (int c = 0; c < ntimes; ++c) {
+        move_bodies(s, times[c]);
+        for (int k = 0; k < n; ++k) {
+            xy[2L * n * c + k] = s->px[k]; xy[2L * n * c + n + k] = s->py[k];
+            uv[2L * n * c + k] = s->pu[k]; uv[2L * n * c + n + k] = s->pv[k];
+        }
+    }
     og_sim_destroy(s);
     return n;
 }

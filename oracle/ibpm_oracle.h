@@ -74,6 +74,7 @@ void og_reg_toboundary(int npts, double dx, int nassoc, const int* bidx, const i
 /* Points [x..,y..] and velocities [u..,v..] of the bodies in a .geom text after
    Geometry::moveBodies(t) (Geometry.cc:115, RigidBody.cc:586-609). */
 int og_geom_points(const char* geom, double t, double* xy, double* uv, int cap);
+int og_geom_sequence(const char* geom, int ntimes, const double* times, double* xy, double* uv, int cap);
 
 /* ---- whole-solver simulation -------------------------------------------- */
 typedef struct og_sim og_sim;
@@ -84,6 +85,9 @@ og_sim* og_sim_create(int nx, int ny, int ngrid, double length, double xoff, dou
                       const char* geom_text, double cg_tol, int force_cg);
 void og_sim_destroy(og_sim* s);
 const char* og_sim_error(void);
+int  og_sim_set_ubf(og_sim* s);
+int  og_baseflow_sequence(const char* geom, int nx, int ny, int ngrid, double length, double xoff, double yoff,
+                          double alpha_deg, int ntimes, const double* times, double* q_out, double* am_out);
 int  og_sim_num_points(const og_sim* s);
 int  og_sim_uses_cg(const og_sim* s);
 /* ibpm.cc:283-305: move bodies to t, regulariser update, build solvers, refresh */

@@ -41,7 +41,7 @@ def lib():
                                     ctypes.c_int, ctypes.c_double, ctypes.c_char_p, ctypes.c_double,
                                     ctypes.c_int]
         for name in ["og_sim_destroy", "og_sim_init", "og_sim_init_timing", "og_sim_advance",
-                     "og_sim_num_points", "og_sim_uses_cg"]:
+                     "og_sim_num_points", "og_sim_uses_cg", "og_sim_set_ubf"]:
             getattr(L, name).argtypes = [ctypes.c_void_p]
         L.og_sim_error.restype = ctypes.c_char_p
         L.og_sim_time.restype = ctypes.c_double
@@ -223,7 +223,7 @@ class Sim:
     """Whole-solver oracle: one flow advanced by the restated IBSolver."""
 
     def __init__(self, nx, ny, ngrid, length, xoffset, yoffset, Re, dt, geom,
-                 scheme="rk3", alpha_deg=0.0, tol=1e-7, force_cg=False):
+                 scheme="rk3", alpha_deg=0.0, tol=1e-7, force_cg=False, ubf=False):
         L = lib()
         self.nx, self.ny, self.ngrid = nx, ny, ngrid
         self.h = L.og_sim_create(nx, ny, ngrid, length, xoffset, yoffset, Re, dt, SCHEMES[scheme],
@@ -231,6 +231,11 @@ class Sim:
         if not self.h:
             raise ValueError("oracle: " + L.og_sim_error().decode())
         self.nb = L.og_sim_num_points(self.h)
+        self.ubf = 0
+        if ubf:   # ibpm -ubf: 1 if the body's motion moved to the base flow
+            self.ubf = L.og_sim_set_ubf(self.h)
+            if self.ubf < 0:
+                raise ValueError("oracle: " + L.og_sim_error().decode())
         self.nsub = 1 if scheme in ("euler", "ab2") else (4 if scheme == "rk3b" else 3)
 
     def __del__(self):
@@ -344,6 +349,35 @@ def geom_points(geom, t, cap=100000):
     if n < 0:
         raise ValueError("oracle: bad geometry")
     return xy[:2 * n].copy(), uv[:2 * n].copy()
+
+
+def geom_sequence(geom, times):
+    """moveBodies at each time in order (stateful motions integrate over the
+    calls); returns points (ntimes, 2n) [x.., y..] and velocities (ntimes, 2n)."""
+    times = np.ascontiguousarray(times, dtype=np.float64)
+    cap = 1 << 22
+    xy = np.zeros(cap)
+    uv = np.zeros(cap)
+    n = lib().og_geom_sequence(geom.encode(), len(times), _p(times), _p(xy), _p(uv), cap)
+    if n < 0:
+        raise ValueError("oracle: bad geometry" if n == -1 else "oracle: sequence too long")
+    k = 2 * n * len(times)
+    return xy[:k].reshape(len(times), 2 * n).copy(), uv[:k].reshape(len(times), 2 * n).copy()
+
+
+def baseflow_sequence(geom, nx, ny, ngrid, length, xoff, yoff, alpha_deg, times):
+    """Unsteady base flow of a one-body geometry: the base flux (all levels)
+    after moveFlow at each time, and (alpha, mag) after it."""
+    times = np.ascontiguousarray(times, dtype=np.float64)
+    nf = ngrid * (2 * nx * ny + nx + ny)
+    q = np.zeros(nf * len(times))
+    am = np.zeros(2 * len(times))
+    D = ctypes.c_double
+    rc = lib().og_baseflow_sequence(geom.encode(), nx, ny, ngrid, D(length), D(xoff), D(yoff), D(alpha_deg),
+                                    len(times), _p(times), _p(q), _p(am))
+    if rc != 0:
+        raise ValueError("oracle: base flow needs one moving body")
+    return q.reshape(len(times), nf), am.reshape(len(times), 2)
 
 
 def ip_scalar(nx, ny, ngrid, dx, f, g):

@@ -1,11 +1,12 @@
 // ref_driver.cc -- extern "C" shim over the FFTW-free part of the reference
 // (Grid, Scalar, Flux, BC, BoundaryVector, Regularizer, Geometry, RigidBody,
-// State), compiled from the sources where they lie under the reference's src/
+// State, BaseFlow, the Motion classes), compiled from the sources where they lie under the reference's src/
 // by oracle/Makefile into oracle/_ref/libibpm_refpart.so.
 //
 // TEST INFRASTRUCTURE ONLY: it pins the CPU restatement (ibpm_oracle.c)
 // against the reference's own code for coarsify/getBC, the regulariser, the
-// .geom parser with body motion, the uniform base flow and the net force.
+// .geom parser with every body motion, the uniform and the unsteady base
+// flow and the net force.
 // The elliptic solvers, VectorOperations and the projection solvers include
 // FFTW3, which this image lacks, so they are not part of this build.
 #include "Grid.h"
@@ -16,6 +17,8 @@
 #include "Regularizer.h"
 #include "Geometry.h"
 #include "State.h"
+#include "BaseFlow.h"
+#include "Motion.h"
 #include <cstring>
 #include <string>
 
@@ -79,6 +82,27 @@ int ref_geometry(const char* path, double t, double* xy, double* uv, int cap) {
     return n;
 }
 
+// Loads a .geom file and calls moveBodies at each time of a sequence, in order
+// (FixedVelocity, Eldredge2 and EldredgeCombined2 integrate over the calls);
+// call k writes points [x.., y..] and velocities [u.., v..] at offset 2 n k.
+// Returns the number of points per call or -1.
+int ref_geometry_sequence(const char* path, int ntimes, const double* times, double* xy, double* uv, int cap) {
+    Geometry geom;
+    if (!geom.load(std::string(path))) return -1;
+    int n = geom.getNumPoints();
+    if (2L * n * ntimes > cap) return -2;
+    for (int c = 0; c < ntimes; ++c) {
+        geom.moveBodies(times[c]);
+        BoundaryVector p = geom.getPoints();
+        BoundaryVector v = geom.getVelocities();
+        for (int i = 0; i < n; ++i) {
+            xy[2L * n * c + i] = p(X, i); xy[2L * n * c + n + i] = p(Y, i);
+            uv[2L * n * c + i] = v(X, i); uv[2L * n * c + n + i] = v(Y, i);
+        }
+    }
+    return n;
+}
+
 int ref_geometry_stationary(const char* path) {
     Geometry geom;
     if (!geom.load(std::string(path))) return -1;
@@ -119,6 +143,36 @@ void ref_uniform_flow(int nx, int ny, int ngrid, double length, double xoff, dou
     long nf = 2L * nx * ny + nx + ny;
     for (int lev = 0; lev < ngrid; ++lev)
         for (long k = 0; k < nf; ++k) q_out[lev * nf + k] = q(lev, (int)k);
+}
+
+// Unsteady base flow of ibpm.cc:166-178 for the one-body geometry in `path`:
+// BaseFlow(grid, 1, alpha) takes the body's motion and centre, then moveFlow
+// at each time of a sequence; writes the base flux (all levels, nf per level)
+// and (getAlpha, getMag) after each call.  Returns 0, or -1 when the geometry
+// is not a single moving body.
+int ref_baseflow_sequence(const char* path, int nx, int ny, int ngrid, double length, double xoff, double yoff,
+                          double alpha, int ntimes, const double* times, double* q_out, double* am_out) {
+    Grid grid(nx, ny, ngrid, length, xoff, yoff);
+    Geometry geom;
+    if (!geom.load(std::string(path))) return -1;
+    if (geom.isStationary() || geom.getNumBodies() != 1) return -1;
+    BaseFlow q(grid, 1.0, alpha);
+    Motion* m = geom.transferMotion();
+    double xC = 0, yC = 0;
+    geom.transferCenter(xC, yC);
+    q.setMotion(*m);
+    q.setCenter(xC, yC);
+    const long nf = 2L * nx * ny + nx + ny;
+    for (int c = 0; c < ntimes; ++c) {
+        q.moveFlow(times[c]);
+        Flux f = q.getFlux();
+        for (int lev = 0; lev < ngrid; ++lev)
+            for (long k = 0; k < nf; ++k) q_out[(long)c * ngrid * nf + lev * nf + k] = f(lev, (int)k);
+        am_out[2 * c] = q.getAlpha();
+        am_out[2 * c + 1] = q.getMag();
+    }
+    delete m;
+    return 0;
 }
 
 // State::computeNetForce (State.cc:53-63).

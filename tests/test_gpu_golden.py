@@ -120,6 +120,29 @@ def test_ibpm_driver_moving_plate_precond(gpu, oracle):
         assert abs(full[k, 2] - cd) <= 1e-7 * scale and abs(full[k, 3] - cl) <= 1e-7 * scale
 
 
+def test_ibpm_driver_unsteady_base_flow(gpu, oracle):
+    """The driver with -ubf 1 on a pitching plate and -alpha 5: the force file
+    holds drag and lift rotated by the base-flow angle of each step
+    (ibpm.cc:347-351), equal to the oracle's body-frame run."""
+    exe = os.path.join(os.path.dirname(HERE), "ibpm_amd", "ibpm")
+    with tempfile.TemporaryDirectory() as d:
+        gpath = os.path.join(d, "plate.geom")
+        open(gpath, "w").write(PLATE)
+        r = subprocess.run([exe, "-geom", gpath, "-nx", "128", "-ny", "128", "-ngrid", "3", "-Re", "300",
+                            "-dt", "0.005", "-nsteps", "5", "-alpha", "5", "-ubf", "1",
+                            "-outdir", d, "-name", "plate"], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        full = np.loadtxt(os.path.join(d, "plate.force17"))
+    s = oracle.Sim(128, 128, 3, 4.0, -2.0, -2.0, 300.0, 0.005, PLATE, alpha_deg=5.0, ubf=True)
+    assert s.ubf == 1
+    s.init()
+    for k in range(5):
+        s.advance(1)
+        cd, cl = s.forces()
+        scale = max(abs(cd), abs(cl))
+        assert abs(full[k, 2] - cd) <= 1e-9 * scale and abs(full[k, 3] - cl) <= 1e-9 * scale
+
+
 @pytest.mark.parametrize("name", ["plate", "cfg4short"])
 @pytest.mark.parametrize("pairs", [0, 8])
 def test_preconditioned_cg_matches_golden(gpu, name, pairs):

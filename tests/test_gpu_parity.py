@@ -181,6 +181,80 @@ def test_moving_plate_cg_matches_oracle(gpu, oracle):
     assert rel(b.omega(), s.omega()) < 1e-7
 
 
+@pytest.mark.parametrize("motion", [
+    "fixedvel 0.3 -0.1 0.5",
+    "eldredge 40 11 0 0.02 0.03 0.05",
+    "eldredge2 0.3 11 0 0.02 0.03 0.05",
+    "eldredgecombined2 0.3 11 0 0.02 0.03 0.05 30 9 0 0.02 0.03 0.05",
+    "lagstep2 20 0.02 0.01 0.005",
+    "sigmoidalstep 30 0.04 0.0",
+    "motionfile",
+])
+def test_other_motions_match_oracle(gpu, oracle, tmp_path, motion):
+    """A plate under each remaining Motion type for 4 steps: the library's
+    host kinematics (stateful ones included) feed the GPU path exactly as the
+    oracle's feed its solver, so the forces agree to the CG tolerance."""
+    if motion == "motionfile":
+        path = tmp_path / "plate.motion"
+        t = np.linspace(0.0, 0.05, 6)
+        rows = np.stack([t, 0.5 * t, -0.2 * t, 3.0 * t, 0.5 + 0 * t, -0.2 + 0 * t, 3.0 + 0 * t], 1)
+        path.write_text("%d\n" % len(rows) + "".join(" ".join("%.17g" % v for v in r) + "\n" for r in rows))
+        motion = "motionfile %s" % path
+    geom = "body P\n line_n -0.5 0 0.5 0 16\n center 0.25 0\n motion %s\nend\n" % motion
+    ctx, b, s = _pair(gpu, oracle, 64, 3, 300.0, 0.01, geom, tol=1e-10)
+    for step in range(4):
+        b.advance(1)
+        s.advance(1)
+        cd_g, cl_g = b.forces()
+        cd_o, cl_o = s.forces()
+        scale = max(abs(cd_o), abs(cl_o))
+        assert abs(cd_g - cd_o) <= 1e-7 * scale, (motion, step)
+        assert abs(cl_g - cl_o) <= 1e-7 * scale, (motion, step)
+    assert rel(b.omega(), s.omega()) < 1e-7
+
+
+@pytest.mark.parametrize("motion", [
+    "pitchplunge 0.3 0.5 0 0.1 0.5 0.2",
+    "fixedvel -0.2 0.1 0.3",
+    "eldredgecombined2 0.3 11 0 0.02 0.03 0.05 30 9 0 0.02 0.03 0.05",
+])
+def test_unsteady_base_flow_matches_oracle(gpu, oracle, motion):
+    """-ubf: the body stays put (Cholesky path) and the per-substep base flux
+    carries its motion; per-step forces are rotated by the base-flow angle.
+    Graph replay is not used for such a batch, so asking for it changes
+    nothing."""
+    geom = "body P\n line_n -0.5 0 0.5 0 16\n center 0.25 0\n motion %s\nend\n" % motion
+    ctx = gpu.Context(0)
+    fl = gpu.Flow(64, 64, 3, 4.0, -2.0, -2.0, 300.0, 0.01, geom, alpha=5.0, ubf=True)
+    b = gpu.Batch(ctx, [fl]).init()
+    g = gpu.Batch(ctx, [fl]).init().use_graph(True)
+    s = oracle.Sim(64, 64, 3, 4.0, -2.0, -2.0, 300.0, 0.01, geom, alpha_deg=5.0, ubf=True)
+    assert s.ubf == 1
+    s.init()
+    want = []
+    for step in range(6):
+        b.advance(1)
+        g.advance(1)
+        s.advance(1)
+        cd_o, cl_o = s.forces()
+        want.append((cd_o, cl_o))
+        cd_g, cl_g = b.forces()
+        scale = max(abs(cd_o), abs(cl_o))
+        assert abs(cd_g - cd_o) <= 1e-9 * scale, (motion, step)
+        assert abs(cl_g - cl_o) <= 1e-9 * scale, (motion, step)
+        assert np.array_equal(b.omega(), g.omega())
+    assert np.abs(b.force_history() - np.array(want)).max() <= 1e-9 * np.abs(want).max()
+    assert rel(b.omega(), s.omega()) < 1e-9
+    assert b.base_flow()[2] is True
+
+
+def test_unsteady_base_flow_needs_a_single_body(gpu):
+    ctx = gpu.Context(0)
+    two = "body P\n line_n -0.5 0 0.5 0 16\n motion fixedvel 0.1 0 0\nend\nbody Q\n point 0.5 0.5\nend\n"
+    with pytest.raises(gpu.IbpmError, match="single moving body"):
+        gpu.Batch(ctx, [gpu.Flow(64, 64, 2, 4.0, -2.0, -2.0, 100.0, 0.01, two, ubf=True)])
+
+
 def test_restart_resume_is_bitwise_identical(gpu):
     """State after 3 steps loaded into a fresh batch (restart), 3 more steps:
     identical to 6 straight steps (Cholesky/RK3: no hidden multistep state)."""

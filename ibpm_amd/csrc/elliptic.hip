@@ -27,6 +27,15 @@ __device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x -
 // -i * a
 __device__ inline double2 cmi(double2 a) { return make_double2(a.y, -a.x); }
 
+// Field pointers held in job descriptors are generic (flat) pointers; the
+// transforms address them as global memory so that loads and stores are
+// global instructions, which do not share the LDS counter and may move 16
+// bytes at 8-byte alignment.
+#define IBPM_GLOBAL __attribute__((address_space(1)))
+__device__ __forceinline__ const IBPM_GLOBAL double* gin(const double* p) { return (const IBPM_GLOBAL double*)p; }
+__device__ __forceinline__ IBPM_GLOBAL double* gout(double* p) { return (IBPM_GLOBAL double*)p; }
+typedef double DoublePair __attribute__((ext_vector_type(2), aligned(8)));
+
 template <int R> __device__ inline void dft(double2* v);
 
 template <> __device__ inline void dft<2>(double2* v) {
@@ -90,6 +99,42 @@ template <> __device__ inline void dft<8>(double2* v) {
     v[1] = cadd(a[1], t1); v[5] = csub(a[1], t1);
     v[2] = cadd(a[2], t2); v[6] = csub(a[2], t2);
     v[3] = cadd(a[3], t3); v[7] = csub(a[3], t3);
+}
+
+// 16-point DFT as 4 x 4: X[k1 + 4 k2] = sum_n2 W16^(n2 k1) W4^(n2 k2)
+// sum_n1 x[4 n1 + n2] W4^(n1 k1).
+__device__ inline double2 mul_w16(double2 a, int k) {   // a * exp(-2 pi i k / 16), k in {1,2,3,6,9}
+    const double c1 = 0.92387953251128675613;   // cos(pi / 8)
+    const double s1 = 0.38268343236508977173;   // sin(pi / 8)
+    const double h = 0.70710678118654752440;    // cos(pi / 4)
+    switch (k) {
+    case 1: return make_double2(c1 * a.x + s1 * a.y, c1 * a.y - s1 * a.x);
+    case 2: return make_double2(h * (a.x + a.y), h * (a.y - a.x));
+    case 3: return make_double2(s1 * a.x + c1 * a.y, s1 * a.y - c1 * a.x);
+    case 6: return make_double2(h * (a.y - a.x), -h * (a.x + a.y));
+    default: return make_double2(-c1 * a.x - s1 * a.y, s1 * a.x - c1 * a.y);   // 9
+    }
+}
+
+__device__ inline void dft16(double2* v) {
+    double2 a[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+        double2 c[4] = {v[n2], v[n2 + 4], v[n2 + 8], v[n2 + 12]};
+        dft<4>(c);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) a[n2][k1] = c[k1];
+    }
+    a[1][1] = mul_w16(a[1][1], 1); a[1][2] = mul_w16(a[1][2], 2); a[1][3] = mul_w16(a[1][3], 3);
+    a[2][1] = mul_w16(a[2][1], 2); a[2][2] = cmi(a[2][2]);        a[2][3] = mul_w16(a[2][3], 6);
+    a[3][1] = mul_w16(a[3][1], 3); a[3][2] = mul_w16(a[3][2], 6); a[3][3] = mul_w16(a[3][3], 9);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        double2 c[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
+        dft<4>(c);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = c[k2];
+    }
 }
 
 // FFT lengths with a compiled plan (M = 2(N+1) for N = 199, 255, 383, 399,
@@ -243,6 +288,115 @@ __device__ inline void fft_any(double2* buf, const double2* tw, const FftPlan& P
     else fft_rt(buf, tw, P);
 }
 
+// ------------------------------------------- 2048-point transforms in registers
+// The 1023-point grids (M = 2048) run one row pair per 128-thread workgroup as
+// three radix stages 16 x 16 x 8 held in registers: the first stage's inputs
+// z[t + 128 r] come straight from global memory, the two exchanges between
+// stages go through LDS, and the last stage leaves Z[t + 128 m], m < 8 -- the
+// half of the spectrum a DST-I needs -- in registers for the caller to write.
+// That is 4 LDS sweeps of the 32 KB buffer per transform instead of 10.
+constexpr int kR16Threads = 128;
+
+template <int CM>
+__host__ __device__ constexpr int pass_threads() { return CM == 2048 ? kR16Threads : kFftThreads; }
+
+// LDS slot of element e: one pad slot after every 16, so the first stage's
+// writes (16 consecutive elements per lane) spread over the banks, and every
+// access of a stage is a per-lane base plus a compile-time offset.
+constexpr int kR16Buf = 2048 + 128;
+__device__ inline int pad16(int e) { return e + (e >> 4); }
+
+// The second stage's twiddles in LDS, w^(8 jm r) at 16 r + jm (r, jm < 16),
+// w = exp(-2 pi i / 2048), so 16 lanes with consecutive jm read consecutive
+// entries.  The third stage reads w^j, j < 256, from the global table.
+constexpr int kR16Tw = 256;
+__device__ inline void load_tw16(double2* tw, const FftPlan& P) {
+    const IBPM_GLOBAL double* g = gin((const double*)P.tw);
+    for (int q = threadIdx.x; q < 256; q += kR16Threads) {
+        const int r = q >> 4, jm = q & 15, e = (8 * jm * r) & 2047;
+        tw[q] = make_double2(g[2 * e], g[2 * e + 1]);
+    }
+}
+
+// Second half of the odd extension from the first: v[r] = z[t + 128 r] is
+// set for r < 8; z[e] = -z[2048 - e] (another lane's) for e > 1024 comes
+// through LDS, z[1024] = 0.  The buffer is free again on return.
+__device__ __forceinline__ void mirror_second_half(double2* v, double2* buf) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) buf[t + 128 * r] = v[r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 8; r < 16; ++r) {
+        const int e = t + 128 * r;
+        if (e == 1024) {
+            v[r] = make_double2(0.0, 0.0);
+        } else {
+            const double2 y = buf[2048 - e];
+            v[r] = make_double2(-y.x, -y.y);
+        }
+    }
+    __syncthreads();
+}
+
+// First half of the odd extension of two rows a, b (b absent: zeros):
+// v[r] = (a, b)[e - 1] at e = t + 128 r, r < 8, and 0 at e = 0.
+__device__ __forceinline__ void load_first_half(double2* v, const IBPM_GLOBAL double* a,
+                                                const IBPM_GLOBAL double* b, bool hb) {
+    const unsigned t = threadIdx.x;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const unsigned e = t + 128u * r, k = e == 0 ? 0u : e - 1;
+        const double xa = a[k], xb = hb ? b[k] : 0.0;
+        v[r] = e == 0 ? make_double2(0.0, 0.0) : make_double2(xa, xb);
+    }
+}
+
+// In: v[r] = z[t + 128 r], r < 16.  Out: o[m] = Z[t + 128 m], m < 8.  The
+// buffer is free again on return.
+__device__ __forceinline__ void fft2048_regs(double2* v, double2* o, double2* buf, const double2* tw,
+                                             const IBPM_GLOBAL double* twg) {
+    const int t = threadIdx.x;
+    dft16(v);   // stage 1 (Ns = 1): outputs to 16 t + r, slots 17 t + r
+    double2* b1 = buf + 17 * t;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) b1[r] = v[r];
+    __syncthreads();
+    const double2* b2 = buf + pad16(t);   // reads t + 128 r: slots pad16(t) + 136 r
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = b2[136 * r];
+    __syncthreads();
+    const int jm = t & 15;   // stage 2 (Ns = 16): twiddles w^(8 jm r) from the table
+#pragma unroll
+    for (int r = 1; r < 16; ++r) v[r] = cmul(v[r], tw[16 * r + jm]);
+    dft16(v);
+    double2* b3 = buf + (t >> 4) * 272 + jm;   // outputs (t/16) 256 + jm + 16 r
+#pragma unroll
+    for (int r = 0; r < 16; ++r) b3[17 * r] = v[r];
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {   // stage 3 (Ns = 256): butterflies j = t, t + 128
+        const int j = t + 128 * h;
+        double2 u[8];
+        const double2* b4 = buf + pad16(j);   // reads j + 256 r: slots pad16(j) + 272 r
+#pragma unroll
+        for (int r = 0; r < 8; ++r) u[r] = b4[272 * r];
+        const double2 w1 = make_double2(twg[2 * j], twg[2 * j + 1]);
+        const double2 w2 = cmul(w1, w1), w3 = cmul(w2, w1), w4 = cmul(w2, w2);
+        u[1] = cmul(u[1], w1);
+        u[2] = cmul(u[2], w2);
+        u[3] = cmul(u[3], w3);
+        u[4] = cmul(u[4], w4);
+        u[5] = cmul(u[5], cmul(w4, w1));
+        u[6] = cmul(u[6], cmul(w4, w2));
+        u[7] = cmul(u[7], cmul(w4, w3));
+        dft<8>(u);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) o[h + 2 * r] = u[r];   // Z[j + 256 r], r < 4
+    }
+    __syncthreads();
+}
+
 // ---------------------------------------------------------------- pass 1
 // getBC (reference Scalar.cc:91-127) evaluated at one boundary index.
 __device__ inline double bc_bottom(const EllJob& J, int i) {
@@ -306,8 +460,9 @@ __device__ inline void load_twiddles(double2* tw, const FftPlan& P) {
     for (int k = threadIdx.x; k < M; k += kFftThreads) tw[k] = P.tw[k];
 }
 
+
 template <int CM>
-__device__ inline void pass1_body(const EllJob& J, double2* smem, int bx) {
+__device__ inline void pass1_lds(const EllJob& J, double2* smem, int bx) {
     const FftPlan& P = J.pj;
     const int M = CM ? CM : P.M, np = CM ? np_of(CM) : P.np;
     const int N1 = J.nx - 1, N2 = M / 2 - 1;
@@ -342,7 +497,7 @@ __device__ inline void pass1_body(const EllJob& J, double2* smem, int bx) {
 }
 
 template <int CM>
-__device__ inline void pass2_body(const EllJob& J, double2* smem, int bx) {
+__device__ inline void pass2_lds(const EllJob& J, double2* smem, int bx) {
     const FftPlan& P = J.pi;
     const int M = CM ? CM : P.M, np = CM ? np_of(CM) : P.np;
     const int N1 = M / 2 - 1, N2 = J.ny - 1;
@@ -401,7 +556,7 @@ __device__ inline void pass2_body(const EllJob& J, double2* smem, int bx) {
 }
 
 template <int CM>
-__device__ inline void pass3_body(const EllJob& J, double2* smem, int bx) {
+__device__ inline void pass3_lds(const EllJob& J, double2* smem, int bx) {
     const FftPlan& P = J.pj;
     const int M = CM ? CM : P.M, np = CM ? np_of(CM) : P.np;
     const int N1 = J.nx - 1, N2 = M / 2 - 1;
@@ -448,6 +603,190 @@ __device__ inline void pass3_body(const EllJob& J, double2* smem, int bx) {
     }
 }
 
+// Two adjacent doubles of a transposed row pair written as one 16-byte
+// store (8-byte aligned), or the first alone for the last, odd row.
+__device__ __forceinline__ void store_pair(IBPM_GLOBAL double* p, double a, double b, bool both) {
+    if (both) *(IBPM_GLOBAL DoublePair*)p = DoublePair{a, b};   // one global_store_dwordx4
+    else p[0] = a;
+}
+
+// The three passes for M = 2048 (N = 1023), the same arithmetic as the LDS
+// passes above around a register-held transform, one row pair q per
+// workgroup: load(q, v) fills the first half of the odd extension, work(q, v)
+// transforms and writes.  (Two pairs per workgroup, the second's loads issued
+// before the first's transform, measured slower: fewer, longer workgroups.)
+constexpr int kR16Pairs = 1;
+
+template <class Load, class Work>
+__device__ __forceinline__ void r16_pairs(int bx, int npairs, const Load& load, const Work& work) {
+    (void)npairs;
+    double2 v[16];
+    load(bx, v);
+    work(bx, v);
+}
+
+__device__ __forceinline__ void pass1_r16(const EllJob& J, double2* smem, int bx) {
+    const int N1 = J.nx - 1, npairs = (N1 + 1) / 2;
+    if (kR16Pairs * bx >= npairs) return;
+    double2* buf = smem;
+    double2* tw = smem + kR16Buf;
+    load_tw16(tw, J.pj);
+    const IBPM_GLOBAL double* twg = gin((const double*)J.pj.tw);
+    const int t = threadIdx.x;
+    const int ey = nx_ext(J.ny), ex = nx_ext(J.nx);
+    const int jc0 = ey, jc1 = J.ny / 2 + ey;   // central columns: jc0 < j < jc1
+    const double sc = J.in_scale;
+    const bool bcs = J.uc || J.bcring;
+    // rhs_value along rows ra, rb = ra + 1 with the row-invariant parts
+    // hoisted: the first half of the odd extension, e = t + 128 r (r < 8) at
+    // column j = e; the two edge rows of the level keep the general form
+    auto load = [&](int q, double2* v) {
+        const int ra = 2 * q, rb = ra + 1;
+        const bool hb = rb < N1;
+        if ((ra == 0 || rb >= N1 - 1) && bcs) {
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                const int e = t + 128 * r;
+                v[r] = e == 0 ? make_double2(0.0, 0.0)
+                              : make_double2(rhs_value(J, ra + 1, e), hb ? rhs_value(J, rb + 1, e) : 0.0);
+            }
+            return;
+        }
+        const bool cra = J.lev >= 1 && ra + 1 > ex && ra + 1 < J.nx / 2 + ex;
+        const bool crb = hb && J.lev >= 1 && rb + 1 > ex && rb + 1 < J.nx / 2 + ex;
+        const IBPM_GLOBAL double* fa = gin(J.f) + sidx(J.nx, J.ny, ra + 1, 1);
+        const IBPM_GLOBAL double* fb = gin(J.f) + sidx(J.nx, J.ny, hb ? rb + 1 : ra + 1, 1);
+        const IBPM_GLOBAL double* Ra = gin(J.R) + sidx(J.nx, J.ny, ra + 1, 1);
+        const IBPM_GLOBAL double* Rb = gin(J.R) + sidx(J.nx, J.ny, hb ? rb + 1 : ra + 1, 1);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int e = t + 128 * r, k = e - 1;
+            const bool colc = e > jc0 && e < jc1;
+            const int kk = e == 0 ? 0 : k;   // e = 0 is the extension's zero
+            const double xa = ((cra && colc) ? Ra : fa)[kk] * sc;
+            const double xb = hb ? ((crb && colc) ? Rb : fb)[kk] * sc : 0.0;
+            v[r] = e == 0 ? make_double2(0.0, 0.0) : make_double2(xa, xb);
+        }
+        if (bcs) {   // getRHS at j = 1 (e = 1) and j = ny - 1 (e = 1023)
+            const double c = J.bc_coef;
+            if (t == 1) {
+                v[0].x -= bc_bottom(J, ra + 1) * c;
+                if (hb) v[0].y -= bc_bottom(J, rb + 1) * c;
+            }
+            if (t == 127) {
+                v[7].x -= bc_top(J, ra + 1) * c;
+                if (hb) v[7].y -= bc_top(J, rb + 1) * c;
+            }
+        }
+    };
+    auto work = [&](int q, double2* v) {
+        const int ra = 2 * q;
+        const bool hb = ra + 1 < N1;
+        double2 o[8];
+        mirror_second_half(v, buf);   // its barriers also publish the twiddles
+        fft2048_regs(v, o, buf, tw, twg);
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            const int kk = t + 128 * m;
+            if (kk == 0) continue;
+            store_pair(gout(J.T1) + (long)(kk - 1) * N1 + ra, -o[m].y, o[m].x, hb);
+        }
+    };
+    r16_pairs(bx, npairs, load, work);
+}
+
+__device__ __forceinline__ void pass2_r16(const EllJob& J, double2* smem, int bx) {
+    const int N1 = 1023, N2 = J.ny - 1, npairs = (N2 + 1) / 2;
+    if (kR16Pairs * bx >= npairs) return;
+    double2* buf = smem;
+    double2* tw = smem + kR16Buf;
+    load_tw16(tw, J.pi);
+    const IBPM_GLOBAL double* twg = gin((const double*)J.pi.tw);
+    const int t = threadIdx.x;
+    const IBPM_GLOBAL double* gcx = gin(J.cosx);
+    auto load = [&](int q, double2* v) {
+        const int la = 2 * q, lb = la + 1;
+        load_first_half(v, gin(J.T1) + (long)la * N1, gin(J.T1) + (long)lb * N1, lb < N2);
+    };
+    auto work = [&](int q, double2* v) {
+        const int la = 2 * q, lb = la + 1;
+        const bool hb = lb < N2;
+        double2 o[8];
+        mirror_second_half(v, buf);
+        fft2048_regs(v, o, buf, tw, twg);
+        // spectral scaling, EllipticSolver2d.cc:42-54 (eigenvalues), :89-93 (scale)
+        const double cya = gin(J.cosy)[la], cyb = hb ? gin(J.cosy)[lb] : 0.0;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            const int kk = t + 128 * m;
+            double ya = 0.0, yb = 0.0;
+            if (kk > 0) {
+                const double cx = gcx[kk - 1];
+                ya = -o[m].y;
+                yb = o[m].x;
+                const double lama = 2. * (cx + cya - 2.) * J.bydx2;
+                ya *= (J.kind == 0) ? 1. / lama : 1. / (1 + J.alpha * lama);
+                if (hb) {
+                    const double lamb = 2. * (cx + cyb - 2.) * J.bydx2;
+                    yb *= (J.kind == 0) ? 1. / lamb : 1. / (1 + J.alpha * lamb);
+                }
+            }
+            v[m] = make_double2(ya, yb);   // odd extension of the scaled spectrum
+        }
+        mirror_second_half(v, buf);
+        fft2048_regs(v, o, buf, tw, twg);
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            const int kk = t + 128 * m;
+            if (kk == 0) continue;
+            store_pair(gout(J.T2) + (long)(kk - 1) * N2 + la, -o[m].y, o[m].x, hb);
+        }
+    };
+    r16_pairs(bx, npairs, load, work);
+}
+
+__device__ __forceinline__ void pass3_r16(const EllJob& J, double2* smem, int bx) {
+    const int N1 = J.nx - 1, N2 = 1023, npairs = (N1 + 1) / 2;
+    if (kR16Pairs * bx >= npairs) return;
+    double2* buf = smem;
+    double2* tw = smem + kR16Buf;
+    load_tw16(tw, J.pj);
+    const IBPM_GLOBAL double* twg = gin((const double*)J.pj.tw);
+    const int t = threadIdx.x;
+    auto load = [&](int q, double2* v) {
+        const int ra = 2 * q, rb = ra + 1;
+        load_first_half(v, gin(J.T2) + (long)ra * N2, gin(J.T2) + (long)rb * N2, rb < N1);
+    };
+    auto work = [&](int q, double2* v) {
+        const int ra = 2 * q, rb = ra + 1;
+        const bool hb = rb < N1;
+        double2 o[8];
+        mirror_second_half(v, buf);
+        fft2048_regs(v, o, buf, tw, twg);
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            const int kk = t + 128 * m;
+            if (kk == 0) continue;
+            const int k = kk - 1;
+            // sinTransformInv normalisation, EllipticSolver2d.cc:80-83; with the
+            // epilogue the finer level's BCs come from v (EllipticSolver.cc:62-66)
+            {
+                const long o2 = (long)ra * N2 + k;
+                const double val_a = -o[m].y * J.norm;
+                gout(J.u)[o2] = J.epilogue ? gin(J.W)[o2] - val_a : val_a;
+                if (J.vside && on_bc_ring(J, ra + 1, k + 1)) gout(J.vside)[o2] = val_a;
+            }
+            if (hb) {
+                const long o2 = (long)rb * N2 + k;
+                const double val_b = o[m].x * J.norm;
+                gout(J.u)[o2] = J.epilogue ? gin(J.W)[o2] - val_b : val_b;
+                if (J.vside && on_bc_ring(J, rb + 1, k + 1)) gout(J.vside)[o2] = val_b;
+            }
+        }
+    };
+    r16_pairs(bx, npairs, load, work);
+}
+
 // Logical (row block, job) of this workgroup.  Workgroups are handed to the
 // 8 XCDs round-robin in launch order, so logical neighbours are mapped to the
 // same XCD: consecutive row blocks write neighbouring 16-byte pieces of the
@@ -462,6 +801,18 @@ __device__ inline int2 xcd_block() {
     return make_int2(lf % gx, lf / gx);
 }
 
+// Pass bodies by length: the register-held transform for M = 2048, the LDS
+// passes for every other length.
+#define IBPM_PASS_BODY(N)                                                                  \
+    template <int CM>                                                                      \
+    __device__ __forceinline__ void pass##N##_body(const EllJob& J, double2* smem, int bx) { \
+        if constexpr (CM == 2048) pass##N##_r16(J, smem, bx);                              \
+        else pass##N##_lds<CM>(J, smem, bx);                                               \
+    }
+IBPM_PASS_BODY(1)
+IBPM_PASS_BODY(2)
+IBPM_PASS_BODY(3)
+
 // One kernel per pass and FFT length (compiled lengths, 0 = runtime plan), so
 // each length gets its own register budget; a launch holds jobs of one length.
 // The step's own solves use k_ell_pass*; the solves inside the force solve
@@ -470,7 +821,7 @@ __device__ inline int2 xcd_block() {
 // does nothing (skipped()).
 #define IBPM_PASS_KERNEL(NAME, BODY)                                             \
     template <int CM>                                                            \
-    __global__ void __launch_bounds__(kFftThreads) NAME(const EllJob* __restrict__ jobs) { \
+    __global__ void __launch_bounds__(pass_threads<CM>()) NAME(const EllJob* __restrict__ jobs) { \
         extern __shared__ double2 smem[];                                         \
         const int2 blk = xcd_block();                                             \
         const EllJob& J = jobs[blk.y];                                            \
@@ -525,6 +876,7 @@ __global__ void __launch_bounds__(kThreads) k_fs_coarsify(const CoarsifyJob* __r
 // ------------------------------------------------------------ host launch
 int lds_for_pass(const EllJob& J, int pass) {
     const FftPlan& P = (pass == 2) ? J.pi : J.pj;
+    if (P.M == 2048) return (int)sizeof(double2) * (kR16Buf + kR16Tw);
     const int ntw = (compiled_len(P.M) && quarter_twiddles(P.M)) ? P.M / 4 : P.M;
     return (int)sizeof(double2) * (P.np * P.M + ntw);
 }
@@ -532,7 +884,7 @@ int lds_for_pass(const EllJob& J, int pass) {
 int blocks_for_pass(const EllJob& J, int pass) {
     const int rows_total = (pass == 2) ? J.ny - 1 : J.nx - 1;
     const FftPlan& P = (pass == 2) ? J.pi : J.pj;
-    const int rows = 2 * P.np;
+    const int rows = P.M == 2048 ? 2 * kR16Pairs : 2 * P.np;
     return (rows_total + rows - 1) / rows;
 }
 
@@ -543,7 +895,7 @@ int blocks_for_pass(const EllJob& J, int pass) {
     case 768: hipLaunchKernelGGL(KER<768>, grid, dim3(kFftThreads), lds_bytes, s, d_jobs); break;   \
     case 800: hipLaunchKernelGGL(KER<800>, grid, dim3(kFftThreads), lds_bytes, s, d_jobs); break;   \
     case 1024: hipLaunchKernelGGL(KER<1024>, grid, dim3(kFftThreads), lds_bytes, s, d_jobs); break; \
-    case 2048: hipLaunchKernelGGL(KER<2048>, grid, dim3(kFftThreads), lds_bytes, s, d_jobs); break; \
+    case 2048: hipLaunchKernelGGL(KER<2048>, grid, dim3(kR16Threads), lds_bytes, s, d_jobs); break; \
     default: hipLaunchKernelGGL(KER<0>, grid, dim3(kFftThreads), lds_bytes, s, d_jobs); break;       \
     }
 

@@ -54,7 +54,8 @@ def test_elliptic2d_closed_forms(ctx, oracle, kind):
 
 @pytest.mark.parametrize("nx,ny,ngrid,kind", [(16, 24, 3, 0), (16, 24, 3, 1), (200, 200, 1, 1),
                                               (400, 400, 4, 0), (400, 400, 4, 1), (512, 512, 5, 1),
-                                              (256, 384, 3, 0)])
+                                              (256, 384, 3, 0), (1024, 1024, 3, 0), (1024, 1024, 2, 1),
+                                              (1024, 256, 2, 1), (256, 1024, 2, 0)])
 def test_multidomain_solve_matches_oracle(ctx, oracle, nx, ny, ngrid, kind):
     print("solve", nx, ny, ngrid, kind, flush=True)
     g = G(nx, ny, ngrid, 4.0, -2.0, -2.0)

@@ -302,21 +302,38 @@ __host__ __device__ constexpr int pass_threads() { return CM == 2048 ? kR16Threa
 
 // LDS slot of element e: one pad slot after every 16, so the first stage's
 // writes (16 consecutive elements per lane) spread over the banks, and every
-// access of a stage is a per-lane base plus a compile-time offset.
-constexpr int kR16Buf = 2048 + 128;
+// access of a stage is a per-lane base plus a compile-time offset.  The
+// exchanges move real and imaginary parts in two rounds through a buffer of
+// doubles (17 KB), which lets twice as many workgroups share a CU as a buffer
+// of complex values would; kR16Buf counts that buffer in complex slots.
+constexpr int kR16Slots = 2048 + 128;
+constexpr int kR16Buf = kR16Slots / 2;
 __device__ inline int pad16(int e) { return e + (e >> 4); }
 
-// The second stage's twiddles in LDS, w^(8 jm r) at 16 r + jm (r, jm < 16),
-// w = exp(-2 pi i / 2048), so 16 lanes with consecutive jm read consecutive
-// entries.  The third stage reads w^j, j < 256, from the global table.
-constexpr int kR16Tw = 256;
-__device__ inline void load_tw16(double2* tw, const FftPlan& P) {
-    const IBPM_GLOBAL double* g = gin((const double*)P.tw);
-    for (int q = threadIdx.x; q < 256; q += kR16Threads) {
-        const int r = q >> 4, jm = q & 15, e = (8 * jm * r) & 2047;
-        tw[q] = make_double2(g[2 * e], g[2 * e + 1]);
-    }
+// One exchange: every lane writes its 16 values at wr[k * ws], reads 16 at
+// rd[k * rs] (per-lane bases, compile-time strides), real parts then
+// imaginary parts.
+template <int WS, int RS>
+__device__ __forceinline__ void exchange16(double2* v, double* wr, const double* rd) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) wr[WS * r] = v[r].x;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r].x = rd[RS * r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 16; ++r) wr[WS * r] = v[r].y;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r].y = rd[RS * r];
+    __syncthreads();
 }
+
+// Twiddles, w = exp(-2 pi i / 2048), come from global memory (cached): the
+// second stage's w^(8 jm r) from the plan's 256-entry table at 16 r + jm, so
+// 16 lanes with consecutive jm read consecutive entries; the third stage's
+// w^j (j < 256) from the full table.
+constexpr int kR16Tw = 0;
 
 // Second half of the odd extension from the first: v[r] = z[t + 128 r] is
 // set for r < 8; z[e] = -z[2048 - e] (another lane's) for e > 1024 comes
@@ -353,34 +370,30 @@ __device__ __forceinline__ void load_first_half(double2* v, const IBPM_GLOBAL do
 }
 
 // In: v[r] = z[t + 128 r], r < 16.  Out: o[m] = Z[t + 128 m], m < 8.  The
-// buffer is free again on return.
-__device__ __forceinline__ void fft2048_regs(double2* v, double2* o, double2* buf, const double2* tw,
+// buffer is free again on return (the last LDS access precedes the last
+// barrier).
+__device__ __forceinline__ void fft2048_regs(double2* v, double2* o, double2* buf, const IBPM_GLOBAL double* tw,
                                              const IBPM_GLOBAL double* twg) {
     const int t = threadIdx.x;
-    dft16(v);   // stage 1 (Ns = 1): outputs to 16 t + r, slots 17 t + r
-    double2* b1 = buf + 17 * t;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) b1[r] = v[r];
-    __syncthreads();
-    const double2* b2 = buf + pad16(t);   // reads t + 128 r: slots pad16(t) + 136 r
-#pragma unroll
-    for (int r = 0; r < 16; ++r) v[r] = b2[136 * r];
-    __syncthreads();
+    double* bd = (double*)buf;
+    dft16(v);   // stage 1 (Ns = 1): outputs 16 t + r to slots 17 t + r; reads
+                // t + 128 r from slots pad16(t) + 136 r
+    exchange16<1, 136>(v, bd + 17 * t, bd + pad16(t));
     const int jm = t & 15;   // stage 2 (Ns = 16): twiddles w^(8 jm r) from the table
 #pragma unroll
-    for (int r = 1; r < 16; ++r) v[r] = cmul(v[r], tw[16 * r + jm]);
+    for (int r = 1; r < 16; ++r) v[r] = cmul(v[r], make_double2(tw[2 * (16 * r + jm)], tw[2 * (16 * r + jm) + 1]));
     dft16(v);
-    double2* b3 = buf + (t >> 4) * 272 + jm;   // outputs (t/16) 256 + jm + 16 r
-#pragma unroll
-    for (int r = 0; r < 16; ++r) b3[17 * r] = v[r];
-    __syncthreads();
+    // outputs (t/16) 256 + jm + 16 r to slots (t/16) 272 + jm + 17 r; stage 3
+    // reads j + 256 r (j = t, t + 128) from slots pad16(j) + 272 r, which is
+    // the exchange16 read pattern with the two butterflies' inputs
+    // interleaved: lane t reads pad16(t) + 136 k for k = 0..15
+    exchange16<17, 136>(v, bd + (t >> 4) * 272 + jm, bd + pad16(t));
 #pragma unroll
     for (int h = 0; h < 2; ++h) {   // stage 3 (Ns = 256): butterflies j = t, t + 128
         const int j = t + 128 * h;
         double2 u[8];
-        const double2* b4 = buf + pad16(j);   // reads j + 256 r: slots pad16(j) + 272 r
 #pragma unroll
-        for (int r = 0; r < 8; ++r) u[r] = b4[272 * r];
+        for (int r = 0; r < 8; ++r) u[r] = v[2 * r + h];   // element j + 256 r = t + 128 (2 r + h)
         const double2 w1 = make_double2(twg[2 * j], twg[2 * j + 1]);
         const double2 w2 = cmul(w1, w1), w3 = cmul(w2, w1), w4 = cmul(w2, w2);
         u[1] = cmul(u[1], w1);
@@ -394,34 +407,35 @@ __device__ __forceinline__ void fft2048_regs(double2* v, double2* o, double2* bu
 #pragma unroll
         for (int r = 0; r < 4; ++r) o[h + 2 * r] = u[r];   // Z[j + 256 r], r < 4
     }
-    __syncthreads();
 }
 
 // ---------------------------------------------------------------- pass 1
 // getBC (reference Scalar.cc:91-127) evaluated at one boundary index.
+// (Global-address-space reads: a flat load would also count against the LDS
+// counter, and the barrier after the right-hand side would wait for it.)
 __device__ inline double bc_bottom(const EllJob& J, int i) {
-    if (J.bcring) return i == 0 ? J.bcring[0] : J.bcring[2 * (J.nx + J.ny) - i];
+    if (J.bcring) return i == 0 ? gin(J.bcring)[0] : gin(J.bcring)[2 * (J.nx + J.ny) - i];
     const int ii = i / 2 + nx_ext(J.nx), jj = nx_ext(J.ny);
-    const double a = J.uc[sidx(J.nx, J.ny, ii, jj)];
-    return (i & 1) ? 0.5 * (a + J.uc[sidx(J.nx, J.ny, ii + 1, jj)]) : a;
+    const double a = gin(J.uc)[sidx(J.nx, J.ny, ii, jj)];
+    return (i & 1) ? 0.5 * (a + gin(J.uc)[sidx(J.nx, J.ny, ii + 1, jj)]) : a;
 }
 __device__ inline double bc_top(const EllJob& J, int i) {
-    if (J.bcring) return J.bcring[J.ny + i];
+    if (J.bcring) return gin(J.bcring)[J.ny + i];
     const int ii = i / 2 + nx_ext(J.nx), jj = J.ny / 2 + nx_ext(J.ny);
-    const double a = J.uc[sidx(J.nx, J.ny, ii, jj)];
-    return (i & 1) ? 0.5 * (a + J.uc[sidx(J.nx, J.ny, ii + 1, jj)]) : a;
+    const double a = gin(J.uc)[sidx(J.nx, J.ny, ii, jj)];
+    return (i & 1) ? 0.5 * (a + gin(J.uc)[sidx(J.nx, J.ny, ii + 1, jj)]) : a;
 }
 __device__ inline double bc_left(const EllJob& J, int j) {
-    if (J.bcring) return J.bcring[j];
+    if (J.bcring) return gin(J.bcring)[j];
     const int ii = nx_ext(J.nx), jj = j / 2 + nx_ext(J.ny);
-    const double a = J.uc[sidx(J.nx, J.ny, ii, jj)];
-    return (j & 1) ? 0.5 * (a + J.uc[sidx(J.nx, J.ny, ii, jj + 1)]) : a;
+    const double a = gin(J.uc)[sidx(J.nx, J.ny, ii, jj)];
+    return (j & 1) ? 0.5 * (a + gin(J.uc)[sidx(J.nx, J.ny, ii, jj + 1)]) : a;
 }
 __device__ inline double bc_right(const EllJob& J, int j) {
-    if (J.bcring) return J.bcring[2 * J.ny + J.nx - j];
+    if (J.bcring) return gin(J.bcring)[2 * J.ny + J.nx - j];
     const int ii = J.nx / 2 + nx_ext(J.nx), jj = j / 2 + nx_ext(J.ny);
-    const double a = J.uc[sidx(J.nx, J.ny, ii, jj)];
-    return (j & 1) ? 0.5 * (a + J.uc[sidx(J.nx, J.ny, ii, jj + 1)]) : a;
+    const double a = gin(J.uc)[sidx(J.nx, J.ny, ii, jj)];
+    return (j & 1) ? 0.5 * (a + gin(J.uc)[sidx(J.nx, J.ny, ii, jj + 1)]) : a;
 }
 
 // Right-hand side of one level at (i, j), 1-based: coarsified copy in the
@@ -431,7 +445,7 @@ __device__ inline double rhs_value(const EllJob& J, int i, int j) {
     const int ex = nx_ext(J.nx), ey = nx_ext(J.ny);
     const bool central = J.lev >= 1 && i > ex && i < J.nx / 2 + ex && j > ey && j < J.ny / 2 + ey;
     const long k = sidx(J.nx, J.ny, i, j);
-    double v = (central ? J.R[k] : J.f[k]) * J.in_scale;
+    double v = (central ? gin(J.R)[k] : gin(J.f)[k]) * J.in_scale;
     if (J.uc || J.bcring) {
         const double c = J.bc_coef;
         if (j == 1) v -= bc_bottom(J, i) * c;
@@ -457,7 +471,8 @@ __device__ inline bool on_bc_ring(const EllJob& J, int i, int j) {
 template <int CM>
 __device__ inline void load_twiddles(double2* tw, const FftPlan& P) {
     const int M = (CM && quarter_twiddles(CM)) ? CM / 4 : (CM ? CM : P.M);
-    for (int k = threadIdx.x; k < M; k += kFftThreads) tw[k] = P.tw[k];
+    const IBPM_GLOBAL double* g = gin((const double*)P.tw);
+    for (int k = threadIdx.x; k < M; k += kFftThreads) tw[k] = make_double2(g[2 * k], g[2 * k + 1]);
 }
 
 
@@ -492,7 +507,7 @@ __device__ inline void pass1_lds(const EllJob& J, double2* smem, int bx) {
         const int row = r0 + r;
         if (row >= N1) continue;
         const double2 z = buf[lds_slot<kSwizzleAll<CM>>((r >> 1) * M + k + 1)];
-        J.T1[(long)k * N1 + row] = (r & 1) ? z.x : -z.y;
+        gout(J.T1)[(long)k * N1 + row] = (r & 1) ? z.x : -z.y;
     }
 }
 
@@ -511,8 +526,8 @@ __device__ inline void pass2_lds(const EllJob& J, double2* smem, int bx) {
     for (int idx = threadIdx.x; idx < np * N1; idx += kFftThreads) {
         const int p = idx / N1, k = idx - p * N1;
         const int la = r0 + 2 * p, lb = la + 1;
-        const double va = (la < N2) ? J.T1[(long)la * N1 + k] : 0.0;
-        const double vb = (lb < N2) ? J.T1[(long)lb * N1 + k] : 0.0;
+        const double va = (la < N2) ? gin(J.T1)[(long)la * N1 + k] : 0.0;
+        const double vb = (lb < N2) ? gin(J.T1)[(long)lb * N1 + k] : 0.0;
         buf[lds_slot<kSwizzleAll<CM>>(p * M + k + 1)] = make_double2(va, vb);
         buf[lds_slot<kSwizzleAll<CM>>(p * M + M - 1 - k)] = make_double2(-va, -vb);
     }
@@ -528,13 +543,13 @@ __device__ inline void pass2_lds(const EllJob& J, double2* smem, int bx) {
         const int la = r0 + 2 * p, lb = la + 1;
         const double2 z = buf[lds_slot<kSwizzleAll<CM>>(p * M + k + 1)];
         double ya = -z.y, yb = z.x;
-        const double cx = J.cosx[k];
+        const double cx = gin(J.cosx)[k];
         if (la < N2) {
-            const double lam = 2. * (cx + J.cosy[la] - 2.) * J.bydx2;
+            const double lam = 2. * (cx + gin(J.cosy)[la] - 2.) * J.bydx2;
             ya *= (J.kind == 0) ? 1. / lam : 1. / (1 + J.alpha * lam);
         }
         if (lb < N2) {
-            const double lam = 2. * (cx + J.cosy[lb] - 2.) * J.bydx2;
+            const double lam = 2. * (cx + gin(J.cosy)[lb] - 2.) * J.bydx2;
             yb *= (J.kind == 0) ? 1. / lam : 1. / (1 + J.alpha * lam);
         }
         buf[lds_slot<kSwizzleAll<CM>>(p * M + k + 1)] = make_double2(ya, yb);
@@ -551,7 +566,7 @@ __device__ inline void pass2_lds(const EllJob& J, double2* smem, int bx) {
         const int l = r0 + r;
         if (l >= N2) continue;
         const double2 z = buf[lds_slot<kSwizzleAll<CM>>((r >> 1) * M + k + 1)];
-        J.T2[(long)k * N2 + l] = (r & 1) ? z.x : -z.y;
+        gout(J.T2)[(long)k * N2 + l] = (r & 1) ? z.x : -z.y;
     }
 }
 
@@ -570,8 +585,8 @@ __device__ inline void pass3_lds(const EllJob& J, double2* smem, int bx) {
     for (int idx = threadIdx.x; idx < np * N2; idx += kFftThreads) {
         const int p = idx / N2, k = idx - p * N2;
         const int ra = r0 + 2 * p, rb = ra + 1;
-        const double va = (ra < N1) ? J.T2[(long)ra * N2 + k] : 0.0;
-        const double vb = (rb < N1) ? J.T2[(long)rb * N2 + k] : 0.0;
+        const double va = (ra < N1) ? gin(J.T2)[(long)ra * N2 + k] : 0.0;
+        const double vb = (rb < N1) ? gin(J.T2)[(long)rb * N2 + k] : 0.0;
         buf[lds_slot<kSwizzleAll<CM>>(p * M + k + 1)] = make_double2(va, vb);
         buf[lds_slot<kSwizzleAll<CM>>(p * M + M - 1 - k)] = make_double2(-va, -vb);
     }
@@ -591,14 +606,14 @@ __device__ inline void pass3_lds(const EllJob& J, double2* smem, int bx) {
         if (ra < N1) {
             const long o = (long)ra * N2 + k;
             const double v = -z.y * J.norm;
-            J.u[o] = J.epilogue ? J.W[o] - v : v;
-            if (J.vside && on_bc_ring(J, ra + 1, k + 1)) J.vside[o] = v;
+            gout(J.u)[o] = J.epilogue ? gin(J.W)[o] - v : v;
+            if (J.vside && on_bc_ring(J, ra + 1, k + 1)) gout(J.vside)[o] = v;
         }
         if (rb < N1) {
             const long o = (long)rb * N2 + k;
             const double v = z.x * J.norm;
-            J.u[o] = J.epilogue ? J.W[o] - v : v;
-            if (J.vside && on_bc_ring(J, rb + 1, k + 1)) J.vside[o] = v;
+            gout(J.u)[o] = J.epilogue ? gin(J.W)[o] - v : v;
+            if (J.vside && on_bc_ring(J, rb + 1, k + 1)) gout(J.vside)[o] = v;
         }
     }
 }
@@ -629,8 +644,7 @@ __device__ __forceinline__ void pass1_r16(const EllJob& J, double2* smem, int bx
     const int N1 = J.nx - 1, npairs = (N1 + 1) / 2;
     if (kR16Pairs * bx >= npairs) return;
     double2* buf = smem;
-    double2* tw = smem + kR16Buf;
-    load_tw16(tw, J.pj);
+    const IBPM_GLOBAL double* tw = gin((const double*)J.pj.tw16);
     const IBPM_GLOBAL double* twg = gin((const double*)J.pj.tw);
     const int t = threadIdx.x;
     const int ey = nx_ext(J.ny), ex = nx_ext(J.nx);
@@ -683,7 +697,7 @@ __device__ __forceinline__ void pass1_r16(const EllJob& J, double2* smem, int bx
         const int ra = 2 * q;
         const bool hb = ra + 1 < N1;
         double2 o[8];
-        mirror_second_half(v, buf);   // its barriers also publish the twiddles
+        mirror_second_half(v, buf);
         fft2048_regs(v, o, buf, tw, twg);
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
@@ -699,8 +713,7 @@ __device__ __forceinline__ void pass2_r16(const EllJob& J, double2* smem, int bx
     const int N1 = 1023, N2 = J.ny - 1, npairs = (N2 + 1) / 2;
     if (kR16Pairs * bx >= npairs) return;
     double2* buf = smem;
-    double2* tw = smem + kR16Buf;
-    load_tw16(tw, J.pi);
+    const IBPM_GLOBAL double* tw = gin((const double*)J.pi.tw16);
     const IBPM_GLOBAL double* twg = gin((const double*)J.pi.tw);
     const int t = threadIdx.x;
     const IBPM_GLOBAL double* gcx = gin(J.cosx);
@@ -749,8 +762,7 @@ __device__ __forceinline__ void pass3_r16(const EllJob& J, double2* smem, int bx
     const int N1 = J.nx - 1, N2 = 1023, npairs = (N1 + 1) / 2;
     if (kR16Pairs * bx >= npairs) return;
     double2* buf = smem;
-    double2* tw = smem + kR16Buf;
-    load_tw16(tw, J.pj);
+    const IBPM_GLOBAL double* tw = gin((const double*)J.pj.tw16);
     const IBPM_GLOBAL double* twg = gin((const double*)J.pj.tw);
     const int t = threadIdx.x;
     auto load = [&](int q, double2* v) {
@@ -845,7 +857,7 @@ __device__ inline double coarse_src(const CoarsifyJob& J, int lf, int i, int j) 
     const int ex = nx_ext(nx), ey = nx_ext(ny);
     const long off = (long)lf * (nx - 1) * (ny - 1) + sidx(nx, ny, i, j);
     const bool central = lf >= 1 && i > ex && i < nx / 2 + ex && j > ey && j < ny / 2 + ey;
-    return central ? J.R[off] : J.f[off];
+    return central ? gin(J.R)[off] : gin(J.f)[off];
 }
 
 __device__ inline void coarsify_body(const CoarsifyJob& J) {
@@ -863,7 +875,7 @@ __device__ inline void coarsify_body(const CoarsifyJob& J) {
                  + coarse_src(J, lf, ii - 1, jj) + coarse_src(J, lf, ii, jj - 1))
         + 0.0625 * (coarse_src(J, lf, ii + 1, jj + 1) + coarse_src(J, lf, ii + 1, jj - 1)
                   + coarse_src(J, lf, ii - 1, jj + 1) + coarse_src(J, lf, ii - 1, jj - 1));
-    J.R[(long)J.lev * (nx - 1) * (ny - 1) + sidx(nx, ny, i, j)] = v;
+    gout(J.R)[(long)J.lev * (nx - 1) * (ny - 1) + sidx(nx, ny, i, j)] = v;
 }
 
 __global__ void __launch_bounds__(kThreads) k_coarsify(const CoarsifyJob* __restrict__ jobs) {
@@ -876,7 +888,7 @@ __global__ void __launch_bounds__(kThreads) k_fs_coarsify(const CoarsifyJob* __r
 // ------------------------------------------------------------ host launch
 int lds_for_pass(const EllJob& J, int pass) {
     const FftPlan& P = (pass == 2) ? J.pi : J.pj;
-    if (P.M == 2048) return (int)sizeof(double2) * (kR16Buf + kR16Tw);
+    if (P.M == 2048) return (int)sizeof(double2) * (kR16Buf + kR16Tw);   // 17 KB
     const int ntw = (compiled_len(P.M) && quarter_twiddles(P.M)) ? P.M / 4 : P.M;
     return (int)sizeof(double2) * (P.np * P.M + ntw);
 }

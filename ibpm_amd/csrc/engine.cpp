@@ -98,6 +98,16 @@ static int get_plan(Ctx* c, int N, FftPlan* out) {
     CKR(c->mem.alloc(&d, P.M));
     CK(hipMemcpy(d, tw.data(), sizeof(double2) * P.M, hipMemcpyHostToDevice));
     P.tw = d;
+    P.tw16 = nullptr;
+    if (P.M == 2048) {   // second-stage table of the register-held 2048-point transform
+        std::vector<double2> t16(256);
+        for (int r = 0; r < 16; ++r)
+            for (int jm = 0; jm < 16; ++jm) t16[16 * r + jm] = tw[(8 * jm * r) % P.M];
+        double2* d16 = nullptr;
+        CKR(c->mem.alloc(&d16, 256));
+        CK(hipMemcpy(d16, t16.data(), sizeof(double2) * 256, hipMemcpyHostToDevice));
+        P.tw16 = d16;
+    }
     c->plans[N] = P;
     *out = P;
     return 0;

@@ -209,6 +209,8 @@ def main():
                   "warm_start": args.warm, "cg_iters_per_step": cg,
                   "batched_m_applications_per_step": (d["m_applications"] + d["build_m_applications"]) / steps,
                   "builds_in_timed_steps": d["builds"], "fallbacks": d["fallbacks"],
+                  "host_ms_per_step_kinematics": round(d["host_us_kinematics"] / steps / 1e3, 3),
+                  "host_ms_per_step_polls": round(d["host_us_polls"] / steps / 1e3, 3),
                   "warmup_s_including_first_build": round(t_warm, 3)}
     # live per-kernel timing (HIP events on the library stream, eager steps)
     stats = b.kernel_stats(2 if moving else 3)

@@ -404,7 +404,7 @@ class Batch:
         return self
 
     SOLVER_STATS = ("iterations", "solves", "m_applications", "build_m_applications", "builds",
-                    "fallbacks", "abandoned")
+                    "fallbacks", "abandoned", "host_us_kinematics", "host_us_polls")
 
     def solver_stats(self):
         out = np.zeros(len(self.SOLVER_STATS), dtype=np.int64)

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -349,6 +350,8 @@ struct Flow {
     double *e_w, *t_w;
     int4* node_nbr;
     int nnodes = 0, nold = 0;
+    std::vector<int> reg_old;      // node list of the previous association (cleared on the device)
+    int reg_old_n = 0;
     // host state
     std::vector<double> px, py, pu, pv;
     int hist_cap = 0;
@@ -424,6 +427,7 @@ struct ibpm_batch {
     // counters: iterations, solves, M applications in solves, M applications
     // in preconditioner builds, builds, fallbacks to plain CG
     long st_iters = 0, st_solves = 0, st_mapps = 0, st_build_mapps = 0, st_builds = 0, st_fallbacks = 0;
+    double st_host_kin_s = 0, st_host_poll_s = 0;   // host time: kinematics + regulariser, solver polls
 };
 
 namespace ibpmg {
@@ -500,11 +504,18 @@ static int move_base_flow(ibpm_batch* B, Flow& F, double t) {
 
 // Regulariser lists of a flow for its current points; uploads them and the
 // constraint b = u_body - E q0.
-static int update_regularizer(ibpm_batch* B, Flow& F) {
-    RegLists& R = F.reg;
-    const int old_nodes = F.nnodes;
-    std::vector<int> old = R.node_idx;
-    build_reg_lists(F.nx, F.ny, F.dx, F.p.xoffset, F.p.yoffset, F.px, F.py, R);
+// Host part: the lists for the flow's current points; the previous node list
+// is kept so the device can clear those nodes.
+static void build_regularizer(Flow& F) {
+    F.reg_old = F.reg.node_idx;
+    F.reg_old_n = F.nnodes;
+    build_reg_lists(F.nx, F.ny, F.dx, F.p.xoffset, F.p.yoffset, F.px, F.py, F.reg);
+}
+
+static int upload_regularizer(ibpm_batch* B, Flow& F) {
+    const RegLists& R = F.reg;
+    const std::vector<int>& old = F.reg_old;
+    const int old_nodes = F.reg_old_n;
     const int cap = 18 * F.nb + 1;
     if ((int)R.e_f.size() > cap || (int)R.t_k.size() > cap || (int)R.node_idx.size() > 4 * cap) {
         set_error("regulariser list exceeds capacity");
@@ -523,6 +534,27 @@ static int update_regularizer(ibpm_batch* B, Flow& F) {
     F.nnodes = (int)R.node_idx.size();
     (void)B;
     return upload_constraint(F);
+}
+
+static int update_regularizer(ibpm_batch* B, Flow& F) {
+    build_regularizer(F);
+    return upload_regularizer(B, F);
+}
+
+// Runs body(q) for q < n on up to 16 host threads (q = tid, tid + T, ...).
+template <class Body>
+static void parallel_for(int n, const Body& body) {
+    const int T = std::max(1, std::min({n, 16, (int)std::thread::hardware_concurrency()}));
+    if (T <= 1) {
+        for (int q = 0; q < n; ++q) body(q);
+        return;
+    }
+    std::vector<std::thread> th;
+    th.reserve(T - 1);
+    for (int tid = 1; tid < T; ++tid)
+        th.emplace_back([&, tid] { for (int q = tid; q < n; q += T) body(q); });
+    for (int q = 0; q < n; q += T) body(q);
+    for (auto& t : th) t.join();
 }
 
 static RegOps make_regops(const Flow& F, double beta) {
@@ -795,10 +827,16 @@ static int reset_states(ibpm_batch* B) {
     return 0;
 }
 
+static double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
 static int read_states(ibpm_batch* B, std::vector<double>& st) {
+    const double t0 = now_s();
     CK(hipStreamSynchronize(B->ctx->c.stream));
     st.resize(8 * B->flows.size());
     CK(hipMemcpy(st.data(), B->cg_states, sizeof(double) * st.size(), hipMemcpyDeviceToHost));
+    B->st_host_poll_s += now_s() - t0;
     return 0;
 }
 
@@ -905,15 +943,24 @@ static int solve_pcg(ibpm_batch* B, int i, int* iters, bool* fallback) {
 static int substep_cg(ibpm_batch* B, int i) {
     hipStream_t s = B->ctx->c.stream;
     const int nfl = (int)B->flows.size();
-    for (auto& F : B->flows) {
-        if (!F.geom.stationary) {   // updateOperators(t + c_n dt), IBSolver.cc:164-166
-            F.geom.move(B->time + B->cn[i] * F.p.dt, F.px, F.py, F.pu, F.pv);
-            CK(hipStreamSynchronize(s));
-            CKR(update_regularizer(B, F));
-        }
+    const double t_kin = now_s();
+    // updateOperators(t + c_n dt), IBSolver.cc:164-166: kinematics and
+    // regulariser lists of the moving flows on host threads while the GPU
+    // finishes the previous substep, then the uploads
+    std::vector<int> moving;
+    for (int k = 0; k < nfl; ++k)
+        if (!B->flows[k].geom.stationary) moving.push_back(k);
+    parallel_for((int)moving.size(), [&](int q) {
+        Flow& F = B->flows[moving[q]];
+        F.geom.move(B->time + B->cn[i] * F.p.dt, F.px, F.py, F.pu, F.pv);
+        build_regularizer(F);
+    });
+    CK(hipStreamSynchronize(s));
+    for (int k : moving) CKR(upload_regularizer(B, B->flows[k]));
+    for (auto& F : B->flows)
         if (F.ubf) CKR(move_base_flow(B, F, B->time + B->cn[i] * F.p.dt));
-    }
     CKR(upload_regops(B));
+    B->st_host_kin_s += now_s() - t_kin;
     Probe* pr = B->probe;
     timed(pr, K_CLEAR, 0, s, [&] { launch_clear_nodes(B->d_clear, nfl, B->max_nodes, s); });
     enqueue_rhs(B, i, s);
@@ -2001,9 +2048,10 @@ int ibpm_batch_set_precond(ibpm_batch* B, int mode, int pairs) {
 
 int ibpm_batch_solver_stats(ibpm_batch* B, long* stats, int nstats) {
     if (!B || !stats) { set_error("null argument"); return -1; }
-    const long v[7] = {B->st_iters, B->st_solves, B->st_mapps, B->st_build_mapps, B->st_builds,
-                       B->st_fallbacks, B->pc_broken ? 1L : 0L};
-    const int n = std::min(nstats, 7);
+    const long v[9] = {B->st_iters, B->st_solves, B->st_mapps, B->st_build_mapps, B->st_builds,
+                       B->st_fallbacks, B->pc_broken ? 1L : 0L, (long)(1e6 * B->st_host_kin_s),
+                       (long)(1e6 * B->st_host_poll_s)};
+    const int n = std::min(nstats, 9);
     for (int k = 0; k < n; ++k) stats[k] = v[k];
     return n;
 }

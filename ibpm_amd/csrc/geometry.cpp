@@ -392,8 +392,13 @@ void build_reg_lists(int nx, int ny, double h, double xoff, double yoff,
     out.e_ptr.assign(2 * nb + 1, 0);
     out.e_f.clear();
     out.e_w.clear();
-    // per flux: (component, weight) in the order the reference visits them
-    std::map<int, std::vector<std::pair<int, double>>> per_flux;
+    // (flux, component, weight) in the order the reference visits them
+    struct Hit {
+        int f, k;
+        double w;
+    };
+    std::vector<Hit> hits;
+    hits.reserve((size_t)nb * 2 * 16);
     for (int dir = 0; dir <= 1; ++dir) {
         for (int p = 0; p < nb; ++p) {
             const int k = dir * nb + p;
@@ -417,35 +422,41 @@ void build_reg_lists(int nx, int ny, double h, double xoff, double yoff,
                         const double w = delta_fn(ddx) * delta_fn(ddy);
                         out.e_f.push_back(ind);
                         out.e_w.push_back(w);
-                        per_flux[ind].push_back({k, w});
+                        hits.push_back({ind, k, w});
                     }
                 }
             }
             out.e_ptr[k + 1] = (int)out.e_f.size();
         }
     }
+    // E^T: per flux in ascending order, components in visiting (= ascending) order
+    std::stable_sort(hits.begin(), hits.end(), [](const Hit& a, const Hit& b) { return a.f < b.f; });
     out.flux_of_slot.clear();
     out.t_ptr.assign(1, 0);
     out.t_k.clear();
     out.t_w.clear();
-    std::map<int, int> slot_of_flux;
-    for (auto& kv : per_flux) {   // ascending flux index; components already ascending
-        slot_of_flux[kv.first] = (int)out.flux_of_slot.size();
-        out.flux_of_slot.push_back(kv.first);
-        for (auto& cw : kv.second) { out.t_k.push_back(cw.first); out.t_w.push_back(cw.second); }
+    for (size_t q = 0; q < hits.size();) {
+        const int f = hits[q].f;
+        out.flux_of_slot.push_back(f);
+        for (; q < hits.size() && hits[q].f == f; ++q) {
+            out.t_k.push_back(hits[q].k);
+            out.t_w.push_back(hits[q].w);
+        }
         out.t_ptr.push_back((int)out.t_k.size());
     }
-    // nodes adjacent to a touched flux; curl slots Y(i,j), Y(i-1,j), X(i,j-1), X(i,j)
-    std::map<int, std::vector<int>> nodes;
-    auto slot = [&](int f) { auto it = slot_of_flux.find(f); return it == slot_of_flux.end() ? -1 : it->second; };
+    // nodes adjacent to a touched flux, ascending; curl slots Y(i,j), Y(i-1,j), X(i,j-1), X(i,j)
+    const std::vector<int>& fs = out.flux_of_slot;
+    auto slot = [&](long f) {
+        auto it = std::lower_bound(fs.begin(), fs.end(), (int)f);
+        return (it != fs.end() && *it == f) ? (int)(it - fs.begin()) : -1;
+    };
+    std::vector<int> keys;
+    keys.reserve(fs.size() * 2);
     auto touch = [&](int i, int j) {
         if (i < 1 || i > nx - 1 || j < 1 || j > ny - 1) return;
-        const int key = (i - 1) * (ny - 1) + (j - 1);
-        if (nodes.count(key)) return;
-        nodes[key] = {slot((int)(nfx + (long)i * (ny + 1) + j)), slot((int)(nfx + (long)(i - 1) * (ny + 1) + j)),
-                      slot(i * ny + j - 1), slot(i * ny + j)};
+        keys.push_back((i - 1) * (ny - 1) + (j - 1));
     };
-    for (int f : out.flux_of_slot) {
+    for (int f : fs) {
         if (f < nfx) {
             const int i = f / ny, j = f % ny;
             touch(i, j);
@@ -457,11 +468,17 @@ void build_reg_lists(int nx, int ny, double h, double xoff, double yoff,
             touch(i + 1, j);
         }
     }
-    out.node_idx.clear();
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    out.node_idx.assign(keys.begin(), keys.end());
     out.node_nbr.clear();
-    for (auto& kv : nodes) {
-        out.node_idx.push_back(kv.first);
-        for (int s : kv.second) out.node_nbr.push_back(s);
+    out.node_nbr.reserve(4 * keys.size());
+    for (int key : keys) {
+        const int i = key / (ny - 1) + 1, j = key % (ny - 1) + 1;
+        out.node_nbr.push_back(slot(nfx + (long)i * (ny + 1) + j));
+        out.node_nbr.push_back(slot(nfx + (long)(i - 1) * (ny + 1) + j));
+        out.node_nbr.push_back(slot((long)i * ny + j - 1));
+        out.node_nbr.push_back(slot((long)i * ny + j));
     }
 }
 

@@ -222,7 +222,10 @@ int ibpm_batch_set_precond(ibpm_batch* b, int mode, int pairs);
    substeps, [1] solves (flow-substeps), [2] batched M applications in solves,
    [3] batched M applications spent building preconditioners, [4] builds,
    [5] solves finished by plain CG after a breakdown, [6] 1 when some M was not
-   positive definite and preconditioning stopped.  Returns the count written. */
+   positive definite and preconditioning stopped, [7] host microseconds in the
+   per-substep kinematics and regulariser updates (GPU idle), [8] host
+   microseconds in solver polls (stream synchronisation and state read-back).
+   Returns the count written. */
 int ibpm_batch_solver_stats(ibpm_batch* b, long* stats, int nstats);
 
 /* Measurement: advances nsteps with a HIP event pair around every kernel

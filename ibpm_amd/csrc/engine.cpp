@@ -67,6 +67,21 @@ struct DevMem {
     }
 };
 
+// Pinned host staging for a flow's per-substep uploads (regulariser lists
+// and constraint), so they go as asynchronous copies on the stream.
+struct HostStage {
+    char* p = nullptr;
+    size_t cap = 0, used = 0;
+    bool pending = false;          // copies from it may still be in flight
+    HostStage() = default;
+    HostStage(const HostStage&) = delete;
+    HostStage& operator=(const HostStage&) = delete;
+    HostStage(HostStage&& o) noexcept : p(o.p), cap(o.cap), used(o.used), pending(o.pending) { o.p = nullptr; }
+    ~HostStage() {
+        if (p) (void)hipHostFree(p);
+    }
+};
+
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -352,6 +367,7 @@ struct Flow {
     int nnodes = 0, nold = 0;
     std::vector<int> reg_old;      // node list of the previous association (cleared on the device)
     int reg_old_n = 0;
+    HostStage stage;
     // host state
     std::vector<double> px, py, pu, pv;
     int hist_cap = 0;
@@ -438,6 +454,27 @@ static int upload(void* d, const void* h, size_t bytes) {
     return 0;
 }
 
+// Starts a round of staged uploads for a flow: waits for copies still in
+// flight from its staging buffer, then reuses it from the start.
+static int stage_begin(hipStream_t s, HostStage& st) {
+    if (st.pending) CK(hipStreamSynchronize(s));
+    st.pending = false;
+    st.used = 0;
+    return 0;
+}
+
+// Copies host data into the staging buffer and enqueues its upload.
+static int stage_upload(hipStream_t s, HostStage& st, void* d, const void* h, size_t bytes) {
+    if (bytes == 0) return 0;
+    const size_t off = (st.used + 255) & ~(size_t)255;
+    if (off + bytes > st.cap) { set_error("upload staging buffer too small"); return -3; }
+    std::memcpy(st.p + off, h, bytes);
+    st.used = off + bytes;
+    CK(hipMemcpyAsync(d, st.p + off, bytes, hipMemcpyHostToDevice, s));
+    st.pending = true;
+    return 0;
+}
+
 // Scheme coefficients, Scheme.h:14-56
 static void scheme_coefs(int scheme, ibpm_batch* B) {
     switch (scheme) {
@@ -462,7 +499,7 @@ static double beta_of(const ibpm_batch* B, const Flow& F, int i) { return (B->an
 // The constraint b = u_body - E q0 (NavierStokesModel.cc:86-91,142-146) with
 // q0 on level 0: the uniform flow, or the unsteady base flow at its last
 // update (Flux.cc:101-129 at the edge coordinates of Grid.cc:104-139).
-static int upload_constraint(Flow& F) {
+static int upload_constraint(hipStream_t s, Flow& F) {
     const RegLists& R = F.reg;
     const double pi = 4. * std::atan(1.);
     const double al = F.p.alpha_deg * pi / 180.;
@@ -487,19 +524,19 @@ static int upload_constraint(Flow& F) {
         const double vel = k < F.nb ? F.pu[k] : F.pv[k - F.nb];
         b[k] = vel - s;
     }
-    CKR(upload(F.b, b.data(), sizeof(double) * b.size()));
-    return 0;
+    return stage_upload(s, F.stage, F.b, b.data(), sizeof(double) * b.size());
 }
 
 // BaseFlow::moveFlow(t) of a flow with an unsteady base flow
 // (NavierStokesModel::updateOperators, NavierStokesModel.cc:93-99): the new
 // u, v, w on the device and the constraint for them.
 static int move_base_flow(ibpm_batch* B, Flow& F, double t) {
+    hipStream_t s = B->ctx->c.stream;
     F.bf.move(t);
-    CK(hipStreamSynchronize(B->ctx->c.stream));
+    CKR(stage_begin(s, F.stage));
     const double uvw[3] = {F.bf.u, F.bf.v, F.bf.w};
-    CKR(upload(F.bf_dev, uvw, sizeof uvw));
-    return upload_constraint(F);
+    CKR(stage_upload(s, F.stage, F.bf_dev, uvw, sizeof uvw));
+    return upload_constraint(s, F);
 }
 
 // Regulariser lists of a flow for its current points; uploads them and the
@@ -521,19 +558,21 @@ static int upload_regularizer(ibpm_batch* B, Flow& F) {
         set_error("regulariser list exceeds capacity");
         return -3;
     }
-    CKR(upload(F.e_ptr, R.e_ptr.data(), sizeof(int) * R.e_ptr.size()));
-    CKR(upload(F.e_f, R.e_f.data(), sizeof(int) * R.e_f.size()));
-    CKR(upload(F.e_w, R.e_w.data(), sizeof(double) * R.e_w.size()));
-    CKR(upload(F.t_ptr, R.t_ptr.data(), sizeof(int) * R.t_ptr.size()));
-    CKR(upload(F.t_k, R.t_k.data(), sizeof(int) * R.t_k.size()));
-    CKR(upload(F.t_w, R.t_w.data(), sizeof(double) * R.t_w.size()));
-    CKR(upload(F.node_idx, R.node_idx.data(), sizeof(int) * R.node_idx.size()));
-    CKR(upload(F.node_nbr, R.node_nbr.data(), sizeof(int) * R.node_nbr.size()));
-    CKR(upload(F.old_idx, old.data(), sizeof(int) * old.size()));
+    hipStream_t s = B->ctx->c.stream;
+    HostStage& st = F.stage;
+    CKR(stage_begin(s, st));
+    CKR(stage_upload(s, st, F.e_ptr, R.e_ptr.data(), sizeof(int) * R.e_ptr.size()));
+    CKR(stage_upload(s, st, F.e_f, R.e_f.data(), sizeof(int) * R.e_f.size()));
+    CKR(stage_upload(s, st, F.e_w, R.e_w.data(), sizeof(double) * R.e_w.size()));
+    CKR(stage_upload(s, st, F.t_ptr, R.t_ptr.data(), sizeof(int) * R.t_ptr.size()));
+    CKR(stage_upload(s, st, F.t_k, R.t_k.data(), sizeof(int) * R.t_k.size()));
+    CKR(stage_upload(s, st, F.t_w, R.t_w.data(), sizeof(double) * R.t_w.size()));
+    CKR(stage_upload(s, st, F.node_idx, R.node_idx.data(), sizeof(int) * R.node_idx.size()));
+    CKR(stage_upload(s, st, F.node_nbr, R.node_nbr.data(), sizeof(int) * R.node_nbr.size()));
+    CKR(stage_upload(s, st, F.old_idx, old.data(), sizeof(int) * old.size()));
     F.nold = old_nodes;
     F.nnodes = (int)R.node_idx.size();
-    (void)B;
-    return upload_constraint(F);
+    return upload_constraint(s, F);
 }
 
 static int update_regularizer(ibpm_batch* B, Flow& F) {
@@ -623,6 +662,12 @@ static int alloc_flow(ibpm_batch* B, Flow& F) {
     CKR(m.alloc(&F.e_ptr, n + 1)); CKR(m.alloc(&F.e_f, cap)); CKR(m.alloc(&F.e_w, cap));
     CKR(m.alloc(&F.t_ptr, cap + 1)); CKR(m.alloc(&F.t_k, cap)); CKR(m.alloc(&F.t_w, cap));
     CKR(m.alloc(&F.node_idx, 4 * cap)); CKR(m.alloc(&F.old_idx, 4 * cap)); CKR(m.alloc(&F.node_nbr, 4 * cap));
+    {   // staging for the lists (capacities above), the constraint and the base flow
+        const size_t bytes = sizeof(int) * (n + 1) + (sizeof(int) + sizeof(double)) * 2 * cap + sizeof(int) * (cap + 1) +
+                             sizeof(int) * 8 * cap + sizeof(int4) * 4 * cap + sizeof(double) * (n + 4) + 12 * 256;
+        CK(hipHostMalloc((void**)&F.stage.p, bytes, hipHostMallocDefault));
+        F.stage.cap = bytes;
+    }
     {   // learned warm start history (small; allocated for every flow)
         const int R = 2 * B->nsub + 2;
         CKR(m.alloc(&F.ring_f, (size_t)R * n)); CKR(m.alloc(&F.ring_b, (size_t)R * n));
@@ -956,6 +1001,7 @@ static int substep_cg(ibpm_batch* B, int i) {
         build_regularizer(F);
     });
     CK(hipStreamSynchronize(s));
+    for (auto& F : B->flows) F.stage.pending = false;
     for (int k : moving) CKR(upload_regularizer(B, B->flows[k]));
     for (auto& F : B->flows)
         if (F.ubf) CKR(move_base_flow(B, F, B->time + B->cn[i] * F.p.dt));

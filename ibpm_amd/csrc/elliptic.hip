@@ -852,30 +852,60 @@ IBPM_PASS_KERNEL(k_fs_pass3, pass3_body)
 // the central region into R; neighbours at level lev-1 come from R where the
 // finer level is itself coarsified (central region of a level >= 1) and from
 // the original field elsewhere.
-__device__ inline double coarse_src(const CoarsifyJob& J, int lf, int i, int j) {
-    const int nx = J.nx, ny = J.ny;
-    const int ex = nx_ext(nx), ey = nx_ext(ny);
-    const long off = (long)lf * (nx - 1) * (ny - 1) + sidx(nx, ny, i, j);
-    const bool central = lf >= 1 && i > ex && i < nx / 2 + ex && j > ey && j < ny / 2 + ey;
-    return central ? gin(J.R)[off] : gin(J.f)[off];
-}
-
+// One workgroup per coarse row i of the central region; each thread makes two
+// neighbouring points (j, j + 1) from the 3 x 5 fine values they share, with
+// the same sums in the same order as Scalar::coarsify.
 __device__ inline void coarsify_body(const CoarsifyJob& J) {
     if (skipped(J.skip)) return;
     const int nx = J.nx, ny = J.ny;
     const int ex = nx_ext(nx), ey = nx_ext(ny);
     const int wi = nx / 2 - 1, wj = ny / 2 - 1;
-    const int t = blockIdx.x * kThreads + threadIdx.x;
-    if (t >= wi * wj) return;
-    const int i = ex + 1 + t / wj, j = ey + 1 + t % wj;
-    const int ii = (i - ex) * 2, jj = (j - ey) * 2;   // Grid::c2f
+    const int row = blockIdx.x;
+    if (row >= wi) return;
+    const int i = ex + 1 + row, ii = (i - ex) * 2;   // Grid::c2f
     const int lf = J.lev - 1;
-    const double v = 0.25 * coarse_src(J, lf, ii, jj)
-        + 0.125 * (coarse_src(J, lf, ii + 1, jj) + coarse_src(J, lf, ii, jj + 1)
-                 + coarse_src(J, lf, ii - 1, jj) + coarse_src(J, lf, ii, jj - 1))
-        + 0.0625 * (coarse_src(J, lf, ii + 1, jj + 1) + coarse_src(J, lf, ii + 1, jj - 1)
-                  + coarse_src(J, lf, ii - 1, jj + 1) + coarse_src(J, lf, ii - 1, jj - 1));
-    gout(J.R)[(long)J.lev * (nx - 1) * (ny - 1) + sidx(nx, ny, i, j)] = v;
+    const long lofs = (long)lf * (nx - 1) * (ny - 1);
+    // fine rows ii - 1, ii, ii + 1: base pointers (value at column c is [c]) and
+    // whether the row crosses the finer level's own central region
+    const IBPM_GLOBAL double* fr[3];
+    const IBPM_GLOBAL double* Rr[3];
+    bool crow[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const int r = ii - 1 + d;
+        fr[d] = gin(J.f) + lofs + sidx(nx, ny, r, 1) - 1;
+        Rr[d] = gin(J.R) + lofs + sidx(nx, ny, r, 1) - 1;
+        crow[d] = lf >= 1 && r > ex && r < nx / 2 + ex;
+    }
+    IBPM_GLOBAL double* out = gout(J.R) + (long)J.lev * (nx - 1) * (ny - 1) + sidx(nx, ny, i, 1) - 1;
+    const int npairs = (wj + 1) / 2;
+    for (int q = threadIdx.x; q < npairs; q += blockDim.x) {
+        const int j0 = ey + 1 + 2 * q;
+        const bool two = j0 + 1 <= ey + wj;
+        const int jj = (j0 - ey) * 2;
+        double x[3][5];   // fine (ii - 1 + d, jj - 1 + c)
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int c = 0; c < 5; ++c) {
+                const int col = jj - 1 + c;
+                if (c >= 3 && !two) {
+                    x[d][c] = 0.0;
+                    continue;
+                }
+                const bool central = crow[d] && col > ey && col < ny / 2 + ey;
+                x[d][c] = central ? Rr[d][col] : fr[d][col];
+            }
+#pragma unroll
+        for (int o = 0; o < 2; ++o) {
+            if (o == 1 && !two) break;
+            const int c = 1 + 2 * o;   // centre column of this output
+            const double v = 0.25 * x[1][c]
+                + 0.125 * (x[2][c] + x[1][c + 1] + x[0][c] + x[1][c - 1])
+                + 0.0625 * (x[2][c + 1] + x[2][c - 1] + x[0][c + 1] + x[0][c - 1]);
+            out[j0 + o] = v;
+        }
+    }
 }
 
 __global__ void __launch_bounds__(kThreads) k_coarsify(const CoarsifyJob* __restrict__ jobs) {
@@ -928,8 +958,8 @@ void launch_ell_pass(int pass, const EllJob* d_jobs, int njobs, int max_blocks, 
     }
 }
 
-void launch_coarsify(const CoarsifyJob* d_jobs, int njobs, int max_points, hipStream_t s, bool force_solve) {
-    dim3 grid((max_points + kThreads - 1) / kThreads, njobs);
+void launch_coarsify(const CoarsifyJob* d_jobs, int njobs, int max_rows, hipStream_t s, bool force_solve) {
+    dim3 grid(max_rows, njobs);
     hipLaunchKernelGGL(force_solve ? k_fs_coarsify : k_coarsify, grid, dim3(kThreads), 0, s, d_jobs);
 }
 

@@ -207,7 +207,7 @@ struct SolvePlan {
     std::vector<std::vector<double>> cjbytes;
     std::vector<double> cbytes;                     // per coarsify level
     std::vector<CoarsifyJob*> cj;                   // lev = 1..nlev-1
-    std::vector<int> cj_n, cj_pts;
+    std::vector<int> cj_n, cj_rows;   // coarsify jobs and the most coarse rows of a job, per level
     std::vector<std::vector<PassGroup>> steps;      // step s = 0..nlev-1 (coarsest level first)
 };
 
@@ -216,7 +216,7 @@ static int build_solve_plan(Ctx* c, DevMem& mem, const std::vector<SolveSpec>& s
     for (auto& s : specs) P.nlev = std::max(P.nlev, s.L);
     P.cj.assign(P.nlev, nullptr);
     P.cj_n.assign(P.nlev, 0);
-    P.cj_pts.assign(P.nlev, 0);
+    P.cj_rows.assign(P.nlev, 0);
     P.cflow.assign(P.nlev, {});
     P.cjbytes.assign(P.nlev, {});
     for (int lev = 1; lev < P.nlev; ++lev) {
@@ -226,7 +226,7 @@ static int build_solve_plan(Ctx* c, DevMem& mem, const std::vector<SolveSpec>& s
             const SolveSpec& s = specs[k];
             if (lev >= s.L) continue;
             jobs.push_back(CoarsifyJob{s.nx, s.ny, lev, s.f, s.R, s.skip});
-            pts = std::max(pts, (s.nx / 2 - 1) * (s.ny / 2 - 1));
+            pts = std::max(pts, s.nx / 2 - 1);
             P.cflow[lev].push_back((int)k);
             P.cjbytes[lev].push_back(8.0 * 5 * (s.nx / 2 - 1) * (s.ny / 2 - 1));
         }
@@ -235,7 +235,7 @@ static int build_solve_plan(Ctx* c, DevMem& mem, const std::vector<SolveSpec>& s
         CK(hipMemcpy(d, jobs.data(), sizeof(CoarsifyJob) * jobs.size(), hipMemcpyHostToDevice));
         P.cj[lev] = d;
         P.cj_n[lev] = (int)jobs.size();
-        P.cj_pts[lev] = pts;
+        P.cj_rows[lev] = pts;
     }
     P.steps.assign(P.nlev, {});
     P.cbytes.assign(P.nlev, 0.0);
@@ -316,7 +316,7 @@ static void run_solve(const SolvePlan& P, hipStream_t s, Probe* pr = nullptr,
     const bool fs = P.force_solve;
     for (int lev = 1; lev < P.nlev; ++lev)
         timed(pr, fs ? K_FSCOARSIFY : K_COARSIFY, pr ? sum(P.cflow[lev], P.cjbytes[lev], P.cbytes[lev]) : 0, s,
-              [&] { launch_coarsify(P.cj[lev], P.cj_n[lev], P.cj_pts[lev], s, fs); });
+              [&] { launch_coarsify(P.cj[lev], P.cj_n[lev], P.cj_rows[lev], s, fs); });
     for (int st = 0; st < P.nlev; ++st)
         for (const PassGroup& G : P.steps[st]) {
             const double by = pr ? sum(G.jflow, G.jbytes, G.bytes) : 0;

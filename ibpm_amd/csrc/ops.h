@@ -104,7 +104,7 @@ int blocks_for_pass(const EllJob& J, int pass);
 int lds_for_pass(const EllJob& J, int pass);
 void launch_ell_pass(int pass, const EllJob* d_jobs, int njobs, int max_blocks, int lds_bytes, hipStream_t s,
                      bool force_solve, int M);
-void launch_coarsify(const CoarsifyJob* d_jobs, int njobs, int max_points, hipStream_t s, bool force_solve = false);
+void launch_coarsify(const CoarsifyJob* d_jobs, int njobs, int max_rows, hipStream_t s, bool force_solve = false);
 void launch_vel(const FlowOps* d, int nflows, long max_pts, hipStream_t s);
 void launch_cross(const FlowOps* d, int nflows, long max_nf, int lev, hipStream_t s);
 void launch_rhs(const FlowOps* d, int nflows, long max_pts, hipStream_t s);

@@ -369,11 +369,13 @@ __device__ __forceinline__ void load_first_half(double2* v, const IBPM_GLOBAL do
     }
 }
 
-// In: v[r] = z[t + 128 r], r < 16.  Out: o[m] = Z[t + 128 m], m < 8.  The
-// buffer is free again on return (the last LDS access precedes the last
-// barrier).
-__device__ __forceinline__ void fft2048_regs(double2* v, double2* o, double2* buf, const IBPM_GLOBAL double* tw,
-                                             const IBPM_GLOBAL double* twg) {
+// In: v[r] = z[t + 128 r], r < 16.  Out: emit(m, Z[t + 128 m]) for m < 8,
+// called as each output is ready (so callers write results without holding
+// all eight).  The buffer is free again on return (the last LDS access
+// precedes the last barrier).
+template <class Emit>
+__device__ __forceinline__ void fft2048_regs(double2* v, double2* buf, const IBPM_GLOBAL double* tw,
+                                             const IBPM_GLOBAL double* twg, const Emit& emit) {
     const int t = threadIdx.x;
     double* bd = (double*)buf;
     dft16(v);   // stage 1 (Ns = 1): outputs 16 t + r to slots 17 t + r; reads
@@ -405,7 +407,7 @@ __device__ __forceinline__ void fft2048_regs(double2* v, double2* o, double2* bu
         u[7] = cmul(u[7], cmul(w4, w3));
         dft<8>(u);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) o[h + 2 * r] = u[r];   // Z[j + 256 r], r < 4
+        for (int r = 0; r < 4; ++r) emit(h + 2 * r, u[r]);   // Z[j + 256 r], r < 4
     }
 }
 
@@ -696,15 +698,11 @@ __device__ __forceinline__ void pass1_r16(const EllJob& J, double2* smem, int bx
     auto work = [&](int q, double2* v) {
         const int ra = 2 * q;
         const bool hb = ra + 1 < N1;
-        double2 o[8];
         mirror_second_half(v, buf);
-        fft2048_regs(v, o, buf, tw, twg);
-#pragma unroll
-        for (int m = 0; m < 8; ++m) {
+        fft2048_regs(v, buf, tw, twg, [&](int m, double2 z) {
             const int kk = t + 128 * m;
-            if (kk == 0) continue;
-            store_pair(gout(J.T1) + (long)(kk - 1) * N1 + ra, -o[m].y, o[m].x, hb);
-        }
+            if (kk > 0) store_pair(gout(J.T1) + (long)(kk - 1) * N1 + ra, -z.y, z.x, hb);
+        });
     };
     r16_pairs(bx, npairs, load, work);
 }
@@ -724,19 +722,18 @@ __device__ __forceinline__ void pass2_r16(const EllJob& J, double2* smem, int bx
     auto work = [&](int q, double2* v) {
         const int la = 2 * q, lb = la + 1;
         const bool hb = lb < N2;
-        double2 o[8];
         mirror_second_half(v, buf);
-        fft2048_regs(v, o, buf, tw, twg);
-        // spectral scaling, EllipticSolver2d.cc:42-54 (eigenvalues), :89-93 (scale)
+        // spectral scaling, EllipticSolver2d.cc:42-54 (eigenvalues), :89-93 (scale),
+        // into the first half of the odd extension for the second transform
         const double cya = gin(J.cosy)[la], cyb = hb ? gin(J.cosy)[lb] : 0.0;
-#pragma unroll
-        for (int m = 0; m < 8; ++m) {
+        double2 y[8];
+        fft2048_regs(v, buf, tw, twg, [&](int m, double2 z) {
             const int kk = t + 128 * m;
             double ya = 0.0, yb = 0.0;
             if (kk > 0) {
                 const double cx = gcx[kk - 1];
-                ya = -o[m].y;
-                yb = o[m].x;
+                ya = -z.y;
+                yb = z.x;
                 const double lama = 2. * (cx + cya - 2.) * J.bydx2;
                 ya *= (J.kind == 0) ? 1. / lama : 1. / (1 + J.alpha * lama);
                 if (hb) {
@@ -744,16 +741,15 @@ __device__ __forceinline__ void pass2_r16(const EllJob& J, double2* smem, int bx
                     yb *= (J.kind == 0) ? 1. / lamb : 1. / (1 + J.alpha * lamb);
                 }
             }
-            v[m] = make_double2(ya, yb);   // odd extension of the scaled spectrum
-        }
-        mirror_second_half(v, buf);
-        fft2048_regs(v, o, buf, tw, twg);
+            y[m] = make_double2(ya, yb);
+        });
 #pragma unroll
-        for (int m = 0; m < 8; ++m) {
+        for (int m = 0; m < 8; ++m) v[m] = y[m];
+        mirror_second_half(v, buf);
+        fft2048_regs(v, buf, tw, twg, [&](int m, double2 z) {
             const int kk = t + 128 * m;
-            if (kk == 0) continue;
-            store_pair(gout(J.T2) + (long)(kk - 1) * N2 + la, -o[m].y, o[m].x, hb);
-        }
+            if (kk > 0) store_pair(gout(J.T2) + (long)(kk - 1) * N2 + la, -z.y, z.x, hb);
+        });
     };
     r16_pairs(bx, npairs, load, work);
 }
@@ -772,29 +768,26 @@ __device__ __forceinline__ void pass3_r16(const EllJob& J, double2* smem, int bx
     auto work = [&](int q, double2* v) {
         const int ra = 2 * q, rb = ra + 1;
         const bool hb = rb < N1;
-        double2 o[8];
         mirror_second_half(v, buf);
-        fft2048_regs(v, o, buf, tw, twg);
-#pragma unroll
-        for (int m = 0; m < 8; ++m) {
+        fft2048_regs(v, buf, tw, twg, [&](int m, double2 z) {
             const int kk = t + 128 * m;
-            if (kk == 0) continue;
+            if (kk == 0) return;
             const int k = kk - 1;
             // sinTransformInv normalisation, EllipticSolver2d.cc:80-83; with the
             // epilogue the finer level's BCs come from v (EllipticSolver.cc:62-66)
             {
                 const long o2 = (long)ra * N2 + k;
-                const double val_a = -o[m].y * J.norm;
+                const double val_a = -z.y * J.norm;
                 gout(J.u)[o2] = J.epilogue ? gin(J.W)[o2] - val_a : val_a;
                 if (J.vside && on_bc_ring(J, ra + 1, k + 1)) gout(J.vside)[o2] = val_a;
             }
             if (hb) {
                 const long o2 = (long)rb * N2 + k;
-                const double val_b = o[m].x * J.norm;
+                const double val_b = z.x * J.norm;
                 gout(J.u)[o2] = J.epilogue ? gin(J.W)[o2] - val_b : val_b;
                 if (J.vside && on_bc_ring(J, rb + 1, k + 1)) gout(J.vside)[o2] = val_b;
             }
-        }
+        });
     };
     r16_pairs(bx, npairs, load, work);
 }

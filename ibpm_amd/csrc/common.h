@@ -31,7 +31,7 @@ struct FftPlan {
     int nst;        // number of Stockham stages
     int rad[kMaxStages];
     const double2* tw;   // tw[k] = exp(-2 pi i k / M), k < M
-    const double2* tw16; // M = 2048: the second radix-16 stage's w^(8 jm r) at 16 r + jm (r, jm < 16)
+    const double2* tw16; // M = 1024, 2048: the second radix-16 stage's w^((M/256) jm r) at 16 r + jm (r, jm < 16)
 };
 
 // One level of one flow in a multi-domain elliptic solve.

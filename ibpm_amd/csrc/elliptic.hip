@@ -288,26 +288,31 @@ __device__ inline void fft_any(double2* buf, const double2* tw, const FftPlan& P
     else fft_rt(buf, tw, P);
 }
 
-// ------------------------------------------- 2048-point transforms in registers
-// The 1023-point grids (M = 2048) run one row pair per 128-thread workgroup as
-// three radix stages 16 x 16 x 8 held in registers: the first stage's inputs
-// z[t + 128 r] come straight from global memory, the two exchanges between
-// stages go through LDS, and the last stage leaves Z[t + 128 m], m < 8 -- the
-// half of the spectrum a DST-I needs -- in registers for the caller to write.
-// That is 4 LDS sweeps of the 32 KB buffer per transform instead of 10.
-constexpr int kR16Threads = 128;
+// ------------------------------------- 1024- and 2048-point transforms in registers
+// The 511- and 1023-point grids (M = 1024, 2048) run one row pair per
+// workgroup of T = M/16 threads as three radix stages 16 x 16 x R3 (R3 =
+// M/256) held in registers: the first stage's inputs z[t + T r] come straight
+// from global memory, the two exchanges between stages go through LDS, and
+// the last stage hands Z[t + T m], m < 8 -- the half of the spectrum a DST-I
+// needs -- to the caller as it is ready.
+// A launch at M = 1024 runs only a quarter wave per CU when it holds few rows
+// (one 511-row level: 256 workgroups of one wave), where the LDS passes'
+// larger workgroups hide latency better; it takes the register-held
+// transform from kRegMinPairs row pairs on.  M = 2048 always takes it.
+template <int M>
+__host__ __device__ constexpr bool r16_len() { return M == 1024 || M == 2048; }
+constexpr long kRegMinPairs = 1024;
 
-template <int CM>
-__host__ __device__ constexpr int pass_threads() { return CM == 2048 ? kR16Threads : kFftThreads; }
+template <int CM, bool REG>
+__host__ __device__ constexpr int pass_threads() { return REG ? CM / 16 : kFftThreads; }
 
 // LDS slot of element e: one pad slot after every 16, so the first stage's
 // writes (16 consecutive elements per lane) spread over the banks, and every
 // access of a stage is a per-lane base plus a compile-time offset.  The
 // exchanges move real and imaginary parts in two rounds through a buffer of
-// doubles (17 KB), which lets twice as many workgroups share a CU as a buffer
-// of complex values would; kR16Buf counts that buffer in complex slots.
-constexpr int kR16Slots = 2048 + 128;
-constexpr int kR16Buf = kR16Slots / 2;
+// M + M/16 doubles, half the footprint of a buffer of complex values, so
+// twice as many workgroups share a CU.  r16_buf counts it in complex slots.
+__host__ __device__ constexpr int r16_buf(int M) { return (M + M / 16) / 2; }
 __device__ inline int pad16(int e) { return e + (e >> 4); }
 
 // One exchange: every lane writes its 16 values at wr[k * ws], reads 16 at
@@ -329,27 +334,23 @@ __device__ __forceinline__ void exchange16(double2* v, double* wr, const double*
     __syncthreads();
 }
 
-// Twiddles, w = exp(-2 pi i / 2048), come from global memory (cached): the
-// second stage's w^(8 jm r) from the plan's 256-entry table at 16 r + jm, so
-// 16 lanes with consecutive jm read consecutive entries; the third stage's
-// w^j (j < 256) from the full table.
-constexpr int kR16Tw = 0;
-
-// Second half of the odd extension from the first: v[r] = z[t + 128 r] is
-// set for r < 8; z[e] = -z[2048 - e] (another lane's) for e > 1024 comes
-// through LDS, z[1024] = 0.  The buffer is free again on return.
+// Second half of the odd extension from the first: v[r] = z[t + T r] is set
+// for r < 8; z[e] = -z[M - e] (another lane's) for e > M/2 comes through LDS,
+// z[M/2] = 0.  The buffer is free again on return.
+template <int M>
 __device__ __forceinline__ void mirror_second_half(double2* v, double2* buf) {
+    constexpr int T = M / 16;
     const int t = threadIdx.x;
 #pragma unroll
-    for (int r = 0; r < 8; ++r) buf[t + 128 * r] = v[r];
+    for (int r = 0; r < 8; ++r) buf[t + T * r] = v[r];
     __syncthreads();
 #pragma unroll
     for (int r = 8; r < 16; ++r) {
-        const int e = t + 128 * r;
-        if (e == 1024) {
+        const int e = t + T * r;
+        if (e == M / 2) {
             v[r] = make_double2(0.0, 0.0);
         } else {
-            const double2 y = buf[2048 - e];
+            const double2 y = buf[M - e];
             v[r] = make_double2(-y.x, -y.y);
         }
     }
@@ -357,57 +358,65 @@ __device__ __forceinline__ void mirror_second_half(double2* v, double2* buf) {
 }
 
 // First half of the odd extension of two rows a, b (b absent: zeros):
-// v[r] = (a, b)[e - 1] at e = t + 128 r, r < 8, and 0 at e = 0.
+// v[r] = (a, b)[e - 1] at e = t + T r, r < 8, and 0 at e = 0.
+template <int M>
 __device__ __forceinline__ void load_first_half(double2* v, const IBPM_GLOBAL double* a,
                                                 const IBPM_GLOBAL double* b, bool hb) {
+    constexpr unsigned T = M / 16;
     const unsigned t = threadIdx.x;
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
-        const unsigned e = t + 128u * r, k = e == 0 ? 0u : e - 1;
+        const unsigned e = t + T * r, k = e == 0 ? 0u : e - 1;
         const double xa = a[k], xb = hb ? b[k] : 0.0;
         v[r] = e == 0 ? make_double2(0.0, 0.0) : make_double2(xa, xb);
     }
 }
 
-// In: v[r] = z[t + 128 r], r < 16.  Out: emit(m, Z[t + 128 m]) for m < 8,
-// called as each output is ready (so callers write results without holding
-// all eight).  The buffer is free again on return (the last LDS access
-// precedes the last barrier).
-template <class Emit>
-__device__ __forceinline__ void fft2048_regs(double2* v, double2* buf, const IBPM_GLOBAL double* tw,
-                                             const IBPM_GLOBAL double* twg, const Emit& emit) {
+// In: v[r] = z[t + T r], r < 16.  Out: emit(m, Z[t + T m]) for m < 8, called
+// as each output is ready (so callers write results without holding all
+// eight).  Twiddles, w = exp(-2 pi i / M), come from global memory (cached):
+// the second stage's w^(R3 jm r) from the plan's 256-entry table at 16 r + jm,
+// so 16 lanes with consecutive jm read consecutive entries; the third
+// stage's w^j (j < 256) from the full table.  The buffer is free again on
+// return (the last LDS access precedes the last barrier).
+template <int M, class Emit>
+__device__ __forceinline__ void fft_regs(double2* v, double2* buf, const IBPM_GLOBAL double* tw,
+                                         const IBPM_GLOBAL double* twg, const Emit& emit) {
+    constexpr int T = M / 16, R3 = M / 256, H = 256 / T, RS = 17 * T / 16;
     const int t = threadIdx.x;
     double* bd = (double*)buf;
     dft16(v);   // stage 1 (Ns = 1): outputs 16 t + r to slots 17 t + r; reads
-                // t + 128 r from slots pad16(t) + 136 r
-    exchange16<1, 136>(v, bd + 17 * t, bd + pad16(t));
-    const int jm = t & 15;   // stage 2 (Ns = 16): twiddles w^(8 jm r) from the table
+                // t + T r from slots pad16(t) + RS r
+    exchange16<1, RS>(v, bd + 17 * t, bd + pad16(t));
+    const int jm = t & 15;   // stage 2 (Ns = 16): twiddles w^(R3 jm r) from the table
 #pragma unroll
     for (int r = 1; r < 16; ++r) v[r] = cmul(v[r], make_double2(tw[2 * (16 * r + jm)], tw[2 * (16 * r + jm) + 1]));
     dft16(v);
     // outputs (t/16) 256 + jm + 16 r to slots (t/16) 272 + jm + 17 r; stage 3
-    // reads j + 256 r (j = t, t + 128) from slots pad16(j) + 272 r, which is
-    // the exchange16 read pattern with the two butterflies' inputs
-    // interleaved: lane t reads pad16(t) + 136 k for k = 0..15
-    exchange16<17, 136>(v, bd + (t >> 4) * 272 + jm, bd + pad16(t));
+    // reads j + 256 r (j = t + T h, h < H) = t + T (h + H r), the exchange16
+    // read pattern: lane t reads pad16(t) + RS k for k = 0..15
+    exchange16<17, RS>(v, bd + (t >> 4) * 272 + jm, bd + pad16(t));
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {   // stage 3 (Ns = 256): butterflies j = t, t + 128
-        const int j = t + 128 * h;
-        double2 u[8];
+    for (int h = 0; h < H; ++h) {   // stage 3 (Ns = 256): butterflies j = t + T h
+        const int j = t + T * h;
+        double2 u[R3];
 #pragma unroll
-        for (int r = 0; r < 8; ++r) u[r] = v[2 * r + h];   // element j + 256 r = t + 128 (2 r + h)
+        for (int r = 0; r < R3; ++r) u[r] = v[h + H * r];
         const double2 w1 = make_double2(twg[2 * j], twg[2 * j + 1]);
-        const double2 w2 = cmul(w1, w1), w3 = cmul(w2, w1), w4 = cmul(w2, w2);
+        const double2 w2 = cmul(w1, w1), w3 = cmul(w2, w1);
         u[1] = cmul(u[1], w1);
         u[2] = cmul(u[2], w2);
         u[3] = cmul(u[3], w3);
-        u[4] = cmul(u[4], w4);
-        u[5] = cmul(u[5], cmul(w4, w1));
-        u[6] = cmul(u[6], cmul(w4, w2));
-        u[7] = cmul(u[7], cmul(w4, w3));
-        dft<8>(u);
+        if constexpr (R3 == 8) {
+            const double2 w4 = cmul(w2, w2);
+            u[4] = cmul(u[4], w4);
+            u[5] = cmul(u[5], cmul(w4, w1));
+            u[6] = cmul(u[6], cmul(w4, w2));
+            u[7] = cmul(u[7], cmul(w4, w3));
+        }
+        dft<R3>(u);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) emit(h + 2 * r, u[r]);   // Z[j + 256 r], r < 4
+        for (int r = 0; r < R3 / 2; ++r) emit(h + H * r, u[r]);   // Z[j + 256 r] with j + 256 r < M/2
     }
 }
 
@@ -627,8 +636,8 @@ __device__ __forceinline__ void store_pair(IBPM_GLOBAL double* p, double a, doub
     else p[0] = a;
 }
 
-// The three passes for M = 2048 (N = 1023), the same arithmetic as the LDS
-// passes above around a register-held transform, one row pair q per
+// The three passes for M = 1024, 2048 (N = M/2 - 1), the same arithmetic as the
+// LDS passes above around a register-held transform, one row pair q per
 // workgroup: load(q, v) fills the first half of the odd extension, work(q, v)
 // transforms and writes.  (Two pairs per workgroup, the second's loads issued
 // before the first's transform, measured slower: fewer, longer workgroups.)
@@ -642,7 +651,9 @@ __device__ __forceinline__ void r16_pairs(int bx, int npairs, const Load& load, 
     work(bx, v);
 }
 
+template <int M>
 __device__ __forceinline__ void pass1_r16(const EllJob& J, double2* smem, int bx) {
+    constexpr int T = M / 16;
     const int N1 = J.nx - 1, npairs = (N1 + 1) / 2;
     if (kR16Pairs * bx >= npairs) return;
     double2* buf = smem;
@@ -654,7 +665,7 @@ __device__ __forceinline__ void pass1_r16(const EllJob& J, double2* smem, int bx
     const double sc = J.in_scale;
     const bool bcs = J.uc || J.bcring;
     // rhs_value along rows ra, rb = ra + 1 with the row-invariant parts
-    // hoisted: the first half of the odd extension, e = t + 128 r (r < 8) at
+    // hoisted: the first half of the odd extension, e = t + T r (r < 8) at
     // column j = e; the two edge rows of the level keep the general form
     auto load = [&](int q, double2* v) {
         const int ra = 2 * q, rb = ra + 1;
@@ -662,7 +673,7 @@ __device__ __forceinline__ void pass1_r16(const EllJob& J, double2* smem, int bx
         if ((ra == 0 || rb >= N1 - 1) && bcs) {
 #pragma unroll
             for (int r = 0; r < 8; ++r) {
-                const int e = t + 128 * r;
+                const int e = t + T * r;
                 v[r] = e == 0 ? make_double2(0.0, 0.0)
                               : make_double2(rhs_value(J, ra + 1, e), hb ? rhs_value(J, rb + 1, e) : 0.0);
             }
@@ -676,20 +687,20 @@ __device__ __forceinline__ void pass1_r16(const EllJob& J, double2* smem, int bx
         const IBPM_GLOBAL double* Rb = gin(J.R) + sidx(J.nx, J.ny, hb ? rb + 1 : ra + 1, 1);
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
-            const int e = t + 128 * r, k = e - 1;
+            const int e = t + T * r, k = e - 1;
             const bool colc = e > jc0 && e < jc1;
             const int kk = e == 0 ? 0 : k;   // e = 0 is the extension's zero
             const double xa = ((cra && colc) ? Ra : fa)[kk] * sc;
             const double xb = hb ? ((crb && colc) ? Rb : fb)[kk] * sc : 0.0;
             v[r] = e == 0 ? make_double2(0.0, 0.0) : make_double2(xa, xb);
         }
-        if (bcs) {   // getRHS at j = 1 (e = 1) and j = ny - 1 (e = 1023)
+        if (bcs) {   // getRHS at j = 1 (e = 1) and j = ny - 1 (e = M/2 - 1 = T - 1 + 7 T)
             const double c = J.bc_coef;
             if (t == 1) {
                 v[0].x -= bc_bottom(J, ra + 1) * c;
                 if (hb) v[0].y -= bc_bottom(J, rb + 1) * c;
             }
-            if (t == 127) {
+            if (t == T - 1) {
                 v[7].x -= bc_top(J, ra + 1) * c;
                 if (hb) v[7].y -= bc_top(J, rb + 1) * c;
             }
@@ -698,17 +709,19 @@ __device__ __forceinline__ void pass1_r16(const EllJob& J, double2* smem, int bx
     auto work = [&](int q, double2* v) {
         const int ra = 2 * q;
         const bool hb = ra + 1 < N1;
-        mirror_second_half(v, buf);
-        fft2048_regs(v, buf, tw, twg, [&](int m, double2 z) {
-            const int kk = t + 128 * m;
+        mirror_second_half<M>(v, buf);
+        fft_regs<M>(v, buf, tw, twg, [&](int m, double2 z) {
+            const int kk = t + T * m;
             if (kk > 0) store_pair(gout(J.T1) + (long)(kk - 1) * N1 + ra, -z.y, z.x, hb);
         });
     };
     r16_pairs(bx, npairs, load, work);
 }
 
+template <int M>
 __device__ __forceinline__ void pass2_r16(const EllJob& J, double2* smem, int bx) {
-    const int N1 = 1023, N2 = J.ny - 1, npairs = (N2 + 1) / 2;
+    constexpr int T = M / 16;
+    const int N1 = M / 2 - 1, N2 = J.ny - 1, npairs = (N2 + 1) / 2;
     if (kR16Pairs * bx >= npairs) return;
     double2* buf = smem;
     const IBPM_GLOBAL double* tw = gin((const double*)J.pi.tw16);
@@ -717,18 +730,18 @@ __device__ __forceinline__ void pass2_r16(const EllJob& J, double2* smem, int bx
     const IBPM_GLOBAL double* gcx = gin(J.cosx);
     auto load = [&](int q, double2* v) {
         const int la = 2 * q, lb = la + 1;
-        load_first_half(v, gin(J.T1) + (long)la * N1, gin(J.T1) + (long)lb * N1, lb < N2);
+        load_first_half<M>(v, gin(J.T1) + (long)la * N1, gin(J.T1) + (long)lb * N1, lb < N2);
     };
     auto work = [&](int q, double2* v) {
         const int la = 2 * q, lb = la + 1;
         const bool hb = lb < N2;
-        mirror_second_half(v, buf);
+        mirror_second_half<M>(v, buf);
         // spectral scaling, EllipticSolver2d.cc:42-54 (eigenvalues), :89-93 (scale),
         // into the first half of the odd extension for the second transform
         const double cya = gin(J.cosy)[la], cyb = hb ? gin(J.cosy)[lb] : 0.0;
         double2 y[8];
-        fft2048_regs(v, buf, tw, twg, [&](int m, double2 z) {
-            const int kk = t + 128 * m;
+        fft_regs<M>(v, buf, tw, twg, [&](int m, double2 z) {
+            const int kk = t + T * m;
             double ya = 0.0, yb = 0.0;
             if (kk > 0) {
                 const double cx = gcx[kk - 1];
@@ -745,17 +758,19 @@ __device__ __forceinline__ void pass2_r16(const EllJob& J, double2* smem, int bx
         });
 #pragma unroll
         for (int m = 0; m < 8; ++m) v[m] = y[m];
-        mirror_second_half(v, buf);
-        fft2048_regs(v, buf, tw, twg, [&](int m, double2 z) {
-            const int kk = t + 128 * m;
+        mirror_second_half<M>(v, buf);
+        fft_regs<M>(v, buf, tw, twg, [&](int m, double2 z) {
+            const int kk = t + T * m;
             if (kk > 0) store_pair(gout(J.T2) + (long)(kk - 1) * N2 + la, -z.y, z.x, hb);
         });
     };
     r16_pairs(bx, npairs, load, work);
 }
 
+template <int M>
 __device__ __forceinline__ void pass3_r16(const EllJob& J, double2* smem, int bx) {
-    const int N1 = J.nx - 1, N2 = 1023, npairs = (N1 + 1) / 2;
+    constexpr int T = M / 16;
+    const int N1 = J.nx - 1, N2 = M / 2 - 1, npairs = (N1 + 1) / 2;
     if (kR16Pairs * bx >= npairs) return;
     double2* buf = smem;
     const IBPM_GLOBAL double* tw = gin((const double*)J.pj.tw16);
@@ -763,14 +778,14 @@ __device__ __forceinline__ void pass3_r16(const EllJob& J, double2* smem, int bx
     const int t = threadIdx.x;
     auto load = [&](int q, double2* v) {
         const int ra = 2 * q, rb = ra + 1;
-        load_first_half(v, gin(J.T2) + (long)ra * N2, gin(J.T2) + (long)rb * N2, rb < N1);
+        load_first_half<M>(v, gin(J.T2) + (long)ra * N2, gin(J.T2) + (long)rb * N2, rb < N1);
     };
     auto work = [&](int q, double2* v) {
         const int ra = 2 * q, rb = ra + 1;
         const bool hb = rb < N1;
-        mirror_second_half(v, buf);
-        fft2048_regs(v, buf, tw, twg, [&](int m, double2 z) {
-            const int kk = t + 128 * m;
+        mirror_second_half<M>(v, buf);
+        fft_regs<M>(v, buf, tw, twg, [&](int m, double2 z) {
+            const int kk = t + T * m;
             if (kk == 0) return;
             const int k = kk - 1;
             // sinTransformInv normalisation, EllipticSolver2d.cc:80-83; with the
@@ -806,12 +821,12 @@ __device__ inline int2 xcd_block() {
     return make_int2(lf % gx, lf / gx);
 }
 
-// Pass bodies by length: the register-held transform for M = 2048, the LDS
-// passes for every other length.
+// Pass bodies: the register-held transform (REG, M = 1024 or 2048) or the LDS
+// passes.
 #define IBPM_PASS_BODY(N)                                                                  \
-    template <int CM>                                                                      \
+    template <int CM, bool REG>                                                            \
     __device__ __forceinline__ void pass##N##_body(const EllJob& J, double2* smem, int bx) { \
-        if constexpr (CM == 2048) pass##N##_r16(J, smem, bx);                              \
+        if constexpr (REG) pass##N##_r16<CM>(J, smem, bx);                                 \
         else pass##N##_lds<CM>(J, smem, bx);                                               \
     }
 IBPM_PASS_BODY(1)
@@ -825,13 +840,14 @@ IBPM_PASS_BODY(3)
 // profiles separate the two.  A job whose flow has finished its force solve
 // does nothing (skipped()).
 #define IBPM_PASS_KERNEL(NAME, BODY)                                             \
-    template <int CM>                                                            \
-    __global__ void __launch_bounds__(pass_threads<CM>()) NAME(const EllJob* __restrict__ jobs) { \
+    template <int CM, bool REG>                                                  \
+    __global__ void __launch_bounds__((pass_threads<CM, REG>())) NAME(const EllJob* __restrict__ jobs) { \
+        static_assert(!REG || r16_len<CM>(), "no register-held transform at this length"); \
         extern __shared__ double2 smem[];                                         \
         const int2 blk = xcd_block();                                             \
         const EllJob& J = jobs[blk.y];                                            \
         if (skipped(J.skip)) return;                                              \
-        BODY<CM>(J, smem, blk.x);                                                 \
+        BODY<CM, REG>(J, smem, blk.x);                                            \
     }
 IBPM_PASS_KERNEL(k_ell_pass1, pass1_body)
 IBPM_PASS_KERNEL(k_ell_pass2, pass2_body)
@@ -909,35 +925,43 @@ __global__ void __launch_bounds__(kThreads) k_fs_coarsify(const CoarsifyJob* __r
 }
 
 // ------------------------------------------------------------ host launch
-int lds_for_pass(const EllJob& J, int pass) {
+bool pass_in_registers(int M, long row_pairs) {
+    return M == 2048 || (M == 1024 && row_pairs >= kRegMinPairs);
+}
+
+int lds_for_pass(const EllJob& J, int pass, bool regs) {
     const FftPlan& P = (pass == 2) ? J.pi : J.pj;
-    if (P.M == 2048) return (int)sizeof(double2) * (kR16Buf + kR16Tw);   // 17 KB
+    if (regs) return (int)sizeof(double2) * r16_buf(P.M);   // 8.5 or 17 KB
     const int ntw = (compiled_len(P.M) && quarter_twiddles(P.M)) ? P.M / 4 : P.M;
     return (int)sizeof(double2) * (P.np * P.M + ntw);
 }
 
-int blocks_for_pass(const EllJob& J, int pass) {
+int blocks_for_pass(const EllJob& J, int pass, bool regs) {
     const int rows_total = (pass == 2) ? J.ny - 1 : J.nx - 1;
     const FftPlan& P = (pass == 2) ? J.pi : J.pj;
-    const int rows = P.M == 2048 ? 2 * kR16Pairs : 2 * P.np;
+    const int rows = regs ? 2 * kR16Pairs : 2 * P.np;
     return (rows_total + rows - 1) / rows;
 }
 
 #define IBPM_LAUNCH_LEN(KER)                                                                   \
     switch (M) {                                                                               \
-    case 400: hipLaunchKernelGGL(KER<400>, grid, dim3(kFftThreads), lds_bytes, s, d_jobs); break;   \
-    case 512: hipLaunchKernelGGL(KER<512>, grid, dim3(kFftThreads), lds_bytes, s, d_jobs); break;   \
-    case 768: hipLaunchKernelGGL(KER<768>, grid, dim3(kFftThreads), lds_bytes, s, d_jobs); break;   \
-    case 800: hipLaunchKernelGGL(KER<800>, grid, dim3(kFftThreads), lds_bytes, s, d_jobs); break;   \
-    case 1024: hipLaunchKernelGGL(KER<1024>, grid, dim3(kFftThreads), lds_bytes, s, d_jobs); break; \
-    case 2048: hipLaunchKernelGGL(KER<2048>, grid, dim3(kR16Threads), lds_bytes, s, d_jobs); break; \
-    default: hipLaunchKernelGGL(KER<0>, grid, dim3(kFftThreads), lds_bytes, s, d_jobs); break;       \
+    case 400: hipLaunchKernelGGL((KER<400, false>), grid, dim3(kFftThreads), lds_bytes, s, d_jobs); break;  \
+    case 512: hipLaunchKernelGGL((KER<512, false>), grid, dim3(kFftThreads), lds_bytes, s, d_jobs); break;  \
+    case 768: hipLaunchKernelGGL((KER<768, false>), grid, dim3(kFftThreads), lds_bytes, s, d_jobs); break;  \
+    case 800: hipLaunchKernelGGL((KER<800, false>), grid, dim3(kFftThreads), lds_bytes, s, d_jobs); break;  \
+    case 1024:                                                                                 \
+        if (regs) hipLaunchKernelGGL((KER<1024, true>), grid, dim3(1024 / 16), lds_bytes, s, d_jobs);    \
+        else hipLaunchKernelGGL((KER<1024, false>), grid, dim3(kFftThreads), lds_bytes, s, d_jobs);     \
+        break;                                                                                 \
+    case 2048: hipLaunchKernelGGL((KER<2048, true>), grid, dim3(2048 / 16), lds_bytes, s, d_jobs); break;  \
+    default: hipLaunchKernelGGL((KER<0, false>), grid, dim3(kFftThreads), lds_bytes, s, d_jobs); break;      \
     }
 
 // M: the FFT length of this pass (J.pj.M for passes 1 and 3, J.pi.M for
-// pass 2), the same for every job of the launch.
+// pass 2), the same for every job of the launch; regs: pass_in_registers for
+// the launch, with max_blocks and lds_bytes sized for that choice.
 void launch_ell_pass(int pass, const EllJob* d_jobs, int njobs, int max_blocks, int lds_bytes, hipStream_t s,
-                     bool force_solve, int M) {
+                     bool force_solve, int M, bool regs) {
     static_assert(compiled_len(400) && compiled_len(512) && compiled_len(768) && compiled_len(800) &&
                       compiled_len(1024) && compiled_len(2048),
                   "the launch switch and compiled_len must list the same lengths");

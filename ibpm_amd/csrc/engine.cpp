@@ -115,10 +115,11 @@ static int get_plan(Ctx* c, int N, FftPlan* out) {
     CK(hipMemcpy(d, tw.data(), sizeof(double2) * P.M, hipMemcpyHostToDevice));
     P.tw = d;
     P.tw16 = nullptr;
-    if (P.M == 2048) {   // second-stage table of the register-held 2048-point transform
+    if (P.M == 1024 || P.M == 2048) {   // second-stage table of the register-held transforms
+        const int tstep = P.M / 256;
         std::vector<double2> t16(256);
         for (int r = 0; r < 16; ++r)
-            for (int jm = 0; jm < 16; ++jm) t16[16 * r + jm] = tw[(8 * jm * r) % P.M];
+            for (int jm = 0; jm < 16; ++jm) t16[16 * r + jm] = tw[(tstep * jm * r) % P.M];
         double2* d16 = nullptr;
         CKR(c->mem.alloc(&d16, 256));
         CK(hipMemcpy(d16, t16.data(), sizeof(double2) * 256, hipMemcpyHostToDevice));
@@ -195,6 +196,7 @@ struct PassGroup {
     int n = 0;
     int mj = 0, mi = 0;                             // FFT lengths along j (passes 1, 3) and i (pass 2)
     int maxb[3] = {0, 0, 0}, lds[3] = {0, 0, 0};
+    bool regs[3] = {false, false, false};           // pass_in_registers per pass
     std::vector<int> jflow;                         // flow of each job
     std::vector<double> jbytes;                     // 16 B per point (one read + one write)
     double bytes = 0;
@@ -289,11 +291,15 @@ static int build_solve_plan(Ctx* c, DevMem& mem, const std::vector<SolveSpec>& s
         for (size_t g = 0; g < groups.size(); ++g) {
             PassGroup& G = groups[g];
             std::vector<EllJob>& jobs = gjobs[g];
-            for (auto& J : jobs)
-                for (int k = 0; k < 3; ++k) {
-                    G.maxb[k] = std::max(G.maxb[k], blocks_for_pass(J, k + 1));
-                    G.lds[k] = std::max(G.lds[k], lds_for_pass(J, k + 1));
+            for (int k = 0; k < 3; ++k) {
+                long pairs = 0;
+                for (auto& J : jobs) pairs += (k == 1 ? J.ny : J.nx) / 2;   // ceil(rows / 2)
+                G.regs[k] = pass_in_registers(k == 1 ? G.mi : G.mj, pairs);
+                for (auto& J : jobs) {
+                    G.maxb[k] = std::max(G.maxb[k], blocks_for_pass(J, k + 1, G.regs[k]));
+                    G.lds[k] = std::max(G.lds[k], lds_for_pass(J, k + 1, G.regs[k]));
                 }
+            }
             CKR(mem.alloc(&G.d, jobs.size()));
             CK(hipMemcpy(G.d, jobs.data(), sizeof(EllJob) * jobs.size(), hipMemcpyHostToDevice));
             G.n = (int)jobs.size();
@@ -323,7 +329,7 @@ static void run_solve(const SolvePlan& P, hipStream_t s, Probe* pr = nullptr,
             for (int pass = 1; pass <= 3; ++pass)
                 timed(pr, (fs ? K_FSPASS1 : K_PASS1) + pass - 1, by, s, [&] {
                     launch_ell_pass(pass, G.d, G.n, G.maxb[pass - 1], G.lds[pass - 1], s, fs,
-                                    pass == 2 ? G.mi : G.mj);
+                                    pass == 2 ? G.mi : G.mj, G.regs[pass - 1]);
                 });
         }
 }
@@ -1488,8 +1494,10 @@ int ibpm_dst2d(ibpm_ctx* c, int n1, int n2, const double* d_in, double* d_out) {
     CKR(mem.alloc(&d, 2));
     CK(hipMemcpyAsync(d, &J, sizeof(EllJob), hipMemcpyHostToDevice, c->c.stream));
     CK(hipMemcpyAsync(d + 1, &J2, sizeof(EllJob), hipMemcpyHostToDevice, c->c.stream));
-    launch_ell_pass(1, d, 1, blocks_for_pass(J, 1), lds_for_pass(J, 1), c->c.stream, false, J.pj.M);
-    launch_ell_pass(1, d + 1, 1, blocks_for_pass(J2, 1), lds_for_pass(J2, 1), c->c.stream, false, J2.pj.M);
+    const bool r1 = pass_in_registers(J.pj.M, (n1 + 1) / 2), r2 = pass_in_registers(J2.pj.M, (n2 + 1) / 2);
+    launch_ell_pass(1, d, 1, blocks_for_pass(J, 1, r1), lds_for_pass(J, 1, r1), c->c.stream, false, J.pj.M, r1);
+    launch_ell_pass(1, d + 1, 1, blocks_for_pass(J2, 1, r2), lds_for_pass(J2, 1, r2), c->c.stream, false,
+                    J2.pj.M, r2);
     CK(hipStreamSynchronize(c->c.stream));
     CK(hipGetLastError());
     return 0;

@@ -100,10 +100,14 @@ struct ForceOps {
     int* count;                    // steps recorded so far (device counter)
 };
 
-int blocks_for_pass(const EllJob& J, int pass);
-int lds_for_pass(const EllJob& J, int pass);
+// Whether a pass launch of FFT length M over row_pairs row pairs (summed
+// over its jobs) uses the register-held transform; the three calls below take
+// that choice.
+bool pass_in_registers(int M, long row_pairs);
+int blocks_for_pass(const EllJob& J, int pass, bool regs);
+int lds_for_pass(const EllJob& J, int pass, bool regs);
 void launch_ell_pass(int pass, const EllJob* d_jobs, int njobs, int max_blocks, int lds_bytes, hipStream_t s,
-                     bool force_solve, int M);
+                     bool force_solve, int M, bool regs);
 void launch_coarsify(const CoarsifyJob* d_jobs, int njobs, int max_rows, hipStream_t s, bool force_solve = false);
 void launch_vel(const FlowOps* d, int nflows, long max_pts, hipStream_t s);
 void launch_cross(const FlowOps* d, int nflows, long max_nf, int lev, hipStream_t s);

@@ -55,7 +55,8 @@ def test_elliptic2d_closed_forms(ctx, oracle, kind):
 @pytest.mark.parametrize("nx,ny,ngrid,kind", [(16, 24, 3, 0), (16, 24, 3, 1), (200, 200, 1, 1),
                                               (400, 400, 4, 0), (400, 400, 4, 1), (512, 512, 5, 1),
                                               (256, 384, 3, 0), (1024, 1024, 3, 0), (1024, 1024, 2, 1),
-                                              (1024, 256, 2, 1), (256, 1024, 2, 0)])
+                                              (1024, 256, 2, 1), (256, 1024, 2, 0), (512, 256, 2, 0),
+                                              (256, 512, 3, 1), (1024, 512, 2, 1)])
 def test_multidomain_solve_matches_oracle(ctx, oracle, nx, ny, ngrid, kind):
     print("solve", nx, ny, ngrid, kind, flush=True)
     g = G(nx, ny, ngrid, 4.0, -2.0, -2.0)
@@ -165,6 +166,24 @@ def test_batch_of_uneven_flows_equals_single_flows(gpu):
     assert np.array_equal(both.omega(0), s1.omega())
     assert np.array_equal(both.omega(1), s2.omega())
     assert np.array_equal(both.f(1), s2.f())
+
+
+def test_batch_of_512_grids_takes_register_transform(gpu):
+    """Four 512^2 flows in one batch give the 1024-point passes 1024 row pairs
+    per launch, enough for the register-held transform; one flow alone (256
+    pairs) runs the LDS passes, which the multi-domain solve tests hold to the
+    oracle.  The two transforms round differently, so the runs agree to
+    1e-11 relative, not bit for bit."""
+    ctx = gpu.Context(0)
+    flows = [gpu.Flow(512, 512, 2, 4.0, -2.0, -2.0, Re, 0.01, CYL40) for Re in (100.0, 150.0, 200.0, 250.0)]
+    print("batch of", len(flows), "512^2 flows", flush=True)
+    both = gpu.Batch(ctx, flows).init()
+    both.advance(2)
+    for k, f in enumerate(flows):
+        one = gpu.Batch(ctx, [f]).init()
+        one.advance(2)
+        assert rel(both.omega(k), one.omega()) < 1e-11, k
+        assert rel(both.f(k), one.f()) < 1e-11, k
 
 
 def test_moving_plate_cg_matches_oracle(gpu, oracle):

@@ -71,8 +71,16 @@ struct CoarsifyJob {
 // Force-solve flags of a flow: flags[0] converged, flags[1] broken down.  A
 // kernel given a flow's flags does nothing for that flow once either is set,
 // so the batched M applications of finished flows cost no bandwidth.
-__host__ __device__ inline bool skipped(const double* flags) {
-    return flags && (flags[0] != 0.0 || flags[1] != 0.0);
+// The flags are uniform over a launch and written only by earlier launches
+// (a launch starts with its caches invalidated), so both are read at once
+// through the scalar cache: a workgroup's first loads then wait on no vector
+// memory round trip, where the two short-circuited vector loads took two.
+__device__ inline bool skipped(const double* flags) {
+    if (!flags) return false;
+    typedef const __attribute__((address_space(4))) double ConstDouble;
+    ConstDouble* c = (ConstDouble*)flags;
+    const double converged = c[0], broken = c[1];
+    return (converged != 0.0) | (broken != 0.0);
 }
 
 __host__ __device__ inline int nx_ext(int nx) { return nx / 4; }   // Grid.h:89-91, zero shift

@@ -358,17 +358,23 @@ __device__ __forceinline__ void mirror_second_half(double2* v, double2* buf) {
 }
 
 // First half of the odd extension of two rows a, b (b absent: zeros):
-// v[r] = (a, b)[e - 1] at e = t + T r, r < 8, and 0 at e = 0.
+// v[r] = (a, b)[e - 1] at e = t + T r, r < 8, and 0 at e = 0.  Row b is read
+// from row a when absent and multiplied by 0.  A select of the loaded value
+// would be compiled as a branch around the load, and a load under a branch
+// gets a wait for every load before it: the row pair's sixteen loads would
+// then complete one after another.
 template <int M>
 __device__ __forceinline__ void load_first_half(double2* v, const IBPM_GLOBAL double* a,
                                                 const IBPM_GLOBAL double* b, bool hb) {
     constexpr unsigned T = M / 16;
     const unsigned t = threadIdx.x;
+    const IBPM_GLOBAL double* bb = hb ? b : a;
+    const double keep = hb ? 1.0 : 0.0;   // a product, not a select: see above
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
         const unsigned e = t + T * r, k = e == 0 ? 0u : e - 1;
-        const double xa = a[k], xb = hb ? b[k] : 0.0;
-        v[r] = e == 0 ? make_double2(0.0, 0.0) : make_double2(xa, xb);
+        const double nz = e == 0 ? 0.0 : 1.0;   // e = 0 holds the extension's zero, also a product
+        v[r] = make_double2(a[k] * nz, bb[k] * keep * nz);
     }
 }
 
@@ -447,6 +453,16 @@ __device__ inline double bc_right(const EllJob& J, int j) {
     const int ii = J.nx / 2 + nx_ext(J.nx), jj = j / 2 + nx_ext(J.ny);
     const double a = gin(J.uc)[sidx(J.nx, J.ny, ii, jj)];
     return (j & 1) ? 0.5 * (a + gin(J.uc)[sidx(J.nx, J.ny, ii, jj + 1)]) : a;
+}
+
+// bc_left (left) or bc_right at j with no load under a lane-dependent branch:
+// for even j the second read repeats the first, and 0.5 (a + a) = a exactly.
+__device__ inline double bc_side_loads(const EllJob& J, bool left, int j) {
+    if (J.bcring) return left ? gin(J.bcring)[j] : gin(J.bcring)[2 * J.ny + J.nx - j];
+    const int ii = left ? nx_ext(J.nx) : J.nx / 2 + nx_ext(J.nx), jj = j / 2 + nx_ext(J.ny);
+    const double a = gin(J.uc)[sidx(J.nx, J.ny, ii, jj)];
+    const double b = gin(J.uc)[sidx(J.nx, J.ny, ii, jj + (j & 1))];
+    return 0.5 * (a + b);
 }
 
 // Right-hand side of one level at (i, j), 1-based: coarsified copy in the
@@ -666,43 +682,59 @@ __device__ __forceinline__ void pass1_r16(const EllJob& J, double2* smem, int bx
     const bool bcs = J.uc || J.bcring;
     // rhs_value along rows ra, rb = ra + 1 with the row-invariant parts
     // hoisted: the first half of the odd extension, e = t + T r (r < 8) at
-    // column j = e; the two edge rows of the level keep the general form
+    // column j = e.  The boundary terms are subtracted in rhs_value's order
+    // (bottom, top, left, right), the left and right ones on the level's edge
+    // rows with all their loads issued before any is used (a row pair whose
+    // loads wait one by one makes the slowest workgroup of the launch).
     auto load = [&](int q, double2* v) {
         const int ra = 2 * q, rb = ra + 1;
         const bool hb = rb < N1;
-        if ((ra == 0 || rb >= N1 - 1) && bcs) {
-#pragma unroll
-            for (int r = 0; r < 8; ++r) {
-                const int e = t + T * r;
-                v[r] = e == 0 ? make_double2(0.0, 0.0)
-                              : make_double2(rhs_value(J, ra + 1, e), hb ? rhs_value(J, rb + 1, e) : 0.0);
-            }
-            return;
-        }
         const bool cra = J.lev >= 1 && ra + 1 > ex && ra + 1 < J.nx / 2 + ex;
         const bool crb = hb && J.lev >= 1 && rb + 1 > ex && rb + 1 < J.nx / 2 + ex;
         const IBPM_GLOBAL double* fa = gin(J.f) + sidx(J.nx, J.ny, ra + 1, 1);
         const IBPM_GLOBAL double* fb = gin(J.f) + sidx(J.nx, J.ny, hb ? rb + 1 : ra + 1, 1);
         const IBPM_GLOBAL double* Ra = gin(J.R) + sidx(J.nx, J.ny, ra + 1, 1);
         const IBPM_GLOBAL double* Rb = gin(J.R) + sidx(J.nx, J.ny, hb ? rb + 1 : ra + 1, 1);
+        const double keep = hb ? 1.0 : 0.0;
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
             const int e = t + T * r, k = e - 1;
             const bool colc = e > jc0 && e < jc1;
-            const int kk = e == 0 ? 0 : k;   // e = 0 is the extension's zero
-            const double xa = ((cra && colc) ? Ra : fa)[kk] * sc;
-            const double xb = hb ? ((crb && colc) ? Rb : fb)[kk] * sc : 0.0;
-            v[r] = e == 0 ? make_double2(0.0, 0.0) : make_double2(xa, xb);
+            const int kk = e == 0 ? 0 : k;
+            // row b absent: read row a, times 0; e = 0, the extension's zero:
+            // times 0 (products, as in load_first_half, so that the sixteen
+            // loads are in flight together)
+            const double nz = e == 0 ? 0.0 : 1.0;
+            const double la = ((cra && colc) ? Ra : fa)[kk], lb = ((crb && colc) ? Rb : fb)[kk];
+            v[r] = make_double2(la * sc * nz, lb * sc * keep * nz);
         }
-        if (bcs) {   // getRHS at j = 1 (e = 1) and j = ny - 1 (e = M/2 - 1 = T - 1 + 7 T)
-            const double c = J.bc_coef;
-            if (t == 1) {
-                v[0].x -= bc_bottom(J, ra + 1) * c;
-                if (hb) v[0].y -= bc_bottom(J, rb + 1) * c;
+        if (!bcs) return;
+        const double c = J.bc_coef;
+        // getRHS at j = 1 (e = 1) and j = ny - 1 (e = M/2 - 1 = T - 1 + 7 T)
+        if (t == 1) {
+            v[0].x -= bc_bottom(J, ra + 1) * c;
+            if (hb) v[0].y -= bc_bottom(J, rb + 1) * c;
+        }
+        if (t == T - 1) {
+            v[7].x -= bc_top(J, ra + 1) * c;
+            if (hb) v[7].y -= bc_top(J, rb + 1) * c;
+        }
+        // i = 1 (row a of the first pair only) and i = nx - 1 (row a or b)
+        const bool left_a = ra + 1 == 1, right_a = ra + 1 == J.nx - 1, right_b = hb && rb + 1 == J.nx - 1;
+        if (left_a || right_a || right_b) {
+            double gl[8], gr[8];
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                const int j = max(t + T * r, 1);   // e = 0 holds the extension's zero and takes no term
+                gl[r] = left_a ? bc_side_loads(J, true, j) : 0.0;
+                gr[r] = (right_a || right_b) ? bc_side_loads(J, false, j) : 0.0;
             }
-            if (t == T - 1) {
-                v[7].x -= bc_top(J, ra + 1) * c;
-                if (hb) v[7].y -= bc_top(J, rb + 1) * c;
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                if (t + T * r == 0) continue;
+                if (left_a) v[r].x -= gl[r] * c;
+                if (right_a) v[r].x -= gr[r] * c;
+                if (right_b) v[r].y -= gr[r] * c;
             }
         }
     };

@@ -47,8 +47,9 @@ struct EllJob {
     const double* R;         // level slice of the coarsified rhs (central region), lev >= 1
     const double* uc;        // level slice lev+1 of the solution (BC source), lev < ngrid-1
     const double* bcring;    // explicit BC ring 2(nx+ny) instead of uc (single-level API)
-    double* T1;              // scratch (ny-1) x (nx-1)
-    double* T2;              // scratch (nx-1) x (ny-1)
+    double* T1;              // scratch (ny-1) rows of nx-1 values, row stride ld1
+    double* T2;              // scratch (nx-1) rows of ny-1 values, row stride ld2
+    int ld1, ld2;            // >= nx-1, ny-1; padded (t1_ld) where the scratch is the library's own
     double* u;               // level slice of the solution
     const double* W;         // epilogue 1 operand (level slice)
     double* vside;           // epilogue solves, levels >= 1: v (before the epilogue) stored at
@@ -84,6 +85,11 @@ __device__ inline bool skipped(const double* flags) {
 }
 
 __host__ __device__ inline int nx_ext(int nx) { return nx / 4; }   // Grid.h:89-91, zero shift
+
+// Row stride of the transposed scratch rows of n values: a multiple of 8
+// doubles, so the 64-byte pieces that the register-held pass 1 writes per
+// column start on 64-byte boundaries (one write request each, not two).
+inline int t1_ld(int n) { return (n + 7) & ~7; }
 
 __device__ inline long sidx(int nx, int ny, int i, int j) {   // within a level slice
     return (long)(i - 1) * (ny - 1) + (j - 1);

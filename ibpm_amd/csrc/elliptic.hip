@@ -303,8 +303,11 @@ template <int M>
 __host__ __device__ constexpr bool r16_len() { return M == 1024 || M == 2048; }
 constexpr long kRegMinPairs = 1024;
 
-template <int CM, bool REG>
-__host__ __device__ constexpr int pass_threads() { return REG ? CM / 16 : kFftThreads; }
+// Row pairs per workgroup of the register-held passes (see pass1_r16).
+__host__ __device__ constexpr int r16_group(int pass) { return pass == 1 ? 4 : pass == 2 ? 2 : 1; }
+
+template <int CM, bool REG, int PASS>
+__host__ __device__ constexpr int pass_threads() { return REG ? r16_group(PASS) * (CM / 16) : kFftThreads; }
 
 // LDS slot of element e: one pad slot after every 16, so the first stage's
 // writes (16 consecutive elements per lane) spread over the banks, and every
@@ -340,7 +343,7 @@ __device__ __forceinline__ void exchange16(double2* v, double* wr, const double*
 template <int M>
 __device__ __forceinline__ void mirror_second_half(double2* v, double2* buf) {
     constexpr int T = M / 16;
-    const int t = threadIdx.x;
+    const int t = threadIdx.x & (T - 1);
 #pragma unroll
     for (int r = 0; r < 8; ++r) buf[t + T * r] = v[r];
     __syncthreads();
@@ -367,7 +370,7 @@ template <int M>
 __device__ __forceinline__ void load_first_half(double2* v, const IBPM_GLOBAL double* a,
                                                 const IBPM_GLOBAL double* b, bool hb) {
     constexpr unsigned T = M / 16;
-    const unsigned t = threadIdx.x;
+    const unsigned t = threadIdx.x & (T - 1);
     const IBPM_GLOBAL double* bb = hb ? b : a;
     const double keep = hb ? 1.0 : 0.0;   // a product, not a select: see above
 #pragma unroll
@@ -389,7 +392,7 @@ template <int M, class Emit>
 __device__ __forceinline__ void fft_regs(double2* v, double2* buf, const IBPM_GLOBAL double* tw,
                                          const IBPM_GLOBAL double* twg, const Emit& emit) {
     constexpr int T = M / 16, R3 = M / 256, H = 256 / T, RS = 17 * T / 16;
-    const int t = threadIdx.x;
+    const int t = threadIdx.x & (T - 1);
     double* bd = (double*)buf;
     dft16(v);   // stage 1 (Ns = 1): outputs 16 t + r to slots 17 t + r; reads
                 // t + T r from slots pad16(t) + RS r
@@ -534,7 +537,7 @@ __device__ inline void pass1_lds(const EllJob& J, double2* smem, int bx) {
         const int row = r0 + r;
         if (row >= N1) continue;
         const double2 z = buf[lds_slot<kSwizzleAll<CM>>((r >> 1) * M + k + 1)];
-        gout(J.T1)[(long)k * N1 + row] = (r & 1) ? z.x : -z.y;
+        gout(J.T1)[(long)k * J.ld1 + row] = (r & 1) ? z.x : -z.y;
     }
 }
 
@@ -553,8 +556,8 @@ __device__ inline void pass2_lds(const EllJob& J, double2* smem, int bx) {
     for (int idx = threadIdx.x; idx < np * N1; idx += kFftThreads) {
         const int p = idx / N1, k = idx - p * N1;
         const int la = r0 + 2 * p, lb = la + 1;
-        const double va = (la < N2) ? gin(J.T1)[(long)la * N1 + k] : 0.0;
-        const double vb = (lb < N2) ? gin(J.T1)[(long)lb * N1 + k] : 0.0;
+        const double va = (la < N2) ? gin(J.T1)[(long)la * J.ld1 + k] : 0.0;
+        const double vb = (lb < N2) ? gin(J.T1)[(long)lb * J.ld1 + k] : 0.0;
         buf[lds_slot<kSwizzleAll<CM>>(p * M + k + 1)] = make_double2(va, vb);
         buf[lds_slot<kSwizzleAll<CM>>(p * M + M - 1 - k)] = make_double2(-va, -vb);
     }
@@ -593,7 +596,7 @@ __device__ inline void pass2_lds(const EllJob& J, double2* smem, int bx) {
         const int l = r0 + r;
         if (l >= N2) continue;
         const double2 z = buf[lds_slot<kSwizzleAll<CM>>((r >> 1) * M + k + 1)];
-        gout(J.T2)[(long)k * N2 + l] = (r & 1) ? z.x : -z.y;
+        gout(J.T2)[(long)k * J.ld2 + l] = (r & 1) ? z.x : -z.y;
     }
 }
 
@@ -612,8 +615,8 @@ __device__ inline void pass3_lds(const EllJob& J, double2* smem, int bx) {
     for (int idx = threadIdx.x; idx < np * N2; idx += kFftThreads) {
         const int p = idx / N2, k = idx - p * N2;
         const int ra = r0 + 2 * p, rb = ra + 1;
-        const double va = (ra < N1) ? gin(J.T2)[(long)ra * N2 + k] : 0.0;
-        const double vb = (rb < N1) ? gin(J.T2)[(long)rb * N2 + k] : 0.0;
+        const double va = (ra < N1) ? gin(J.T2)[(long)ra * J.ld2 + k] : 0.0;
+        const double vb = (rb < N1) ? gin(J.T2)[(long)rb * J.ld2 + k] : 0.0;
         buf[lds_slot<kSwizzleAll<CM>>(p * M + k + 1)] = make_double2(va, vb);
         buf[lds_slot<kSwizzleAll<CM>>(p * M + M - 1 - k)] = make_double2(-va, -vb);
     }
@@ -653,29 +656,61 @@ __device__ __forceinline__ void store_pair(IBPM_GLOBAL double* p, double a, doub
 }
 
 // The three passes for M = 1024, 2048 (N = M/2 - 1), the same arithmetic as the
-// LDS passes above around a register-held transform, one row pair q per
-// workgroup: load(q, v) fills the first half of the odd extension, work(q, v)
-// transforms and writes.  (Two pairs per workgroup, the second's loads issued
-// before the first's transform, measured slower: fewer, longer workgroups.)
-constexpr int kR16Pairs = 1;
+// LDS passes above around a register-held transform.  A workgroup holds G =
+// r16_group(pass) row pairs, one per sub-group of T = M/16 threads; the
+// sub-groups share the transform's barriers and each has its own LDS buffer.
+// Passes 1 and 2 write their outputs transposed, 2 G neighbouring values per
+// output column, so the outputs are staged in LDS (all buffers are free once
+// the last exchange is done) and written in pieces of 2 G values: with one
+// pair per workgroup each 16-byte piece was a write request of its own, and
+// the request rate, not the bytes, bounded those passes.  A sub-group past
+// the last pair transforms the last pair again and writes nothing, so every
+// thread reaches every barrier.  (Two pairs per workgroup processed one after
+// the other, the second's loads issued before the first's transform,
+// measured slower: fewer, longer workgroups.)
 
-template <class Load, class Work>
-__device__ __forceinline__ void r16_pairs(int bx, int npairs, const Load& load, const Work& work) {
-    (void)npairs;
-    double2 v[16];
-    load(bx, v);
-    work(bx, v);
+// Staged output (a, b) of sub-group g for column k: 16-byte slot k G + (g ^
+// sw(k)), the swizzle sw spreading one wave's writes (fixed g, consecutive k)
+// over the banks; write_staged reads the slots in order.
+template <int G>
+__device__ __forceinline__ int stage_slot(int k, int g) {
+    return k * G + (g ^ ((k / (16 / G)) & (G - 1)));
+}
+
+template <int G>
+__device__ __forceinline__ void stage_pair(double2* st, int k, int g, double a, double b) {
+    st[stage_slot<G>(k, g)] = make_double2(a, b);
+}
+
+// Rows row0 + 2 g + s (those < nrows) of the staged columns k < N to
+// out[k * ld + row], one 16-byte piece per thread, the G pieces of a column
+// from neighbouring threads (with ld a multiple of 2 G, one aligned piece of
+// 16 G bytes).
+template <int M, int G>
+__device__ __forceinline__ void write_staged(IBPM_GLOBAL double* out, const double2* st, int nrows, int ld,
+                                             int row0) {
+    constexpr int N = M / 2 - 1, NT = G * (M / 16);
+    for (int idx = threadIdx.x; idx < N * G; idx += NT) {
+        const int k = idx / G, slot = idx % G;
+        const int g = slot ^ ((k / (16 / G)) & (G - 1));
+        const int row = row0 + 2 * g;
+        if (row < nrows) {
+            const double2 z = st[idx];
+            store_pair(out + (long)k * ld + row, z.x, z.y, row + 1 < nrows);
+        }
+    }
 }
 
 template <int M>
 __device__ __forceinline__ void pass1_r16(const EllJob& J, double2* smem, int bx) {
-    constexpr int T = M / 16;
+    constexpr int T = M / 16, G = r16_group(1);
     const int N1 = J.nx - 1, npairs = (N1 + 1) / 2;
-    if (kR16Pairs * bx >= npairs) return;
-    double2* buf = smem;
+    if (G * bx >= npairs) return;
+    const int g = threadIdx.x / T, t = threadIdx.x & (T - 1);
+    const int q = min(G * bx + g, npairs - 1);
+    double2* buf = smem + g * r16_buf(M);
     const IBPM_GLOBAL double* tw = gin((const double*)J.pj.tw16);
     const IBPM_GLOBAL double* twg = gin((const double*)J.pj.tw);
-    const int t = threadIdx.x;
     const int ey = nx_ext(J.ny), ex = nx_ext(J.nx);
     const int jc0 = ey, jc1 = J.ny / 2 + ey;   // central columns: jc0 < j < jc1
     const double sc = J.in_scale;
@@ -738,105 +773,98 @@ __device__ __forceinline__ void pass1_r16(const EllJob& J, double2* smem, int bx
             }
         }
     };
-    auto work = [&](int q, double2* v) {
-        const int ra = 2 * q;
-        const bool hb = ra + 1 < N1;
-        mirror_second_half<M>(v, buf);
-        fft_regs<M>(v, buf, tw, twg, [&](int m, double2 z) {
-            const int kk = t + T * m;
-            if (kk > 0) store_pair(gout(J.T1) + (long)(kk - 1) * N1 + ra, -z.y, z.x, hb);
-        });
-    };
-    r16_pairs(bx, npairs, load, work);
+    double2 v[16];
+    load(q, v);
+    mirror_second_half<M>(v, buf);
+    fft_regs<M>(v, buf, tw, twg, [&](int m, double2 z) {
+        const int kk = t + T * m;
+        if (kk > 0) stage_pair<G>(smem, kk - 1, g, -z.y, z.x);
+    });
+    __syncthreads();
+    write_staged<M, G>(gout(J.T1), smem, N1, J.ld1, 2 * G * bx);
 }
 
 template <int M>
 __device__ __forceinline__ void pass2_r16(const EllJob& J, double2* smem, int bx) {
-    constexpr int T = M / 16;
+    constexpr int T = M / 16, G = r16_group(2);
     const int N1 = M / 2 - 1, N2 = J.ny - 1, npairs = (N2 + 1) / 2;
-    if (kR16Pairs * bx >= npairs) return;
-    double2* buf = smem;
+    if (G * bx >= npairs) return;
+    const int g = threadIdx.x / T, t = threadIdx.x & (T - 1);
+    const int q = min(G * bx + g, npairs - 1);
+    double2* buf = smem + g * r16_buf(M);
     const IBPM_GLOBAL double* tw = gin((const double*)J.pi.tw16);
     const IBPM_GLOBAL double* twg = gin((const double*)J.pi.tw);
-    const int t = threadIdx.x;
     const IBPM_GLOBAL double* gcx = gin(J.cosx);
-    auto load = [&](int q, double2* v) {
-        const int la = 2 * q, lb = la + 1;
-        load_first_half<M>(v, gin(J.T1) + (long)la * N1, gin(J.T1) + (long)lb * N1, lb < N2);
-    };
-    auto work = [&](int q, double2* v) {
-        const int la = 2 * q, lb = la + 1;
-        const bool hb = lb < N2;
-        mirror_second_half<M>(v, buf);
-        // spectral scaling, EllipticSolver2d.cc:42-54 (eigenvalues), :89-93 (scale),
-        // into the first half of the odd extension for the second transform
-        const double cya = gin(J.cosy)[la], cyb = hb ? gin(J.cosy)[lb] : 0.0;
-        double2 y[8];
-        fft_regs<M>(v, buf, tw, twg, [&](int m, double2 z) {
-            const int kk = t + T * m;
-            double ya = 0.0, yb = 0.0;
-            if (kk > 0) {
-                const double cx = gcx[kk - 1];
-                ya = -z.y;
-                yb = z.x;
-                const double lama = 2. * (cx + cya - 2.) * J.bydx2;
-                ya *= (J.kind == 0) ? 1. / lama : 1. / (1 + J.alpha * lama);
-                if (hb) {
-                    const double lamb = 2. * (cx + cyb - 2.) * J.bydx2;
-                    yb *= (J.kind == 0) ? 1. / lamb : 1. / (1 + J.alpha * lamb);
-                }
+    const int la = 2 * q, lb = la + 1;
+    const bool hb = lb < N2;
+    double2 v[16];
+    load_first_half<M>(v, gin(J.T1) + (long)la * J.ld1, gin(J.T1) + (long)lb * J.ld1, hb);
+    mirror_second_half<M>(v, buf);
+    // spectral scaling, EllipticSolver2d.cc:42-54 (eigenvalues), :89-93 (scale),
+    // into the first half of the odd extension for the second transform
+    const double cya = gin(J.cosy)[la], cyb = hb ? gin(J.cosy)[lb] : 0.0;
+    double2 y[8];
+    fft_regs<M>(v, buf, tw, twg, [&](int m, double2 z) {
+        const int kk = t + T * m;
+        double ya = 0.0, yb = 0.0;
+        if (kk > 0) {
+            const double cx = gcx[kk - 1];
+            ya = -z.y;
+            yb = z.x;
+            const double lama = 2. * (cx + cya - 2.) * J.bydx2;
+            ya *= (J.kind == 0) ? 1. / lama : 1. / (1 + J.alpha * lama);
+            if (hb) {
+                const double lamb = 2. * (cx + cyb - 2.) * J.bydx2;
+                yb *= (J.kind == 0) ? 1. / lamb : 1. / (1 + J.alpha * lamb);
             }
-            y[m] = make_double2(ya, yb);
-        });
+        }
+        y[m] = make_double2(ya, yb);
+    });
 #pragma unroll
-        for (int m = 0; m < 8; ++m) v[m] = y[m];
-        mirror_second_half<M>(v, buf);
-        fft_regs<M>(v, buf, tw, twg, [&](int m, double2 z) {
-            const int kk = t + T * m;
-            if (kk > 0) store_pair(gout(J.T2) + (long)(kk - 1) * N2 + la, -z.y, z.x, hb);
-        });
-    };
-    r16_pairs(bx, npairs, load, work);
+    for (int m = 0; m < 8; ++m) v[m] = y[m];
+    mirror_second_half<M>(v, buf);
+    fft_regs<M>(v, buf, tw, twg, [&](int m, double2 z) {
+        const int kk = t + T * m;
+        if (kk > 0) stage_pair<G>(smem, kk - 1, g, -z.y, z.x);
+    });
+    __syncthreads();
+    write_staged<M, G>(gout(J.T2), smem, N2, J.ld2, 2 * G * bx);
 }
 
 template <int M>
 __device__ __forceinline__ void pass3_r16(const EllJob& J, double2* smem, int bx) {
+    static_assert(r16_group(3) == 1, "pass 3 writes rows: one pair per workgroup");
     constexpr int T = M / 16;
     const int N1 = J.nx - 1, N2 = M / 2 - 1, npairs = (N1 + 1) / 2;
-    if (kR16Pairs * bx >= npairs) return;
+    if (bx >= npairs) return;
     double2* buf = smem;
     const IBPM_GLOBAL double* tw = gin((const double*)J.pj.tw16);
     const IBPM_GLOBAL double* twg = gin((const double*)J.pj.tw);
     const int t = threadIdx.x;
-    auto load = [&](int q, double2* v) {
-        const int ra = 2 * q, rb = ra + 1;
-        load_first_half<M>(v, gin(J.T2) + (long)ra * N2, gin(J.T2) + (long)rb * N2, rb < N1);
-    };
-    auto work = [&](int q, double2* v) {
-        const int ra = 2 * q, rb = ra + 1;
-        const bool hb = rb < N1;
-        mirror_second_half<M>(v, buf);
-        fft_regs<M>(v, buf, tw, twg, [&](int m, double2 z) {
-            const int kk = t + T * m;
-            if (kk == 0) return;
-            const int k = kk - 1;
-            // sinTransformInv normalisation, EllipticSolver2d.cc:80-83; with the
-            // epilogue the finer level's BCs come from v (EllipticSolver.cc:62-66)
-            {
-                const long o2 = (long)ra * N2 + k;
-                const double val_a = -z.y * J.norm;
-                gout(J.u)[o2] = J.epilogue ? gin(J.W)[o2] - val_a : val_a;
-                if (J.vside && on_bc_ring(J, ra + 1, k + 1)) gout(J.vside)[o2] = val_a;
-            }
-            if (hb) {
-                const long o2 = (long)rb * N2 + k;
-                const double val_b = z.x * J.norm;
-                gout(J.u)[o2] = J.epilogue ? gin(J.W)[o2] - val_b : val_b;
-                if (J.vside && on_bc_ring(J, rb + 1, k + 1)) gout(J.vside)[o2] = val_b;
-            }
-        });
-    };
-    r16_pairs(bx, npairs, load, work);
+    const int ra = 2 * bx, rb = ra + 1;
+    const bool hb = rb < N1;
+    double2 v[16];
+    load_first_half<M>(v, gin(J.T2) + (long)ra * J.ld2, gin(J.T2) + (long)rb * J.ld2, hb);
+    mirror_second_half<M>(v, buf);
+    fft_regs<M>(v, buf, tw, twg, [&](int m, double2 z) {
+        const int kk = t + T * m;
+        if (kk == 0) return;
+        const int k = kk - 1;
+        // sinTransformInv normalisation, EllipticSolver2d.cc:80-83; with the
+        // epilogue the finer level's BCs come from v (EllipticSolver.cc:62-66)
+        {
+            const long o2 = (long)ra * N2 + k;
+            const double val_a = -z.y * J.norm;
+            gout(J.u)[o2] = J.epilogue ? gin(J.W)[o2] - val_a : val_a;
+            if (J.vside && on_bc_ring(J, ra + 1, k + 1)) gout(J.vside)[o2] = val_a;
+        }
+        if (hb) {
+            const long o2 = (long)rb * N2 + k;
+            const double val_b = z.x * J.norm;
+            gout(J.u)[o2] = J.epilogue ? gin(J.W)[o2] - val_b : val_b;
+            if (J.vside && on_bc_ring(J, rb + 1, k + 1)) gout(J.vside)[o2] = val_b;
+        }
+    });
 }
 
 // Logical (row block, job) of this workgroup.  Workgroups are handed to the
@@ -871,9 +899,9 @@ IBPM_PASS_BODY(3)
 // (M applications) use k_fs_pass*, the same code under another name so that
 // profiles separate the two.  A job whose flow has finished its force solve
 // does nothing (skipped()).
-#define IBPM_PASS_KERNEL(NAME, BODY)                                             \
+#define IBPM_PASS_KERNEL(NAME, BODY, PASS)                                       \
     template <int CM, bool REG>                                                  \
-    __global__ void __launch_bounds__((pass_threads<CM, REG>())) NAME(const EllJob* __restrict__ jobs) { \
+    __global__ void __launch_bounds__((pass_threads<CM, REG, PASS>())) NAME(const EllJob* __restrict__ jobs) { \
         static_assert(!REG || r16_len<CM>(), "no register-held transform at this length"); \
         extern __shared__ double2 smem[];                                         \
         const int2 blk = xcd_block();                                             \
@@ -881,12 +909,12 @@ IBPM_PASS_BODY(3)
         if (skipped(J.skip)) return;                                              \
         BODY<CM, REG>(J, smem, blk.x);                                            \
     }
-IBPM_PASS_KERNEL(k_ell_pass1, pass1_body)
-IBPM_PASS_KERNEL(k_ell_pass2, pass2_body)
-IBPM_PASS_KERNEL(k_ell_pass3, pass3_body)
-IBPM_PASS_KERNEL(k_fs_pass1, pass1_body)
-IBPM_PASS_KERNEL(k_fs_pass2, pass2_body)
-IBPM_PASS_KERNEL(k_fs_pass3, pass3_body)
+IBPM_PASS_KERNEL(k_ell_pass1, pass1_body, 1)
+IBPM_PASS_KERNEL(k_ell_pass2, pass2_body, 2)
+IBPM_PASS_KERNEL(k_ell_pass3, pass3_body, 3)
+IBPM_PASS_KERNEL(k_fs_pass1, pass1_body, 1)
+IBPM_PASS_KERNEL(k_fs_pass2, pass2_body, 2)
+IBPM_PASS_KERNEL(k_fs_pass3, pass3_body, 3)
 
 // --------------------------------------------------------------- coarsify
 // Scalar::coarsify (reference Scalar.cc:46-63) for one level, writing only
@@ -963,7 +991,7 @@ bool pass_in_registers(int M, long row_pairs) {
 
 int lds_for_pass(const EllJob& J, int pass, bool regs) {
     const FftPlan& P = (pass == 2) ? J.pi : J.pj;
-    if (regs) return (int)sizeof(double2) * r16_buf(P.M);   // 8.5 or 17 KB
+    if (regs) return (int)sizeof(double2) * r16_group(pass) * r16_buf(P.M);   // 8.5 or 17 KB per pair
     const int ntw = (compiled_len(P.M) && quarter_twiddles(P.M)) ? P.M / 4 : P.M;
     return (int)sizeof(double2) * (P.np * P.M + ntw);
 }
@@ -971,21 +999,21 @@ int lds_for_pass(const EllJob& J, int pass, bool regs) {
 int blocks_for_pass(const EllJob& J, int pass, bool regs) {
     const int rows_total = (pass == 2) ? J.ny - 1 : J.nx - 1;
     const FftPlan& P = (pass == 2) ? J.pi : J.pj;
-    const int rows = regs ? 2 * kR16Pairs : 2 * P.np;
+    const int rows = regs ? 2 * r16_group(pass) : 2 * P.np;
     return (rows_total + rows - 1) / rows;
 }
 
-#define IBPM_LAUNCH_LEN(KER)                                                                   \
+#define IBPM_LAUNCH_LEN(KER, PASS)                                                             \
     switch (M) {                                                                               \
     case 400: hipLaunchKernelGGL((KER<400, false>), grid, dim3(kFftThreads), lds_bytes, s, d_jobs); break;  \
     case 512: hipLaunchKernelGGL((KER<512, false>), grid, dim3(kFftThreads), lds_bytes, s, d_jobs); break;  \
     case 768: hipLaunchKernelGGL((KER<768, false>), grid, dim3(kFftThreads), lds_bytes, s, d_jobs); break;  \
     case 800: hipLaunchKernelGGL((KER<800, false>), grid, dim3(kFftThreads), lds_bytes, s, d_jobs); break;  \
     case 1024:                                                                                 \
-        if (regs) hipLaunchKernelGGL((KER<1024, true>), grid, dim3(1024 / 16), lds_bytes, s, d_jobs);    \
+        if (regs) hipLaunchKernelGGL((KER<1024, true>), grid, dim3(r16_group(PASS) * 1024 / 16), lds_bytes, s, d_jobs); \
         else hipLaunchKernelGGL((KER<1024, false>), grid, dim3(kFftThreads), lds_bytes, s, d_jobs);     \
         break;                                                                                 \
-    case 2048: hipLaunchKernelGGL((KER<2048, true>), grid, dim3(2048 / 16), lds_bytes, s, d_jobs); break;  \
+    case 2048: hipLaunchKernelGGL((KER<2048, true>), grid, dim3(r16_group(PASS) * 2048 / 16), lds_bytes, s, d_jobs); break; \
     default: hipLaunchKernelGGL((KER<0, false>), grid, dim3(kFftThreads), lds_bytes, s, d_jobs); break;      \
     }
 
@@ -999,11 +1027,11 @@ void launch_ell_pass(int pass, const EllJob* d_jobs, int njobs, int max_blocks, 
                   "the launch switch and compiled_len must list the same lengths");
     dim3 grid(max_blocks, njobs);
     if (pass == 1) {
-        if (force_solve) { IBPM_LAUNCH_LEN(k_fs_pass1) } else { IBPM_LAUNCH_LEN(k_ell_pass1) }
+        if (force_solve) { IBPM_LAUNCH_LEN(k_fs_pass1, 1) } else { IBPM_LAUNCH_LEN(k_ell_pass1, 1) }
     } else if (pass == 2) {
-        if (force_solve) { IBPM_LAUNCH_LEN(k_fs_pass2) } else { IBPM_LAUNCH_LEN(k_ell_pass2) }
+        if (force_solve) { IBPM_LAUNCH_LEN(k_fs_pass2, 2) } else { IBPM_LAUNCH_LEN(k_ell_pass2, 2) }
     } else {
-        if (force_solve) { IBPM_LAUNCH_LEN(k_fs_pass3) } else { IBPM_LAUNCH_LEN(k_ell_pass3) }
+        if (force_solve) { IBPM_LAUNCH_LEN(k_fs_pass3, 3) } else { IBPM_LAUNCH_LEN(k_ell_pass3, 3) }
     }
 }
 

@@ -269,6 +269,7 @@ static int build_solve_plan(Ctx* c, DevMem& mem, const std::vector<SolveSpec>& s
             J.vside = (s.vside && lev >= 1) ? s.vside + lev * ns : nullptr;
             J.bcring = (s.L == 1) ? s.bcring : nullptr;
             J.T1 = s.T1; J.T2 = s.T2;
+            J.ld1 = t1_ld(s.nx - 1); J.ld2 = t1_ld(s.ny - 1);   // the scratch is sized for the padding
             J.u = s.u + lev * ns;
             J.W = s.W ? s.W + lev * ns : nullptr;
             J.cosx = s.cosx; J.cosy = s.cosy;
@@ -655,8 +656,8 @@ static int alloc_flow(ibpm_batch* B, Flow& F) {
     CKR(m.alloc(&F.omega, ns)); CKR(m.alloc(&F.q, nf)); CKR(m.alloc(&F.nprev, ns));
     CKR(m.alloc(&F.a, ns)); CKR(m.alloc(&F.wstar, ns)); CKR(m.alloc(&F.psi, ns));
     CKR(m.alloc(&F.c, ns)); CKR(m.alloc(&F.U, ns)); CKR(m.alloc(&F.V, ns));
-    CKR(m.alloc(&F.cross, nf)); CKR(m.alloc(&F.R, ns)); CKR(m.alloc(&F.T1, F.ns));
-    CKR(m.alloc(&F.T2, F.ns)); CKR(m.alloc(&F.mt, ns));
+    CKR(m.alloc(&F.cross, nf)); CKR(m.alloc(&F.R, ns)); CKR(m.alloc(&F.T1, (size_t)(F.ny - 1) * t1_ld(F.nx - 1)));
+    CKR(m.alloc(&F.T2, (size_t)(F.nx - 1) * t1_ld(F.ny - 1))); CKR(m.alloc(&F.mt, ns));
     CKR(cos_tables(m, F.nx, &F.cosx)); CKR(cos_tables(m, F.ny, &F.cosy));
     const int n = 2 * F.nb + 1;
     CKR(m.alloc(&F.f, n)); CKR(m.alloc(&F.rhs, n)); CKR(m.alloc(&F.b, n));
@@ -1484,12 +1485,14 @@ int ibpm_dst2d(ibpm_ctx* c, int n1, int n2, const double* d_in, double* d_out) {
     EllJob J{};
     J.nx = n1 + 1; J.ny = n2 + 1; J.lev = 0; J.ngrid = 1; J.in_scale = 1.0;
     J.f = d_in; J.T1 = T; J.T2 = T; J.u = d_out; J.norm = 1.0;
+    J.ld1 = n1; J.ld2 = n2;   // unpadded: the second launch writes the caller's array
     CKR(get_plan(&c->c, n2, &J.pj));
     CKR(get_plan(&c->c, n1, &J.pi));
     // pass 1: rows along j -> T (transposed); a second pass-1 style launch over
     // the transposed rows transforms along i and transposes back.
     EllJob J2 = J;
     J2.nx = n2 + 1; J2.ny = n1 + 1; J2.f = T; J2.T1 = d_out; J2.pj = J.pi;
+    J2.ld1 = n2; J2.ld2 = n1;
     EllJob* d = nullptr;
     CKR(mem.alloc(&d, 2));
     CK(hipMemcpyAsync(d, &J, sizeof(EllJob), hipMemcpyHostToDevice, c->c.stream));
@@ -1510,8 +1513,8 @@ static int solve_common(ibpm_ctx* c, int kind, int nx, int ny, int ngrid, double
     const long ns = (long)(nx - 1) * (ny - 1);
     double *R = nullptr, *T1 = nullptr, *T2 = nullptr, *cosx = nullptr, *cosy = nullptr, *fin = nullptr;
     CKR(mem.alloc(&R, ns * ngrid));
-    CKR(mem.alloc(&T1, ns));
-    CKR(mem.alloc(&T2, ns));
+    CKR(mem.alloc(&T1, (size_t)(ny - 1) * t1_ld(nx - 1)));
+    CKR(mem.alloc(&T2, (size_t)(nx - 1) * t1_ld(ny - 1)));
     CKR(mem.alloc(&fin, ns * ngrid));
     CK(hipMemcpyAsync(fin, d_f, sizeof(double) * ns * ngrid, hipMemcpyDeviceToDevice, c->c.stream));
     CKR(cos_tables(mem, nx, &cosx));

@@ -303,8 +303,20 @@ template <int M>
 __host__ __device__ constexpr bool r16_len() { return M == 1024 || M == 2048; }
 constexpr long kRegMinPairs = 1024;
 
-// Row pairs per workgroup of the register-held passes (see pass1_r16).
-__host__ __device__ constexpr int r16_group(int pass) { return pass == 1 ? 4 : pass == 2 ? 2 : 1; }
+// Row pairs per workgroup of the register-held passes 1 and 2 (see
+// pass1_r16; powers of two, at most 8); pass 3 writes rows and takes one.
+#ifndef IBPM_R16_GROUP1
+#define IBPM_R16_GROUP1 4
+#endif
+#ifndef IBPM_R16_GROUP2
+#define IBPM_R16_GROUP2 2
+#endif
+__host__ __device__ constexpr int r16_group(int pass) {
+    return pass == 1 ? IBPM_R16_GROUP1 : pass == 2 ? IBPM_R16_GROUP2 : 1;
+}
+static_assert((IBPM_R16_GROUP1 & (IBPM_R16_GROUP1 - 1)) == 0 && IBPM_R16_GROUP1 <= 8 &&
+                  (IBPM_R16_GROUP2 & (IBPM_R16_GROUP2 - 1)) == 0 && IBPM_R16_GROUP2 <= 8,
+              "row pairs per workgroup: a power of two, at most 8");
 
 template <int CM, bool REG, int PASS>
 __host__ __device__ constexpr int pass_threads() { return REG ? r16_group(PASS) * (CM / 16) : kFftThreads; }
@@ -347,6 +359,29 @@ __device__ __forceinline__ void mirror_second_half(double2* v, double2* buf) {
 #pragma unroll
     for (int r = 0; r < 8; ++r) buf[t + T * r] = v[r];
     __syncthreads();
+#pragma unroll
+    for (int r = 8; r < 16; ++r) {
+        const int e = t + T * r;
+        if (e == M / 2) {
+            v[r] = make_double2(0.0, 0.0);
+        } else {
+            const double2 y = buf[M - e];
+            v[r] = make_double2(-y.x, -y.y);
+        }
+    }
+    __syncthreads();
+}
+
+// The whole odd extension from its first half z[e], e < M/2, left in buf by
+// the caller: v[r] = z[t + T r], with z[e] = -z[M - e] for e > M/2 and
+// z[M/2] = 0.  The buffer is free again on return.
+template <int M>
+__device__ __forceinline__ void extension_from_lds(double2* v, double2* buf) {
+    constexpr int T = M / 16;
+    const int t = threadIdx.x & (T - 1);
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = buf[t + T * r];
 #pragma unroll
     for (int r = 8; r < 16; ++r) {
         const int e = t + T * r;
@@ -801,9 +836,10 @@ __device__ __forceinline__ void pass2_r16(const EllJob& J, double2* smem, int bx
     load_first_half<M>(v, gin(J.T1) + (long)la * J.ld1, gin(J.T1) + (long)lb * J.ld1, hb);
     mirror_second_half<M>(v, buf);
     // spectral scaling, EllipticSolver2d.cc:42-54 (eigenvalues), :89-93 (scale),
-    // into the first half of the odd extension for the second transform
+    // into the first half of the odd extension for the second transform,
+    // kept in the (free) LDS buffer rather than in registers beside the
+    // transform's own
     const double cya = gin(J.cosy)[la], cyb = hb ? gin(J.cosy)[lb] : 0.0;
-    double2 y[8];
     fft_regs<M>(v, buf, tw, twg, [&](int m, double2 z) {
         const int kk = t + T * m;
         double ya = 0.0, yb = 0.0;
@@ -818,11 +854,9 @@ __device__ __forceinline__ void pass2_r16(const EllJob& J, double2* smem, int bx
                 yb *= (J.kind == 0) ? 1. / lamb : 1. / (1 + J.alpha * lamb);
             }
         }
-        y[m] = make_double2(ya, yb);
+        buf[kk] = make_double2(ya, yb);
     });
-#pragma unroll
-    for (int m = 0; m < 8; ++m) v[m] = y[m];
-    mirror_second_half<M>(v, buf);
+    extension_from_lds<M>(v, buf);
     fft_regs<M>(v, buf, tw, twg, [&](int m, double2 z) {
         const int kk = t + T * m;
         if (kk > 0) stage_pair<G>(smem, kk - 1, g, -z.y, z.x);

@@ -120,12 +120,13 @@ def timed_steps(b, nflows, steps, moving):
 
 
 def plain_form(ctx, flow, total_steps, steps):
-    """The plain GPU form on one flow: plain CG from the previous force (the
-    reference's solver), no batching; advances total_steps and times the
-    last `steps`.  Returns throughput, iterations per step and final Cd, Cl."""
+    """The plain GPU form on one flow: plain CG from the previous force with
+    every A^-1 of M by the DST solve (the reference's solver), no batching;
+    advances total_steps and times the last `steps`.  Returns throughput,
+    iterations per step and final Cd, Cl."""
     import ibpm_amd
     from ibpm_amd.workloads import grid_points
-    b = ibpm_amd.Batch(ctx, [flow]).init()
+    b = ibpm_amd.Batch(ctx, [flow]).init().set_local_helmholtz(False)
     b.advance(total_steps - steps)
     sec, its = timed_steps(b, 1, steps, True)
     cd, cl = b.forces(0)
@@ -211,6 +212,8 @@ def main():
                   "builds_in_timed_steps": d["builds"], "fallbacks": d["fallbacks"],
                   "host_ms_per_step_kinematics": round(d["host_us_kinematics"] / steps / 1e3, 3),
                   "host_ms_per_step_polls": round(d["host_us_polls"] / steps / 1e3, 3),
+                  "m_applications_with_local_helmholtz": d["local_helmholtz_m_applications"] / max(
+                      1, d["m_applications"] + d["build_m_applications"]),
                   "warmup_s_including_first_build": round(t_warm, 3)}
     # live per-kernel timing (HIP events on the library stream, eager steps)
     stats = b.kernel_stats(2 if moving else 3)

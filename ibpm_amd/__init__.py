@@ -86,6 +86,7 @@ def load():
         "ibpm_batch_nn_params": [vp, vp, vp],
         "ibpm_nn_loss_grad": [vp, vp, vp, vp, ctypes.c_long, vp, _D],
         "ibpm_batch_set_precond": [vp, ctypes.c_int, ctypes.c_int],
+        "ibpm_batch_set_local_helmholtz": [vp, ctypes.c_int],
         "ibpm_batch_cholesky_cache": [vp, ctypes.c_int, ctypes.c_char_p],
         "ibpm_batch_cholesky_loaded": [vp, ctypes.c_int],
         "ibpm_rhs": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
@@ -403,8 +404,15 @@ class Batch:
         _check(load().ibpm_batch_set_precond(self.h, m, k))
         return self
 
+    def set_local_helmholtz(self, on=True):
+        """Inside the moving-body force solve, A^-1 by Jacobi sweeps over a box
+        around the body on each level where that equals the DST solve to
+        rounding (the default), or always by the DST solve (on=False)."""
+        _check(load().ibpm_batch_set_local_helmholtz(self.h, 1 if on else 0))
+        return self
+
     SOLVER_STATS = ("iterations", "solves", "m_applications", "build_m_applications", "builds",
-                    "fallbacks", "abandoned", "host_us_kinematics", "host_us_polls")
+                    "fallbacks", "abandoned", "host_us_kinematics", "host_us_polls", "local_helmholtz_m_applications")
 
     def solver_stats(self):
         out = np.zeros(len(self.SOLVER_STATS), dtype=np.int64)

@@ -61,6 +61,21 @@ struct EllJob {
     const double* skip;      // null, or the flow's solver flags (see skipped())
 };
 
+// Local solve of A u = r, A = 1 + alpha Lap (the force solve's A^-1 of one
+// level of one flow) by Jacobi sweeps over a box: u <- (r - c S u) / (1 - 4 c),
+// c = alpha / dx^2, S the sum of the four neighbours, u = 0 off the box.
+struct HelmJob {
+    int nx, ny, lev;
+    int i0, i1, j0, j1;      // box (1-based, inclusive), inside 2..nx-2 x 2..ny-2
+    double c, inv_diag;      // alpha / dx_lev^2 and 1 / (1 - 4 c)
+    double in_scale;
+    const double* f;         // level slice of the right-hand side
+    const double* R;         // level slice of the coarsified right-hand side (central region, lev >= 1)
+    double* out;             // level slice written by the last sweep
+    double* tmp;             // level slice of the other sweep buffer
+    const double* skip;      // null, or the flow's solver flags
+};
+
 // Coarsify job: one level (lev >= 1) of one flow (reference Scalar.cc:46-63).
 struct CoarsifyJob {
     int nx, ny, lev;

@@ -822,7 +822,7 @@ __device__ __forceinline__ void pass1_r16(const EllJob& J, double2* smem, int bx
 template <int M>
 __device__ __forceinline__ void pass2_r16(const EllJob& J, double2* smem, int bx) {
     constexpr int T = M / 16, G = r16_group(2);
-    const int N1 = M / 2 - 1, N2 = J.ny - 1, npairs = (N2 + 1) / 2;
+    const int N2 = J.ny - 1, npairs = (N2 + 1) / 2;
     if (G * bx >= npairs) return;
     const int g = threadIdx.x / T, t = threadIdx.x & (T - 1);
     const int q = min(G * bx + g, npairs - 1);
@@ -1018,6 +1018,42 @@ __global__ void __launch_bounds__(kThreads) k_fs_coarsify(const CoarsifyJob* __r
     coarsify_body(jobs[blockIdx.y]);
 }
 
+// ------------------------------------------------- local Helmholtz sweeps
+// One Jacobi sweep of A u = r, A = 1 + alpha Lap, over a job's box, for the
+// force solve's A^-1.  With |alpha| / dx^2 small the solution decays by a
+// factor 4|c| / (1 - 4c) per cell away from the right-hand side, so a box
+// around the body's smeared force holds it to rounding and the host chooses
+// the sweep count for an error below 1e-18 of the solution; it uses the DST
+// solve wherever the box would reach a level's boundary ring.  Sweep 0 starts
+// from u = 0; sweeps alternate between tmp and out so that the last writes
+// out.  Points off the box are never written in either buffer (the boxes of
+// a flow only grow), so they read as 0.  The right-hand side is rhs_value's
+// without boundary terms, which the box keeps away from.
+__global__ void __launch_bounds__(kThreads) k_fs_helm(const HelmJob* __restrict__ jobs, int sweep, int nsweeps) {
+    const HelmJob& J = jobs[blockIdx.y];
+    if (skipped(J.skip)) return;
+    const int w = J.j1 - J.j0 + 1, h = J.i1 - J.i0 + 1;
+    const int p = blockIdx.x * kThreads + threadIdx.x;
+    if (p >= w * h) return;
+    const int i = J.i0 + p / w, j = J.j0 + p % w;
+    const int ex = nx_ext(J.nx), ey = nx_ext(J.ny);
+    const bool central = J.lev >= 1 && i > ex && i < J.nx / 2 + ex && j > ey && j < J.ny / 2 + ey;
+    const long k = sidx(J.nx, J.ny, i, j);
+    const double r = (central ? gin(J.R)[k] : gin(J.f)[k]) * J.in_scale;
+    const bool to_out = ((nsweeps - 1 - sweep) & 1) == 0;
+    IBPM_GLOBAL double* dst = gout(to_out ? J.out : J.tmp);
+    double v;
+    if (sweep == 0) {
+        v = r * J.inv_diag;
+    } else {
+        const IBPM_GLOBAL double* src = gin(to_out ? J.tmp : J.out);
+        const long ny1 = J.ny - 1;
+        const double s4 = (src[k - ny1] + src[k + ny1]) + (src[k - 1] + src[k + 1]);
+        v = (r - J.c * s4) * J.inv_diag;
+    }
+    dst[k] = v;
+}
+
 // ------------------------------------------------------------ host launch
 bool pass_in_registers(int M, long row_pairs) {
     return M == 2048 || (M == 1024 && row_pairs >= kRegMinPairs);
@@ -1067,6 +1103,10 @@ void launch_ell_pass(int pass, const EllJob* d_jobs, int njobs, int max_blocks, 
     } else {
         if (force_solve) { IBPM_LAUNCH_LEN(k_fs_pass3, 3) } else { IBPM_LAUNCH_LEN(k_ell_pass3, 3) }
     }
+}
+
+void launch_helm_sweep(const HelmJob* d_jobs, int njobs, int max_blocks, int sweep, int nsweeps, hipStream_t s) {
+    hipLaunchKernelGGL(k_fs_helm, dim3(max_blocks, njobs), dim3(kThreads), 0, s, d_jobs, sweep, nsweeps);
 }
 
 void launch_coarsify(const CoarsifyJob* d_jobs, int njobs, int max_rows, hipStream_t s, bool force_solve) {

@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <climits>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -158,11 +160,11 @@ struct SolveSpec {
 // solve (kernels k_fs_*), counted with the bytes of the flows still iterating.
 enum KClass { K_PASS1, K_PASS2, K_PASS3, K_COARSIFY, K_VEL, K_CROSS, K_RHS, K_CURLQ, K_INTERP,
               K_SMEAR, K_MINV, K_CG, K_FORCES, K_CLEAR, K_FSPASS1, K_FSPASS2, K_FSPASS3, K_FSCOARSIFY,
-              K_NCLASS };
+              K_FSHELM, K_NCLASS };
 static const char* kclass_names[K_NCLASS] = {"ell_pass1", "ell_pass2", "ell_pass3", "coarsify", "vel",
                                              "cross", "rhs", "curl_q", "interp", "smear_curl", "minv",
                                              "cg", "forces", "clear_nodes", "fs_pass1", "fs_pass2",
-                                             "fs_pass3", "fs_coarsify"};
+                                             "fs_pass3", "fs_coarsify", "fs_helm"};
 struct Probe {
     struct Rec { int cls; hipEvent_t a, b; double bytes; };
     std::vector<Rec> recs;
@@ -309,6 +311,23 @@ static int build_solve_plan(Ctx* c, DevMem& mem, const std::vector<SolveSpec>& s
     return 0;
 }
 
+// The fine-to-coarse chain of a solve's right-hand side (central regions of
+// levels >= 1 into R).  active (probe only): flows whose jobs do work; the
+// others are skipped by their flags, so their bytes are not counted.
+static void run_coarsify(const SolvePlan& P, hipStream_t s, Probe* pr, const std::vector<char>* active) {
+    auto sum = [&](const std::vector<int>& fl, const std::vector<double>& by, double all) {
+        if (!active) return all;
+        double b = 0;
+        for (size_t j = 0; j < fl.size(); ++j)
+            if ((*active)[fl[j]]) b += by[j];
+        return b;
+    };
+    const bool fs = P.force_solve;
+    for (int lev = 1; lev < P.nlev; ++lev)
+        timed(pr, fs ? K_FSCOARSIFY : K_COARSIFY, pr ? sum(P.cflow[lev], P.cjbytes[lev], P.cbytes[lev]) : 0, s,
+              [&] { launch_coarsify(P.cj[lev], P.cj_n[lev], P.cj_rows[lev], s, fs); });
+}
+
 // active (probe only): flows whose jobs do work; the others are skipped by
 // their flags, so their bytes are not counted.
 static void run_solve(const SolvePlan& P, hipStream_t s, Probe* pr = nullptr,
@@ -321,9 +340,7 @@ static void run_solve(const SolvePlan& P, hipStream_t s, Probe* pr = nullptr,
         return b;
     };
     const bool fs = P.force_solve;
-    for (int lev = 1; lev < P.nlev; ++lev)
-        timed(pr, fs ? K_FSCOARSIFY : K_COARSIFY, pr ? sum(P.cflow[lev], P.cjbytes[lev], P.cbytes[lev]) : 0, s,
-              [&] { launch_coarsify(P.cj[lev], P.cj_n[lev], P.cj_rows[lev], s, fs); });
+    run_coarsify(P, s, pr, active);
     for (int st = 0; st < P.nlev; ++st)
         for (const PassGroup& G : P.steps[st]) {
             const double by = pr ? sum(G.jflow, G.jbytes, G.bytes) : 0;
@@ -375,6 +392,10 @@ struct Flow {
     std::vector<int> reg_old;      // node list of the previous association (cleared on the device)
     int reg_old_n = 0;
     HostStage stage;
+    // local Helmholtz path of the force solve's A^-1: result and sweep buffer
+    // (all levels) and, per level, the union of the boxes swept so far
+    double *m1 = nullptr, *m1tmp = nullptr;
+    std::vector<std::array<int, 4>> hbox;
     // host state
     std::vector<double> px, py, pu, pv;
     int hist_cap = 0;
@@ -411,6 +432,15 @@ struct ibpm_batch {
     int max_comp = 0, max_nodes = 0, max_n = 0;
     // plans per substep
     ibpmg::SolvePlan S1[4], S2[4], S3[4], S4, M1[4], M2;
+    // local Helmholtz path (plan_local_helm): on per substep, its jobs, sweeps
+    // and the DST solve of L^-1 that reads its result
+    bool helm_allowed = true, helm_on[4] = {false, false, false, false}, m2l_built = false;
+    ibpmg::SolvePlan M2L;
+    ibpmg::HelmJob* d_helm = nullptr;
+    int helm_cap = 0, helm_n = 0, helm_blocks = 0, helm_sweeps = 0;
+    std::vector<int> helm_flow;
+    std::vector<double> helm_bytes;
+    long st_local_mapps = 0;
     ibpmg::FlowOps* d_fops[4] = {};
     ibpmg::RegOps *d_interp_c = nullptr, *d_interp_m = nullptr, *d_smear_f[4] = {}, *d_smear_d[4] = {};
     ibpmg::RegOps* d_clear = nullptr;
@@ -781,14 +811,137 @@ static void mark_nprev_valid(ibpm_batch* B) {
     }
 }
 
+// The local Helmholtz path of substep i's M applications (k_fs_helm).  The
+// right-hand side of A^-1 inside M is beta B f: nonzero only at the
+// regulariser's nodes on level 0, and on coarser levels where the coarsify
+// chain carries it (coarse point ic sees fine rows 2 (ic - ex) - 1 .. + 1).
+// Per flow and level the box is that support grown by the sweep count S,
+// chosen so that rho^S <= 1e-18, rho = 4|c| / |1 - 4c| the Jacobi
+// contraction.  The path is off for the substep (the DST solve runs) when any
+// box would reach a level's first or last interior row or column, where
+// getBC's values enter, or a coarser level's ring, where the finer level
+// reads them; the solution there is then below 1e-18 of its maximum.
+constexpr int kHelmMaxSweeps = 40;
+
+static int plan_local_helm(ibpm_batch* B, int i) {
+    B->helm_on[i] = false;
+    if (!B->helm_allowed) return 0;
+    const int nfl = (int)B->flows.size();
+    std::vector<std::vector<std::array<int, 4>>> nbox(nfl);
+    int sweeps = 1;
+    for (int k = 0; k < nfl; ++k) {
+        Flow& F = B->flows[k];
+        const std::vector<int>& nodes = F.reg.node_idx;
+        if (nodes.empty()) return 0;
+        const int ny1 = F.ny - 1;
+        int s0 = INT_MAX, s1 = INT_MIN, t0 = INT_MAX, t1 = INT_MIN;   // support: rows i, columns j (1-based)
+        for (int idx : nodes) {
+            const int ii = idx / ny1 + 1, jj = idx % ny1 + 1;
+            s0 = std::min(s0, ii); s1 = std::max(s1, ii);
+            t0 = std::min(t0, jj); t1 = std::max(t1, jj);
+        }
+        const double alpha = -beta_of(B, F, i) / 2. * (1. / F.p.reynolds);   // as the M1 solve's
+        const int ex = nx_ext(F.nx), ey = nx_ext(F.ny);
+        nbox[k].resize(F.L);
+        for (int lev = 0; lev < F.L; ++lev) {
+            if (lev >= 1) {   // support after coarsify, inside the central region
+                s0 = std::max((s0 - 1 + 1) / 2 + ex, ex + 1);   // ceil((s0 - 1) / 2), s0 >= 1
+                s1 = std::min((s1 + 1) / 2 + ex, ex + F.nx / 2 - 1);
+                t0 = std::max((t0 - 1 + 1) / 2 + ey, ey + 1);
+                t1 = std::min((t1 + 1) / 2 + ey, ey + F.ny / 2 - 1);
+                if (s0 > s1 || t0 > t1) return 0;
+            }
+            const double dxl = F.dx * (1 << lev), c = alpha / (dxl * dxl);
+            const double rho = std::fabs(4 * c) / std::fabs(1 - 4 * c);
+            if (!(rho < 0.5)) return 0;
+            const int S = rho < 1e-300 ? 1 : (int)std::ceil(std::log(1e-18) / std::log(rho));
+            if (S > kHelmMaxSweeps) return 0;
+            std::array<int, 4> b = {s0 - S, s1 + S, t0 - S, t1 + S};
+            if (b[0] < 2 || b[1] > F.nx - 2 || b[2] < 2 || b[3] > F.ny - 2) return 0;
+            if (lev >= 1 && !(b[0] > ex && b[1] < ex + F.nx / 2 && b[2] > ey && b[3] < ey + F.ny / 2)) return 0;
+            if (lev < (int)F.hbox.size()) {   // the boxes of a flow only grow
+                const std::array<int, 4>& o = F.hbox[lev];
+                b = {std::min(b[0], o[0]), std::max(b[1], o[1]), std::min(b[2], o[2]), std::max(b[3], o[3])};
+            }
+            nbox[k][lev] = b;
+            sweeps = std::max(sweeps, S);
+        }
+    }
+    std::vector<HelmJob> jobs;
+    std::vector<int> jflow;
+    std::vector<double> jbytes;
+    long maxarea = 0;
+    for (int k = 0; k < nfl; ++k) {
+        Flow& F = B->flows[k];
+        const long ns = F.ns * F.L;
+        if (!F.m1) {
+            CKR(B->mem.alloc(&F.m1, ns));
+            CKR(B->mem.alloc(&F.m1tmp, ns));
+        }
+        F.hbox = nbox[k];
+        const double alpha = -beta_of(B, F, i) / 2. * (1. / F.p.reynolds);
+        for (int lev = 0; lev < F.L; ++lev) {
+            const std::array<int, 4>& b = nbox[k][lev];
+            const double dxl = F.dx * (1 << lev), c = alpha / (dxl * dxl);
+            HelmJob J{};
+            J.nx = F.nx; J.ny = F.ny; J.lev = lev;
+            J.i0 = b[0]; J.i1 = b[1]; J.j0 = b[2]; J.j1 = b[3];
+            J.c = c; J.inv_diag = 1. / (1 - 4 * c); J.in_scale = 1.0;
+            J.f = F.c + lev * F.ns; J.R = F.R + lev * F.ns;
+            J.out = F.m1 + lev * F.ns; J.tmp = F.m1tmp + lev * F.ns;
+            J.skip = F.cg_state + 3;
+            const long area = (long)(b[1] - b[0] + 1) * (b[3] - b[2] + 1);
+            maxarea = std::max(maxarea, area);
+            jobs.push_back(J);
+            jflow.push_back(k);
+            jbytes.push_back(24.0 * area);   // right-hand side and one neighbour stream in, result out
+        }
+    }
+    if (!B->m2l_built) {   // L^-1 of the local result (the DST solve M2 reads mt instead)
+        std::vector<SolveSpec> m2l;
+        for (auto& F : B->flows) {
+            m2l.push_back(spec_of(F, 0, 0.0, F.m1, F.psi, -1.0, 0, nullptr));
+            m2l.back().skip = F.cg_state + 3;
+        }
+        B->M2L.force_solve = true;
+        CKR(build_solve_plan(&B->ctx->c, B->mem, m2l, B->M2L));
+        B->m2l_built = true;
+    }
+    if ((int)jobs.size() > B->helm_cap) {
+        CKR(B->mem.alloc(&B->d_helm, jobs.size()));
+        B->helm_cap = (int)jobs.size();
+    }
+    CKR(upload(B->d_helm, jobs.data(), sizeof(HelmJob) * jobs.size()));
+    B->helm_n = (int)jobs.size();
+    B->helm_blocks = (int)((maxarea + kThreads - 1) / kThreads);
+    B->helm_sweeps = sweeps;
+    B->helm_flow = jflow;
+    B->helm_bytes = jbytes;
+    B->helm_on[i] = true;
+    return 0;
+}
+
 // M f = C A^-1 (beta B f) for the vector behind d_smear (ProjectionSolver.cc:96-101).
 static void enqueue_M(ibpm_batch* B, int i, RegOps* smear, hipStream_t s,
                       const std::vector<char>* active = nullptr) {
     const int nfl = (int)B->flows.size();
     Probe* pr = B->probe;
     timed(pr, K_SMEAR, 0, s, [&] { launch_smear_curl(smear, nfl, B->max_nodes, s); });
-    run_solve(B->M1[i], s, pr, active);
-    run_solve(B->M2, s, pr, active);
+    if (B->helm_on[i]) {   // A^-1 by local sweeps (plan_local_helm), then L^-1 of their result
+        run_coarsify(B->M1[i], s, pr, active);
+        double by = 0;
+        if (pr)
+            for (size_t j = 0; j < B->helm_flow.size(); ++j)
+                if (!active || (*active)[B->helm_flow[j]]) by += B->helm_bytes[j];
+        for (int sw = 0; sw < B->helm_sweeps; ++sw)
+            timed(pr, K_FSHELM, by, s,
+                  [&] { launch_helm_sweep(B->d_helm, B->helm_n, B->helm_blocks, sw, B->helm_sweeps, s); });
+        run_solve(B->M2L, s, pr, active);
+        B->st_local_mapps += 1;
+    } else {
+        run_solve(B->M1[i], s, pr, active);
+        run_solve(B->M2, s, pr, active);
+    }
     timed(pr, K_INTERP, 0, s, [&] { launch_interp(B->d_interp_m, nfl, B->max_comp, s); });
 }
 
@@ -1013,6 +1166,7 @@ static int substep_cg(ibpm_batch* B, int i) {
     for (auto& F : B->flows)
         if (F.ubf) CKR(move_base_flow(B, F, B->time + B->cn[i] * F.p.dt));
     CKR(upload_regops(B));
+    CKR(plan_local_helm(B, i));
     B->st_host_kin_s += now_s() - t_kin;
     Probe* pr = B->probe;
     timed(pr, K_CLEAR, 0, s, [&] { launch_clear_nodes(B->d_clear, nfl, B->max_nodes, s); });
@@ -2103,12 +2257,20 @@ int ibpm_batch_set_precond(ibpm_batch* B, int mode, int pairs) {
     return 0;
 }
 
+int ibpm_batch_set_local_helmholtz(ibpm_batch* B, int on) {
+    if (!B) { set_error("null batch"); return -1; }
+    B->helm_allowed = on != 0;
+    if (!on)
+        for (bool& h : B->helm_on) h = false;
+    return 0;
+}
+
 int ibpm_batch_solver_stats(ibpm_batch* B, long* stats, int nstats) {
     if (!B || !stats) { set_error("null argument"); return -1; }
-    const long v[9] = {B->st_iters, B->st_solves, B->st_mapps, B->st_build_mapps, B->st_builds,
-                       B->st_fallbacks, B->pc_broken ? 1L : 0L, (long)(1e6 * B->st_host_kin_s),
-                       (long)(1e6 * B->st_host_poll_s)};
-    const int n = std::min(nstats, 9);
+    const long v[10] = {B->st_iters, B->st_solves, B->st_mapps, B->st_build_mapps, B->st_builds,
+                        B->st_fallbacks, B->pc_broken ? 1L : 0L, (long)(1e6 * B->st_host_kin_s),
+                        (long)(1e6 * B->st_host_poll_s), B->st_local_mapps};
+    const int n = std::min(nstats, 10);
     for (int k = 0; k < n; ++k) stats[k] = v[k];
     return n;
 }

@@ -109,6 +109,10 @@ int lds_for_pass(const EllJob& J, int pass, bool regs);
 void launch_ell_pass(int pass, const EllJob* d_jobs, int njobs, int max_blocks, int lds_bytes, hipStream_t s,
                      bool force_solve, int M, bool regs);
 void launch_coarsify(const CoarsifyJob* d_jobs, int njobs, int max_rows, hipStream_t s, bool force_solve = false);
+// Sweep `sweep` of `nsweeps` of the local Helmholtz solves (HelmJob); the last
+// sweep writes each job's out.  max_blocks: kThreads-point blocks of the
+// largest box.
+void launch_helm_sweep(const HelmJob* d_jobs, int njobs, int max_blocks, int sweep, int nsweeps, hipStream_t s);
 void launch_vel(const FlowOps* d, int nflows, long max_pts, hipStream_t s);
 void launch_cross(const FlowOps* d, int nflows, long max_nf, int lev, hipStream_t s);
 void launch_rhs(const FlowOps* d, int nflows, long max_pts, hipStream_t s);

@@ -224,9 +224,21 @@ int ibpm_batch_set_precond(ibpm_batch* b, int mode, int pairs);
    [5] solves finished by plain CG after a breakdown, [6] 1 when some M was not
    positive definite and preconditioning stopped, [7] host microseconds in the
    per-substep kinematics and regulariser updates (GPU idle), [8] host
-   microseconds in solver polls (stream synchronisation and state read-back).
+   microseconds in solver polls (stream synchronisation and state read-back),
+   [9] batched M applications that took the local Helmholtz path.
    Returns the count written. */
 int ibpm_batch_solver_stats(ibpm_batch* b, long* stats, int nstats);
+/* No counterpart in the reference (its M application always solves A^-1 =
+   (1 + alpha Lap)^-1 on every level with the DST, ProjectionSolver.cc:96-101,
+   EllipticSolver.cc:40-70).  on = 1 (the default): inside the moving-body
+   force solve, A^-1 of beta B f is computed by Jacobi sweeps over a box around
+   the body on each level, wherever alpha / dx^2 is small enough that the
+   solution decays below 1e-18 of its maximum before the box reaches a level's
+   boundary or the ring of the finer level (the substep falls back to the DST
+   solve otherwise); the result equals the DST solve's to rounding.  on = 0:
+   the DST solve always.  The L^-1 solve that follows is the DST solve in both
+   cases. */
+int ibpm_batch_set_local_helmholtz(ibpm_batch* b, int on);
 
 /* Measurement: advances nsteps with a HIP event pair around every kernel
    launch on the batch stream and returns per kernel class the launch count,

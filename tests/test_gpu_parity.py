@@ -186,6 +186,30 @@ def test_batch_of_512_grids_takes_register_transform(gpu):
         assert rel(both.f(k), one.f()) < 1e-11, k
 
 
+def test_local_helmholtz_equals_dst_solve(gpu):
+    """Inside the moving-plate force solve, A^-1 by Jacobi sweeps over boxes
+    around the plate (the default) and by the DST solve on every level give
+    the same run: both solve the same system, to rounding, so the CG runs
+    agree to within their tolerance's effect (forces 1e-8 relative to max|F|,
+    vorticity 1e-8 relative L2 after 6 steps), and the local path was taken."""
+    ctx = gpu.Context(0)
+    geom = ("body Plate\n  line_n -0.5 0 0.5 0 33\n  center 0 0\n"
+            "  motion pitchplunge 0.2617993877991494 0.4 0 0.2 0.4 0\nend\n")
+    fl = gpu.Flow(128, 128, 3, 4.0, -2.0, -2.0, 300.0, 0.005, geom, tol=1e-10)
+    loc = gpu.Batch(ctx, [fl]).init()
+    dst = gpu.Batch(ctx, [fl]).init().set_local_helmholtz(False)
+    for _ in range(6):
+        loc.advance(1)
+        dst.advance(1)
+    hl, hd = loc.force_history(), dst.force_history()
+    assert np.abs(hl - hd).max() <= 1e-8 * np.abs(hd).max()
+    assert rel(loc.omega(), dst.omega()) < 1e-8
+    sl, sd = loc.solver_stats(), dst.solver_stats()
+    print("local", sl, "dst", sd, flush=True)
+    assert sl["local_helmholtz_m_applications"] == sl["m_applications"] > 0
+    assert sd["local_helmholtz_m_applications"] == 0
+
+
 def test_moving_plate_cg_matches_oracle(gpu, oracle):
     geom = plate_geom(16, 1.0, 0.2, 0.2, 0.4)   # spacing 1/15 ~ dx = 1/16
     ctx, b, s = _pair(gpu, oracle, 64, 3, 300.0, 0.01, geom, tol=1e-10)

@@ -67,10 +67,14 @@ struct EllJob {
 struct HelmJob {
     int nx, ny, lev;
     int i0, i1, j0, j1;      // box (1-based, inclusive), inside 2..nx-2 x 2..ny-2
+    int s0, s1, t0, t1;      // support of the right-hand side on this level (zero off it)
+    int fs0, fs1, ft0, ft1;  // support on the next finer level (lev >= 1)
     double c, inv_diag;      // alpha / dx_lev^2 and 1 / (1 - 4 c)
     double in_scale;
     const double* f;         // level slice of the right-hand side
-    const double* R;         // level slice of the coarsified right-hand side (central region, lev >= 1)
+    double* R;               // level slice of the coarsified right-hand side (central region, lev >= 1)
+    const double* f_fine;    // the next finer level's slices of f and R (lev >= 1)
+    const double* R_fine;
     double* out;             // level slice written by the last sweep
     double* tmp;             // level slice of the other sweep buffer
     const double* skip;      // null, or the flow's solver flags

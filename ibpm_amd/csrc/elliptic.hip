@@ -1038,8 +1038,9 @@ __global__ void __launch_bounds__(kThreads) k_fs_helm(const HelmJob* __restrict_
     const int i = J.i0 + p / w, j = J.j0 + p % w;
     const int ex = nx_ext(J.nx), ey = nx_ext(J.ny);
     const bool central = J.lev >= 1 && i > ex && i < J.nx / 2 + ex && j > ey && j < J.ny / 2 + ey;
+    const bool support = i >= J.s0 && i <= J.s1 && j >= J.t0 && j <= J.t1;
     const long k = sidx(J.nx, J.ny, i, j);
-    const double r = (central ? gin(J.R)[k] : gin(J.f)[k]) * J.in_scale;
+    const double r = support ? (central ? gin(J.R)[k] : gin(J.f)[k]) * J.in_scale : 0.0;
     const bool to_out = ((nsweeps - 1 - sweep) & 1) == 0;
     IBPM_GLOBAL double* dst = gout(to_out ? J.out : J.tmp);
     double v;
@@ -1052,6 +1053,38 @@ __global__ void __launch_bounds__(kThreads) k_fs_helm(const HelmJob* __restrict_
         v = (r - J.c * s4) * J.inv_diag;
     }
     dst[k] = v;
+}
+
+// The coarsify chain of the local solves' right-hand side, level lev of every
+// job, over the job's support only: Scalar::coarsify's weights and order (as
+// coarsify_body) with the finer level's value taken as 0 in its central
+// region off its support (R holds other solves' values there) and from f
+// outside the central region.
+__global__ void __launch_bounds__(kThreads) k_fs_helm_coarsify(const HelmJob* __restrict__ jobs, int lev) {
+    const HelmJob& J = jobs[blockIdx.y];
+    if (J.lev != lev || skipped(J.skip)) return;
+    const int w = J.t1 - J.t0 + 1, h = J.s1 - J.s0 + 1;
+    const int p = blockIdx.x * kThreads + threadIdx.x;
+    if (p >= w * h) return;
+    const int i = J.s0 + p / w, j = J.t0 + p % w;
+    const int ex = nx_ext(J.nx), ey = nx_ext(J.ny);
+    const int ii = (i - ex) * 2, jj = (j - ey) * 2;   // Grid::c2f
+    const int lf = lev - 1;
+    auto fine = [&](int fi, int fj) {
+        const long k = sidx(J.nx, J.ny, fi, fj);
+        const bool central = lf >= 1 && fi > ex && fi < J.nx / 2 + ex && fj > ey && fj < J.ny / 2 + ey;
+        if (!central) return gin(J.f_fine)[k];
+        const bool sup = fi >= J.fs0 && fi <= J.fs1 && fj >= J.ft0 && fj <= J.ft1;
+        return sup ? gin(J.R_fine)[k] : 0.0;
+    };
+    double x[3][3];   // fine (ii - 1 + d, jj - 1 + c)
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) x[d][c] = fine(ii - 1 + d, jj - 1 + c);
+    const double v = 0.25 * x[1][1] + 0.125 * (x[2][1] + x[1][2] + x[0][1] + x[1][0])
+        + 0.0625 * (x[2][2] + x[2][0] + x[0][2] + x[0][0]);
+    gout(J.R)[sidx(J.nx, J.ny, i, j)] = v;
 }
 
 // ------------------------------------------------------------ host launch
@@ -1103,6 +1136,10 @@ void launch_ell_pass(int pass, const EllJob* d_jobs, int njobs, int max_blocks, 
     } else {
         if (force_solve) { IBPM_LAUNCH_LEN(k_fs_pass3, 3) } else { IBPM_LAUNCH_LEN(k_ell_pass3, 3) }
     }
+}
+
+void launch_helm_coarsify(const HelmJob* d_jobs, int njobs, int max_blocks, int lev, hipStream_t s) {
+    hipLaunchKernelGGL(k_fs_helm_coarsify, dim3(max_blocks, njobs), dim3(kThreads), 0, s, d_jobs, lev);
 }
 
 void launch_helm_sweep(const HelmJob* d_jobs, int njobs, int max_blocks, int sweep, int nsweeps, hipStream_t s) {

@@ -437,9 +437,9 @@ struct ibpm_batch {
     bool helm_allowed = true, helm_on[4] = {false, false, false, false}, m2l_built = false;
     ibpmg::SolvePlan M2L;
     ibpmg::HelmJob* d_helm = nullptr;
-    int helm_cap = 0, helm_n = 0, helm_blocks = 0, helm_sweeps = 0;
+    int helm_cap = 0, helm_n = 0, helm_blocks = 0, helm_cblocks = 0, helm_sweeps = 0;
     std::vector<int> helm_flow;
-    std::vector<double> helm_bytes;
+    std::vector<double> helm_bytes, helm_cbytes;
     long st_local_mapps = 0;
     ibpmg::FlowOps* d_fops[4] = {};
     ibpmg::RegOps *d_interp_c = nullptr, *d_interp_m = nullptr, *d_smear_f[4] = {}, *d_smear_d[4] = {};
@@ -827,7 +827,7 @@ static int plan_local_helm(ibpm_batch* B, int i) {
     B->helm_on[i] = false;
     if (!B->helm_allowed) return 0;
     const int nfl = (int)B->flows.size();
-    std::vector<std::vector<std::array<int, 4>>> nbox(nfl);
+    std::vector<std::vector<std::array<int, 4>>> nbox(nfl), nsup(nfl);
     int sweeps = 1;
     for (int k = 0; k < nfl; ++k) {
         Flow& F = B->flows[k];
@@ -843,6 +843,7 @@ static int plan_local_helm(ibpm_batch* B, int i) {
         const double alpha = -beta_of(B, F, i) / 2. * (1. / F.p.reynolds);   // as the M1 solve's
         const int ex = nx_ext(F.nx), ey = nx_ext(F.ny);
         nbox[k].resize(F.L);
+        nsup[k].resize(F.L);
         for (int lev = 0; lev < F.L; ++lev) {
             if (lev >= 1) {   // support after coarsify, inside the central region
                 s0 = std::max((s0 - 1 + 1) / 2 + ex, ex + 1);   // ceil((s0 - 1) / 2), s0 >= 1
@@ -864,13 +865,14 @@ static int plan_local_helm(ibpm_batch* B, int i) {
                 b = {std::min(b[0], o[0]), std::max(b[1], o[1]), std::min(b[2], o[2]), std::max(b[3], o[3])};
             }
             nbox[k][lev] = b;
+            nsup[k][lev] = {s0, s1, t0, t1};
             sweeps = std::max(sweeps, S);
         }
     }
     std::vector<HelmJob> jobs;
     std::vector<int> jflow;
-    std::vector<double> jbytes;
-    long maxarea = 0;
+    std::vector<double> jbytes, cbytes;
+    long maxarea = 0, maxsup = 0;
     for (int k = 0; k < nfl; ++k) {
         Flow& F = B->flows[k];
         const long ns = F.ns * F.L;
@@ -882,10 +884,22 @@ static int plan_local_helm(ibpm_batch* B, int i) {
         const double alpha = -beta_of(B, F, i) / 2. * (1. / F.p.reynolds);
         for (int lev = 0; lev < F.L; ++lev) {
             const std::array<int, 4>& b = nbox[k][lev];
+            const std::array<int, 4>& sp = nsup[k][lev];
             const double dxl = F.dx * (1 << lev), c = alpha / (dxl * dxl);
             HelmJob J{};
             J.nx = F.nx; J.ny = F.ny; J.lev = lev;
             J.i0 = b[0]; J.i1 = b[1]; J.j0 = b[2]; J.j1 = b[3];
+            J.s0 = sp[0]; J.s1 = sp[1]; J.t0 = sp[2]; J.t1 = sp[3];
+            if (lev >= 1) {
+                const std::array<int, 4>& fs = nsup[k][lev - 1];
+                J.fs0 = fs[0]; J.fs1 = fs[1]; J.ft0 = fs[2]; J.ft1 = fs[3];
+                J.f_fine = F.c + (lev - 1) * F.ns; J.R_fine = F.R + (lev - 1) * F.ns;
+                const long sa = (long)(sp[1] - sp[0] + 1) * (sp[3] - sp[2] + 1);
+                maxsup = std::max(maxsup, sa);
+                cbytes.push_back(40.0 * sa);   // 9 fine reads (4 new per point) and one write
+            } else {
+                cbytes.push_back(0.0);
+            }
             J.c = c; J.inv_diag = 1. / (1 - 4 * c); J.in_scale = 1.0;
             J.f = F.c + lev * F.ns; J.R = F.R + lev * F.ns;
             J.out = F.m1 + lev * F.ns; J.tmp = F.m1tmp + lev * F.ns;
@@ -914,9 +928,11 @@ static int plan_local_helm(ibpm_batch* B, int i) {
     CKR(upload(B->d_helm, jobs.data(), sizeof(HelmJob) * jobs.size()));
     B->helm_n = (int)jobs.size();
     B->helm_blocks = (int)((maxarea + kThreads - 1) / kThreads);
+    B->helm_cblocks = (int)((maxsup + kThreads - 1) / kThreads);
     B->helm_sweeps = sweeps;
     B->helm_flow = jflow;
     B->helm_bytes = jbytes;
+    B->helm_cbytes = cbytes;
     B->helm_on[i] = true;
     return 0;
 }
@@ -928,11 +944,16 @@ static void enqueue_M(ibpm_batch* B, int i, RegOps* smear, hipStream_t s,
     Probe* pr = B->probe;
     timed(pr, K_SMEAR, 0, s, [&] { launch_smear_curl(smear, nfl, B->max_nodes, s); });
     if (B->helm_on[i]) {   // A^-1 by local sweeps (plan_local_helm), then L^-1 of their result
-        run_coarsify(B->M1[i], s, pr, active);
-        double by = 0;
+        double by = 0, cby = 0;
         if (pr)
             for (size_t j = 0; j < B->helm_flow.size(); ++j)
-                if (!active || (*active)[B->helm_flow[j]]) by += B->helm_bytes[j];
+                if (!active || (*active)[B->helm_flow[j]]) {
+                    by += B->helm_bytes[j];
+                    cby += B->helm_cbytes[j];
+                }
+        for (int lev = 1; lev < B->M1[i].nlev; ++lev)
+            timed(pr, K_FSCOARSIFY, cby / std::max(1, B->M1[i].nlev - 1), s,
+                  [&] { launch_helm_coarsify(B->d_helm, B->helm_n, B->helm_cblocks, lev, s); });
         for (int sw = 0; sw < B->helm_sweeps; ++sw)
             timed(pr, K_FSHELM, by, s,
                   [&] { launch_helm_sweep(B->d_helm, B->helm_n, B->helm_blocks, sw, B->helm_sweeps, s); });

@@ -113,6 +113,9 @@ void launch_coarsify(const CoarsifyJob* d_jobs, int njobs, int max_rows, hipStre
 // sweep writes each job's out.  max_blocks: kThreads-point blocks of the
 // largest box.
 void launch_helm_sweep(const HelmJob* d_jobs, int njobs, int max_blocks, int sweep, int nsweeps, hipStream_t s);
+// The right-hand side's coarsify chain at level lev (>= 1) over each job's
+// support (HelmJob); max_blocks: kThreads-point blocks of the largest support.
+void launch_helm_coarsify(const HelmJob* d_jobs, int njobs, int max_blocks, int lev, hipStream_t s);
 void launch_vel(const FlowOps* d, int nflows, long max_pts, hipStream_t s);
 void launch_cross(const FlowOps* d, int nflows, long max_nf, int lev, hipStream_t s);
 void launch_rhs(const FlowOps* d, int nflows, long max_pts, hipStream_t s);

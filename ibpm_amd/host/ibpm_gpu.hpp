@@ -211,6 +211,10 @@ public:
     // plain CG, 1 CG preconditioned by a stale FP64 inverse of M with `pairs`
     // low-rank pairs (ibpm_batch_set_precond).  Before init().
     void setPrecond(int mode, int pairs) { _pc_mode = mode; _pc_pairs = pairs; }
+    // A^-1 inside the moving-body force solve by local sweeps where they equal
+    // the DST solve to rounding (true, the default) or always by the DST
+    // solve (ibpm_batch_set_local_helmholtz).  Before init().
+    void setLocalHelmholtz(bool on) { _local_helm = on; }
     // IBSolver::load/save (src/IBSolver.cc:77-108): the Cholesky factors live
     // in <basename>_0i.cholesky.  load() only names the cache and returns false;
     // init() then reads matching files or builds the factors and writes them,
@@ -238,6 +242,7 @@ public:
         gpu_check(ibpm_batch_init(_batch));
         if (!_model.geTimeDependent()) gpu_check(ibpm_batch_use_graph(_batch, 1));
         else if (_pc_mode) gpu_check(ibpm_batch_set_precond(_batch, _pc_mode, _pc_pairs));
+        if (!_local_helm) gpu_check(ibpm_batch_set_local_helmholtz(_batch, 0));
     }
     // IBSolver::advance, reference src/IBSolver.cc:142-150
     void advance(State& x) {
@@ -287,6 +292,7 @@ private:
     double _tol;
     int _device;
     int _pc_mode = 0, _pc_pairs = 0;
+    bool _local_helm = true;
     std::string _cache;
     ibpm_ctx* _ctx = nullptr;
     ibpm_batch* _batch = nullptr;

@@ -85,16 +85,18 @@ int main(int argc, char** argv) {
     const double tol = P.f("tol", 1e-7);
     const int device = P.i("device", 0);
     const std::string precond = P.str("precond", "none");
+    const std::string helmholtz = P.str("helmholtz", "local");
     const bool ubf = P.i("ubf", 0) != 0;
     std::map<std::string, Scheme::SchemeType> schemes = {
         {"euler", Scheme::EULER}, {"ab2", Scheme::AB2}, {"rk3", Scheme::RK3}, {"rk3b", Scheme::RK3b}};
     std::map<std::string, std::pair<int, int>> preconds = {{"none", {0, 0}}, {"stale", {1, 0}}, {"lowrank", {1, 8}}};
-    if (!P.valid() || P.help || !schemes.count(scheme) || model != "nonlinear" || !preconds.count(precond)) {
+    if (!P.valid() || P.help || !schemes.count(scheme) || model != "nonlinear" || !preconds.count(precond) ||
+        (helmholtz != "local" && helmholtz != "dst")) {
         std::cerr << "usage: ibpm [-name ibpm] [-outdir .] [-force 1] [-nx 200] [-ny 200] [-ngrid 1]\n"
                      "            [-length 4] [-xoffset -2] [-yoffset -2] [-alpha 0] [-geom name.geom]\n"
                      "            [-Re 100] [-dt 0.02] [-nsteps 250] [-scheme euler|ab2|rk3|rk3b]\n"
                      "            [-model nonlinear] [-tol 1e-7] [-device 0] [-precond none|stale|lowrank]\n"
-                     "            [-ubf 0|1]\n";
+                     "            [-ubf 0|1] [-helmholtz local|dst]\n";
         return 1;
     }
     if (outdir.empty() || outdir.back() != '/') outdir += "/";
@@ -126,6 +128,7 @@ int main(int argc, char** argv) {
     NavierStokesModel nsm(grid, geom, Re, q_potential);
     NonlinearIBSolver solver(grid, nsm, dt, schemes[scheme], tol, device);
     solver.setPrecond(preconds[precond].first, preconds[precond].second);
+    solver.setLocalHelmholtz(helmholtz == "local");
     State x(grid, geom.getNumPoints());
     nsm.init();
     try {

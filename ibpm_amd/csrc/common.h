@@ -59,6 +59,13 @@ struct EllJob {
     FftPlan pj;              // transform along j (length ny-1)
     FftPlan pi;              // transform along i (length nx-1)
     const double* skip;      // null, or the flow's solver flags (see skipped())
+    // Pass 3 over part of the level (the register-held pass 3 only): row pairs
+    // in [full_lo, full_hi] and pairs fq0, fq1 are transformed; of the other
+    // rows, those in [ring_lo, ring_hi] (1-based) get u at columns rj0 and rj1
+    // alone, by direct sums against ringsin (2 sin(pi l j / ny), l = 1..ny-1,
+    // for j = rj0 then rj1); the rest is not written.
+    int full_lo, full_hi, fq0, fq1, ring_lo, ring_hi, rj0, rj1;
+    const double* ringsin;
 };
 
 // Local solve of A u = r, A = 1 + alpha Lap (the force solve's A^-1 of one

@@ -865,12 +865,55 @@ __device__ __forceinline__ void pass2_r16(const EllJob& J, double2* smem, int bx
     write_staged<M, G>(gout(J.T2), smem, N2, J.ld2, 2 * G * bx);
 }
 
+// Pass 3 of a row pair that is not transformed: u at columns rj0, rj1 of its
+// rows in [ring_lo, ring_hi], each the sum over the row's modes l of
+// T2[l] 2 sin(pi l j / ny) (the inverse DST-I at one point), times norm.
+template <int M>
+__device__ __forceinline__ void pass3_ring(const EllJob& J, double2* smem, int bx) {
+    constexpr int T = M / 16;
+    const int N1 = J.nx - 1, N2 = M / 2 - 1;
+    const int t = threadIdx.x;
+    const IBPM_GLOBAL double* sn = gin(J.ringsin);
+    double part[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int r = 2 * bx + h, i = r + 1;
+        if (r >= N1 || i < J.ring_lo || i > J.ring_hi) continue;
+        const IBPM_GLOBAL double* row = gin(J.T2) + (long)r * J.ld2;
+        double a = 0.0, b = 0.0;
+        for (int l = t; l < N2; l += T) {
+            const double x = row[l];
+            a += x * sn[l];
+            b += x * sn[N2 + l];
+        }
+        part[2 * h] = a;
+        part[2 * h + 1] = b;
+    }
+    double* red = (double*)smem;   // 4 x T partial sums
+#pragma unroll
+    for (int q = 0; q < 4; ++q) red[q * T + t] = part[q];
+    __syncthreads();
+    if (t < 4) {
+        const int r = 2 * bx + (t >> 1), i = r + 1;
+        if (r < N1 && i >= J.ring_lo && i <= J.ring_hi) {
+            double v = 0.0;
+            for (int k = 0; k < T; ++k) v += red[t * T + k];
+            const int j = (t & 1) ? J.rj1 : J.rj0;
+            gout(J.u)[(long)r * N2 + (j - 1)] = v * J.norm;
+        }
+    }
+}
+
 template <int M>
 __device__ __forceinline__ void pass3_r16(const EllJob& J, double2* smem, int bx) {
     static_assert(r16_group(3) == 1, "pass 3 writes rows: one pair per workgroup");
     constexpr int T = M / 16;
     const int N1 = J.nx - 1, N2 = M / 2 - 1, npairs = (N1 + 1) / 2;
     if (bx >= npairs) return;
+    if (!((bx >= J.full_lo && bx <= J.full_hi) || bx == J.fq0 || bx == J.fq1)) {
+        if (J.ring_lo <= J.ring_hi) pass3_ring<M>(J, smem, bx);
+        return;
+    }
     double2* buf = smem;
     const IBPM_GLOBAL double* tw = gin((const double*)J.pj.tw16);
     const IBPM_GLOBAL double* twg = gin((const double*)J.pj.tw);

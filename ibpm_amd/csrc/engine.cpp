@@ -193,6 +193,12 @@ static void timed(Probe* pr, int cls, double bytes, hipStream_t s, Fn fn) {
 
 // The jobs of one level step that share their transform lengths (one launch
 // per pass).
+// Pass 3 over the whole level (EllJob's partial-pass-3 fields off).
+static void full_pass3(EllJob& J) {
+    J.full_lo = 0; J.full_hi = INT_MAX; J.fq0 = J.fq1 = -1;
+    J.ring_lo = 1; J.ring_hi = 0; J.rj0 = J.rj1 = 1; J.ringsin = nullptr;
+}
+
 struct PassGroup {
     EllJob* d = nullptr;
     int n = 0;
@@ -202,6 +208,7 @@ struct PassGroup {
     std::vector<int> jflow;                         // flow of each job
     std::vector<double> jbytes;                     // 16 B per point (one read + one write)
     double bytes = 0;
+    std::vector<EllJob> h;                          // host copy of the jobs
 };
 
 struct SolvePlan {
@@ -256,6 +263,7 @@ static int build_solve_plan(Ctx* c, DevMem& mem, const std::vector<SolveSpec>& s
             const long ns = (long)(s.nx - 1) * (s.ny - 1);
             const double dxl = s.dx * (1 << lev);
             EllJob J{};
+            full_pass3(J);
             J.nx = s.nx; J.ny = s.ny; J.lev = lev; J.ngrid = s.L;
             J.kind = s.kind; J.epilogue = s.epilogue; J.in_scale = s.in_scale;
             J.alpha = s.alpha;
@@ -306,6 +314,7 @@ static int build_solve_plan(Ctx* c, DevMem& mem, const std::vector<SolveSpec>& s
             CKR(mem.alloc(&G.d, jobs.size()));
             CK(hipMemcpy(G.d, jobs.data(), sizeof(EllJob) * jobs.size(), hipMemcpyHostToDevice));
             G.n = (int)jobs.size();
+            G.h = jobs;
         }
     }
     return 0;
@@ -395,6 +404,7 @@ struct Flow {
     // local Helmholtz path of the force solve's A^-1: result and sweep buffer
     // (all levels) and, per level, the union of the boxes swept so far
     double *m1 = nullptr, *m1tmp = nullptr;
+    double* ringsin = nullptr;     // 2 sin(pi l j / ny) at the ring columns (restrict_m2l_pass3)
     std::vector<std::array<int, 4>> hbox;
     // host state
     std::vector<double> px, py, pu, pv;
@@ -823,6 +833,55 @@ static void mark_nprev_valid(ibpm_batch* B) {
 // reads them; the solution there is then below 1e-18 of its maximum.
 constexpr int kHelmMaxSweeps = 40;
 
+// Pass 3 of the L^-1 solve that follows the local A^-1 writes only the psi
+// that is read afterwards: on level 0, the rows of the regulariser's nodes
+// (and one more on each side), where the interpolation takes Curl psi; on
+// coarser levels, the ring that the next finer level's getBC reads (rows ex
+// and ex + nx/2 whole, columns ey and ey + ny/2 of the rows between).  The
+// solve's other consumers are none: the step's refresh solves psi again.
+// Jobs whose pass 3 is not the register-held one stay whole.
+static int restrict_m2l_pass3(ibpm_batch* B, const std::vector<std::vector<std::array<int, 4>>>& nsup) {
+    for (auto& F : B->flows) {
+        if (F.ringsin) continue;
+        const int n = F.ny - 1, ey = nx_ext(F.ny);
+        std::vector<double> t(2 * (size_t)n);
+        const double pi = 4. * std::atan(1.);
+        for (int l = 1; l <= n; ++l) {
+            t[l - 1] = 2. * std::sin(pi * l * ey / F.ny);
+            t[n + l - 1] = 2. * std::sin(pi * l * (ey + F.ny / 2) / F.ny);
+        }
+        CKR(B->mem.alloc(&F.ringsin, t.size()));
+        CKR(upload(F.ringsin, t.data(), sizeof(double) * t.size()));
+    }
+    for (auto& step : B->M2L.steps)
+        for (auto& G : step) {
+            bool changed = false;
+            for (size_t j = 0; j < G.h.size(); ++j) {
+                EllJob& J = G.h[j];
+                const int k = G.jflow[j];
+                const EllJob before = J;
+                full_pass3(J);
+                if (G.regs[2] && J.epilogue == 0 && !J.vside) {
+                    const int N1 = J.nx - 1, ex = nx_ext(J.nx), ey = nx_ext(J.ny);
+                    if (J.lev == 0) {
+                        const std::array<int, 4>& sp = nsup[k][0];   // node rows, 1-based
+                        J.full_lo = std::max(sp[0] - 2, 0) / 2;
+                        J.full_hi = std::min(sp[1], N1 - 1) / 2;
+                    } else {
+                        J.full_lo = 1; J.full_hi = 0;
+                        J.fq0 = (ex - 1) / 2; J.fq1 = (ex + J.nx / 2 - 1) / 2;
+                        J.ring_lo = ex + 1; J.ring_hi = ex + J.nx / 2 - 1;
+                        J.rj0 = ey; J.rj1 = ey + J.ny / 2;
+                        J.ringsin = B->flows[k].ringsin;
+                    }
+                }
+                if (std::memcmp(&before, &J, sizeof(EllJob)) != 0) changed = true;
+            }
+            if (changed) CKR(upload(G.d, G.h.data(), sizeof(EllJob) * G.h.size()));
+        }
+    return 0;
+}
+
 static int plan_local_helm(ibpm_batch* B, int i) {
     B->helm_on[i] = false;
     if (!B->helm_allowed) return 0;
@@ -921,6 +980,7 @@ static int plan_local_helm(ibpm_batch* B, int i) {
         CKR(build_solve_plan(&B->ctx->c, B->mem, m2l, B->M2L));
         B->m2l_built = true;
     }
+    CKR(restrict_m2l_pass3(B, nsup));
     if ((int)jobs.size() > B->helm_cap) {
         CKR(B->mem.alloc(&B->d_helm, jobs.size()));
         B->helm_cap = (int)jobs.size();
@@ -1658,6 +1718,7 @@ int ibpm_dst2d(ibpm_ctx* c, int n1, int n2, const double* d_in, double* d_out) {
     CKR(cos_tables(mem, n1 + 1, &cosx));
     CKR(cos_tables(mem, n2 + 1, &cosy));
     EllJob J{};
+    full_pass3(J);
     J.nx = n1 + 1; J.ny = n2 + 1; J.lev = 0; J.ngrid = 1; J.in_scale = 1.0;
     J.f = d_in; J.T1 = T; J.T2 = T; J.u = d_out; J.norm = 1.0;
     J.ld1 = n1; J.ld2 = n2;   // unpadded: the second launch writes the caller's array

@@ -66,6 +66,12 @@ struct EllJob {
     // for j = rj0 then rj1); the rest is not written.
     int full_lo, full_hi, fq0, fq1, ring_lo, ring_hi, rj0, rj1;
     const double* ringsin;
+    // Pass 1 (register-held): rows i outside [sup_lo, sup_hi] (1-based) have
+    // no interior right-hand side; a workgroup whose rows are all such rows,
+    // none of them 1 or nx-1, writes their transform in closed form from the
+    // two boundary terms, sin1[k - 1] = 2 sin(pi k / ny) (null: never).
+    int sup_lo, sup_hi;
+    const double* sin1;
 };
 
 // Local solve of A u = r, A = 1 + alpha Lap (the force solve's A^-1 of one

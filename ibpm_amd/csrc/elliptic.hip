@@ -736,11 +736,52 @@ __device__ __forceinline__ void write_staged(IBPM_GLOBAL double* out, const doub
     }
 }
 
+// Pass 1 of a workgroup whose rows have no interior right-hand side: the
+// rows' only nonzeros are getRHS's terms x_1 = -bottom c and x_N = -top c
+// (rhs_value's, in its order), so the DST-I of a row is
+// 2 sin(pi k / ny) (x_1 + (-1)^(k+1) x_N), written like write_staged.
+template <int M, int G>
+__device__ __forceinline__ void pass1_bc_rows(const EllJob& J, double2* smem, int row0) {
+    constexpr int N = M / 2 - 1, NT = G * (M / 16);
+    const int N1 = J.nx - 1;
+    const int t = threadIdx.x;
+    double* bv = (double*)smem;   // x_1 of the 2G rows, then x_N
+    if (t < 2 * G) {
+        const int i = row0 + t + 1;
+        double x1 = 0.0, xn = 0.0;
+        if (i <= N1 && (J.uc || J.bcring)) {
+            x1 = 0.0 - bc_bottom(J, i) * J.bc_coef;
+            xn = 0.0 - bc_top(J, i) * J.bc_coef;
+        }
+        bv[t] = x1;
+        bv[2 * G + t] = xn;
+    }
+    __syncthreads();
+    const IBPM_GLOBAL double* s1 = gin(J.sin1);
+    IBPM_GLOBAL double* out = gout(J.T1);
+    for (int idx = t; idx < N * G; idx += NT) {
+        const int k = idx / G, g = idx % G;   // mode k + 1
+        const int row = row0 + 2 * g;
+        if (row >= N1) continue;
+        const double s = s1[k];
+        const double a = s * ((k & 1) ? bv[2 * g] - bv[2 * G + 2 * g] : bv[2 * g] + bv[2 * G + 2 * g]);
+        const double b = s * ((k & 1) ? bv[2 * g + 1] - bv[2 * G + 2 * g + 1] : bv[2 * g + 1] + bv[2 * G + 2 * g + 1]);
+        store_pair(out + (long)k * J.ld1 + row, a, b, row + 1 < N1);
+    }
+}
+
 template <int M>
 __device__ __forceinline__ void pass1_r16(const EllJob& J, double2* smem, int bx) {
     constexpr int T = M / 16, G = r16_group(1);
     const int N1 = J.nx - 1, npairs = (N1 + 1) / 2;
     if (G * bx >= npairs) return;
+    if (J.sin1) {   // a workgroup of rows off the right-hand side's support, none on the edge
+        const int i_first = 2 * G * bx + 1, i_last = min(2 * G * bx + 2 * G, N1);
+        if ((i_last < J.sup_lo || i_first > J.sup_hi) && i_first > 1 && i_last < J.nx - 1) {
+            pass1_bc_rows<M, G>(J, smem, 2 * G * bx);
+            return;
+        }
+    }
     const int g = threadIdx.x / T, t = threadIdx.x & (T - 1);
     const int q = min(G * bx + g, npairs - 1);
     double2* buf = smem + g * r16_buf(M);

@@ -193,10 +193,11 @@ static void timed(Probe* pr, int cls, double bytes, hipStream_t s, Fn fn) {
 
 // The jobs of one level step that share their transform lengths (one launch
 // per pass).
-// Pass 3 over the whole level (EllJob's partial-pass-3 fields off).
-static void full_pass3(EllJob& J) {
+// Passes 1 and 3 over the whole level (EllJob's partial-pass fields off).
+static void whole_level(EllJob& J) {
     J.full_lo = 0; J.full_hi = INT_MAX; J.fq0 = J.fq1 = -1;
     J.ring_lo = 1; J.ring_hi = 0; J.rj0 = J.rj1 = 1; J.ringsin = nullptr;
+    J.sup_lo = 1; J.sup_hi = INT_MAX; J.sin1 = nullptr;
 }
 
 struct PassGroup {
@@ -263,7 +264,7 @@ static int build_solve_plan(Ctx* c, DevMem& mem, const std::vector<SolveSpec>& s
             const long ns = (long)(s.nx - 1) * (s.ny - 1);
             const double dxl = s.dx * (1 << lev);
             EllJob J{};
-            full_pass3(J);
+            whole_level(J);
             J.nx = s.nx; J.ny = s.ny; J.lev = lev; J.ngrid = s.L;
             J.kind = s.kind; J.epilogue = s.epilogue; J.in_scale = s.in_scale;
             J.alpha = s.alpha;
@@ -405,6 +406,7 @@ struct Flow {
     // (all levels) and, per level, the union of the boxes swept so far
     double *m1 = nullptr, *m1tmp = nullptr;
     double* ringsin = nullptr;     // 2 sin(pi l j / ny) at the ring columns (restrict_m2l_pass3)
+    double* sin1 = nullptr;        // 2 sin(pi k / ny), k = 1..ny-1
     std::vector<std::array<int, 4>> hbox;
     // host state
     std::vector<double> px, py, pu, pv;
@@ -833,25 +835,50 @@ static void mark_nprev_valid(ibpm_batch* B) {
 // reads them; the solution there is then below 1e-18 of its maximum.
 constexpr int kHelmMaxSweeps = 40;
 
-// Pass 3 of the L^-1 solve that follows the local A^-1 writes only the psi
-// that is read afterwards: on level 0, the rows of the regulariser's nodes
-// (and one more on each side), where the interpolation takes Curl psi; on
-// coarser levels, the ring that the next finer level's getBC reads (rows ex
-// and ex + nx/2 whole, columns ey and ey + ny/2 of the rows between).  The
-// solve's other consumers are none: the step's refresh solves psi again.
-// Jobs whose pass 3 is not the register-held one stay whole.
+// The L^-1 solve that follows the local A^-1 runs its passes over part of
+// each level.  Pass 3 writes only the psi that is read afterwards: on level
+// 0, the rows of the regulariser's nodes (and one more on each side), where
+// the interpolation takes Curl psi; on coarser levels, the ring that the next
+// finer level's getBC reads (rows ex and ex + nx/2 whole, columns ey and
+// ey + ny/2 of the rows between).  The solve's other consumers are none: the
+// step's refresh solves psi again.  Pass 1 takes rows off the right-hand
+// side's support in closed form: the support is the local result's box on
+// each level, with coarsify's reach from the finer level inside the central
+// region.  (The LDS passes ignore both restrictions and do the whole level.)
 static int restrict_m2l_pass3(ibpm_batch* B, const std::vector<std::vector<std::array<int, 4>>>& nsup) {
     for (auto& F : B->flows) {
         if (F.ringsin) continue;
         const int n = F.ny - 1, ey = nx_ext(F.ny);
-        std::vector<double> t(2 * (size_t)n);
+        std::vector<double> t(2 * (size_t)n), s1(n);
         const double pi = 4. * std::atan(1.);
         for (int l = 1; l <= n; ++l) {
             t[l - 1] = 2. * std::sin(pi * l * ey / F.ny);
             t[n + l - 1] = 2. * std::sin(pi * l * (ey + F.ny / 2) / F.ny);
+            s1[l - 1] = 2. * std::sin(pi * l / F.ny);
         }
         CKR(B->mem.alloc(&F.ringsin, t.size()));
         CKR(upload(F.ringsin, t.data(), sizeof(double) * t.size()));
+        CKR(B->mem.alloc(&F.sin1, s1.size()));
+        CKR(upload(F.sin1, s1.data(), sizeof(double) * s1.size()));
+    }
+    // rows of the right-hand side's support per flow and level
+    std::vector<std::vector<std::array<int, 2>>> rows(B->flows.size());
+    for (size_t k = 0; k < B->flows.size(); ++k) {
+        const Flow& F = B->flows[k];
+        const int ex = nx_ext(F.nx);
+        rows[k].resize(F.L);
+        for (int lev = 0; lev < F.L; ++lev) {
+            int lo = F.hbox[lev][0], hi = F.hbox[lev][1];
+            if (lev >= 1) {
+                const int clo = std::max(rows[k][lev - 1][0] / 2 + ex, ex + 1);
+                const int chi = std::min((rows[k][lev - 1][1] + 1) / 2 + ex, ex + F.nx / 2 - 1);
+                if (clo <= chi) {
+                    lo = std::min(lo, clo);
+                    hi = std::max(hi, chi);
+                }
+            }
+            rows[k][lev] = {lo, hi};
+        }
     }
     for (auto& step : B->M2L.steps)
         for (auto& G : step) {
@@ -860,7 +887,12 @@ static int restrict_m2l_pass3(ibpm_batch* B, const std::vector<std::vector<std::
                 EllJob& J = G.h[j];
                 const int k = G.jflow[j];
                 const EllJob before = J;
-                full_pass3(J);
+                whole_level(J);
+                if (J.epilogue == 0 && !J.vside) {
+                    J.sup_lo = rows[k][J.lev][0];
+                    J.sup_hi = rows[k][J.lev][1];
+                    J.sin1 = B->flows[k].sin1;
+                }
                 if (G.regs[2] && J.epilogue == 0 && !J.vside) {
                     const int N1 = J.nx - 1, ex = nx_ext(J.nx), ey = nx_ext(J.ny);
                     if (J.lev == 0) {
@@ -1718,7 +1750,7 @@ int ibpm_dst2d(ibpm_ctx* c, int n1, int n2, const double* d_in, double* d_out) {
     CKR(cos_tables(mem, n1 + 1, &cosx));
     CKR(cos_tables(mem, n2 + 1, &cosy));
     EllJob J{};
-    full_pass3(J);
+    whole_level(J);
     J.nx = n1 + 1; J.ny = n2 + 1; J.lev = 0; J.ngrid = 1; J.in_scale = 1.0;
     J.f = d_in; J.T1 = T; J.T2 = T; J.u = d_out; J.norm = 1.0;
     J.ld1 = n1; J.ld2 = n2;   // unpadded: the second launch writes the caller's array

@@ -72,6 +72,10 @@ struct EllJob {
     // two boundary terms, sin1[k - 1] = 2 sin(pi k / ny) (null: never).
     int sup_lo, sup_hi;
     const double* sin1;
+    // Pass 1 (register-held): in the central region R is read inside this box
+    // only and taken as 0 outside it, where the solve's coarsify chain, run
+    // over the box alone, left other solves' values.
+    int rb_i0, rb_i1, rb_j0, rb_j1;
 };
 
 // Local solve of A u = r, A = 1 + alpha Lap (the force solve's A^-1 of one

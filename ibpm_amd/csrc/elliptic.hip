@@ -807,17 +807,21 @@ __device__ __forceinline__ void pass1_r16(const EllJob& J, double2* smem, int bx
         const IBPM_GLOBAL double* Ra = gin(J.R) + sidx(J.nx, J.ny, ra + 1, 1);
         const IBPM_GLOBAL double* Rb = gin(J.R) + sidx(J.nx, J.ny, hb ? rb + 1 : ra + 1, 1);
         const double keep = hb ? 1.0 : 0.0;
+        const bool ina = ra + 1 >= J.rb_i0 && ra + 1 <= J.rb_i1, inb = rb + 1 >= J.rb_i0 && rb + 1 <= J.rb_i1;
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
             const int e = t + T * r, k = e - 1;
             const bool colc = e > jc0 && e < jc1;
+            const bool colr = e >= J.rb_j0 && e <= J.rb_j1;
             const int kk = e == 0 ? 0 : k;
             // row b absent: read row a, times 0; e = 0, the extension's zero:
-            // times 0 (products, as in load_first_half, so that the sixteen
-            // loads are in flight together)
+            // times 0; R off its box: times 0 (products, as in
+            // load_first_half, so that the sixteen loads are in flight together)
             const double nz = e == 0 ? 0.0 : 1.0;
+            const double ma = (cra && colc && !(ina && colr)) ? 0.0 : 1.0;
+            const double mb = (crb && colc && !(inb && colr)) ? 0.0 : 1.0;
             const double la = ((cra && colc) ? Ra : fa)[kk], lb = ((crb && colc) ? Rb : fb)[kk];
-            v[r] = make_double2(la * sc * nz, lb * sc * keep * nz);
+            v[r] = make_double2(la * sc * nz * ma, lb * sc * keep * nz * mb);
         }
         if (!bcs) return;
         const double c = J.bc_coef;
@@ -1118,7 +1122,7 @@ __global__ void __launch_bounds__(kThreads) k_fs_helm(const HelmJob* __restrict_
     if (skipped(J.skip)) return;
     const int w = J.j1 - J.j0 + 1, h = J.i1 - J.i0 + 1;
     const int p = blockIdx.x * kThreads + threadIdx.x;
-    if (p >= w * h) return;
+    if (w <= 0 || h <= 0 || p >= w * h) return;
     const int i = J.i0 + p / w, j = J.j0 + p % w;
     const int ex = nx_ext(J.nx), ey = nx_ext(J.ny);
     const bool central = J.lev >= 1 && i > ex && i < J.nx / 2 + ex && j > ey && j < J.ny / 2 + ey;
@@ -1149,7 +1153,7 @@ __global__ void __launch_bounds__(kThreads) k_fs_helm_coarsify(const HelmJob* __
     if (J.lev != lev || skipped(J.skip)) return;
     const int w = J.t1 - J.t0 + 1, h = J.s1 - J.s0 + 1;
     const int p = blockIdx.x * kThreads + threadIdx.x;
-    if (p >= w * h) return;
+    if (w <= 0 || h <= 0 || p >= w * h) return;
     const int i = J.s0 + p / w, j = J.t0 + p % w;
     const int ex = nx_ext(J.nx), ey = nx_ext(J.ny);
     const int ii = (i - ex) * 2, jj = (j - ey) * 2;   // Grid::c2f

@@ -198,6 +198,7 @@ static void whole_level(EllJob& J) {
     J.full_lo = 0; J.full_hi = INT_MAX; J.fq0 = J.fq1 = -1;
     J.ring_lo = 1; J.ring_hi = 0; J.rj0 = J.rj1 = 1; J.ringsin = nullptr;
     J.sup_lo = 1; J.sup_hi = INT_MAX; J.sin1 = nullptr;
+    J.rb_i0 = 0; J.rb_i1 = INT_MAX; J.rb_j0 = 0; J.rb_j1 = INT_MAX;
 }
 
 struct PassGroup {
@@ -340,8 +341,9 @@ static void run_coarsify(const SolvePlan& P, hipStream_t s, Probe* pr, const std
 
 // active (probe only): flows whose jobs do work; the others are skipped by
 // their flags, so their bytes are not counted.
+// coarsify = false: the caller has built the coarse right-hand sides.
 static void run_solve(const SolvePlan& P, hipStream_t s, Probe* pr = nullptr,
-                      const std::vector<char>* active = nullptr) {
+                      const std::vector<char>* active = nullptr, bool coarsify = true) {
     auto sum = [&](const std::vector<int>& fl, const std::vector<double>& by, double all) {
         if (!active) return all;
         double b = 0;
@@ -350,7 +352,7 @@ static void run_solve(const SolvePlan& P, hipStream_t s, Probe* pr = nullptr,
         return b;
     };
     const bool fs = P.force_solve;
-    run_coarsify(P, s, pr, active);
+    if (coarsify) run_coarsify(P, s, pr, active);
     for (int st = 0; st < P.nlev; ++st)
         for (const PassGroup& G : P.steps[st]) {
             const double by = pr ? sum(G.jflow, G.jbytes, G.bytes) : 0;
@@ -452,6 +454,13 @@ struct ibpm_batch {
     int helm_cap = 0, helm_n = 0, helm_blocks = 0, helm_cblocks = 0, helm_sweeps = 0;
     std::vector<int> helm_flow;
     std::vector<double> helm_bytes, helm_cbytes;
+    // the L^-1 solve's coarsify chain over its right-hand side's support
+    // (restrict_m2l_pass3), when every pass 1 of that solve is register-held
+    bool m2l_boxed = false;
+    ibpmg::HelmJob* d_m2lc = nullptr;
+    int m2lc_cap = 0, m2lc_n = 0, m2lc_blocks = 0;
+    std::vector<int> m2lc_flow;
+    std::vector<double> m2lc_bytes;
     long st_local_mapps = 0;
     ibpmg::FlowOps* d_fops[4] = {};
     ibpmg::RegOps *d_interp_c = nullptr, *d_interp_m = nullptr, *d_smear_f[4] = {}, *d_smear_d[4] = {};
@@ -861,25 +870,36 @@ static int restrict_m2l_pass3(ibpm_batch* B, const std::vector<std::vector<std::
         CKR(B->mem.alloc(&F.sin1, s1.size()));
         CKR(upload(F.sin1, s1.data(), sizeof(double) * s1.size()));
     }
-    // rows of the right-hand side's support per flow and level
-    std::vector<std::vector<std::array<int, 2>>> rows(B->flows.size());
-    for (size_t k = 0; k < B->flows.size(); ++k) {
+    // per flow and level: rbox, where the coarsify chain puts the right-hand
+    // side's R (coarsify's reach from the finer level's support, inside the
+    // central region; empty on level 0), and sup, the bounding box of the
+    // right-hand side (rbox and the local result's box)
+    const int nfl = (int)B->flows.size();
+    std::vector<std::vector<std::array<int, 4>>> rbox(nfl), sup(nfl);
+    for (int k = 0; k < nfl; ++k) {
         const Flow& F = B->flows[k];
-        const int ex = nx_ext(F.nx);
-        rows[k].resize(F.L);
-        for (int lev = 0; lev < F.L; ++lev) {
-            int lo = F.hbox[lev][0], hi = F.hbox[lev][1];
-            if (lev >= 1) {
-                const int clo = std::max(rows[k][lev - 1][0] / 2 + ex, ex + 1);
-                const int chi = std::min((rows[k][lev - 1][1] + 1) / 2 + ex, ex + F.nx / 2 - 1);
-                if (clo <= chi) {
-                    lo = std::min(lo, clo);
-                    hi = std::max(hi, chi);
-                }
-            }
-            rows[k][lev] = {lo, hi};
+        const int ex = nx_ext(F.nx), ey = nx_ext(F.ny);
+        rbox[k].resize(F.L);
+        sup[k].resize(F.L);
+        rbox[k][0] = {1, 0, 1, 0};
+        sup[k][0] = F.hbox[0];
+        for (int lev = 1; lev < F.L; ++lev) {
+            const std::array<int, 4>& fs = sup[k][lev - 1];
+            std::array<int, 4> r = {std::max(fs[0] / 2 + ex, ex + 1), std::min((fs[1] + 1) / 2 + ex, ex + F.nx / 2 - 1),
+                                    std::max(fs[2] / 2 + ey, ey + 1), std::min((fs[3] + 1) / 2 + ey, ey + F.ny / 2 - 1)};
+            rbox[k][lev] = r;
+            const std::array<int, 4>& h = F.hbox[lev];
+            sup[k][lev] = (r[0] <= r[1] && r[2] <= r[3])
+                              ? std::array<int, 4>{std::min(r[0], h[0]), std::max(r[1], h[1]), std::min(r[2], h[2]),
+                                                   std::max(r[3], h[3])}
+                              : h;
         }
     }
+    // the boxed coarsify chain needs every pass 1 of the solve to mask R
+    bool all_regs1 = true;
+    for (auto& step : B->M2L.steps)
+        for (auto& G : step) all_regs1 = all_regs1 && G.regs[0];
+    B->m2l_boxed = all_regs1;
     for (auto& step : B->M2L.steps)
         for (auto& G : step) {
             bool changed = false;
@@ -889,9 +909,13 @@ static int restrict_m2l_pass3(ibpm_batch* B, const std::vector<std::vector<std::
                 const EllJob before = J;
                 whole_level(J);
                 if (J.epilogue == 0 && !J.vside) {
-                    J.sup_lo = rows[k][J.lev][0];
-                    J.sup_hi = rows[k][J.lev][1];
+                    J.sup_lo = sup[k][J.lev][0];
+                    J.sup_hi = sup[k][J.lev][1];
                     J.sin1 = B->flows[k].sin1;
+                    if (B->m2l_boxed) {
+                        const std::array<int, 4>& r = rbox[k][J.lev];
+                        J.rb_i0 = r[0]; J.rb_i1 = r[1]; J.rb_j0 = r[2]; J.rb_j1 = r[3];
+                    }
                 }
                 if (G.regs[2] && J.epilogue == 0 && !J.vside) {
                     const int N1 = J.nx - 1, ex = nx_ext(J.nx), ey = nx_ext(J.ny);
@@ -911,6 +935,37 @@ static int restrict_m2l_pass3(ibpm_batch* B, const std::vector<std::vector<std::
             }
             if (changed) CKR(upload(G.d, G.h.data(), sizeof(EllJob) * G.h.size()));
         }
+    if (!B->m2l_boxed) return 0;
+    std::vector<HelmJob> cj;
+    B->m2lc_flow.clear();
+    B->m2lc_bytes.clear();
+    long maxa = 1;
+    for (int k = 0; k < nfl; ++k) {
+        const Flow& F = B->flows[k];
+        for (int lev = 1; lev < F.L; ++lev) {
+            const std::array<int, 4>& r = rbox[k][lev];
+            const std::array<int, 4>& fr = rbox[k][lev - 1];
+            HelmJob J{};
+            J.nx = F.nx; J.ny = F.ny; J.lev = lev;
+            J.s0 = r[0]; J.s1 = r[1]; J.t0 = r[2]; J.t1 = r[3];
+            J.fs0 = fr[0]; J.fs1 = fr[1]; J.ft0 = fr[2]; J.ft1 = fr[3];
+            J.f_fine = F.m1 + (lev - 1) * F.ns; J.R_fine = F.R + (lev - 1) * F.ns;
+            J.R = F.R + lev * F.ns;
+            J.skip = F.cg_state + 3;
+            const long area = (r[0] <= r[1] && r[2] <= r[3]) ? (long)(r[1] - r[0] + 1) * (r[3] - r[2] + 1) : 0;
+            maxa = std::max(maxa, area);
+            cj.push_back(J);
+            B->m2lc_flow.push_back(k);
+            B->m2lc_bytes.push_back(40.0 * area);
+        }
+    }
+    if ((int)cj.size() > B->m2lc_cap) {
+        CKR(B->mem.alloc(&B->d_m2lc, cj.size()));
+        B->m2lc_cap = (int)cj.size();
+    }
+    CKR(upload(B->d_m2lc, cj.data(), sizeof(HelmJob) * cj.size()));
+    B->m2lc_n = (int)cj.size();
+    B->m2lc_blocks = (int)((maxa + kThreads - 1) / kThreads);
     return 0;
 }
 
@@ -1049,7 +1104,16 @@ static void enqueue_M(ibpm_batch* B, int i, RegOps* smear, hipStream_t s,
         for (int sw = 0; sw < B->helm_sweeps; ++sw)
             timed(pr, K_FSHELM, by, s,
                   [&] { launch_helm_sweep(B->d_helm, B->helm_n, B->helm_blocks, sw, B->helm_sweeps, s); });
-        run_solve(B->M2L, s, pr, active);
+        if (B->m2l_boxed) {
+            double cb = 0;
+            if (pr)
+                for (size_t j = 0; j < B->m2lc_flow.size(); ++j)
+                    if (!active || (*active)[B->m2lc_flow[j]]) cb += B->m2lc_bytes[j];
+            for (int lev = 1; lev < B->M2L.nlev; ++lev)
+                timed(pr, K_FSCOARSIFY, cb / std::max(1, B->M2L.nlev - 1), s,
+                      [&] { launch_helm_coarsify(B->d_m2lc, B->m2lc_n, B->m2lc_blocks, lev, s); });
+        }
+        run_solve(B->M2L, s, pr, active, !B->m2l_boxed);
         B->st_local_mapps += 1;
     } else {
         run_solve(B->M1[i], s, pr, active);

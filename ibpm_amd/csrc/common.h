@@ -76,6 +76,9 @@ struct EllJob {
     // only and taken as 0 outside it, where the solve's coarsify chain, run
     // over the box alone, left other solves' values.
     int rb_i0, rb_i1, rb_j0, rb_j1;
+    // Pass 2 (register-held): rows of T2 (0-based i) that pass 3 reads; the
+    // others are not written.
+    int t2_lo, t2_hi;
 };
 
 // Local solve of A u = r, A = 1 + alpha Lap (the force solve's A^-1 of one

@@ -723,10 +723,11 @@ __device__ __forceinline__ void stage_pair(double2* st, int k, int g, double a, 
 // 16 G bytes).
 template <int M, int G>
 __device__ __forceinline__ void write_staged(IBPM_GLOBAL double* out, const double2* st, int nrows, int ld,
-                                             int row0) {
+                                             int row0, int k_lo = 0, int k_hi = M) {
     constexpr int N = M / 2 - 1, NT = G * (M / 16);
     for (int idx = threadIdx.x; idx < N * G; idx += NT) {
         const int k = idx / G, slot = idx % G;
+        if (k < k_lo || k > k_hi) continue;   // columns k outside [k_lo, k_hi] are not read
         const int g = slot ^ ((k / (16 / G)) & (G - 1));
         const int row = row0 + 2 * g;
         if (row < nrows) {
@@ -907,7 +908,7 @@ __device__ __forceinline__ void pass2_r16(const EllJob& J, double2* smem, int bx
         if (kk > 0) stage_pair<G>(smem, kk - 1, g, -z.y, z.x);
     });
     __syncthreads();
-    write_staged<M, G>(gout(J.T2), smem, N2, J.ld2, 2 * G * bx);
+    write_staged<M, G>(gout(J.T2), smem, N2, J.ld2, 2 * G * bx, J.t2_lo, J.t2_hi);
 }
 
 // Pass 3 of a row pair that is not transformed: u at columns rj0, rj1 of its

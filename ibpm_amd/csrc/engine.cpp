@@ -199,6 +199,7 @@ static void whole_level(EllJob& J) {
     J.ring_lo = 1; J.ring_hi = 0; J.rj0 = J.rj1 = 1; J.ringsin = nullptr;
     J.sup_lo = 1; J.sup_hi = INT_MAX; J.sin1 = nullptr;
     J.rb_i0 = 0; J.rb_i1 = INT_MAX; J.rb_j0 = 0; J.rb_j1 = INT_MAX;
+    J.t2_lo = 0; J.t2_hi = INT_MAX;
 }
 
 struct PassGroup {
@@ -923,12 +924,20 @@ static int restrict_m2l_pass3(ibpm_batch* B, const std::vector<std::vector<std::
                         const std::array<int, 4>& sp = nsup[k][0];   // node rows, 1-based
                         J.full_lo = std::max(sp[0] - 2, 0) / 2;
                         J.full_hi = std::min(sp[1], N1 - 1) / 2;
+                        J.t2_lo = 2 * J.full_lo;
+                        J.t2_hi = 2 * J.full_hi + 1;
                     } else {
                         J.full_lo = 1; J.full_hi = 0;
                         J.fq0 = (ex - 1) / 2; J.fq1 = (ex + J.nx / 2 - 1) / 2;
                         J.ring_lo = ex + 1; J.ring_hi = ex + J.nx / 2 - 1;
                         J.rj0 = ey; J.rj1 = ey + J.ny / 2;
                         J.ringsin = B->flows[k].ringsin;
+                        J.t2_lo = 2 * J.fq0;
+                        J.t2_hi = 2 * J.fq1 + 1;
+                    }
+                    if (!G.regs[1]) {   // an LDS pass 2 writes every row
+                        J.t2_lo = 0;
+                        J.t2_hi = INT_MAX;
                     }
                 }
                 if (std::memcmp(&before, &J, sizeof(EllJob)) != 0) changed = true;

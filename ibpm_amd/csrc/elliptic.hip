@@ -542,7 +542,7 @@ __device__ inline void load_twiddles(double2* tw, const FftPlan& P) {
 
 
 template <int CM>
-__device__ inline void pass1_lds(const EllJob& J, double2* smem, int bx) {
+__device__ __forceinline__ void pass1_lds(const EllJob& J, double2* smem, int bx) {
     const FftPlan& P = J.pj;
     const int M = CM ? CM : P.M, np = CM ? np_of(CM) : P.np;
     const int N1 = J.nx - 1, N2 = M / 2 - 1;
@@ -577,7 +577,7 @@ __device__ inline void pass1_lds(const EllJob& J, double2* smem, int bx) {
 }
 
 template <int CM>
-__device__ inline void pass2_lds(const EllJob& J, double2* smem, int bx) {
+__device__ __forceinline__ void pass2_lds(const EllJob& J, double2* smem, int bx) {
     const FftPlan& P = J.pi;
     const int M = CM ? CM : P.M, np = CM ? np_of(CM) : P.np;
     const int N1 = M / 2 - 1, N2 = J.ny - 1;
@@ -636,7 +636,7 @@ __device__ inline void pass2_lds(const EllJob& J, double2* smem, int bx) {
 }
 
 template <int CM>
-__device__ inline void pass3_lds(const EllJob& J, double2* smem, int bx) {
+__device__ __forceinline__ void pass3_lds(const EllJob& J, double2* smem, int bx) {
     const FftPlan& P = J.pj;
     const int M = CM ? CM : P.M, np = CM ? np_of(CM) : P.np;
     const int N1 = J.nx - 1, N2 = M / 2 - 1;
@@ -771,7 +771,7 @@ __device__ __forceinline__ void pass1_bc_rows(const EllJob& J, double2* smem, in
     }
 }
 
-template <int M>
+template <int M, bool FS>
 __device__ __forceinline__ void pass1_r16(const EllJob& J, double2* smem, int bx) {
     constexpr int T = M / 16, G = r16_group(1);
     const int N1 = J.nx - 1, npairs = (N1 + 1) / 2;
@@ -819,10 +819,14 @@ __device__ __forceinline__ void pass1_r16(const EllJob& J, double2* smem, int bx
             // times 0; R off its box: times 0 (products, as in
             // load_first_half, so that the sixteen loads are in flight together)
             const double nz = e == 0 ? 0.0 : 1.0;
-            const double ma = (cra && colc && !(ina && colr)) ? 0.0 : 1.0;
-            const double mb = (crb && colc && !(inb && colr)) ? 0.0 : 1.0;
             const double la = ((cra && colc) ? Ra : fa)[kk], lb = ((crb && colc) ? Rb : fb)[kk];
-            v[r] = make_double2(la * sc * nz * ma, lb * sc * keep * nz * mb);
+            if constexpr (FS) {
+                const double ma = (cra && colc && !(ina && colr)) ? 0.0 : 1.0;
+                const double mb = (crb && colc && !(inb && colr)) ? 0.0 : 1.0;
+                v[r] = make_double2(la * sc * nz * ma, lb * sc * keep * nz * mb);
+            } else {
+                v[r] = make_double2(la * sc * nz, lb * sc * keep * nz);
+            }
         }
         if (!bcs) return;
         const double c = J.bc_coef;
@@ -1006,15 +1010,17 @@ __device__ inline int2 xcd_block() {
 
 // Pass bodies: the register-held transform (REG, M = 1024 or 2048) or the LDS
 // passes.
-#define IBPM_PASS_BODY(N)                                                                  \
-    template <int CM, bool REG>                                                            \
+// FS: an instance for the force solve, whose register-held pass 1 also masks
+// R off its box (EllJob::rb_*); the step's own solves never restrict it.
+#define IBPM_PASS_BODY(N, R16)                                                             \
+    template <int CM, bool REG, bool FS>                                                   \
     __device__ __forceinline__ void pass##N##_body(const EllJob& J, double2* smem, int bx) { \
-        if constexpr (REG) pass##N##_r16<CM>(J, smem, bx);                                 \
+        if constexpr (REG) R16;                                                            \
         else pass##N##_lds<CM>(J, smem, bx);                                               \
     }
-IBPM_PASS_BODY(1)
-IBPM_PASS_BODY(2)
-IBPM_PASS_BODY(3)
+IBPM_PASS_BODY(1, (pass1_r16<CM, FS>(J, smem, bx)))
+IBPM_PASS_BODY(2, (pass2_r16<CM>(J, smem, bx)))
+IBPM_PASS_BODY(3, (pass3_r16<CM>(J, smem, bx)))
 
 // One kernel per pass and FFT length (compiled lengths, 0 = runtime plan), so
 // each length gets its own register budget; a launch holds jobs of one length.
@@ -1022,7 +1028,7 @@ IBPM_PASS_BODY(3)
 // (M applications) use k_fs_pass*, the same code under another name so that
 // profiles separate the two.  A job whose flow has finished its force solve
 // does nothing (skipped()).
-#define IBPM_PASS_KERNEL(NAME, BODY, PASS)                                       \
+#define IBPM_PASS_KERNEL(NAME, BODY, PASS, FS)                                   \
     template <int CM, bool REG>                                                  \
     __global__ void __launch_bounds__((pass_threads<CM, REG, PASS>())) NAME(const EllJob* __restrict__ jobs) { \
         static_assert(!REG || r16_len<CM>(), "no register-held transform at this length"); \
@@ -1030,14 +1036,14 @@ IBPM_PASS_BODY(3)
         const int2 blk = xcd_block();                                             \
         const EllJob& J = jobs[blk.y];                                            \
         if (skipped(J.skip)) return;                                              \
-        BODY<CM, REG>(J, smem, blk.x);                                            \
+        BODY<CM, REG, FS>(J, smem, blk.x);                                        \
     }
-IBPM_PASS_KERNEL(k_ell_pass1, pass1_body, 1)
-IBPM_PASS_KERNEL(k_ell_pass2, pass2_body, 2)
-IBPM_PASS_KERNEL(k_ell_pass3, pass3_body, 3)
-IBPM_PASS_KERNEL(k_fs_pass1, pass1_body, 1)
-IBPM_PASS_KERNEL(k_fs_pass2, pass2_body, 2)
-IBPM_PASS_KERNEL(k_fs_pass3, pass3_body, 3)
+IBPM_PASS_KERNEL(k_ell_pass1, pass1_body, 1, false)
+IBPM_PASS_KERNEL(k_ell_pass2, pass2_body, 2, false)
+IBPM_PASS_KERNEL(k_ell_pass3, pass3_body, 3, false)
+IBPM_PASS_KERNEL(k_fs_pass1, pass1_body, 1, true)
+IBPM_PASS_KERNEL(k_fs_pass2, pass2_body, 2, true)
+IBPM_PASS_KERNEL(k_fs_pass3, pass3_body, 3, true)
 
 // --------------------------------------------------------------- coarsify
 // Scalar::coarsify (reference Scalar.cc:46-63) for one level, writing only

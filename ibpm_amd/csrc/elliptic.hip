@@ -17,6 +17,8 @@
 // staged through registers so one LDS buffer suffices.
 #include "ops.h"
 
+#include <algorithm>
+
 namespace ibpmg {
 
 __device__ inline double2 cmul(double2 a, double2 b) {
@@ -1114,40 +1116,83 @@ __global__ void __launch_bounds__(kThreads) k_fs_coarsify(const CoarsifyJob* __r
 }
 
 // ------------------------------------------------- local Helmholtz sweeps
-// One Jacobi sweep of A u = r, A = 1 + alpha Lap, over a job's box, for the
+// Jacobi sweeps of A u = r, A = 1 + alpha Lap, over a job's box, for the
 // force solve's A^-1.  With |alpha| / dx^2 small the solution decays by a
 // factor 4|c| / (1 - 4c) per cell away from the right-hand side, so a box
 // around the body's smeared force holds it to rounding and the host chooses
 // the sweep count for an error below 1e-18 of the solution; it uses the DST
-// solve wherever the box would reach a level's boundary ring.  Sweep 0 starts
-// from u = 0; sweeps alternate between tmp and out so that the last writes
-// out.  Points off the box are never written in either buffer (the boxes of
-// a flow only grow), so they read as 0.  The right-hand side is rhs_value's
-// without boundary terms, which the box keeps away from.
-__global__ void __launch_bounds__(kThreads) k_fs_helm(const HelmJob* __restrict__ jobs, int sweep, int nsweeps) {
+// solve wherever the box would reach a level's boundary ring.  The first
+// sweep starts from u = 0, u is 0 off the box, and the right-hand side is
+// rhs_value's without boundary terms, which the box keeps away from.
+//
+// One launch runs up to kHelmDepth sweeps on tiles of kHelmTile^2 points:
+// a workgroup loads its tile with a kHelmDepth-point halo into LDS and
+// sweeps there, the valid region shrinking by one point a sweep, then writes
+// the tile.  Each point's update is the same arithmetic in the same order as
+// a sweep over global memory, so the result does not depend on the blocking.
+// Launches alternate between tmp and out so that the last writes out.
+constexpr int kHelmTile = 32, kHelmDepth = 4, kHelmReg = kHelmTile + 2 * kHelmDepth;
+
+__global__ void __launch_bounds__(kThreads) k_fs_helm(const HelmJob* __restrict__ jobs, int sweep0, int nsw,
+                                                     bool to_out) {
     const HelmJob& J = jobs[blockIdx.y];
     if (skipped(J.skip)) return;
     const int w = J.j1 - J.j0 + 1, h = J.i1 - J.i0 + 1;
-    const int p = blockIdx.x * kThreads + threadIdx.x;
-    if (w <= 0 || h <= 0 || p >= w * h) return;
-    const int i = J.i0 + p / w, j = J.j0 + p % w;
+    if (w <= 0 || h <= 0) return;
+    const int tw = (w + kHelmTile - 1) / kHelmTile, th = (h + kHelmTile - 1) / kHelmTile;
+    if ((int)blockIdx.x >= tw * th) return;
+    const int ti0 = J.i0 + (blockIdx.x / tw) * kHelmTile - kHelmDepth;   // region origin
+    const int tj0 = J.j0 + (blockIdx.x % tw) * kHelmTile - kHelmDepth;
+    __shared__ double rr[kHelmReg * kHelmReg], ua[kHelmReg * kHelmReg], ub[kHelmReg * kHelmReg];
     const int ex = nx_ext(J.nx), ey = nx_ext(J.ny);
-    const bool central = J.lev >= 1 && i > ex && i < J.nx / 2 + ex && j > ey && j < J.ny / 2 + ey;
-    const bool support = i >= J.s0 && i <= J.s1 && j >= J.t0 && j <= J.t1;
-    const long k = sidx(J.nx, J.ny, i, j);
-    const double r = support ? (central ? gin(J.R)[k] : gin(J.f)[k]) * J.in_scale : 0.0;
-    const bool to_out = ((nsweeps - 1 - sweep) & 1) == 0;
-    IBPM_GLOBAL double* dst = gout(to_out ? J.out : J.tmp);
-    double v;
-    if (sweep == 0) {
-        v = r * J.inv_diag;
-    } else {
-        const IBPM_GLOBAL double* src = gin(to_out ? J.tmp : J.out);
-        const long ny1 = J.ny - 1;
-        const double s4 = (src[k - ny1] + src[k + ny1]) + (src[k - 1] + src[k + 1]);
-        v = (r - J.c * s4) * J.inv_diag;
+    const IBPM_GLOBAL double* src = gin(to_out ? J.tmp : J.out);
+    for (int q = threadIdx.x; q < kHelmReg * kHelmReg; q += kThreads) {
+        const int i = ti0 + q / kHelmReg, j = tj0 + q % kHelmReg;
+        const bool inbox = i >= J.i0 && i <= J.i1 && j >= J.j0 && j <= J.j1;
+        double r = 0.0, u = 0.0;
+        if (inbox) {
+            const long k = sidx(J.nx, J.ny, i, j);
+            const bool central = J.lev >= 1 && i > ex && i < J.nx / 2 + ex && j > ey && j < J.ny / 2 + ey;
+            const bool support = i >= J.s0 && i <= J.s1 && j >= J.t0 && j <= J.t1;
+            r = support ? (central ? gin(J.R)[k] : gin(J.f)[k]) * J.in_scale : 0.0;
+            if (sweep0 > 0) u = src[k];
+        }
+        rr[q] = r;
+        ua[q] = u;
     }
-    dst[k] = v;
+    __syncthreads();
+    double* cur = ua;
+    double* nxt = ub;
+    for (int d = 0; d < nsw; ++d) {
+        const int m = d + 1;   // points within m of the region's edge are no longer valid
+        for (int q = threadIdx.x; q < kHelmReg * kHelmReg; q += kThreads) {
+            const int a = q / kHelmReg, b = q % kHelmReg;
+            if (a < m || b < m || a >= kHelmReg - m || b >= kHelmReg - m) continue;
+            const int i = ti0 + a, j = tj0 + b;
+            const bool inbox = i >= J.i0 && i <= J.i1 && j >= J.j0 && j <= J.j1;
+            double v = 0.0;
+            if (inbox) {
+                if (sweep0 + d == 0) {
+                    v = rr[q] * J.inv_diag;
+                } else {
+                    const double s4 = (cur[q - kHelmReg] + cur[q + kHelmReg]) + (cur[q - 1] + cur[q + 1]);
+                    v = (rr[q] - J.c * s4) * J.inv_diag;
+                }
+            }
+            nxt[q] = v;
+        }
+        __syncthreads();
+        double* t = cur;
+        cur = nxt;
+        nxt = t;
+    }
+    IBPM_GLOBAL double* dst = gout(to_out ? J.out : J.tmp);
+    for (int q = threadIdx.x; q < kHelmTile * kHelmTile; q += kThreads) {
+        const int a = kHelmDepth + q / kHelmTile, b = kHelmDepth + q % kHelmTile;
+        const int i = ti0 + a, j = tj0 + b;
+        if (i > J.i1 || j > J.j1) continue;
+        dst[sidx(J.nx, J.ny, i, j)] = cur[a * kHelmReg + b];
+    }
 }
 
 // The coarsify chain of the local solves' right-hand side, level lev of every
@@ -1237,8 +1282,18 @@ void launch_helm_coarsify(const HelmJob* d_jobs, int njobs, int max_blocks, int 
     hipLaunchKernelGGL(k_fs_helm_coarsify, dim3(max_blocks, njobs), dim3(kThreads), 0, s, d_jobs, lev);
 }
 
-void launch_helm_sweep(const HelmJob* d_jobs, int njobs, int max_blocks, int sweep, int nsweeps, hipStream_t s) {
-    hipLaunchKernelGGL(k_fs_helm, dim3(max_blocks, njobs), dim3(kThreads), 0, s, d_jobs, sweep, nsweeps);
+int helm_tiles(int h, int w) { return ((h + kHelmTile - 1) / kHelmTile) * ((w + kHelmTile - 1) / kHelmTile); }
+
+void launch_helm_sweeps(const HelmJob* d_jobs, int njobs, int max_tiles, int nsweeps, hipStream_t s,
+                        const std::function<void(const std::function<void()>&)>& timed_launch) {
+    const int nl = (nsweeps + kHelmDepth - 1) / kHelmDepth;
+    for (int n = 0; n < nl; ++n) {
+        const int s0 = n * kHelmDepth, nsw = std::min(kHelmDepth, nsweeps - s0);
+        const bool to_out = ((nl - 1 - n) & 1) == 0;
+        timed_launch([&] {
+            hipLaunchKernelGGL(k_fs_helm, dim3(max_tiles, njobs), dim3(kThreads), 0, s, d_jobs, s0, nsw, to_out);
+        });
+    }
 }
 
 void launch_coarsify(const CoarsifyJob* d_jobs, int njobs, int max_rows, hipStream_t s, bool force_solve) {

@@ -15,6 +15,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <string>
@@ -1060,10 +1061,10 @@ static int plan_local_helm(ibpm_batch* B, int i) {
             J.out = F.m1 + lev * F.ns; J.tmp = F.m1tmp + lev * F.ns;
             J.skip = F.cg_state + 3;
             const long area = (long)(b[1] - b[0] + 1) * (b[3] - b[2] + 1);
-            maxarea = std::max(maxarea, area);
+            maxarea = std::max(maxarea, (long)helm_tiles(b[1] - b[0] + 1, b[3] - b[2] + 1));
             jobs.push_back(J);
             jflow.push_back(k);
-            jbytes.push_back(24.0 * area);   // right-hand side and one neighbour stream in, result out
+            jbytes.push_back(24.0 * area * std::max(1, sweeps));   // per sweep: r and u in, u out
         }
     }
     if (!B->m2l_built) {   // L^-1 of the local result (the DST solve M2 reads mt instead)
@@ -1083,7 +1084,7 @@ static int plan_local_helm(ibpm_batch* B, int i) {
     }
     CKR(upload(B->d_helm, jobs.data(), sizeof(HelmJob) * jobs.size()));
     B->helm_n = (int)jobs.size();
-    B->helm_blocks = (int)((maxarea + kThreads - 1) / kThreads);
+    B->helm_blocks = (int)maxarea;   // tiles of the largest box
     B->helm_cblocks = (int)((maxsup + kThreads - 1) / kThreads);
     B->helm_sweeps = sweeps;
     B->helm_flow = jflow;
@@ -1110,9 +1111,9 @@ static void enqueue_M(ibpm_batch* B, int i, RegOps* smear, hipStream_t s,
         for (int lev = 1; lev < B->M1[i].nlev; ++lev)
             timed(pr, K_FSCOARSIFY, cby / std::max(1, B->M1[i].nlev - 1), s,
                   [&] { launch_helm_coarsify(B->d_helm, B->helm_n, B->helm_cblocks, lev, s); });
-        for (int sw = 0; sw < B->helm_sweeps; ++sw)
-            timed(pr, K_FSHELM, by, s,
-                  [&] { launch_helm_sweep(B->d_helm, B->helm_n, B->helm_blocks, sw, B->helm_sweeps, s); });
+        const int nl = (B->helm_sweeps + 3) / 4;   // launches of up to 4 sweeps
+        launch_helm_sweeps(B->d_helm, B->helm_n, B->helm_blocks, B->helm_sweeps, s,
+                           [&](const std::function<void()>& fn) { timed(pr, K_FSHELM, by / nl, s, fn); });
         if (B->m2l_boxed) {
             double cb = 0;
             if (pr)

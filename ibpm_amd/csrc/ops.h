@@ -1,5 +1,7 @@
 // ops.h -- per-flow descriptors and launchers of the projection-step kernels.
 #pragma once
+
+#include <functional>
 #include "common.h"
 
 namespace ibpmg {
@@ -109,10 +111,13 @@ int lds_for_pass(const EllJob& J, int pass, bool regs);
 void launch_ell_pass(int pass, const EllJob* d_jobs, int njobs, int max_blocks, int lds_bytes, hipStream_t s,
                      bool force_solve, int M, bool regs);
 void launch_coarsify(const CoarsifyJob* d_jobs, int njobs, int max_rows, hipStream_t s, bool force_solve = false);
-// Sweep `sweep` of `nsweeps` of the local Helmholtz solves (HelmJob); the last
-// sweep writes each job's out.  max_blocks: kThreads-point blocks of the
-// largest box.
-void launch_helm_sweep(const HelmJob* d_jobs, int njobs, int max_blocks, int sweep, int nsweeps, hipStream_t s);
+// The nsweeps Jacobi sweeps of the local Helmholtz solves (HelmJob), several
+// per launch; the last launch writes each job's out.  max_tiles: helm_tiles
+// of the largest box.  timed_launch(fn) runs fn, the launch (a caller may
+// time it).
+int helm_tiles(int h, int w);
+void launch_helm_sweeps(const HelmJob* d_jobs, int njobs, int max_tiles, int nsweeps, hipStream_t s,
+                        const std::function<void(const std::function<void()>&)>& timed_launch);
 // The right-hand side's coarsify chain at level lev (>= 1) over each job's
 // support (HelmJob); max_blocks: kThreads-point blocks of the largest support.
 void launch_helm_coarsify(const HelmJob* d_jobs, int njobs, int max_blocks, int lev, hipStream_t s);

@@ -51,6 +51,7 @@ struct EllJob {
     double* T2;              // scratch (nx-1) rows of ny-1 values, row stride ld2
     int ld1, ld2;            // >= nx-1, ny-1; padded (t1_ld) where the scratch is the library's own
     double* u;               // level slice of the solution
+    double* u2;              // null, or a second output of the same values (pass 3)
     const double* W;         // epilogue 1 operand (level slice)
     double* vside;           // epilogue solves, levels >= 1: v (before the epilogue) stored at
                              // the points the next finer level's getBC reads (level slice)
@@ -98,6 +99,8 @@ struct HelmJob {
     double* out;             // level slice written by the last sweep
     double* tmp;             // level slice of the other sweep buffer
     const double* skip;      // null, or the flow's solver flags
+    double* eo;              // epilogue (launch_helm_epilogue): eo = ei - out over the box
+    const double* ei;
 };
 
 // Coarsify job: one level (lev >= 1) of one flow (reference Scalar.cc:46-63).

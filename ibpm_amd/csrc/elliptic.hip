@@ -673,13 +673,17 @@ __device__ __forceinline__ void pass3_lds(const EllJob& J, double2* smem, int bx
         if (ra < N1) {
             const long o = (long)ra * N2 + k;
             const double v = -z.y * J.norm;
-            gout(J.u)[o] = J.epilogue ? gin(J.W)[o] - v : v;
+            const double uv = J.epilogue ? gin(J.W)[o] - v : v;
+            gout(J.u)[o] = uv;
+            if (J.u2) gout(J.u2)[o] = uv;
             if (J.vside && on_bc_ring(J, ra + 1, k + 1)) gout(J.vside)[o] = v;
         }
         if (rb < N1) {
             const long o = (long)rb * N2 + k;
             const double v = z.x * J.norm;
-            gout(J.u)[o] = J.epilogue ? gin(J.W)[o] - v : v;
+            const double uv = J.epilogue ? gin(J.W)[o] - v : v;
+            gout(J.u)[o] = uv;
+            if (J.u2) gout(J.u2)[o] = uv;
             if (J.vside && on_bc_ring(J, rb + 1, k + 1)) gout(J.vside)[o] = v;
         }
     }
@@ -984,13 +988,17 @@ __device__ __forceinline__ void pass3_r16(const EllJob& J, double2* smem, int bx
         {
             const long o2 = (long)ra * N2 + k;
             const double val_a = -z.y * J.norm;
-            gout(J.u)[o2] = J.epilogue ? gin(J.W)[o2] - val_a : val_a;
+            const double ua = J.epilogue ? gin(J.W)[o2] - val_a : val_a;
+            gout(J.u)[o2] = ua;
+            if (J.u2) gout(J.u2)[o2] = ua;
             if (J.vside && on_bc_ring(J, ra + 1, k + 1)) gout(J.vside)[o2] = val_a;
         }
         if (hb) {
             const long o2 = (long)rb * N2 + k;
             const double val_b = z.x * J.norm;
-            gout(J.u)[o2] = J.epilogue ? gin(J.W)[o2] - val_b : val_b;
+            const double ub = J.epilogue ? gin(J.W)[o2] - val_b : val_b;
+            gout(J.u)[o2] = ub;
+            if (J.u2) gout(J.u2)[o2] = ub;
             if (J.vside && on_bc_ring(J, rb + 1, k + 1)) gout(J.vside)[o2] = val_b;
         }
     });
@@ -1195,6 +1203,18 @@ __global__ void __launch_bounds__(kThreads) k_fs_helm(const HelmJob* __restrict_
     }
 }
 
+// u = W - v over each job's box, v the local result (J.out): the epilogue of
+// the step's omega = omega* - A^-1 c, whose omega already holds omega* off the
+// boxes (the omega* solve writes both).
+__global__ void __launch_bounds__(kThreads) k_helm_epilogue(const HelmJob* __restrict__ jobs) {
+    const HelmJob& J = jobs[blockIdx.y];
+    const int w = J.j1 - J.j0 + 1, h = J.i1 - J.i0 + 1;
+    const int p = blockIdx.x * kThreads + threadIdx.x;
+    if (w <= 0 || h <= 0 || p >= w * h) return;
+    const long k = sidx(J.nx, J.ny, J.i0 + p / w, J.j0 + p % w);
+    gout(J.eo)[k] = gin(J.ei)[k] - gin(J.out)[k];
+}
+
 // The coarsify chain of the local solves' right-hand side, level lev of every
 // job, over the job's support only: Scalar::coarsify's weights and order (as
 // coarsify_body) with the finer level's value taken as 0 in its central
@@ -1280,6 +1300,25 @@ void launch_ell_pass(int pass, const EllJob* d_jobs, int njobs, int max_blocks, 
 
 void launch_helm_coarsify(const HelmJob* d_jobs, int njobs, int max_blocks, int lev, hipStream_t s) {
     hipLaunchKernelGGL(k_fs_helm_coarsify, dim3(max_blocks, njobs), dim3(kThreads), 0, s, d_jobs, lev);
+}
+
+// out = tmp = 0 over each job's box.
+__global__ void __launch_bounds__(kThreads) k_helm_clear(const HelmJob* __restrict__ jobs) {
+    const HelmJob& J = jobs[blockIdx.y];
+    const int w = J.j1 - J.j0 + 1, h = J.i1 - J.i0 + 1;
+    const int p = blockIdx.x * kThreads + threadIdx.x;
+    if (w <= 0 || h <= 0 || p >= w * h) return;
+    const long k = sidx(J.nx, J.ny, J.i0 + p / w, J.j0 + p % w);
+    gout(J.out)[k] = 0.0;
+    gout(J.tmp)[k] = 0.0;
+}
+
+void launch_helm_clear(const HelmJob* d_jobs, int njobs, int max_blocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_helm_clear, dim3(max_blocks, njobs), dim3(kThreads), 0, s, d_jobs);
+}
+
+void launch_helm_epilogue(const HelmJob* d_jobs, int njobs, int max_blocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_helm_epilogue, dim3(max_blocks, njobs), dim3(kThreads), 0, s, d_jobs);
 }
 
 int helm_tiles(int h, int w) { return ((h + kHelmTile - 1) / kHelmTile) * ((w + kHelmTile - 1) / kHelmTile); }

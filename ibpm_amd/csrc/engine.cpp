@@ -153,6 +153,7 @@ struct SolveSpec {
     const double* bcring;  // single-level explicit boundary values (L == 1 only)
     const double* skip;    // null, or the flow's force-solve flags
     double* vside;         // epilogue solves: side buffer (L levels) for the BC values of v
+    double* u2 = nullptr;  // null, or a second output (L levels) of the same values
 };
 
 // Optional per-launch timing: a HIP event pair around every kernel launch on
@@ -161,11 +162,11 @@ struct SolveSpec {
 // solve (kernels k_fs_*), counted with the bytes of the flows still iterating.
 enum KClass { K_PASS1, K_PASS2, K_PASS3, K_COARSIFY, K_VEL, K_CROSS, K_RHS, K_CURLQ, K_INTERP,
               K_SMEAR, K_MINV, K_CG, K_FORCES, K_CLEAR, K_FSPASS1, K_FSPASS2, K_FSPASS3, K_FSCOARSIFY,
-              K_FSHELM, K_NCLASS };
+              K_FSHELM, K_HELM, K_NCLASS };
 static const char* kclass_names[K_NCLASS] = {"ell_pass1", "ell_pass2", "ell_pass3", "coarsify", "vel",
                                              "cross", "rhs", "curl_q", "interp", "smear_curl", "minv",
                                              "cg", "forces", "clear_nodes", "fs_pass1", "fs_pass2",
-                                             "fs_pass3", "fs_coarsify", "fs_helm"};
+                                             "fs_pass3", "fs_coarsify", "fs_helm", "helm"};
 struct Probe {
     struct Rec { int cls; hipEvent_t a, b; double bytes; };
     std::vector<Rec> recs;
@@ -285,6 +286,7 @@ static int build_solve_plan(Ctx* c, DevMem& mem, const std::vector<SolveSpec>& s
             J.T1 = s.T1; J.T2 = s.T2;
             J.ld1 = t1_ld(s.nx - 1); J.ld2 = t1_ld(s.ny - 1);   // the scratch is sized for the padding
             J.u = s.u + lev * ns;
+            J.u2 = s.u2 ? s.u2 + lev * ns : nullptr;
             J.W = s.W ? s.W + lev * ns : nullptr;
             J.cosx = s.cosx; J.cosy = s.cosy;
             J.skip = s.skip;
@@ -407,7 +409,7 @@ struct Flow {
     int reg_old_n = 0;
     HostStage stage;
     // local Helmholtz path of the force solve's A^-1: result and sweep buffer
-    // (all levels) and, per level, the union of the boxes swept so far
+    // (all levels) and, per level, the box they were last swept over (zero off it)
     double *m1 = nullptr, *m1tmp = nullptr;
     double* ringsin = nullptr;     // 2 sin(pi l j / ny) at the ring columns (restrict_m2l_pass3)
     double* sin1 = nullptr;        // 2 sin(pi k / ny), k = 1..ny-1
@@ -453,7 +455,10 @@ struct ibpm_batch {
     bool helm_allowed = true, helm_on[4] = {false, false, false, false}, m2l_built = false;
     ibpmg::SolvePlan M2L;
     ibpmg::HelmJob* d_helm = nullptr;
-    int helm_cap = 0, helm_n = 0, helm_blocks = 0, helm_cblocks = 0, helm_sweeps = 0;
+    ibpmg::HelmJob* d_helm3 = nullptr;   // the same solves for omega = omega* - A^-1 c (no skip flags)
+    ibpmg::HelmJob* d_hclear = nullptr;  // boxes to clear when a flow's boxes change
+    int hclear_cap = 0;
+    int helm_cap = 0, helm_n = 0, helm_blocks = 0, helm_cblocks = 0, helm_ablocks = 0, helm_sweeps = 0;
     std::vector<int> helm_flow;
     std::vector<double> helm_bytes, helm_cbytes;
     // the L^-1 solve's coarsify chain over its right-hand side's support
@@ -755,6 +760,9 @@ static int build_plans(ibpm_batch* B) {
         for (auto& F : B->flows) {
             const double alpha = -beta_of(B, F, i) / 2. * (1. / F.p.reynolds);   // ProjectionSolver.cc:33
             s1.push_back(spec_of(F, 1, alpha, F.a, F.wstar, 1.0, 0, nullptr));
+            // moving-body batches: omega = omega* too, for the local form of
+            // omega = omega* - A^-1 c (plan_local_helm)
+            if (B->any_cg) s1.back().u2 = F.omega;
             s2.push_back(spec_of(F, 0, 0.0, F.wstar, F.psi, -1.0, 0, nullptr));
             s3.push_back(spec_of(F, 1, alpha, F.c, F.omega, 1.0, 1, F.wstar));
             s3.back().vside = F.mt;   // M-temp is free after the force solve
@@ -1016,19 +1024,15 @@ static int plan_local_helm(ibpm_batch* B, int i) {
             std::array<int, 4> b = {s0 - S, s1 + S, t0 - S, t1 + S};
             if (b[0] < 2 || b[1] > F.nx - 2 || b[2] < 2 || b[3] > F.ny - 2) return 0;
             if (lev >= 1 && !(b[0] > ex && b[1] < ex + F.nx / 2 && b[2] > ey && b[3] < ey + F.ny / 2)) return 0;
-            if (lev < (int)F.hbox.size()) {   // the boxes of a flow only grow
-                const std::array<int, 4>& o = F.hbox[lev];
-                b = {std::min(b[0], o[0]), std::max(b[1], o[1]), std::min(b[2], o[2]), std::max(b[3], o[3])};
-            }
             nbox[k][lev] = b;
             nsup[k][lev] = {s0, s1, t0, t1};
             sweeps = std::max(sweeps, S);
         }
     }
-    std::vector<HelmJob> jobs;
+    std::vector<HelmJob> jobs, jobs3, clears;
     std::vector<int> jflow;
     std::vector<double> jbytes, cbytes;
-    long maxarea = 0, maxsup = 0;
+    long maxarea = 0, maxsup = 0, maxpts = 1;
     for (int k = 0; k < nfl; ++k) {
         Flow& F = B->flows[k];
         const long ns = F.ns * F.L;
@@ -1036,6 +1040,17 @@ static int plan_local_helm(ibpm_batch* B, int i) {
             CKR(B->mem.alloc(&F.m1, ns));
             CKR(B->mem.alloc(&F.m1tmp, ns));
         }
+        // the result and sweep buffers are zero off the current boxes: clear
+        // what an earlier box covered (the result then depends on the current
+        // boxes alone, not on the run's history)
+        for (int lev = 0; lev < (int)F.hbox.size() && lev < F.L; ++lev)
+            if (F.hbox[lev] != nbox[k][lev]) {
+                HelmJob C{};
+                C.nx = F.nx; C.ny = F.ny; C.lev = lev;
+                C.i0 = F.hbox[lev][0]; C.i1 = F.hbox[lev][1]; C.j0 = F.hbox[lev][2]; C.j1 = F.hbox[lev][3];
+                C.out = F.m1 + lev * F.ns; C.tmp = F.m1tmp + lev * F.ns;
+                clears.push_back(C);
+            }
         F.hbox = nbox[k];
         const double alpha = -beta_of(B, F, i) / 2. * (1. / F.p.reynolds);
         for (int lev = 0; lev < F.L; ++lev) {
@@ -1062,7 +1077,13 @@ static int plan_local_helm(ibpm_batch* B, int i) {
             J.skip = F.cg_state + 3;
             const long area = (long)(b[1] - b[0] + 1) * (b[3] - b[2] + 1);
             maxarea = std::max(maxarea, (long)helm_tiles(b[1] - b[0] + 1, b[3] - b[2] + 1));
+            maxpts = std::max(maxpts, area);
             jobs.push_back(J);
+            HelmJob J3 = J;   // omega = omega* - A^-1 c after the force solve: every flow, always
+            J3.skip = nullptr;
+            J3.eo = F.omega + lev * F.ns;
+            J3.ei = F.wstar + lev * F.ns;
+            jobs3.push_back(J3);
             jflow.push_back(k);
             jbytes.push_back(24.0 * area * std::max(1, sweeps));   // per sweep: r and u in, u out
         }
@@ -1080,9 +1101,23 @@ static int plan_local_helm(ibpm_batch* B, int i) {
     CKR(restrict_m2l_pass3(B, nsup));
     if ((int)jobs.size() > B->helm_cap) {
         CKR(B->mem.alloc(&B->d_helm, jobs.size()));
+        CKR(B->mem.alloc(&B->d_helm3, jobs.size()));
         B->helm_cap = (int)jobs.size();
     }
     CKR(upload(B->d_helm, jobs.data(), sizeof(HelmJob) * jobs.size()));
+    CKR(upload(B->d_helm3, jobs3.data(), sizeof(HelmJob) * jobs3.size()));
+    if (!clears.empty()) {
+        long maxc = 1;
+        for (const HelmJob& C : clears) maxc = std::max(maxc, (long)(C.i1 - C.i0 + 1) * (C.j1 - C.j0 + 1));
+        if ((int)clears.size() > B->hclear_cap) {
+            CKR(B->mem.alloc(&B->d_hclear, clears.size()));
+            B->hclear_cap = (int)clears.size();
+        }
+        CKR(upload(B->d_hclear, clears.data(), sizeof(HelmJob) * clears.size()));
+        launch_helm_clear(B->d_hclear, (int)clears.size(), (int)((maxc + kThreads - 1) / kThreads),
+                          B->ctx->c.stream);
+    }
+    B->helm_ablocks = (int)((maxpts + kThreads - 1) / kThreads);
     B->helm_n = (int)jobs.size();
     B->helm_blocks = (int)maxarea;   // tiles of the largest box
     B->helm_cblocks = (int)((maxsup + kThreads - 1) / kThreads);
@@ -1414,7 +1449,21 @@ static int substep_cg(ibpm_batch* B, int i) {
     CK(hipStreamSynchronize(s));
     B->ksub += 1;
     timed(pr, K_SMEAR, 0, s, [&] { launch_smear_curl(B->d_smear_f[i], nfl, B->max_nodes, s); });
-    run_solve(B->S3[i], s, pr);
+    if (B->helm_on[i]) {
+        // omega = omega* - A^-1 c with c = beta B f local like M's right-hand
+        // side: the same sweeps (plan_local_helm), then the subtraction over
+        // the boxes; off them omega already holds omega* (the S1 solve wrote it)
+        for (int lev = 1; lev < B->S3[i].nlev; ++lev)
+            timed(pr, K_HELM, 0, s, [&] { launch_helm_coarsify(B->d_helm3, B->helm_n, B->helm_cblocks, lev, s); });
+        const int nl = (B->helm_sweeps + 3) / 4;
+        double by = 0;
+        for (double b : B->helm_bytes) by += b;
+        launch_helm_sweeps(B->d_helm3, B->helm_n, B->helm_blocks, B->helm_sweeps, s,
+                           [&](const std::function<void()>& fn) { timed(pr, K_HELM, by / nl, s, fn); });
+        timed(pr, K_HELM, 0, s, [&] { launch_helm_epilogue(B->d_helm3, B->helm_n, B->helm_ablocks, s); });
+    } else {
+        run_solve(B->S3[i], s, pr);
+    }
     enqueue_refresh(B, s);
     return 0;
 }

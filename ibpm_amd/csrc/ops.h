@@ -116,6 +116,11 @@ void launch_coarsify(const CoarsifyJob* d_jobs, int njobs, int max_rows, hipStre
 // of the largest box.  timed_launch(fn) runs fn, the launch (a caller may
 // time it).
 int helm_tiles(int h, int w);
+// eo = ei - out over each job's box; max_blocks: kThreads-point blocks of the
+// largest box.
+void launch_helm_epilogue(const HelmJob* d_jobs, int njobs, int max_blocks, hipStream_t s);
+// out = tmp = 0 over each job's box.
+void launch_helm_clear(const HelmJob* d_jobs, int njobs, int max_blocks, hipStream_t s);
 void launch_helm_sweeps(const HelmJob* d_jobs, int njobs, int max_tiles, int nsweeps, hipStream_t s,
                         const std::function<void(const std::function<void()>&)>& timed_launch);
 // The right-hand side's coarsify chain at level lev (>= 1) over each job's

@@ -228,16 +228,18 @@ int ibpm_batch_set_precond(ibpm_batch* b, int mode, int pairs);
    [9] batched M applications that took the local Helmholtz path.
    Returns the count written. */
 int ibpm_batch_solver_stats(ibpm_batch* b, long* stats, int nstats);
-/* No counterpart in the reference (its M application always solves A^-1 =
-   (1 + alpha Lap)^-1 on every level with the DST, ProjectionSolver.cc:96-101,
-   EllipticSolver.cc:40-70).  on = 1 (the default): inside the moving-body
-   force solve, A^-1 of beta B f is computed by Jacobi sweeps over a box around
-   the body on each level, wherever alpha / dx^2 is small enough that the
-   solution decays below 1e-18 of its maximum before the box reaches a level's
-   boundary or the ring of the finer level (the substep falls back to the DST
-   solve otherwise); the result equals the DST solve's to rounding.  on = 0:
-   the DST solve always.  The L^-1 solve that follows is the DST solve in both
-   cases. */
+/* No counterpart in the reference (its M application and its step solve
+   A^-1 = (1 + alpha Lap)^-1 on every level with the DST,
+   ProjectionSolver.cc:57-101, EllipticSolver.cc:40-70).  on = 1 (the
+   default): in moving-body batches, A^-1 of a smeared force beta B f (inside
+   each M application of the force solve, and in the step's
+   omega = omega* - A^-1 c) is computed by Jacobi sweeps over a box around the
+   body on each level, wherever alpha / dx^2 is small enough that the solution
+   decays below 1e-18 of its maximum before the box reaches a level's boundary
+   or the ring of the finer level (the substep falls back to the DST solve
+   otherwise); the result equals the DST solve's to rounding.  The L^-1 solve
+   inside M stays a DST solve, run over the parts of each level whose psi is
+   read.  on = 0: the DST solves always. */
 int ibpm_batch_set_local_helmholtz(ibpm_batch* b, int on);
 
 /* Measurement: advances nsteps with a HIP event pair around every kernel

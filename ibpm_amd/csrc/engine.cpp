@@ -864,7 +864,34 @@ constexpr int kHelmMaxSweeps = 40;
 // side's support in closed form: the support is the local result's box on
 // each level, with coarsify's reach from the finer level inside the central
 // region.  (The LDS passes ignore both restrictions and do the whole level.)
-static int restrict_m2l_pass3(ibpm_batch* B, const std::vector<std::vector<std::array<int, 4>>>& nsup) {
+// Passes 3 and 2 of a Poisson solve whose psi is read only by the
+// interpolation (at the regulariser's nodes, level-0 rows node_lo..node_hi)
+// and by the finer levels' getBC (the coarser levels' rings): see
+// restrict_m2l_pass3.  Register-held pass 3 only.
+static void read_rows_only(EllJob& J, const PassGroup& G, int node_lo, int node_hi, const double* ringsin) {
+    if (!G.regs[2] || J.epilogue != 0 || J.vside) return;
+    const int N1 = J.nx - 1, ex = nx_ext(J.nx), ey = nx_ext(J.ny);
+    if (J.lev == 0) {
+        J.full_lo = std::max(node_lo - 2, 0) / 2;
+        J.full_hi = std::min(node_hi, N1 - 1) / 2;
+        J.t2_lo = 2 * J.full_lo;
+        J.t2_hi = 2 * J.full_hi + 1;
+    } else {
+        J.full_lo = 1; J.full_hi = 0;
+        J.fq0 = (ex - 1) / 2; J.fq1 = (ex + J.nx / 2 - 1) / 2;
+        J.ring_lo = ex + 1; J.ring_hi = ex + J.nx / 2 - 1;
+        J.rj0 = ey; J.rj1 = ey + J.ny / 2;
+        J.ringsin = ringsin;
+        J.t2_lo = 2 * J.fq0;
+        J.t2_hi = 2 * J.fq1 + 1;
+    }
+    if (!G.regs[1]) {   // an LDS pass 2 writes every row
+        J.t2_lo = 0;
+        J.t2_hi = INT_MAX;
+    }
+}
+
+static int ring_tables(ibpm_batch* B) {
     for (auto& F : B->flows) {
         if (F.ringsin) continue;
         const int n = F.ny - 1, ey = nx_ext(F.ny);
@@ -880,6 +907,44 @@ static int restrict_m2l_pass3(ibpm_batch* B, const std::vector<std::vector<std::
         CKR(B->mem.alloc(&F.sin1, s1.size()));
         CKR(upload(F.sin1, s1.data(), sizeof(double) * s1.size()));
     }
+    return 0;
+}
+
+// The step's psi = L^-1(-omega*) of substep i feeds only the interpolation of
+// C omega* (the force solve overwrites psi and the refresh solves it again),
+// so its passes 3 and 2 take read_rows_only from the current node rows.
+static int restrict_s2(ibpm_batch* B, int i) {
+    CKR(ring_tables(B));
+    std::vector<std::array<int, 2>> rows(B->flows.size(), std::array<int, 2>{1, INT_MAX});
+    for (size_t k = 0; k < B->flows.size(); ++k) {
+        const Flow& F = B->flows[k];
+        if (F.reg.node_idx.empty()) continue;
+        int lo = INT_MAX, hi = INT_MIN;
+        for (int idx : F.reg.node_idx) {
+            lo = std::min(lo, idx / (F.ny - 1) + 1);
+            hi = std::max(hi, idx / (F.ny - 1) + 1);
+        }
+        rows[k] = {lo, hi};
+    }
+    for (auto& step : B->S2[i].steps)
+        for (auto& G : step) {
+            bool changed = false;
+            for (size_t j = 0; j < G.h.size(); ++j) {
+                EllJob& J = G.h[j];
+                const EllJob before = J;
+                whole_level(J);
+                const int k = G.jflow[j];
+                if (B->helm_allowed && rows[k][1] != INT_MAX)
+                    read_rows_only(J, G, rows[k][0], rows[k][1], B->flows[k].ringsin);
+                if (std::memcmp(&before, &J, sizeof(EllJob)) != 0) changed = true;
+            }
+            if (changed) CKR(upload(G.d, G.h.data(), sizeof(EllJob) * G.h.size()));
+        }
+    return 0;
+}
+
+static int restrict_m2l_pass3(ibpm_batch* B, const std::vector<std::vector<std::array<int, 4>>>& nsup) {
+    CKR(ring_tables(B));
     // per flow and level: rbox, where the coarsify chain puts the right-hand
     // side's R (coarsify's reach from the finer level's support, inside the
     // central region; empty on level 0), and sup, the bounding box of the
@@ -927,28 +992,7 @@ static int restrict_m2l_pass3(ibpm_batch* B, const std::vector<std::vector<std::
                         J.rb_i0 = r[0]; J.rb_i1 = r[1]; J.rb_j0 = r[2]; J.rb_j1 = r[3];
                     }
                 }
-                if (G.regs[2] && J.epilogue == 0 && !J.vside) {
-                    const int N1 = J.nx - 1, ex = nx_ext(J.nx), ey = nx_ext(J.ny);
-                    if (J.lev == 0) {
-                        const std::array<int, 4>& sp = nsup[k][0];   // node rows, 1-based
-                        J.full_lo = std::max(sp[0] - 2, 0) / 2;
-                        J.full_hi = std::min(sp[1], N1 - 1) / 2;
-                        J.t2_lo = 2 * J.full_lo;
-                        J.t2_hi = 2 * J.full_hi + 1;
-                    } else {
-                        J.full_lo = 1; J.full_hi = 0;
-                        J.fq0 = (ex - 1) / 2; J.fq1 = (ex + J.nx / 2 - 1) / 2;
-                        J.ring_lo = ex + 1; J.ring_hi = ex + J.nx / 2 - 1;
-                        J.rj0 = ey; J.rj1 = ey + J.ny / 2;
-                        J.ringsin = B->flows[k].ringsin;
-                        J.t2_lo = 2 * J.fq0;
-                        J.t2_hi = 2 * J.fq1 + 1;
-                    }
-                    if (!G.regs[1]) {   // an LDS pass 2 writes every row
-                        J.t2_lo = 0;
-                        J.t2_hi = INT_MAX;
-                    }
-                }
+                read_rows_only(J, G, nsup[k][0][0], nsup[k][0][1], B->flows[k].ringsin);
                 if (std::memcmp(&before, &J, sizeof(EllJob)) != 0) changed = true;
             }
             if (changed) CKR(upload(G.d, G.h.data(), sizeof(EllJob) * G.h.size()));
@@ -1389,6 +1433,7 @@ static int substep_cg(ibpm_batch* B, int i) {
         if (F.ubf) CKR(move_base_flow(B, F, B->time + B->cn[i] * F.p.dt));
     CKR(upload_regops(B));
     CKR(plan_local_helm(B, i));
+    CKR(restrict_s2(B, i));
     B->st_host_kin_s += now_s() - t_kin;
     Probe* pr = B->probe;
     timed(pr, K_CLEAR, 0, s, [&] { launch_clear_nodes(B->d_clear, nfl, B->max_nodes, s); });

@@ -1321,6 +1321,8 @@ void launch_helm_epilogue(const HelmJob* d_jobs, int njobs, int max_blocks, hipS
     hipLaunchKernelGGL(k_helm_epilogue, dim3(max_blocks, njobs), dim3(kThreads), 0, s, d_jobs);
 }
 
+int r16_pass_group(int pass) { return r16_group(pass); }
+
 int helm_tiles(int h, int w) { return ((h + kHelmTile - 1) / kHelmTile) * ((w + kHelmTile - 1) / kHelmTile); }
 
 void launch_helm_sweeps(const HelmJob* d_jobs, int njobs, int max_tiles, int nsweeps, hipStream_t s,

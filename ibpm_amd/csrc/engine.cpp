@@ -211,8 +211,8 @@ struct PassGroup {
     int maxb[3] = {0, 0, 0}, lds[3] = {0, 0, 0};
     bool regs[3] = {false, false, false};           // pass_in_registers per pass
     std::vector<int> jflow;                         // flow of each job
-    std::vector<double> jbytes;                     // 16 B per point (one read + one write)
-    double bytes = 0;
+    std::vector<double> jbytes[3];                  // per pass: 16 B per point (one read + one write),
+    double bytes[3] = {0, 0, 0};                    // less where a pass is restricted (pass_bytes)
     std::vector<EllJob> h;                          // host copy of the jobs
 };
 
@@ -300,9 +300,9 @@ static int build_solve_plan(Ctx* c, DevMem& mem, const std::vector<SolveSpec>& s
                 groups.back().mi = J.pi.M;
                 gjobs.emplace_back();
             }
-            groups[g].bytes += 16.0 * ns;
+            for (int q = 0; q < 3; ++q) groups[g].bytes[q] += 16.0 * ns;
             groups[g].jflow.push_back((int)k);
-            groups[g].jbytes.push_back(16.0 * ns);
+            for (int q = 0; q < 3; ++q) groups[g].jbytes[q].push_back(16.0 * ns);
             gjobs[g].push_back(J);
         }
         for (size_t g = 0; g < groups.size(); ++g) {
@@ -359,9 +359,9 @@ static void run_solve(const SolvePlan& P, hipStream_t s, Probe* pr = nullptr,
     if (coarsify) run_coarsify(P, s, pr, active);
     for (int st = 0; st < P.nlev; ++st)
         for (const PassGroup& G : P.steps[st]) {
-            const double by = pr ? sum(G.jflow, G.jbytes, G.bytes) : 0;
             for (int pass = 1; pass <= 3; ++pass)
-                timed(pr, (fs ? K_FSPASS1 : K_PASS1) + pass - 1, by, s, [&] {
+                timed(pr, (fs ? K_FSPASS1 : K_PASS1) + pass - 1,
+                      pr ? sum(G.jflow, G.jbytes[pass - 1], G.bytes[pass - 1]) : 0, s, [&] {
                     launch_ell_pass(pass, G.d, G.n, G.maxb[pass - 1], G.lds[pass - 1], s, fs,
                                     pass == 2 ? G.mi : G.mj, G.regs[pass - 1]);
                 });
@@ -864,6 +864,56 @@ constexpr int kHelmMaxSweeps = 40;
 // side's support in closed form: the support is the local result's box on
 // each level, with coarsify's reach from the finer level inside the central
 // region.  (The LDS passes ignore both restrictions and do the whole level.)
+// Algorithmic bytes of each pass of a job under its restrictions (EllJob's
+// partial-pass fields): 8 B per point read and per point written, counting
+// only the rows a pass reads and writes.
+static void pass_bytes(const EllJob& J, const PassGroup& G, double out[3]) {
+    const double N1 = J.nx - 1, N2 = J.ny - 1;
+    const int kG1 = r16_pass_group(1);   // row pairs per workgroup of the register-held pass 1
+    // pass 1: T1 whole; rows read unless their workgroup takes the closed form
+    double read1 = N1;
+    if (J.sin1 && G.regs[0]) {
+        read1 = 0;
+        for (int r0 = 0; r0 < (int)N1; r0 += 2 * kG1) {
+            const int i_first = r0 + 1, i_last = std::min(r0 + 2 * kG1, (int)N1);
+            const bool closed = (i_last < J.sup_lo || i_first > J.sup_hi) && i_first > 1 && i_last < J.nx - 1;
+            if (!closed) read1 += i_last - i_first + 1;
+        }
+    }
+    out[0] = 8.0 * N2 * (read1 + N1);
+    // pass 2: T1 whole in, T2 rows [t2_lo, t2_hi] out
+    const double w2 = std::max(0.0, std::min((double)J.t2_hi, N1 - 1) - std::max(J.t2_lo, 0) + 1);
+    out[1] = 8.0 * N2 * (N1 + w2);
+    // pass 3: rows of transformed pairs in and out, ring rows in and two values out
+    double full = 0, ring = 0;
+    for (int q = 0; 2 * q < (int)N1; ++q) {
+        const int rows = std::min(2, (int)N1 - 2 * q);
+        const bool f = (q >= J.full_lo && q <= J.full_hi) || q == J.fq0 || q == J.fq1;
+        if (f) {
+            full += rows;
+        } else {
+            for (int h = 0; h < rows; ++h) {
+                const int i = 2 * q + h + 1;
+                if (i >= J.ring_lo && i <= J.ring_hi) ring += 1;
+            }
+        }
+    }
+    out[2] = 8.0 * (N2 * (2 * full + ring) + 2 * ring);
+}
+
+// Per-pass bytes of a group's jobs after their restrictions changed.
+static void group_bytes(PassGroup& G) {
+    for (int q = 0; q < 3; ++q) G.bytes[q] = 0;
+    for (size_t j = 0; j < G.h.size(); ++j) {
+        double b[3];
+        pass_bytes(G.h[j], G, b);
+        for (int q = 0; q < 3; ++q) {
+            G.jbytes[q][j] = b[q];
+            G.bytes[q] += b[q];
+        }
+    }
+}
+
 // Passes 3 and 2 of a Poisson solve whose psi is read only by the
 // interpolation (at the regulariser's nodes, level-0 rows node_lo..node_hi)
 // and by the finer levels' getBC (the coarser levels' rings): see
@@ -938,7 +988,10 @@ static int restrict_s2(ibpm_batch* B, int i) {
                     read_rows_only(J, G, rows[k][0], rows[k][1], B->flows[k].ringsin);
                 if (std::memcmp(&before, &J, sizeof(EllJob)) != 0) changed = true;
             }
-            if (changed) CKR(upload(G.d, G.h.data(), sizeof(EllJob) * G.h.size()));
+            if (changed) {
+                CKR(upload(G.d, G.h.data(), sizeof(EllJob) * G.h.size()));
+                group_bytes(G);
+            }
         }
     return 0;
 }
@@ -995,7 +1048,10 @@ static int restrict_m2l_pass3(ibpm_batch* B, const std::vector<std::vector<std::
                 read_rows_only(J, G, nsup[k][0][0], nsup[k][0][1], B->flows[k].ringsin);
                 if (std::memcmp(&before, &J, sizeof(EllJob)) != 0) changed = true;
             }
-            if (changed) CKR(upload(G.d, G.h.data(), sizeof(EllJob) * G.h.size()));
+            if (changed) {
+                CKR(upload(G.d, G.h.data(), sizeof(EllJob) * G.h.size()));
+                group_bytes(G);
+            }
         }
     if (!B->m2l_boxed) return 0;
     std::vector<HelmJob> cj;

@@ -116,6 +116,8 @@ void launch_coarsify(const CoarsifyJob* d_jobs, int njobs, int max_rows, hipStre
 // of the largest box.  timed_launch(fn) runs fn, the launch (a caller may
 // time it).
 int helm_tiles(int h, int w);
+// Row pairs per workgroup of register-held pass `pass` (1, 2, 3).
+int r16_pass_group(int pass);
 // eo = ei - out over each job's box; max_blocks: kThreads-point blocks of the
 // largest box.
 void launch_helm_epilogue(const HelmJob* d_jobs, int njobs, int max_blocks, hipStream_t s);

@@ -233,6 +233,30 @@ def test_local_helmholtz_equals_dst_solve_register_passes(gpu):
     assert sl["local_helmholtz_m_applications"] == sl["m_applications"] > 0
 
 
+def test_local_helmholtz_equals_dst_solve_uneven_batch(gpu):
+    """Two moving flows of different grids, level counts and motions in one
+    batch (their force solves converge at different iterations, so the local
+    sweeps and the restricted L^-1 passes skip one flow while the other
+    iterates): local and DST paths agree as for one flow."""
+    ctx = gpu.Context(0)
+    g1 = ("body Plate\n  line_n -0.5 0 0.5 0 33\n  center 0 0\n"
+          "  motion pitchplunge 0.2617993877991494 0.4 0 0.2 0.4 0\nend\n")
+    g2 = ("body Plate\n  line_n -0.4 0.1 0.4 -0.1 25\n  center 0 0\n"
+          "  motion pitchplunge 0.1 0.7 0.2 0.1 0.5 0.3\nend\n")
+    f1 = gpu.Flow(128, 128, 3, 4.0, -2.0, -2.0, 300.0, 0.005, g1, tol=1e-10)
+    f2 = gpu.Flow(96, 128, 2, 3.0, -1.5, -2.0, 200.0, 0.005, g2, tol=1e-9)
+    loc = gpu.Batch(ctx, [f1, f2]).init()
+    dst = gpu.Batch(ctx, [f1, f2]).init().set_local_helmholtz(False)
+    for _ in range(4):
+        loc.advance(1)
+        dst.advance(1)
+    for k in range(2):
+        hl, hd = loc.force_history(k), dst.force_history(k)
+        assert np.abs(hl - hd).max() <= 1e-7 * np.abs(hd).max(), k
+        assert rel(loc.omega(k), dst.omega(k)) < 1e-7, k
+    assert loc.solver_stats()["local_helmholtz_m_applications"] > 0
+
+
 def test_moving_plate_cg_matches_oracle(gpu, oracle):
     geom = plate_geom(16, 1.0, 0.2, 0.2, 0.4)   # spacing 1/15 ~ dx = 1/16
     ctx, b, s = _pair(gpu, oracle, 64, 3, 300.0, 0.01, geom, tol=1e-10)

@@ -2,7 +2,7 @@
 configured workload on one GPU, with the dominant kernel's roofline and the
 CPU oracle timed beside it.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload cfg5]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload tandem_16x1024x6]
 
 A step is one full RK3 timestep (3 substeps) of every flow in the workload;
 grid-pt counts the stored interior nodes of all levels.  Inputs are resident
@@ -24,14 +24,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
-DEFAULT_STEPS = {"cfg1": 500, "cfg2": 200, "cfg3": 20, "cfg4": 40, "cfg5": 40}
+DEFAULT_STEPS = {"cylinder_200": 500, "cylinder_400x4": 200, "mixed_64x3": 20, "plate_512x5": 40, "tandem_16x1024x6": 40}
 METRIC = "grid-pt\u00b7steps/s learned+batched vs plain GPU at Cd/Cl err\u22641e-6; CG iters/step"
 CONFIG_TEXT = {
-    "cfg1": "stationary cylinder D=1 (157 pts), 200x200, 1 level, Re 100, dt 0.02, Cholesky",
-    "cfg2": "stationary cylinder D=1 (314 pts), 400x400 finest, 4 levels, Re 200, dt 0.01, Cholesky",
-    "cfg3": "64 stationary flows (cylinder/ellipse/plate), n in {256,384,512}, 3 levels, dt 0.005",
-    "cfg4": "pitching/plunging plate (100 pts), 512x512, 5 levels, Re 300, dt 0.0025, CG tol 1e-10",
-    "cfg5": "16 x 3 tandem pitching/plunging plates, 1024x1024, 6 levels, Re 500, dt 0.001, CG tol 1e-10",
+    "cylinder_200": "stationary cylinder D=1 (157 pts), 200x200, 1 level, Re 100, dt 0.02, Cholesky",
+    "cylinder_400x4": "stationary cylinder D=1 (314 pts), 400x400 finest, 4 levels, Re 200, dt 0.01, Cholesky",
+    "mixed_64x3": "64 stationary flows (cylinder/ellipse/plate), n in {256,384,512}, 3 levels, dt 0.005",
+    "plate_512x5": "pitching/plunging plate (100 pts), 512x512, 5 levels, Re 300, dt 0.0025, CG tol 1e-10",
+    "tandem_16x1024x6": "16 x 3 tandem pitching/plunging plates, 1024x1024, 6 levels, Re 500, dt 0.001, CG tol 1e-10",
 }
 
 
@@ -140,8 +140,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="cfg5", choices=sorted(CONFIG_TEXT),
-                    help="cfg5 (default): the moving-plate batch the metric is quoted on")
+    ap.add_argument("--workload", default="tandem_16x1024x6", choices=sorted(CONFIG_TEXT),
+                    help="tandem_16x1024x6 (default): the moving-plate batch the metric is quoted on")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")

@@ -2,17 +2,17 @@
 ibpm_amd.Flow.  Random cases are drawn from fixed seeds so every run (and the
 CPU oracle) sees the same flows.
 
-  cfg1  stationary cylinder D=1 (circle_n 157), 200^2, 1 level, [-2,2]^2,
+  cylinder_200  stationary cylinder D=1 (circle_n 157), 200^2, 1 level, [-2,2]^2,
         Re 100, dt 0.02, 250 steps, Cholesky force solve
-  cfg2  cylinder D=1 (circle_n 314), 400^2 finest, 4 levels, [-2,2]^2
+  cylinder_400x4  cylinder D=1 (circle_n 314), 400^2 finest, 4 levels, [-2,2]^2
         finest, Re 200, dt 0.01, 2000 steps, Cholesky
-  cfg3  64 stationary flows: cylinder / ellipse (aspect U(2,5)) / plate of
+  mixed_64x3  64 stationary flows: cylinder / ellipse (aspect U(2,5)) / plate of
         chord 1, angle of attack U(0,30) deg, Re U(50,300), n in {256,384,512},
         3 levels, dt 0.005
-  cfg4  pitching/plunging plate (100 points, mid-chord centre), plunge U(.1,.3),
+  plate_512x5  pitching/plunging plate (100 points, mid-chord centre), plunge U(.1,.3),
         pitch U(5,20) deg, frequency U(.2,.5), Re 300, 512^2, 5 levels,
         dt 0.0025, CG tolerance 1e-10
-  cfg5  16 cases of 3 tandem plates (spacing U(1,2)) with cfg4 kinematics,
+  tandem_16x1024x6  16 cases of 3 tandem plates (spacing U(1,2)) with plate_512x5 kinematics,
         Re 500, 1024^2, 6 levels, [-4,4]^2, dt 0.001, CG tolerance 1e-10
 """
 import math
@@ -60,15 +60,15 @@ def pitch_plunge_plate(rng, xc=0.0, npts=100, chord=1.0):
             % (xc - chord / 2, xc + chord / 2, npts, xc, ap, f, ah, f))
 
 
-def cfg1():
+def cylinder_200():
     return [Flow(200, 200, 1, 4.0, -2.0, -2.0, 100.0, 0.02, cylinder(157))]
 
 
-def cfg2():
+def cylinder_400x4():
     return [Flow(400, 400, 4, 4.0, -2.0, -2.0, 200.0, 0.01, cylinder(314))]
 
 
-def cfg3(nflows=64, seed=3):
+def mixed_64x3(nflows=64, seed=3):
     rng = np.random.default_rng(seed)
     flows = []
     for _ in range(nflows):
@@ -87,12 +87,12 @@ def cfg3(nflows=64, seed=3):
     return flows
 
 
-def cfg4(seed=4):
+def plate_512x5(seed=4):
     rng = np.random.default_rng(seed)
     return [Flow(512, 512, 5, 4.0, -2.0, -2.0, 300.0, 0.0025, pitch_plunge_plate(rng), tol=1e-10)]
 
 
-def cfg5(ncases=16, seed=5):
+def tandem_16x1024x6(ncases=16, seed=5):
     rng = np.random.default_rng(seed)
     flows = []
     for _ in range(ncases):
@@ -102,7 +102,7 @@ def cfg5(ncases=16, seed=5):
     return flows
 
 
-WORKLOADS = {"cfg1": cfg1, "cfg2": cfg2, "cfg3": cfg3, "cfg4": cfg4, "cfg5": cfg5}
+WORKLOADS = {"cylinder_200": cylinder_200, "cylinder_400x4": cylinder_400x4, "mixed_64x3": mixed_64x3, "plate_512x5": plate_512x5, "tandem_16x1024x6": tandem_16x1024x6}
 
 
 def grid_points(flow):

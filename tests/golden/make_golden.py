@@ -1,10 +1,10 @@
 """Writes the golden fixtures of the GPU parity tests from the FP64 CPU oracle
 (oracle/ibpm_oracle.c).  Run in this repository (takes minutes to ~1.5 h):
 
-    python tests/golden/make_golden.py cfg1        # 250 steps, ~1 min
-    python tests/golden/make_golden.py cfg2        # 2000 steps, ~1.5 h
+    python tests/golden/make_golden.py cylinder_200        # 250 steps, ~1 min
+    python tests/golden/make_golden.py cylinder_400x4        # 2000 steps, ~1.5 h
     python tests/golden/make_golden.py plate       # moving plate, 30 steps
-    python tests/golden/make_golden.py cfg4short   # configured moving plate, 12 steps
+    python tests/golden/make_golden.py plate_512x5_short   # configured moving plate, 12 steps
 
 Each fixture holds the per-step Cd/Cl series (full precision), CG iterations
 per substep where CG is used, per-level vorticity norms and the level-0
@@ -27,10 +27,10 @@ PLATE = ("body Plate\n  line_n -0.5 0 0.5 0 33\n  center 0 0\n"
          "  motion pitchplunge 0.2617993877991494 0.4 0 0.2 0.4 0\nend\n")
 
 CASES = {
-    "cfg1": (W.cfg1()[0], 250),
-    "cfg2": (W.cfg2()[0], 2000),
+    "cylinder_200": (W.cylinder_200()[0], 250),
+    "cylinder_400x4": (W.cylinder_400x4()[0], 2000),
     "plate": (W.Flow(128, 128, 3, 4.0, -2.0, -2.0, 300.0, 0.005, PLATE, tol=1e-10), 30),
-    "cfg4short": (W.cfg4()[0], 12),
+    "plate_512x5_short": (W.plate_512x5()[0], 12),
 }
 
 

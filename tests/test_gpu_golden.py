@@ -34,12 +34,12 @@ def _fixture(name):
 def _flow(name):
     if name == "plate":
         return W.Flow(128, 128, 3, 4.0, -2.0, -2.0, 300.0, 0.005, PLATE, tol=1e-10)
-    if name == "cfg4short":
-        return W.cfg4()[0]
+    if name == "plate_512x5_short":
+        return W.plate_512x5()[0]
     return W.WORKLOADS[name]()[0]
 
 
-@pytest.mark.parametrize("name", ["cfg1", "cfg2", "plate", "cfg4short"])
+@pytest.mark.parametrize("name", ["cylinder_200", "cylinder_400x4", "plate", "plate_512x5_short"])
 def test_full_run_matches_golden(gpu, name):
     g = _fixture(name)
     f = _flow(name)
@@ -143,7 +143,7 @@ def test_ibpm_driver_unsteady_base_flow(gpu, oracle):
         assert abs(full[k, 2] - cd) <= 1e-9 * scale and abs(full[k, 3] - cl) <= 1e-9 * scale
 
 
-@pytest.mark.parametrize("name", ["plate", "cfg4short"])
+@pytest.mark.parametrize("name", ["plate", "plate_512x5_short"])
 @pytest.mark.parametrize("pairs", [0, 8])
 def test_preconditioned_cg_matches_golden(gpu, name, pairs):
     """The preconditioned force solve (stale FP64 inverse of M, with and

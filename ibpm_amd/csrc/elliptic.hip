@@ -1139,7 +1139,11 @@ __global__ void __launch_bounds__(kThreads) k_fs_coarsify(const CoarsifyJob* __r
 // the tile.  Each point's update is the same arithmetic in the same order as
 // a sweep over global memory, so the result does not depend on the blocking.
 // Launches alternate between tmp and out so that the last writes out.
-constexpr int kHelmTile = 32, kHelmDepth = 4, kHelmReg = kHelmTile + 2 * kHelmDepth;
+#ifndef IBPM_HELM_DEPTH
+#define IBPM_HELM_DEPTH 4
+#endif
+constexpr int kHelmTile = 32, kHelmDepth = IBPM_HELM_DEPTH, kHelmReg = kHelmTile + 2 * kHelmDepth;
+static_assert(kHelmDepth >= 1 && kHelmDepth <= 12, "sweeps per launch: 1..12 (LDS tiles with the halo)");
 
 __global__ void __launch_bounds__(kThreads) k_fs_helm(const HelmJob* __restrict__ jobs, int sweep0, int nsw,
                                                      bool to_out) {
@@ -1322,6 +1326,8 @@ void launch_helm_epilogue(const HelmJob* d_jobs, int njobs, int max_blocks, hipS
 }
 
 int r16_pass_group(int pass) { return r16_group(pass); }
+
+int helm_launches(int nsweeps) { return (nsweeps + kHelmDepth - 1) / kHelmDepth; }
 
 int helm_tiles(int h, int w) { return ((h + kHelmTile - 1) / kHelmTile) * ((w + kHelmTile - 1) / kHelmTile); }
 

@@ -1246,7 +1246,7 @@ static void enqueue_M(ibpm_batch* B, int i, RegOps* smear, hipStream_t s,
         for (int lev = 1; lev < B->M1[i].nlev; ++lev)
             timed(pr, K_FSCOARSIFY, cby / std::max(1, B->M1[i].nlev - 1), s,
                   [&] { launch_helm_coarsify(B->d_helm, B->helm_n, B->helm_cblocks, lev, s); });
-        const int nl = (B->helm_sweeps + 3) / 4;   // launches of up to 4 sweeps
+        const int nl = helm_launches(B->helm_sweeps);
         launch_helm_sweeps(B->d_helm, B->helm_n, B->helm_blocks, B->helm_sweeps, s,
                            [&](const std::function<void()>& fn) { timed(pr, K_FSHELM, by / nl, s, fn); });
         if (B->m2l_boxed) {
@@ -1556,7 +1556,7 @@ static int substep_cg(ibpm_batch* B, int i) {
         // the boxes; off them omega already holds omega* (the S1 solve wrote it)
         for (int lev = 1; lev < B->S3[i].nlev; ++lev)
             timed(pr, K_HELM, 0, s, [&] { launch_helm_coarsify(B->d_helm3, B->helm_n, B->helm_cblocks, lev, s); });
-        const int nl = (B->helm_sweeps + 3) / 4;
+        const int nl = helm_launches(B->helm_sweeps);
         double by = 0;
         for (double b : B->helm_bytes) by += b;
         launch_helm_sweeps(B->d_helm3, B->helm_n, B->helm_blocks, B->helm_sweeps, s,

@@ -116,6 +116,7 @@ void launch_coarsify(const CoarsifyJob* d_jobs, int njobs, int max_rows, hipStre
 // of the largest box.  timed_launch(fn) runs fn, the launch (a caller may
 // time it).
 int helm_tiles(int h, int w);
+int helm_launches(int nsweeps);   // launches of launch_helm_sweeps for nsweeps sweeps
 // Row pairs per workgroup of register-held pass `pass` (1, 2, 3).
 int r16_pass_group(int pass);
 // eo = ei - out over each job's box; max_blocks: kThreads-point blocks of the

@@ -257,6 +257,26 @@ def test_local_helmholtz_equals_dst_solve_uneven_batch(gpu):
     assert loc.solver_stats()["local_helmholtz_m_applications"] > 0
 
 
+def test_local_helmholtz_falls_back_near_the_boundary(gpu):
+    """A plate whose grown box would reach the fine grid's first rows (it sits
+    4 cells from the edge): the planner turns the local path off for every
+    substep, so no M application takes it, and the run equals the DST run bit
+    for bit (128^2 grids run the LDS passes, which the pass restrictions leave
+    whole)."""
+    ctx = gpu.Context(0)
+    geom = ("body Plate\n  line_n -1.6 -1.85 -0.6 -1.85 33\n  center -1.1 -1.85\n"
+            "  motion pitchplunge 0.1 0.4 0 0.02 0.4 0\nend\n")
+    fl = gpu.Flow(128, 128, 2, 4.0, -2.0, -2.0, 300.0, 0.005, geom, tol=1e-10)
+    loc = gpu.Batch(ctx, [fl]).init()
+    dst = gpu.Batch(ctx, [fl]).init().set_local_helmholtz(False)
+    for _ in range(3):
+        loc.advance(1)
+        dst.advance(1)
+    assert loc.solver_stats()["local_helmholtz_m_applications"] == 0
+    assert np.array_equal(loc.force_history(), dst.force_history())
+    assert np.array_equal(loc.omega(), dst.omega())
+
+
 def test_moving_plate_cg_matches_oracle(gpu, oracle):
     geom = plate_geom(16, 1.0, 0.2, 0.2, 0.4)   # spacing 1/15 ~ dx = 1/16
     ctx, b, s = _pair(gpu, oracle, 64, 3, 300.0, 0.01, geom, tol=1e-10)

@@ -257,6 +257,26 @@ def test_local_helmholtz_equals_dst_solve_uneven_batch(gpu):
     assert loc.solver_stats()["local_helmholtz_m_applications"] > 0
 
 
+def test_local_helmholtz_equals_dst_solve_tandem_plates(gpu):
+    """One case of the 16-case tandem-plate batch (three moving plates,
+    1024^2, 6 levels): boxes that span several bodies and all six levels of
+    the register-held restricted passes, local against DST for 2 steps."""
+    from ibpm_amd import workloads as W
+    ctx = gpu.Context(0)
+    fl = W.tandem_16x1024x6(ncases=1)[0]
+    loc = gpu.Batch(ctx, [fl]).init()
+    dst = gpu.Batch(ctx, [fl]).init().set_local_helmholtz(False)
+    for step in range(2):
+        print("tandem step", step, flush=True)
+        loc.advance(1)
+        dst.advance(1)
+    hl, hd = loc.force_history(), dst.force_history()
+    assert np.abs(hl - hd).max() <= 1e-8 * np.abs(hd).max()
+    assert rel(loc.omega(), dst.omega()) < 1e-8
+    sl = loc.solver_stats()
+    assert sl["local_helmholtz_m_applications"] == sl["m_applications"] > 0
+
+
 def test_local_helmholtz_falls_back_near_the_boundary(gpu):
     """A plate whose grown box would reach the fine grid's first rows (it sits
     4 cells from the edge): the planner turns the local path off for every

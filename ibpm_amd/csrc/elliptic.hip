@@ -12,9 +12,13 @@
 //   pass 3: rows of T2 -> DST along l -> * 1/(4 nx ny) -> u (or u = W - .)
 // Each DST-I of length N is the imaginary part of a complex FFT of length
 // M = 2(N+1) of the odd extension; two rows share one FFT (real part row a,
-// imaginary part row b).  The FFT is a mixed-radix (2,3,4,5) Stockham
-// transform held in LDS (<= 2048 complex per workgroup), with each stage
-// staged through registers so one LDS buffer suffices.
+// imaginary part row b).  M = 1024 and 2048 use a radix 16 x 16 x M/256
+// transform held in registers (exchanges through LDS); the other lengths a
+// mixed-radix (2,3,4,5) Stockham transform held in LDS, with each stage
+// staged through registers so one LDS buffer suffices.  Passes may run over
+// part of a level when only part of the result is read (EllJob's partial-pass
+// fields).  The file also holds the coarsify chain and the local Jacobi solves
+// of the force solve's A^-1 (k_fs_helm).
 #include "ops.h"
 
 #include <algorithm>

@@ -11,6 +11,14 @@
 
 namespace ibpmg {
 
+// Field pointers held in job descriptors are generic (flat) pointers; kernels
+// address them as global memory so that loads and stores are global
+// instructions, which do not share the LDS counter and may move 16 bytes at
+// 8-byte alignment.
+#define IBPM_GLOBAL __attribute__((address_space(1)))
+__device__ __forceinline__ const IBPM_GLOBAL double* gin(const double* p) { return (const IBPM_GLOBAL double*)p; }
+__device__ __forceinline__ IBPM_GLOBAL double* gout(double* p) { return (IBPM_GLOBAL double*)p; }
+
 constexpr int kThreads = 256;          // workgroup size of the stencil kernels (4 wave64)
 #ifndef IBPM_FFT_THREADS
 #define IBPM_FFT_THREADS 256

@@ -33,13 +33,6 @@ __device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x -
 // -i * a
 __device__ inline double2 cmi(double2 a) { return make_double2(a.y, -a.x); }
 
-// Field pointers held in job descriptors are generic (flat) pointers; the
-// transforms address them as global memory so that loads and stores are
-// global instructions, which do not share the LDS counter and may move 16
-// bytes at 8-byte alignment.
-#define IBPM_GLOBAL __attribute__((address_space(1)))
-__device__ __forceinline__ const IBPM_GLOBAL double* gin(const double* p) { return (const IBPM_GLOBAL double*)p; }
-__device__ __forceinline__ IBPM_GLOBAL double* gout(double* p) { return (IBPM_GLOBAL double*)p; }
 typedef double DoublePair __attribute__((ext_vector_type(2), aligned(8)));
 
 template <int R> __device__ inline void dft(double2* v);

@@ -230,25 +230,36 @@ __global__ void __launch_bounds__(kThreads) k_rhs(const FlowOps* __restrict__ fl
     const Lay g{F.nx, F.ny, F.L};
     const long t = (long)blockIdx.x * kThreads + threadIdx.x;
     if (t >= g.L * g.ns()) return;
-    const int lev = (int)(t / g.ns());
-    const long r = t - lev * g.ns();
-    const int i = (int)(r / (g.ny - 1)) + 1, j = (int)(r % (g.ny - 1)) + 1;
+    // (sizes below 2^32: 32-bit divisions)
+    const unsigned ns = (unsigned)g.ns(), ny1 = (unsigned)(g.ny - 1);
+    const int lev = (int)((unsigned)t / ns);
+    const unsigned r = (unsigned)t - lev * ns;
+    const int i = (int)(r / ny1) + 1, j = (int)(r % ny1) + 1;
     const double d = F.dx * (1 << lev);
     const double bydx2 = 1. / (d * d);
-    const double* C = F.cross;
+    const IBPM_GLOBAL double* C = gin(F.cross);
     const double n = (C[g.fy(lev, i, j)] - C[g.fy(lev, i - 1, j)]
                       + C[g.fx(lev, i, j - 1)] - C[g.fx(lev, i, j)]) * bydx2;
     const double* W = F.omega;
-    double lap = (curl_y(g, W, lev, i, j) - curl_y(g, W, lev, i - 1, j)
-                  + curl_x(g, W, lev, i, j - 1) - curl_x(g, W, lev, i, j)) * bydx2;
+    double lap;
+    if (i >= 2 && i <= g.nx - 2 && j >= 2 && j <= g.ny - 2) {
+        // away from the level's edge the curls are plain differences: the
+        // general expression below, term for term and in the same order
+        const IBPM_GLOBAL double* Wg = gin(W);
+        const double c = Wg[t], e = Wg[t + ny1], w = Wg[t - ny1], up = Wg[t + 1], dn = Wg[t - 1];
+        lap = ((c - e) - (w - c) + (c - dn) - (up - c)) * bydx2;
+    } else {
+        lap = (curl_y(g, W, lev, i, j) - curl_y(g, W, lev, i - 1, j)
+               + curl_x(g, W, lev, i, j - 1) - curl_x(g, W, lev, i, j)) * bydx2;
+    }
     lap *= -1;
     double a = lap * F.lap_coef;
     a += F.an * n;
-    if (F.use_nprev) a += F.bn * (F.nprev_valid ? F.nprev[t] : n);
+    if (F.use_nprev) a += F.bn * (F.nprev_valid ? gin(F.nprev)[t] : n);
     a *= F.dt;
-    a += F.w_coef * W[t];
-    F.a[t] = a;
-    F.nprev[t] = n;
+    a += F.w_coef * gin(W)[t];
+    gout(F.a)[t] = a;
+    gout(F.nprev)[t] = n;
 }
 
 // q = Curl(psi) + q0 on every level (NavierStokesModel.cc:121-129): q0 is
@@ -260,31 +271,49 @@ __global__ void __launch_bounds__(kThreads) k_curl_q(const FlowOps* __restrict__
     const Lay g{F.nx, F.ny, F.L};
     const long t = (long)blockIdx.x * kThreads + threadIdx.x;
     if (t >= g.L * g.nf()) return;
-    const int lev = (int)(t / g.nf());
-    const long r = t - lev * g.nf();
+    // (sizes below 2^32: 32-bit divisions)
+    const unsigned nf = (unsigned)g.nf(), nfx = (unsigned)g.nfx();
+    const int lev = (int)((unsigned)t / nf);
+    const unsigned r = (unsigned)t - lev * nf;
     const double d = F.dx * (1 << lev);
+    const IBPM_GLOBAL double* P = gin(F.psi);
+    const long ny1 = g.ny - 1;
     double v;
     // level offsets: offset + 0.5 (shift - 1) (2^lev - 1) n dx with shift 0
     const double lofs = -0.5 * ((1 << lev) - 1);
-    if (r < g.nfx()) {
-        const int i = (int)(r / g.ny), j = (int)(r % g.ny);
+    if (r < nfx) {
+        const int i = (int)(r / (unsigned)g.ny), j = (int)(r % (unsigned)g.ny);
         double q0 = F.ubase * d;
         if (F.ubf) {
             const double y = (F.yoff + lofs * (F.ny * F.dx)) + (j + 0.5) * d;
             q0 = (F.bf[0] - F.bf[2] * (y - F.yc)) * d;
         }
-        v = curl_x(g, F.psi, lev, i, j) + q0;
+        double c;
+        if (i >= 1 && i <= g.nx - 1 && j >= 1 && j <= g.ny - 2) {   // curl_x's interior difference
+            const long k = g.s(lev, i, j);
+            c = P[k + 1] - P[k];
+        } else {
+            c = curl_x(g, F.psi, lev, i, j);
+        }
+        v = c + q0;
     } else {
-        const long u = r - g.nfx();
-        const int i = (int)(u / (g.ny + 1)), j = (int)(u % (g.ny + 1));
+        const unsigned u = r - nfx;
+        const int i = (int)(u / (unsigned)(g.ny + 1)), j = (int)(u % (unsigned)(g.ny + 1));
         double q0 = F.vbase * d;
         if (F.ubf) {
             const double x = (F.xoff + lofs * (F.nx * F.dx)) + (i + 0.5) * d;
             q0 = (F.bf[1] + F.bf[2] * (x - F.xc)) * d;
         }
-        v = curl_y(g, F.psi, lev, i, j) + q0;
+        double c;
+        if (j >= 1 && j <= g.ny - 1 && i >= 1 && i <= g.nx - 2) {   // curl_y's interior difference
+            const long k = g.s(lev, i, j);
+            c = P[k] - P[k + ny1];
+        } else {
+            c = curl_y(g, F.psi, lev, i, j);
+        }
+        v = c + q0;
     }
-    F.q[t] = v;
+    gout(F.q)[t] = v;
 }
 
 // ---------------------------------------------------------- regulariser

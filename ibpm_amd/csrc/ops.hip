@@ -136,23 +136,23 @@ __device__ inline double xflux_from(const Lay& g, const double* S, const double*
         if (lev == g.L - 1) return 0.0;
         const int ic = (i == 0) ? nx2 : g.nx / 2 + nx2;
         const int jj = j / 2 + ny2;
-        const double a = S[g.s(lev + 1, ic, jj)], b = S[g.s(lev + 1, ic, jj + 1)];
+        const double a = gin(S)[g.s(lev + 1, ic, jj)], b = gin(S)[g.s(lev + 1, ic, jj + 1)];
         return (j & 1) ? (0.25 * a + 0.75 * b) * dx : (0.75 * a + 0.25 * b) * dx;
     }
     if (j == 0 || j == g.ny - 1) {   // C
         const int jin = (j == 0) ? 1 : g.ny - 1;
-        const double s0 = S[g.s(lev, i, jin)];
+        const double s0 = gin(S)[g.s(lev, i, jin)];
         if (lev == g.L - 1) return s0 * 0.5 * dx;
         const int jc = (j == 0) ? ny2 : g.ny / 2 + ny2;
         const int ii = i / 2 + nx2;
-        if ((i & 1) == 0) return (s0 + S[g.s(lev + 1, ii, jc)]) * 0.5 * dx;
-        return (0.5 * s0 + 0.25 * S[g.s(lev + 1, ii, jc)] + 0.25 * S[g.s(lev + 1, ii + 1, jc)]) * dx;
+        if ((i & 1) == 0) return (s0 + gin(S)[g.s(lev + 1, ii, jc)]) * 0.5 * dx;
+        return (0.5 * s0 + 0.25 * gin(S)[g.s(lev + 1, ii, jc)] + 0.25 * gin(S)[g.s(lev + 1, ii + 1, jc)]) * dx;
     }
     if (lev > 0 && i > nx2 && i < g.nx / 2 + nx2 && j >= ny2 && j < g.ny / 2 + ny2) {   // G
         const int ii = 2 * (i - nx2), jj = 2 * (j - ny2);
-        return C[g.fx(lev - 1, ii, jj)] + C[g.fx(lev - 1, ii, jj + 1)];
+        return gin(C)[g.fx(lev - 1, ii, jj)] + gin(C)[g.fx(lev - 1, ii, jj + 1)];
     }
-    return (S[g.s(lev, i, j)] + S[g.s(lev, i, j + 1)]) * 0.5 * dx;
+    return (gin(S)[g.s(lev, i, j)] + gin(S)[g.s(lev, i, j + 1)]) * 0.5 * dx;
 }
 
 __device__ inline double yflux_from(const Lay& g, const double* S, const double* C, double dx0,
@@ -163,23 +163,23 @@ __device__ inline double yflux_from(const Lay& g, const double* S, const double*
         if (lev == g.L - 1) return 0.0;
         const int jc = (j == 0) ? ny2 : g.ny / 2 + ny2;
         const int ii = i / 2 + nx2;
-        const double a = S[g.s(lev + 1, ii, jc)], b = S[g.s(lev + 1, ii + 1, jc)];
+        const double a = gin(S)[g.s(lev + 1, ii, jc)], b = gin(S)[g.s(lev + 1, ii + 1, jc)];
         return (i & 1) ? (0.25 * a + 0.75 * b) * dx : (0.75 * a + 0.25 * b) * dx;
     }
     if (i == 0 || i == g.nx - 1) {   // C
         const int iin = (i == 0) ? 1 : g.nx - 1;
-        const double s0 = S[g.s(lev, iin, j)];
+        const double s0 = gin(S)[g.s(lev, iin, j)];
         if (lev == g.L - 1) return s0 * 0.5 * dx;
         const int ic = (i == 0) ? nx2 : g.nx / 2 + nx2;
         const int jj = j / 2 + ny2;
-        if ((j & 1) == 0) return (s0 + S[g.s(lev + 1, ic, jj)]) * 0.5 * dx;
-        return (0.5 * s0 + 0.25 * S[g.s(lev + 1, ic, jj)] + 0.25 * S[g.s(lev + 1, ic, jj + 1)]) * dx;
+        if ((j & 1) == 0) return (s0 + gin(S)[g.s(lev + 1, ic, jj)]) * 0.5 * dx;
+        return (0.5 * s0 + 0.25 * gin(S)[g.s(lev + 1, ic, jj)] + 0.25 * gin(S)[g.s(lev + 1, ic, jj + 1)]) * dx;
     }
     if (lev > 0 && j > ny2 && j < g.ny / 2 + ny2 && i >= nx2 && i < g.nx / 2 + nx2) {   // G
         const int ii = 2 * (i - nx2), jj = 2 * (j - ny2);
-        return C[g.fy(lev - 1, ii, jj)] + C[g.fy(lev - 1, ii + 1, jj)];
+        return gin(C)[g.fy(lev - 1, ii, jj)] + gin(C)[g.fy(lev - 1, ii + 1, jj)];
     }
-    return (S[g.s(lev, i, j)] + S[g.s(lev, i + 1, j)]) * 0.5 * dx;
+    return (gin(S)[g.s(lev, i, j)] + gin(S)[g.s(lev, i + 1, j)]) * 0.5 * dx;
 }
 
 // ------------------------------------------------------------- kernels
@@ -190,17 +190,18 @@ __global__ void __launch_bounds__(kThreads) k_vel(const FlowOps* __restrict__ fl
     const Lay g{F.nx, F.ny, F.L};
     const long t = (long)blockIdx.x * kThreads + threadIdx.x;
     if (t >= g.L * g.ns()) return;
-    const int lev = (int)(t / g.ns());
-    const long r = t - lev * g.ns();
-    const int i = (int)(r / (g.ny - 1)) + 1, j = (int)(r % (g.ny - 1)) + 1;
-    const double w = F.omega[t];
+    const unsigned ns = (unsigned)g.ns(), ny1 = (unsigned)(g.ny - 1);   // sizes below 2^32
+    const int lev = (int)((unsigned)t / ns);
+    const unsigned r = (unsigned)t - lev * ns;
+    const int i = (int)(r / ny1) + 1, j = (int)(r % ny1) + 1;
+    const double w = gin(F.omega)[t];
     double u = xvel(g, F.q, F.dx, lev, i, j);
     u *= w;
     u *= -1;
     double v = yvel(g, F.q, F.dx, lev, i, j);
     v *= w;
-    F.U[t] = u;
-    F.V[t] = v;
+    gout(F.U)[t] = u;
+    gout(F.V)[t] = v;
 }
 
 // cross = VelocityToFlux(V, U) on one level (levels run fine -> coarse).
@@ -210,16 +211,17 @@ __global__ void __launch_bounds__(kThreads) k_cross(const FlowOps* __restrict__ 
     if (lev >= g.L) return;
     const long t = (long)blockIdx.x * kThreads + threadIdx.x;
     if (t >= g.nf()) return;
+    const unsigned tu = (unsigned)t, nfx = (unsigned)g.nfx();   // sizes below 2^32
     double v;
-    if (t < g.nfx()) {
-        const int i = (int)(t / g.ny), j = (int)(t % g.ny);
+    if (tu < nfx) {
+        const int i = (int)(tu / (unsigned)g.ny), j = (int)(tu % (unsigned)g.ny);
         v = xflux_from(g, F.V, F.cross, F.dx, lev, i, j);
     } else {
-        const long u = t - g.nfx();
-        const int i = (int)(u / (g.ny + 1)), j = (int)(u % (g.ny + 1));
+        const unsigned u = tu - nfx;
+        const int i = (int)(u / (unsigned)(g.ny + 1)), j = (int)(u % (unsigned)(g.ny + 1));
         v = yflux_from(g, F.U, F.cross, F.dx, lev, i, j);
     }
-    F.cross[lev * g.nf() + t] = v;
+    gout(F.cross)[lev * g.nf() + t] = v;
 }
 
 // N = Curl(cross) (IBSolver.cc:205-209) and the right-hand side

@@ -503,6 +503,12 @@ __device__ inline void block_sums(double (&v)[K], double* red) {   // red: (kThr
 
 constexpr int kRed = (kThreads / 64 + 1) * kLmpMax;
 
+// Row of the pair arrays read for pair a when nk pairs are kept (nk >= 1):
+// pairs past the last read its row again, so every pair's load is issued
+// unconditionally; what they add is either discarded (sums for a >= nk are
+// never used) or multiplied by a coefficient that is exactly 0.
+__device__ inline int pair_row(int a, int nk) { return a < nk ? a : nk - 1; }
+
 // First half of the preconditioner for the current residual:
 //   c = G S^T r,  t = r - Y c   (t = r without pairs).
 __device__ inline void pcg_project(const PcgOps& C, int nk, const double* G, double* red) {
@@ -517,8 +523,7 @@ __device__ inline void pcg_project(const PcgOps& C, int nk, const double* G, dou
     for (int k = threadIdx.x; k < n; k += kThreads) {
         const double r = C.r[k];
 #pragma unroll
-        for (int a = 0; a < kLmpMax; ++a)
-            if (a < nk) v[a] += C.rs[(long)a * n + k] * r;
+        for (int a = 0; a < kLmpMax; ++a) v[a] += C.rs[(long)pair_row(a, nk) * n + k] * r;
     }
     block_sums<kLmpMax>(v, red);
     double c[kLmpMax];
@@ -538,8 +543,7 @@ __device__ inline void pcg_project(const PcgOps& C, int nk, const double* G, dou
     for (int k = threadIdx.x; k < n; k += kThreads) {
         double s = C.r[k];
 #pragma unroll
-        for (int a = 0; a < kLmpMax; ++a)
-            if (a < nk) s -= C.ry[(long)a * n + k] * c[a];
+        for (int a = 0; a < kLmpMax; ++a) s -= C.ry[(long)pair_row(a, nk) * n + k] * c[a];
         C.t[k] = s;
     }
 }
@@ -571,8 +575,10 @@ __global__ void __launch_bounds__(kThreads) k_pcg_begin(const PcgOps* __restrict
         for (int k = threadIdx.x; k < n; k += kThreads) {
             const double sa = C.rs[(long)a * n + k], ya = C.ry[(long)a * n + k];
 #pragma unroll
-            for (int b = 0; b < kLmpMax; ++b)
-                if (b < nk) v[b] += sa * C.ry[(long)b * n + k] + C.rs[(long)b * n + k] * ya;
+            for (int b = 0; b < kLmpMax; ++b) {
+                const long o = (long)pair_row(b, nk) * n + k;
+                v[b] += sa * C.ry[o] + C.rs[o] * ya;
+            }
         }
         block_sums<kLmpMax>(v, red);
         if (threadIdx.x == 0) {
@@ -688,8 +694,7 @@ __global__ void __launch_bounds__(kThreads) k_pcg_dir(const PcgOps* __restrict__
         for (int k = threadIdx.x; k < n; k += kThreads) {
             const double u = C.u[k];
 #pragma unroll
-            for (int a = 0; a < kLmpMax; ++a)
-                if (a < nk) v[a] += C.ry[(long)a * n + k] * u;
+            for (int a = 0; a < kLmpMax; ++a) v[a] += C.ry[(long)pair_row(a, nk) * n + k] * u;
         }
         block_sums<kLmpMax>(v, red);
 #pragma unroll
@@ -704,9 +709,10 @@ __global__ void __launch_bounds__(kThreads) k_pcg_dir(const PcgOps* __restrict__
     double v1[1] = {0.0};
     for (int k = threadIdx.x; k < n; k += kThreads) {
         double z = C.u[k];
+        if (nk > 0) {
 #pragma unroll
-        for (int a = 0; a < kLmpMax; ++a)
-            if (a < nk) z += C.rs[(long)a * n + k] * w[a];
+            for (int a = 0; a < kLmpMax; ++a) z += C.rs[(long)pair_row(a, nk) * n + k] * w[a];
+        }
         C.u[k] = z;
         v1[0] += C.r[k] * z;
     }

@@ -326,16 +326,60 @@ __device__ inline double curl_flat(const Lay& g, const double* psi, int fidx) {
     return curl_y(g, psi, 0, u / (g.ny + 1), u % (g.ny + 1));
 }
 
+// The two level-0 values whose difference is Curl(psi) at a flat flux index,
+// Curl = psi[p0] - psi[p1], when the flux is interior (curl_x / curl_y take
+// their last branch); false, with p0 = p1 = 0, otherwise.
+__device__ inline bool interior_curl(const Lay& g, int fidx, int& p0, int& p1) {
+    p0 = p1 = 0;
+    const int nfx = (g.nx + 1) * g.ny, ny1 = g.ny - 1;
+    if (fidx < nfx) {
+        const int i = fidx / g.ny, j = fidx % g.ny;
+        if (i < 1 || i > g.nx - 1 || j < 1 || j > g.ny - 2) return false;
+        p1 = (i - 1) * ny1 + (j - 1);
+        p0 = p1 + 1;
+        return true;
+    }
+    const int u = fidx - nfx, i = u / (g.ny + 1), j = u % (g.ny + 1);
+    if (j < 1 || j > g.ny - 1 || i < 1 || i > g.nx - 2) return false;
+    p0 = (i - 1) * ny1 + (j - 1);
+    p1 = p0 + ny1;
+    return true;
+}
+
 // f[k] = E(Curl psi)[k] - b[k]: toBoundary (Regularizer.cc:111-128) of the
 // flux of psi, each component summed in ascending flux order as the
-// reference's association list does; b may be null.
+// reference's association list does; b may be null.  Four entries of the
+// list are read at a time, their psi values loaded together from valid
+// addresses; a flux next to the level's edge takes curl_flat instead.
 __global__ void __launch_bounds__(kThreads) k_interp(const RegOps* __restrict__ regs) {
     const RegOps& R = regs[blockIdx.y];
     const int k = blockIdx.x * kThreads + threadIdx.x;
     if (k >= 2 * R.nb || skipped(R.skip)) return;
     const Lay g{R.nx, R.ny, R.L};
+    const int a0 = R.e_ptr[k], a1 = R.e_ptr[k + 1];
     double s = 0.0;
-    for (int a = R.e_ptr[k]; a < R.e_ptr[k + 1]; ++a) s += R.e_w[a] * curl_flat(g, R.psi, R.e_f[a]);
+    for (int a = a0; a < a1; a += 4) {
+        int f[4];
+        double w[4], c[4], m[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = min(a + u, a1 - 1);
+            f[u] = R.e_f[e];
+            w[u] = R.e_w[e];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            int p0, p1;
+            m[u] = interior_curl(g, f[u], p0, p1) ? 1.0 : 0.0;
+            c[u] = R.psi[p0] - R.psi[p1];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const double edge = (m[u] != 0.0) ? 0.0 : curl_flat(g, R.psi, f[u]);
+            const double v = c[u] * m[u] + edge;   // exactly c or exactly edge
+            if (a + u < a1) s += w[u] * v;
+        }
+    }
     s *= R.inv_dx;
     if (R.b) s -= R.b[k];
     R.out[k] = s;
@@ -401,6 +445,7 @@ __global__ void __launch_bounds__(kThreads) k_minv(const MinvOps* __restrict__ m
     if (row >= P.n || skipped(P.skip)) return;
     const double* a = P.minv + (long)row * P.n;
     double s = 0.0;
+#pragma unroll 4   // loads of four iterations in flight; each lane's sum keeps its order
     for (int k = lane; k < P.n; k += 64) s += a[k] * P.rhs[k];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);

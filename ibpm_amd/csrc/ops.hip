@@ -264,58 +264,79 @@ __global__ void __launch_bounds__(kThreads) k_rhs(const FlowOps* __restrict__ fl
     gout(F.nprev)[t] = n;
 }
 
+// One flux of q = Curl psi + q0 (flat index t over all levels of one flow):
+// the two psi offsets of curl_x's / curl_y's interior difference,
+// Curl = psi[ka] - psi[kb] (ka = kb = 0 at a flux next to the level's edge,
+// whose Curl the getBC branches give), and q0.
+struct CurlQAt {
+    unsigned ka, kb;
+    bool interior, xflux;
+    int lev, i, j;
+    double q0;
+};
+__device__ inline CurlQAt curl_q_at(const FlowOps& F, const Lay& g, unsigned t) {
+    // (sizes below 2^32: 32-bit divisions)
+    const unsigned nf = (unsigned)g.nf(), nfx = (unsigned)g.nfx(), ny1 = (unsigned)(g.ny - 1);
+    CurlQAt a;
+    a.lev = (int)(t / nf);
+    const unsigned r = t - a.lev * nf, lbase = a.lev * (unsigned)g.ns();
+    const double d = F.dx * (1 << a.lev);
+    // level offsets: offset + 0.5 (shift - 1) (2^lev - 1) n dx with shift 0
+    const double lofs = -0.5 * ((1 << a.lev) - 1);
+    a.xflux = r < nfx;
+    if (a.xflux) {
+        a.i = (int)(r / (unsigned)g.ny);
+        a.j = (int)(r % (unsigned)g.ny);
+        a.q0 = F.ubase * d;
+        if (F.ubf) {
+            const double y = (F.yoff + lofs * (F.ny * F.dx)) + (a.j + 0.5) * d;
+            a.q0 = (F.bf[0] - F.bf[2] * (y - F.yc)) * d;
+        }
+        a.interior = a.i >= 1 && a.i <= g.nx - 1 && a.j >= 1 && a.j <= g.ny - 2;
+        const unsigned k = lbase + (a.i - 1) * ny1 + (a.j - 1);
+        a.ka = a.interior ? k + 1 : 0;
+        a.kb = a.interior ? k : 0;
+    } else {
+        const unsigned u = r - nfx;
+        a.i = (int)(u / (unsigned)(g.ny + 1));
+        a.j = (int)(u % (unsigned)(g.ny + 1));
+        a.q0 = F.vbase * d;
+        if (F.ubf) {
+            const double x = (F.xoff + lofs * (F.nx * F.dx)) + (a.i + 0.5) * d;
+            a.q0 = (F.bf[1] + F.bf[2] * (x - F.xc)) * d;
+        }
+        a.interior = a.j >= 1 && a.j <= g.ny - 1 && a.i >= 1 && a.i <= g.nx - 2;
+        const unsigned k = lbase + (a.i - 1) * ny1 + (a.j - 1);
+        a.ka = a.interior ? k : 0;
+        a.kb = a.interior ? k + ny1 : 0;
+    }
+    return a;
+}
+
 // q = Curl(psi) + q0 on every level (NavierStokesModel.cc:121-129): q0 is
 // the uniform base flow (Flux.cc:79-99) or the unsteady one, free stream plus
 // rotation about the body centre at the edge coordinates of each level
-// (Flux.cc:101-129, Grid.cc:104-139).
+// (Flux.cc:101-129, Grid.cc:104-139).  Two consecutive fluxes per thread:
+// their four psi values are loaded together from valid offsets, and a flux
+// next to the level's edge takes curl_x / curl_y instead, blended in exactly
+// (the interior difference times 1 plus 0, or times 0 plus the edge value).
 __global__ void __launch_bounds__(kThreads) k_curl_q(const FlowOps* __restrict__ flows) {
     const FlowOps& F = flows[blockIdx.y];
     const Lay g{F.nx, F.ny, F.L};
-    const long t = (long)blockIdx.x * kThreads + threadIdx.x;
-    if (t >= g.L * g.nf()) return;
-    // (sizes below 2^32: 32-bit divisions)
-    const unsigned nf = (unsigned)g.nf(), nfx = (unsigned)g.nfx();
-    const int lev = (int)((unsigned)t / nf);
-    const unsigned r = (unsigned)t - lev * nf;
-    const double d = F.dx * (1 << lev);
+    const long total = g.L * g.nf();
+    const long t0 = 2 * ((long)blockIdx.x * kThreads + threadIdx.x);
+    if (t0 >= total) return;
+    const bool two = t0 + 1 < total;
+    const CurlQAt a = curl_q_at(F, g, (unsigned)t0), b = curl_q_at(F, g, (unsigned)(two ? t0 + 1 : t0));
     const IBPM_GLOBAL double* P = gin(F.psi);
-    const long ny1 = g.ny - 1;
-    double v;
-    // level offsets: offset + 0.5 (shift - 1) (2^lev - 1) n dx with shift 0
-    const double lofs = -0.5 * ((1 << lev) - 1);
-    if (r < nfx) {
-        const int i = (int)(r / (unsigned)g.ny), j = (int)(r % (unsigned)g.ny);
-        double q0 = F.ubase * d;
-        if (F.ubf) {
-            const double y = (F.yoff + lofs * (F.ny * F.dx)) + (j + 0.5) * d;
-            q0 = (F.bf[0] - F.bf[2] * (y - F.yc)) * d;
-        }
-        double c;
-        if (i >= 1 && i <= g.nx - 1 && j >= 1 && j <= g.ny - 2) {   // curl_x's interior difference
-            const long k = g.s(lev, i, j);
-            c = P[k + 1] - P[k];
-        } else {
-            c = curl_x(g, F.psi, lev, i, j);
-        }
-        v = c + q0;
-    } else {
-        const unsigned u = r - nfx;
-        const int i = (int)(u / (unsigned)(g.ny + 1)), j = (int)(u % (unsigned)(g.ny + 1));
-        double q0 = F.vbase * d;
-        if (F.ubf) {
-            const double x = (F.xoff + lofs * (F.nx * F.dx)) + (i + 0.5) * d;
-            q0 = (F.bf[1] + F.bf[2] * (x - F.xc)) * d;
-        }
-        double c;
-        if (j >= 1 && j <= g.ny - 1 && i >= 1 && i <= g.nx - 2) {   // curl_y's interior difference
-            const long k = g.s(lev, i, j);
-            c = P[k] - P[k + ny1];
-        } else {
-            c = curl_y(g, F.psi, lev, i, j);
-        }
-        v = c + q0;
-    }
-    gout(F.q)[t] = v;
+    const double pa0 = P[a.ka], pa1 = P[a.kb], pb0 = P[b.ka], pb1 = P[b.kb];
+    const double ma = a.interior ? 1.0 : 0.0, mb = b.interior ? 1.0 : 0.0;
+    const double ea = a.interior ? 0.0 : (a.xflux ? curl_x(g, F.psi, a.lev, a.i, a.j) : curl_y(g, F.psi, a.lev, a.i, a.j));
+    const double eb = b.interior ? 0.0 : (b.xflux ? curl_x(g, F.psi, b.lev, b.i, b.j) : curl_y(g, F.psi, b.lev, b.i, b.j));
+    const double ca = (pa0 - pa1) * ma + ea, cb = (pb0 - pb1) * mb + eb;
+    IBPM_GLOBAL double* q = gout(F.q);
+    q[t0] = ca + a.q0;
+    if (two) q[t0 + 1] = cb + b.q0;
 }
 
 // ---------------------------------------------------------- regulariser
@@ -809,7 +830,7 @@ void launch_rhs(const FlowOps* d, int nflows, long max_pts, hipStream_t s) {
     hipLaunchKernelGGL(k_rhs, dim3(blocks_for(max_pts), nflows), dim3(kThreads), 0, s, d);
 }
 void launch_curl_q(const FlowOps* d, int nflows, long max_nf_all, hipStream_t s) {
-    hipLaunchKernelGGL(k_curl_q, dim3(blocks_for(max_nf_all), nflows), dim3(kThreads), 0, s, d);
+    hipLaunchKernelGGL(k_curl_q, dim3(blocks_for((max_nf_all + 1) / 2), nflows), dim3(kThreads), 0, s, d);
 }
 void launch_interp(const RegOps* d, int nflows, int max_comp, hipStream_t s) {
     hipLaunchKernelGGL(k_interp, dim3(blocks_for(max_comp), nflows), dim3(kThreads), 0, s, d);

@@ -204,24 +204,78 @@ __global__ void __launch_bounds__(kThreads) k_vel(const FlowOps* __restrict__ fl
     gout(F.V)[t] = v;
 }
 
-// cross = VelocityToFlux(V, U) on one level (levels run fine -> coarse).
+// One flux of cross = VelocityToFlux(V, U) on level lev (flat flux index t
+// within the level), as xflux_from / yflux_from read it:
+//  - next to the level's edge (their A and C cases): edge = true, whose value
+//    those functions give;
+//  - in the central region of a coarse level (G): src[ka] + src[kb] with src
+//    the finer level's cross;
+//  - otherwise (src[ka] + src[kb]) * 0.5 * dx with src = V (x-fluxes) or U.
+// ka = kb = 0 with src = V or U for an edge flux.
+struct CrossAt {
+    const double* src;
+    unsigned ka, kb;
+    bool edge, central, xflux;
+    int i, j;
+};
+__device__ inline CrossAt cross_at(const FlowOps& F, const Lay& g, int lev, unsigned t) {
+    // (sizes below 2^32: 32-bit offsets)
+    const unsigned nfx = (unsigned)g.nfx(), nf = (unsigned)g.nf(), ny1 = (unsigned)(g.ny - 1);
+    const int nx2 = nx_ext(g.nx), ny2 = nx_ext(g.ny);
+    CrossAt a;
+    a.xflux = t < nfx;
+    if (a.xflux) {
+        a.i = (int)(t / (unsigned)g.ny);
+        a.j = (int)(t % (unsigned)g.ny);
+        a.edge = a.i == 0 || a.i == g.nx || a.j == 0 || a.j == g.ny - 1;
+        a.central = !a.edge && lev > 0 && a.i > nx2 && a.i < g.nx / 2 + nx2 && a.j >= ny2 && a.j < g.ny / 2 + ny2;
+        const unsigned kc = (lev - 1) * nf + (unsigned)(2 * (a.i - nx2)) * g.ny + 2 * (a.j - ny2);   // fx(lev-1, ii, jj)
+        const unsigned ks = lev * (unsigned)g.ns() + (a.i - 1) * ny1 + (a.j - 1);                     // s(lev, i, j)
+        a.ka = a.edge ? 0 : (a.central ? kc : ks);
+        a.kb = a.edge ? 0 : a.ka + 1;
+        a.src = a.central ? F.cross : F.V;
+    } else {
+        const unsigned u = t - nfx;
+        a.i = (int)(u / (unsigned)(g.ny + 1));
+        a.j = (int)(u % (unsigned)(g.ny + 1));
+        a.edge = a.j == 0 || a.j == g.ny || a.i == 0 || a.i == g.nx - 1;
+        a.central = !a.edge && lev > 0 && a.j > ny2 && a.j < g.ny / 2 + ny2 && a.i >= nx2 && a.i < g.nx / 2 + nx2;
+        const unsigned kc = (lev - 1) * nf + nfx + (unsigned)(2 * (a.i - nx2)) * (g.ny + 1) + 2 * (a.j - ny2);   // fy
+        const unsigned ks = lev * (unsigned)g.ns() + (a.i - 1) * ny1 + (a.j - 1);
+        a.ka = a.edge ? 0 : (a.central ? kc : ks);
+        a.kb = a.edge ? 0 : a.ka + (a.central ? (unsigned)(g.ny + 1) : ny1);
+        a.src = a.central ? F.cross : F.U;
+    }
+    return a;
+}
+
+__device__ inline double cross_value(const FlowOps& F, const Lay& g, int lev, const CrossAt& a, double x0,
+                                     double x1) {
+    const double dx = F.dx * (1 << lev);
+    const double sum = x0 + x1, mean = sum * 0.5 * dx;
+    const double mc = a.central ? 1.0 : 0.0, md = (a.edge || a.central) ? 0.0 : 1.0;
+    const double edge = a.edge ? (a.xflux ? xflux_from(g, F.V, F.cross, F.dx, lev, a.i, a.j)
+                                          : yflux_from(g, F.U, F.cross, F.dx, lev, a.i, a.j))
+                               : 0.0;
+    return (sum * mc + mean * md) + edge;   // exactly one of the three terms is kept
+}
+
+// cross = VelocityToFlux(V, U) on one level (levels run fine -> coarse), two
+// consecutive fluxes per thread with their four loads issued together.
 __global__ void __launch_bounds__(kThreads) k_cross(const FlowOps* __restrict__ flows, int lev) {
     const FlowOps& F = flows[blockIdx.y];
     const Lay g{F.nx, F.ny, F.L};
     if (lev >= g.L) return;
-    const long t = (long)blockIdx.x * kThreads + threadIdx.x;
-    if (t >= g.nf()) return;
-    const unsigned tu = (unsigned)t, nfx = (unsigned)g.nfx();   // sizes below 2^32
-    double v;
-    if (tu < nfx) {
-        const int i = (int)(tu / (unsigned)g.ny), j = (int)(tu % (unsigned)g.ny);
-        v = xflux_from(g, F.V, F.cross, F.dx, lev, i, j);
-    } else {
-        const unsigned u = tu - nfx;
-        const int i = (int)(u / (unsigned)(g.ny + 1)), j = (int)(u % (unsigned)(g.ny + 1));
-        v = yflux_from(g, F.U, F.cross, F.dx, lev, i, j);
-    }
-    gout(F.cross)[lev * g.nf() + t] = v;
+    const long total = g.nf();
+    const long t0 = 2 * ((long)blockIdx.x * kThreads + threadIdx.x);
+    if (t0 >= total) return;
+    const bool two = t0 + 1 < total;
+    const CrossAt a = cross_at(F, g, lev, (unsigned)t0), b = cross_at(F, g, lev, (unsigned)(two ? t0 + 1 : t0));
+    const double a0 = gin(a.src)[a.ka], a1 = gin(a.src)[a.kb], b0 = gin(b.src)[b.ka], b1 = gin(b.src)[b.kb];
+    const double va = cross_value(F, g, lev, a, a0, a1), vb = cross_value(F, g, lev, b, b0, b1);
+    IBPM_GLOBAL double* out = gout(F.cross) + lev * total;
+    out[t0] = va;
+    if (two) out[t0 + 1] = vb;
 }
 
 // N = Curl(cross) (IBSolver.cc:205-209) and the right-hand side
@@ -824,7 +878,7 @@ void launch_vel(const FlowOps* d, int nflows, long max_pts, hipStream_t s) {
     hipLaunchKernelGGL(k_vel, dim3(blocks_for(max_pts), nflows), dim3(kThreads), 0, s, d);
 }
 void launch_cross(const FlowOps* d, int nflows, long max_nf, int lev, hipStream_t s) {
-    hipLaunchKernelGGL(k_cross, dim3(blocks_for(max_nf), nflows), dim3(kThreads), 0, s, d, lev);
+    hipLaunchKernelGGL(k_cross, dim3(blocks_for((max_nf + 1) / 2), nflows), dim3(kThreads), 0, s, d, lev);
 }
 void launch_rhs(const FlowOps* d, int nflows, long max_pts, hipStream_t s) {
     hipLaunchKernelGGL(k_rhs, dim3(blocks_for(max_pts), nflows), dim3(kThreads), 0, s, d);

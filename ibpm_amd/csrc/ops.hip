@@ -294,26 +294,33 @@ __global__ void __launch_bounds__(kThreads) k_rhs(const FlowOps* __restrict__ fl
     const double d = F.dx * (1 << lev);
     const double bydx2 = 1. / (d * d);
     const IBPM_GLOBAL double* C = gin(F.cross);
-    const double n = (C[g.fy(lev, i, j)] - C[g.fy(lev, i - 1, j)]
-                      + C[g.fx(lev, i, j - 1)] - C[g.fx(lev, i, j)]) * bydx2;
     const double* W = F.omega;
-    double lap;
-    if (i >= 2 && i <= g.nx - 2 && j >= 2 && j <= g.ny - 2) {
-        // away from the level's edge the curls are plain differences: the
-        // general expression below, term for term and in the same order
-        const IBPM_GLOBAL double* Wg = gin(W);
-        const double c = Wg[t], e = Wg[t + ny1], w = Wg[t - ny1], up = Wg[t + 1], dn = Wg[t - 1];
-        lap = ((c - e) - (w - c) + (c - dn) - (up - c)) * bydx2;
-    } else {
-        lap = (curl_y(g, W, lev, i, j) - curl_y(g, W, lev, i - 1, j)
-               + curl_x(g, W, lev, i, j - 1) - curl_x(g, W, lev, i, j)) * bydx2;
-    }
+    const IBPM_GLOBAL double* Wg = gin(W);
+    // Away from the level's edge the curls of Lap(omega) are plain differences
+    // of the four neighbours (the general expression, term for term and in the
+    // same order).  Their offsets fall back to t itself next to the edge, so
+    // all ten loads below are issued before any value is used; N_prev is read
+    // from its buffer (always allocated) and used only when valid.
+    const bool interior = i >= 2 && i <= g.nx - 2 && j >= 2 && j <= g.ny - 2;
+    const unsigned tu = (unsigned)t;
+    const unsigned te = interior ? tu + ny1 : tu, tw = interior ? tu - ny1 : tu;
+    const unsigned tn = interior ? tu + 1 : tu, ts = interior ? tu - 1 : tu;
+    const double cy1 = C[g.fy(lev, i, j)], cy0 = C[g.fy(lev, i - 1, j)];
+    const double cx0 = C[g.fx(lev, i, j - 1)], cx1 = C[g.fx(lev, i, j)];
+    const double c = Wg[tu], e = Wg[te], w = Wg[tw], up = Wg[tn], dn = Wg[ts];
+    const double np = gin(F.nprev)[tu];
+    const double n = (cy1 - cy0 + cx0 - cx1) * bydx2;
+    // the interior difference times 1 plus 0, or times 0 plus the edge value
+    const double edge = interior ? 0.0
+                                 : (curl_y(g, W, lev, i, j) - curl_y(g, W, lev, i - 1, j)
+                                    + curl_x(g, W, lev, i, j - 1) - curl_x(g, W, lev, i, j)) * bydx2;
+    double lap = ((c - e) - (w - c) + (c - dn) - (up - c)) * bydx2 * (interior ? 1.0 : 0.0) + edge;
     lap *= -1;
     double a = lap * F.lap_coef;
     a += F.an * n;
-    if (F.use_nprev) a += F.bn * (F.nprev_valid ? gin(F.nprev)[t] : n);
+    if (F.use_nprev) a += F.bn * (F.nprev_valid ? np : n);
     a *= F.dt;
-    a += F.w_coef * gin(W)[t];
+    a += F.w_coef * c;
     gout(F.a)[t] = a;
     gout(F.nprev)[t] = n;
 }

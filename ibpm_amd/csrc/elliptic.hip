@@ -308,7 +308,7 @@ constexpr long kRegMinPairs = 1024;
 #define IBPM_R16_GROUP1 4
 #endif
 #ifndef IBPM_R16_GROUP2
-#define IBPM_R16_GROUP2 2
+#define IBPM_R16_GROUP2 1
 #endif
 __host__ __device__ constexpr int r16_group(int pass) {
     return pass == 1 ? IBPM_R16_GROUP1 : pass == 2 ? IBPM_R16_GROUP2 : 1;

@@ -1137,7 +1137,7 @@ __global__ void __launch_bounds__(kThreads) k_fs_coarsify(const CoarsifyJob* __r
 // a sweep over global memory, so the result does not depend on the blocking.
 // Launches alternate between tmp and out so that the last writes out.
 #ifndef IBPM_HELM_DEPTH
-#define IBPM_HELM_DEPTH 4
+#define IBPM_HELM_DEPTH 6
 #endif
 constexpr int kHelmTile = 32, kHelmDepth = IBPM_HELM_DEPTH, kHelmReg = kHelmTile + 2 * kHelmDepth;
 static_assert(kHelmDepth >= 1 && kHelmDepth <= 12, "sweeps per launch: 1..12 (LDS tiles with the halo)");

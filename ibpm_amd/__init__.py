@@ -65,6 +65,8 @@ def load():
                              ctypes.c_double, ctypes.c_double, vp, vp],
         "ibpm_nonlinear": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, vp, vp, vp],
         "ibpm_laplacian": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, vp, vp],
+        "ibpm_cross_product": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, vp, vp, vp],
+        "ibpm_cross_flux_flux": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, vp, vp, vp],
         "ibpm_batch_create": [vp, ctypes.c_int, ctypes.POINTER(FlowParams), ctypes.POINTER(ctypes.c_void_p)],
         "ibpm_batch_destroy": [vp], "ibpm_batch_init": [vp],
         "ibpm_batch_advance": [vp, ctypes.c_int],
@@ -218,6 +220,22 @@ class Context:
         d_n = DeviceArray(self, ngrid * (nx - 1) * (ny - 1))
         _check(load().ibpm_nonlinear(self.h, nx, ny, ngrid, dx, d_q.ptr, d_w.ptr, d_n.ptr))
         return d_n.get()
+
+    def cross_product(self, nx, ny, ngrid, dx, q, w):
+        """CrossProduct(Flux, Scalar) -> Flux."""
+        d_q = DeviceArray.of(self, q)
+        d_w = DeviceArray.of(self, w)
+        d_o = DeviceArray(self, ngrid * (2 * nx * ny + nx + ny))
+        _check(load().ibpm_cross_product(self.h, nx, ny, ngrid, dx, d_q.ptr, d_w.ptr, d_o.ptr))
+        return d_o.get()
+
+    def cross_flux_flux(self, nx, ny, ngrid, dx, q1, q2):
+        """CrossProduct(Flux, Flux) -> Scalar."""
+        d_1 = DeviceArray.of(self, q1)
+        d_2 = DeviceArray.of(self, q2)
+        d_o = DeviceArray(self, ngrid * (nx - 1) * (ny - 1))
+        _check(load().ibpm_cross_flux_flux(self.h, nx, ny, ngrid, dx, d_1.ptr, d_2.ptr, d_o.ptr))
+        return d_o.get()
 
     def rhs(self, nx, ny, ngrid, dx, dt, nu, an, bn, q, w, nprev=None):
         """(a, N) of one substep (IBSolver::advanceSubstep)."""

@@ -2134,6 +2134,53 @@ int ibpm_nonlinear(ibpm_ctx* c, int nx, int ny, int ngrid, double dx, const doub
     return 0;
 }
 
+int ibpm_cross_product(ibpm_ctx* c, int nx, int ny, int ngrid, double dx, const double* d_q, const double* d_w,
+                       double* d_out) {
+    DevMem mem;
+    FlowOps o{};
+    const long ns = (long)(nx - 1) * (ny - 1) * ngrid;
+    const long nf1 = 2L * nx * ny + nx + ny;
+    o.q = const_cast<double*>(d_q);
+    o.omega = const_cast<double*>(d_w);
+    o.cross = d_out;
+    CKR(ops_common(c, nx, ny, ngrid, dx, o, mem));
+    FlowOps* d = nullptr;
+    CKR(mem.alloc(&d, 1));
+    CK(hipMemcpyAsync(d, &o, sizeof(o), hipMemcpyHostToDevice, c->c.stream));
+    launch_vel(d, 1, ns, c->c.stream);
+    for (int lev = 0; lev < ngrid; ++lev) launch_cross(d, 1, nf1, lev, c->c.stream);
+    CK(hipStreamSynchronize(c->c.stream));
+    CK(hipGetLastError());
+    return 0;
+}
+
+// Scalar::coarsify in place over levels 1..ngrid-1, in order (Scalar.cc:46-63):
+// each level reads the finer one, already coarsified.
+static void coarsify_in_place(int nx, int ngrid, const CoarsifyJob* d_jobs, hipStream_t s) {
+    for (int lev = 1; lev < ngrid; ++lev) launch_coarsify(d_jobs + lev, 1, nx / 2 - 1, s);
+}
+
+int ibpm_cross_flux_flux(ibpm_ctx* c, int nx, int ny, int ngrid, double dx, const double* d_q1,
+                         const double* d_q2, double* d_out) {
+    CKR(check_grid(nx, ny, ngrid));
+    DevMem mem;
+    const long ns = (long)(nx - 1) * (ny - 1) * ngrid;
+    CrossFFOps o{nx, ny, ngrid, dx, d_q1, d_q2, d_out};
+    std::vector<CoarsifyJob> jobs(ngrid);
+    for (int lev = 0; lev < ngrid; ++lev) jobs[lev] = CoarsifyJob{nx, ny, lev, d_out, d_out, nullptr};
+    CrossFFOps* d = nullptr;
+    CoarsifyJob* dj = nullptr;
+    CKR(mem.alloc(&d, 1));
+    CKR(mem.alloc(&dj, ngrid));
+    CK(hipMemcpyAsync(d, &o, sizeof(o), hipMemcpyHostToDevice, c->c.stream));
+    CK(hipMemcpyAsync(dj, jobs.data(), sizeof(CoarsifyJob) * ngrid, hipMemcpyHostToDevice, c->c.stream));
+    launch_cross_ff(d, 1, ns, c->c.stream);
+    coarsify_in_place(nx, ngrid, dj, c->c.stream);
+    CK(hipStreamSynchronize(c->c.stream));
+    CK(hipGetLastError());
+    return 0;
+}
+
 int ibpm_rhs(ibpm_ctx* c, int nx, int ny, int ngrid, double dx, double dt, double nu, double an, double bn,
              const double* d_q, const double* d_w, const double* d_nprev, double* d_a, double* d_n) {
     DevMem mem;

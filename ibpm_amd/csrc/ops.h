@@ -22,6 +22,15 @@ struct FlowOps {
     double xc, yc, xoff, yoff;     // centre of rotation; level-0 grid offsets
 };
 
+// CrossProduct(Flux, Flux) of one flow before its coarsify: out = X(q1) Y(q2)
+// - X(q2) Y(q1) at the nodes of every level.
+struct CrossFFOps {
+    int nx, ny, L;
+    double dx;
+    const double *q1, *q2;
+    double* out;
+};
+
 // Regulariser gathers (E, and beta Curl E^T) of one flow.
 struct RegOps {
     int nx, ny, L, nb;
@@ -131,6 +140,7 @@ void launch_helm_sweeps(const HelmJob* d_jobs, int njobs, int max_tiles, int nsw
 void launch_helm_coarsify(const HelmJob* d_jobs, int njobs, int max_blocks, int lev, hipStream_t s);
 void launch_vel(const FlowOps* d, int nflows, long max_pts, hipStream_t s);
 void launch_cross(const FlowOps* d, int nflows, long max_nf, int lev, hipStream_t s);
+void launch_cross_ff(const CrossFFOps* d, int nflows, long max_pts, hipStream_t s);
 void launch_rhs(const FlowOps* d, int nflows, long max_pts, hipStream_t s);
 void launch_curl_q(const FlowOps* d, int nflows, long max_nf_all, hipStream_t s);
 void launch_interp(const RegOps* d, int nflows, int max_comp, hipStream_t s);

@@ -204,6 +204,24 @@ __global__ void __launch_bounds__(kThreads) k_vel(const FlowOps* __restrict__ fl
     gout(F.V)[t] = v;
 }
 
+// CrossProduct(Flux, Flux) -> Scalar before its coarsify (VectorOperations.cc
+// :465-486): u1 v2 - u2 v1 at every node of every level, with the two
+// products rounded separately as the reference forms them.
+__global__ void __launch_bounds__(kThreads) k_cross_ff(const CrossFFOps* __restrict__ ops) {
+#pragma clang fp contract(off)
+    const CrossFFOps& F = ops[blockIdx.y];
+    const Lay g{F.nx, F.ny, F.L};
+    const long t = (long)blockIdx.x * kThreads + threadIdx.x;
+    if (t >= g.L * g.ns()) return;
+    const unsigned ns = (unsigned)g.ns(), ny1 = (unsigned)(g.ny - 1);
+    const int lev = (int)((unsigned)t / ns);
+    const unsigned r = (unsigned)t - lev * ns;
+    const int i = (int)(r / ny1) + 1, j = (int)(r % ny1) + 1;
+    const double a = xvel(g, F.q1, F.dx, lev, i, j) * yvel(g, F.q2, F.dx, lev, i, j);
+    const double b = xvel(g, F.q2, F.dx, lev, i, j) * yvel(g, F.q1, F.dx, lev, i, j);
+    gout(F.out)[t] = a - b;
+}
+
 // One flux of cross = VelocityToFlux(V, U) on level lev (flat flux index t
 // within the level), as xflux_from / yflux_from read it:
 //  - next to the level's edge (their A and C cases): edge = true, whose value
@@ -886,6 +904,9 @@ void launch_vel(const FlowOps* d, int nflows, long max_pts, hipStream_t s) {
 }
 void launch_cross(const FlowOps* d, int nflows, long max_nf, int lev, hipStream_t s) {
     hipLaunchKernelGGL(k_cross, dim3(blocks_for((max_nf + 1) / 2), nflows), dim3(kThreads), 0, s, d, lev);
+}
+void launch_cross_ff(const CrossFFOps* d, int nflows, long max_pts, hipStream_t s) {
+    hipLaunchKernelGGL(k_cross_ff, dim3(blocks_for(max_pts), nflows), dim3(kThreads), 0, s, d);
 }
 void launch_rhs(const FlowOps* d, int nflows, long max_pts, hipStream_t s) {
     hipLaunchKernelGGL(k_rhs, dim3(blocks_for(max_pts), nflows), dim3(kThreads), 0, s, d);

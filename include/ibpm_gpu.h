@@ -63,6 +63,13 @@ int ibpm_curl_scalar(ibpm_ctx* ctx, int nx, int ny, int ngrid, double dx, double
 /* NonlinearIBSolver::N = Curl(CrossProduct(q, omega)), src/IBSolver.cc:205-209 */
 int ibpm_nonlinear(ibpm_ctx* ctx, int nx, int ny, int ngrid, double dx, const double* d_q,
                    const double* d_w, double* d_n);
+/* CrossProduct(Flux q, Scalar omega) -> Flux (omega v, -omega u), src/VectorOperations.cc:438-457 */
+int ibpm_cross_product(ibpm_ctx* ctx, int nx, int ny, int ngrid, double dx, const double* d_q,
+                       const double* d_w, double* d_out);
+/* CrossProduct(Flux q1, Flux q2) -> Scalar u1 v2 - u2 v1, coarsified over the
+   overlap regions, src/VectorOperations.cc:465-486 */
+int ibpm_cross_flux_flux(ibpm_ctx* ctx, int nx, int ny, int ngrid, double dx, const double* d_q1,
+                         const double* d_q2, double* d_out);
 /* Laplacian = -Curl(Curl(omega)), src/VectorOperations.cc:133-140 */
 int ibpm_laplacian(ibpm_ctx* ctx, int nx, int ny, int ngrid, double dx, const double* d_w, double* d_out);
 /* Right-hand side of one substep, IBSolver::advanceSubstep (src/IBSolver.cc:167-183) with

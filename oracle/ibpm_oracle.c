@@ -691,6 +691,22 @@ void og_cross_product(int nx, int ny, int ngrid, double dx, const double* q, con
     free(u); free(v);
 }
 
+/* CrossProduct(Flux, Flux) -> Scalar, VectorOperations.cc:465-486:
+   u1 v2 - u2 v1 at the nodes of every level, then coarsify over the overlap. */
+void og_cross_flux_flux(int nx, int ny, int ngrid, double dx, const double* q1, const double* q2, double* out) {
+    const long n = SN(nx, ny) * ngrid;
+    double* u = (double*)calloc(n, sizeof(double));
+    double* v = (double*)calloc(n, sizeof(double));
+    og_flux_to_xvel(nx, ny, ngrid, dx, q1, u);
+    og_flux_to_yvel(nx, ny, ngrid, dx, q2, v);
+    for (long k = 0; k < n; ++k) out[k] = u[k] * v[k];
+    og_flux_to_xvel(nx, ny, ngrid, dx, q2, u);
+    og_flux_to_yvel(nx, ny, ngrid, dx, q1, v);
+    for (long k = 0; k < n; ++k) out[k] -= u[k] * v[k];
+    og_coarsify(nx, ny, ngrid, out);
+    free(u); free(v);
+}
+
 /* NonlinearIBSolver::N, IBSolver.cc:205-209 */
 void og_nonlinear(int nx, int ny, int ngrid, double dx, const double* q, const double* w, double* n_out) {
     double* cross = (double*)calloc(NF(nx, ny) * ngrid, sizeof(double));

@@ -54,6 +54,8 @@ void og_xvel_to_flux(int nx, int ny, int ngrid, double dx, const double* u, doub
 void og_yvel_to_flux(int nx, int ny, int ngrid, double dx, const double* v, double* q); /* :793-915 */
 void og_cross_product(int nx, int ny, int ngrid, double dx, const double* q, const double* w,
                       double* out);                                                    /* :438-457 */
+void og_cross_flux_flux(int nx, int ny, int ngrid, double dx, const double* q1, const double* q2,
+                        double* out);   /* VectorOperations.cc:465-486 */
 void og_nonlinear(int nx, int ny, int ngrid, double dx, const double* q, const double* w,
                   double* n_out);                                  /* IBSolver.cc:205-209 */
 

@@ -185,6 +185,14 @@ def cross_product(nx, ny, ngrid, dx, q, w):
     return out
 
 
+def cross_flux_flux(nx, ny, ngrid, dx, q1, q2):
+    q1 = _c(q1)
+    q2 = _c(q2)
+    out = np.zeros(scalar_size(nx, ny, ngrid))
+    lib().og_cross_flux_flux(nx, ny, ngrid, ctypes.c_double(dx), _p(q1), _p(q2), _p(out))
+    return out
+
+
 def reg_build(nx, ny, dx, xoff, yoff, px, py):
     px = _c(px)
     py = _c(py)

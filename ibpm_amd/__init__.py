@@ -60,6 +60,8 @@ def load():
                                   ctypes.c_double, vp, vp, vp],
         "ibpm_elliptic_solve": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                 ctypes.c_double, ctypes.c_double, vp, vp],
+        "ibpm_elliptic_solve_noalloc": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_double, ctypes.c_double, vp, vp],
         "ibpm_curl_flux": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, vp, vp],
         "ibpm_curl_scalar": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                              ctypes.c_double, ctypes.c_double, vp, vp],
@@ -184,6 +186,14 @@ class Context:
         d_f = DeviceArray.of(self, f)
         d_u = DeviceArray(self, ngrid * (nx - 1) * (ny - 1))
         _check(load().ibpm_elliptic_solve(self.h, kind, nx, ny, ngrid, dx, alpha, d_f.ptr, d_u.ptr))
+        return d_u.get()
+
+    def elliptic_solve_noalloc(self, kind, nx, ny, ngrid, dx, alpha, f):
+        """ibpm_elliptic_solve through the allocation-free entry point (the
+        library's own buffers come from its fixed device scratch array)."""
+        d_f = DeviceArray.of(self, f)
+        d_u = DeviceArray(self, ngrid * (nx - 1) * (ny - 1))
+        _check(load().ibpm_elliptic_solve_noalloc(self.h, kind, nx, ny, ngrid, dx, alpha, d_f.ptr, d_u.ptr))
         return d_u.get()
 
     # -- learned warm start: loss and gradient by the training kernels -----

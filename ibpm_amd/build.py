@@ -12,7 +12,7 @@ OBJ = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("IBPM_OFFLOAD_ARCH", "gfx950")
 
-DEVICE_SOURCES = ["elliptic.hip", "ops.hip", "learn.hip"]
+DEVICE_SOURCES = ["elliptic.hip", "ops.hip", "learn.hip", "scratch.hip"]
 HOST_SOURCES = ["engine.cpp", "geometry.cpp", "restart.cpp"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function"]
 # Packed FP32 instructions are kept out of every code object.

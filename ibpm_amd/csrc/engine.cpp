@@ -18,6 +18,7 @@
 #include <functional>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -26,6 +27,7 @@
 #include "geometry.h"
 #include "learn.h"
 #include "ops.h"
+#include "scratch.h"
 
 namespace ibpmg {
 
@@ -92,37 +94,55 @@ struct Ctx {
     DevMem mem;
 };
 
+// Shape of the plan of a DST-I of length N: FFT length M = 2(N+1), its
+// Stockham radices and the row pairs per workgroup (table pointers unset).
+static int plan_shape(int N, FftPlan* P) {
+    *P = FftPlan{};
+    P->N = N;
+    P->M = 2 * (N + 1);
+    if (P->M > kFftLds) { set_error("DST length %d exceeds the LDS plan limit (%d)", N, kFftLds / 2 - 1); return -2; }
+    int m = P->M;
+    while (m % 8 == 0) { P->rad[P->nst++] = 8; m /= 8; }
+    while (m % 4 == 0) { P->rad[P->nst++] = 4; m /= 4; }
+    while (m % 2 == 0) { P->rad[P->nst++] = 2; m /= 2; }
+    while (m % 5 == 0) { P->rad[P->nst++] = 5; m /= 5; }
+    while (m % 3 == 0) { P->rad[P->nst++] = 3; m /= 3; }
+    if (m != 1) { set_error("DST length %d: 2(N+1) has a prime factor > 5", N); return -2; }
+    P->np = std::max(1, std::min(kFftNpMax, kFftLds / P->M));
+    return 0;
+}
+
+// Whether a plan of FFT length M has the second-stage table tw16.
+static bool plan_has_tw16(int M) { return M == 1024 || M == 2048; }
+
+// Host values of a plan's twiddle tables: tw (M entries) and, where
+// plan_has_tw16, t16 (256 entries).
+static void plan_tables(int M, double2* tw, double2* t16) {
+    const double twopi = 8.0 * std::atan(1.0);
+    for (int k = 0; k < M; ++k) {
+        const double a = -twopi * (double)k / (double)M;
+        tw[k] = make_double2(std::cos(a), std::sin(a));
+    }
+    if (plan_has_tw16(M)) {   // second-stage table of the register-held transforms
+        const int tstep = M / 256;
+        for (int r = 0; r < 16; ++r)
+            for (int jm = 0; jm < 16; ++jm) t16[16 * r + jm] = tw[(tstep * jm * r) % M];
+    }
+}
+
 static int get_plan(Ctx* c, int N, FftPlan* out) {
     auto it = c->plans.find(N);
     if (it != c->plans.end()) { *out = it->second; return 0; }
     FftPlan P{};
-    P.N = N;
-    P.M = 2 * (N + 1);
-    if (P.M > kFftLds) { set_error("DST length %d exceeds the LDS plan limit (%d)", N, kFftLds / 2 - 1); return -2; }
-    int m = P.M;
-    while (m % 8 == 0) { P.rad[P.nst++] = 8; m /= 8; }
-    while (m % 4 == 0) { P.rad[P.nst++] = 4; m /= 4; }
-    while (m % 2 == 0) { P.rad[P.nst++] = 2; m /= 2; }
-    while (m % 5 == 0) { P.rad[P.nst++] = 5; m /= 5; }
-    while (m % 3 == 0) { P.rad[P.nst++] = 3; m /= 3; }
-    if (m != 1) { set_error("DST length %d: 2(N+1) has a prime factor > 5", N); return -2; }
-    P.np = std::max(1, std::min(kFftNpMax, kFftLds / P.M));
-    std::vector<double2> tw(P.M);
-    const double twopi = 8.0 * std::atan(1.0);
-    for (int k = 0; k < P.M; ++k) {
-        const double a = -twopi * (double)k / (double)P.M;
-        tw[k] = make_double2(std::cos(a), std::sin(a));
-    }
+    CKR(plan_shape(N, &P));
+    std::vector<double2> tw(P.M), t16(256);
+    plan_tables(P.M, tw.data(), t16.data());
     double2* d = nullptr;
     CKR(c->mem.alloc(&d, P.M));
     CK(hipMemcpy(d, tw.data(), sizeof(double2) * P.M, hipMemcpyHostToDevice));
     P.tw = d;
     P.tw16 = nullptr;
-    if (P.M == 1024 || P.M == 2048) {   // second-stage table of the register-held transforms
-        const int tstep = P.M / 256;
-        std::vector<double2> t16(256);
-        for (int r = 0; r < 16; ++r)
-            for (int jm = 0; jm < 16; ++jm) t16[16 * r + jm] = tw[(tstep * jm * r) % P.M];
+    if (plan_has_tw16(P.M)) {
         double2* d16 = nullptr;
         CKR(c->mem.alloc(&d16, 256));
         CK(hipMemcpy(d16, t16.data(), sizeof(double2) * 256, hipMemcpyHostToDevice));
@@ -204,6 +224,45 @@ static void whole_level(EllJob& J) {
     J.t2_lo = 0; J.t2_hi = INT_MAX;
 }
 
+// The job of level lev of one solve (every field but the two FFT plans).
+static void fill_ell_job(const SolveSpec& s, int lev, EllJob& J) {
+    const long ns = (long)(s.nx - 1) * (s.ny - 1);
+    const double dxl = s.dx * (1 << lev);
+    whole_level(J);
+    J.nx = s.nx; J.ny = s.ny; J.lev = lev; J.ngrid = s.L;
+    J.kind = s.kind; J.epilogue = s.epilogue; J.in_scale = s.in_scale;
+    J.alpha = s.alpha;
+    J.bydx2 = 1. / (dxl * dxl);                                  // EllipticSolver2d.cc:44
+    J.bc_coef = (s.kind == 0) ? 1 / (dxl * dxl) : s.alpha / (dxl * dxl);   // :137, :178
+    J.norm = 1. / (2 * s.nx * 2 * s.ny);                         // :80
+    J.f = s.f + lev * ns;
+    J.R = s.R ? s.R + lev * ns : nullptr;
+    // getBC source: the coarser level of the solution; for u = W - v
+    // that is v, kept on the BC ring by the coarser level's pass 3
+    const double* bcsrc = s.vside ? s.vside : s.u;
+    J.uc = (lev < s.L - 1) ? bcsrc + (lev + 1) * ns : nullptr;
+    J.vside = (s.vside && lev >= 1) ? s.vside + lev * ns : nullptr;
+    J.bcring = (s.L == 1) ? s.bcring : nullptr;
+    J.T1 = s.T1; J.T2 = s.T2;
+    J.ld1 = t1_ld(s.nx - 1); J.ld2 = t1_ld(s.ny - 1);   // the scratch is sized for the padding
+    J.u = s.u + lev * ns;
+    J.u2 = s.u2 ? s.u2 + lev * ns : nullptr;
+    J.W = s.W ? s.W + lev * ns : nullptr;
+    J.cosx = s.cosx; J.cosy = s.cosy;
+    J.skip = s.skip;
+}
+
+// Launch shape of the three passes of a single-job group: registers or LDS,
+// workgroups and dynamic LDS bytes per pass (as build_solve_plan sizes them).
+static void pass_shape(const EllJob& J, bool regs[3], int maxb[3], int lds[3]) {
+    for (int k = 0; k < 3; ++k) {
+        const long pairs = (k == 1 ? J.ny : J.nx) / 2;
+        regs[k] = pass_in_registers(k == 1 ? J.pi.M : J.pj.M, pairs);
+        maxb[k] = blocks_for_pass(J, k + 1, regs[k]);
+        lds[k] = lds_for_pass(J, k + 1, regs[k]);
+    }
+}
+
 struct PassGroup {
     EllJob* d = nullptr;
     int n = 0;
@@ -266,30 +325,8 @@ static int build_solve_plan(Ctx* c, DevMem& mem, const std::vector<SolveSpec>& s
             if (st >= s.L) continue;
             const int lev = s.L - 1 - st;
             const long ns = (long)(s.nx - 1) * (s.ny - 1);
-            const double dxl = s.dx * (1 << lev);
             EllJob J{};
-            whole_level(J);
-            J.nx = s.nx; J.ny = s.ny; J.lev = lev; J.ngrid = s.L;
-            J.kind = s.kind; J.epilogue = s.epilogue; J.in_scale = s.in_scale;
-            J.alpha = s.alpha;
-            J.bydx2 = 1. / (dxl * dxl);                                  // EllipticSolver2d.cc:44
-            J.bc_coef = (s.kind == 0) ? 1 / (dxl * dxl) : s.alpha / (dxl * dxl);   // :137, :178
-            J.norm = 1. / (2 * s.nx * 2 * s.ny);                         // :80
-            J.f = s.f + lev * ns;
-            J.R = s.R ? s.R + lev * ns : nullptr;
-            // getBC source: the coarser level of the solution; for u = W - v
-            // that is v, kept on the BC ring by the coarser level's pass 3
-            const double* bcsrc = s.vside ? s.vside : s.u;
-            J.uc = (lev < s.L - 1) ? bcsrc + (lev + 1) * ns : nullptr;
-            J.vside = (s.vside && lev >= 1) ? s.vside + lev * ns : nullptr;
-            J.bcring = (s.L == 1) ? s.bcring : nullptr;
-            J.T1 = s.T1; J.T2 = s.T2;
-            J.ld1 = t1_ld(s.nx - 1); J.ld2 = t1_ld(s.ny - 1);   // the scratch is sized for the padding
-            J.u = s.u + lev * ns;
-            J.u2 = s.u2 ? s.u2 + lev * ns : nullptr;
-            J.W = s.W ? s.W + lev * ns : nullptr;
-            J.cosx = s.cosx; J.cosy = s.cosy;
-            J.skip = s.skip;
+            fill_ell_job(s, lev, J);
             CKR(get_plan(c, s.ny - 1, &J.pj));
             CKR(get_plan(c, s.nx - 1, &J.pi));
             size_t g = 0;
@@ -368,10 +405,15 @@ static void run_solve(const SolvePlan& P, hipStream_t s, Probe* pr = nullptr,
         }
 }
 
+// cos(pi i / n), i = 1..n-1 (EllipticSolver2d.cc:47-48) into t[0..n-2].
+static void cos_fill(int n, double* t) {
+    const double pi = 4. * std::atan(1.0);
+    for (int i = 1; i < n; ++i) t[i - 1] = std::cos((pi * i) / n);
+}
+
 static int cos_tables(DevMem& mem, int n, double** out) {
     std::vector<double> t(n - 1);
-    const double pi = 4. * std::atan(1.0);
-    for (int i = 1; i < n; ++i) t[i - 1] = std::cos((pi * i) / n);   // EllipticSolver2d.cc:47-48
+    cos_fill(n, t.data());
     CKR(mem.alloc(out, n - 1));
     CK(hipMemcpy(*out, t.data(), sizeof(double) * (n - 1), hipMemcpyHostToDevice));
     return 0;
@@ -1916,6 +1958,11 @@ int ibpm_ctx_create(int device, ibpm_ctx** out) {
         delete c;
         return -1;
     }
+    if (scratch_attach(device) != 0) {   // the allocation-free entry point's array on this device
+        (void)hipStreamDestroy(c->c.stream);
+        delete c;
+        return -1;
+    }
     *out = c;
     return 0;
 }
@@ -2031,6 +2078,103 @@ int ibpm_elliptic2d_solve(ibpm_ctx* c, int kind, int nx, int ny, double dx, doub
 int ibpm_elliptic_solve(ibpm_ctx* c, int kind, int nx, int ny, int ngrid, double dx, double alpha,
                         const double* d_f, double* d_u) {
     return solve_common(c, kind, nx, ny, ngrid, dx, alpha, d_f, nullptr, d_u);
+}
+
+// ibpm_elliptic_solve_noalloc: the multi-domain solve of ibpm_elliptic_solve
+// with every buffer of its own (right-hand-side copy, coarsify buffer,
+// transpose buffers, cosine and twiddle tables, job descriptors) carved from
+// the device's fixed scratch array instead of allocated.  The host values of
+// the tables and descriptors are staged in static memory under a lock, which
+// is held until the stream has drained.
+static constexpr int kNoAllocMaxLevels = 16;
+struct NoAllocStage {
+    double cosx[kFftLds / 2], cosy[kFftLds / 2];
+    double2 twj[kFftLds], t16j[256], twi[kFftLds], t16i[256];
+    EllJob ej[kNoAllocMaxLevels];
+    CoarsifyJob cj[kNoAllocMaxLevels];
+};
+static std::mutex g_noalloc_mu;
+static NoAllocStage g_noalloc_stage;
+
+int ibpm_elliptic_solve_noalloc(ibpm_ctx* c, int kind, int nx, int ny, int ngrid, double dx, double alpha,
+                                const double* d_f, double* d_u) {
+    if (!c || !d_f || !d_u) { set_error("ibpm_elliptic_solve_noalloc: null context or array"); return -1; }
+    if (kind != 0 && kind != 1) { set_error("ibpm_elliptic_solve_noalloc: kind %d is not 0 or 1", kind); return -1; }
+    CKR(check_grid(nx, ny, ngrid));
+    if (ngrid > kNoAllocMaxLevels) {
+        set_error("ibpm_elliptic_solve_noalloc: %d levels exceed the limit of %d", ngrid, kNoAllocMaxLevels);
+        return -2;
+    }
+    FftPlan pj{}, pi{};
+    CKR(plan_shape(ny - 1, &pj));
+    CKR(plan_shape(nx - 1, &pi));
+    const int L = ngrid;
+    const long ns = (long)(nx - 1) * (ny - 1);
+    size_t off = 0;
+    auto take = [&](size_t bytes) { const size_t o = off; off += (bytes + 255) & ~size_t(255); return o; };
+    const size_t o_fin = take(8 * ns * L), o_R = take(8 * ns * L);
+    const size_t o_T1 = take(8 * (size_t)(ny - 1) * t1_ld(nx - 1)), o_T2 = take(8 * (size_t)(nx - 1) * t1_ld(ny - 1));
+    const size_t zero_lo = o_R, zero_hi = off;   // R, T1 and T2 start at zero, as freshly allocated ones
+    const size_t o_cx = take(8 * (size_t)(nx - 1)), o_cy = take(8 * (size_t)(ny - 1));
+    const size_t o_twj = take(16 * (size_t)pj.M), o_t16j = take(16 * 256);
+    const size_t o_twi = take(16 * (size_t)pi.M), o_t16i = take(16 * 256);
+    const size_t o_ej = take(sizeof(EllJob) * L), o_cj = take(sizeof(CoarsifyJob) * L);
+    if (off > kScratchBytes) {
+        set_error("ibpm_elliptic_solve_noalloc: a %dx%d grid of %d levels needs %zu bytes of scratch; "
+                  "the fixed scratch holds %zu", nx, ny, ngrid, off, kScratchBytes);
+        return -2;
+    }
+    void* base = nullptr;
+    CKR(scratch_device_ptr(c->c.device, &base));
+    char* b = (char*)base;
+    double* fin = (double*)(b + o_fin);
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    NoAllocStage& H = g_noalloc_stage;
+    cos_fill(nx, H.cosx);
+    cos_fill(ny, H.cosy);
+    plan_tables(pj.M, H.twj, H.t16j);
+    plan_tables(pi.M, H.twi, H.t16i);
+    pj.tw = (const double2*)(b + o_twj);
+    pj.tw16 = plan_has_tw16(pj.M) ? (const double2*)(b + o_t16j) : nullptr;
+    pi.tw = (const double2*)(b + o_twi);
+    pi.tw16 = plan_has_tw16(pi.M) ? (const double2*)(b + o_t16i) : nullptr;
+    SolveSpec sp{};
+    sp.nx = nx; sp.ny = ny; sp.L = L; sp.dx = dx; sp.kind = kind; sp.alpha = alpha;
+    sp.f = fin; sp.u = d_u; sp.R = (double*)(b + o_R); sp.T1 = (double*)(b + o_T1); sp.T2 = (double*)(b + o_T2);
+    sp.in_scale = 1.0; sp.epilogue = 0;
+    sp.cosx = (const double*)(b + o_cx); sp.cosy = (const double*)(b + o_cy);
+    for (int st = 0; st < L; ++st) {   // coarsest level first
+        H.ej[st] = EllJob{};
+        fill_ell_job(sp, L - 1 - st, H.ej[st]);
+        H.ej[st].pj = pj;
+        H.ej[st].pi = pi;
+    }
+    for (int lev = 1; lev < L; ++lev) H.cj[lev - 1] = CoarsifyJob{nx, ny, lev, sp.f, sp.R, nullptr};
+    hipStream_t s = c->c.stream;
+    EllJob* d_ej = (EllJob*)(b + o_ej);
+    CoarsifyJob* d_cj = (CoarsifyJob*)(b + o_cj);
+    CK(hipMemsetAsync(b + zero_lo, 0, zero_hi - zero_lo, s));
+    CK(hipMemcpyAsync(fin, d_f, 8 * ns * L, hipMemcpyDeviceToDevice, s));
+    CK(hipMemcpyAsync(b + o_cx, H.cosx, 8 * (size_t)(nx - 1), hipMemcpyHostToDevice, s));
+    CK(hipMemcpyAsync(b + o_cy, H.cosy, 8 * (size_t)(ny - 1), hipMemcpyHostToDevice, s));
+    CK(hipMemcpyAsync(b + o_twj, H.twj, 16 * (size_t)pj.M, hipMemcpyHostToDevice, s));
+    CK(hipMemcpyAsync(b + o_t16j, H.t16j, 16 * 256, hipMemcpyHostToDevice, s));
+    CK(hipMemcpyAsync(b + o_twi, H.twi, 16 * (size_t)pi.M, hipMemcpyHostToDevice, s));
+    CK(hipMemcpyAsync(b + o_t16i, H.t16i, 16 * 256, hipMemcpyHostToDevice, s));
+    CK(hipMemcpyAsync(d_ej, H.ej, sizeof(EllJob) * L, hipMemcpyHostToDevice, s));
+    if (L > 1) CK(hipMemcpyAsync(d_cj, H.cj, sizeof(CoarsifyJob) * (L - 1), hipMemcpyHostToDevice, s));
+    for (int lev = 1; lev < L; ++lev) launch_coarsify(d_cj + (lev - 1), 1, nx / 2 - 1, s, false);
+    for (int st = 0; st < L; ++st) {
+        bool regs[3];
+        int maxb[3], lds[3];
+        pass_shape(H.ej[st], regs, maxb, lds);
+        for (int pass = 1; pass <= 3; ++pass)
+            launch_ell_pass(pass, d_ej + st, 1, maxb[pass - 1], lds[pass - 1], s, false,
+                            pass == 2 ? pi.M : pj.M, regs[pass - 1]);
+    }
+    CK(hipGetLastError());
+    CK(hipStreamSynchronize(s));
+    return 0;
 }
 
 // Stand-alone stencil entry points run the batch kernels on a one-flow

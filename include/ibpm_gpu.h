@@ -53,6 +53,21 @@ int ibpm_elliptic2d_solve(ibpm_ctx* ctx, int kind, int nx, int ny, double dx, do
 int ibpm_elliptic_solve(ibpm_ctx* ctx, int kind, int nx, int ny, int ngrid, double dx,
                         double alpha, const double* d_f, double* d_u);
 
+/* The multi-domain solve of ibpm_elliptic_solve without any allocation.  No
+   counterpart in the reference: it is for callers that cannot allocate device
+   memory (code under HIP graph capture, tight serving loops, harnesses that
+   forbid hipMalloc).  Its own buffers (right-hand-side copy, coarsify and
+   transpose buffers, tables, job descriptors) live in a fixed 32 MiB device
+   array that every GPU the library runs on holds until the process ends; a
+   grid needs 16 (nx-1)(ny-1) ngrid bytes plus about 16 (nx-1)(ny-1) for the
+   transposes, so e.g. 512 x 512 with 5 levels fits and 1024 x 1024 with 6 does
+   not: sizes beyond the array, or more than 16 levels, return -2 with a
+   message.  The caller passes d_f and d_u (which may alias).  Calls on one GPU
+   share its array, so calls that overlap on one GPU (other threads, other
+   contexts of the same device) race; calls are synchronous. */
+int ibpm_elliptic_solve_noalloc(ibpm_ctx* ctx, int kind, int nx, int ny, int ngrid, double dx,
+                                double alpha, const double* d_f, double* d_u);
+
 /* ---- grid operators (VectorOperations) ----------------------------------- */
 /* Curl(Flux -> Scalar), src/VectorOperations.cc:33-54 */
 int ibpm_curl_flux(ibpm_ctx* ctx, int nx, int ny, int ngrid, double dx, const double* d_q, double* d_w);

@@ -8,11 +8,18 @@ A step is one full RK3 timestep (3 substeps) of every flow in the workload;
 grid-pt counts the stored interior nodes of all levels.  Inputs are resident
 in HBM when the timed region starts; the timed region is K hipGraph replays
 (stationary bodies) or K eager steps with the CG force solve (moving bodies).
-For moving bodies the batched, preconditioned form is timed, then the plain
-GPU form (one flow, plain CG as the reference) on the first flow for the same
-steps, with the speed-up and the Cd/Cl difference between the two.  The
-default workload is the batch of 16 tandem pitching/plunging plate cases
-(1024x1024, 6 levels).
+
+Moving bodies (the default workload, 16 tandem pitching/plunging plate cases
+at 1024x1024 with 6 levels): the warm-start network is first trained on a
+separate run of the workload's first steps; the timed form is then the
+batched, preconditioned solver with the learned warm start, from a zero
+initial vorticity (W warm-up steps, K timed, then --cycle-steps more that
+count preconditioner rebuilds, whose cost is reported amortised per step).
+Beside it the same steps are timed for the batched solver with the
+reference's warm start, for batched plain CG, and for the plain GPU form
+(one flow, plain CG, every A^-1 by the DST solve: the reference's solver).
+Flow 0's Cd/Cl at every step is compared with the FP64 oracle's fixture
+(tests/golden), and the run exits with status 3 if the error exceeds 1e-6.
 """
 import argparse
 import json
@@ -102,6 +109,11 @@ def traffic_per_launch(workload, kernel, alg_bytes):
     return t["traffic_over_algorithmic"] * alg_bytes
 
 
+def log(msg):
+    """Progress line on stderr (the JSON result line goes to stdout)."""
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
 def timed_steps(b, nflows, steps, moving):
     """Advances `steps` steps; returns (seconds, CG iterations per step per flow)."""
     import ibpm_amd
@@ -119,27 +131,134 @@ def timed_steps(b, nflows, steps, moving):
     return time.perf_counter() - t0, (sum(iters) / len(iters) if iters else None)
 
 
-def plain_form(ctx, flow, total_steps, steps):
-    """The plain GPU form on one flow: plain CG from the previous force with
-    every A^-1 of M by the DST solve (the reference's solver), no batching;
-    advances total_steps and times the last `steps`.  Returns throughput,
-    iterations per step and final Cd, Cl."""
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+# Oracle fixtures (FP64 CPU restatement, per-step Cd/Cl) of flows of the workloads:
+# workload -> {flow index: fixture file}
+FIXTURES = {
+    "cylinder_200": {0: "cylinder_200.npz"},
+    "cylinder_400x4": {0: "cylinder_400x4.npz"},
+    "plate_512x5": {0: "plate_512x5_short.npz"},
+    "tandem_16x1024x6": {0: "tandem_16x1024x6_c0.npz"},
+    "mixed_64x3": {k: "mixed_64x3_f%d.npz" % k for k in (2, 5, 4, 20, 3, 11, 0, 1, 51)},
+}
+GATE = 1e-6
+
+
+def accuracy_vs_oracle(b, workload):
+    """Largest relative Cd and Cl error, over the steps both cover, of the
+    flows of `workload` that have an oracle fixture (Cl relative to max|Cd|
+    for the mirror-symmetric cylinders, where Cl is roundoff-seeded, and to
+    max|Cl| otherwise)."""
+    import numpy as np
+    out = {"steps_compared": None, "Cd_rel_err": 0.0, "Cl_rel_err": 0.0, "flows": []}
+    for k, name in sorted(FIXTURES.get(workload, {}).items()):
+        path = os.path.join(GOLDEN, name)
+        if not os.path.exists(path) or k >= len(b.flows):
+            continue
+        ref = np.load(path)["forces"]
+        got = b.force_history(k)
+        n = min(len(ref), len(got))
+        if n == 0:
+            continue
+        r, g = ref[:n], got[:n]
+        cd = float(np.max(np.abs(g[:, 0] - r[:, 0]) / np.abs(r[:, 0])))
+        sym = "circle" in b.flows[k].geom   # mirror-symmetric cylinder: Cl is roundoff-seeded
+        cl_scale = np.max(np.abs(r[:, 0])) if sym else np.max(np.abs(r[:, 1]))
+        cl = float(np.max(np.abs(g[:, 1] - r[:, 1]) / cl_scale))
+        out["flows"].append(k)
+        out["steps_compared"] = n if out["steps_compared"] is None else min(n, out["steps_compared"])
+        out["Cd_rel_err"] = max(out["Cd_rel_err"], cd)
+        out["Cl_rel_err"] = max(out["Cl_rel_err"], cl)
+    if not out["flows"]:
+        return None
+    out["gate"] = GATE
+    out["ok"] = out["Cd_rel_err"] <= GATE and out["Cl_rel_err"] <= GATE
+    return out
+
+
+def moving_form(ctx, flows, warmup, steps, precond="lowrank", pairs=8, local=True, warm="previous",
+                nn=None, cycle_steps=0):
+    """One form of the moving-body solver on `flows` from a zero initial
+    vorticity: warmup steps (the first builds the preconditioner), then
+    `steps` timed steps, then (cycle_steps > 0) more steps that count
+    preconditioner rebuilds.  Returns the form's figures and the batch."""
     import ibpm_amd
     from ibpm_amd.workloads import grid_points
-    b = ibpm_amd.Batch(ctx, [flow]).init().set_local_helmholtz(False)
-    b.advance(total_steps - steps)
-    sec, its = timed_steps(b, 1, steps, True)
-    cd, cl = b.forces(0)
+    b = ibpm_amd.Batch(ctx, flows).init()
+    b.set_precond(precond, pairs).set_local_helmholtz(local)
+    if nn is not None:
+        b.nn_params(nn)
+    if warm != "previous":
+        b.set_warm_start(warm)
+    sync = ibpm_amd.load().ibpm_ctx_synchronize
+    sync(ctx.h)
+    t0 = time.perf_counter()
+    b.advance(1)
+    sync(ctx.h)
+    t_first = time.perf_counter() - t0
+    b.advance(max(0, warmup - 1))
+    st0 = b.solver_stats()
+    sec, its = timed_steps(b, len(flows), steps, True)
+    st1 = b.solver_stats()
+    gp = sum(grid_points(f) for f in flows)
+    d = {k: st1[k] - st0[k] for k in st1}
+    ms = sec / steps * 1e3
+    out = {"flows": len(flows), "precond": precond, "pairs": pairs if precond == "lowrank" else 0,
+           "local_helmholtz": local, "warm_start": warm,
+           "value": gp * steps / sec, "ms_per_step": ms, "cg_iters_per_step": its,
+           "batched_m_applications_per_step": (d["m_applications"] + d["build_m_applications"]) / steps,
+           "builds_in_timed_steps": d["builds"], "fallbacks": d["fallbacks"],
+           "host_ms_per_step_kinematics": round(d["host_us_kinematics"] / steps / 1e3, 3),
+           "host_ms_per_step_polls": round(d["host_us_polls"] / steps / 1e3, 3)}
+    if precond != "none":
+        # the first step holds the first preconditioner build
+        t_build = max(0.0, t_first - ms / 1e3)
+        if cycle_steps:
+            b.advance(cycle_steps)
+        st2 = b.solver_stats()
+        done = warmup + steps + cycle_steps
+        per_step = st2["builds"] / done
+        out["preconditioner"] = {
+            "build_s": round(t_build, 3), "builds_since_init": st2["builds"], "steps_since_init": done,
+            "builds_per_step": per_step,
+            "value_with_builds_amortised": gp / (ms / 1e3 + per_step * t_build)}
+    log("form %s/%s/%s warm start, %d flows: %.1f ms/step, %.1f CG iterations/step"
+        % (precond, "local" if local else "dst", warm, len(flows), ms, its or 0.0))
+    return out, b
+
+
+def plain_form(ctx, flow, warmup, steps):
+    """The plain GPU form on one flow: plain CG from the previous force with
+    every A^-1 of M by the DST solve (the reference's solver), no batching."""
+    out, b = moving_form(ctx, [flow], warmup, steps, precond="none", local=False)
     b.close()
-    return {"value": grid_points(flow) * steps / sec, "ms_per_step": sec / steps * 1e3,
-            "cg_iters_per_step": its, "Cd": cd, "Cl": cl}
+    return out
+
+
+def train_network(ctx, flows, train_steps):
+    """Trains the warm-start network on (kinematics, right-hand side, force)
+    samples of a separate run of the workload's flows (its first train_steps
+    steps), by the library's own training kernels; returns its parameters."""
+    import ibpm_amd
+    b = ibpm_amd.Batch(ctx, flows).init().set_precond("lowrank", 8)
+    b.collect(True, 4 << 20)
+    b.advance(train_steps)
+    t0 = time.perf_counter()
+    loss0 = b.train(iters=1, lr=0.0)
+    loss = b.train(iters=800, lr=3e-3)
+    info = {"train_steps": train_steps, "samples": b.num_samples(), "loss_before": loss0,
+            "loss_after": loss, "train_s": round(time.perf_counter() - t0, 3)}
+    params = b.nn_params()
+    b.close()
+    log("network trained on %d samples in %.1f s" % (info["samples"], info["train_s"]))
+    return params, info
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="tandem_16x1024x6", choices=sorted(CONFIG_TEXT),
                     help="tandem_16x1024x6 (default): the moving-plate batch the metric is quoted on")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -149,12 +268,15 @@ def main():
                     help="force solve of moving bodies: plain CG (the reference's), CG preconditioned "
                          "by a stale FP64 inverse of M, or that plus the low-rank update")
     ap.add_argument("--pairs", type=int, default=8)
-    ap.add_argument("--warm", default="previous", choices=["previous", "zero", "learned"],
-                    help="CG start of moving-body workloads (previous = the reference's)")
+    ap.add_argument("--warm", default="learned", choices=["previous", "zero", "learned"],
+                    help="CG start of the timed moving-body form (previous = the reference's)")
     ap.add_argument("--train-steps", type=int, default=30,
-                    help="steps of sample collection before training the learned warm start")
-    ap.add_argument("--no-plain", action="store_true",
-                    help="skip the plain GPU form (moving bodies) run beside the batched one")
+                    help="steps of the training run of the learned warm start")
+    ap.add_argument("--cycle-steps", type=int, default=150,
+                    help="steps after the timed ones that count preconditioner rebuilds")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="moving bodies: time only the headline form (no plain, batched-plain or "
+                         "previous-start forms beside it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
@@ -164,71 +286,60 @@ def main():
     steps = args.steps or DEFAULT_STEPS[args.workload]
     flows = WORKLOADS[args.workload]()
     ctx = ibpm_amd.Context(0)
-    b = ibpm_amd.Batch(ctx, flows)
-    t_init0 = time.perf_counter()
-    b.init()
-    t_init = time.perf_counter() - t_init0
     moving = any("pitchplunge" in f.geom for f in flows)
-    if not args.no_graph and not moving:
-        b.use_graph(True)
-    learn = None
-    done_steps = 0
-    if moving:
-        b.set_precond(args.precond, args.pairs)
-    if moving and args.warm == "learned":
-        # sample collection, then training on the GPU
-        b.collect(True, 4 << 20)
-        its = []
-        for _ in range(args.train_steps):
-            b.advance(1)
-            its.append(sum(int(b.cg_iters(k).sum()) for k in range(len(flows))) / len(flows))
-        done_steps += args.train_steps
-        t_tr = time.perf_counter()
-        loss0 = b.train(iters=1, lr=0.0)
-        loss = b.train(iters=800, lr=3e-3)
-        learn = {"train_steps": args.train_steps, "samples": b.num_samples(), "loss_before": loss0,
-                 "loss_after": loss, "train_s": round(time.perf_counter() - t_tr, 3),
-                 "cg_iters_per_step_previous_warm_start": sum(its[-10:]) / len(its[-10:])}
-        b.collect(False).set_warm_start("learned")
-    elif moving and args.warm == "zero":
-        b.set_warm_start("zero")
-    t_w0 = time.perf_counter()
-    b.advance(args.warmup)
-    t_warm = time.perf_counter() - t_w0
-    done_steps += args.warmup
-    st0 = b.solver_stats() if moving else None
-    sec, cg = timed_steps(b, len(flows), steps, moving)
-    done_steps += steps
     gp = sum(grid_points(f) for f in flows)
-    value = gp * steps / sec
-    cd, cl = b.forces(0)
-    solver = None
+    forms = None
+    learn = None
     if moving:
-        st1 = b.solver_stats()
-        d = {k: st1[k] - st0[k] for k in st1}
-        solver = {"precond": args.precond, "pairs": args.pairs if args.precond == "lowrank" else 0,
-                  "warm_start": args.warm, "cg_iters_per_step": cg,
-                  "batched_m_applications_per_step": (d["m_applications"] + d["build_m_applications"]) / steps,
-                  "builds_in_timed_steps": d["builds"], "fallbacks": d["fallbacks"],
-                  "host_ms_per_step_kinematics": round(d["host_us_kinematics"] / steps / 1e3, 3),
-                  "host_ms_per_step_polls": round(d["host_us_polls"] / steps / 1e3, 3),
-                  "m_applications_with_local_helmholtz": d["local_helmholtz_m_applications"] / max(
-                      1, d["m_applications"] + d["build_m_applications"]),
-                  "warmup_s_including_first_build": round(t_warm, 3)}
-    # live per-kernel timing (HIP events on the library stream, eager steps)
-    stats = b.kernel_stats(2 if moving else 3)
-    b.close()
+        nn = None
+        if args.warm == "learned":
+            nn, learn = train_network(ctx, flows, args.train_steps)
+        t_i0 = time.perf_counter()
+        head, b = moving_form(ctx, flows, args.warmup, steps, args.precond, args.pairs, True, args.warm,
+                              nn, args.cycle_steps)
+        t_init = time.perf_counter() - t_i0
+        value, sec, cg = head["value"], head["ms_per_step"] * steps / 1e3, head["cg_iters_per_step"]
+        accuracy = accuracy_vs_oracle(b, args.workload)
+        cd, cl = b.forces(0)
+        stats = b.kernel_stats(2)
+        b.close()
+        if not args.no_forms:
+            forms = {}
+            if args.warm != "previous":
+                prev, bp = moving_form(ctx, flows, args.warmup, steps, args.precond, args.pairs, True)
+                bp.close()
+                forms["batched_preconditioned_previous_start"] = prev
+                if learn is not None:
+                    learn["cg_iters_per_step_previous_start"] = prev["cg_iters_per_step"]
+                    learn["cg_iters_per_step_learned_start"] = cg
+            bpl, bb = moving_form(ctx, flows, 1, 3, precond="none", local=False)
+            bb.close()
+            forms["batched_plain_cg"] = bpl
+            plain = plain_form(ctx, flows[0], args.warmup, steps)
+            forms["plain_gpu_form"] = plain
+            forms["speedup_vs_plain_gpu_form"] = value / plain["value"]
+        solver = head
+    else:
+        b = ibpm_amd.Batch(ctx, flows)
+        t_i0 = time.perf_counter()
+        b.init()
+        t_init = time.perf_counter() - t_i0
+        if not args.no_graph:
+            b.use_graph(True)
+        b.advance(args.warmup)
+        sec, cg = timed_steps(b, len(flows), steps, False)
+        value = gp * steps / sec
+        log("%d steps timed: %.3f ms/step" % (steps, sec / steps * 1e3))
+        accuracy = accuracy_vs_oracle(b, args.workload)
+        cd, cl = b.forces(0)
+        solver = None
+        stats = b.kernel_stats(3)
+        b.close()
     # dominant kernel: most time among the classes with algorithmic bytes
     dom = max(stats.items(), key=lambda kv: kv[1][1] if kv[1][2] else 0.0)
     name, (cnt, ms, by) = dom
     avg_s = ms / cnt / 1e3
     achieved = by / cnt / avg_s / 1e9
-    plain = None
-    if moving and not args.no_plain:
-        plain = plain_form(ctx, flows[0], done_steps, steps)
-        plain["speedup_of_this_form"] = (value / plain["value"])
-        plain["Cd_rel_err_vs_this_form_flow0"] = abs(cd - plain["Cd"]) / abs(plain["Cd"])
-        plain["Cl_rel_err_vs_this_form_flow0"] = abs(cl - plain["Cl"]) / max(abs(plain["Cl"]), 1e-300)
     line = {
         "metric": METRIC if moving else "grid-pt\u00b7steps/s (stationary bodies, Cholesky force solve)",
         "value": value,
@@ -244,7 +355,9 @@ def main():
         "data": "synthetic (configured geometry and parameters, zero initial vorticity)",
         "config": {"workload": args.workload, "description": CONFIG_TEXT[args.workload],
                    "flows": len(flows), "grid_points": gp, "parallelism": "single GPU",
-                   "solver": solver, "learned": learn, "plain_gpu_form": plain,
+                   "form": ("batched, preconditioned, %s warm start" % args.warm) if moving else "graph replay",
+                   "solver": solver, "learned": learn, "forms": forms,
+                   "accuracy_vs_oracle_fixture": accuracy,
                    "init_s": round(t_init, 3), "Cd_flow0": cd, "Cl_flow0": cl},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
@@ -254,9 +367,13 @@ def main():
                     for k, (c, m, bb) in sorted(stats.items(), key=lambda kv: -kv[1][1])},
     }
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(flows, args.cpu_seconds,
-                                            plain["cg_iters_per_step"] if plain else None)
+        log("CPU baseline")
+        plain_its = forms["plain_gpu_form"]["cg_iters_per_step"] if forms else None
+        line["cpu_baseline"] = cpu_baseline(flows, args.cpu_seconds, plain_its)
     print(json.dumps(line))
+    if accuracy is not None and not accuracy["ok"]:
+        print("bench.py: flow Cd/Cl error against the oracle fixture exceeds %g" % GATE, file=sys.stderr)
+        sys.exit(3)
 
 
 if __name__ == "__main__":

@@ -62,6 +62,9 @@ def load():
                                 ctypes.c_double, ctypes.c_double, vp, vp],
         "ibpm_elliptic_solve_noalloc": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_double, ctypes.c_double, vp, vp],
+        "ibpm_elliptic_solve_precision": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_double, ctypes.c_double, vp, vp, ctypes.c_int],
+        "ibpm_batch_set_precision": [vp, ctypes.c_int],
         "ibpm_curl_flux": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, vp, vp],
         "ibpm_curl_scalar": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                              ctypes.c_double, ctypes.c_double, vp, vp],
@@ -186,6 +189,14 @@ class Context:
         d_f = DeviceArray.of(self, f)
         d_u = DeviceArray(self, ngrid * (nx - 1) * (ny - 1))
         _check(load().ibpm_elliptic_solve(self.h, kind, nx, ny, ngrid, dx, alpha, d_f.ptr, d_u.ptr))
+        return d_u.get()
+
+    def elliptic_solve_precision(self, kind, nx, ny, ngrid, dx, alpha, f, precision):
+        """ibpm_elliptic_solve with FP64 (0) or FP32 (1) transforms."""
+        d_f = DeviceArray.of(self, f)
+        d_u = DeviceArray(self, ngrid * (nx - 1) * (ny - 1))
+        _check(load().ibpm_elliptic_solve_precision(self.h, kind, nx, ny, ngrid, dx, alpha, d_f.ptr, d_u.ptr,
+                                                    precision))
         return d_u.get()
 
     def elliptic_solve_noalloc(self, kind, nx, ny, ngrid, dx, alpha, f):
@@ -430,6 +441,15 @@ class Batch:
         m = {"none": 0, "stale": 1, "lowrank": 1}[mode]
         k = pairs if mode == "lowrank" else 0
         _check(load().ibpm_batch_set_precond(self.h, m, k))
+        return self
+
+    PRECISION = {"fp64": 0, "fp32_step": 1, "fp32": 2}
+
+    def set_precision(self, mode="fp64"):
+        """Transform precision of the elliptic solves: "fp64" (the reference's),
+        "fp32_step" (FP32 transforms in the step's own solves) or "fp32" (also
+        inside the force solve's M applications); fields stay FP64."""
+        _check(load().ibpm_batch_set_precision(self.h, self.PRECISION[mode]))
         return self
 
     def set_local_helmholtz(self, on=True):

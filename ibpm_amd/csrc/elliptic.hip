@@ -32,72 +32,89 @@ __device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x +
 __device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 // -i * a
 __device__ inline double2 cmi(double2 a) { return make_double2(a.y, -a.x); }
+// The same for the FP32 transforms (float2).
+__device__ inline float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ inline float2 cmi(float2 a) { return make_float2(a.y, -a.x); }
+
+// Real type of a complex type, and a complex value from two reals.
+template <class C> struct RealOf;
+template <> struct RealOf<double2> { typedef double T; };
+template <> struct RealOf<float2> { typedef float T; };
+template <class C>
+__device__ inline C cpx(typename RealOf<C>::T x, typename RealOf<C>::T y) {
+    C r;
+    r.x = x;
+    r.y = y;
+    return r;
+}
 
 typedef double DoublePair __attribute__((ext_vector_type(2), aligned(8)));
 
-template <int R> __device__ inline void dft(double2* v);
-
-template <> __device__ inline void dft<2>(double2* v) {
-    double2 a = v[0], b = v[1];
-    v[0] = cadd(a, b);
-    v[1] = csub(a, b);
-}
-
-template <> __device__ inline void dft<3>(double2* v) {
-    const double h = 0.86602540378443864676;   // sin(2 pi / 3)
-    double2 t1 = cadd(v[1], v[2]);
-    double2 t2 = csub(v[1], v[2]);
-    double2 m = make_double2(v[0].x - 0.5 * t1.x, v[0].y - 0.5 * t1.y);
-    double2 r = cmi(t2);
-    v[0] = cadd(v[0], t1);
-    v[1] = make_double2(m.x + h * r.x, m.y + h * r.y);
-    v[2] = make_double2(m.x - h * r.x, m.y - h * r.y);
-}
-
-template <> __device__ inline void dft<4>(double2* v) {
-    double2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
-    double2 s13 = cadd(v[1], v[3]), d13 = cmi(csub(v[1], v[3]));
-    v[0] = cadd(s02, s13);
-    v[1] = cadd(d02, d13);
-    v[2] = csub(s02, s13);
-    v[3] = csub(d02, d13);
-}
-
-template <> __device__ inline void dft<5>(double2* v) {
-    const double c1 = 0.30901699437494742410;    // cos(2 pi / 5)
-    const double c2 = -0.80901699437494742410;   // cos(4 pi / 5)
-    const double s1 = 0.95105651629515357212;    // sin(2 pi / 5)
-    const double s2 = 0.58778525229247312917;    // sin(4 pi / 5)
-    double2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
-    double2 t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
-    double2 a1 = make_double2(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
-    double2 a2 = make_double2(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
-    double2 b1 = make_double2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y);
-    double2 b2 = make_double2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y);
-    b1 = cmi(b1);   // -i b1
-    b2 = cmi(b2);
-    v[0] = cadd(v[0], cadd(t1, t2));
-    v[1] = cadd(a1, b1);
-    v[4] = csub(a1, b1);
-    v[2] = cadd(a2, b2);
-    v[3] = csub(a2, b2);
-}
-
-template <> __device__ inline void dft<8>(double2* v) {
-    // radix 8 = two radix-4 DFTs of the even and odd samples + radix-2 merge
-    const double c = 0.70710678118654752440;
-    double2 a[4] = {v[0], v[2], v[4], v[6]};
-    double2 b[4] = {v[1], v[3], v[5], v[7]};
-    dft<4>(a);
-    dft<4>(b);
-    const double2 t0 = b[0];
-    const double2 t1 = make_double2(c * (b[1].x + b[1].y), c * (b[1].y - b[1].x));    // W8^1 b1
-    const double2 t2 = make_double2(b[2].y, -b[2].x);                                // W8^2 b2 = -i b2
-    const double2 t3 = make_double2(c * (b[3].y - b[3].x), -c * (b[3].x + b[3].y));   // W8^3 b3
-    v[0] = cadd(a[0], t0); v[4] = csub(a[0], t0);
-    v[1] = cadd(a[1], t1); v[5] = csub(a[1], t1);
-    v[2] = cadd(a[2], t2); v[6] = csub(a[2], t2);
-    v[3] = cadd(a[3], t3); v[7] = csub(a[3], t3);
+// Radix-R DFT in place (R = 2, 3, 4, 5, 8), FP64 (double2) or FP32 (float2):
+// the same operations in the same order for both, constants rounded to the
+// type.
+template <int R, class C>
+__device__ inline void dft(C* v) {
+    typedef typename RealOf<C>::T T;
+    if constexpr (R == 2) {
+        C a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if constexpr (R == 3) {
+        const T h = T(0.86602540378443864676);   // sin(2 pi / 3)
+        C t1 = cadd(v[1], v[2]);
+        C t2 = csub(v[1], v[2]);
+        C m = cpx<C>(v[0].x - T(0.5) * t1.x, v[0].y - T(0.5) * t1.y);
+        C r = cmi(t2);
+        v[0] = cadd(v[0], t1);
+        v[1] = cpx<C>(m.x + h * r.x, m.y + h * r.y);
+        v[2] = cpx<C>(m.x - h * r.x, m.y - h * r.y);
+    } else if constexpr (R == 4) {
+        C s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+        C s13 = cadd(v[1], v[3]), d13 = cmi(csub(v[1], v[3]));
+        v[0] = cadd(s02, s13);
+        v[1] = cadd(d02, d13);
+        v[2] = csub(s02, s13);
+        v[3] = csub(d02, d13);
+    } else if constexpr (R == 5) {
+        const T c1 = T(0.30901699437494742410);    // cos(2 pi / 5)
+        const T c2 = T(-0.80901699437494742410);   // cos(4 pi / 5)
+        const T s1 = T(0.95105651629515357212);    // sin(2 pi / 5)
+        const T s2 = T(0.58778525229247312917);    // sin(4 pi / 5)
+        C t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+        C t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+        C a1 = cpx<C>(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
+        C a2 = cpx<C>(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
+        C b1 = cpx<C>(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y);
+        C b2 = cpx<C>(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y);
+        b1 = cmi(b1);   // -i b1
+        b2 = cmi(b2);
+        v[0] = cadd(v[0], cadd(t1, t2));
+        v[1] = cadd(a1, b1);
+        v[4] = csub(a1, b1);
+        v[2] = cadd(a2, b2);
+        v[3] = csub(a2, b2);
+    } else {
+        static_assert(R == 8, "radix 2, 3, 4, 5 or 8");
+        // radix 8 = two radix-4 DFTs of the even and odd samples + radix-2 merge
+        const T c = T(0.70710678118654752440);
+        C a[4] = {v[0], v[2], v[4], v[6]};
+        C b[4] = {v[1], v[3], v[5], v[7]};
+        dft<4>(a);
+        dft<4>(b);
+        const C t0 = b[0];
+        const C t1 = cpx<C>(c * (b[1].x + b[1].y), c * (b[1].y - b[1].x));    // W8^1 b1
+        const C t2 = cpx<C>(b[2].y, -b[2].x);                                // W8^2 b2 = -i b2
+        const C t3 = cpx<C>(c * (b[3].y - b[3].x), -c * (b[3].x + b[3].y));   // W8^3 b3
+        v[0] = cadd(a[0], t0); v[4] = csub(a[0], t0);
+        v[1] = cadd(a[1], t1); v[5] = csub(a[1], t1);
+        v[2] = cadd(a[2], t2); v[6] = csub(a[2], t2);
+        v[3] = cadd(a[3], t3); v[7] = csub(a[3], t3);
+    }
 }
 
 // 16-point DFT as 4 x 4: X[k1 + 4 k2] = sum_n2 W16^(n2 k1) W4^(n2 k2)
@@ -148,15 +165,16 @@ __host__ __device__ constexpr bool compiled_len(int M) {
 // lengths are not LDS-limited and read the full table.
 __host__ __device__ constexpr bool quarter_twiddles(int M) { return M >= 2048 && M % 4 == 0; }
 
-template <int CM>
-__device__ inline double2 twiddle(const double2* tw, int k) {
+template <int CM, class C>
+__device__ inline C twiddle(const C* tw, int k) {
     if constexpr (CM != 0 && quarter_twiddles(CM)) {
+        typedef typename RealOf<C>::T T;
         constexpr int Q = CM / 4;
         const int q = k / Q, r = k - q * Q;
-        const double2 t = tw[r];
-        double x = t.x, y = t.y;
+        const C t = tw[r];
+        T x = t.x, y = t.y;
         if (q & 1) {   // times -i
-            const double u = x;
+            const T u = x;
             x = y;
             y = -u;
         }
@@ -164,7 +182,7 @@ __device__ inline double2 twiddle(const double2* tw, int k) {
             x = -x;
             y = -y;
         }
-        return make_double2(x, y);
+        return cpx<C>(x, y);
     } else {
         return tw[k];
     }
@@ -193,8 +211,8 @@ constexpr bool kSwizzleAll = CM >= 2048;
 // LDS; inputs are staged through registers so the stage runs in place.
 // Twiddles come from the LDS table (see twiddle()).  Template
 // arguments that are 0 are taken from the runtime values.
-template <int R, int CM, int CNP, int CNS, bool SWIN = false, bool SWOUT = false>
-__device__ inline void fft_stage(double2* buf, const double2* tw, int Mr, int npr, int Nsr) {
+template <int R, int CM, int CNP, int CNS, bool SWIN = false, bool SWOUT = false, class C = double2>
+__device__ inline void fft_stage(C* buf, const C* tw, int Mr, int npr, int Nsr) {
     const int M = CM ? CM : Mr;
     const int np = CM ? CNP : npr;
     const int Ns = CM ? CNS : Nsr;
@@ -203,7 +221,7 @@ __device__ inline void fft_stage(double2* buf, const double2* tw, int Mr, int np
     const int MR = M / R;
     const int total = np * MR;
     const int tstep = M / (Ns * R);
-    double2 v[MAXB][R];
+    C v[MAXB][R];
 #pragma unroll
     for (int b = 0; b < MAXB; ++b) {
         const int g = threadIdx.x + b * kFftThreads;
@@ -219,7 +237,7 @@ __device__ inline void fft_stage(double2* buf, const double2* tw, int Mr, int np
             } else {
                 // w^r by binary splitting, w^r = w^(r/2) w^(r - r/2): at most
                 // 3 dependent products for R <= 8
-                double2 pw[R];
+                C pw[R];
                 pw[1] = twiddle<CM>(tw, jm * tstep);
 #pragma unroll
                 for (int r = 2; r < R; ++r) pw[r] = cmul(pw[r / 2], pw[r - r / 2]);
@@ -250,28 +268,29 @@ __host__ __device__ constexpr int pick_radix(int rem) {
 }
 
 // Compile-time stage list: radix 8 first, then 4, 2, 5, 3.
-template <int M, int NP, int Ns>
-__device__ inline void fft_ct(double2* buf, const double2* tw) {
+template <int M, int NP, int Ns, class C>
+__device__ inline void fft_ct(C* buf, const C* tw) {
     if constexpr (Ns < M) {
         constexpr int R = pick_radix(M / Ns);
         constexpr bool first = Ns == 1, second = Ns > 1 && Ns == pick_radix(M);
         constexpr bool all = kSwizzleAll<M>;
-        fft_stage<R, M, NP, Ns, all || second, all || (first && R < M)>(buf, tw, 0, 0, 0);
-        fft_ct<M, NP, Ns * R>(buf, tw);
+        fft_stage<R, M, NP, Ns, all || second, all || (first && R < M), C>(buf, tw, 0, 0, 0);
+        fft_ct<M, NP, Ns * R, C>(buf, tw);
     }
 }
 
 // Runtime stage list of a host-built plan (any 2(N+1) with factors 2,3,5).
-__device__ inline void fft_rt(double2* buf, const double2* tw, const FftPlan& P) {
+template <class C>
+__device__ inline void fft_rt(C* buf, const C* tw, const FftPlan& P) {
     int Ns = 1;
     for (int s = 0; s < P.nst; ++s) {
         const int R = P.rad[s];
         switch (R) {
-        case 8: fft_stage<8, 0, 0, 0>(buf, tw, P.M, P.np, Ns); break;
-        case 4: fft_stage<4, 0, 0, 0>(buf, tw, P.M, P.np, Ns); break;
-        case 2: fft_stage<2, 0, 0, 0>(buf, tw, P.M, P.np, Ns); break;
-        case 3: fft_stage<3, 0, 0, 0>(buf, tw, P.M, P.np, Ns); break;
-        default: fft_stage<5, 0, 0, 0>(buf, tw, P.M, P.np, Ns); break;
+        case 8: fft_stage<8, 0, 0, 0, false, false, C>(buf, tw, P.M, P.np, Ns); break;
+        case 4: fft_stage<4, 0, 0, 0, false, false, C>(buf, tw, P.M, P.np, Ns); break;
+        case 2: fft_stage<2, 0, 0, 0, false, false, C>(buf, tw, P.M, P.np, Ns); break;
+        case 3: fft_stage<3, 0, 0, 0, false, false, C>(buf, tw, P.M, P.np, Ns); break;
+        default: fft_stage<5, 0, 0, 0, false, false, C>(buf, tw, P.M, P.np, Ns); break;
         }
         Ns *= R;
     }
@@ -281,10 +300,10 @@ __host__ __device__ constexpr int np_of(int M) {
     return kFftLds / M < 1 ? 1 : (kFftLds / M > kFftNpMax ? kFftNpMax : kFftLds / M);
 }
 
-template <int CM>
-__device__ inline void fft_any(double2* buf, const double2* tw, const FftPlan& P) {
-    if constexpr (CM != 0) fft_ct<CM, np_of(CM), 1>(buf, tw);
-    else fft_rt(buf, tw, P);
+template <int CM, class C>
+__device__ inline void fft_any(C* buf, const C* tw, const FftPlan& P) {
+    if constexpr (CM != 0) fft_ct<CM, np_of(CM), 1, C>(buf, tw);
+    else fft_rt<C>(buf, tw, P);
 }
 
 // ------------------------------------- 1024- and 2048-point transforms in registers
@@ -532,37 +551,49 @@ __device__ inline bool on_bc_ring(const EllJob& J, int i, int j) {
 // The three passes are written once over a template length CM (0: runtime
 // plan).  Row length N, FFT length M and rows per block follow from the plan,
 // so with CM != 0 every index computation folds to constants.
-template <int CM>
-__device__ inline void load_twiddles(double2* tw, const FftPlan& P) {
+template <int CM, class C>
+__device__ inline void load_twiddles(C* tw, const FftPlan& P) {
+    typedef typename RealOf<C>::T T;
     const int M = (CM && quarter_twiddles(CM)) ? CM / 4 : (CM ? CM : P.M);
     const IBPM_GLOBAL double* g = gin((const double*)P.tw);
-    for (int k = threadIdx.x; k < M; k += kFftThreads) tw[k] = make_double2(g[2 * k], g[2 * k + 1]);
+    for (int k = threadIdx.x; k < M; k += kFftThreads) tw[k] = cpx<C>(T(g[2 * k]), T(g[2 * k + 1]));
 }
 
+// The transposed scratch rows T1, T2 as the transform's real type: FP64, or
+// FP32 in the FP32 passes (the same buffers, half of each used).
+template <class T>
+__device__ inline const IBPM_GLOBAL T* tin(const double* p) { return (const IBPM_GLOBAL T*)p; }
+template <class T>
+__device__ inline IBPM_GLOBAL T* tout(double* p) { return (IBPM_GLOBAL T*)p; }
 
-template <int CM>
-__device__ __forceinline__ void pass1_lds(const EllJob& J, double2* smem, int bx) {
+
+// The LDS passes are also instanced over C = float2: FP32 transforms and
+// transposed scratch, with the right-hand side, boundary terms, eigenvalue
+// scaling and output in FP64.
+template <int CM, class C = double2>
+__device__ __forceinline__ void pass1_lds(const EllJob& J, C* smem, int bx) {
+    typedef typename RealOf<C>::T T;
     const FftPlan& P = J.pj;
     const int M = CM ? CM : P.M, np = CM ? np_of(CM) : P.np;
     const int N1 = J.nx - 1, N2 = M / 2 - 1;
     const int rows = 2 * np;
     const int r0 = bx * rows;
     if (r0 >= N1) return;
-    double2* buf = smem;
-    double2* tw = smem + np * M;
+    C* buf = smem;
+    C* tw = smem + np * M;
     load_twiddles<CM>(tw, P);
     const int n = N2 + 1;
     for (int idx = threadIdx.x; idx < np * N2; idx += kFftThreads) {
         const int p = idx / N2, k = idx - p * N2;
         const int ra = r0 + 2 * p, rb = ra + 1;
-        const double va = (ra < N1) ? rhs_value(J, ra + 1, k + 1) : 0.0;
-        const double vb = (rb < N1) ? rhs_value(J, rb + 1, k + 1) : 0.0;
-        buf[lds_slot<kSwizzleAll<CM>>(p * M + k + 1)] = make_double2(va, vb);
-        buf[lds_slot<kSwizzleAll<CM>>(p * M + M - 1 - k)] = make_double2(-va, -vb);
+        const T va = (ra < N1) ? T(rhs_value(J, ra + 1, k + 1)) : T(0);
+        const T vb = (rb < N1) ? T(rhs_value(J, rb + 1, k + 1)) : T(0);
+        buf[lds_slot<kSwizzleAll<CM>>(p * M + k + 1)] = cpx<C>(va, vb);
+        buf[lds_slot<kSwizzleAll<CM>>(p * M + M - 1 - k)] = cpx<C>(-va, -vb);
     }
     for (int p = threadIdx.x; p < np; p += kFftThreads) {
-        buf[lds_slot<kSwizzleAll<CM>>(p * M)] = make_double2(0.0, 0.0);
-        buf[lds_slot<kSwizzleAll<CM>>(p * M + n)] = make_double2(0.0, 0.0);
+        buf[lds_slot<kSwizzleAll<CM>>(p * M)] = cpx<C>(T(0), T(0));
+        buf[lds_slot<kSwizzleAll<CM>>(p * M + n)] = cpx<C>(T(0), T(0));
     }
     __syncthreads();
     fft_any<CM>(buf, tw, P);
@@ -570,34 +601,35 @@ __device__ __forceinline__ void pass1_lds(const EllJob& J, double2* smem, int bx
         const int r = idx % rows, k = idx / rows;
         const int row = r0 + r;
         if (row >= N1) continue;
-        const double2 z = buf[lds_slot<kSwizzleAll<CM>>((r >> 1) * M + k + 1)];
-        gout(J.T1)[(long)k * J.ld1 + row] = (r & 1) ? z.x : -z.y;
+        const C z = buf[lds_slot<kSwizzleAll<CM>>((r >> 1) * M + k + 1)];
+        tout<T>(J.T1)[(long)k * J.ld1 + row] = (r & 1) ? z.x : -z.y;
     }
 }
 
-template <int CM>
-__device__ __forceinline__ void pass2_lds(const EllJob& J, double2* smem, int bx) {
+template <int CM, class C = double2>
+__device__ __forceinline__ void pass2_lds(const EllJob& J, C* smem, int bx) {
+    typedef typename RealOf<C>::T T;
     const FftPlan& P = J.pi;
     const int M = CM ? CM : P.M, np = CM ? np_of(CM) : P.np;
     const int N1 = M / 2 - 1, N2 = J.ny - 1;
     const int rows = 2 * np;
     const int r0 = bx * rows;   // rows are l = 0..N2-1
     if (r0 >= N2) return;
-    double2* buf = smem;
-    double2* tw = smem + np * M;
+    C* buf = smem;
+    C* tw = smem + np * M;
     load_twiddles<CM>(tw, P);
     const int n = N1 + 1;
     for (int idx = threadIdx.x; idx < np * N1; idx += kFftThreads) {
         const int p = idx / N1, k = idx - p * N1;
         const int la = r0 + 2 * p, lb = la + 1;
-        const double va = (la < N2) ? gin(J.T1)[(long)la * J.ld1 + k] : 0.0;
-        const double vb = (lb < N2) ? gin(J.T1)[(long)lb * J.ld1 + k] : 0.0;
-        buf[lds_slot<kSwizzleAll<CM>>(p * M + k + 1)] = make_double2(va, vb);
-        buf[lds_slot<kSwizzleAll<CM>>(p * M + M - 1 - k)] = make_double2(-va, -vb);
+        const T va = (la < N2) ? tin<T>(J.T1)[(long)la * J.ld1 + k] : T(0);
+        const T vb = (lb < N2) ? tin<T>(J.T1)[(long)lb * J.ld1 + k] : T(0);
+        buf[lds_slot<kSwizzleAll<CM>>(p * M + k + 1)] = cpx<C>(va, vb);
+        buf[lds_slot<kSwizzleAll<CM>>(p * M + M - 1 - k)] = cpx<C>(-va, -vb);
     }
     for (int p = threadIdx.x; p < np; p += kFftThreads) {
-        buf[lds_slot<kSwizzleAll<CM>>(p * M)] = make_double2(0.0, 0.0);
-        buf[lds_slot<kSwizzleAll<CM>>(p * M + n)] = make_double2(0.0, 0.0);
+        buf[lds_slot<kSwizzleAll<CM>>(p * M)] = cpx<C>(T(0), T(0));
+        buf[lds_slot<kSwizzleAll<CM>>(p * M + n)] = cpx<C>(T(0), T(0));
     }
     __syncthreads();
     fft_any<CM>(buf, tw, P);
@@ -605,7 +637,7 @@ __device__ __forceinline__ void pass2_lds(const EllJob& J, double2* smem, int bx
     for (int idx = threadIdx.x; idx < np * N1; idx += kFftThreads) {
         const int p = idx / N1, k = idx - p * N1;
         const int la = r0 + 2 * p, lb = la + 1;
-        const double2 z = buf[lds_slot<kSwizzleAll<CM>>(p * M + k + 1)];
+        const C z = buf[lds_slot<kSwizzleAll<CM>>(p * M + k + 1)];
         double ya = -z.y, yb = z.x;
         const double cx = gin(J.cosx)[k];
         if (la < N2) {
@@ -616,12 +648,12 @@ __device__ __forceinline__ void pass2_lds(const EllJob& J, double2* smem, int bx
             const double lam = 2. * (cx + gin(J.cosy)[lb] - 2.) * J.bydx2;
             yb *= (J.kind == 0) ? 1. / lam : 1. / (1 + J.alpha * lam);
         }
-        buf[lds_slot<kSwizzleAll<CM>>(p * M + k + 1)] = make_double2(ya, yb);
-        buf[lds_slot<kSwizzleAll<CM>>(p * M + M - 1 - k)] = make_double2(-ya, -yb);
+        buf[lds_slot<kSwizzleAll<CM>>(p * M + k + 1)] = cpx<C>(T(ya), T(yb));
+        buf[lds_slot<kSwizzleAll<CM>>(p * M + M - 1 - k)] = cpx<C>(T(-ya), T(-yb));
     }
     for (int p = threadIdx.x; p < np; p += kFftThreads) {
-        buf[lds_slot<kSwizzleAll<CM>>(p * M)] = make_double2(0.0, 0.0);
-        buf[lds_slot<kSwizzleAll<CM>>(p * M + n)] = make_double2(0.0, 0.0);
+        buf[lds_slot<kSwizzleAll<CM>>(p * M)] = cpx<C>(T(0), T(0));
+        buf[lds_slot<kSwizzleAll<CM>>(p * M + n)] = cpx<C>(T(0), T(0));
     }
     __syncthreads();
     fft_any<CM>(buf, tw, P);
@@ -629,47 +661,48 @@ __device__ __forceinline__ void pass2_lds(const EllJob& J, double2* smem, int bx
         const int r = idx % rows, k = idx / rows;
         const int l = r0 + r;
         if (l >= N2) continue;
-        const double2 z = buf[lds_slot<kSwizzleAll<CM>>((r >> 1) * M + k + 1)];
-        gout(J.T2)[(long)k * J.ld2 + l] = (r & 1) ? z.x : -z.y;
+        const C z = buf[lds_slot<kSwizzleAll<CM>>((r >> 1) * M + k + 1)];
+        tout<T>(J.T2)[(long)k * J.ld2 + l] = (r & 1) ? z.x : -z.y;
     }
 }
 
-template <int CM>
-__device__ __forceinline__ void pass3_lds(const EllJob& J, double2* smem, int bx) {
+template <int CM, class C = double2>
+__device__ __forceinline__ void pass3_lds(const EllJob& J, C* smem, int bx) {
+    typedef typename RealOf<C>::T T;
     const FftPlan& P = J.pj;
     const int M = CM ? CM : P.M, np = CM ? np_of(CM) : P.np;
     const int N1 = J.nx - 1, N2 = M / 2 - 1;
     const int rows = 2 * np;
     const int r0 = bx * rows;
     if (r0 >= N1) return;
-    double2* buf = smem;
-    double2* tw = smem + np * M;
+    C* buf = smem;
+    C* tw = smem + np * M;
     load_twiddles<CM>(tw, P);
     const int n = N2 + 1;
     for (int idx = threadIdx.x; idx < np * N2; idx += kFftThreads) {
         const int p = idx / N2, k = idx - p * N2;
         const int ra = r0 + 2 * p, rb = ra + 1;
-        const double va = (ra < N1) ? gin(J.T2)[(long)ra * J.ld2 + k] : 0.0;
-        const double vb = (rb < N1) ? gin(J.T2)[(long)rb * J.ld2 + k] : 0.0;
-        buf[lds_slot<kSwizzleAll<CM>>(p * M + k + 1)] = make_double2(va, vb);
-        buf[lds_slot<kSwizzleAll<CM>>(p * M + M - 1 - k)] = make_double2(-va, -vb);
+        const T va = (ra < N1) ? tin<T>(J.T2)[(long)ra * J.ld2 + k] : T(0);
+        const T vb = (rb < N1) ? tin<T>(J.T2)[(long)rb * J.ld2 + k] : T(0);
+        buf[lds_slot<kSwizzleAll<CM>>(p * M + k + 1)] = cpx<C>(va, vb);
+        buf[lds_slot<kSwizzleAll<CM>>(p * M + M - 1 - k)] = cpx<C>(-va, -vb);
     }
     for (int p = threadIdx.x; p < np; p += kFftThreads) {
-        buf[lds_slot<kSwizzleAll<CM>>(p * M)] = make_double2(0.0, 0.0);
-        buf[lds_slot<kSwizzleAll<CM>>(p * M + n)] = make_double2(0.0, 0.0);
+        buf[lds_slot<kSwizzleAll<CM>>(p * M)] = cpx<C>(T(0), T(0));
+        buf[lds_slot<kSwizzleAll<CM>>(p * M + n)] = cpx<C>(T(0), T(0));
     }
     __syncthreads();
     fft_any<CM>(buf, tw, P);
     for (int idx = threadIdx.x; idx < np * N2; idx += kFftThreads) {
         const int p = idx / N2, k = idx - p * N2;
-        const double2 z = buf[lds_slot<kSwizzleAll<CM>>(p * M + k + 1)];
+        const C z = buf[lds_slot<kSwizzleAll<CM>>(p * M + k + 1)];
         const int ra = r0 + 2 * p, rb = ra + 1;
         // sinTransformInv normalisation, EllipticSolver2d.cc:80-83
         // with the epilogue, the finer level's BCs must come from v itself
         // (EllipticSolver.cc:62-66 solves c = A^-1 c before omega = omega* - c)
         if (ra < N1) {
             const long o = (long)ra * N2 + k;
-            const double v = -z.y * J.norm;
+            const double v = -(double)z.y * J.norm;
             const double uv = J.epilogue ? gin(J.W)[o] - v : v;
             gout(J.u)[o] = uv;
             if (J.u2) gout(J.u2)[o] = uv;
@@ -677,7 +710,7 @@ __device__ __forceinline__ void pass3_lds(const EllJob& J, double2* smem, int bx
         }
         if (rb < N1) {
             const long o = (long)rb * N2 + k;
-            const double v = z.x * J.norm;
+            const double v = (double)z.x * J.norm;
             const double uv = J.epilogue ? gin(J.W)[o] - v : v;
             gout(J.u)[o] = uv;
             if (J.u2) gout(J.u2)[o] = uv;
@@ -1052,6 +1085,25 @@ IBPM_PASS_KERNEL(k_fs_pass1, pass1_body, 1, true)
 IBPM_PASS_KERNEL(k_fs_pass2, pass2_body, 2, true)
 IBPM_PASS_KERNEL(k_fs_pass3, pass3_body, 3, true)
 
+// FP32 instances of the LDS passes (every length, runtime plan for lengths
+// without a compiled one): the step's own solves (k32_ell_pass*) and those
+// inside the force solve (k32_fs_pass*).
+#define IBPM_PASS32_KERNEL(NAME, LDSFN)                                                        \
+    template <int CM>                                                                          \
+    __global__ void __launch_bounds__(kFftThreads) NAME(const EllJob* __restrict__ jobs) {    \
+        extern __shared__ float2 smem32[];                                                     \
+        const int2 blk = xcd_block();                                                          \
+        const EllJob& J = jobs[blk.y];                                                         \
+        if (skipped(J.skip)) return;                                                           \
+        LDSFN<CM, float2>(J, smem32, blk.x);                                                   \
+    }
+IBPM_PASS32_KERNEL(k32_ell_pass1, pass1_lds)
+IBPM_PASS32_KERNEL(k32_ell_pass2, pass2_lds)
+IBPM_PASS32_KERNEL(k32_ell_pass3, pass3_lds)
+IBPM_PASS32_KERNEL(k32_fs_pass1, pass1_lds)
+IBPM_PASS32_KERNEL(k32_fs_pass2, pass2_lds)
+IBPM_PASS32_KERNEL(k32_fs_pass3, pass3_lds)
+
 // --------------------------------------------------------------- coarsify
 // Scalar::coarsify (reference Scalar.cc:46-63) for one level, writing only
 // the central region into R; neighbours at level lev-1 come from R where the
@@ -1280,6 +1332,37 @@ int blocks_for_pass(const EllJob& J, int pass, bool regs) {
     case 2048: hipLaunchKernelGGL((KER<2048, true>), grid, dim3(r16_group(PASS) * 2048 / 16), lds_bytes, s, d_jobs); break; \
     default: hipLaunchKernelGGL((KER<0, false>), grid, dim3(kFftThreads), lds_bytes, s, d_jobs); break;      \
     }
+
+int lds_for_pass32(const EllJob& J, int pass) {
+    const FftPlan& P = (pass == 2) ? J.pi : J.pj;
+    const int ntw = (compiled_len(P.M) && quarter_twiddles(P.M)) ? P.M / 4 : P.M;
+    return (int)sizeof(float2) * (P.np * P.M + ntw);
+}
+
+#define IBPM_LAUNCH32_LEN(KER)                                                                    \
+    switch (M) {                                                                                  \
+    case 400: hipLaunchKernelGGL(KER<400>, grid, dim3(kFftThreads), lds_bytes, s, d_jobs); break;   \
+    case 512: hipLaunchKernelGGL(KER<512>, grid, dim3(kFftThreads), lds_bytes, s, d_jobs); break;   \
+    case 768: hipLaunchKernelGGL(KER<768>, grid, dim3(kFftThreads), lds_bytes, s, d_jobs); break;   \
+    case 800: hipLaunchKernelGGL(KER<800>, grid, dim3(kFftThreads), lds_bytes, s, d_jobs); break;   \
+    case 1024: hipLaunchKernelGGL(KER<1024>, grid, dim3(kFftThreads), lds_bytes, s, d_jobs); break; \
+    case 2048: hipLaunchKernelGGL(KER<2048>, grid, dim3(kFftThreads), lds_bytes, s, d_jobs); break; \
+    default: hipLaunchKernelGGL(KER<0>, grid, dim3(kFftThreads), lds_bytes, s, d_jobs); break;      \
+    }
+
+// FP32 passes (LDS transforms); max_blocks from blocks_for_pass(J, pass,
+// false), lds_bytes from lds_for_pass32.
+void launch_ell_pass32(int pass, const EllJob* d_jobs, int njobs, int max_blocks, int lds_bytes, hipStream_t s,
+                       bool force_solve, int M) {
+    dim3 grid(max_blocks, njobs);
+    if (pass == 1) {
+        if (force_solve) { IBPM_LAUNCH32_LEN(k32_fs_pass1) } else { IBPM_LAUNCH32_LEN(k32_ell_pass1) }
+    } else if (pass == 2) {
+        if (force_solve) { IBPM_LAUNCH32_LEN(k32_fs_pass2) } else { IBPM_LAUNCH32_LEN(k32_ell_pass2) }
+    } else {
+        if (force_solve) { IBPM_LAUNCH32_LEN(k32_fs_pass3) } else { IBPM_LAUNCH32_LEN(k32_ell_pass3) }
+    }
+}
 
 // M: the FFT length of this pass (J.pj.M for passes 1 and 3, J.pi.M for
 // pass 2), the same for every job of the launch; regs: pass_in_registers for

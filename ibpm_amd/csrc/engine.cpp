@@ -180,13 +180,17 @@ struct SolveSpec {
 // the batch stream, accumulated per kernel class with its algorithmic bytes.
 // fs_*: the DST passes and coarsify of the M applications inside the force
 // solve (kernels k_fs_*), counted with the bytes of the flows still iterating.
+// ell32_* / fs32_*: the same passes with FP32 transforms (precision modes).
 enum KClass { K_PASS1, K_PASS2, K_PASS3, K_COARSIFY, K_VEL, K_CROSS, K_RHS, K_CURLQ, K_INTERP,
               K_SMEAR, K_MINV, K_CG, K_FORCES, K_CLEAR, K_FSPASS1, K_FSPASS2, K_FSPASS3, K_FSCOARSIFY,
-              K_FSHELM, K_HELM, K_NCLASS };
+              K_FSHELM, K_HELM, K_PASS32_1, K_PASS32_2, K_PASS32_3, K_FSPASS32_1, K_FSPASS32_2,
+              K_FSPASS32_3, K_NCLASS };
 static const char* kclass_names[K_NCLASS] = {"ell_pass1", "ell_pass2", "ell_pass3", "coarsify", "vel",
                                              "cross", "rhs", "curl_q", "interp", "smear_curl", "minv",
                                              "cg", "forces", "clear_nodes", "fs_pass1", "fs_pass2",
-                                             "fs_pass3", "fs_coarsify", "fs_helm", "helm"};
+                                             "fs_pass3", "fs_coarsify", "fs_helm", "helm", "ell32_pass1",
+                                             "ell32_pass2", "ell32_pass3", "fs32_pass1", "fs32_pass2",
+                                             "fs32_pass3"};
 struct Probe {
     struct Rec { int cls; hipEvent_t a, b; double bytes; };
     std::vector<Rec> recs;
@@ -269,6 +273,9 @@ struct PassGroup {
     int mj = 0, mi = 0;                             // FFT lengths along j (passes 1, 3) and i (pass 2)
     int maxb[3] = {0, 0, 0}, lds[3] = {0, 0, 0};
     bool regs[3] = {false, false, false};           // pass_in_registers per pass
+    bool fp32 = false;                              // FP32 transforms (LDS passes, k32_*)
+    int maxb64[3] = {0, 0, 0}, lds64[3] = {0, 0, 0};   // the FP64 launch shape, kept for switching back
+    bool regs64[3] = {false, false, false};
     std::vector<int> jflow;                         // flow of each job
     std::vector<double> jbytes[3];                  // per pass: 16 B per point (one read + one write),
     double bytes[3] = {0, 0, 0};                    // less where a pass is restricted (pass_bytes)
@@ -277,6 +284,7 @@ struct PassGroup {
 
 struct SolvePlan {
     bool force_solve = false;                       // M application (k_fs_* kernels)
+    bool fp32 = false;                              // every group's transforms in FP32 (set_plan_fp32)
     int nlev = 0;                                   // max L over flows
     std::vector<std::vector<int>> cflow;            // flow of each coarsify job, per level
     std::vector<std::vector<double>> cjbytes;
@@ -354,6 +362,11 @@ static int build_solve_plan(Ctx* c, DevMem& mem, const std::vector<SolveSpec>& s
                     G.lds[k] = std::max(G.lds[k], lds_for_pass(J, k + 1, G.regs[k]));
                 }
             }
+            for (int k = 0; k < 3; ++k) {
+                G.regs64[k] = G.regs[k];
+                G.maxb64[k] = G.maxb[k];
+                G.lds64[k] = G.lds[k];
+            }
             CKR(mem.alloc(&G.d, jobs.size()));
             CK(hipMemcpy(G.d, jobs.data(), sizeof(EllJob) * jobs.size(), hipMemcpyHostToDevice));
             G.n = (int)jobs.size();
@@ -361,6 +374,36 @@ static int build_solve_plan(Ctx* c, DevMem& mem, const std::vector<SolveSpec>& s
         }
     }
     return 0;
+}
+
+static void group_bytes(PassGroup& G);
+
+// Switches a plan's transforms between FP64 (the register-held or LDS
+// passes chosen at build) and FP32 (LDS passes over every row: the partial-
+// pass restrictions of register-held passes do not apply, so an FP32 pass
+// computes a superset of what the FP64 one writes).
+static void set_plan_fp32(SolvePlan& P, bool on) {
+    P.fp32 = on;
+    for (auto& step : P.steps)
+        for (auto& G : step) {
+            G.fp32 = on;
+            for (int k = 0; k < 3; ++k) {
+                if (!on) {
+                    G.regs[k] = G.regs64[k];
+                    G.maxb[k] = G.maxb64[k];
+                    G.lds[k] = G.lds64[k];
+                    continue;
+                }
+                G.regs[k] = false;
+                G.maxb[k] = 0;
+                G.lds[k] = 0;
+                for (auto& J : G.h) {
+                    G.maxb[k] = std::max(G.maxb[k], blocks_for_pass(J, k + 1, false));
+                    G.lds[k] = std::max(G.lds[k], lds_for_pass32(J, k + 1));
+                }
+            }
+            if (!G.jbytes[0].empty()) group_bytes(G);
+        }
 }
 
 // The fine-to-coarse chain of a solve's right-hand side (central regions of
@@ -396,12 +439,17 @@ static void run_solve(const SolvePlan& P, hipStream_t s, Probe* pr = nullptr,
     if (coarsify) run_coarsify(P, s, pr, active);
     for (int st = 0; st < P.nlev; ++st)
         for (const PassGroup& G : P.steps[st]) {
-            for (int pass = 1; pass <= 3; ++pass)
-                timed(pr, (fs ? K_FSPASS1 : K_PASS1) + pass - 1,
-                      pr ? sum(G.jflow, G.jbytes[pass - 1], G.bytes[pass - 1]) : 0, s, [&] {
-                    launch_ell_pass(pass, G.d, G.n, G.maxb[pass - 1], G.lds[pass - 1], s, fs,
-                                    pass == 2 ? G.mi : G.mj, G.regs[pass - 1]);
+            for (int pass = 1; pass <= 3; ++pass) {
+                const int cls = G.fp32 ? (fs ? K_FSPASS32_1 : K_PASS32_1) : (fs ? K_FSPASS1 : K_PASS1);
+                timed(pr, cls + pass - 1, pr ? sum(G.jflow, G.jbytes[pass - 1], G.bytes[pass - 1]) : 0, s, [&] {
+                    if (G.fp32)
+                        launch_ell_pass32(pass, G.d, G.n, G.maxb[pass - 1], G.lds[pass - 1], s, fs,
+                                          pass == 2 ? G.mi : G.mj);
+                    else
+                        launch_ell_pass(pass, G.d, G.n, G.maxb[pass - 1], G.lds[pass - 1], s, fs,
+                                        pass == 2 ? G.mi : G.mj, G.regs[pass - 1]);
                 });
+            }
         }
 }
 
@@ -506,6 +554,7 @@ struct ibpm_batch {
     // the L^-1 solve's coarsify chain over its right-hand side's support
     // (restrict_m2l_pass3), when every pass 1 of that solve is register-held
     bool m2l_boxed = false;
+    int precision = 0;             // ibpm_batch_set_precision
     ibpmg::HelmJob* d_m2lc = nullptr;
     int m2lc_cap = 0, m2lc_n = 0, m2lc_blocks = 0;
     std::vector<int> m2lc_flow;
@@ -911,6 +960,12 @@ constexpr int kHelmMaxSweeps = 40;
 // only the rows a pass reads and writes.
 static void pass_bytes(const EllJob& J, const PassGroup& G, double out[3]) {
     const double N1 = J.nx - 1, N2 = J.ny - 1;
+    if (G.fp32) {   // whole level: FP64 in, FP32 T1; FP32 T1 -> T2; FP32 T2, FP64 out
+        out[0] = 12.0 * N1 * N2;
+        out[1] = 8.0 * N1 * N2;
+        out[2] = 12.0 * N1 * N2;
+        return;
+    }
     const int kG1 = r16_pass_group(1);   // row pairs per workgroup of the register-held pass 1
     // pass 1: T1 whole; rows read unless their workgroup takes the closed form
     double read1 = N1;
@@ -1238,6 +1293,7 @@ static int plan_local_helm(ibpm_batch* B, int i) {
         }
         B->M2L.force_solve = true;
         CKR(build_solve_plan(&B->ctx->c, B->mem, m2l, B->M2L));
+        if (B->precision >= 2) set_plan_fp32(B->M2L, true);
         B->m2l_built = true;
     }
     CKR(restrict_m2l_pass3(B, nsup));
@@ -1291,7 +1347,8 @@ static void enqueue_M(ibpm_batch* B, int i, RegOps* smear, hipStream_t s,
         const int nl = helm_launches(B->helm_sweeps);
         launch_helm_sweeps(B->d_helm, B->helm_n, B->helm_blocks, B->helm_sweeps, s,
                            [&](const std::function<void()>& fn) { timed(pr, K_FSHELM, by / nl, s, fn); });
-        if (B->m2l_boxed) {
+        const bool boxed = B->m2l_boxed && !B->M2L.fp32;   // the boxed chain feeds register-held passes only
+        if (boxed) {
             double cb = 0;
             if (pr)
                 for (size_t j = 0; j < B->m2lc_flow.size(); ++j)
@@ -1300,7 +1357,7 @@ static void enqueue_M(ibpm_batch* B, int i, RegOps* smear, hipStream_t s,
                 timed(pr, K_FSCOARSIFY, cb / std::max(1, B->M2L.nlev - 1), s,
                       [&] { launch_helm_coarsify(B->d_m2lc, B->m2lc_n, B->m2lc_blocks, lev, s); });
         }
-        run_solve(B->M2L, s, pr, active, !B->m2l_boxed);
+        run_solve(B->M2L, s, pr, active, !boxed);
         B->st_local_mapps += 1;
     } else {
         run_solve(B->M1[i], s, pr, active);
@@ -2046,7 +2103,8 @@ int ibpm_dst2d(ibpm_ctx* c, int n1, int n2, const double* d_in, double* d_out) {
 }
 
 static int solve_common(ibpm_ctx* c, int kind, int nx, int ny, int ngrid, double dx, double alpha,
-                        const double* d_f, const double* d_bc, double* d_u, double in_scale = 1.0) {
+                        const double* d_f, const double* d_bc, double* d_u, double in_scale = 1.0,
+                        int precision = 0) {
     CKR(check_grid(nx, ny, ngrid));
     DevMem mem;
     const long ns = (long)(nx - 1) * (ny - 1);
@@ -2064,6 +2122,7 @@ static int solve_common(ibpm_ctx* c, int kind, int nx, int ny, int ngrid, double
     s.cosx = cosx; s.cosy = cosy; s.bcring = d_bc;
     SolvePlan P;
     CKR(build_solve_plan(&c->c, mem, {s}, P));
+    if (precision == 1) set_plan_fp32(P, true);
     run_solve(P, c->c.stream);
     CK(hipStreamSynchronize(c->c.stream));
     CK(hipGetLastError());
@@ -2078,6 +2137,15 @@ int ibpm_elliptic2d_solve(ibpm_ctx* c, int kind, int nx, int ny, double dx, doub
 int ibpm_elliptic_solve(ibpm_ctx* c, int kind, int nx, int ny, int ngrid, double dx, double alpha,
                         const double* d_f, double* d_u) {
     return solve_common(c, kind, nx, ny, ngrid, dx, alpha, d_f, nullptr, d_u);
+}
+
+int ibpm_elliptic_solve_precision(ibpm_ctx* c, int kind, int nx, int ny, int ngrid, double dx, double alpha,
+                                  const double* d_f, double* d_u, int precision) {
+    if (precision != 0 && precision != 1) {
+        set_error("precision %d: 0 (FP64 transforms) or 1 (FP32 transforms)", precision);
+        return -1;
+    }
+    return solve_common(c, kind, nx, ny, ngrid, dx, alpha, d_f, nullptr, d_u, 1.0, precision);
 }
 
 // ibpm_elliptic_solve_noalloc: the multi-domain solve of ibpm_elliptic_solve
@@ -2783,6 +2851,26 @@ int ibpm_batch_set_precond(ibpm_batch* B, int mode, int pairs) {
     B->pc_mode = mode;
     B->pc_pairs = mode ? pairs : 0;
     if (mode) CKR(setup_pcg(B));
+    return 0;
+}
+
+int ibpm_batch_set_precision(ibpm_batch* B, int mode) {
+    if (!B || mode < 0 || mode > 2) { set_error("precision mode %d: 0, 1 or 2", mode); return -1; }
+    B->precision = mode;
+    const bool step32 = mode >= 1, fs32 = mode >= 2;
+    for (int i = 0; i < B->nsub; ++i) {
+        set_plan_fp32(B->S1[i], step32);
+        set_plan_fp32(B->S2[i], step32);
+        set_plan_fp32(B->S3[i], step32);
+        set_plan_fp32(B->M1[i], fs32);
+    }
+    set_plan_fp32(B->S4, step32);
+    set_plan_fp32(B->M2, fs32);
+    if (!B->M2L.steps.empty()) set_plan_fp32(B->M2L, fs32);
+    if (B->graph) {   // the captured step holds the old kernels
+        CK(hipGraphExecDestroy(B->graph));
+        B->graph = nullptr;
+    }
     return 0;
 }
 

@@ -117,6 +117,9 @@ struct ForceOps {
 bool pass_in_registers(int M, long row_pairs);
 int blocks_for_pass(const EllJob& J, int pass, bool regs);
 int lds_for_pass(const EllJob& J, int pass, bool regs);
+int lds_for_pass32(const EllJob& J, int pass);
+void launch_ell_pass32(int pass, const EllJob* d_jobs, int njobs, int max_blocks, int lds_bytes, hipStream_t s,
+                       bool force_solve, int M);
 void launch_ell_pass(int pass, const EllJob* d_jobs, int njobs, int max_blocks, int lds_bytes, hipStream_t s,
                      bool force_solve, int M, bool regs);
 void launch_coarsify(const CoarsifyJob* d_jobs, int njobs, int max_rows, hipStream_t s, bool force_solve = false);

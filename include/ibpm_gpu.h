@@ -68,6 +68,15 @@ int ibpm_elliptic_solve(ibpm_ctx* ctx, int kind, int nx, int ny, int ngrid, doub
 int ibpm_elliptic_solve_noalloc(ibpm_ctx* ctx, int kind, int nx, int ny, int ngrid, double dx,
                                 double alpha, const double* d_f, double* d_u);
 
+/* ibpm_elliptic_solve with a choice of transform precision: 0 FP64 (the
+   reference's arithmetic; ibpm_elliptic_solve), 1 FP32 transforms and
+   transposed intermediates with the right-hand side, boundary terms,
+   eigenvalue scaling and output in FP64.  No counterpart in the reference
+   (whose solve is FP64 FFTW, src/EllipticSolver2d.cc:57-104); for measuring
+   the reduced-precision path's error per solve. */
+int ibpm_elliptic_solve_precision(ibpm_ctx* ctx, int kind, int nx, int ny, int ngrid, double dx,
+                                  double alpha, const double* d_f, double* d_u, int precision);
+
 /* ---- grid operators (VectorOperations) ----------------------------------- */
 /* Curl(Flux -> Scalar), src/VectorOperations.cc:33-54 */
 int ibpm_curl_flux(ibpm_ctx* ctx, int nx, int ny, int ngrid, double dx, const double* d_q, double* d_w);
@@ -263,6 +272,16 @@ int ibpm_batch_solver_stats(ibpm_batch* b, long* stats, int nstats);
    inside M stays a DST solve, run over the parts of each level whose psi is
    read.  on = 0: the DST solves always. */
 int ibpm_batch_set_local_helmholtz(ibpm_batch* b, int on);
+
+/* Transform precision of the batch's elliptic solves (no counterpart in the
+   reference, whose solves are FP64): 0 (the default) FP64 everywhere; 1 the
+   step's own solves (A^-1 a, L^-1 omega*, A^-1 c, the refresh's L^-1) run FP32
+   transforms and transposed intermediates, with FP64 right-hand sides, boundary
+   terms, eigenvalue scaling and fields, so omega, psi and q stay FP64; 2 also
+   the solves inside the force solve's M applications (CG's vectors, its
+   residual test and the Cholesky factor stay FP64).  Each reduced-precision
+   stage perturbs the FP64 answer by about the FP32 rounding of one solve. */
+int ibpm_batch_set_precision(ibpm_batch* b, int mode);
 
 /* Measurement: advances nsteps with a HIP event pair around every kernel
    launch on the batch stream and returns per kernel class the launch count,

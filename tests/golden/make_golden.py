@@ -5,10 +5,17 @@
     python tests/golden/make_golden.py cylinder_400x4        # 2000 steps, ~1.5 h
     python tests/golden/make_golden.py plate       # moving plate, 30 steps
     python tests/golden/make_golden.py plate_512x5_short   # configured moving plate, 12 steps
+    python tests/golden/make_golden.py mixed_64x3_f0 ...   # one flow of the 64-flow batch, 40 steps
+    python tests/golden/make_golden.py tandem_16x1024x6_c0 # first tandem case, 30 steps, ~1 h
 
 Each fixture holds the per-step Cd/Cl series (full precision), CG iterations
 per substep where CG is used, per-level vorticity norms and the level-0
-vorticity (every 2nd node of grids larger than 200^2).
+vorticity (every 2nd node of grids larger than 200^2, every 4th node for the
+fixtures of single flows drawn from the batched workloads).
+
+The mixed_64x3 fixtures cover one flow of each grid size and body type of the
+64-flow batch (flow indices MIXED_FLOWS); the GPU test runs them inside the
+whole batch.
 """
 import os
 import sys
@@ -26,12 +33,20 @@ from ibpm_amd import workloads as W  # noqa: E402
 PLATE = ("body Plate\n  line_n -0.5 0 0.5 0 33\n  center 0 0\n"
          "  motion pitchplunge 0.2617993877991494 0.4 0 0.2 0.4 0\nend\n")
 
+# first flow of each (grid size, body) pair of mixed_64x3:
+# 256: cylinder 2, ellipse 5, plate 4; 384: 20, 3, 11; 512: 0, 1, 51
+MIXED_FLOWS = (2, 5, 4, 20, 3, 11, 0, 1, 51)
+
 CASES = {
     "cylinder_200": (W.cylinder_200()[0], 250),
     "cylinder_400x4": (W.cylinder_400x4()[0], 2000),
     "plate": (W.Flow(128, 128, 3, 4.0, -2.0, -2.0, 300.0, 0.005, PLATE, tol=1e-10), 30),
     "plate_512x5_short": (W.plate_512x5()[0], 12),
+    "tandem_16x1024x6_c0": (W.tandem_16x1024x6()[0], 30),
 }
+for _k in MIXED_FLOWS:
+    CASES["mixed_64x3_f%d" % _k] = (W.mixed_64x3()[_k], 40)
+STRIDE4 = ("mixed_64x3_", "tandem_")
 
 
 def run(name):
@@ -54,6 +69,8 @@ def run(name):
     norms = [float(np.linalg.norm(w[l * ns:(l + 1) * ns])) for l in range(f.ngrid)]
     w0 = w[:ns].reshape(f.nx - 1, f.ny - 1)
     stride = 1 if f.nx <= 200 else 2
+    if name.startswith(STRIDE4):
+        stride = 4
     out = os.path.join(HERE, name + ".npz")
     np.savez_compressed(out, forces=np.array(forces), cg_iters=np.array(iters, dtype=np.int32),
                         omega_norms=np.array(norms), omega0=w0[::stride, ::stride], stride=stride,

@@ -50,6 +50,7 @@ def build(verbose=False, defines=(), out=OUT, obj_dir=OBJ):
     os.replace(tmp, out)
     if out == OUT:
         build_driver()
+        build_boundary_test()
     return out
 
 
@@ -63,6 +64,21 @@ def build_driver():
                            "-L" + HERE, "-libpm_gpu", "-Wl,-rpath,$ORIGIN"])
     os.replace(DRIVER + ".tmp", DRIVER)
     return DRIVER
+
+
+BOUNDARY_TEST = os.path.join(os.path.dirname(HERE), "tests", "cpp", "boundary_test")
+
+
+def build_boundary_test():
+    """The reference's solver-class tests restated over the C++ classes of
+    host/ibpm_gpu.hpp (run by tests/test_gpu_boundary.py)."""
+    src = BOUNDARY_TEST + ".cpp"
+    if not os.path.exists(src):
+        return None
+    subprocess.check_call(["g++", "-O2", "-std=c++17", "-Wall", "-o", BOUNDARY_TEST + ".tmp", src,
+                           "-L" + HERE, "-libpm_gpu", "-Wl,-rpath,$ORIGIN/../../ibpm_amd"])
+    os.replace(BOUNDARY_TEST + ".tmp", BOUNDARY_TEST)
+    return BOUNDARY_TEST
 
 
 if __name__ == "__main__":

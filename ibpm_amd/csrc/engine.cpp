@@ -2511,6 +2511,255 @@ int ibpm_reg_to_boundary(ibpm_reg* r, const double* d_q, double* d_f) {
     return 0;
 }
 
+// ------------------------------------------------------------------ model
+// One flow's NavierStokesModel operators over the stand-alone entry points:
+// the regulariser of the geometry's points at the model's time, and the
+// elliptic solves, curls and stencils of one grid.  The batch is the fast
+// path; this object is the reference's operator-level interface.
+}  // extern "C"
+
+struct ibpm_model {
+    ibpm_ctx* ctx = nullptr;
+    ibpm_flow_params p{};
+    std::string geom;
+    int nb = 0;
+    double dx = 0, u0 = 0, v0 = 0, time = 0;
+    std::vector<double> xy, uv;     // points and velocities at `time`
+    ibpm_reg* reg = nullptr;
+    ibpmg::DevMem mem;
+    double *d_q = nullptr, *d_w = nullptr, *d_u = nullptr, *d_f = nullptr, *d_y = nullptr;
+    long ns = 0, nf = 0;
+    ~ibpm_model() {
+        if (reg) ibpm_reg_destroy(reg);
+    }
+};
+
+namespace ibpmg {
+static int model_points(ibpm_model* m, double t) {
+    if (m->nb == 0) return 0;
+    if (ibpm_geom_points(m->geom.c_str(), 1, &t, m->xy.data(), m->uv.data(), 2L * m->nb) != m->nb) {
+        set_error("the geometry does not parse");
+        return -1;
+    }
+    return 0;
+}
+}  // namespace ibpmg
+
+extern "C" {
+
+int ibpm_model_create(ibpm_ctx* c, const ibpm_flow_params* p, double t0, ibpm_model** out) {
+    if (!c || !p || !out || !p->geom) { set_error("bad model arguments"); return -1; }
+    CKR(check_grid(p->nx, p->ny, p->ngrid));
+    if (!(p->reynolds > 0)) { set_error("Reynolds number must be positive"); return -1; }
+    auto m = std::make_unique<ibpm_model>();
+    m->ctx = c;
+    m->p = *p;
+    m->geom = p->geom;
+    m->p.geom = nullptr;
+    m->nb = ibpm_geom_num_points(m->geom.c_str());
+    if (m->nb < 0) { set_error("the geometry does not parse"); return -1; }
+    m->dx = p->length / p->nx;
+    double sa, ca;   // BaseFlow(grid, 1, alpha): UniformFlow of magnitude 1 (Flux.cc:79-99)
+    const double pi = 4. * std::atan(1.);
+    sin_cos(p->alpha_deg * pi / 180., sa, ca);
+    m->u0 = ca;
+    m->v0 = sa;
+    m->time = t0;
+    m->ns = (long)(p->nx - 1) * (p->ny - 1) * p->ngrid;
+    m->nf = (2L * p->nx * p->ny + p->nx + p->ny) * p->ngrid;
+    CKR(m->mem.alloc(&m->d_q, m->nf));
+    CKR(m->mem.alloc(&m->d_w, m->ns));
+    CKR(m->mem.alloc(&m->d_u, m->ns));
+    CKR(m->mem.alloc(&m->d_f, std::max(1, 2 * m->nb)));
+    CKR(m->mem.alloc(&m->d_y, std::max(1, 2 * m->nb)));
+    m->xy.assign(2 * (size_t)m->nb, 0.0);
+    m->uv.assign(2 * (size_t)m->nb, 0.0);
+    CKR(model_points(m.get(), t0));
+    if (m->nb > 0)
+        CKR(ibpm_reg_create(c, p->nx, p->ny, p->ngrid, p->length, p->xoffset, p->yoffset, m->nb, m->xy.data(),
+                            &m->reg));
+    *out = m.release();
+    return 0;
+}
+
+int ibpm_model_destroy(ibpm_model* m) {
+    delete m;
+    return 0;
+}
+
+int ibpm_model_num_points(ibpm_model* m) { return m ? m->nb : -1; }
+
+int ibpm_model_set_base_flow(ibpm_model* m, double mag, double alpha) {
+    if (!m) { set_error("null model"); return -1; }
+    double sa, ca;   // UniformFlow, Flux.cc:79-99
+    sin_cos(alpha, sa, ca);
+    m->u0 = mag * ca;
+    m->v0 = mag * sa;
+    return 0;
+}
+
+int ibpm_model_update_operators(ibpm_model* m, double time) {
+    if (!m) { set_error("null model"); return -1; }
+    m->time = time;
+    CKR(model_points(m, time));
+    if (m->reg) CKR(ibpm_reg_update(m->reg, m->xy.data()));
+    return 0;
+}
+
+int ibpm_model_B(ibpm_model* m, const double* d_f, double* d_omega) {
+    if (!m) { set_error("null model"); return -1; }
+    const ibpm_flow_params& p = m->p;
+    if (!m->reg) {   // no points: E^T f = 0
+        CK(hipMemsetAsync(d_omega, 0, sizeof(double) * m->ns, m->ctx->c.stream));
+        CK(hipStreamSynchronize(m->ctx->c.stream));
+        return 0;
+    }
+    CKR(ibpm_reg_to_flux(m->reg, d_f, m->d_q));
+    return ibpm_curl_flux(m->ctx, p.nx, p.ny, p.ngrid, m->dx, m->d_q, d_omega);
+}
+
+int ibpm_model_C(ibpm_model* m, const double* d_omega, double* d_f) {
+    if (!m) { set_error("null model"); return -1; }
+    if (!m->reg) return 0;
+    const ibpm_flow_params& p = m->p;
+    CKR(ibpm_flux_from_vorticity(m->ctx, p.nx, p.ny, p.ngrid, m->dx, 0.0, 0.0, d_omega, m->d_q));
+    return ibpm_reg_to_boundary(m->reg, m->d_q, d_f);
+}
+
+int ibpm_model_compute_flux(ibpm_model* m, const double* d_omega, double* d_q) {
+    if (!m) { set_error("null model"); return -1; }
+    const ibpm_flow_params& p = m->p;
+    return ibpm_flux_from_vorticity(m->ctx, p.nx, p.ny, p.ngrid, m->dx, m->u0, m->v0, d_omega, d_q);
+}
+
+int ibpm_model_constraints(ibpm_model* m, double* h_b) {
+    if (!m || (!h_b && m->nb)) { set_error("bad model arguments"); return -1; }
+    if (!m->reg) return 0;
+    const ibpm_flow_params& p = m->p;
+    // q0 = Curl(0) + base flux, then E q0 (getBaseFlowBoundaryVelocities)
+    CK(hipMemsetAsync(m->d_w, 0, sizeof(double) * m->ns, m->ctx->c.stream));
+    CKR(ibpm_curl_scalar(m->ctx, p.nx, p.ny, p.ngrid, m->dx, m->u0, m->v0, m->d_w, m->d_q));
+    CKR(ibpm_reg_to_boundary(m->reg, m->d_q, m->d_y));
+    std::vector<double> b0(2 * (size_t)m->nb);
+    CK(hipMemcpy(b0.data(), m->d_y, sizeof(double) * b0.size(), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < b0.size(); ++i) h_b[i] = m->uv[i] - b0[i];   // NavierStokesModel.cc:86-91
+    return 0;
+}
+
+int ibpm_model_M(ibpm_model* m, double beta, const double* d_f, double* d_y) {
+    if (!m) { set_error("null model"); return -1; }
+    if (!m->reg) return 0;
+    const ibpm_flow_params& p = m->p;
+    CKR(ibpm_model_B(m, d_f, m->d_w));                                   // B f
+    CKR(solve_common(m->ctx, 1, p.nx, p.ny, p.ngrid, m->dx, -beta / 2. / p.reynolds, m->d_w, nullptr, m->d_u,
+                     beta));                                              // A^-1 (beta B f)
+    return ibpm_model_C(m, m->d_u, d_y);                                  // C A^-1 beta B f
+}
+
+int ibpm_cholesky_build(ibpm_model* m, double beta, double* h_M) {
+    if (!m) { set_error("null model"); return -1; }
+    const int n = 2 * m->nb;
+    std::vector<double> e(n, 0.0), col(n);
+    for (int j = 0; j < n; ++j) {   // CholeskySolver::computeMatrixM, column j = M e_j
+        e.assign(n, 0.0);
+        e[j] = 1.0;
+        CK(hipMemcpy(m->d_f, e.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+        CKR(ibpm_model_M(m, beta, m->d_f, m->d_y));
+        CK(hipMemcpy(col.data(), m->d_y, sizeof(double) * n, hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; ++i) h_M[(size_t)i * n + j] = col[i];
+    }
+    return 0;
+}
+
+int ibpm_cholesky_factor(int n, const double* h_M, double* h_lower, double* h_diag) {
+    if (n < 0 || (n && (!h_M || !h_lower || !h_diag))) { set_error("bad Cholesky arguments"); return -1; }
+    std::vector<double> Mt((size_t)n * n), lower, d;
+    for (int r = 0; r < n; ++r)
+        for (int c2 = 0; c2 < n; ++c2) Mt[(size_t)c2 * n + r] = h_M[(size_t)r * n + c2];
+    const int bad = cholesky_factor(n, Mt, lower, d);
+    if (bad >= 0) { set_error("M is not positive definite (pivot %d)", bad); return -2; }
+    std::copy(lower.begin(), lower.end(), h_lower);
+    std::copy(d.begin(), d.end(), h_diag);
+    return 0;
+}
+
+int ibpm_cholesky_solve(int n, const double* h_lower, const double* h_diag, const double* h_b, double* h_x) {
+    if (n < 0 || (n && (!h_lower || !h_diag || !h_b || !h_x))) { set_error("bad Cholesky arguments"); return -1; }
+    // CholeskySolver::Minv (CholeskySolver.cc:200-224); x may alias b
+    for (int i = 0; i < n; ++i) {
+        double sum = h_b[i];
+        for (int k = i - 1; k >= 0; --k) sum -= h_lower[(size_t)i * n + k] * h_x[k];
+        h_x[i] = sum / h_diag[i];
+    }
+    for (int i = n - 1; i >= 0; --i) {
+        double sum = h_x[i];
+        for (int k = i + 1; k < n; ++k) sum -= h_lower[(size_t)k * n + i] * h_x[k];
+        h_x[i] = sum / h_diag[i];
+    }
+    return 0;
+}
+
+int ibpm_cholesky_save(const char* path, int n, double alpha_beta, const double* h_lower, const double* h_diag) {
+    if (!path || n < 0) { set_error("bad Cholesky file arguments"); return -1; }
+    std::vector<double> lower(h_lower, h_lower + (size_t)n * n), d(h_diag, h_diag + n);
+    if (!cholesky_write(path, n, alpha_beta, lower, d)) { set_error("could not write %s", path); return -1; }
+    return 0;
+}
+
+int ibpm_cholesky_load(const char* path, int n, double alpha_beta, double* h_lower, double* h_diag) {
+    if (!path || n < 0) { set_error("bad Cholesky file arguments"); return -1; }
+    std::vector<double> lower, d;
+    if (!cholesky_read(path, n, alpha_beta, lower, d)) {
+        set_error("%s: missing, or not a factor of this size and alpha*beta", path);
+        return -2;
+    }
+    std::copy(lower.begin(), lower.end(), h_lower);
+    std::copy(d.begin(), d.end(), h_diag);
+    return 0;
+}
+
+int ibpm_cg_solve(ibpm_model* m, double beta, double tol, int max_it, const double* h_b, const double* h_x0,
+                  double* h_x, int* it_count, int* converged) {
+    if (!m || (m->nb && (!h_b || !h_x))) { set_error("bad CG arguments"); return -1; }
+    const int n = 2 * m->nb;
+    std::vector<double> x(n), r(n), d(n), q(n);
+    for (int i = 0; i < n; ++i) x[i] = h_x0 ? h_x0[i] : 0.0;
+    auto M = [&](const std::vector<double>& v, std::vector<double>& out) -> int {
+        if (n == 0) return 0;
+        CK(hipMemcpy(m->d_f, v.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+        CKR(ibpm_model_M(m, beta, m->d_f, m->d_y));
+        CK(hipMemcpy(out.data(), m->d_y, sizeof(double) * n, hipMemcpyDeviceToHost));
+        return 0;
+    };
+    auto dot = [&](const std::vector<double>& a, const std::vector<double>& b) {   // BoundaryVector.h:240-252
+        double s2 = 0;
+        for (int i = 0; i < n; ++i) s2 += a[i] * b[i];
+        return s2;
+    };
+    // ConjugateGradientSolver::Minv (ConjugateGradientSolver.cc:35-65)
+    CKR(M(x, q));
+    for (int i = 0; i < n; ++i) r[i] = h_b[i] - q[i];
+    d = r;
+    double delta = dot(r, r);
+    const double tol2 = tol * tol;
+    int it = 0;
+    while (delta > tol2 && (max_it <= 0 || it < max_it)) {
+        CKR(M(d, q));
+        const double alpha = delta / dot(d, q);
+        for (int i = 0; i < n; ++i) x[i] += alpha * d[i];
+        for (int i = 0; i < n; ++i) r[i] -= alpha * q[i];
+        const double delta_old = delta;
+        delta = dot(r, r);
+        const double b2 = delta / delta_old;
+        for (int i = 0; i < n; ++i) d[i] = r[i] + b2 * d[i];
+        ++it;
+    }
+    std::copy(x.begin(), x.end(), h_x);
+    if (it_count) *it_count = it;
+    if (converged) *converged = delta <= tol2 ? 1 : 0;
+    return 0;
+}
+
 // ------------------------------------------------------------------ batch
 int ibpm_batch_create(ibpm_ctx* ctx, int nflows, const ibpm_flow_params* params, ibpm_batch** out) {
     if (!ctx || !params || !out || nflows < 1) { set_error("bad batch arguments"); return -1; }

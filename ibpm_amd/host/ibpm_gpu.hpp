@@ -16,6 +16,7 @@
 #pragma once
 #include <cmath>
 #include <fstream>
+#include <memory>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -50,6 +51,123 @@ private:
 };
 
 class Motion {};   // reference src/Motion.h: a token here; the library holds the motion
+
+// The context of the operator-level classes below (one per device, kept for
+// the life of the process).
+inline ibpm_ctx* default_context(int device = 0) {
+    static ibpm_ctx* ctx[64] = {};
+    if (device < 0 || device >= 64) throw std::runtime_error("ibpm_gpu: bad device");
+    if (!ctx[device]) gpu_check(ibpm_ctx_create(device, &ctx[device]));
+    return ctx[device];
+}
+
+// Device copy of a host array for one call (ibpm_malloc / ibpm_memcpy_*).
+class DeviceBuffer {
+public:
+    explicit DeviceBuffer(size_t n, ibpm_ctx* ctx = default_context()) : _ctx(ctx), _n(n) {
+        gpu_check(ibpm_malloc(_ctx, sizeof(double) * (n ? n : 1), &_p));
+    }
+    DeviceBuffer(const std::vector<double>& h, ibpm_ctx* ctx = default_context()) : DeviceBuffer(h.size(), ctx) {
+        if (_n) gpu_check(ibpm_memcpy_h2d(_ctx, _p, h.data(), sizeof(double) * _n));
+    }
+    ~DeviceBuffer() { ibpm_free(_ctx, _p); }
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    double* get() const { return (double*)_p; }
+    void to(std::vector<double>& h) const {
+        if (_n) gpu_check(ibpm_memcpy_d2h(_ctx, h.data(), _p, sizeof(double) * _n));
+    }
+
+private:
+    ibpm_ctx* _ctx;
+    size_t _n;
+    void* _p = nullptr;
+};
+
+// 2-D array of a single-level solve, indices (1..n1, 1..n2), the second
+// contiguous: the reference's Array2<double>(nx-1, ny-1, 1, 1).
+class Array2d : public std::vector<double> {
+public:
+    Array2d(int n1, int n2) : std::vector<double>((size_t)n1 * n2, 0.0), _n1(n1), _n2(n2) {}
+    double& operator()(int i, int j) { return (*this)[(size_t)(i - 1) * _n2 + (j - 1)]; }
+    double operator()(int i, int j) const { return (*this)[(size_t)(i - 1) * _n2 + (j - 1)]; }
+    Array2d& operator=(double v) { assign(size(), v); return *this; }
+    int n1() const { return _n1; }
+    int n2() const { return _n2; }
+
+private:
+    int _n1, _n2;
+};
+
+// Boundary values of one level, reference src/BC.h:22-79: the ring of
+// 2(nx+ny) values, left(j) j = 0..ny, top(i) i = 1..nx, right(j) from
+// j = ny-1 down, bottom(i) from i = nx-1 down (bottom(0) = left(0)).
+class BC : public std::vector<double> {
+public:
+    BC(int nx, int ny) : std::vector<double>(2 * (size_t)(nx + ny), 0.0), _nx(nx), _ny(ny) {}
+    BC& operator=(double v) { assign(size(), v); return *this; }
+    double& left(int j) { return (*this)[j]; }
+    double& top(int i) { return (*this)[_ny + i]; }
+    double& right(int j) { return (*this)[2 * _ny + _nx - j]; }
+    double& bottom(int i) { return i == 0 ? (*this)[0] : (*this)[2 * (_nx + _ny) - i]; }
+
+private:
+    int _nx, _ny;
+};
+
+// Nodal field on every level, reference src/Scalar.h (layout Scalar.cc:81-89).
+class Scalar : public std::vector<double> {
+public:
+    explicit Scalar(const Grid& g)
+        : std::vector<double>((size_t)g.Ngrid() * (g.Nx() - 1) * (g.Ny() - 1), 0.0), _grid(g) {}
+    double& operator()(int lev, int i, int j) { return (*this)[index(lev, i, j)]; }
+    double operator()(int lev, int i, int j) const { return (*this)[index(lev, i, j)]; }
+    Scalar& operator=(double v) { assign(size(), v); return *this; }
+    Scalar& operator+=(const Scalar& o) { for (size_t k = 0; k < size(); ++k) (*this)[k] += o[k]; return *this; }
+    Scalar& operator-=(const Scalar& o) { for (size_t k = 0; k < size(); ++k) (*this)[k] -= o[k]; return *this; }
+    Scalar& operator*=(double a) { for (double& v : *this) v *= a; return *this; }
+    const Grid& getGrid() const { return _grid; }
+    size_t index(int lev, int i, int j) const {
+        return ((size_t)lev * (_grid.Nx() - 1) + (i - 1)) * (_grid.Ny() - 1) + (j - 1);
+    }
+
+private:
+    Grid _grid;
+};
+inline Scalar operator-(const Scalar& a, const Scalar& b) { Scalar c = a; c -= b; return c; }
+inline Scalar operator*(double s, const Scalar& a) { Scalar c = a; c *= s; return c; }
+
+// Edge fluxes on every level, reference src/Flux.h (layout Flux.cc:42-50):
+// per level the X fluxes (i = 0..nx, j = 0..ny-1), then the Y fluxes
+// (i = 0..nx-1, j = 0..ny).
+class Flux : public std::vector<double> {
+public:
+    enum Direction { X = 0, Y = 1 };
+    explicit Flux(const Grid& g)
+        : std::vector<double>((size_t)g.Ngrid() * (2 * g.Nx() * g.Ny() + g.Nx() + g.Ny()), 0.0), _grid(g) {}
+    double& operator()(int dir, int lev, int i, int j) {
+        const int nx = _grid.Nx(), ny = _grid.Ny();
+        const size_t base = (size_t)lev * (2 * nx * ny + nx + ny);
+        return dir == X ? (*this)[base + (size_t)i * ny + j] : (*this)[base + (size_t)(nx + 1) * ny + (size_t)i * (ny + 1) + j];
+    }
+    Flux& operator=(double v) { assign(size(), v); return *this; }
+    const Grid& getGrid() const { return _grid; }
+
+private:
+    Grid _grid;
+};
+
+// Per-point 2-vectors, reference src/BoundaryVector.h: [X_0.., Y_0..].
+class BoundaryVector : public std::vector<double> {
+public:
+    enum Direction { X = 0, Y = 1 };
+    explicit BoundaryVector(int numPoints = 0) : std::vector<double>(2 * (size_t)numPoints, 0.0) {}
+    double& operator()(int dir, int i) { return (*this)[(size_t)dir * (size() / 2) + i]; }
+    double operator()(int dir, int i) const { return (*this)[(size_t)dir * (size() / 2) + i]; }
+    BoundaryVector& operator=(double v) { assign(size(), v); return *this; }
+    BoundaryVector& operator-=(const BoundaryVector& o) { for (size_t k = 0; k < size(); ++k) (*this)[k] -= o[k]; return *this; }
+    int getNumPoints() const { return (int)(size() / 2); }
+};
 
 class Geometry {   // reference src/Geometry.h (.geom text is parsed by the library)
 public:
@@ -94,11 +212,146 @@ public:
     }
     bool motionTransferred() const { return _transferred; }
     const std::string& text() const { return _text; }
+    // Geometry::moveBodies / getPoints / getVelocities (Geometry.cc:115-128):
+    // the points and velocities at the time of the last moveBodies call
+    // (motions evaluated at that time by the library).
+    void moveBodies(double time) { _time = time; }
+    double time() const { return _time; }
+    BoundaryVector getPoints() const { return pointsAt(false); }
+    BoundaryVector getVelocities() const { return pointsAt(true); }
 
 private:
+    BoundaryVector pointsAt(bool vel) const {
+        BoundaryVector xy(_n), uv(_n);
+        if (_n && ibpm_geom_points(_text.c_str(), 1, &_time, xy.data(), uv.data(), 2L * _n) != _n)
+            throw std::runtime_error("ibpm_gpu: the geometry does not parse");
+        return vel ? uv : xy;
+    }
     std::string _text;
     int _n = 0;
     bool _transferred = false;
+    double _time = 0.0;
+};
+
+// Multi-domain-free solves of one level, reference src/EllipticSolver2d.h:26-99:
+// solve(f, u) with zero boundary values, solve(f, bc, u) with the given ones.
+class EllipticSolver2d {
+public:
+    virtual ~EllipticSolver2d() = default;
+    void solve(const Array2d& f, Array2d& u) const { run(f, nullptr, u); }
+    void solve(const Array2d& f, const BC& bc, Array2d& u) const { run(f, &bc, u); }
+
+protected:
+    EllipticSolver2d(int nx, int ny, double dx, int kind, double alpha)
+        : _nx(nx), _ny(ny), _dx(dx), _kind(kind), _alpha(alpha) {}
+    int _nx, _ny;
+    double _dx;
+
+private:
+    void run(const Array2d& f, const BC* bc, Array2d& u) const {
+        ibpm_ctx* c = default_context();
+        DeviceBuffer df(f, c), du(u.size(), c);
+        std::unique_ptr<DeviceBuffer> db(bc ? new DeviceBuffer(*bc, c) : nullptr);
+        gpu_check(ibpm_elliptic2d_solve(c, _kind, _nx, _ny, _dx, _alpha, df.get(), db ? db->get() : nullptr,
+                                        du.get()));
+        du.to(u);
+    }
+    int _kind;
+    double _alpha;
+};
+
+// L u = f (reference PoissonSolver2d, EllipticSolver2d.h:66-77)
+class PoissonSolver2d : public EllipticSolver2d {
+public:
+    PoissonSolver2d(int nx, int ny, double dx) : EllipticSolver2d(nx, ny, dx, 0, 0.0) {}
+};
+
+// (1 + alpha L) u = f (reference HelmholtzSolver2d, EllipticSolver2d.h:85-97)
+class HelmholtzSolver2d : public EllipticSolver2d {
+public:
+    HelmholtzSolver2d(int nx, int ny, double dx, double alpha) : EllipticSolver2d(nx, ny, dx, 1, alpha) {}
+};
+
+// Multi-domain solve with zero boundary values on the outer domain,
+// reference src/EllipticSolver.h:26-76 (coarsify, coarsest level, finer
+// levels with boundary values from the coarser solution).
+class EllipticSolver {
+public:
+    virtual ~EllipticSolver() = default;
+    void solve(const Scalar& f, Scalar& u) const {
+        ibpm_ctx* c = default_context();
+        DeviceBuffer df(f, c), du(u.size(), c);
+        gpu_check(ibpm_elliptic_solve(c, _kind, _grid.Nx(), _grid.Ny(), _grid.Ngrid(), _grid.Dx(), _alpha,
+                                      df.get(), du.get()));
+        du.to(u);
+    }
+    Scalar solve(const Scalar& f) const {
+        Scalar u(_grid);
+        solve(f, u);
+        return u;
+    }
+
+protected:
+    EllipticSolver(const Grid& grid, int kind, double alpha) : _grid(grid), _kind(kind), _alpha(alpha) {}
+
+private:
+    Grid _grid;
+    int _kind;
+    double _alpha;
+};
+
+class PoissonSolver : public EllipticSolver {   // reference EllipticSolver.h:54-62
+public:
+    explicit PoissonSolver(const Grid& grid) : EllipticSolver(grid, 0, 0.0) {}
+};
+
+class HelmholtzSolver : public EllipticSolver {   // reference EllipticSolver.h:66-76
+public:
+    HelmholtzSolver(const Grid& grid, double alpha) : EllipticSolver(grid, 1, alpha) {}
+};
+
+// Regularizer of a geometry's points on the finest level, reference
+// src/Regularizer.h:50-91: update() for the geometry's current points,
+// toFlux = E^T (smear, times dx), toBoundary = E (interpolate, over dx).
+class Regularizer {
+public:
+    Regularizer(const Grid& grid, const Geometry& geometry) : _grid(grid), _geom(geometry) {}
+    ~Regularizer() {
+        if (_reg) ibpm_reg_destroy(_reg);
+    }
+    Regularizer(const Regularizer&) = delete;
+    Regularizer& operator=(const Regularizer&) = delete;
+    void update() {
+        const int n = _geom.getNumPoints();
+        if (n == 0) return;
+        BoundaryVector xy = _geom.getPoints();
+        if (!_reg)
+            gpu_check(ibpm_reg_create(default_context(), _grid.Nx(), _grid.Ny(), _grid.Ngrid(), _grid.length(),
+                                      _grid.xOffset(), _grid.yOffset(), n, xy.data(), &_reg));
+        else
+            gpu_check(ibpm_reg_update(_reg, xy.data()));
+    }
+    Flux toFlux(const BoundaryVector& u) const {
+        Flux q(_grid);
+        if (!_reg) return q;
+        DeviceBuffer du(u), dq(q.size());
+        gpu_check(ibpm_reg_to_flux(_reg, du.get(), dq.get()));
+        dq.to(q);
+        return q;
+    }
+    BoundaryVector toBoundary(const Flux& q) const {
+        BoundaryVector f(_geom.getNumPoints());
+        if (!_reg) return f;
+        DeviceBuffer dq(q), df(f.size());
+        gpu_check(ibpm_reg_to_boundary(_reg, dq.get(), df.get()));
+        df.to(f);
+        return f;
+    }
+
+private:
+    Grid _grid;
+    const Geometry& _geom;
+    ibpm_reg* _reg = nullptr;
 };
 
 class BaseFlow {   // reference src/BaseFlow.h: uniform flow of magnitude mag at angle alpha
@@ -126,9 +379,7 @@ struct Scheme {   // reference src/Scheme.h:12
 class State {      // reference src/State.h
 public:
     State(const Grid& grid, int numPoints)
-        : omega((size_t)grid.Ngrid() * (grid.Nx() - 1) * (grid.Ny() - 1)),
-          q((size_t)grid.Ngrid() * (2 * grid.Nx() * grid.Ny() + grid.Nx() + grid.Ny())),
-          f(2 * (size_t)numPoints), _dx(grid.Dx()), _grid(grid) {}
+        : omega(grid), q(grid), f(numPoints), _dx(grid.Dx()), _grid(grid) {}
     // Restart file in the reference layout, src/State.cc:65-223.
     bool save(const std::string& filename) const {
         return ibpm_state_save(filename.c_str(), _grid.Nx(), _grid.Ny(), _grid.Ngrid(), _grid.Dx(),
@@ -147,7 +398,9 @@ public:
         return ibpm_state_load(filename.c_str(), &nx, &ny, &ng, &dx, &x0, &y0, &np, q.data(), omega.data(),
                                f.data(), &timestep, &time) == 0;
     }
-    std::vector<double> omega, q, f;
+    Scalar omega;
+    Flux q;
+    BoundaryVector f;
     int timestep = 0;
     double time = 0.;
     // State::computeNetForce, reference src/State.cc:53-63
@@ -165,11 +418,68 @@ private:
     Grid _grid;
 };
 
-class NavierStokesModel {   // reference src/NavierStokesModel.h:40-106 (nonlinear model)
+// reference src/NavierStokesModel.h:40-106 (nonlinear model).  init() builds
+// the operators (regulariser of the geometry's points at the geometry's
+// time) over ibpm_model_*; the whole-step solver below does not need it.
+class NavierStokesModel {
 public:
     NavierStokesModel(const Grid& grid, const Geometry& geom, double Reynolds, const BaseFlow& q0)
         : _grid(grid), _geom(geom), _Re(Reynolds), _q0(q0) {}
-    void init() {}
+    NavierStokesModel(const Grid& grid, const Geometry& geom, double Reynolds)
+        : _grid(grid), _geom(geom), _Re(Reynolds), _q0(grid, 0.0, 0.0) {}
+    ~NavierStokesModel() {
+        if (_m) ibpm_model_destroy(_m);
+    }
+    NavierStokesModel(const NavierStokesModel&) = delete;
+    NavierStokesModel& operator=(const NavierStokesModel&) = delete;
+    void init() {
+        if (_m) return;
+        ibpm_flow_params p{};
+        p.nx = _grid.Nx(); p.ny = _grid.Ny(); p.ngrid = _grid.Ngrid();
+        p.length = _grid.length(); p.xoffset = _grid.xOffset(); p.yoffset = _grid.yOffset();
+        p.reynolds = _Re;
+        p.alpha_deg = _q0.getAlpha() * 180. / (4. * std::atan(1.));
+        p.geom = _geom.text().c_str();
+        gpu_check(ibpm_model_create(default_context(), &p, _geom.time(), &_m));
+        gpu_check(ibpm_model_set_base_flow(_m, _q0.getMag(), _q0.getAlpha()));
+    }
+    int getNumPoints() const { return _geom.getNumPoints(); }
+    // NavierStokesModel::updateOperators (NavierStokesModel.cc:93-99)
+    void updateOperators(double time) {
+        init();
+        if (geTimeDependent()) gpu_check(ibpm_model_update_operators(_m, time));
+    }
+    // omega = B(f) = Curl(E^T f) (NavierStokesModel.cc:101-105)
+    void B(const BoundaryVector& f, Scalar& omega) const {
+        DeviceBuffer df(f), dw(omega.size());
+        gpu_check(ibpm_model_B(handle(), df.get(), dw.get()));
+        dw.to(omega);
+    }
+    // f = C(omega) = E Curl(psi), L psi = -omega (NavierStokesModel.cc:107-112)
+    void C(const Scalar& omega, BoundaryVector& f) const {
+        DeviceBuffer dw(omega), df(f.size());
+        gpu_check(ibpm_model_C(handle(), dw.get(), df.get()));
+        df.to(f);
+    }
+    // q = Curl(psi) + q0 (NavierStokesModel.cc:121-125)
+    void computeFlux(const Scalar& omega, Flux& q) const {
+        DeviceBuffer dw(omega), dq(q.size());
+        gpu_check(ibpm_model_compute_flux(handle(), dw.get(), dq.get()));
+        dq.to(q);
+    }
+    void refreshState(State& x) const { computeFlux(x.omega, x.q); }   // NavierStokesModel.cc:127-129
+    // L psi = -omega with zero boundary values (NavierStokesModel.cc:133-140)
+    Scalar vorticityToStreamfunction(const Scalar& omega) const {
+        Scalar psi = -1. * omega;
+        PoissonSolver(_grid).solve(psi, psi);
+        return psi;
+    }
+    // body velocities minus the base flow at the points (NavierStokesModel.cc:86-91)
+    BoundaryVector getConstraints() const {
+        BoundaryVector b(getNumPoints());
+        gpu_check(ibpm_model_constraints(handle(), b.data()));
+        return b;
+    }
     double getAlpha() const { return 1. / _Re; }
     bool geTimeDependent() const { return !_geom.isStationary(); }          // NavierStokesModel.cc:65-69
     bool bfTimeDependent() const { return !_q0.isStationary(); }            // NavierStokesModel.cc:71-75
@@ -179,11 +489,137 @@ public:
     const BaseFlow& getBaseFlow() const { return _q0; }
     double Reynolds() const { return _Re; }
 
+    ibpm_model* handle() const {
+        if (!_m) throw std::runtime_error("ibpm_gpu: NavierStokesModel::init() not called");
+        return _m;
+    }
+
 private:
     const Grid& _grid;
     const Geometry& _geom;
     double _Re;
     BaseFlow _q0;
+    ibpm_model* _m = nullptr;
+};
+
+// Projection step of the fractional-step method, reference
+// src/ProjectionSolver.h:54-126 and ProjectionSolver.cc:57-101:
+//   A omega* = a,  M f = C omega* - b,  omega = omega* - A^-1 B f,
+// with A = 1 - beta/(2 Re) L, B = beta model.B, C = model.C and
+// M = C A^-1 B.  Minv is the subclass's (Cholesky or CG).
+class ProjectionSolver {
+public:
+    ProjectionSolver(const Grid& grid, const NavierStokesModel& model, double beta)
+        : _beta(beta), _grid(grid), _model(model), _helmholtz(grid, -beta / 2. * model.getAlpha()) {}
+    virtual ~ProjectionSolver() = default;
+    virtual void init() {}
+    virtual bool save(const std::string&) { return false; }
+    virtual bool load(const std::string&) { return false; }
+    void solve(const Scalar& a, const BoundaryVector& b, Scalar& omega, BoundaryVector& f) {
+        Scalar omegaStar(a.getGrid());
+        Ainv(a, omegaStar);
+        BoundaryVector rhs(f.getNumPoints());
+        C(omegaStar, rhs);
+        rhs -= b;
+        Minv(rhs, f);
+        Scalar c(a.getGrid());
+        B(f, c);
+        Ainv(c, c);
+        omega = omegaStar - c;
+    }
+
+protected:
+    void Ainv(const Scalar& x, Scalar& y) { _helmholtz.solve(x, y); }
+    void B(const BoundaryVector& f, Scalar& y) {
+        _model.B(f, y);
+        y *= _beta;
+    }
+    void C(const Scalar& x, BoundaryVector& y) { _model.C(x, y); }
+    // y = M f on the GPU in one call (ibpm_model_M: the same B, A^-1, C)
+    void M(const BoundaryVector& f, BoundaryVector& y) {
+        if (f.empty()) return;
+        DeviceBuffer df(f), dy(y.size());
+        gpu_check(ibpm_model_M(_model.handle(), _beta, df.get(), dy.get()));
+        dy.to(y);
+    }
+    BoundaryVector M(const BoundaryVector& f) {
+        BoundaryVector y(f.getNumPoints());
+        M(f, y);
+        return y;
+    }
+    virtual void Minv(const BoundaryVector& b, BoundaryVector& x) = 0;
+    double beta() const { return _beta; }
+    const NavierStokesModel& model() const { return _model; }
+
+private:
+    double _beta;
+    Grid _grid;
+    const NavierStokesModel& _model;
+    HelmholtzSolver _helmholtz;
+};
+
+// M^-1 by the FP64 Cholesky factor of M, built column by column on the GPU
+// (reference src/CholeskySolver.h:36-78, CholeskySolver.cc:49-224); the
+// factor is saved to and loaded from <basename>.cholesky in the reference's
+// layout.
+class CholeskySolver : public ProjectionSolver {
+public:
+    CholeskySolver(const Grid& grid, const NavierStokesModel& model, double beta)
+        : ProjectionSolver(grid, model, beta), _size(2 * model.getNumPoints()),
+          _alphaBeta(model.getAlpha() * beta), _lower((size_t)_size * _size), _diag(_size) {}
+    void init() override {
+        if (_init) return;
+        std::vector<double> Mmat((size_t)_size * _size);
+        if (_size) gpu_check(ibpm_cholesky_build(model().handle(), beta(), Mmat.data()));
+        gpu_check(ibpm_cholesky_factor(_size, Mmat.data(), _lower.data(), _diag.data()));
+        _init = true;
+    }
+    bool load(const std::string& basename) override {
+        if (ibpm_cholesky_load((basename + ".cholesky").c_str(), _size, _alphaBeta, _lower.data(), _diag.data()) != 0)
+            return false;
+        _init = true;
+        return true;
+    }
+    bool save(const std::string& basename) override {
+        return _init && ibpm_cholesky_save((basename + ".cholesky").c_str(), _size, _alphaBeta, _lower.data(),
+                                           _diag.data()) == 0;
+    }
+
+protected:
+    void Minv(const BoundaryVector& b, BoundaryVector& x) override {
+        if (!_init) throw std::runtime_error("ibpm_gpu: CholeskySolver::init() not called");
+        gpu_check(ibpm_cholesky_solve(_size, _lower.data(), _diag.data(), b.data(), x.data()));
+    }
+
+private:
+    int _size;
+    double _alphaBeta;
+    std::vector<double> _lower, _diag;
+    bool _init = false;
+};
+
+// M^-1 by conjugate gradients from the previous x, absolute tolerance on
+// the residual (reference src/ConjugateGradientSolver.h:28-57,
+// ConjugateGradientSolver.cc:35-65), each M application on the GPU.
+class ConjugateGradientSolver : public ProjectionSolver {
+public:
+    ConjugateGradientSolver(const Grid& grid, const NavierStokesModel& model, double beta, double tolerance)
+        : ProjectionSolver(grid, model, beta), _tol(tolerance) {}
+    void setTolerance(double tolerance) { _tol = tolerance; }
+    double getTolerance() const { return _tol; }
+    int lastIterations() const { return _its; }
+
+protected:
+    void Minv(const BoundaryVector& b, BoundaryVector& x) override {
+        int conv = 0;
+        if (b.empty()) { _its = 0; return; }
+        BoundaryVector x0 = x;
+        gpu_check(ibpm_cg_solve(model().handle(), beta(), _tol, 0, b.data(), x0.data(), x.data(), &_its, &conv));
+    }
+
+private:
+    double _tol;
+    int _its = 0;
 };
 
 // IBSolver with the nonlinear term (reference src/IBSolver.h:20-90).  Cholesky

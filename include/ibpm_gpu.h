@@ -158,6 +158,65 @@ typedef struct {
                                    body frame; more than one body is an error */
 } ibpm_flow_params;
 
+/* ---- NavierStokesModel / ProjectionSolver operators of one flow -------------
+   An operator-level interface over the stand-alone entry points above (one
+   flow, synchronous calls); the batch below is the fast path for stepping.
+   Device arrays: Scalar (ns doubles), Flux, BoundaryVector (2 Nb). */
+typedef struct ibpm_model ibpm_model;
+/* NavierStokesModel(grid, geometry, Re, BaseFlow(grid, 1, alpha)) + init():
+   p gives the grid, Reynolds number, base-flow angle (alpha_deg) and .geom
+   text (other fields unused); the regulariser is built for the points at
+   time t0.  Reference src/NavierStokesModel.h:40-56, NavierStokesModel.cc:19-57. */
+int ibpm_model_create(ibpm_ctx* ctx, const ibpm_flow_params* p, double t0, ibpm_model** out);
+int ibpm_model_destroy(ibpm_model* m);
+/* NavierStokesModel::getNumPoints */
+int ibpm_model_num_points(ibpm_model* m);
+/* The uniform base flow q0 of magnitude mag at angle alpha (radians):
+   BaseFlow(grid, mag, alpha) (BaseFlow.h, UniformFlow Flux.cc:79-99);
+   ibpm_model_create sets magnitude 1 at p->alpha_deg, mag 0 is the
+   model without a potential-flow part (NavierStokesModel.h:48-52). */
+int ibpm_model_set_base_flow(ibpm_model* m, double mag, double alpha);
+/* NavierStokesModel::updateOperators(time) (NavierStokesModel.cc:93-99): the
+   bodies move to `time` (the positions of one Geometry::moveBodies call at
+   that time) and the regulariser is updated. */
+int ibpm_model_update_operators(ibpm_model* m, double time);
+/* NavierStokesModel::B: omega = Curl(E^T f)  (NavierStokesModel.cc:101-105) */
+int ibpm_model_B(ibpm_model* m, const double* d_f, double* d_omega);
+/* NavierStokesModel::C: f = E Curl(psi), L psi = -coarsify(omega)  (:107-119, 133-140) */
+int ibpm_model_C(ibpm_model* m, const double* d_omega, double* d_f);
+/* NavierStokesModel::computeFlux: q = Curl(psi) + q0  (:121-125) */
+int ibpm_model_compute_flux(ibpm_model* m, const double* d_omega, double* d_q);
+/* NavierStokesModel::getConstraints: h_b = body velocities - E q0 (host
+   BoundaryVector, :86-91, 142-146) */
+int ibpm_model_constraints(ibpm_model* m, double* h_b);
+/* ProjectionSolver::M: y = C A^-1 (beta B f), A = 1 - beta/(2 Re) L
+   (ProjectionSolver.cc:85-101, the Helmholtz alpha of :33) */
+int ibpm_model_M(ibpm_model* m, double beta, const double* d_f, double* d_y);
+/* CholeskySolver::computeMatrixM (CholeskySolver.cc:63-79): the n x n
+   (n = 2 Nb) matrix M, row-major, one GPU M application per column. */
+int ibpm_cholesky_build(ibpm_model* m, double beta, double* h_M);
+/* CholeskySolver::computeFactorization (CholeskySolver.cc:89-109), FP64: the
+   factor L of M from M's upper part; h_lower n x n row-major with the strict
+   lower triangle (zeros above), h_diag its diagonal.  -2 if M is not
+   positive definite. */
+int ibpm_cholesky_factor(int n, const double* h_M, double* h_lower, double* h_diag);
+/* CholeskySolver::Minv (CholeskySolver.cc:200-224), FP64 forward and back
+   substitution with the reference's summation order; h_x may alias h_b. */
+int ibpm_cholesky_solve(int n, const double* h_lower, const double* h_diag, const double* h_b, double* h_x);
+/* CholeskySolver::save / load (CholeskySolver.cc:113-197): the factor in the
+   reference's .cholesky text layout (size, alpha*beta, diagonal, '#', strict
+   lower triangle by rows).  load returns -2 when the file is missing or its
+   size or alpha*beta differ (the reference's "false"). */
+int ibpm_cholesky_save(const char* path, int n, double alpha_beta, const double* h_lower, const double* h_diag);
+int ibpm_cholesky_load(const char* path, int n, double alpha_beta, double* h_lower, double* h_diag);
+/* ConjugateGradientSolver::Minv (ConjugateGradientSolver.cc:35-65): CG on M
+   from h_x0 (NULL: zero) with FP64 vectors and residual, each M application
+   on the GPU; stops when r.r <= tol^2 (absolute) or after max_it iterations
+   (max_it <= 0: no cap, as the reference).  Writes the iteration count and
+   whether the test was met. */
+int ibpm_cg_solve(ibpm_model* m, double beta, double tol, int max_it, const double* h_b, const double* h_x0,
+                  double* h_x, int* it_count, int* converged);
+
 /* A batch of independent flows advanced together by the same kernels. */
 int ibpm_batch_create(ibpm_ctx* ctx, int nflows, const ibpm_flow_params* params, ibpm_batch** out);
 int ibpm_batch_destroy(ibpm_batch* b);

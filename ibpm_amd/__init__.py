@@ -20,6 +20,7 @@ _I = ctypes.POINTER(ctypes.c_int)
 _lib = None
 
 SCHEMES = {"euler": 0, "ab2": 1, "rk3": 2, "rk3b": 3}
+MODELS = {"nonlinear": 0, "linear": 1, "adjoint": 2, "linearperiodic": 3, "sfd": 4}
 WARM = {"zero": 0, "previous": 1, "learned": 2}
 NN_IN, NN_HID = 13, 32
 NN_PARAMS = NN_HID * NN_IN + NN_HID + 2 * NN_HID + 2
@@ -34,7 +35,8 @@ class FlowParams(ctypes.Structure):
                 ("length", ctypes.c_double), ("xoffset", ctypes.c_double), ("yoffset", ctypes.c_double),
                 ("reynolds", ctypes.c_double), ("dt", ctypes.c_double), ("scheme", ctypes.c_int),
                 ("alpha_deg", ctypes.c_double), ("cg_tol", ctypes.c_double), ("force_cg", ctypes.c_int),
-                ("geom", ctypes.c_char_p), ("ubf", ctypes.c_int)]
+                ("geom", ctypes.c_char_p), ("ubf", ctypes.c_int), ("model", ctypes.c_int),
+                ("period", ctypes.c_int), ("sfd_chi", ctypes.c_double), ("sfd_delta", ctypes.c_double)]
 
 
 def load():
@@ -85,6 +87,9 @@ def load():
         "ibpm_geom_points": [ctypes.c_char_p, ctypes.c_int, vp, vp, vp, ctypes.c_long],
         "ibpm_batch_cg_iters": [vp, ctypes.c_int, vp, ctypes.c_int],
         "ibpm_batch_use_graph": [vp, ctypes.c_int],
+        "ibpm_batch_set_base_state": [vp, ctypes.c_int, ctypes.c_int, vp, vp],
+        "ibpm_batch_get_filtered": [vp, ctypes.c_int, vp],
+        "ibpm_batch_set_filtered": [vp, ctypes.c_int, vp],
         "ibpm_batch_kernel_stats": [vp, ctypes.c_int, vp, vp, vp, ctypes.c_int],
         "ibpm_batch_set_warm_start": [vp, ctypes.c_int],
         "ibpm_batch_set_state": [vp, ctypes.c_int, vp, vp, ctypes.c_double, ctypes.c_int],
@@ -333,11 +338,16 @@ class Flow:
     tol: float = 1e-7
     force_cg: bool = False
     ubf: bool = False          # unsteady base flow (ibpm -ubf): solve in the frame of the only body
+    model: str = "nonlinear"   # ibpm -model: nonlinear, linear, adjoint, linearperiodic, sfd
+    period: int = 1            # linearperiodic: number of base states
+    chi: float = 0.02          # sfd gain (ibpm -chi)
+    Delta: float = 15.0        # sfd filter width (ibpm -Delta)
 
     def params(self):
         return FlowParams(self.nx, self.ny, self.ngrid, self.length, self.xoffset, self.yoffset,
                           self.Re, self.dt, SCHEMES[self.scheme], self.alpha, self.tol,
-                          int(self.force_cg), self.geom.encode(), int(self.ubf))
+                          int(self.force_cg), self.geom.encode(), int(self.ubf), MODELS[self.model],
+                          self.period, self.chi, self.Delta)
 
 
 class Batch:
@@ -371,6 +381,24 @@ class Batch:
 
     def cholesky_loaded(self, k=0):
         return _check(load().ibpm_batch_cholesky_loaded(self.h, k))
+
+    def set_base_state(self, omega, q, phase=0, k=0):
+        """Base state (vorticity, flux) of a linear model's flow k: x0 of the
+        linear and adjoint models, phase `phase` of the linear periodic one.
+        Before init()."""
+        omega, q = _c(omega), _c(q)
+        _check(load().ibpm_batch_set_base_state(self.h, k, phase, omega.ctypes.data, q.ctypes.data))
+        return self
+
+    def filtered(self, k=0):
+        """SFD: the filtered vorticity."""
+        f = self.flows[k]
+        return self._get("ibpm_batch_get_filtered", k, f.ngrid * (f.nx - 1) * (f.ny - 1))
+
+    def set_filtered(self, w, k=0):
+        w = _c(w)
+        _check(load().ibpm_batch_set_filtered(self.h, k, w.ctypes.data))
+        return self
 
     def use_graph(self, on=True):
         _check(load().ibpm_batch_use_graph(self.h, int(on)))

@@ -479,6 +479,12 @@ struct Flow {
     // device fields
     double *omega, *q, *nprev, *a, *wstar, *psi, *c, *U, *V, *cross, *R, *T1, *T2, *mt, *cosx, *cosy;
     double *f, *rhs, *b, *cg_r, *cg_d, *cg_q, *cg_state, *hist, *unit, *mcol;
+    // models: base states (linear models; one per phase of the periodic one),
+    // the adjoint's Flux x Flux cross product, SFD's filtered state
+    std::vector<double*> bq, bw;
+    std::vector<bool> base_set;
+    double *sff = nullptr, *what = nullptr, *sfd_prev = nullptr;
+    double base_mag = 1.0;        // magnitude of the uniform base flow (0: no potential flow)
     int* hist_count;
     double* minv[4] = {nullptr, nullptr, nullptr, nullptr};
     // preconditioned CG: dense P, its input/output, the Krylov pairs of the
@@ -533,6 +539,12 @@ struct ibpm_batch {
     int nsub = 3;
     double an[4], bn[4], cn[4];
     bool all_stationary = true, any_cg = false, initialized = false, any_ubf = false;
+    bool any_periodic = false, any_adjoint = false;
+    ibpmg::CrossFFOps* d_cff = nullptr;                // adjoint flows' Flux x Flux cross products
+    int cff_n = 0;
+    std::vector<ibpmg::CoarsifyJob*> d_cffc;          // their coarsify, per level
+    std::vector<int> cffc_n;
+    int cffc_rows = 0;                                 // coarse rows of the largest of them
     bool old_saved = false;
     double time = 0.0;
     int timestep = 0;
@@ -661,7 +673,7 @@ static int upload_constraint(hipStream_t s, Flow& F) {
     const double al = F.p.alpha_deg * pi / 180.;
     double sal, cal;
     sin_cos(al, sal, cal);
-    const double u0 = 1 * cal * F.dx, v0 = 1 * sal * F.dx;
+    const double u0 = F.base_mag * cal * F.dx, v0 = F.base_mag * sal * F.dx;
     const long nfx = (long)(F.nx + 1) * F.ny;
     auto q0 = [&](int fidx) {
         if (!F.ubf) return fidx < nfx ? u0 : v0;
@@ -824,6 +836,21 @@ static int alloc_flow(ibpm_batch* B, Flow& F) {
         CK(hipHostMalloc((void**)&F.stage.p, bytes, hipHostMallocDefault));
         F.stage.cap = bytes;
     }
+    if (F.p.model >= 1 && F.p.model <= 3) {   // base states
+        const int np = F.p.model == 3 ? F.p.period : 1;
+        F.bq.assign(np, nullptr);
+        F.bw.assign(np, nullptr);
+        F.base_set.assign(np, false);
+        for (int k = 0; k < np; ++k) {
+            CKR(m.alloc(&F.bq[k], nf));
+            CKR(m.alloc(&F.bw[k], ns));
+        }
+    }
+    if (F.p.model == 2) CKR(m.alloc(&F.sff, ns));
+    if (F.p.model == 4) {
+        CKR(m.alloc(&F.what, ns));
+        CKR(m.alloc(&F.sfd_prev, ns));
+    }
     {   // learned warm start history (small; allocated for every flow)
         const int R = 2 * B->nsub + 2;
         CKR(m.alloc(&F.ring_f, (size_t)R * n)); CKR(m.alloc(&F.ring_b, (size_t)R * n));
@@ -892,12 +919,20 @@ static int build_plans(ibpm_batch* B) {
             const double al = F.p.alpha_deg * pi / 180.;
             double sal, cal;
             sin_cos(al, sal, cal);
-            o.ubase = 1 * cal;
-            o.vbase = 1 * sal;
+            o.ubase = F.base_mag * cal;
+            o.vbase = F.base_mag * sal;
             o.ubf = F.ubf ? 1 : 0;
             o.bf = F.bf_dev;
             o.xc = F.bf.xc; o.yc = F.bf.yc;
             o.xoff = F.p.xoffset; o.yoff = F.p.yoffset;
+            o.model = F.p.model;
+            o.q0b = F.bq.empty() ? nullptr : F.bq[0];
+            o.w0b = F.bw.empty() ? nullptr : F.bw[0];
+            o.sff = F.sff;
+            o.what = F.what;
+            o.sfd_prev = F.sfd_prev;
+            o.sfd_chi = F.p.sfd_chi;
+            o.sfd_inv_delta = F.p.model == 4 ? 1.0 / F.p.sfd_delta : 0.0;   // Array.h:335-340
             B->h_fops[i].push_back(o);
         }
         CKR(B->mem.alloc(&B->d_fops[i], nfl));
@@ -927,7 +962,7 @@ static void mark_nprev_valid(ibpm_batch* B) {
     if (B->old_saved) return;
     B->old_saved = true;
     for (int i = 0; i < B->nsub; ++i) {
-        for (auto& o : B->h_fops[i]) o.nprev_valid = 1;
+        for (auto& o : B->h_fops[i]) o.nprev_valid = o.sfd_valid = 1;
         (void)hipMemcpy(B->d_fops[i], B->h_fops[i].data(), sizeof(FlowOps) * B->h_fops[i].size(),
                         hipMemcpyHostToDevice);
     }
@@ -1372,6 +1407,11 @@ static void enqueue_rhs(ibpm_batch* B, int i, hipStream_t s) {
     int lmax = 0;
     for (auto& F : B->flows) lmax = std::max(lmax, F.L);
     Probe* pr = B->probe;
+    if (B->cff_n) {   // adjoint flows: s = CrossProduct(q0, q), coarsified (VectorOperations.cc:465-486)
+        launch_cross_ff(B->d_cff, B->cff_n, B->max_ns_all, s);
+        for (size_t lev = 1; lev < B->d_cffc.size(); ++lev)
+            if (B->cffc_n[lev]) launch_coarsify(B->d_cffc[lev], B->cffc_n[lev], B->cffc_rows, s);
+    }
     timed(pr, K_VEL, B->bytes_vel, s, [&] { launch_vel(B->d_fops[i], nfl, B->max_ns_all, s); });
     for (int lev = 0; lev < lmax; ++lev)
         timed(pr, K_CROSS, B->bytes_cross / lmax, s, [&] { launch_cross(B->d_fops[i], nfl, B->max_nf, lev, s); });
@@ -2806,6 +2846,18 @@ int ibpm_batch_create(ibpm_ctx* ctx, int nflows, const ibpm_flow_params* params,
                 B->any_ubf = true;
             }
         }
+        if (F.p.model < 0 || F.p.model > 4) { set_error("flow %d: model %d is not 0..4", k, F.p.model); return -1; }
+        if (F.p.model == 3 && (F.p.period < 1 || F.p.period > 4096)) {
+            set_error("flow %d: linear periodic model needs a period of 1..4096 steps", k);
+            return -1;
+        }
+        if (F.p.model == 4 && !(F.p.sfd_delta > 0)) { set_error("flow %d: SFD needs Delta > 0", k); return -1; }
+        if (F.p.model >= 1 && F.p.model <= 3) {   // NavierStokesModel(grid, geom, Re): no potential flow
+            if (F.p.ubf) { set_error("flow %d: the linear models take no unsteady base flow", k); return -1; }
+            F.base_mag = 0.0;
+        }
+        if (F.p.model == 3) B->any_periodic = true;
+        if (F.p.model == 2) B->any_adjoint = true;
         F.cg = !F.geom.stationary || F.p.force_cg;
         if (F.cg) B->any_cg = true;
         if (!F.geom.stationary) B->all_stationary = false;
@@ -2835,6 +2887,34 @@ int ibpm_batch_create(ibpm_ctx* ctx, int nflows, const ibpm_flow_params* params,
     B->max_comp = std::max(B->max_comp, 1);
     B->max_n = std::max(B->max_n, 1);
     CKR(build_plans(B.get()));
+    if (B->any_adjoint) {   // the adjoint N's Flux x Flux cross product and its in-place coarsify
+        std::vector<CrossFFOps> ops;
+        int lmax = 0;
+        for (auto& F : B->flows)
+            if (F.p.model == 2) {
+                ops.push_back(CrossFFOps{F.nx, F.ny, F.L, F.dx, F.bq[0], F.q, F.sff});
+                lmax = std::max(lmax, F.L);
+            }
+        B->cff_n = (int)ops.size();
+        CKR(B->mem.alloc(&B->d_cff, ops.size()));
+        CKR(upload(B->d_cff, ops.data(), sizeof(CrossFFOps) * ops.size()));
+        B->d_cffc.assign(lmax, nullptr);
+        B->cffc_n.assign(lmax, 0);
+        int rows = 0;
+        for (int lev = 1; lev < lmax; ++lev) {
+            std::vector<CoarsifyJob> jobs;
+            for (auto& F : B->flows)
+                if (F.p.model == 2 && lev < F.L) {
+                    jobs.push_back(CoarsifyJob{F.nx, F.ny, lev, F.sff, F.sff, nullptr});
+                    rows = std::max(rows, F.nx / 2 - 1);
+                }
+            B->cffc_n[lev] = (int)jobs.size();
+            if (jobs.empty()) continue;
+            CKR(B->mem.alloc(&B->d_cffc[lev], jobs.size()));
+            CKR(upload(B->d_cffc[lev], jobs.data(), sizeof(CoarsifyJob) * jobs.size()));
+        }
+        B->cffc_rows = rows;
+    }
     if (B->any_cg) {
         ibpm_batch* b = B.get();
         b->ring = 2 * b->nsub + 2;
@@ -2868,6 +2948,12 @@ int ibpm_batch_destroy(ibpm_batch* b) {
 
 int ibpm_batch_init(ibpm_batch* B) {
     hipStream_t s = B->ctx->c.stream;
+    for (size_t k = 0; k < B->flows.size(); ++k)
+        for (size_t ph = 0; ph < B->flows[k].base_set.size(); ++ph)
+            if (!B->flows[k].base_set[ph]) {
+                set_error("flow %zu: base state %zu of the linear model not set (ibpm_batch_set_base_state)", k, ph);
+                return -1;
+            }
     for (auto& F : B->flows) {
         F.geom.move(B->time, F.px, F.py, F.pu, F.pv);   // ibpm.cc:285
         CKR(update_regularizer(B, F));                   // model->init()
@@ -2883,10 +2969,30 @@ int ibpm_batch_init(ibpm_batch* B) {
     return 0;
 }
 
+// LinearizedPeriodicIBSolver::N (IBSolver.cc:224-231): the base state of phase
+// timestep % period for every substep of the coming step.
+static int select_phase(ibpm_batch* B) {
+    for (int i = 0; i < B->nsub; ++i) {
+        bool changed = false;
+        for (size_t k = 0; k < B->flows.size(); ++k) {
+            Flow& F = B->flows[k];
+            if (F.p.model != 3) continue;
+            const int ph = B->timestep % F.p.period;
+            FlowOps& o = B->h_fops[i][k];
+            changed = changed || o.q0b != F.bq[ph];
+            o.q0b = F.bq[ph];
+            o.w0b = F.bw[ph];
+        }
+        if (changed) CKR(upload(B->d_fops[i], B->h_fops[i].data(), sizeof(FlowOps) * B->h_fops[i].size()));
+    }
+    return 0;
+}
+
 int ibpm_batch_advance(ibpm_batch* B, int nsteps) {
     if (!B->initialized) { set_error("ibpm_batch_init not called"); return -1; }
     hipStream_t s = B->ctx->c.stream;
     for (int step = 0; step < nsteps; ++step) {
+        if (B->any_periodic) CKR(select_phase(B));
         for (auto& F : B->flows)
             if (F.hist_cap <= B->timestep) { set_error("force history capacity exceeded"); return -1; }
         if (B->any_cg) {
@@ -2895,7 +3001,7 @@ int ibpm_batch_advance(ibpm_batch* B, int nsteps) {
                 if (i == 0) mark_nprev_valid(B);
             }
             timed(B->probe, K_FORCES, 0, s, [&] { launch_forces(B->d_force, (int)B->flows.size(), s); });
-        } else if (B->use_graph && B->old_saved && !B->probe && !B->any_ubf) {
+        } else if (B->use_graph && B->old_saved && !B->probe && !B->any_ubf && !B->any_periodic) {
             if (!B->graph) {
                 hipGraph_t g;
                 CK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -2941,6 +3047,46 @@ static Flow* flow_of(ibpm_batch* B, int k) {
     if (!B || k < 0 || k >= (int)B->flows.size()) { set_error("bad flow index %d", k); return nullptr; }
     return &B->flows[k];
 }
+
+int ibpm_batch_set_base_state(ibpm_batch* B, int k, int ph, const double* h_omega, const double* h_q) {
+    Flow* F = flow_of(B, k);
+    if (!F) return -1;
+    if (ph < 0 || ph >= (int)F->bq.size()) {
+        set_error("flow %d: no base state %d (model %d, period %d)", k, ph, F->p.model, F->p.period);
+        return -1;
+    }
+    CKR(upload(F->bw[ph], h_omega, sizeof(double) * F->ns * F->L));
+    CKR(upload(F->bq[ph], h_q, sizeof(double) * F->nf * F->L));
+    F->base_set[ph] = true;
+    return 0;
+}
+
+int ibpm_batch_get_filtered(ibpm_batch* B, int k, double* h) {
+    Flow* F = flow_of(B, k);
+    if (!F) return -1;
+    if (!F->what) { set_error("flow %d is not an SFD flow", k); return -1; }
+    CK(hipStreamSynchronize(B->ctx->c.stream));
+    CK(hipMemcpy(h, F->what, sizeof(double) * F->ns * F->L, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int ibpm_batch_set_filtered(ibpm_batch* B, int k, const double* h) {
+    Flow* F = flow_of(B, k);
+    if (!F) return -1;
+    if (!F->what) { set_error("flow %d is not an SFD flow", k); return -1; }
+    CK(hipStreamSynchronize(B->ctx->c.stream));
+    CKR(upload(F->what, h, sizeof(double) * F->ns * F->L));
+    for (int i = 0; i < B->nsub; ++i) {   // the filtered state is set (SFDSolver::_xhatSaved)
+        B->h_fops[i][k].sfd_valid = 1;
+        CKR(upload(B->d_fops[i], B->h_fops[i].data(), sizeof(FlowOps) * B->h_fops[i].size()));
+    }
+    if (B->graph) {
+        CK(hipGraphExecDestroy(B->graph));
+        B->graph = nullptr;
+    }
+    return 0;
+}
+
 
 int ibpm_batch_num_points(ibpm_batch* B, int k) {
     Flow* F = flow_of(B, k);

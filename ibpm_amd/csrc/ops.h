@@ -20,6 +20,15 @@ struct FlowOps {
     int ubf;
     const double* bf;
     double xc, yc, xoff, yoff;     // centre of rotation; level-0 grid offsets
+    // model (IBSolver.cc:203-246): 0 nonlinear, 1 linear, 2 adjoint, 3 linear
+    // periodic, 4 SFD.  q0b, w0b: base state of the linear models; sff: the
+    // adjoint's CrossProduct(q0, q), coarsified; what, sfd_prev: SFD's
+    // filtered vorticity and its previous right-hand side (updated in place)
+    int model;
+    const double *q0b, *w0b, *sff;
+    double *what, *sfd_prev;
+    double sfd_chi, sfd_inv_delta;
+    int sfd_valid;                 // what holds the filtered state (else: zero in N, omega at the update)
 };
 
 // CrossProduct(Flux, Flux) of one flow before its coarsify: out = X(q1) Y(q2)

@@ -195,11 +195,39 @@ __global__ void __launch_bounds__(kThreads) k_vel(const FlowOps* __restrict__ fl
     const unsigned r = (unsigned)t - lev * ns;
     const int i = (int)(r / ny1) + 1, j = (int)(r % ny1) + 1;
     const double w = gin(F.omega)[t];
-    double u = xvel(g, F.q, F.dx, lev, i, j);
-    u *= w;
-    u *= -1;
-    double v = yvel(g, F.q, F.dx, lev, i, j);
-    v *= w;
+    double u, v;
+    if (F.model == 1 || F.model == 3) {
+        // LinearizedIBSolver::N (IBSolver.cc:211-216): the velocities of
+        // CrossProduct(q0, omega) + CrossProduct(q, omega0), summed per node
+        const double w0 = gin(F.w0b)[t];
+        double u1 = xvel(g, F.q0b, F.dx, lev, i, j);
+        u1 *= w;
+        u1 *= -1;
+        double u2 = xvel(g, F.q, F.dx, lev, i, j);
+        u2 *= w0;
+        u2 *= -1;
+        double v1 = yvel(g, F.q0b, F.dx, lev, i, j);
+        v1 *= w;
+        double v2 = yvel(g, F.q, F.dx, lev, i, j);
+        v2 *= w0;
+        u = u1 + u2;
+        v = v1 + v2;
+    } else if (F.model == 2) {
+        // AdjointIBSolver::N (IBSolver.cc:218-222): -CrossProduct(q, omega0),
+        // so that k_rhs's Curl(cross) is the subtracted term
+        const double w0 = gin(F.w0b)[t];
+        u = xvel(g, F.q, F.dx, lev, i, j);
+        u *= w0;
+        v = yvel(g, F.q, F.dx, lev, i, j);
+        v *= w0;
+        v *= -1;
+    } else {
+        u = xvel(g, F.q, F.dx, lev, i, j);
+        u *= w;
+        u *= -1;
+        v = yvel(g, F.q, F.dx, lev, i, j);
+        v *= w;
+    }
     gout(F.U)[t] = u;
     gout(F.V)[t] = v;
 }
@@ -327,7 +355,22 @@ __global__ void __launch_bounds__(kThreads) k_rhs(const FlowOps* __restrict__ fl
     const double cx0 = C[g.fx(lev, i, j - 1)], cx1 = C[g.fx(lev, i, j)];
     const double c = Wg[tu], e = Wg[te], w = Wg[tw], up = Wg[tn], dn = Wg[ts];
     const double np = gin(F.nprev)[tu];
-    const double n = (cy1 - cy0 + cx0 - cx1) * bydx2;
+    double n = (cy1 - cy0 + cx0 - cx1) * bydx2;
+    if (F.model == 2) {   // AdjointIBSolver::N = Laplacian(s) - Curl(CrossProduct(q, omega0))
+        const IBPM_GLOBAL double* Sg = gin(F.sff);
+        const double sc = Sg[tu], se = Sg[te], sw = Sg[tw], su = Sg[tn], sd = Sg[ts];
+        const double sedge = interior ? 0.0
+                                      : (curl_y(g, F.sff, lev, i, j) - curl_y(g, F.sff, lev, i - 1, j)
+                                         + curl_x(g, F.sff, lev, i, j - 1) - curl_x(g, F.sff, lev, i, j)) * bydx2;
+        double lap_s = ((sc - se) - (sw - sc) + (sc - sd) - (su - sc)) * bydx2 * (interior ? 1.0 : 0.0) + sedge;
+        lap_s *= -1;
+        n = lap_s + n;
+    }
+    double sfd_w = 0.0;   // SFD: the filtered vorticity (zero before the first substep)
+    if (F.model == 4) {   // SFDSolver::N (IBSolver.cc:238-244): N - chi (omega - omega_hat)
+        sfd_w = F.sfd_valid ? gin(F.what)[tu] : 0.0;
+        n -= F.sfd_chi * (c - sfd_w);
+    }
     // the interior difference times 1 plus 0, or times 0 plus the edge value
     const double edge = interior ? 0.0
                                  : (curl_y(g, W, lev, i, j) - curl_y(g, W, lev, i - 1, j)
@@ -341,6 +384,15 @@ __global__ void __launch_bounds__(kThreads) k_rhs(const FlowOps* __restrict__ fl
     a += F.w_coef * c;
     gout(F.a)[t] = a;
     gout(F.nprev)[t] = n;
+    if (F.model == 4) {   // the filtered state's substep (IBSolver.cc:247-279), omega_hat = omega at the first
+        const double wh = F.sfd_valid ? sfd_w : c;
+        const double rs = (c - wh) * F.sfd_inv_delta;
+        double av = F.an * rs;
+        if (F.bn != 0) av += F.bn * (F.nprev_valid ? gin(F.sfd_prev)[tu] : rs);
+        av *= F.dt;
+        gout(F.what)[tu] = wh + av;
+        gout(F.sfd_prev)[tu] = rs;
+    }
 }
 
 // One flux of q = Curl psi + q0 (flat index t over all levels of one flow):

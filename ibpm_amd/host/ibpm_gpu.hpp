@@ -151,6 +151,7 @@ public:
         return dir == X ? (*this)[base + (size_t)i * ny + j] : (*this)[base + (size_t)(nx + 1) * ny + (size_t)i * (ny + 1) + j];
     }
     Flux& operator=(double v) { assign(size(), v); return *this; }
+    Flux& operator-=(const Flux& o) { for (size_t k = 0; k < size(); ++k) (*this)[k] -= o[k]; return *this; }
     const Grid& getGrid() const { return _grid; }
 
 private:
@@ -622,19 +623,19 @@ private:
     int _its = 0;
 };
 
-// IBSolver with the nonlinear term (reference src/IBSolver.h:20-90).  Cholesky
-// for stationary bodies and CG for moving ones, chosen as in IBSolver.cc:122-135.
-class NonlinearIBSolver {
+// IBSolver (reference src/IBSolver.h:20-63): the timestepper of one flow on
+// the GPU (a batch of one).  Cholesky for stationary bodies and CG for moving
+// ones, chosen as in IBSolver.cc:122-135.  The model's N term is the
+// subclass's: NonlinearIBSolver, LinearizedIBSolver, AdjointIBSolver,
+// LinearizedPeriodicIBSolver, SFDSolver (IBSolver.h:69-222).
+class IBSolver {
 public:
-    NonlinearIBSolver(const Grid& grid, NavierStokesModel& model, double dt, Scheme::SchemeType scheme,
-                      double tol = 1e-7, int device = 0)
-        : _grid(grid), _model(model), _dt(dt), _scheme(scheme), _tol(tol), _device(device) {}
-    ~NonlinearIBSolver() {
+    virtual ~IBSolver() {
         if (_batch) ibpm_batch_destroy(_batch);
         if (_ctx) ibpm_ctx_destroy(_ctx);
     }
-    NonlinearIBSolver(const NonlinearIBSolver&) = delete;
-    NonlinearIBSolver& operator=(const NonlinearIBSolver&) = delete;
+    IBSolver(const IBSolver&) = delete;
+    IBSolver& operator=(const IBSolver&) = delete;
 
     std::string getName() const {
         static const char* names[] = {"Explicit Euler", "Adams Bashforth", "3rd-order Runge Kutta (3-step)",
@@ -672,7 +673,13 @@ public:
         p.cg_tol = _tol; p.force_cg = 0;
         p.geom = _model.geometry().text().c_str();
         p.ubf = _model.bfTimeDependent() ? 1 : 0;
+        p.model = _kind;
+        p.period = (int)std::max<size_t>(1, _base.size());
+        p.sfd_chi = _chi;
+        p.sfd_delta = _Delta;
         gpu_check(ibpm_batch_create(_ctx, 1, &p, &_batch));
+        for (size_t k = 0; k < _base.size(); ++k)
+            gpu_check(ibpm_batch_set_base_state(_batch, 0, (int)k, _base[k].omega.data(), _base[k].q.data()));
         if (!_cache.empty() && !_model.geTimeDependent())
             gpu_check(ibpm_batch_cholesky_cache(_batch, 0, _cache.c_str()));
         gpu_check(ibpm_batch_init(_batch));
@@ -720,18 +727,81 @@ public:
         return it[substep & 3];
     }
 
-private:
+protected:
+    IBSolver(const Grid& grid, NavierStokesModel& model, double dt, Scheme::SchemeType scheme, double tol,
+             int device, int kind)
+        : _grid(grid), _model(model), _dt(dt), _scheme(scheme), _tol(tol), _device(device), _kind(kind) {}
     const Grid& _grid;
     NavierStokesModel& _model;
     double _dt;
     Scheme::SchemeType _scheme;
     double _tol;
     int _device;
+    int _kind;                      // ibpm_flow_params::model
+    std::vector<State> _base;       // base states of the linear models
+    double _chi = 0.02, _Delta = 15.0;
     int _pc_mode = 0, _pc_pairs = 0;
     bool _local_helm = true;
     std::string _cache;
     ibpm_ctx* _ctx = nullptr;
     ibpm_batch* _batch = nullptr;
+};
+
+// N = Curl(q x omega) (IBSolver.cc:203-207)
+class NonlinearIBSolver : public IBSolver {
+public:
+    NonlinearIBSolver(const Grid& grid, NavierStokesModel& model, double dt, Scheme::SchemeType scheme,
+                      double tol = 1e-7, int device = 0)
+        : IBSolver(grid, model, dt, scheme, tol, device, 0) {}
+};
+
+// N = Curl(q0 x omega + q x omega0) about the base state x0 (IBSolver.cc:209-214)
+class LinearizedIBSolver : public IBSolver {
+public:
+    LinearizedIBSolver(const Grid& grid, NavierStokesModel& model, double dt, Scheme::SchemeType scheme,
+                       const State& x0, double tol = 1e-7, int device = 0)
+        : IBSolver(grid, model, dt, scheme, tol, device, 1) {
+        _base.push_back(x0);
+    }
+};
+
+// N = Laplacian(q0 x q) - Curl(q x omega0) (IBSolver.cc:216-220)
+class AdjointIBSolver : public IBSolver {
+public:
+    AdjointIBSolver(const Grid& grid, NavierStokesModel& model, double dt, Scheme::SchemeType scheme,
+                    const State& x0, double tol = 1e-7, int device = 0)
+        : IBSolver(grid, model, dt, scheme, tol, device, 2) {
+        _base.push_back(x0);
+    }
+};
+
+// The linearized N about base state x0[timestep % period] (IBSolver.cc:222-229)
+class LinearizedPeriodicIBSolver : public IBSolver {
+public:
+    LinearizedPeriodicIBSolver(const Grid& grid, NavierStokesModel& model, double dt, Scheme::SchemeType scheme,
+                               const std::vector<State>& x0, int period, double tol = 1e-7, int device = 0)
+        : IBSolver(grid, model, dt, scheme, tol, device, 3) {
+        _base.assign(x0.begin(), x0.begin() + period);
+    }
+};
+
+// Selective frequency damping: N = Curl(q x omega) - chi (omega - omega_hat),
+// with the filtered state omega_hat advanced by the same scheme
+// (IBSolver.cc:236-311, IBSolver.h:189-222)
+class SFDSolver : public IBSolver {
+public:
+    SFDSolver(const Grid& grid, NavierStokesModel& model, double dt, Scheme::SchemeType scheme, double Delta,
+              double chi, double tol = 1e-7, int device = 0)
+        : IBSolver(grid, model, dt, scheme, tol, device, 4) {
+        _Delta = Delta;
+        _chi = chi;
+    }
+    // the filtered vorticity in and out (loadFilteredState / saveFilteredState)
+    void getFilteredState(Scalar& omega_hat) { gpu_check(ibpm_batch_get_filtered(_batch, 0, omega_hat.data())); }
+    void setFilteredState(const Scalar& omega_hat) {
+        init();
+        gpu_check(ibpm_batch_set_filtered(_batch, 0, omega_hat.data()));
+    }
 };
 
 }  // namespace ibpm

@@ -156,6 +156,12 @@ typedef struct {
     int ubf;                    /* 1: unsteady base flow (ibpm.cc -ubf, :166-178): the motion of the
                                    only body moves to the base flow and the flow is solved in the
                                    body frame; more than one body is an error */
+    int model;                  /* ibpm.cc -model (:185-235): 0 nonlinear, 1 linear, 2 adjoint,
+                                   3 linear periodic, 4 SFD.  The linear models (1-3) have no
+                                   potential-flow part (NavierStokesModel(grid, geom, Re)) and need
+                                   base states (ibpm_batch_set_base_state) before init */
+    int period;                 /* linear periodic: number of base states (phase = step % period) */
+    double sfd_chi, sfd_delta;  /* SFD gain and filter width (ibpm.cc -chi -Delta) */
 } ibpm_flow_params;
 
 /* ---- NavierStokesModel / ProjectionSolver operators of one flow -------------
@@ -245,6 +251,15 @@ int ibpm_batch_base_flow(ibpm_batch* b, int flow, double* alpha, double* mag);
 /* CG iterations of each substep of the last step (-1 when Cholesky is used). */
 int ibpm_batch_cg_iters(ibpm_batch* b, int flow, int* h_iters, int max_substeps);
 double ibpm_batch_time(ibpm_batch* b);
+/* Base state k (vorticity h_omega, flux h_q of the flow's grid) of a linear
+   model: LinearizedIBSolver / AdjointIBSolver x0 (k = 0), the k-th phase of
+   LinearizedPeriodicIBSolver (k < period); reference IBSolver.h:92-187,
+   ibpm.cc:190-219.  Before ibpm_batch_init. */
+int ibpm_batch_set_base_state(ibpm_batch* b, int flow, int k, const double* h_omega, const double* h_q);
+/* SFD: the filtered vorticity (SFDSolver::_xhat.omega) of a flow, read or
+   written (loadFilteredState / saveFilteredState, IBSolver.cc:281-311). */
+int ibpm_batch_get_filtered(ibpm_batch* b, int flow, double* h_omega);
+int ibpm_batch_set_filtered(ibpm_batch* b, int flow, const double* h_omega);
 /* Capture one timestep as a hipGraph and replay it (stationary bodies). */
 int ibpm_batch_use_graph(ibpm_batch* b, int enable);
 /* Cholesky factor cache of a stationary-body flow, named before ibpm_batch_init:

@@ -715,6 +715,36 @@ void og_nonlinear(int nx, int ny, int ngrid, double dx, const double* q, const d
     free(cross);
 }
 
+/* LinearizedIBSolver::N, IBSolver.cc:211-216: Curl(q0 x omega + q x omega0),
+   the two cross products formed and summed flux by flux as the reference does */
+void og_linear_N(int nx, int ny, int ngrid, double dx, const double* q0, const double* w0, const double* q,
+                 const double* w, double* n_out) {
+    const long nf = NF(nx, ny) * ngrid;
+    double* v = (double*)calloc(nf, sizeof(double));
+    double* v2 = (double*)calloc(nf, sizeof(double));
+    og_cross_product(nx, ny, ngrid, dx, q0, w, v);
+    og_cross_product(nx, ny, ngrid, dx, q, w0, v2);
+    for (long k = 0; k < nf; ++k) v[k] += v2[k];
+    og_curl_flux(nx, ny, ngrid, dx, v, n_out);
+    free(v); free(v2);
+}
+
+/* AdjointIBSolver::N, IBSolver.cc:218-222:
+   Laplacian(CrossProduct(q0, q)) - Curl(CrossProduct(q, omega0)) */
+void og_adjoint_N(int nx, int ny, int ngrid, double dx, const double* q0, const double* w0, const double* q,
+                  double* n_out) {
+    const long n = SN(nx, ny) * ngrid, nf = NF(nx, ny) * ngrid;
+    double* s = (double*)calloc(n, sizeof(double));
+    double* v = (double*)calloc(nf, sizeof(double));
+    double* c = (double*)calloc(n, sizeof(double));
+    og_cross_flux_flux(nx, ny, ngrid, dx, q0, q, s);
+    og_laplacian(nx, ny, ngrid, dx, s, n_out);
+    og_cross_product(nx, ny, ngrid, dx, q, w0, v);
+    og_curl_flux(nx, ny, ngrid, dx, v, c);
+    for (long k = 0; k < n; ++k) n_out[k] -= c[k];
+    free(s); free(v); free(c);
+}
+
 /* ======================================================================= */
 /* Regulariser (Regularizer.cc)                                             */
 /* ======================================================================= */
@@ -831,6 +861,13 @@ struct og_sim {
     og_body bf;
     double bf_alpha, bf_mag, bf_u, bf_v, bf_w;
     double force_alpha;
+    /* models of ibpm.cc:185-235: 0 nonlinear, 1 linear, 2 adjoint,
+       3 linear periodic, 4 SFD */
+    int model, period;
+    double *bq[64], *bw[64];   /* base states (LinearizedPeriodic: one per phase) */
+    double chi, delta;         /* SFD gain and filter width (SFDSolver, IBSolver.h:189-222) */
+    double *what, *sfd_prev;   /* filtered vorticity and the filter's previous right-hand side */
+    int what_saved, sfd_saved;
 };
 
 static char g_err[256];
@@ -1399,6 +1436,8 @@ void og_sim_destroy(og_sim* s) {
     free(s->refx); free(s->refy); free(s->px); free(s->py); free(s->pu); free(s->pv);
     free(s->abi); free(s->afi); free(s->aw);
     free(s->omega); free(s->nprev); free(s->q); free(s->q0); free(s->f);
+    for (int k = 0; k < 64; ++k) { free(s->bq[k]); free(s->bw[k]); }
+    free(s->what); free(s->sfd_prev);
     for (int i = 0; i < 4; ++i) { free(s->chol_diag[i]); free(s->chol_low[i]); }
     free(s);
 }
@@ -1436,6 +1475,53 @@ int og_baseflow_sequence(const char* geom, int nx, int ny, int ngrid, double len
         am_out[2 * c + 1] = s->bf_mag;
     }
     og_sim_destroy(s);
+    return 0;
+}
+
+/* The model of ibpm.cc:185-235 (call before og_sim_init).  The linear,
+   adjoint and linear periodic models use NavierStokesModel(grid, geom, Re),
+   with no potential-flow part (q0 = 0); SFD keeps it.  period: base states of
+   the linear periodic model. */
+int og_sim_set_model(og_sim* s, int model, int period, double chi, double delta) {
+    if (model < 0 || model > 4 || period < 1 || period > 64) {
+        snprintf(g_err, sizeof g_err, "bad model %d or period %d", model, period);
+        return -1;
+    }
+    s->model = model;
+    s->period = model == 3 ? period : 1;
+    s->chi = chi;
+    s->delta = delta;
+    if (model >= 1 && model <= 3) memset(s->q0, 0, sizeof(double) * f_n(s));
+    if (model == 4) {
+        s->what = (double*)calloc(s_n(s), sizeof(double));   /* the reference's is unset memory; here zero */
+        s->sfd_prev = (double*)calloc(s_n(s), sizeof(double));
+    }
+    return 0;
+}
+
+/* Base state k (vorticity and flux) of the linear models (x00.load in
+   ibpm.cc:191-218). */
+int og_sim_set_base_state(og_sim* s, int k, const double* w, const double* q) {
+    if (k < 0 || k >= 64) return -1;
+    if (!s->bw[k]) s->bw[k] = (double*)malloc(sizeof(double) * s_n(s));
+    if (!s->bq[k]) s->bq[k] = (double*)malloc(sizeof(double) * f_n(s));
+    memcpy(s->bw[k], w, sizeof(double) * s_n(s));
+    memcpy(s->bq[k], q, sizeof(double) * f_n(s));
+    return 0;
+}
+
+/* SFDSolver::loadFilteredState (IBSolver.cc:289-311): the filtered state is
+   set (and marked saved); call after og_sim_init. */
+int og_sim_set_filtered(og_sim* s, const double* w) {
+    if (s->model != 4) return -1;
+    memcpy(s->what, w, sizeof(double) * s_n(s));
+    s->what_saved = 1;
+    return 0;
+}
+
+int og_sim_get_filtered(const og_sim* s, double* w) {
+    if (s->model != 4) return -1;
+    memcpy(w, s->what, sizeof(double) * s_n(s));
     return 0;
 }
 
@@ -1479,8 +1565,26 @@ int og_sim_advance(og_sim* s) {
     double* nl = (double*)malloc(sizeof(double) * n);
     double* a = (double*)malloc(sizeof(double) * n);
     double* b = (double*)malloc(sizeof(double) * (2 * s->nb + 1));
+    double* wtemp = s->model == 4 ? (double*)malloc(sizeof(double) * n) : NULL;
     for (int i = 0; i < s->nsub; ++i) {
-        og_nonlinear(s->nx, s->ny, s->ngrid, s->dx, s->q, s->omega, nl);
+        switch (s->model) {   /* IBSolver::advance: N(x) of the model, IBSolver.cc:142-150, 203-246 */
+        case 1: og_linear_N(s->nx, s->ny, s->ngrid, s->dx, s->bq[0], s->bw[0], s->q, s->omega, nl); break;
+        case 2: og_adjoint_N(s->nx, s->ny, s->ngrid, s->dx, s->bq[0], s->bw[0], s->q, nl); break;
+        case 3: {   /* phase k = timestep % period, IBSolver.cc:224-231 */
+            const int k = s->timestep % s->period;
+            og_linear_N(s->nx, s->ny, s->ngrid, s->dx, s->bq[k], s->bw[k], s->q, s->omega, nl);
+            break;
+        }
+        case 4:   /* SFDSolver::N, IBSolver.cc:238-244 */
+            og_nonlinear(s->nx, s->ny, s->ngrid, s->dx, s->q, s->omega, nl);
+            for (long k = 0; k < n; ++k) nl[k] -= s->chi * (s->omega[k] - s->what[k]);
+            break;
+        default: og_nonlinear(s->nx, s->ny, s->ngrid, s->dx, s->q, s->omega, nl);
+        }
+        if (s->model == 4) {   /* SFDSolver::advanceSubstep, IBSolver.cc:247-252 */
+            if (!s->what_saved) { memcpy(s->what, s->omega, sizeof(double) * n); s->what_saved = 1; }
+            memcpy(wtemp, s->omega, sizeof(double) * n);
+        }
         if (s->ubf) bf_move(s, s->time + s->cn[i] * s->dt);   /* NavierStokesModel.cc:93-99 */
         if (!s->stationary) {
             move_bodies(s, s->time + s->cn[i] * s->dt);
@@ -1501,7 +1605,23 @@ int og_sim_advance(og_sim* s) {
         refresh_state(s);
         memcpy(s->nprev, nl, sizeof(double) * n);
         s->old_saved = 1;
+        if (s->model == 4) {   /* the filtered state, IBSolver.cc:257-279 */
+            const double inv = 1.0 / s->delta;   /* Scalar / double multiplies by the reciprocal (Array.h:335-340) */
+            for (long k = 0; k < n; ++k) {
+                const double rhs = (wtemp[k] - s->what[k]) * inv;
+                double av = an * rhs;
+                if (bn != 0) {
+                    if (!s->sfd_saved) s->sfd_prev[k] = rhs;
+                    av += bn * s->sfd_prev[k];
+                }
+                av *= s->dt;
+                s->what[k] += av;
+                s->sfd_prev[k] = rhs;
+            }
+            s->sfd_saved = 1;
+        }
     }
+    free(wtemp);
     s->time += s->dt;
     s->timestep++;
     if (s->ubf) {   /* ibpm.cc:347-351: forces rotated by the base-flow angle at the new time */

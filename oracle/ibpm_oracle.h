@@ -58,6 +58,10 @@ void og_cross_flux_flux(int nx, int ny, int ngrid, double dx, const double* q1, 
                         double* out);   /* VectorOperations.cc:465-486 */
 void og_nonlinear(int nx, int ny, int ngrid, double dx, const double* q, const double* w,
                   double* n_out);                                  /* IBSolver.cc:205-209 */
+void og_linear_N(int nx, int ny, int ngrid, double dx, const double* q0, const double* w0, const double* q,
+                 const double* w, double* n_out);                  /* IBSolver.cc:211-216 */
+void og_adjoint_N(int nx, int ny, int ngrid, double dx, const double* q0, const double* w0, const double* q,
+                  double* n_out);                                  /* IBSolver.cc:218-222 */
 
 /* Multi-domain inner products, VectorOperations.cc:223-405 */
 double og_ip_scalar(int nx, int ny, int ngrid, double dx, const double* f, const double* g);
@@ -88,6 +92,15 @@ og_sim* og_sim_create(int nx, int ny, int ngrid, double length, double xoff, dou
 void og_sim_destroy(og_sim* s);
 const char* og_sim_error(void);
 int  og_sim_set_ubf(og_sim* s);
+/* Model of ibpm.cc:185-235: 0 nonlinear, 1 linear, 2 adjoint, 3 linear
+   periodic (period base states), 4 SFD (gain chi, filter width delta); call
+   before og_sim_init. */
+int  og_sim_set_model(og_sim* s, int model, int period, double chi, double delta);
+/* Base state k (vorticity, flux) of the linear models. */
+int  og_sim_set_base_state(og_sim* s, int k, const double* w, const double* q);
+/* SFD filtered vorticity: set (marked saved, as loadFilteredState) and read. */
+int  og_sim_set_filtered(og_sim* s, const double* w);
+int  og_sim_get_filtered(const og_sim* s, double* w);
 int  og_baseflow_sequence(const char* geom, int nx, int ny, int ngrid, double length, double xoff, double yoff,
                           double alpha_deg, int ntimes, const double* times, double* q_out, double* am_out);
 int  og_sim_num_points(const og_sim* s);

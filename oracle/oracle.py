@@ -185,6 +185,22 @@ def cross_product(nx, ny, ngrid, dx, q, w):
     return out
 
 
+def linear_N(nx, ny, ngrid, dx, q0, w0, q, w):
+    """LinearizedIBSolver::N: Curl(q0 x omega + q x omega0)."""
+    q0, w0, q, w = _c(q0), _c(w0), _c(q), _c(w)
+    out = np.zeros(scalar_size(nx, ny, ngrid))
+    lib().og_linear_N(nx, ny, ngrid, ctypes.c_double(dx), _p(q0), _p(w0), _p(q), _p(w), _p(out))
+    return out
+
+
+def adjoint_N(nx, ny, ngrid, dx, q0, w0, q):
+    """AdjointIBSolver::N: Laplacian(q0 x q) - Curl(q x omega0)."""
+    q0, w0, q = _c(q0), _c(w0), _c(q)
+    out = np.zeros(scalar_size(nx, ny, ngrid))
+    lib().og_adjoint_N(nx, ny, ngrid, ctypes.c_double(dx), _p(q0), _p(w0), _p(q), _p(out))
+    return out
+
+
 def cross_flux_flux(nx, ny, ngrid, dx, q1, q2):
     q1 = _c(q1)
     q2 = _c(q2)
@@ -225,13 +241,15 @@ def reg_toboundary(npts, dx, assoc, q):
 
 
 SCHEMES = {"euler": 0, "ab2": 1, "rk3": 2, "rk3b": 3}
+MODELS = {"nonlinear": 0, "linear": 1, "adjoint": 2, "linearperiodic": 3, "sfd": 4}
 
 
 class Sim:
     """Whole-solver oracle: one flow advanced by the restated IBSolver."""
 
     def __init__(self, nx, ny, ngrid, length, xoffset, yoffset, Re, dt, geom,
-                 scheme="rk3", alpha_deg=0.0, tol=1e-7, force_cg=False, ubf=False):
+                 scheme="rk3", alpha_deg=0.0, tol=1e-7, force_cg=False, ubf=False,
+                 model="nonlinear", period=1, chi=0.02, delta=15.0):
         L = lib()
         self.nx, self.ny, self.ngrid = nx, ny, ngrid
         self.h = L.og_sim_create(nx, ny, ngrid, length, xoffset, yoffset, Re, dt, SCHEMES[scheme],
@@ -245,6 +263,25 @@ class Sim:
             if self.ubf < 0:
                 raise ValueError("oracle: " + L.og_sim_error().decode())
         self.nsub = 1 if scheme in ("euler", "ab2") else (4 if scheme == "rk3b" else 3)
+        if model != "nonlinear":
+            if L.og_sim_set_model(ctypes.c_void_p(self.h), MODELS[model], period, ctypes.c_double(chi),
+                                  ctypes.c_double(delta)) != 0:
+                raise ValueError("oracle: " + L.og_sim_error().decode())
+
+    def set_filtered(self, w):
+        """SFD: set the filtered vorticity (loadFilteredState); after init()."""
+        w = _c(w)
+        lib().og_sim_set_filtered(ctypes.c_void_p(self.h), _p(w))
+
+    def filtered(self):
+        out = np.zeros(scalar_size(self.nx, self.ny, self.ngrid))
+        lib().og_sim_get_filtered(ctypes.c_void_p(self.h), _p(out))
+        return out
+
+    def set_base_state(self, omega, q, k=0):
+        """Base state k (vorticity, flux) of the linear models; before init()."""
+        w, qq = _c(omega), _c(q)
+        lib().og_sim_set_base_state(ctypes.c_void_p(self.h), k, _p(w), _p(qq))
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -179,3 +179,58 @@ def test_preconditioned_cg_matches_golden(gpu, name, pairs):
     assert st["fallbacks"] == 0 and st["abandoned"] == 0
     assert st["iterations"] == int(np.sum(iters))
     assert np.sum(iters) <= 0.4 * g["cg_iters"].sum()
+
+
+def _read_state(path, nx, ny, ngrid, npts):
+    import ctypes
+    from ibpm_amd import load
+    lib = load()
+    lib.ibpm_state_load.argtypes = [ctypes.c_char_p] + [ctypes.c_void_p] * 12
+    hdr = [ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double(), ctypes.c_double(),
+           ctypes.c_double(), ctypes.c_int()]
+    q = np.zeros(ngrid * (2 * nx * ny + nx + ny))
+    w = np.zeros(ngrid * (nx - 1) * (ny - 1))
+    f = np.zeros(2 * npts)
+    ts, tm = ctypes.c_int(), ctypes.c_double()
+    assert lib.ibpm_state_load(path.encode(), *[ctypes.byref(h) for h in hdr], q.ctypes.data, w.ctypes.data,
+                               f.ctypes.data, ctypes.byref(ts), ctypes.byref(tm)) == 0
+    return w, q
+
+
+@pytest.mark.parametrize("model", ["linear", "adjoint", "sfd"])
+def test_ibpm_driver_other_models(gpu, oracle, model):
+    """The driver's -model linear|adjoint (base flow and initial condition from
+    a nonlinear run's restart file) and -model sfd (that file as initial
+    condition, no filtered-state file: filtered vorticity zero, as
+    loadFilteredState leaves it): forces equal the oracle's model run."""
+    exe = os.path.join(os.path.dirname(HERE), "ibpm_amd", "ibpm")
+    geom = "body C\n circle_n 0.1 0.05 0.5 30\nend\n"
+    args = ["-nx", "48", "-ny", "48", "-ngrid", "2", "-Re", "100", "-dt", "0.02"]
+    with tempfile.TemporaryDirectory() as d:
+        gpath = os.path.join(d, "cyl.geom")
+        open(gpath, "w").write(geom)
+        r = subprocess.run([exe, "-geom", gpath] + args + ["-nsteps", "5", "-restart", "5", "-outdir", d,
+                                                          "-name", "base"], capture_output=True, text=True,
+                           timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        bin5 = os.path.join(d, "base00005.bin")
+        extra = ["-chi", "0.5", "-Delta", "2"] if model == "sfd" else ["-baseflow", bin5]
+        r = subprocess.run([exe, "-geom", gpath] + args + ["-model", model, "-ic", bin5, "-nsteps", "3",
+                                                          "-restart", "0", "-outdir", d, "-name", "m"] + extra,
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        full = np.loadtxt(os.path.join(d, "m.force17"))
+        w0, q0 = _read_state(bin5, 48, 48, 2, 30)
+    s = oracle.Sim(48, 48, 2, 4.0, -2.0, -2.0, 100.0, 0.02, geom, model=model, chi=0.5, delta=2.0)
+    if model != "sfd":
+        s.set_base_state(w0, q0)
+    s.init()
+    s.set_omega(w0)
+    if model == "sfd":
+        s.set_filtered(np.zeros_like(w0))
+    for k in range(3):
+        s.advance(1)
+        cd, cl = s.forces()
+        scale = max(abs(cd), abs(cl))
+        print(model, k, full[k, 2], cd, full[k, 3], cl, flush=True)
+        assert abs(full[k, 2] - cd) <= 1e-9 * scale and abs(full[k, 3] - cl) <= 1e-9 * scale

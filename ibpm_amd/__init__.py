@@ -67,6 +67,7 @@ def load():
         "ibpm_elliptic_solve_precision": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_double, ctypes.c_double, vp, vp, ctypes.c_int],
         "ibpm_batch_set_precision": [vp, ctypes.c_int],
+        "ibpm_batch_precision_report": [vp, vp, vp, vp, vp, ctypes.c_int],
         "ibpm_curl_flux": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, vp, vp],
         "ibpm_curl_scalar": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                              ctypes.c_double, ctypes.c_double, vp, vp],
@@ -137,6 +138,7 @@ class DeviceArray:
 
     def __init__(self, ctx, n):
         self.ctx, self.n = ctx, int(n)
+        self._L = load()   # kept: module globals may be gone when __del__ runs at shutdown
         self.ptr = ctypes.c_void_p()
         _check(load().ibpm_malloc(ctx.h, 8 * max(self.n, 1), ctypes.byref(self.ptr)))
 
@@ -154,7 +156,7 @@ class DeviceArray:
 
     def __del__(self):
         if getattr(self, "ptr", None) and self.ptr.value:
-            load().ibpm_free(self.ctx.h, self.ptr)
+            self._L.ibpm_free(self.ctx.h, self.ptr)
             self.ptr = None
 
 
@@ -162,12 +164,13 @@ class Context:
     """One GPU and one HIP stream (ibpm_ctx)."""
 
     def __init__(self, device=0):
+        self._L = load()   # kept: module globals may be gone when __del__ runs at shutdown
         self.h = ctypes.c_void_p()
-        _check(load().ibpm_ctx_create(device, ctypes.byref(self.h)))
+        _check(self._L.ibpm_ctx_create(device, ctypes.byref(self.h)))
 
     def close(self):
         if getattr(self, "h", None) and self.h.value:
-            load().ibpm_ctx_destroy(self.h)
+            self._L.ibpm_ctx_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -291,13 +294,14 @@ class Regularizer:
         self.ctx, self.nx, self.ny, self.ngrid = ctx, nx, ny, ngrid
         xy = _c(xy)
         self.npts = len(xy) // 2
+        self._reg_lib = load()
         self.h = ctypes.c_void_p()
         _check(load().ibpm_reg_create(ctx.h, nx, ny, ngrid, length, xoffset, yoffset, self.npts,
                                       xy.ctypes.data, ctypes.byref(self.h)))
 
     def close(self):
         if getattr(self, "h", None) and self.h.value:
-            load().ibpm_reg_destroy(self.h)
+            self._reg_lib.ibpm_reg_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -358,12 +362,13 @@ class Batch:
         self.flows = list(flows)
         arr = (FlowParams * len(self.flows))(*[f.params() for f in self.flows])
         self._keep = arr
+        self._L = load()   # kept: module globals may be gone when __del__ runs at shutdown
         self.h = ctypes.c_void_p()
         _check(load().ibpm_batch_create(ctx.h, len(self.flows), arr, ctypes.byref(self.h)))
 
     def close(self):
         if getattr(self, "h", None) and self.h.value:
-            load().ibpm_batch_destroy(self.h)
+            self._L.ibpm_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -471,12 +476,26 @@ class Batch:
         _check(load().ibpm_batch_set_precond(self.h, m, k))
         return self
 
-    PRECISION = {"fp64": 0, "fp32_step": 1, "fp32": 2}
+    PRECISION = {"fp64": 0, "fp32_step": 1, "fp32": 2, "reduced_tuned": 3, "gemm_step": 4}
+
+    def precision_report(self):
+        """{N: (uses GEMM-DST, radix us, GEMM us)} of the step's solves under
+        modes reduced_tuned / gemm_step."""
+        n = np.zeros(16, dtype=np.int32)
+        g = np.zeros(16, dtype=np.int32)
+        tr = np.zeros(16)
+        tg = np.zeros(16)
+        k = _check(load().ibpm_batch_precision_report(self.h, n.ctypes.data, g.ctypes.data, tr.ctypes.data,
+                                                      tg.ctypes.data, 16))
+        return {int(n[i]): (bool(g[i]), float(tr[i]), float(tg[i])) for i in range(k)}
 
     def set_precision(self, mode="fp64"):
         """Transform precision of the elliptic solves: "fp64" (the reference's),
-        "fp32_step" (FP32 transforms in the step's own solves) or "fp32" (also
-        inside the force solve's M applications); fields stay FP64."""
+        "fp32_step" (FP32 transforms in the step's own solves), "fp32" (also
+        inside the force solve's M applications), "reduced_tuned" (step solves
+        by FP32 radix or split-BF16 GEMM-DST on MFMA, whichever the autotuner
+        times faster per length N <= 511) or "gemm_step" (GEMM-DST wherever it
+        applies); fields stay FP64."""
         _check(load().ibpm_batch_set_precision(self.h, self.PRECISION[mode]))
         return self
 

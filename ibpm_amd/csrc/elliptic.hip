@@ -1300,6 +1300,162 @@ __global__ void __launch_bounds__(kThreads) k_fs_helm_coarsify(const HelmJob* __
     gout(J.R)[sidx(J.nx, J.ny, i, j)] = v;
 }
 
+// ------------------------------------------------ dense sine-matrix GEMM on MFMA
+// A level solve as four dense products with the DST-I matrix S (S[a][b] =
+// 2 sin(pi (a+1)(b+1) / n), symmetric), each C = A S of an Np x Np operand A
+// (Np = N rounded up to 64, zero padded) held as two bf16 planes, hi =
+// bf16(x) and lo = bf16(x - hi): C = Ah Sh + Ah Sl + Al Sh accumulated in FP32
+// on the 16x16x32 bf16 MFMA (split-BF16 compensated products; the dropped
+// Al Sl term and FP32 accumulation leave a relative error near 1e-6).
+//   prep: F = the level's right-hand side (rhs_value) -> planes X0
+//   G1: X1^T = (X0 S)^T              along j, written transposed
+//   G2: X2^T = X1^T S, / eigenvalue  along i, the spectral scaling
+//   G3: X3 = (X2^T S)^T              inverse along i, written transposed
+//   G4: u = norm X3 S                inverse along j, FP64 output (pass 3's epilogue)
+// A and S fragments are read straight from global memory (16 bytes per lane:
+// 8 consecutive bf16 along k, as the MFMA takes them; S symmetric makes its
+// columns rows); one wave computes a 32 x 64 tile of C.
+typedef __bf16 Bf16x8 __attribute__((ext_vector_type(8)));
+typedef float FloatX4 __attribute__((ext_vector_type(4)));
+
+__host__ __device__ constexpr int gemm_np(int n) { return (n + 63) / 64 * 64; }
+
+// The two planes of a level's operand inside the job's transposed scratch:
+// plane pair 0 in T1, pair 1 in T2 (each 2 Np^2 bf16 <= the FP64 scratch).
+__device__ inline __bf16* gemm_plane(const EllJob& J, int pair, int lohi) {
+    const int Np = gemm_np(J.nx - 1);
+    __bf16* base = (__bf16*)(pair == 0 ? J.T1 : J.T2);
+    return base + (size_t)lohi * Np * Np;
+}
+
+__device__ inline void split_bf16(float x, __bf16& hi, __bf16& lo) {
+    hi = (__bf16)x;
+    lo = (__bf16)(x - (float)hi);
+}
+
+// prep: X0 = rhs (FP64 -> FP32 -> hi, lo), zero in the padding.
+__global__ void __launch_bounds__(kThreads) k_gemm_prep(const EllJob* __restrict__ jobs) {
+    const EllJob& J = jobs[blockIdx.y];
+    if (skipped(J.skip)) return;
+    const int N1 = J.nx - 1, N2 = J.ny - 1, Np = gemm_np(N1);
+    const long t = (long)blockIdx.x * kThreads + threadIdx.x;
+    if (t >= (long)Np * Np) return;
+    const int i = (int)(t / Np), j = (int)(t % Np);
+    const float v = (i < N1 && j < N2) ? (float)rhs_value(J, i + 1, j + 1) : 0.0f;
+    __bf16 hi, lo;
+    split_bf16(v, hi, lo);
+    gemm_plane(J, 0, 0)[t] = hi;
+    gemm_plane(J, 0, 1)[t] = lo;
+}
+
+// One C = A S product of every job (stage 1..4 as above).  Grid: (Np / 64
+// column tiles, Np / 32 row tiles, jobs); one wave per workgroup.
+template <int STAGE>
+__global__ void __launch_bounds__(64) k_gemm_dst(const EllJob* __restrict__ jobs, const __bf16* __restrict__ Sh,
+                                                 const __bf16* __restrict__ Sl, int Np_max) {
+    const EllJob& J = jobs[blockIdx.z];
+    if (skipped(J.skip)) return;
+    const int N1 = J.nx - 1, N2 = J.ny - 1, Np = gemm_np(N1);
+    const int c0 = blockIdx.x * 64, r0 = blockIdx.y * 32;
+    if (c0 >= Np || r0 >= Np) return;
+    const int in_pair = (STAGE == 1 || STAGE == 3) ? 0 : 1;
+    const __bf16* Ah = gemm_plane(J, in_pair, 0);
+    const __bf16* Al = gemm_plane(J, in_pair, 1);
+    const int l = threadIdx.x, lr = l & 15, lk = (l >> 4) * 8;
+    FloatX4 acc[2][4];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = FloatX4{0.f, 0.f, 0.f, 0.f};
+    // fragments of step k0 are loaded while step k0 - 32 multiplies
+    Bf16x8 ah[2], al[2], sh[4], sl[4];
+    auto load = [&](int k0, Bf16x8* xh, Bf16x8* xl, Bf16x8* yh, Bf16x8* yl) {
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            const size_t o = (size_t)(r0 + 16 * a + lr) * Np + k0 + lk;
+            xh[a] = *(const Bf16x8*)(Ah + o);
+            xl[a] = *(const Bf16x8*)(Al + o);
+        }
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {   // S[k][c] = S[c][k]: row c, 8 consecutive k
+            const size_t o = (size_t)(c0 + 16 * b + lr) * Np_max + k0 + lk;
+            yh[b] = *(const Bf16x8*)(Sh + o);
+            yl[b] = *(const Bf16x8*)(Sl + o);
+        }
+    };
+    load(0, ah, al, sh, sl);
+    for (int k0 = 0; k0 < Np; k0 += 32) {
+        Bf16x8 nah[2], nal[2], nsh[4], nsl[4];
+        const int kn = k0 + 32 < Np ? k0 + 32 : k0;   // the last step reloads its own (in bounds)
+        load(kn, nah, nal, nsh, nsl);
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], sh[b], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], sl[b], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], sh[b], acc[a][b], 0, 0, 0);
+            }
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            ah[a] = nah[a];
+            al[a] = nal[a];
+        }
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            sh[b] = nsh[b];
+            sl[b] = nsl[b];
+        }
+    }
+    // C[r][c] of block (a, b): r = r0 + 16 a + 4 (l / 16) + e, c = c0 + 16 b + l % 16
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int r = r0 + 16 * a + 4 * (l >> 4) + e, c = c0 + 16 * b + lr;
+                float v = acc[a][b][e];
+                if constexpr (STAGE == 4) {   // u = norm X3 S (+ pass 3's epilogue), rows i = r, columns j = c
+                    if (r < N1 && c < N2) {
+                        const long o = (long)r * N2 + c;
+                        const double vv = (double)v * J.norm;
+                        const double uv = J.epilogue ? gin(J.W)[o] - vv : vv;
+                        gout(J.u)[o] = uv;
+                        if (J.u2) gout(J.u2)[o] = uv;
+                        if (J.vside && on_bc_ring(J, r + 1, c + 1)) gout(J.vside)[o] = vv;
+                    }
+                } else {
+                    if constexpr (STAGE == 2) {   // X2^T[l][k] / eigenvalue (EllipticSolver2d.cc:42-54, 89-93)
+                        if (r < N2 && c < N1) {
+                            const double lam = 2. * (gin(J.cosx)[c] + gin(J.cosy)[r] - 2.) * J.bydx2;
+                            v = (float)((double)v * ((J.kind == 0) ? 1. / lam : 1. / (1 + J.alpha * lam)));
+                        } else {
+                            v = 0.0f;
+                        }
+                    }
+                    __bf16 hi, lo;
+                    split_bf16(v, hi, lo);
+                    const size_t o = (STAGE == 2) ? (size_t)r * Np + c : (size_t)c * Np + r;   // 1, 3 transposed
+                    gemm_plane(J, 1 - in_pair, 0)[o] = hi;
+                    gemm_plane(J, 1 - in_pair, 1)[o] = lo;
+                }
+            }
+}
+
+// GEMM-DST of every job of one group (N1 = N2 = N of the plan, <= 511).
+// Sh, Sl: the split sine matrix of that N, Np_max columns per row.
+void launch_gemm_dst(const EllJob* d_jobs, int njobs, int n, const __bf16* Sh, const __bf16* Sl, hipStream_t s) {
+    const int Np = gemm_np(n);
+    hipLaunchKernelGGL(k_gemm_prep, dim3((unsigned)(((long)Np * Np + kThreads - 1) / kThreads), njobs),
+                       dim3(kThreads), 0, s, d_jobs);
+    const dim3 grid(Np / 64, Np / 32, njobs);
+    hipLaunchKernelGGL(k_gemm_dst<1>, grid, dim3(64), 0, s, d_jobs, Sh, Sl, Np);
+    hipLaunchKernelGGL(k_gemm_dst<2>, grid, dim3(64), 0, s, d_jobs, Sh, Sl, Np);
+    hipLaunchKernelGGL(k_gemm_dst<3>, grid, dim3(64), 0, s, d_jobs, Sh, Sl, Np);
+    hipLaunchKernelGGL(k_gemm_dst<4>, grid, dim3(64), 0, s, d_jobs, Sh, Sl, Np);
+}
+
 // ------------------------------------------------------------ host launch
 bool pass_in_registers(int M, long row_pairs) {
     return M == 2048 || (M == 1024 && row_pairs >= kRegMinPairs);

@@ -19,6 +19,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <set>
 #include <string>
 #include <thread>
 #include <vector>
@@ -91,8 +92,55 @@ struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::map<int, FftPlan> plans;   // by DST length
+    std::map<int, std::pair<__bf16*, __bf16*>> sines;   // split sine matrices of the GEMM-DST, by N
     DevMem mem;
 };
+
+// Round to nearest even of an FP32 value to bf16 (the device conversion's rule).
+static __bf16 host_bf16(float x) {
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    const uint16_t h = (uint16_t)(u >> 16);
+    __bf16 b;
+    std::memcpy(&b, &h, 2);
+    return b;
+}
+
+static float host_float(__bf16 b) {
+    uint16_t h;
+    std::memcpy(&h, &b, 2);
+    const uint32_t u = (uint32_t)h << 16;
+    float x;
+    std::memcpy(&x, &u, 4);
+    return x;
+}
+
+// The DST-I matrix S[a][b] = 2 sin(pi (a+1)(b+1) / (N+1)) of length N as two
+// bf16 planes (hi = bf16(S), lo = bf16(S - hi)), Np x Np, zero padded.
+static int get_sines(Ctx* c, int N, const __bf16** Sh, const __bf16** Sl) {
+    auto it = c->sines.find(N);
+    if (it == c->sines.end()) {
+        const int Np = gemm_np_host(N), n = N + 1;
+        std::vector<__bf16> h((size_t)2 * Np * Np, host_bf16(0.0f));
+        const double pi = 4. * std::atan(1.0);
+        for (int a = 0; a < N; ++a)
+            for (int b = 0; b < N; ++b) {
+                const long m = (long)(a + 1) * (b + 1) % (2 * n);   // exact argument reduction
+                const double v = 2. * std::sin(pi * (double)m / n);
+                const __bf16 hi = host_bf16((float)v);
+                h[(size_t)a * Np + b] = hi;
+                h[(size_t)Np * Np + (size_t)a * Np + b] = host_bf16((float)(v - (double)host_float(hi)));
+            }
+        __bf16* d = nullptr;
+        CKR(c->mem.alloc(&d, h.size()));
+        CK(hipMemcpy(d, h.data(), sizeof(__bf16) * h.size(), hipMemcpyHostToDevice));
+        it = c->sines.emplace(N, std::make_pair(d, d + (size_t)Np * Np)).first;
+    }
+    *Sh = it->second.first;
+    *Sl = it->second.second;
+    return 0;
+}
 
 // Shape of the plan of a DST-I of length N: FFT length M = 2(N+1), its
 // Stockham radices and the row pairs per workgroup (table pointers unset).
@@ -184,13 +232,13 @@ struct SolveSpec {
 enum KClass { K_PASS1, K_PASS2, K_PASS3, K_COARSIFY, K_VEL, K_CROSS, K_RHS, K_CURLQ, K_INTERP,
               K_SMEAR, K_MINV, K_CG, K_FORCES, K_CLEAR, K_FSPASS1, K_FSPASS2, K_FSPASS3, K_FSCOARSIFY,
               K_FSHELM, K_HELM, K_PASS32_1, K_PASS32_2, K_PASS32_3, K_FSPASS32_1, K_FSPASS32_2,
-              K_FSPASS32_3, K_NCLASS };
+              K_FSPASS32_3, K_GEMM, K_NCLASS };
 static const char* kclass_names[K_NCLASS] = {"ell_pass1", "ell_pass2", "ell_pass3", "coarsify", "vel",
                                              "cross", "rhs", "curl_q", "interp", "smear_curl", "minv",
                                              "cg", "forces", "clear_nodes", "fs_pass1", "fs_pass2",
                                              "fs_pass3", "fs_coarsify", "fs_helm", "helm", "ell32_pass1",
                                              "ell32_pass2", "ell32_pass3", "fs32_pass1", "fs32_pass2",
-                                             "fs32_pass3"};
+                                             "fs32_pass3", "ell_gemm"};
 struct Probe {
     struct Rec { int cls; hipEvent_t a, b; double bytes; };
     std::vector<Rec> recs;
@@ -274,6 +322,10 @@ struct PassGroup {
     int maxb[3] = {0, 0, 0}, lds[3] = {0, 0, 0};
     bool regs[3] = {false, false, false};           // pass_in_registers per pass
     bool fp32 = false;                              // FP32 transforms (LDS passes, k32_*)
+    bool gemm = false;                              // split-BF16 GEMM-DST on MFMA (launch_gemm_dst)
+    const __bf16 *Sh = nullptr, *Sl = nullptr;      // its sine matrix
+    int gemm_n = 0;                                 // its N (square levels only)
+    double t_radix_us = 0, t_gemm_us = 0;           // autotuner timings of the group (0: not timed)
     int maxb64[3] = {0, 0, 0}, lds64[3] = {0, 0, 0};   // the FP64 launch shape, kept for switching back
     bool regs64[3] = {false, false, false};
     std::vector<int> jflow;                         // flow of each job
@@ -387,6 +439,7 @@ static void set_plan_fp32(SolvePlan& P, bool on) {
     for (auto& step : P.steps)
         for (auto& G : step) {
             G.fp32 = on;
+            G.gemm = false;
             for (int k = 0; k < 3; ++k) {
                 if (!on) {
                     G.regs[k] = G.regs64[k];
@@ -404,6 +457,76 @@ static void set_plan_fp32(SolvePlan& P, bool on) {
             }
             if (!G.jbytes[0].empty()) group_bytes(G);
         }
+}
+
+// Whether a group can take the GEMM-DST: square levels of N <= 511 whose
+// transposed scratch holds the two bf16 plane pairs (2 Np^2 each).
+static bool gemm_eligible(const PassGroup& G) {
+    if (G.h.empty()) return false;
+    for (const EllJob& J : G.h) {
+        const int N = J.nx - 1, Np = gemm_np_host(N);
+        if (J.nx != J.ny || N > 511 || N != G.h[0].nx - 1) return false;
+        if (4.0 * Np * Np > 8.0 * (double)(J.ny - 1) * t1_ld(J.nx - 1)) return false;
+    }
+    return true;
+}
+
+// The reduced-precision plan's transform per group: mode 3 times the FP32
+// radix passes and the split-BF16 GEMM-DST on the group's own jobs (outputs
+// sent to scratch, so no field changes) and keeps the faster; mode 4 takes the
+// GEMM wherever it is eligible.  Groups that are not eligible keep the FP32
+// radix passes.  The plan must already be FP32 (set_plan_fp32).
+static int tune_plan_gemm(Ctx* c, DevMem& mem, SolvePlan& P, int mode) {
+    hipStream_t s = c->stream;
+    for (auto& step : P.steps)
+        for (auto& G : step) {
+            G.gemm = false;
+            if (!gemm_eligible(G)) continue;
+            const int N = G.h[0].nx - 1;
+            CKR(get_sines(c, N, &G.Sh, &G.Sl));
+            G.gemm_n = N;
+            if (mode == 4) { G.gemm = true; continue; }
+            std::vector<EllJob> jobs = G.h;
+            long maxns = 0;
+            for (auto& J : jobs) maxns = std::max(maxns, (long)(J.nx - 1) * (J.ny - 1));
+            double* out = nullptr;
+            EllJob* d = nullptr;
+            CKR(mem.alloc(&out, maxns));
+            for (auto& J : jobs) {   // write to scratch; no epilogue, side outputs or skip flags
+                J.u = out; J.u2 = nullptr; J.vside = nullptr; J.epilogue = 0; J.W = nullptr; J.skip = nullptr;
+            }
+            CKR(mem.alloc(&d, jobs.size()));
+            CK(hipMemcpy(d, jobs.data(), sizeof(EllJob) * jobs.size(), hipMemcpyHostToDevice));
+            auto radix = [&] {
+                for (int pass = 1; pass <= 3; ++pass)
+                    launch_ell_pass32(pass, d, G.n, G.maxb[pass - 1], G.lds[pass - 1], s, P.force_solve,
+                                      pass == 2 ? G.mi : G.mj);
+            };
+            auto gemm = [&] { launch_gemm_dst(d, G.n, N, G.Sh, G.Sl, s); };
+            hipEvent_t e0, e1;
+            CK(hipEventCreate(&e0));
+            CK(hipEventCreate(&e1));
+            float ms_r = 0, ms_g = 0;
+            for (int rep = 0; rep < 2; ++rep) {   // a warm-up round, then the timed one
+                CK(hipEventRecord(e0, s));
+                for (int k = 0; k < 3; ++k) radix();
+                CK(hipEventRecord(e1, s));
+                CK(hipEventSynchronize(e1));
+                CK(hipEventElapsedTime(&ms_r, e0, e1));
+                CK(hipEventRecord(e0, s));
+                for (int k = 0; k < 3; ++k) gemm();
+                CK(hipEventRecord(e1, s));
+                CK(hipEventSynchronize(e1));
+                CK(hipEventElapsedTime(&ms_g, e0, e1));
+            }
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            CK(hipGetLastError());
+            G.t_radix_us = ms_r * 1e3 / 3;
+            G.t_gemm_us = ms_g * 1e3 / 3;
+            G.gemm = G.t_gemm_us < G.t_radix_us;
+        }
+    return 0;
 }
 
 // The fine-to-coarse chain of a solve's right-hand side (central regions of
@@ -439,6 +562,14 @@ static void run_solve(const SolvePlan& P, hipStream_t s, Probe* pr = nullptr,
     if (coarsify) run_coarsify(P, s, pr, active);
     for (int st = 0; st < P.nlev; ++st)
         for (const PassGroup& G : P.steps[st]) {
+            if (G.gemm) {
+                double by = 0;
+                if (pr)
+                    for (size_t j = 0; j < G.jflow.size(); ++j)
+                        if (!active || (*active)[G.jflow[j]]) by += 48.0 * (G.h[j].nx - 1) * (G.h[j].ny - 1);
+                timed(pr, K_GEMM, by, s, [&] { launch_gemm_dst(G.d, G.n, G.gemm_n, G.Sh, G.Sl, s); });
+                continue;
+            }
             for (int pass = 1; pass <= 3; ++pass) {
                 const int cls = G.fp32 ? (fs ? K_FSPASS32_1 : K_PASS32_1) : (fs ? K_FSPASS1 : K_PASS1);
                 timed(pr, cls + pass - 1, pr ? sum(G.jflow, G.jbytes[pass - 1], G.bytes[pass - 1]) : 0, s, [&] {
@@ -567,6 +698,7 @@ struct ibpm_batch {
     // (restrict_m2l_pass3), when every pass 1 of that solve is register-held
     bool m2l_boxed = false;
     int precision = 0;             // ibpm_batch_set_precision
+    bool fs32 = false;             // the force solve's solves in FP32
     ibpmg::HelmJob* d_m2lc = nullptr;
     int m2lc_cap = 0, m2lc_n = 0, m2lc_blocks = 0;
     std::vector<int> m2lc_flow;
@@ -1328,7 +1460,7 @@ static int plan_local_helm(ibpm_batch* B, int i) {
         }
         B->M2L.force_solve = true;
         CKR(build_solve_plan(&B->ctx->c, B->mem, m2l, B->M2L));
-        if (B->precision >= 2) set_plan_fp32(B->M2L, true);
+        if (B->fs32) set_plan_fp32(B->M2L, true);
         B->m2l_built = true;
     }
     CKR(restrict_m2l_pass3(B, nsup));
@@ -2162,7 +2294,16 @@ static int solve_common(ibpm_ctx* c, int kind, int nx, int ny, int ngrid, double
     s.cosx = cosx; s.cosy = cosy; s.bcring = d_bc;
     SolvePlan P;
     CKR(build_solve_plan(&c->c, mem, {s}, P));
-    if (precision == 1) set_plan_fp32(P, true);
+    if (precision >= 1) set_plan_fp32(P, true);
+    if (precision == 2) {
+        CKR(tune_plan_gemm(&c->c, mem, P, 4));
+        for (auto& step : P.steps)
+            for (auto& G : step)
+                if (!G.gemm) {
+                    set_error("GEMM-DST: square levels of at most 511 unknowns per side only (got %dx%d)", nx, ny);
+                    return -1;
+                }
+    }
     run_solve(P, c->c.stream);
     CK(hipStreamSynchronize(c->c.stream));
     CK(hipGetLastError());
@@ -2181,8 +2322,8 @@ int ibpm_elliptic_solve(ibpm_ctx* c, int kind, int nx, int ny, int ngrid, double
 
 int ibpm_elliptic_solve_precision(ibpm_ctx* c, int kind, int nx, int ny, int ngrid, double dx, double alpha,
                                   const double* d_f, double* d_u, int precision) {
-    if (precision != 0 && precision != 1) {
-        set_error("precision %d: 0 (FP64 transforms) or 1 (FP32 transforms)", precision);
+    if (precision < 0 || precision > 2) {
+        set_error("precision %d: 0 (FP64 transforms), 1 (FP32 transforms) or 2 (split-BF16 GEMM-DST)", precision);
         return -1;
     }
     return solve_common(c, kind, nx, ny, ngrid, dx, alpha, d_f, nullptr, d_u, 1.0, precision);
@@ -2938,8 +3079,40 @@ int ibpm_batch_create(ibpm_ctx* ctx, int nflows, const ibpm_flow_params* params,
     return 0;
 }
 
+// Captured step graphs still alive when the process exits (a batch its
+// owner never destroyed) are destroyed by an exit handler that runs before
+// the HIP runtime's own teardown (registered after the runtime started, so
+// it runs first), instead of being left to that teardown.
+namespace {
+std::mutex g_graph_mu;
+std::set<ibpm_batch*> g_graph_batches;
+void destroy_live_graphs() {
+    std::lock_guard<std::mutex> lk(g_graph_mu);
+    for (ibpm_batch* b : g_graph_batches) {
+        (void)hipStreamSynchronize(b->ctx->c.stream);
+        if (b->graph) (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    g_graph_batches.clear();
+}
+void track_graph(ibpm_batch* b, bool live) {
+    static bool registered = false;
+    std::lock_guard<std::mutex> lk(g_graph_mu);
+    if (live) {
+        if (!registered) {
+            std::atexit(destroy_live_graphs);
+            registered = true;
+        }
+        g_graph_batches.insert(b);
+    } else {
+        g_graph_batches.erase(b);
+    }
+}
+}  // namespace
+
 int ibpm_batch_destroy(ibpm_batch* b) {
     if (!b) return 0;
+    track_graph(b, false);
     (void)hipStreamSynchronize(b->ctx->c.stream);
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
     delete b;
@@ -3008,6 +3181,7 @@ int ibpm_batch_advance(ibpm_batch* B, int nsteps) {
                 enqueue_step_chol(B, s);
                 CK(hipStreamEndCapture(s, &g));
                 CK(hipGraphInstantiate(&B->graph, g, nullptr, nullptr, 0));
+                track_graph(B, true);
                 CK(hipGraphDestroy(g));
             }
             CK(hipGraphLaunch(B->graph, s));
@@ -3250,9 +3424,9 @@ int ibpm_batch_set_precond(ibpm_batch* B, int mode, int pairs) {
 }
 
 int ibpm_batch_set_precision(ibpm_batch* B, int mode) {
-    if (!B || mode < 0 || mode > 2) { set_error("precision mode %d: 0, 1 or 2", mode); return -1; }
+    if (!B || mode < 0 || mode > 4) { set_error("precision mode %d: 0..4", mode); return -1; }
     B->precision = mode;
-    const bool step32 = mode >= 1, fs32 = mode >= 2;
+    const bool step32 = mode >= 1, fs32 = mode == 2;
     for (int i = 0; i < B->nsub; ++i) {
         set_plan_fp32(B->S1[i], step32);
         set_plan_fp32(B->S2[i], step32);
@@ -3260,13 +3434,51 @@ int ibpm_batch_set_precision(ibpm_batch* B, int mode) {
         set_plan_fp32(B->M1[i], fs32);
     }
     set_plan_fp32(B->S4, step32);
+    if (mode >= 3) {   // the step's solves: FP32 radix or split-BF16 GEMM-DST per group
+        Ctx* c = &B->ctx->c;
+        for (int i = 0; i < B->nsub; ++i) {
+            CKR(tune_plan_gemm(c, B->mem, B->S1[i], mode));
+            CKR(tune_plan_gemm(c, B->mem, B->S2[i], mode));
+            CKR(tune_plan_gemm(c, B->mem, B->S3[i], mode));
+        }
+        CKR(tune_plan_gemm(c, B->mem, B->S4, mode));
+    }
     set_plan_fp32(B->M2, fs32);
     if (!B->M2L.steps.empty()) set_plan_fp32(B->M2L, fs32);
+    B->fs32 = fs32;
     if (B->graph) {   // the captured step holds the old kernels
         CK(hipGraphExecDestroy(B->graph));
         B->graph = nullptr;
     }
     return 0;
+}
+
+int ibpm_batch_precision_report(ibpm_batch* B, int* n, int* use_gemm, double* t_radix_us, double* t_gemm_us,
+                                int max) {
+    if (!B) { set_error("null batch"); return -1; }
+    std::map<int, const PassGroup*> seen;   // by N: the first group of the step's solves that was timed
+    auto visit = [&](const SolvePlan& P) {
+        for (auto& step : P.steps)
+            for (auto& G : step)
+                if (G.gemm_n && (!seen.count(G.gemm_n) || (seen[G.gemm_n]->t_gemm_us == 0 && G.t_gemm_us > 0)))
+                    seen[G.gemm_n] = &G;
+    };
+    for (int i = 0; i < B->nsub; ++i) {
+        visit(B->S1[i]);
+        visit(B->S2[i]);
+        visit(B->S3[i]);
+    }
+    visit(B->S4);
+    int k = 0;
+    for (auto& kv : seen) {
+        if (k >= max) break;
+        if (n) n[k] = kv.first;
+        if (use_gemm) use_gemm[k] = kv.second->gemm ? 1 : 0;
+        if (t_radix_us) t_radix_us[k] = kv.second->t_radix_us;
+        if (t_gemm_us) t_gemm_us[k] = kv.second->t_gemm_us;
+        ++k;
+    }
+    return k;
 }
 
 int ibpm_batch_set_local_helmholtz(ibpm_batch* B, int on) {

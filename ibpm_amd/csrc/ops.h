@@ -132,6 +132,11 @@ void launch_ell_pass32(int pass, const EllJob* d_jobs, int njobs, int max_blocks
 void launch_ell_pass(int pass, const EllJob* d_jobs, int njobs, int max_blocks, int lds_bytes, hipStream_t s,
                      bool force_solve, int M, bool regs);
 void launch_coarsify(const CoarsifyJob* d_jobs, int njobs, int max_rows, hipStream_t s, bool force_solve = false);
+// Split-BF16 GEMM-DST level solve of every job (square levels of n = N = nx - 1
+// <= 511 unknowns per side, planes in the jobs' T1 / T2): Sh, Sl the split sine
+// matrix of n with gemm_np(n) columns per row.
+constexpr int gemm_np_host(int n) { return (n + 63) / 64 * 64; }
+void launch_gemm_dst(const EllJob* d_jobs, int njobs, int n, const __bf16* Sh, const __bf16* Sl, hipStream_t s);
 // The nsweeps Jacobi sweeps of the local Helmholtz solves (HelmJob), several
 // per launch; the last launch writes each job's out.  max_tiles: helm_tiles
 // of the largest box.  timed_launch(fn) runs fn, the launch (a caller may

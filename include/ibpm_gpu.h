@@ -73,7 +73,8 @@ int ibpm_elliptic_solve_noalloc(ibpm_ctx* ctx, int kind, int nx, int ny, int ngr
    transposed intermediates with the right-hand side, boundary terms,
    eigenvalue scaling and output in FP64.  No counterpart in the reference
    (whose solve is FP64 FFTW, src/EllipticSolver2d.cc:57-104); for measuring
-   the reduced-precision path's error per solve. */
+   the reduced-precision path's error per solve.  2: the split-BF16 GEMM-DST
+   on MFMA (square grids of N = nx - 1 <= 511 only). */
 int ibpm_elliptic_solve_precision(ibpm_ctx* ctx, int kind, int nx, int ny, int ngrid, double dx,
                                   double alpha, const double* d_f, double* d_u, int precision);
 
@@ -356,6 +357,15 @@ int ibpm_batch_set_local_helmholtz(ibpm_batch* b, int on);
    residual test and the Cholesky factor stay FP64).  Each reduced-precision
    stage perturbs the FP64 answer by about the FP32 rounding of one solve. */
 int ibpm_batch_set_precision(ibpm_batch* b, int mode);
+/* Modes 3 and 4 of ibpm_batch_set_precision take, for the step's solves on
+   square levels of N <= 511, the dense sine-matrix GEMM-DST on MFMA
+   (split-BF16 products, FP32 accumulation): mode 4 wherever it applies, mode 3
+   where the library's autotuner, timing both on the batch's own jobs at the
+   call, finds it faster than the FP32 radix passes.  This reports per N the
+   choice (use_gemm 1) and the timed microseconds per solve of the group's
+   jobs (0 when not timed); returns the number of lengths written. */
+int ibpm_batch_precision_report(ibpm_batch* b, int* n, int* use_gemm, double* t_radix_us, double* t_gemm_us,
+                                int max);
 
 /* Measurement: advances nsteps with a HIP event pair around every kernel
    launch on the batch stream and returns per kernel class the launch count,

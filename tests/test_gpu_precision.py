@@ -80,3 +80,62 @@ def test_fp32_force_solve_moving_plate(gpu):
     print("fp32 plate_512x5: Cd %.2e Cl %.2e" % (cd, cl), flush=True)
     assert cd <= 1e-3 and cl <= 1e-3
     assert b.solver_stats()["fallbacks"] == 0
+
+
+@pytest.mark.parametrize("nx,ngrid", [(200, 1), (256, 3), (400, 4), (512, 5)])
+def test_gemm_dst_error_per_solve(gpu, nx, ngrid):
+    """The split-BF16 GEMM-DST on MFMA (precision 2) against the FP64 solve:
+    split products with FP32 accumulation, about 1e-5 relative L2."""
+    ctx = gpu.Context(0)
+    rng = np.random.default_rng(nx + 1)
+    dx = 4.0 / nx
+    f = rng.standard_normal(ngrid * (nx - 1) ** 2)
+    for kind, alpha in ((0, 0.0), (1, -1.3e-5)):
+        print("gemm solve nx=%d ngrid=%d kind=%d" % (nx, ngrid, kind), flush=True)
+        u64 = ctx.elliptic_solve_precision(kind, nx, nx, ngrid, dx, alpha, f, 0)
+        ug = ctx.elliptic_solve_precision(kind, nx, nx, ngrid, dx, alpha, f, 2)
+        err = np.linalg.norm(ug - u64) / np.linalg.norm(u64)
+        print("  rel L2 %.2e" % err, flush=True)
+        assert 1e-8 < err <= 5e-5
+
+
+def test_gemm_dst_only_for_square_grids_up_to_511(gpu):
+    ctx = gpu.Context(0)
+    f = np.zeros(1023 * 1023)
+    with pytest.raises(gpu.IbpmError, match="GEMM-DST"):
+        ctx.elliptic_solve_precision(0, 1024, 1024, 1, 4.0 / 1024, 0.0, f, 2)
+
+
+def test_autotuner_and_gemm_step_run(gpu):
+    """cylinder_200 (N = 199): the autotuner times both reduced-precision
+    transforms and keeps the faster; the GEMM-DST forced on the step's solves
+    runs 250 steps within 1e-2 of the oracle's Cd (2.5e-3 measured: its
+    per-solve error is about 1e-5), and switching back to FP64 restores the
+    FP64 run bitwise."""
+    g = np.load(os.path.join(GOLD, "cylinder_200.npz"))
+    flow = W.cylinder_200()[0]
+    ctx = gpu.Context(0)
+    b = gpu.Batch(ctx, [flow]).init().set_precision("reduced_tuned")
+    rep = b.precision_report()
+    print("autotuner:", rep, flush=True)
+    assert list(rep) == [199]
+    use_gemm, t_radix, t_gemm = rep[199]
+    assert t_radix > 0 and t_gemm > 0 and use_gemm == (t_gemm < t_radix)
+    b.close()
+    b = gpu.Batch(ctx, [flow]).init().use_graph(True).set_precision("gemm_step")
+    try:
+        assert b.precision_report()[199][0]
+        b.advance(int(g["nsteps"]))
+        h = b.force_history()
+    finally:
+        b.close()
+    cd = np.max(np.abs(h[:, 0] - g["forces"][:, 0]) / np.abs(g["forces"][:, 0]))
+    print("gemm_step cylinder_200: Cd %.2e" % cd, flush=True)
+    assert 1e-8 < cd <= 1e-2
+    ref = gpu.Batch(ctx, [flow]).init().use_graph(True)
+    ref.advance(10)
+    sw = gpu.Batch(ctx, [flow]).init().use_graph(True)
+    sw.advance(2)
+    sw.set_precision("gemm_step").set_precision("fp64")
+    sw.advance(8)
+    assert np.array_equal(sw.omega(), ref.omega())

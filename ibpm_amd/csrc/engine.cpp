@@ -632,6 +632,9 @@ struct Flow {
     double *e_w, *t_w;
     int4* node_nbr;
     int nnodes = 0, nold = 0;
+    // algorithmic bytes of one k_interp (without the constraint b) and one
+    // k_smear_curl launch over this flow's current lists (reg_bytes)
+    double by_interp = 0, by_smear = 0;
     std::vector<int> reg_old;      // node list of the previous association (cleared on the device)
     int reg_old_n = 0;
     HostStage stage;
@@ -872,6 +875,15 @@ static int upload_regularizer(ibpm_batch* B, Flow& F) {
     CKR(stage_upload(s, st, F.old_idx, old.data(), sizeof(int) * old.size()));
     F.nold = old_nodes;
     F.nnodes = (int)R.node_idx.size();
+    // k_interp: per E entry its flux index, weight and two psi values (28 B),
+    // per component the row pointer and the output (12 B); k_smear_curl: per
+    // node its four slots, scalar index and output (28 B), per slot used the
+    // row pointers (8 B) and per E^T entry its component, weight and f (20 B)
+    F.by_interp = 28.0 * (double)R.e_f.size() + 12.0 * (double)(2 * F.nb);
+    double sm = 28.0 * (double)F.nnodes;
+    for (int slot : R.node_nbr)
+        if (slot >= 0) sm += 8.0 + 20.0 * (double)(R.t_ptr[slot + 1] - R.t_ptr[slot]);
+    F.by_smear = sm;
     return upload_constraint(s, F);
 }
 
@@ -1494,12 +1506,25 @@ static int plan_local_helm(ibpm_batch* B, int i) {
     return 0;
 }
 
+// Algorithmic bytes of a k_interp (with_b: also the constraint b, 8 B per
+// component) or k_smear_curl launch over the flows of `active` (null: all).
+static double reg_bytes(const ibpm_batch* B, bool smear, bool with_b, const std::vector<char>* active = nullptr) {
+    double by = 0;
+    for (size_t k = 0; k < B->flows.size(); ++k) {
+        if (active && !(*active)[k]) continue;
+        const Flow& F = B->flows[k];
+        by += smear ? F.by_smear : F.by_interp + (with_b ? 16.0 * F.nb : 0.0);
+    }
+    return by;
+}
+
 // M f = C A^-1 (beta B f) for the vector behind d_smear (ProjectionSolver.cc:96-101).
 static void enqueue_M(ibpm_batch* B, int i, RegOps* smear, hipStream_t s,
                       const std::vector<char>* active = nullptr) {
     const int nfl = (int)B->flows.size();
     Probe* pr = B->probe;
-    timed(pr, K_SMEAR, 0, s, [&] { launch_smear_curl(smear, nfl, B->max_nodes, s); });
+    timed(pr, K_SMEAR, pr ? reg_bytes(B, true, false, active) : 0, s,
+          [&] { launch_smear_curl(smear, nfl, B->max_nodes, s); });
     if (B->helm_on[i]) {   // A^-1 by local sweeps (plan_local_helm), then L^-1 of their result
         double by = 0, cby = 0;
         if (pr)
@@ -1530,7 +1555,8 @@ static void enqueue_M(ibpm_batch* B, int i, RegOps* smear, hipStream_t s,
         run_solve(B->M1[i], s, pr, active);
         run_solve(B->M2, s, pr, active);
     }
-    timed(pr, K_INTERP, 0, s, [&] { launch_interp(B->d_interp_m, nfl, B->max_comp, s); });
+    timed(pr, K_INTERP, pr ? reg_bytes(B, false, false, active) : 0, s,
+          [&] { launch_interp(B->d_interp_m, nfl, B->max_comp, s); });
 }
 
 // Nonlinear term and right-hand side a of substep i (IBSolver.cc:144,169-183).
@@ -1563,9 +1589,14 @@ static void enqueue_substep_chol(ibpm_batch* B, int i, hipStream_t s) {
     enqueue_rhs(B, i, s);
     run_solve(B->S1[i], s, pr);                     // omega* = A^-1 a
     run_solve(B->S2[i], s, pr);                     // psi = L^-1(-omega*)
-    timed(pr, K_INTERP, 0, s, [&] { launch_interp(B->d_interp_c, nfl, B->max_comp, s); });   // rhs = C omega* - b
-    timed(pr, K_MINV, 0, s, [&] { launch_minv(B->d_minv[i], nfl, B->max_n, s); });          // f = M^-1 rhs
-    timed(pr, K_SMEAR, 0, s, [&] { launch_smear_curl(B->d_smear_f[i], nfl, B->max_nodes, s); });   // c = beta B f
+    timed(pr, K_INTERP, pr ? reg_bytes(B, false, true) : 0, s,
+          [&] { launch_interp(B->d_interp_c, nfl, B->max_comp, s); });   // rhs = C omega* - b
+    double by_minv = 0;   // the n x n inverse, the right-hand side and the output, FP64
+    if (pr)
+        for (auto& F : B->flows) by_minv += 8.0 * (4.0 * F.nb * F.nb + 4.0 * F.nb);
+    timed(pr, K_MINV, by_minv, s, [&] { launch_minv(B->d_minv[i], nfl, B->max_n, s); });   // f = M^-1 rhs
+    timed(pr, K_SMEAR, pr ? reg_bytes(B, true, false) : 0, s,
+          [&] { launch_smear_curl(B->d_smear_f[i], nfl, B->max_nodes, s); });   // c = beta B f
     run_solve(B->S3[i], s, pr);                     // omega = omega* - A^-1 c
     enqueue_refresh(B, s);                          // q = Curl psi + q0
 }
@@ -1767,7 +1798,8 @@ static int substep_cg(ibpm_batch* B, int i) {
     enqueue_rhs(B, i, s);
     run_solve(B->S1[i], s, pr);
     run_solve(B->S2[i], s, pr);
-    timed(pr, K_INTERP, 0, s, [&] { launch_interp(B->d_interp_c, nfl, B->max_comp, s); });
+    timed(pr, K_INTERP, pr ? reg_bytes(B, false, true) : 0, s,
+          [&] { launch_interp(B->d_interp_c, nfl, B->max_comp, s); });
     // x0: previous f (ProjectionSolver.cc:72 warm start), zero, or the
     // learned prediction; then r0 = rhs - M x0
     if (B->warm_mode == 0) {
@@ -1820,7 +1852,8 @@ static int substep_cg(ibpm_batch* B, int i) {
     }
     CK(hipStreamSynchronize(s));
     B->ksub += 1;
-    timed(pr, K_SMEAR, 0, s, [&] { launch_smear_curl(B->d_smear_f[i], nfl, B->max_nodes, s); });
+    timed(pr, K_SMEAR, pr ? reg_bytes(B, true, false) : 0, s,
+          [&] { launch_smear_curl(B->d_smear_f[i], nfl, B->max_nodes, s); });
     if (B->helm_on[i]) {
         // omega = omega* - A^-1 c with c = beta B f local like M's right-hand
         // side: the same sweeps (plan_local_helm), then the subtraction over

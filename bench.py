@@ -13,9 +13,9 @@ Moving bodies (the default workload, 16 tandem pitching/plunging plate cases
 at 1024x1024 with 6 levels): the warm-start network is first trained on a
 separate run of the workload's first steps; the timed form is then the
 batched, preconditioned solver with the learned warm start, from a zero
-initial vorticity (W warm-up steps, K timed, then --cycle-steps more that
-count preconditioner rebuilds, whose cost is reported amortised per step).
-Beside it the same steps are timed for the batched solver with the
+initial vorticity: W warm-up steps (the first builds the preconditioner),
+then K timed steps with every preconditioner rebuild that falls due inside
+them.  Beside it the same steps are timed for the batched solver with the
 reference's warm start, for batched plain CG, and for the plain GPU form
 (one flow, plain CG, every A^-1 by the DST solve: the reference's solver).
 Flow 0's Cd/Cl at every step is compared with the FP64 oracle's fixture
@@ -31,7 +31,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
-DEFAULT_STEPS = {"cylinder_200": 500, "cylinder_400x4": 200, "mixed_64x3": 20, "plate_512x5": 40, "tandem_16x1024x6": 40}
+DEFAULT_STEPS = {"cylinder_200": 500, "cylinder_400x4": 200, "mixed_64x3": 20, "plate_512x5": 150,
+                 "tandem_16x1024x6": 150}
 METRIC = "grid-pt\u00b7steps/s learned+batched vs plain GPU at Cd/Cl err\u22641e-6; CG iters/step"
 CONFIG_TEXT = {
     "cylinder_200": "stationary cylinder D=1 (157 pts), 200x200, 1 level, Re 100, dt 0.02, Cholesky",
@@ -177,26 +178,25 @@ def accuracy_vs_oracle(b, workload):
 
 
 def moving_form(ctx, flows, warmup, steps, precond="lowrank", pairs=8, local=True, warm="previous",
-                nn=None, cycle_steps=0):
+                nn=None, build="finest", rebuild_cost=0.45):
     """One form of the moving-body solver on `flows` from a zero initial
     vorticity: warmup steps (the first builds the preconditioner), then
-    `steps` timed steps, then (cycle_steps > 0) more steps that count
-    preconditioner rebuilds.  Returns the form's figures and the batch."""
+    `steps` timed steps, with the preconditioner rebuilds that fall due inside
+    them.  build / rebuild_cost: how P is built (Batch.set_precond_build).
+    Returns the form's figures and the batch."""
     import ibpm_amd
     from ibpm_amd.workloads import grid_points
     b = ibpm_amd.Batch(ctx, flows).init()
     b.set_precond(precond, pairs).set_local_helmholtz(local)
+    if precond != "none":
+        b.set_precond_build(build, rebuild_cost)
     if nn is not None:
         b.nn_params(nn)
     if warm != "previous":
         b.set_warm_start(warm)
     sync = ibpm_amd.load().ibpm_ctx_synchronize
     sync(ctx.h)
-    t0 = time.perf_counter()
-    b.advance(1)
-    sync(ctx.h)
-    t_first = time.perf_counter() - t0
-    b.advance(max(0, warmup - 1))
+    b.advance(max(1, warmup))
     st0 = b.solver_stats()
     sec, its = timed_steps(b, len(flows), steps, True)
     st1 = b.solver_stats()
@@ -206,31 +206,28 @@ def moving_form(ctx, flows, warmup, steps, precond="lowrank", pairs=8, local=Tru
     out = {"flows": len(flows), "precond": precond, "pairs": pairs if precond == "lowrank" else 0,
            "local_helmholtz": local, "warm_start": warm,
            "value": gp * steps / sec, "ms_per_step": ms, "cg_iters_per_step": its,
-           "batched_m_applications_per_step": (d["m_applications"] + d["build_m_applications"]) / steps,
-           "builds_in_timed_steps": d["builds"], "fallbacks": d["fallbacks"],
+           "batched_m_applications_per_step": d["m_applications"] / steps, "fallbacks": d["fallbacks"],
            "host_ms_per_step_kinematics": round(d["host_us_kinematics"] / steps / 1e3, 3),
            "host_ms_per_step_polls": round(d["host_us_polls"] / steps / 1e3, 3)}
     if precond != "none":
-        # the first step holds the first preconditioner build
-        t_build = max(0.0, t_first - ms / 1e3)
-        if cycle_steps:
-            b.advance(cycle_steps)
-        st2 = b.solver_stats()
-        done = warmup + steps + cycle_steps
-        per_step = st2["builds"] / done
+        nb = d["builds"]
         out["preconditioner"] = {
-            "build_s": round(t_build, 3), "builds_since_init": st2["builds"], "steps_since_init": done,
-            "builds_per_step": per_step,
-            "value_with_builds_amortised": gp / (ms / 1e3 + per_step * t_build)}
-    log("form %s/%s/%s warm start, %d flows: %.1f ms/step, %.1f CG iterations/step"
-        % (precond, "local" if local else "dst", warm, len(flows), ms, its or 0.0))
+            "build": build, "rebuild_cost": rebuild_cost,
+            "first_build_s": round(st0["build_us"] / 1e6 / max(1, st0["builds"]), 3),
+            "builds_in_timed_steps": nb,
+            "build_s_each": round(d["build_us"] / 1e6 / nb, 3) if nb else None,
+            "build_host_s_each": round(d["build_host_us"] / 1e6 / nb, 3) if nb else None,
+            "build_m_applications_per_step": d["build_m_applications"] / steps,
+            "share_of_timed_s": round(d["build_us"] / 1e6 / sec, 4)}
+    log("form %s/%s/%s warm start, %d flows: %.1f ms/step over %d steps (%d rebuilds), %.1f CG iterations/step"
+        % (precond, "local" if local else "dst", warm, len(flows), ms, steps, d["builds"], its or 0.0))
     return out, b
 
 
 def plain_form(ctx, flow, warmup, steps):
     """The plain GPU form on one flow: plain CG from the previous force with
     every A^-1 of M by the DST solve (the reference's solver), no batching."""
-    out, b = moving_form(ctx, [flow], warmup, steps, precond="none", local=False)
+    out, b = moving_form(ctx, [flow], warmup, steps, precond="none", local=False, build=None)
     b.close()
     return out
 
@@ -272,8 +269,11 @@ def main():
                     help="CG start of the timed moving-body form (previous = the reference's)")
     ap.add_argument("--train-steps", type=int, default=30,
                     help="steps of the training run of the learned warm start")
-    ap.add_argument("--cycle-steps", type=int, default=150,
-                    help="steps after the timed ones that count preconditioner rebuilds")
+    ap.add_argument("--precond-build", default="finest", choices=["all", "finest"],
+                    help="P of the preconditioned forms from M itself or from its finest-level form")
+    ap.add_argument("--rebuild-cost", type=float, default=0.45,
+                    help="a build's cost in M applications per unknown, the rebuild rule's break-even "
+                         "(measured: 0.42 for the finest-level build of the tandem batch)")
     ap.add_argument("--no-forms", action="store_true",
                     help="moving bodies: time only the headline form (no plain, batched-plain or "
                          "previous-start forms beside it)")
@@ -296,7 +296,7 @@ def main():
             nn, learn = train_network(ctx, flows, args.train_steps)
         t_i0 = time.perf_counter()
         head, b = moving_form(ctx, flows, args.warmup, steps, args.precond, args.pairs, True, args.warm,
-                              nn, args.cycle_steps)
+                              nn, args.precond_build, args.rebuild_cost)
         t_init = time.perf_counter() - t_i0
         value, sec, cg = head["value"], head["ms_per_step"] * steps / 1e3, head["cg_iters_per_step"]
         accuracy = accuracy_vs_oracle(b, args.workload)
@@ -306,13 +306,14 @@ def main():
         if not args.no_forms:
             forms = {}
             if args.warm != "previous":
-                prev, bp = moving_form(ctx, flows, args.warmup, steps, args.precond, args.pairs, True)
+                prev, bp = moving_form(ctx, flows, args.warmup, steps, args.precond, args.pairs, True,
+                                       build=args.precond_build, rebuild_cost=args.rebuild_cost)
                 bp.close()
                 forms["batched_preconditioned_previous_start"] = prev
                 if learn is not None:
                     learn["cg_iters_per_step_previous_start"] = prev["cg_iters_per_step"]
                     learn["cg_iters_per_step_learned_start"] = cg
-            bpl, bb = moving_form(ctx, flows, 1, 3, precond="none", local=False)
+            bpl, bb = moving_form(ctx, flows, 1, 3, precond="none", local=False, build=None)
             bb.close()
             forms["batched_plain_cg"] = bpl
             plain = plain_form(ctx, flows[0], args.warmup, steps)
@@ -355,7 +356,9 @@ def main():
         "data": "synthetic (configured geometry and parameters, zero initial vorticity)",
         "config": {"workload": args.workload, "description": CONFIG_TEXT[args.workload],
                    "flows": len(flows), "grid_points": gp, "parallelism": "single GPU",
-                   "form": ("batched, preconditioned, %s warm start" % args.warm) if moving else "graph replay",
+                   "form": ("batched, preconditioned (P from %s), %s warm start, rebuilds inside the timed steps"
+                            % ("M" if args.precond_build == "all" else "M's finest-level form", args.warm))
+                           if moving else "graph replay",
                    "solver": solver, "learned": learn, "forms": forms,
                    "accuracy_vs_oracle_fixture": accuracy,
                    "init_s": round(t_init, 3), "Cd_flow0": cd, "Cl_flow0": cl},

@@ -99,6 +99,7 @@ def load():
         "ibpm_batch_nn_params": [vp, vp, vp],
         "ibpm_nn_loss_grad": [vp, vp, vp, vp, ctypes.c_long, vp, _D],
         "ibpm_batch_set_precond": [vp, ctypes.c_int, ctypes.c_int],
+        "ibpm_batch_set_precond_build": [vp, ctypes.c_int, ctypes.c_double],
         "ibpm_batch_set_local_helmholtz": [vp, ctypes.c_int],
         "ibpm_batch_cholesky_cache": [vp, ctypes.c_int, ctypes.c_char_p],
         "ibpm_batch_cholesky_loaded": [vp, ctypes.c_int],
@@ -476,6 +477,15 @@ class Batch:
         _check(load().ibpm_batch_set_precond(self.h, m, k))
         return self
 
+    def set_precond_build(self, levels="all", rebuild_cost=1.0):
+        """How the preconditioner's P is built: from M ("all" levels of the
+        L^-1 solve, the default) or from M's finest-level form ("finest": L^-1
+        on level 0 under zero boundary values, a fraction of the cost per
+        column; needs the local Helmholtz path).  rebuild_cost: a build's cost
+        in M applications per unknown, the rebuild rule's break-even."""
+        _check(load().ibpm_batch_set_precond_build(self.h, {"all": 0, "finest": 1}[levels], float(rebuild_cost)))
+        return self
+
     PRECISION = {"fp64": 0, "fp32_step": 1, "fp32": 2, "reduced_tuned": 3, "gemm_step": 4}
 
     def precision_report(self):
@@ -507,7 +517,8 @@ class Batch:
         return self
 
     SOLVER_STATS = ("iterations", "solves", "m_applications", "build_m_applications", "builds",
-                    "fallbacks", "abandoned", "host_us_kinematics", "host_us_polls", "local_helmholtz_m_applications")
+                    "fallbacks", "abandoned", "host_us_kinematics", "host_us_polls", "local_helmholtz_m_applications",
+                    "build_us", "build_host_us")
 
     def solver_stats(self):
         out = np.zeros(len(self.SOLVER_STATS), dtype=np.int64)

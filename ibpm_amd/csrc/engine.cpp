@@ -731,6 +731,7 @@ struct ibpm_batch {
     double *data_x = nullptr, *data_y = nullptr;
     long data_cap = 0, data_n = 0;
     ibpmg::NnOps* d_nn = nullptr;
+    ibpmg::NnOps* d_nn_inf = nullptr;   // nn_infer_table
     // force-solve state of every flow, 8 doubles each: [rho or delta, tol^2,
     // iterations, converged, broken down, pairs in use, r.r, spare]
     double* cg_states = nullptr;
@@ -741,12 +742,26 @@ struct ibpm_batch {
     int pc_base[4] = {-1, -1, -1, -1};   // iterations of the first solve after a build
     int pc_last[4] = {0, 0, 0, 0};
     double pc_excess = 0.0;              // iterations above pc_base since the build
+    // P from M (0) or from its finest-level form (1): M with L^-1 on level 0
+    // alone, zero boundary values (plan M1L, build_inverse)
+    int pc_build_levels = 0;
+    double pc_build_cost = 1.0;          // a build's cost in M applications per unknown (rebuild rule)
+    ibpmg::SolvePlan M1L;
+    bool m1l_built = false;
+    // wall time of the last build and of a PCG iteration (moving average):
+    // the rebuild rule's costs
+    double pc_t_build = 0.0, pc_t_iter = 0.0;
+    // build_inverse: interpolation descriptors of every column, unit vectors
+    ibpmg::RegOps* d_interp_cols = nullptr;
+    size_t interp_cols_cap = 0;
+    ibpmg::UnitOps* d_units = nullptr;
     ibpmg::PcgOps* d_pcg[4][2] = {};
     ibpmg::MinvOps* d_pminv[4] = {};
     // counters: iterations, solves, M applications in solves, M applications
     // in preconditioner builds, builds, fallbacks to plain CG
     long st_iters = 0, st_solves = 0, st_mapps = 0, st_build_mapps = 0, st_builds = 0, st_fallbacks = 0;
     double st_host_kin_s = 0, st_host_poll_s = 0;   // host time: kinematics + regulariser, solver polls
+    double st_build_s = 0, st_build_host_s = 0;     // preconditioner builds: wall time, host factorisation
 };
 
 namespace ibpmg {
@@ -1519,8 +1534,12 @@ static double reg_bytes(const ibpm_batch* B, bool smear, bool with_b, const std:
 }
 
 // M f = C A^-1 (beta B f) for the vector behind d_smear (ProjectionSolver.cc:96-101).
+// interp: the interpolation's descriptors (null: B->d_interp_m); single_level:
+// L^-1 on the finest level alone (plan M1L: the preconditioner's
+// approximate M), local A^-1 path only.
 static void enqueue_M(ibpm_batch* B, int i, RegOps* smear, hipStream_t s,
-                      const std::vector<char>* active = nullptr) {
+                      const std::vector<char>* active = nullptr, const RegOps* interp = nullptr,
+                      bool single_level = false) {
     const int nfl = (int)B->flows.size();
     Probe* pr = B->probe;
     timed(pr, K_SMEAR, pr ? reg_bytes(B, true, false, active) : 0, s,
@@ -1533,14 +1552,16 @@ static void enqueue_M(ibpm_batch* B, int i, RegOps* smear, hipStream_t s,
                     by += B->helm_bytes[j];
                     cby += B->helm_cbytes[j];
                 }
-        for (int lev = 1; lev < B->M1[i].nlev; ++lev)
+        for (int lev = 1; lev < (single_level ? 1 : B->M1[i].nlev); ++lev)
             timed(pr, K_FSCOARSIFY, cby / std::max(1, B->M1[i].nlev - 1), s,
                   [&] { launch_helm_coarsify(B->d_helm, B->helm_n, B->helm_cblocks, lev, s); });
         const int nl = helm_launches(B->helm_sweeps);
         launch_helm_sweeps(B->d_helm, B->helm_n, B->helm_blocks, B->helm_sweeps, s,
                            [&](const std::function<void()>& fn) { timed(pr, K_FSHELM, by / nl, s, fn); });
         const bool boxed = B->m2l_boxed && !B->M2L.fp32;   // the boxed chain feeds register-held passes only
-        if (boxed) {
+        if (single_level) {   // levels >= 1 of the sweeps' result are not read
+            run_solve(B->M1L, s, pr, active, false);
+        } else if (boxed) {
             double cb = 0;
             if (pr)
                 for (size_t j = 0; j < B->m2lc_flow.size(); ++j)
@@ -1549,14 +1570,14 @@ static void enqueue_M(ibpm_batch* B, int i, RegOps* smear, hipStream_t s,
                 timed(pr, K_FSCOARSIFY, cb / std::max(1, B->M2L.nlev - 1), s,
                       [&] { launch_helm_coarsify(B->d_m2lc, B->m2lc_n, B->m2lc_blocks, lev, s); });
         }
-        run_solve(B->M2L, s, pr, active, !boxed);
+        if (!single_level) run_solve(B->M2L, s, pr, active, !boxed);
         B->st_local_mapps += 1;
     } else {
         run_solve(B->M1[i], s, pr, active);
         run_solve(B->M2, s, pr, active);
     }
     timed(pr, K_INTERP, pr ? reg_bytes(B, false, false, active) : 0, s,
-          [&] { launch_interp(B->d_interp_m, nfl, B->max_comp, s); });
+          [&] { launch_interp(interp ? interp : B->d_interp_m, nfl, B->max_comp, s); });
 }
 
 // Nonlinear term and right-hand side a of substep i (IBSolver.cc:144,169-183).
@@ -1609,29 +1630,51 @@ static int enqueue_step_chol(ibpm_batch* B, hipStream_t s) {
 
 // Descriptor of the learned warm start for substep k = B->ksub (target: the
 // final force of the substep, for sample collection).
-static int upload_nn(ibpm_batch* B, int i, bool with_target) {
-    const int nfl = (int)B->flows.size();
+// The descriptors of every flow for substep k, RK substep i (ring slots
+// relative to k, whose position in the ring alone matters).
+static void fill_nn(const ibpm_batch* B, long k, int i, NnOps* ops) {
     const int S = B->nsub, R = B->ring;
-    const long k = B->ksub;
     auto slot = [&](const Flow& F, long j) { return F.ring_f + (size_t)(((j % R) + R) % R) * (2 * F.nb + 1); };
     auto bslot = [&](const Flow& F, long j) { return F.ring_b + (size_t)(((j % R) + R) % R) * (2 * F.nb + 1); };
-    std::vector<NnOps> ops(nfl);
-    long off = B->data_n;
-    for (int q = 0; q < nfl; ++q) {
-        Flow& F = B->flows[q];
+    for (size_t q = 0; q < B->flows.size(); ++q) {
+        const Flow& F = B->flows[q];
         NnOps& o = ops[q];
+        o = NnOps{};
         o.nb = F.nb; o.substep = i;
         o.fa = slot(F, k - 1); o.fb = slot(F, k - S); o.fc = slot(F, k - S - 1);
         o.fd = slot(F, k - 2 * S); o.fe = slot(F, k - 2 * S - 1);
         o.b_now = F.b; o.b_prev = bslot(F, k - 1);
         o.X = F.nn_x; o.scale = F.nn_scale; o.params = B->nn_params; o.x0 = F.f;
-        if (with_target) {
-            o.target = F.f; o.data_x = B->data_x; o.data_y = B->data_y; o.data_off = off;
-            off += F.nb;
-        }
     }
+}
+
+static int upload_nn(ibpm_batch* B, int i, bool with_target) {
+    const int nfl = (int)B->flows.size();
+    std::vector<NnOps> ops(nfl);
+    fill_nn(B, B->ksub, i, ops.data());
+    long off = B->data_n;
+    if (with_target)
+        for (int q = 0; q < nfl; ++q) {
+            NnOps& o = ops[q];
+            o.target = B->flows[q].f; o.data_x = B->data_x; o.data_y = B->data_y; o.data_off = off;
+            off += B->flows[q].nb;
+        }
     CKR(upload(B->d_nn, ops.data(), sizeof(NnOps) * nfl));
     if (with_target) B->data_n = off;
+    return 0;
+}
+
+// Inference descriptors of every (ring position, RK substep), uploaded once,
+// so that the learned warm start launches without a host round trip:
+// entry (k mod ring) nsub + i for substep k.
+static int nn_infer_table(ibpm_batch* B) {
+    if (B->d_nn_inf) return 0;
+    const int nfl = (int)B->flows.size();
+    std::vector<NnOps> all((size_t)B->ring * B->nsub * nfl);
+    for (int p = 0; p < B->ring; ++p)
+        for (int i = 0; i < B->nsub; ++i) fill_nn(B, p, i, all.data() + ((size_t)p * B->nsub + i) * nfl);
+    CKR(B->mem.alloc(&B->d_nn_inf, all.size()));
+    CKR(upload(B->d_nn_inf, all.data(), sizeof(NnOps) * all.size()));
     return 0;
 }
 
@@ -1805,12 +1848,12 @@ static int substep_cg(ibpm_batch* B, int i) {
     if (B->warm_mode == 0) {
         for (auto& F : B->flows) CK(hipMemsetAsync(F.f, 0, sizeof(double) * 2 * F.nb, s));
     } else if (B->warm_mode == 2 && B->nn_ready && nn_history_ready(B)) {
-        CK(hipStreamSynchronize(s));
-        CKR(upload_nn(B, i, false));
+        CKR(nn_infer_table(B));
+        const NnOps* d = B->d_nn_inf + ((size_t)(B->ksub % B->ring) * B->nsub + i) * nfl;
         int maxnb = 1;
         for (auto& F : B->flows) maxnb = std::max(maxnb, F.nb);
-        launch_nn_features(B->d_nn, nfl, s);
-        launch_nn_infer(B->d_nn, nfl, maxnb, s);
+        launch_nn_features(d, nfl, s);
+        launch_nn_infer(d, nfl, maxnb, s);
     }
     std::vector<int> its(nfl, 0);
     if (B->pc_mode && !B->pc_broken && !B->pc_valid) CKR(build_precond(B, i));
@@ -1818,13 +1861,14 @@ static int substep_cg(ibpm_batch* B, int i) {
         bool fallback = false;
         CKR(solve_pcg(B, i, its.data(), &fallback));
         // rebuild P once the iterations it costs above its first solves
-        // exceed what a rebuild costs (one M application per column)
+        // exceed what a rebuild costs: one M application per column, times
+        // pc_build_cost (below 1 for the finest-level build)
         int itmax = 0;
         for (int k = 0; k < nfl; ++k) itmax = std::max(itmax, its[k]);
         B->pc_last[i] = itmax;
         if (B->pc_base[i] < 0) B->pc_base[i] = itmax;
         else B->pc_excess += std::max(0, itmax - B->pc_base[i]);
-        if (B->pc_excess > B->max_n) B->pc_valid = false;
+        if (B->pc_excess > B->max_n * B->pc_build_cost) B->pc_valid = false;
         if (fallback) {
             B->st_fallbacks += 1;
             CKR(solve_cg(B, i, its.data()));
@@ -1980,7 +2024,6 @@ static int build_inverse(ibpm_batch* B, int i, bool precond) {
     const int nfl = (int)B->flows.size();
     int nmax = 0;
     for (auto& F : B->flows) nmax = std::max(nmax, 2 * F.nb);
-    static const double one = 1.0, zero = 0.0;
     // factors of flows with a cache file that matches (Cholesky force solve only)
     std::vector<std::vector<double>> lower(nfl), diag(nfl);
     std::vector<char> cached(nfl, 0);
@@ -1993,27 +2036,58 @@ static int build_inverse(ibpm_batch* B, int i, bool precond) {
         if (cached[k]) F.chol_loaded += 1;
         else all_cached = false;
     }
-    for (int j = 0; j < (all_cached ? 0 : nmax); ++j) {
-        std::vector<RegOps> sm(nfl), ip(nfl);
+    // the preconditioner from M's finest-level form (pc_build_levels 1)
+    const bool single = precond && B->pc_build_levels == 1 && B->helm_on[i];
+    if (single && !B->m1l_built) {
+        std::vector<SolveSpec> sp;
+        for (auto& F : B->flows) {
+            SolveSpec q = spec_of(F, 0, 0.0, F.m1, F.psi, -1.0, 0, nullptr);
+            q.L = 1;            // level 0 alone: zero boundary values, no coarsify
+            q.R = nullptr;
+            q.skip = F.cg_state + 3;
+            sp.push_back(q);
+        }
+        B->M1L.force_solve = true;
+        CKR(build_solve_plan(&B->ctx->c, B->mem, sp, B->M1L));
+        B->m1l_built = true;
+    }
+    if (!all_cached) {
+        // one smear descriptor per flow (the unit vector moves in place), one
+        // interpolation descriptor per column and flow (column j of M into
+        // mcol; column n is a dummy), so that the columns run without host
+        // round trips
+        std::vector<RegOps> sm(nfl), ip((size_t)(nmax + 1) * nfl);
+        std::vector<UnitOps> uo(nfl);
         for (int k = 0; k < nfl; ++k) {
             Flow& F = B->flows[k];
             const int n = 2 * F.nb;
             sm[k] = make_regops(F, beta_of(B, F, i));
             sm[k].fin = F.unit; sm[k].cout = F.c;
-            ip[k] = make_regops(F, 0);
-            ip[k].psi = F.psi; ip[k].b = nullptr;
-            ip[k].out = F.mcol + (size_t)std::min(j, n) * n;   // column n is a dummy
-            if (j < n) {
-                if (j > 0) CK(hipMemcpyAsync(F.unit + j - 1, &zero, sizeof(double), hipMemcpyHostToDevice, s));
-                CK(hipMemcpyAsync(F.unit + j, &one, sizeof(double), hipMemcpyHostToDevice, s));
+            for (int j = 0; j <= nmax; ++j) {
+                RegOps& r = ip[(size_t)j * nfl + k];
+                r = make_regops(F, 0);
+                r.psi = F.psi; r.b = nullptr;
+                r.out = F.mcol + (size_t)std::min(j, n) * n;
             }
+            uo[k] = UnitOps{F.unit, n};
+            CK(hipMemsetAsync(F.unit, 0, sizeof(double) * (n + 1), s));
         }
+        if (ip.size() > B->interp_cols_cap) {
+            CKR(B->mem.alloc(&B->d_interp_cols, ip.size()));
+            B->interp_cols_cap = ip.size();
+        }
+        if (!B->d_units) CKR(B->mem.alloc(&B->d_units, (size_t)nfl));
         CK(hipStreamSynchronize(s));
         CKR(upload(B->d_smear_d[i], sm.data(), sizeof(RegOps) * nfl));
-        CKR(upload(B->d_interp_m, ip.data(), sizeof(RegOps) * nfl));
-        enqueue_M(B, i, B->d_smear_d[i], s);
+        CKR(upload(B->d_interp_cols, ip.data(), sizeof(RegOps) * ip.size()));
+        CKR(upload(B->d_units, uo.data(), sizeof(UnitOps) * nfl));
+        for (int j = 0; j < nmax; ++j) {
+            launch_unit_column(B->d_units, nfl, j, s);
+            enqueue_M(B, i, B->d_smear_d[i], s, nullptr, B->d_interp_cols + (size_t)j * nfl, single);
+        }
     }
     CK(hipStreamSynchronize(s));
+    const double t_host = now_s();
     // factorisation and inverse assembly on the host, threads over flows
     std::vector<std::vector<double>> minv_h(nfl);
     std::vector<int> bad(nfl, -1);
@@ -2045,6 +2119,7 @@ static int build_inverse(ibpm_batch* B, int i, bool precond) {
             }
         });
     for (auto& x : th) x.join();
+    B->st_build_host_s += now_s() - t_host;
     for (int k = 0; k < nfl; ++k)
         if (!saved[k]) {
             set_error("could not write %s", cholesky_path(B->flows[k].chol_cache, i).c_str());
@@ -2085,8 +2160,11 @@ static int build_cholesky(ibpm_batch* B) {
 // for the batch (plain CG from then on).
 static int build_precond(ibpm_batch* B, int i) {
     const int nfl = (int)B->flows.size();
+    const double t0 = now_s();
     CKR(reset_states(B));   // the M applications below must not be skipped
     const int rc = build_inverse(B, i, true);
+    B->pc_t_build = now_s() - t0;
+    B->st_build_s += B->pc_t_build;
     B->st_builds += 1;
     B->st_build_mapps += B->max_n;
     if (rc == -5) {
@@ -3456,6 +3534,20 @@ int ibpm_batch_set_precond(ibpm_batch* B, int mode, int pairs) {
     return 0;
 }
 
+int ibpm_batch_set_precond_build(ibpm_batch* B, int levels, double rebuild_cost) {
+    if (!B) { set_error("null batch"); return -1; }
+    if (levels < 0 || levels > 1) { set_error("preconditioner build levels must be 0 (all) or 1 (finest)"); return -1; }
+    if (!(rebuild_cost > 0.0)) { set_error("rebuild cost must be positive"); return -1; }
+    if (levels == 1 && !B->helm_allowed) {
+        set_error("the finest-level build needs the local Helmholtz path of the force solve");
+        return -1;
+    }
+    B->pc_build_levels = levels;
+    B->pc_build_cost = rebuild_cost;
+    B->pc_valid = false;   // the next solve builds P the chosen way
+    return 0;
+}
+
 int ibpm_batch_set_precision(ibpm_batch* B, int mode) {
     if (!B || mode < 0 || mode > 4) { set_error("precision mode %d: 0..4", mode); return -1; }
     B->precision = mode;
@@ -3524,10 +3616,11 @@ int ibpm_batch_set_local_helmholtz(ibpm_batch* B, int on) {
 
 int ibpm_batch_solver_stats(ibpm_batch* B, long* stats, int nstats) {
     if (!B || !stats) { set_error("null argument"); return -1; }
-    const long v[10] = {B->st_iters, B->st_solves, B->st_mapps, B->st_build_mapps, B->st_builds,
+    const long v[12] = {B->st_iters, B->st_solves, B->st_mapps, B->st_build_mapps, B->st_builds,
                         B->st_fallbacks, B->pc_broken ? 1L : 0L, (long)(1e6 * B->st_host_kin_s),
-                        (long)(1e6 * B->st_host_poll_s), B->st_local_mapps};
-    const int n = std::min(nstats, 10);
+                        (long)(1e6 * B->st_host_poll_s), B->st_local_mapps, (long)(1e6 * B->st_build_s),
+                        (long)(1e6 * B->st_build_host_s)};
+    const int n = std::min(nstats, 12);
     for (int k = 0; k < n; ++k) stats[k] = v[k];
     return n;
 }

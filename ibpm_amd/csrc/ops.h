@@ -113,6 +113,13 @@ struct PcgOps {
     double* c;                     // kLmpMax: G S^T r of the current residual
 };
 
+// The unit vector of one flow's preconditioner build (n unknowns, one spare
+// slot): column j moves the 1 from j - 1 to j (no 1 for j >= n).
+struct UnitOps {
+    double* u;
+    int n;
+};
+
 struct ForceOps {
     int nb;
     const double* f;
@@ -169,6 +176,7 @@ void launch_pcg_begin(const PcgOps* d, int nflows, hipStream_t s);
 void launch_pcg_step(const PcgOps* d, int nflows, hipStream_t s);
 void launch_pcg_dir(const PcgOps* d, int nflows, int begin, hipStream_t s);
 void launch_forces(const ForceOps* d, int nflows, hipStream_t s);
+void launch_unit_column(const UnitOps* d, int nflows, int j, hipStream_t s);
 void launch_reg_to_flux(const RegFluxOps* d, int nslots, const double* f, double* q, hipStream_t s);
 void launch_reg_to_boundary(const RegFluxOps* d, int nb, const double* q, double* f, hipStream_t s);
 

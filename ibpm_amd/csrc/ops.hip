@@ -966,6 +966,19 @@ void launch_rhs(const FlowOps* d, int nflows, long max_pts, hipStream_t s) {
 void launch_curl_q(const FlowOps* d, int nflows, long max_nf_all, hipStream_t s) {
     hipLaunchKernelGGL(k_curl_q, dim3(blocks_for((max_nf_all + 1) / 2), nflows), dim3(kThreads), 0, s, d);
 }
+// One thread per flow: e_{j-1} -> e_j.
+__global__ void k_unit_column(const UnitOps* __restrict__ ops, int nflows, int j) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nflows) return;
+    const UnitOps& o = ops[k];
+    if (j > 0 && j - 1 < o.n) o.u[j - 1] = 0.0;
+    if (j < o.n) o.u[j] = 1.0;
+}
+
+void launch_unit_column(const UnitOps* d, int nflows, int j, hipStream_t s) {
+    hipLaunchKernelGGL(k_unit_column, dim3((nflows + 63) / 64), dim3(64), 0, s, d, nflows, j);
+}
+
 void launch_interp(const RegOps* d, int nflows, int max_comp, hipStream_t s) {
     hipLaunchKernelGGL(k_interp, dim3(blocks_for(max_comp), nflows), dim3(kThreads), 0, s, d);
 }

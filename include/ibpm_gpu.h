@@ -324,6 +324,15 @@ int ibpm_nn_loss_grad(ibpm_ctx* ctx, const double* h_params, const double* h_x, 
    plain CG finishes any solve in which the preconditioner breaks down, so the
    answer is the plain solver's to within its tolerance. */
 int ibpm_batch_set_precond(ibpm_batch* b, int mode, int pairs);
+/* How P is built (no counterpart in the reference).  levels 0: from M itself.
+   levels 1: from M's finest-level form, E L0^-1 A0^-1 (beta B) with L0^-1 the
+   level-0 Poisson solve under zero boundary values, symmetric positive
+   definite like M and a fraction of its cost per column (needs the local
+   Helmholtz path, ibpm_batch_set_local_helmholtz; without it P comes from M).
+   P only steers the iterations: the solves still end on M's own residual.
+   rebuild_cost: a build's cost in M applications per unknown, which sets the
+   rebuild rule's break-even (1 for levels 0).  Marks P for a rebuild. */
+int ibpm_batch_set_precond_build(ibpm_batch* b, int levels, double rebuild_cost);
 /* Force-solve counters since init: [0] iterations summed over flows and
    substeps, [1] solves (flow-substeps), [2] batched M applications in solves,
    [3] batched M applications spent building preconditioners, [4] builds,
@@ -331,7 +340,9 @@ int ibpm_batch_set_precond(ibpm_batch* b, int mode, int pairs);
    positive definite and preconditioning stopped, [7] host microseconds in the
    per-substep kinematics and regulariser updates (GPU idle), [8] host
    microseconds in solver polls (stream synchronisation and state read-back),
-   [9] batched M applications that took the local Helmholtz path.
+   [9] batched M applications that took the local Helmholtz path,
+   [10] wall microseconds in preconditioner builds, [11] of those, host
+   microseconds in the factorisations and inverse assemblies.
    Returns the count written. */
 int ibpm_batch_solver_stats(ibpm_batch* b, long* stats, int nstats);
 /* No counterpart in the reference (its M application and its step solve

@@ -6,7 +6,8 @@ flows (tests/golden/make_golden.py):
     flow of each grid size (256, 384, 512) and body (cylinder, ellipse,
     plate) is checked;
   * tandem_16x1024x6: the 16 moving tandem-plate cases (preconditioned CG
-    with 8 pairs, local Helmholtz solves, restricted L^-1 passes) for 30
+    with 8 pairs and P from M's finest-level form, local Helmholtz solves,
+    restricted L^-1 passes) for 30
     steps; case 0 is checked, with the reference's warm start and with the
     learned one.
 
@@ -69,7 +70,7 @@ def test_mixed_batch_matches_oracle_fixtures(gpu):
 
 
 def _tandem(gpu, ctx, flows, nsteps, nn=None):
-    b = gpu.Batch(ctx, flows).init().set_precond("lowrank", 8)
+    b = gpu.Batch(ctx, flows).init().set_precond("lowrank", 8).set_precond_build("finest", 0.45)
     if nn is not None:
         b.nn_params(nn)
         b.set_warm_start("learned")

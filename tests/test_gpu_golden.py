@@ -145,16 +145,19 @@ def test_ibpm_driver_unsteady_base_flow(gpu, oracle):
 
 @pytest.mark.parametrize("name", ["plate", "plate_512x5_short"])
 @pytest.mark.parametrize("pairs", [0, 8])
-def test_preconditioned_cg_matches_golden(gpu, name, pairs):
+@pytest.mark.parametrize("build", ["all", "finest"])
+def test_preconditioned_cg_matches_golden(gpu, name, pairs, build):
     """The preconditioned force solve (stale FP64 inverse of M, with and
-    without the low-rank update) against the oracle's plain-CG fixtures, at
-    the same gates: only iteration counts may change."""
+    without the low-rank update; P from M or from its finest-level form)
+    against the oracle's plain-CG fixtures, at the same gates: only iteration
+    counts may change."""
     g = _fixture(name)
     f = _flow(name)
     nsteps = int(g["nsteps"])
-    print("preconditioned", name, "pairs", pairs, nsteps, "steps", flush=True)
+    print("preconditioned", name, "pairs", pairs, "build", build, nsteps, "steps", flush=True)
     ctx = gpu.Context(0)
     b = gpu.Batch(ctx, [f]).init().set_precond("lowrank" if pairs else "stale", pairs)
+    b.set_precond_build(build, 1.0 if build == "all" else 0.45)
     iters = []
     for k in range(nsteps):
         b.advance(1)

@@ -1769,8 +1769,13 @@ static int solve_pcg(ibpm_batch* B, int i, int* iters, bool* fallback) {
     PcgOps* ops = B->d_pcg[i][par];
     CKR(reset_states(B));
     CKR(start_residual(B, i));
+    std::vector<char> active(nfl, 1);
     auto precondition = [&](int begin) {
-        timed(pr, K_MINV, 0, s, [&] { launch_minv(B->d_pminv[i], nfl, B->max_n, s); });
+        double by = 0;   // P (n x n), its input and output, FP64, of the flows still iterating
+        if (pr)
+            for (int k = 0; k < nfl; ++k)
+                if (active[k]) by += 8.0 * (4.0 * B->flows[k].nb * B->flows[k].nb + 4.0 * B->flows[k].nb);
+        timed(pr, K_MINV, by, s, [&] { launch_minv(B->d_pminv[i], nfl, B->max_n, s); });
         timed(pr, K_CG, 0, s, [&] { launch_pcg_dir(ops, nfl, begin, s); });
     };
     timed(pr, K_CG, 0, s, [&] { launch_pcg_begin(ops, nfl, s); });
@@ -1782,7 +1787,6 @@ static int solve_pcg(ibpm_batch* B, int i, int* iters, bool* fallback) {
     const int limit = std::max(100, 2 * B->max_n);
     int chunk = pr ? 0 : std::max(0, B->pc_last[i] - 1), done = 0;
     std::vector<double> st;
-    std::vector<char> active(nfl, 1);
     for (;;) {
         for (int c = 0; c < chunk; ++c) {
             enqueue_M(B, i, B->d_smear_d[i], s, pr ? &active : nullptr);

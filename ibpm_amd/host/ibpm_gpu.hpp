@@ -648,6 +648,13 @@ public:
     // plain CG, 1 CG preconditioned by a stale FP64 inverse of M with `pairs`
     // low-rank pairs (ibpm_batch_set_precond).  Before init().
     void setPrecond(int mode, int pairs) { _pc_mode = mode; _pc_pairs = pairs; }
+    // P built from M (levels 0) or from M's finest-level form (levels 1),
+    // with a build's cost in M applications per unknown for the rebuild rule
+    // (ibpm_batch_set_precond_build).  Before init().
+    void setPrecondBuild(int levels, double rebuild_cost) {
+        _pc_levels = levels;
+        _pc_cost = rebuild_cost;
+    }
     // A^-1 inside the moving-body force solve by local sweeps where they equal
     // the DST solve to rounding (true, the default) or always by the DST
     // solve (ibpm_batch_set_local_helmholtz).  Before init().
@@ -686,6 +693,8 @@ public:
         if (!_model.geTimeDependent()) gpu_check(ibpm_batch_use_graph(_batch, 1));
         else if (_pc_mode) gpu_check(ibpm_batch_set_precond(_batch, _pc_mode, _pc_pairs));
         if (!_local_helm) gpu_check(ibpm_batch_set_local_helmholtz(_batch, 0));
+        if (_pc_mode && _model.geTimeDependent() && _local_helm)
+            gpu_check(ibpm_batch_set_precond_build(_batch, _pc_levels, _pc_cost));
     }
     // IBSolver::advance, reference src/IBSolver.cc:142-150
     void advance(State& x) {
@@ -740,7 +749,8 @@ protected:
     int _kind;                      // ibpm_flow_params::model
     std::vector<State> _base;       // base states of the linear models
     double _chi = 0.02, _Delta = 15.0;
-    int _pc_mode = 0, _pc_pairs = 0;
+    int _pc_mode = 0, _pc_pairs = 0, _pc_levels = 0;
+    double _pc_cost = 1.0;
     bool _local_helm = true;
     std::string _cache;
     ibpm_ctx* _ctx = nullptr;

@@ -99,16 +99,19 @@ def test_ibpm_driver_force_file(gpu, oracle):
         assert abs(float(rows[k][2]) - cd) <= 1e-5 * abs(cd)
 
 
-def test_ibpm_driver_moving_plate_precond(gpu, oracle):
-    """The driver on a pitching plate with -precond lowrank and -tol: forces
-    agree with the oracle's plain CG to the CG tolerance's effect."""
+@pytest.mark.parametrize("pcbuild", ["all", "finest"])
+def test_ibpm_driver_moving_plate_precond(gpu, oracle, pcbuild):
+    """The driver on a pitching plate with -precond lowrank, -tol and either
+    preconditioner build: forces agree with the oracle's plain CG to the CG
+    tolerance's effect."""
     exe = os.path.join(os.path.dirname(HERE), "ibpm_amd", "ibpm")
     with tempfile.TemporaryDirectory() as d:
         gpath = os.path.join(d, "plate.geom")
         open(gpath, "w").write(PLATE)
         r = subprocess.run([exe, "-geom", gpath, "-nx", "128", "-ny", "128", "-ngrid", "3", "-Re", "300",
                             "-dt", "0.005", "-nsteps", "5", "-tol", "1e-10", "-precond", "lowrank",
-                            "-outdir", d, "-name", "plate"], capture_output=True, text=True, timeout=120)
+                            "-pcbuild", pcbuild, "-outdir", d, "-name", "plate"],
+                           capture_output=True, text=True, timeout=120)
         assert r.returncode == 0, r.stderr
         full = np.loadtxt(os.path.join(d, "plate.force17"))
     s = oracle.Sim(128, 128, 3, 4.0, -2.0, -2.0, 300.0, 0.005, PLATE, tol=1e-10)

@@ -1965,6 +1965,61 @@ static void cholesky_assemble(int n, const std::vector<double>& lower, const std
     }
 }
 
+// The preconditioner's P = M^-1, in place, from M's upper triangle as
+// cholesky_factor reads it (Mt column-major in, P row-major out), in an order
+// whose inner loops are contiguous row updates: U^T U = M by rows of U
+// (right-looking), W = U^-1 by rows from the last, P = W W^T from W's rows and
+// its transpose's.  About n^3/2 multiply-adds against the 4n^3/3 of the
+// factor and unit-vector substitutions.  P only steers the iterations, so its
+// rounding need not be the reference's (the Cholesky force solve keeps
+// cholesky_factor / cholesky_assemble).  Returns -1, or the row where M is not
+// positive definite.
+static int spd_inverse_rows(int n, std::vector<double>& Mt) {
+    std::vector<double> U((size_t)n * n, 0.0), W((size_t)n * n, 0.0);
+    for (int r = 0; r < n; ++r)
+        for (int c = r; c < n; ++c) U[(size_t)r * n + c] = Mt[(size_t)c * n + r];
+    for (int k = 0; k < n; ++k) {
+        double* uk = &U[(size_t)k * n];
+        if (!(uk[k] > 0)) return k;
+        const double d = std::sqrt(uk[k]), inv = 1.0 / d;
+        uk[k] = d;
+        for (int c = k + 1; c < n; ++c) uk[c] *= inv;
+        for (int r = k + 1; r < n; ++r) {
+            const double l = uk[r];
+            double* ur = &U[(size_t)r * n];
+            for (int c = r; c < n; ++c) ur[c] -= l * uk[c];
+        }
+    }
+    // U W = I: row i of W is (e_i - sum over k > i of U[i][k] W[k]) / U[i][i]
+    for (int i = n - 1; i >= 0; --i) {
+        double* wi = &W[(size_t)i * n];
+        const double* ui = &U[(size_t)i * n];
+        wi[i] = 1.0;
+        for (int k = i + 1; k < n; ++k) {
+            const double u = ui[k];
+            const double* wk = &W[(size_t)k * n];
+            for (int j = k; j < n; ++j) wi[j] -= u * wk[j];
+        }
+        const double inv = 1.0 / ui[i];
+        for (int j = i; j < n; ++j) wi[j] *= inv;
+    }
+    // P[i][j] = sum over k >= i of W[i][k] W[j][k] for j <= i (U reused as W^T), then mirrored
+    for (int k = 0; k < n; ++k)
+        for (int j = 0; j < n; ++j) U[(size_t)k * n + j] = W[(size_t)j * n + k];
+    std::fill(Mt.begin(), Mt.end(), 0.0);
+    for (int i = 0; i < n; ++i) {
+        double* pi = &Mt[(size_t)i * n];
+        for (int k = i; k < n; ++k) {
+            const double w = W[(size_t)i * n + k];
+            const double* t = &U[(size_t)k * n];
+            for (int j = 0; j <= i; ++j) pi[j] += w * t[j];
+        }
+    }
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < i; ++j) Mt[(size_t)j * n + i] = Mt[(size_t)i * n + j];
+    return -1;
+}
+
 // In place: Mt (M column-major) -> M^-1 row-major.
 static int cholesky_inverse(int n, std::vector<double>& Mt) {
     std::vector<double> lower, d;
@@ -2112,6 +2167,10 @@ static int build_inverse(ibpm_batch* B, int i, bool precond) {
             for (int k = t; k < nfl; k += nthreads) {
                 Flow& F = B->flows[k];
                 const int n = 2 * F.nb;
+                if (precond) {   // P: the vectorised order
+                    bad[k] = spd_inverse_rows(n, minv_h[k]);
+                    continue;
+                }
                 if (!cached[k]) {
                     bad[k] = cholesky_factor(n, minv_h[k], lower[k], diag[k]);
                     if (bad[k] >= 0) continue;

@@ -692,6 +692,10 @@ struct ibpm_batch {
     ibpmg::SolvePlan M2L;
     ibpmg::HelmJob* d_helm = nullptr;
     ibpmg::HelmJob* d_helm3 = nullptr;   // the same solves for omega = omega* - A^-1 c (no skip flags)
+    // level-0 jobs of d_helm alone (the finest-level build of the preconditioner)
+    ibpmg::HelmJob* d_helm0 = nullptr;
+    int helm0_n = 0, helm0_blocks = 1;
+    double helm0_bytes = 0;
     ibpmg::HelmJob* d_hclear = nullptr;  // boxes to clear when a flow's boxes change
     int hclear_cap = 0;
     int helm_cap = 0, helm_n = 0, helm_blocks = 0, helm_cblocks = 0, helm_ablocks = 0, helm_sweeps = 0;
@@ -1498,6 +1502,21 @@ static int plan_local_helm(ibpm_batch* B, int i) {
     }
     CKR(upload(B->d_helm, jobs.data(), sizeof(HelmJob) * jobs.size()));
     CKR(upload(B->d_helm3, jobs3.data(), sizeof(HelmJob) * jobs3.size()));
+    {
+        std::vector<HelmJob> j0;
+        long t0 = 1;
+        B->helm0_bytes = 0;
+        for (size_t q = 0; q < jobs.size(); ++q)
+            if (jobs[q].lev == 0) {
+                j0.push_back(jobs[q]);
+                t0 = std::max(t0, (long)helm_tiles(jobs[q].i1 - jobs[q].i0 + 1, jobs[q].j1 - jobs[q].j0 + 1));
+                B->helm0_bytes += jbytes[q];
+            }
+        if (!B->d_helm0) CKR(B->mem.alloc(&B->d_helm0, (size_t)nfl));
+        CKR(upload(B->d_helm0, j0.data(), sizeof(HelmJob) * j0.size()));
+        B->helm0_n = (int)j0.size();
+        B->helm0_blocks = (int)t0;
+    }
     if (!clears.empty()) {
         long maxc = 1;
         for (const HelmJob& C : clears) maxc = std::max(maxc, (long)(C.i1 - C.i0 + 1) * (C.j1 - C.j0 + 1));
@@ -1556,10 +1575,14 @@ static void enqueue_M(ibpm_batch* B, int i, RegOps* smear, hipStream_t s,
             timed(pr, K_FSCOARSIFY, cby / std::max(1, B->M1[i].nlev - 1), s,
                   [&] { launch_helm_coarsify(B->d_helm, B->helm_n, B->helm_cblocks, lev, s); });
         const int nl = helm_launches(B->helm_sweeps);
-        launch_helm_sweeps(B->d_helm, B->helm_n, B->helm_blocks, B->helm_sweeps, s,
-                           [&](const std::function<void()>& fn) { timed(pr, K_FSHELM, by / nl, s, fn); });
+        if (single_level)   // level 0's sweeps alone: the coarser levels' results are not read
+            launch_helm_sweeps(B->d_helm0, B->helm0_n, B->helm0_blocks, B->helm_sweeps, s,
+                               [&](const std::function<void()>& fn) { timed(pr, K_FSHELM, B->helm0_bytes / nl, s, fn); });
+        else
+            launch_helm_sweeps(B->d_helm, B->helm_n, B->helm_blocks, B->helm_sweeps, s,
+                               [&](const std::function<void()>& fn) { timed(pr, K_FSHELM, by / nl, s, fn); });
         const bool boxed = B->m2l_boxed && !B->M2L.fp32;   // the boxed chain feeds register-held passes only
-        if (single_level) {   // levels >= 1 of the sweeps' result are not read
+        if (single_level) {
             run_solve(B->M1L, s, pr, active, false);
         } else if (boxed) {
             double cb = 0;
@@ -2109,6 +2132,37 @@ static int build_inverse(ibpm_batch* B, int i, bool precond) {
         B->M1L.force_solve = true;
         CKR(build_solve_plan(&B->ctx->c, B->mem, sp, B->M1L));
         B->m1l_built = true;
+    }
+    if (single) {
+        // the level-0 solve read as the force solve's: pass 1 in closed form
+        // (zeros: no boundary values) off the local result's rows, passes 3 and
+        // 2 over the node rows only
+        for (auto& step : B->M1L.steps)
+            for (auto& G : step) {
+                bool changed = false;
+                for (size_t j = 0; j < G.h.size(); ++j) {
+                    EllJob& J = G.h[j];
+                    const Flow& F = B->flows[G.jflow[j]];
+                    const EllJob before = J;
+                    whole_level(J);
+                    if (!F.reg.node_idx.empty() && !F.hbox.empty()) {
+                        int lo = INT_MAX, hi = INT_MIN;
+                        for (int idx : F.reg.node_idx) {
+                            lo = std::min(lo, idx / (F.ny - 1) + 1);
+                            hi = std::max(hi, idx / (F.ny - 1) + 1);
+                        }
+                        J.sup_lo = F.hbox[0][0];
+                        J.sup_hi = F.hbox[0][1];
+                        J.sin1 = F.sin1;
+                        read_rows_only(J, G, lo, hi, F.ringsin);
+                    }
+                    if (std::memcmp(&before, &J, sizeof(EllJob)) != 0) changed = true;
+                }
+                if (changed) {
+                    CKR(upload(G.d, G.h.data(), sizeof(EllJob) * G.h.size()));
+                    group_bytes(G);
+                }
+            }
     }
     if (!all_cached) {
         // one smear descriptor per flow (the unit vector moves in place), one

@@ -273,7 +273,7 @@ def main():
                     help="P of the preconditioned forms from M itself or from its finest-level form")
     ap.add_argument("--rebuild-cost", type=float, default=0.45,
                     help="a build's cost in M applications per unknown, the rebuild rule's break-even "
-                         "(measured: 0.42 for the finest-level build of the tandem batch)")
+                         "(tandem batch: 0.25 and 0.45 give the same throughput, 0.15 rebuilds too often)")
     ap.add_argument("--no-forms", action="store_true",
                     help="moving bodies: time only the headline form (no plain, batched-plain or "
                          "previous-start forms beside it)")

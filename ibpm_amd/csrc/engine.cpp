@@ -637,7 +637,20 @@ struct Flow {
     double by_interp = 0, by_smear = 0;
     std::vector<int> reg_old;      // node list of the previous association (cleared on the device)
     int reg_old_n = 0;
+    // the next substep's kinematics, regulariser lists and constraint, computed
+    // while the GPU runs this substep (kinematics_ahead), with the motions'
+    // running state from before that move (restored if they go unused)
+    struct {
+        bool valid = false;
+        double t = 0;
+        std::vector<double> px, py, pu, pv, b;
+        RegLists reg;
+        std::vector<std::array<double, 4>> state;
+    } ahead;
+    std::vector<double> b_host;    // the constraint taken from `ahead` for upload_constraint
+    bool b_host_ready = false;
     HostStage stage;
+    HostStage stage2[2];           // the substep prep's list uploads, alternate substeps (ibpm_batch::upar)
     // local Helmholtz path of the force solve's A^-1: result and sweep buffer
     // (all levels) and, per level, the box they were last swept over (zero off it)
     double *m1 = nullptr, *m1tmp = nullptr;
@@ -756,6 +769,14 @@ struct ibpm_batch {
     // the rebuild rule's costs
     double pc_t_build = 0.0, pc_t_iter = 0.0;
     // build_inverse: interpolation descriptors of every column, unit vectors
+    // substep prep uploads (moving bodies): asynchronous on the stream from
+    // pinned buffers, alternate substeps; uev[p]: after the last copy made from
+    // half p (the next reuse of half p waits for it)
+    ibpmg::HostStage ustage[2];
+    hipEvent_t uev[2] = {nullptr, nullptr};
+    bool upending[2] = {false, false};
+    int upar = 0;
+    bool async_up = false;
     ibpmg::RegOps* d_interp_cols = nullptr;
     size_t interp_cols_cap = 0;
     ibpmg::UnitOps* d_units = nullptr;
@@ -797,6 +818,53 @@ static int stage_upload(hipStream_t s, HostStage& st, void* d, const void* h, si
     return 0;
 }
 
+// A descriptor upload: during a moving-body substep's prep (async_up), an
+// asynchronous copy on the stream from this substep's pinned half, ordered
+// after the kernels of earlier substeps that read the old descriptors; else
+// (or past the half's capacity) a synchronous copy.
+static int up(ibpm_batch* B, void* d, const void* h, size_t bytes) {
+    if (!B->async_up) return upload(d, h, bytes);
+    HostStage& st = B->ustage[B->upar];
+    if (((st.used + 255) & ~(size_t)255) + bytes > st.cap) {
+        CK(hipStreamSynchronize(B->ctx->c.stream));
+        return upload(d, h, bytes);
+    }
+    CKR(stage_upload(B->ctx->c.stream, st, d, h, bytes));
+    B->upending[B->upar] = true;
+    return 0;
+}
+
+// Starts a moving-body substep's prep: the other pinned half, once the copies
+// made from it two substeps ago have run.
+static int async_prep_begin(ibpm_batch* B) {
+    if (!B->ustage[0].p) {
+        for (auto& st : B->ustage) {
+            st.cap = (size_t)8 << 20;
+            CK(hipHostMalloc((void**)&st.p, st.cap, hipHostMallocDefault));
+        }
+        for (auto& e : B->uev) CK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    B->upar ^= 1;
+    const int p = B->upar;
+    if (B->upending[p]) CK(hipEventSynchronize(B->uev[p]));
+    B->upending[p] = false;
+    B->ustage[p].used = 0;
+    B->ustage[p].pending = false;
+    for (auto& F : B->flows) {
+        F.stage2[p].used = 0;
+        F.stage2[p].pending = false;
+    }
+    B->async_up = true;
+    return 0;
+}
+
+static int async_prep_end(ibpm_batch* B) {
+    B->async_up = false;
+    CK(hipEventRecord(B->uev[B->upar], B->ctx->c.stream));
+    B->upending[B->upar] = true;
+    return 0;
+}
+
 // Scheme coefficients, Scheme.h:14-56
 static void scheme_coefs(int scheme, ibpm_batch* B) {
     switch (scheme) {
@@ -821,8 +889,8 @@ static double beta_of(const ibpm_batch* B, const Flow& F, int i) { return (B->an
 // The constraint b = u_body - E q0 (NavierStokesModel.cc:86-91,142-146) with
 // q0 on level 0: the uniform flow, or the unsteady base flow at its last
 // update (Flux.cc:101-129 at the edge coordinates of Grid.cc:104-139).
-static int upload_constraint(hipStream_t s, Flow& F) {
-    const RegLists& R = F.reg;
+static void compute_constraint(const Flow& F, const RegLists& R, const std::vector<double>& pu,
+                               const std::vector<double>& pv, std::vector<double>& b) {
     const double pi = 4. * std::atan(1.);
     const double al = F.p.alpha_deg * pi / 180.;
     double sal, cal;
@@ -838,15 +906,27 @@ static int upload_constraint(hipStream_t s, Flow& F) {
         const int u = fidx - (int)nfx, i = u / (F.ny + 1), j = u % (F.ny + 1);
         return F.bf.flux(1, F.p.xoffset + (i + 0.5) * F.dx, F.p.yoffset + j * F.dx, F.dx);
     };
-    std::vector<double> b(2 * F.nb);
+    b.assign(2 * F.nb, 0.0);
     for (int k = 0; k < 2 * F.nb; ++k) {
         double s = 0;
         for (int a = R.e_ptr[k]; a < R.e_ptr[k + 1]; ++a) s += R.e_w[a] * q0(R.e_f[a]);
         s *= 1.0 / F.dx;
-        const double vel = k < F.nb ? F.pu[k] : F.pv[k - F.nb];
+        const double vel = k < F.nb ? pu[k] : pv[k - F.nb];
         b[k] = vel - s;
     }
-    return stage_upload(s, F.stage, F.b, b.data(), sizeof(double) * b.size());
+}
+
+// Uploads the constraint of F's current lists and velocities (computed ahead
+// by kinematics_ahead where it took that route).
+static int upload_constraint(hipStream_t s, Flow& F, HostStage& st) {
+    std::vector<double> b;
+    if (F.b_host_ready) {
+        b.swap(F.b_host);
+        F.b_host_ready = false;
+    } else {
+        compute_constraint(F, F.reg, F.pu, F.pv, b);
+    }
+    return stage_upload(s, st, F.b, b.data(), sizeof(double) * b.size());
 }
 
 // BaseFlow::moveFlow(t) of a flow with an unsteady base flow
@@ -858,7 +938,7 @@ static int move_base_flow(ibpm_batch* B, Flow& F, double t) {
     CKR(stage_begin(s, F.stage));
     const double uvw[3] = {F.bf.u, F.bf.v, F.bf.w};
     CKR(stage_upload(s, F.stage, F.bf_dev, uvw, sizeof uvw));
-    return upload_constraint(s, F);
+    return upload_constraint(s, F, F.stage);
 }
 
 // Regulariser lists of a flow for its current points; uploads them and the
@@ -871,6 +951,64 @@ static void build_regularizer(Flow& F) {
     build_reg_lists(F.nx, F.ny, F.dx, F.p.xoffset, F.p.yoffset, F.px, F.py, F.reg);
 }
 
+// The running state of the motions that integrate over successive calls.
+static void save_motion_state(const Flow& F, std::vector<std::array<double, 4>>& st) {
+    st.clear();
+    for (const Body& b : F.geom.bodies) st.push_back({b.acc[0], b.acc[1], b.acc[2], b.oldtime});
+}
+
+static void restore_motion_state(Flow& F, const std::vector<std::array<double, 4>>& st) {
+    for (size_t k = 0; k < st.size() && k < F.geom.bodies.size(); ++k) {
+        const Body& b = F.geom.bodies[k];
+        b.acc[0] = st[k][0]; b.acc[1] = st[k][1]; b.acc[2] = st[k][2]; b.oldtime = st[k][3];
+    }
+}
+
+// A moving flow's kinematics, regulariser lists and constraint at time t,
+// ahead of the substep that uses them (the motion is moved now, in the
+// reference's sequence of times; its prior state is kept).
+static void kinematics_ahead(Flow& F, double t) {
+    auto& A = F.ahead;
+    save_motion_state(F, A.state);
+    F.geom.move(t, A.px, A.py, A.pu, A.pv);
+    build_reg_lists(F.nx, F.ny, F.dx, F.p.xoffset, F.p.yoffset, A.px, A.py, A.reg);
+    compute_constraint(F, A.reg, A.pu, A.pv, A.b);
+    A.t = t;
+    A.valid = true;
+}
+
+// A moving flow's kinematics for its substep at time t: those computed ahead
+// for t, or computed now (after undoing a move made ahead for another time).
+static void kinematics_at(Flow& F, double t) {
+    auto& A = F.ahead;
+    if (A.valid && A.t == t) {
+        F.reg_old = F.reg.node_idx;
+        F.reg_old_n = F.nnodes;
+        std::swap(F.reg, A.reg);
+        F.px.swap(A.px); F.py.swap(A.py); F.pu.swap(A.pu); F.pv.swap(A.pv);
+        F.b_host.swap(A.b);
+        F.b_host_ready = true;
+        A.valid = false;
+        return;
+    }
+    if (A.valid) {
+        restore_motion_state(F, A.state);
+        A.valid = false;
+    }
+    F.geom.move(t, F.px, F.py, F.pu, F.pv);
+    build_regularizer(F);
+    F.b_host_ready = false;
+}
+
+// Undoes every move made ahead (the batch's time or bodies are set anew).
+static void cancel_ahead(ibpm_batch* B) {
+    for (auto& F : B->flows)
+        if (F.ahead.valid) {
+            restore_motion_state(F, F.ahead.state);
+            F.ahead.valid = false;
+        }
+}
+
 static int upload_regularizer(ibpm_batch* B, Flow& F) {
     const RegLists& R = F.reg;
     const std::vector<int>& old = F.reg_old;
@@ -881,8 +1019,10 @@ static int upload_regularizer(ibpm_batch* B, Flow& F) {
         return -3;
     }
     hipStream_t s = B->ctx->c.stream;
-    HostStage& st = F.stage;
-    CKR(stage_begin(s, st));
+    // during a substep's prep: this substep's pinned half (async_prep_begin
+    // made it free); else the flow's own buffer, once its last copies have run
+    HostStage& st = B->async_up ? F.stage2[B->upar] : F.stage;
+    if (!B->async_up) CKR(stage_begin(s, st));
     CKR(stage_upload(s, st, F.e_ptr, R.e_ptr.data(), sizeof(int) * R.e_ptr.size()));
     CKR(stage_upload(s, st, F.e_f, R.e_f.data(), sizeof(int) * R.e_f.size()));
     CKR(stage_upload(s, st, F.e_w, R.e_w.data(), sizeof(double) * R.e_w.size()));
@@ -903,7 +1043,7 @@ static int upload_regularizer(ibpm_batch* B, Flow& F) {
     for (int slot : R.node_nbr)
         if (slot >= 0) sm += 8.0 + 20.0 * (double)(R.t_ptr[slot + 1] - R.t_ptr[slot]);
     F.by_smear = sm;
-    return upload_constraint(s, F);
+    return upload_constraint(s, F, st);
 }
 
 static int update_regularizer(ibpm_batch* B, Flow& F) {
@@ -955,9 +1095,9 @@ static int upload_regops(ibpm_batch* B) {
         cl[k].cout = F.c;
         B->max_nodes = std::max(B->max_nodes, std::max(F.nnodes, F.nold));
     }
-    CKR(upload(B->d_interp_c, ic.data(), sizeof(RegOps) * nf));
-    CKR(upload(B->d_interp_m, im.data(), sizeof(RegOps) * nf));
-    CKR(upload(B->d_clear, cl.data(), sizeof(RegOps) * nf));
+    CKR(up(B, B->d_interp_c, ic.data(), sizeof(RegOps) * nf));
+    CKR(up(B, B->d_interp_m, im.data(), sizeof(RegOps) * nf));
+    CKR(up(B, B->d_clear, cl.data(), sizeof(RegOps) * nf));
     for (int i = 0; i < B->nsub; ++i) {
         std::vector<RegOps> sf(nf), sd(nf);
         for (int k = 0; k < nf; ++k) {
@@ -968,8 +1108,8 @@ static int upload_regops(ibpm_batch* B) {
             sd[k].fin = F.cg_d; sd[k].cout = F.c;
             sd[k].skip = F.cg_state + 3;
         }
-        CKR(upload(B->d_smear_f[i], sf.data(), sizeof(RegOps) * nf));
-        CKR(upload(B->d_smear_d[i], sd.data(), sizeof(RegOps) * nf));
+        CKR(up(B, B->d_smear_f[i], sf.data(), sizeof(RegOps) * nf));
+        CKR(up(B, B->d_smear_d[i], sd.data(), sizeof(RegOps) * nf));
     }
     return 0;
 }
@@ -998,6 +1138,10 @@ static int alloc_flow(ibpm_batch* B, Flow& F) {
                              sizeof(int) * 8 * cap + sizeof(int4) * 4 * cap + sizeof(double) * (n + 4) + 12 * 256;
         CK(hipHostMalloc((void**)&F.stage.p, bytes, hipHostMallocDefault));
         F.stage.cap = bytes;
+        for (auto& st : F.stage2) {
+            CK(hipHostMalloc((void**)&st.p, bytes, hipHostMallocDefault));
+            st.cap = bytes;
+        }
     }
     if (F.p.model >= 1 && F.p.model <= 3) {   // base states
         const int np = F.p.model == 3 ? F.p.period : 1;
@@ -1284,7 +1428,7 @@ static int restrict_s2(ibpm_batch* B, int i) {
                 if (std::memcmp(&before, &J, sizeof(EllJob)) != 0) changed = true;
             }
             if (changed) {
-                CKR(upload(G.d, G.h.data(), sizeof(EllJob) * G.h.size()));
+                CKR(up(B, G.d, G.h.data(), sizeof(EllJob) * G.h.size()));
                 group_bytes(G);
             }
         }
@@ -1344,7 +1488,7 @@ static int restrict_m2l_pass3(ibpm_batch* B, const std::vector<std::vector<std::
                 if (std::memcmp(&before, &J, sizeof(EllJob)) != 0) changed = true;
             }
             if (changed) {
-                CKR(upload(G.d, G.h.data(), sizeof(EllJob) * G.h.size()));
+                CKR(up(B, G.d, G.h.data(), sizeof(EllJob) * G.h.size()));
                 group_bytes(G);
             }
         }
@@ -1376,7 +1520,7 @@ static int restrict_m2l_pass3(ibpm_batch* B, const std::vector<std::vector<std::
         CKR(B->mem.alloc(&B->d_m2lc, cj.size()));
         B->m2lc_cap = (int)cj.size();
     }
-    CKR(upload(B->d_m2lc, cj.data(), sizeof(HelmJob) * cj.size()));
+    CKR(up(B, B->d_m2lc, cj.data(), sizeof(HelmJob) * cj.size()));
     B->m2lc_n = (int)cj.size();
     B->m2lc_blocks = (int)((maxa + kThreads - 1) / kThreads);
     return 0;
@@ -1500,8 +1644,8 @@ static int plan_local_helm(ibpm_batch* B, int i) {
         CKR(B->mem.alloc(&B->d_helm3, jobs.size()));
         B->helm_cap = (int)jobs.size();
     }
-    CKR(upload(B->d_helm, jobs.data(), sizeof(HelmJob) * jobs.size()));
-    CKR(upload(B->d_helm3, jobs3.data(), sizeof(HelmJob) * jobs3.size()));
+    CKR(up(B, B->d_helm, jobs.data(), sizeof(HelmJob) * jobs.size()));
+    CKR(up(B, B->d_helm3, jobs3.data(), sizeof(HelmJob) * jobs3.size()));
     {
         std::vector<HelmJob> j0;
         long t0 = 1;
@@ -1513,7 +1657,7 @@ static int plan_local_helm(ibpm_batch* B, int i) {
                 B->helm0_bytes += jbytes[q];
             }
         if (!B->d_helm0) CKR(B->mem.alloc(&B->d_helm0, (size_t)nfl));
-        CKR(upload(B->d_helm0, j0.data(), sizeof(HelmJob) * j0.size()));
+        CKR(up(B, B->d_helm0, j0.data(), sizeof(HelmJob) * j0.size()));
         B->helm0_n = (int)j0.size();
         B->helm0_blocks = (int)t0;
     }
@@ -1524,7 +1668,7 @@ static int plan_local_helm(ibpm_batch* B, int i) {
             CKR(B->mem.alloc(&B->d_hclear, clears.size()));
             B->hclear_cap = (int)clears.size();
         }
-        CKR(upload(B->d_hclear, clears.data(), sizeof(HelmJob) * clears.size()));
+        CKR(up(B, B->d_hclear, clears.data(), sizeof(HelmJob) * clears.size()));
         launch_helm_clear(B->d_hclear, (int)clears.size(), (int)((maxc + kThreads - 1) / kThreads),
                           B->ctx->c.stream);
     }
@@ -1844,24 +1988,31 @@ static int substep_cg(ibpm_batch* B, int i) {
     const int nfl = (int)B->flows.size();
     const double t_kin = now_s();
     // updateOperators(t + c_n dt), IBSolver.cc:164-166: kinematics and
-    // regulariser lists of the moving flows on host threads while the GPU
-    // finishes the previous substep, then the uploads
+    // regulariser lists of the moving flows, computed during the previous
+    // substep (kinematics_ahead) or here on host threads, then the uploads
     std::vector<int> moving;
     for (int k = 0; k < nfl; ++k)
         if (!B->flows[k].geom.stationary) moving.push_back(k);
     parallel_for((int)moving.size(), [&](int q) {
         Flow& F = B->flows[moving[q]];
-        F.geom.move(B->time + B->cn[i] * F.p.dt, F.px, F.py, F.pu, F.pv);
-        build_regularizer(F);
+        kinematics_at(F, B->time + B->cn[i] * F.p.dt);
     });
-    CK(hipStreamSynchronize(s));
-    for (auto& F : B->flows) F.stage.pending = false;
-    for (int k : moving) CKR(upload_regularizer(B, B->flows[k]));
-    for (auto& F : B->flows)
-        if (F.ubf) CKR(move_base_flow(B, F, B->time + B->cn[i] * F.p.dt));
-    CKR(upload_regops(B));
-    CKR(plan_local_helm(B, i));
-    CKR(restrict_s2(B, i));
+    // uploads of lists and descriptors queued on the stream behind the previous
+    // substep's kernels, which read the old ones, while the GPU runs them
+    {
+        struct Guard {
+            ibpm_batch* b;
+            ~Guard() { b->async_up = false; }
+        } guard{B};
+        CKR(async_prep_begin(B));
+        for (int k : moving) CKR(upload_regularizer(B, B->flows[k]));
+        for (auto& F : B->flows)
+            if (F.ubf) CKR(move_base_flow(B, F, B->time + B->cn[i] * F.p.dt));
+        CKR(upload_regops(B));
+        CKR(plan_local_helm(B, i));
+        CKR(restrict_s2(B, i));
+        CKR(async_prep_end(B));
+    }
     B->st_host_kin_s += now_s() - t_kin;
     Probe* pr = B->probe;
     timed(pr, K_CLEAR, 0, s, [&] { launch_clear_nodes(B->d_clear, nfl, B->max_nodes, s); });
@@ -1870,6 +2021,17 @@ static int substep_cg(ibpm_batch* B, int i) {
     run_solve(B->S2[i], s, pr);
     timed(pr, K_INTERP, pr ? reg_bytes(B, false, true) : 0, s,
           [&] { launch_interp(B->d_interp_c, nfl, B->max_comp, s); });
+    // the next substep's kinematics on host threads while the GPU runs this
+    // substep's solves (its time exactly as that substep will form it)
+    {
+        const bool next_step = i + 1 == B->nsub;
+        const int ni = next_step ? 0 : i + 1;
+        const double t_next = next_step ? B->time + B->flows[0].p.dt : B->time;
+        parallel_for((int)moving.size(), [&](int q) {
+            Flow& F = B->flows[moving[q]];
+            kinematics_ahead(F, t_next + B->cn[ni] * F.p.dt);
+        });
+    }
     // x0: previous f (ProjectionSolver.cc:72 warm start), zero, or the
     // learned prediction; then r0 = rhs - M x0
     if (B->warm_mode == 0) {
@@ -1921,7 +2083,6 @@ static int substep_cg(ibpm_batch* B, int i) {
         CK(hipMemcpyAsync(F.ring_f + slot * n, F.f, sizeof(double) * 2 * F.nb, hipMemcpyDeviceToDevice, s));
         CK(hipMemcpyAsync(F.ring_b + slot * n, F.b, sizeof(double) * 2 * F.nb, hipMemcpyDeviceToDevice, s));
     }
-    CK(hipStreamSynchronize(s));
     B->ksub += 1;
     timed(pr, K_SMEAR, pr ? reg_bytes(B, true, false) : 0, s,
           [&] { launch_smear_curl(B->d_smear_f[i], nfl, B->max_nodes, s); });
@@ -3343,6 +3504,8 @@ int ibpm_batch_destroy(ibpm_batch* b) {
     track_graph(b, false);
     (void)hipStreamSynchronize(b->ctx->c.stream);
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    for (auto e : b->uev)
+        if (e) (void)hipEventDestroy(e);
     delete b;
     return 0;
 }
@@ -3519,6 +3682,7 @@ int ibpm_batch_set_state(ibpm_batch* B, int k, const double* h_omega, const doub
         set_error("set_state: the batch has advanced %d steps already", B->timestep);
         return -1;
     }
+    cancel_ahead(B);   // the time is set anew
     CK(hipMemcpy(F->omega, h_omega, sizeof(double) * F->ns * F->L, hipMemcpyHostToDevice));
     if (h_f) CK(hipMemcpy(F->f, h_f, sizeof(double) * 2 * F->nb, hipMemcpyHostToDevice));
     B->time = time;

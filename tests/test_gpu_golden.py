@@ -240,3 +240,24 @@ def test_ibpm_driver_other_models(gpu, oracle, model):
         scale = max(abs(cd), abs(cl))
         print(model, k, full[k, 2], cd, full[k, 3], cl, flush=True)
         assert abs(full[k, 2] - cd) <= 1e-9 * scale and abs(full[k, 3] - cl) <= 1e-9 * scale
+
+
+def test_precond_build_arguments(gpu):
+    """ibpm_batch_set_precond_build refuses an unknown level count, a
+    non-positive rebuild cost, and the finest-level build without the local
+    Helmholtz path; every refusal leaves an error message."""
+    from ibpm_amd import IbpmError, load
+    ctx = gpu.Context(0)
+    b = gpu.Batch(ctx, [_flow("plate")]).init().set_precond("lowrank", 8)
+    lib = load()
+    assert lib.ibpm_batch_set_precond_build(b.h, 2, 1.0) != 0
+    assert b"levels" in lib.ibpm_last_error()
+    assert lib.ibpm_batch_set_precond_build(b.h, 1, 0.0) != 0
+    assert b"rebuild cost" in lib.ibpm_last_error()
+    with pytest.raises(IbpmError):
+        b.set_precond_build("finest", -1.0)
+    b.set_precond_build("finest", 0.45).set_precond_build("all", 1.0)
+    b.set_local_helmholtz(False)
+    with pytest.raises(IbpmError, match="local Helmholtz"):
+        b.set_precond_build("finest", 0.45)
+    b.close()

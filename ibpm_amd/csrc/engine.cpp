@@ -650,7 +650,6 @@ struct Flow {
     std::vector<double> b_host;    // the constraint taken from `ahead` for upload_constraint
     bool b_host_ready = false;
     HostStage stage;
-    HostStage stage2[2];           // the substep prep's list uploads, alternate substeps (ibpm_batch::upar)
     // local Helmholtz path of the force solve's A^-1: result and sweep buffer
     // (all levels) and, per level, the box they were last swept over (zero off it)
     double *m1 = nullptr, *m1tmp = nullptr;
@@ -772,7 +771,16 @@ struct ibpm_batch {
     // substep prep uploads (moving bodies): asynchronous on the stream from
     // pinned buffers, alternate substeps; uev[p]: after the last copy made from
     // half p (the next reuse of half p waits for it)
+    // solver polls: states copied to pinned memory behind the queued work
+    double* h_states = nullptr;   // 8 per flow (poll_begin)
+    double* h_reset = nullptr;    // reset_states' source
+    hipEvent_t poll_ev = nullptr;
     ibpmg::HostStage ustage[2];
+    unsigned char* d_ustage[2] = {nullptr, nullptr};   // device images of the halves
+    std::vector<ibpmg::CopyPiece> upieces;              // this prep's pieces (async_flush)
+    // k_copy_pieces tables: cg_d = f per flow (start_residual); the history
+    // ring's f and b copies per ring slot
+    ibpmg::CopyPiece *d_start_pieces = nullptr, *d_ring_pieces = nullptr;
     hipEvent_t uev[2] = {nullptr, nullptr};
     bool upending[2] = {false, false};
     int upar = 0;
@@ -818,29 +826,60 @@ static int stage_upload(hipStream_t s, HostStage& st, void* d, const void* h, si
     return 0;
 }
 
-// A descriptor upload: during a moving-body substep's prep (async_up), an
-// asynchronous copy on the stream from this substep's pinned half, ordered
-// after the kernels of earlier substeps that read the old descriptors; else
-// (or past the half's capacity) a synchronous copy.
-static int up(ibpm_batch* B, void* d, const void* h, size_t bytes) {
-    if (!B->async_up) return upload(d, h, bytes);
+// Sends this prep's pieces: their data and the piece list in one copy of the
+// pinned half to its device image, then one k_copy_pieces launch that puts
+// every piece in place (instead of one runtime copy per piece).
+static int async_flush(ibpm_batch* B) {
+    if (B->upieces.empty()) return 0;
+    hipStream_t s = B->ctx->c.stream;
+    const int p = B->upar;
+    HostStage& st = B->ustage[p];
+    const size_t off = (st.used + 255) & ~(size_t)255, bytes = sizeof(CopyPiece) * B->upieces.size();
+    if (off + bytes > st.cap) { set_error("upload staging buffer too small for the piece list"); return -3; }
+    std::memcpy(st.p + off, B->upieces.data(), bytes);
+    st.used = off + bytes;
+    CK(hipMemcpyAsync(B->d_ustage[p], st.p, st.used, hipMemcpyHostToDevice, s));
+    launch_copy_pieces((const CopyPiece*)(B->d_ustage[p] + off), (int)B->upieces.size(), s);
+    B->upieces.clear();
+    B->upending[p] = true;
+    return 0;
+}
+
+// One upload of a substep's prep: its data into this substep's pinned half
+// and a piece for async_flush, ordered on the stream after the kernels of
+// earlier substeps that read the old contents.  Past the half's capacity
+// (the piece list keeps 64 KB), the pieces so far are sent and this one is
+// copied synchronously.
+static int async_piece(ibpm_batch* B, void* d, const void* h, size_t bytes) {
+    if (bytes == 0) return 0;
     HostStage& st = B->ustage[B->upar];
-    if (((st.used + 255) & ~(size_t)255) + bytes > st.cap) {
+    const size_t off = (st.used + 255) & ~(size_t)255;
+    if (bytes % 4 != 0 || off + bytes + ((size_t)64 << 10) > st.cap) {
+        CKR(async_flush(B));
         CK(hipStreamSynchronize(B->ctx->c.stream));
         return upload(d, h, bytes);
     }
-    CKR(stage_upload(B->ctx->c.stream, st, d, h, bytes));
-    B->upending[B->upar] = true;
+    std::memcpy(st.p + off, h, bytes);
+    st.used = off + bytes;
+    B->upieces.push_back(CopyPiece{(const unsigned int*)(B->d_ustage[B->upar] + off), (unsigned int*)d,
+                                   (long)(bytes / 4)});
     return 0;
+}
+
+// A descriptor upload: during a moving-body substep's prep (async_up) an
+// async_piece, else a synchronous copy.
+static int up(ibpm_batch* B, void* d, const void* h, size_t bytes) {
+    return B->async_up ? async_piece(B, d, h, bytes) : upload(d, h, bytes);
 }
 
 // Starts a moving-body substep's prep: the other pinned half, once the copies
 // made from it two substeps ago have run.
 static int async_prep_begin(ibpm_batch* B) {
     if (!B->ustage[0].p) {
-        for (auto& st : B->ustage) {
-            st.cap = (size_t)8 << 20;
-            CK(hipHostMalloc((void**)&st.p, st.cap, hipHostMallocDefault));
+        for (int p = 0; p < 2; ++p) {
+            B->ustage[p].cap = (size_t)8 << 20;
+            CK(hipHostMalloc((void**)&B->ustage[p].p, B->ustage[p].cap, hipHostMallocDefault));
+            CKR(B->mem.alloc(&B->d_ustage[p], B->ustage[p].cap));
         }
         for (auto& e : B->uev) CK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
@@ -850,15 +889,13 @@ static int async_prep_begin(ibpm_batch* B) {
     B->upending[p] = false;
     B->ustage[p].used = 0;
     B->ustage[p].pending = false;
-    for (auto& F : B->flows) {
-        F.stage2[p].used = 0;
-        F.stage2[p].pending = false;
-    }
+    B->upieces.clear();
     B->async_up = true;
     return 0;
 }
 
 static int async_prep_end(ibpm_batch* B) {
+    CKR(async_flush(B));
     B->async_up = false;
     CK(hipEventRecord(B->uev[B->upar], B->ctx->c.stream));
     B->upending[B->upar] = true;
@@ -918,14 +955,18 @@ static void compute_constraint(const Flow& F, const RegLists& R, const std::vect
 
 // Uploads the constraint of F's current lists and velocities (computed ahead
 // by kinematics_ahead where it took that route).
-static int upload_constraint(hipStream_t s, Flow& F, HostStage& st) {
-    std::vector<double> b;
+static void take_constraint(Flow& F, std::vector<double>& b) {
     if (F.b_host_ready) {
         b.swap(F.b_host);
         F.b_host_ready = false;
     } else {
         compute_constraint(F, F.reg, F.pu, F.pv, b);
     }
+}
+
+static int upload_constraint(hipStream_t s, Flow& F, HostStage& st) {
+    std::vector<double> b;
+    take_constraint(F, b);
     return stage_upload(s, st, F.b, b.data(), sizeof(double) * b.size());
 }
 
@@ -1019,19 +1060,22 @@ static int upload_regularizer(ibpm_batch* B, Flow& F) {
         return -3;
     }
     hipStream_t s = B->ctx->c.stream;
-    // during a substep's prep: this substep's pinned half (async_prep_begin
-    // made it free); else the flow's own buffer, once its last copies have run
-    HostStage& st = B->async_up ? F.stage2[B->upar] : F.stage;
+    // during a substep's prep: pieces of its pinned half (async_piece); else
+    // the flow's own buffer, once its last copies have run
+    HostStage& st = F.stage;
     if (!B->async_up) CKR(stage_begin(s, st));
-    CKR(stage_upload(s, st, F.e_ptr, R.e_ptr.data(), sizeof(int) * R.e_ptr.size()));
-    CKR(stage_upload(s, st, F.e_f, R.e_f.data(), sizeof(int) * R.e_f.size()));
-    CKR(stage_upload(s, st, F.e_w, R.e_w.data(), sizeof(double) * R.e_w.size()));
-    CKR(stage_upload(s, st, F.t_ptr, R.t_ptr.data(), sizeof(int) * R.t_ptr.size()));
-    CKR(stage_upload(s, st, F.t_k, R.t_k.data(), sizeof(int) * R.t_k.size()));
-    CKR(stage_upload(s, st, F.t_w, R.t_w.data(), sizeof(double) * R.t_w.size()));
-    CKR(stage_upload(s, st, F.node_idx, R.node_idx.data(), sizeof(int) * R.node_idx.size()));
-    CKR(stage_upload(s, st, F.node_nbr, R.node_nbr.data(), sizeof(int) * R.node_nbr.size()));
-    CKR(stage_upload(s, st, F.old_idx, old.data(), sizeof(int) * old.size()));
+    auto put = [&](void* d, const void* h, size_t bytes) {
+        return B->async_up ? async_piece(B, d, h, bytes) : stage_upload(s, st, d, h, bytes);
+    };
+    CKR(put(F.e_ptr, R.e_ptr.data(), sizeof(int) * R.e_ptr.size()));
+    CKR(put(F.e_f, R.e_f.data(), sizeof(int) * R.e_f.size()));
+    CKR(put(F.e_w, R.e_w.data(), sizeof(double) * R.e_w.size()));
+    CKR(put(F.t_ptr, R.t_ptr.data(), sizeof(int) * R.t_ptr.size()));
+    CKR(put(F.t_k, R.t_k.data(), sizeof(int) * R.t_k.size()));
+    CKR(put(F.t_w, R.t_w.data(), sizeof(double) * R.t_w.size()));
+    CKR(put(F.node_idx, R.node_idx.data(), sizeof(int) * R.node_idx.size()));
+    CKR(put(F.node_nbr, R.node_nbr.data(), sizeof(int) * R.node_nbr.size()));
+    CKR(put(F.old_idx, old.data(), sizeof(int) * old.size()));
     F.nold = old_nodes;
     F.nnodes = (int)R.node_idx.size();
     // k_interp: per E entry its flux index, weight and two psi values (28 B),
@@ -1043,7 +1087,9 @@ static int upload_regularizer(ibpm_batch* B, Flow& F) {
     for (int slot : R.node_nbr)
         if (slot >= 0) sm += 8.0 + 20.0 * (double)(R.t_ptr[slot + 1] - R.t_ptr[slot]);
     F.by_smear = sm;
-    return upload_constraint(s, F, st);
+    std::vector<double> b;
+    take_constraint(F, b);
+    return put(F.b, b.data(), sizeof(double) * b.size());
 }
 
 static int update_regularizer(ibpm_batch* B, Flow& F) {
@@ -1138,10 +1184,6 @@ static int alloc_flow(ibpm_batch* B, Flow& F) {
                              sizeof(int) * 8 * cap + sizeof(int4) * 4 * cap + sizeof(double) * (n + 4) + 12 * 256;
         CK(hipHostMalloc((void**)&F.stage.p, bytes, hipHostMallocDefault));
         F.stage.cap = bytes;
-        for (auto& st : F.stage2) {
-            CK(hipHostMalloc((void**)&st.p, bytes, hipHostMallocDefault));
-            st.cap = bytes;
-        }
     }
     if (F.p.model >= 1 && F.p.model <= 3) {   // base states
         const int np = F.p.model == 3 ? F.p.period : 1;
@@ -1669,6 +1711,7 @@ static int plan_local_helm(ibpm_batch* B, int i) {
             B->hclear_cap = (int)clears.size();
         }
         CKR(up(B, B->d_hclear, clears.data(), sizeof(HelmJob) * clears.size()));
+        if (B->async_up) CKR(async_flush(B));   // the pieces in place before the launch that reads them
         launch_helm_clear(B->d_hclear, (int)clears.size(), (int)((maxc + kThreads - 1) / kThreads),
                           B->ctx->c.stream);
     }
@@ -1856,13 +1899,26 @@ static int nn_samples_room(const ibpm_batch* B) {
 static int build_precond(ibpm_batch* B, int i);
 
 // Force-solve state of every flow: zero, with tol^2 in slot 1.
+// Pinned host buffers and the event of the solver polls (first use).
+static int poll_buffers(ibpm_batch* B) {
+    if (B->h_states) return 0;
+    const size_t n = 8 * B->flows.size();
+    CK(hipHostMalloc((void**)&B->h_states, sizeof(double) * n, hipHostMallocDefault));
+    CK(hipHostMalloc((void**)&B->h_reset, sizeof(double) * n, hipHostMallocDefault));
+    CK(hipEventCreateWithFlags(&B->poll_ev, hipEventDisableTiming));
+    return 0;
+}
+
+// Queued on the stream (no wait): the states are read by the kernels after
+// it, and the pinned source is rewritten only by the next solve, after polls
+// that have waited past this copy.
 static int reset_states(ibpm_batch* B) {
     hipStream_t s = B->ctx->c.stream;
     const int nfl = (int)B->flows.size();
-    std::vector<double> st(8 * (size_t)nfl, 0.0);
-    for (int k = 0; k < nfl; ++k) st[8 * k + 1] = B->flows[k].p.cg_tol * B->flows[k].p.cg_tol;
-    CK(hipMemcpyAsync(B->cg_states, st.data(), sizeof(double) * st.size(), hipMemcpyHostToDevice, s));
-    CK(hipStreamSynchronize(s));
+    CKR(poll_buffers(B));
+    for (int k = 0; k < 8 * nfl; ++k) B->h_reset[k] = 0.0;
+    for (int k = 0; k < nfl; ++k) B->h_reset[8 * k + 1] = B->flows[k].p.cg_tol * B->flows[k].p.cg_tol;
+    CK(hipMemcpyAsync(B->cg_states, B->h_reset, sizeof(double) * 8 * nfl, hipMemcpyHostToDevice, s));
     return 0;
 }
 
@@ -1870,13 +1926,28 @@ static double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-static int read_states(ibpm_batch* B, std::vector<double>& st) {
+// A poll in two halves: the states copied to pinned memory right behind the
+// work queued so far (poll_begin), so that the GPU may be handed more work
+// before the host waits for the copy (poll_wait).
+static int poll_begin(ibpm_batch* B) {
+    hipStream_t s = B->ctx->c.stream;
+    CKR(poll_buffers(B));
+    CK(hipMemcpyAsync(B->h_states, B->cg_states, sizeof(double) * 8 * B->flows.size(), hipMemcpyDeviceToHost, s));
+    CK(hipEventRecord(B->poll_ev, s));
+    return 0;
+}
+
+static int poll_wait(ibpm_batch* B, std::vector<double>& st) {
     const double t0 = now_s();
-    CK(hipStreamSynchronize(B->ctx->c.stream));
-    st.resize(8 * B->flows.size());
-    CK(hipMemcpy(st.data(), B->cg_states, sizeof(double) * st.size(), hipMemcpyDeviceToHost));
+    CK(hipEventSynchronize(B->poll_ev));
+    st.assign(B->h_states, B->h_states + 8 * B->flows.size());
     B->st_host_poll_s += now_s() - t0;
     return 0;
+}
+
+static int read_states(ibpm_batch* B, std::vector<double>& st) {
+    CKR(poll_begin(B));
+    return poll_wait(B, st);
 }
 
 // q = M x0 for r0 = b - q.  A zero start (warm start mode 0) needs no M
@@ -1887,7 +1958,13 @@ static int start_residual(ibpm_batch* B, int i) {
         for (auto& F : B->flows) CK(hipMemsetAsync(F.cg_q, 0, sizeof(double) * 2 * F.nb, s));
         return 0;
     }
-    for (auto& F : B->flows) CK(hipMemcpyAsync(F.cg_d, F.f, sizeof(double) * 2 * F.nb, hipMemcpyDeviceToDevice, s));
+    if (!B->d_start_pieces) {   // cg_d = f of every flow, one launch
+        std::vector<CopyPiece> pc;
+        for (auto& F : B->flows) pc.push_back(CopyPiece{(const unsigned int*)F.f, (unsigned int*)F.cg_d, 4L * F.nb});
+        CKR(B->mem.alloc(&B->d_start_pieces, pc.size()));
+        CKR(upload(B->d_start_pieces, pc.data(), sizeof(CopyPiece) * pc.size()));
+    }
+    launch_copy_pieces(B->d_start_pieces, (int)B->flows.size(), s);
     enqueue_M(B, i, B->d_smear_d[i], s);
     B->st_mapps += 1;
     return 0;
@@ -1906,19 +1983,25 @@ static int solve_cg(ibpm_batch* B, int i, int* iters) {
     std::vector<char> active(nfl, 1);
     const int max_iter = 100000;
     const int chunk = pr ? 1 : B->cg_chunk;   // measuring: exact active flows per launch
-    for (int it = 0; it < max_iter; it += chunk) {
-        CKR(read_states(B, st));
+    auto iterate = [&] {
+        enqueue_M(B, i, B->d_smear_d[i], s, pr ? &active : nullptr);
+        timed(pr, K_CG, 0, s, [&] { launch_cg(B->d_cg, nfl, 1, s); });
+        B->st_mapps += 1;
+    };
+    // (not measuring: each poll reads the states while the first iteration of
+    // the next chunk runs; past convergence it is a no-op)
+    for (int it = 0; it < max_iter;) {
+        CKR(poll_begin(B));
+        if (!pr) iterate();
+        CKR(poll_wait(B, st));
         bool all = true;
         for (int k = 0; k < nfl; ++k) {
             active[k] = st[8 * k + 3] == 0.0;
             if (active[k]) all = false;
         }
         if (all) break;
-        for (int c = 0; c < chunk; ++c) {
-            enqueue_M(B, i, B->d_smear_d[i], s, pr ? &active : nullptr);
-            timed(pr, K_CG, 0, s, [&] { launch_cg(B->d_cg, nfl, 1, s); });
-            B->st_mapps += 1;
-        }
+        for (int c = pr ? 0 : 1; c < chunk; ++c) iterate();
+        it += chunk;
     }
     CKR(read_states(B, st));
     for (int k = 0; k < nfl; ++k) iters[k] += (int)st[8 * k + 2];
@@ -1954,22 +2037,37 @@ static int solve_pcg(ibpm_batch* B, int i, int* iters, bool* fallback) {
     const int limit = std::max(100, 2 * B->max_n);
     int chunk = pr ? 0 : std::max(0, B->pc_last[i] - 1), done = 0;
     std::vector<double> st;
+    auto iterate = [&] {
+        enqueue_M(B, i, B->d_smear_d[i], s, pr ? &active : nullptr);
+        timed(pr, K_CG, 0, s, [&] { launch_pcg_step(ops, nfl, s); });
+        precondition(0);
+        B->st_mapps += 1;
+    };
+    bool ahead = false;   // an iteration queued after the states being read
     for (;;) {
-        for (int c = 0; c < chunk; ++c) {
-            enqueue_M(B, i, B->d_smear_d[i], s, pr ? &active : nullptr);
-            timed(pr, K_CG, 0, s, [&] { launch_pcg_step(ops, nfl, s); });
-            precondition(0);
-            B->st_mapps += 1;
-        }
+        for (int c = 0; c < chunk; ++c) iterate();
         done += chunk;
-        chunk = pr ? 1 : 2;
-        CKR(read_states(B, st));
+        if (pr) {   // measuring: the states of every launch exactly
+            chunk = 1;
+            CKR(read_states(B, st));
+        } else {    // the next iteration runs while the host waits for the states
+            CKR(poll_begin(B));
+            iterate();
+            ahead = true;
+            done += 1;
+            chunk = 1;
+            CKR(poll_wait(B, st));
+        }
         bool all = true;
         for (int k = 0; k < nfl; ++k) {
             active[k] = st[8 * k + 3] == 0.0 && st[8 * k + 4] == 0.0;
             if (active[k]) all = false;
         }
-        if (all || done >= limit) break;
+        if (all) break;   // the iteration ahead found every flow finished: a no-op
+        if (done >= limit) {
+            if (ahead) CKR(read_states(B, st));   // counts and flags after the iteration ahead
+            break;
+        }
     }
     B->pc_par[i] = 1 - par;
     *fallback = false;
@@ -2077,12 +2175,18 @@ static int substep_cg(ibpm_batch* B, int i) {
         CKR(upload_nn(B, i, true));
         launch_nn_features(B->d_nn, nfl, s);
     }
-    for (auto& F : B->flows) {
-        const size_t n = 2 * F.nb + 1;
-        const long slot = B->ksub % B->ring;
-        CK(hipMemcpyAsync(F.ring_f + slot * n, F.f, sizeof(double) * 2 * F.nb, hipMemcpyDeviceToDevice, s));
-        CK(hipMemcpyAsync(F.ring_b + slot * n, F.b, sizeof(double) * 2 * F.nb, hipMemcpyDeviceToDevice, s));
+    if (!B->d_ring_pieces) {   // f and b into ring slot r of every flow: pieces [r 2 nflows, (r + 1) 2 nflows)
+        std::vector<CopyPiece> pc;
+        for (int r = 0; r < B->ring; ++r)
+            for (auto& F : B->flows) {
+                const size_t n = 2 * F.nb + 1;
+                pc.push_back(CopyPiece{(const unsigned int*)F.f, (unsigned int*)(F.ring_f + r * n), 4L * F.nb});
+                pc.push_back(CopyPiece{(const unsigned int*)F.b, (unsigned int*)(F.ring_b + r * n), 4L * F.nb});
+            }
+        CKR(B->mem.alloc(&B->d_ring_pieces, pc.size()));
+        CKR(upload(B->d_ring_pieces, pc.data(), sizeof(CopyPiece) * pc.size()));
     }
+    launch_copy_pieces(B->d_ring_pieces + (size_t)(B->ksub % B->ring) * 2 * nfl, 2 * nfl, s);
     B->ksub += 1;
     timed(pr, K_SMEAR, pr ? reg_bytes(B, true, false) : 0, s,
           [&] { launch_smear_curl(B->d_smear_f[i], nfl, B->max_nodes, s); });
@@ -3506,6 +3610,9 @@ int ibpm_batch_destroy(ibpm_batch* b) {
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
     for (auto e : b->uev)
         if (e) (void)hipEventDestroy(e);
+    if (b->poll_ev) (void)hipEventDestroy(b->poll_ev);
+    if (b->h_states) (void)hipHostFree(b->h_states);
+    if (b->h_reset) (void)hipHostFree(b->h_reset);
     delete b;
     return 0;
 }

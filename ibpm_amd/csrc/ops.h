@@ -120,6 +120,14 @@ struct UnitOps {
     int n;
 };
 
+// One contiguous copy of n 4-byte words (k_copy_pieces: many copies in one
+// launch instead of one runtime copy each).
+struct CopyPiece {
+    const unsigned int* src;
+    unsigned int* dst;
+    long n;
+};
+
 struct ForceOps {
     int nb;
     const double* f;
@@ -177,6 +185,7 @@ void launch_pcg_step(const PcgOps* d, int nflows, hipStream_t s);
 void launch_pcg_dir(const PcgOps* d, int nflows, int begin, hipStream_t s);
 void launch_forces(const ForceOps* d, int nflows, hipStream_t s);
 void launch_unit_column(const UnitOps* d, int nflows, int j, hipStream_t s);
+void launch_copy_pieces(const CopyPiece* d, int npieces, hipStream_t s);
 void launch_reg_to_flux(const RegFluxOps* d, int nslots, const double* f, double* q, hipStream_t s);
 void launch_reg_to_boundary(const RegFluxOps* d, int nb, const double* q, double* f, hipStream_t s);
 

@@ -979,6 +979,17 @@ void launch_unit_column(const UnitOps* d, int nflows, int j, hipStream_t s) {
     hipLaunchKernelGGL(k_unit_column, dim3((nflows + 63) / 64), dim3(64), 0, s, d, nflows, j);
 }
 
+// Piece blockIdx.y, strided over gridDim.x workgroups.
+__global__ void __launch_bounds__(kThreads) k_copy_pieces(const CopyPiece* __restrict__ pieces) {
+    const CopyPiece P = pieces[blockIdx.y];
+    for (long w = (long)blockIdx.x * kThreads + threadIdx.x; w < P.n; w += (long)gridDim.x * kThreads)
+        P.dst[w] = P.src[w];
+}
+
+void launch_copy_pieces(const CopyPiece* d, int npieces, hipStream_t s) {
+    if (npieces > 0) hipLaunchKernelGGL(k_copy_pieces, dim3(16, npieces), dim3(kThreads), 0, s, d);
+}
+
 void launch_interp(const RegOps* d, int nflows, int max_comp, hipStream_t s) {
     hipLaunchKernelGGL(k_interp, dim3(blocks_for(max_comp), nflows), dim3(kThreads), 0, s, d);
 }

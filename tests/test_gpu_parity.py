@@ -455,3 +455,23 @@ def test_regularizer_matches_oracle(gpu, ctx, oracle, geom):
         q = rng.standard_normal(ngrid * (2 * nx * ny + nx + ny))
         assert rel(reg.to_flux(f), oracle.reg_toflux(nx, ny, ngrid, dx, assoc, f)) < 1e-14
         assert rel(reg.to_boundary(q), oracle.reg_toboundary(nb, dx, assoc, q)) < 1e-14
+
+
+@pytest.mark.parametrize("motion", ["eldredge2 0.3 11 0 0.02 0.03 0.05", "pitchplunge 0.3 0.5 0 0.1 0.5 0.2"])
+def test_set_state_after_advancing_undoes_the_move_made_ahead(gpu, motion):
+    """Each substep moves the bodies of the next one ahead of time; set_state
+    at the current timestep undoes that move, so that a batch given its own
+    state back continues exactly like one that was not (for Eldredge2, whose
+    motion integrates over successive calls, a kept move would shift it)."""
+    geom = "body P\n line_n -0.5 0 0.5 0 16\n center 0.25 0\n motion %s\nend\n" % motion
+    ctx = gpu.Context(0)
+    fl = gpu.Flow(64, 64, 3, 4.0, -2.0, -2.0, 300.0, 0.01, geom, tol=1e-10)
+    a = gpu.Batch(ctx, [fl]).init()
+    b = gpu.Batch(ctx, [fl]).init()
+    a.advance(2)
+    b.advance(2)
+    a.set_state(a.omega(), a.f(), a.time, 2)
+    a.advance(2)
+    b.advance(2)
+    assert np.array_equal(a.omega(), b.omega())
+    assert np.array_equal(a.force_history(), b.force_history())

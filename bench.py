@@ -175,8 +175,14 @@ def accuracy_vs_oracle(b, workload):
     return out
 
 
-def moving_form(ctx, flows, warmup, steps, precond="lowrank", pairs=8, local=True, warm="previous",
-                nn=None, build="finest", rebuild_cost=0.45):
+REBUILD_COST = {"all": 1.0, "finest": 0.45, "table": 0.1, "tracked": 0.1}
+P_SOURCE = {"all": "M", "finest": "M's finest-level form",
+            "table": "the kernel-table M at the build's points",
+            "tracked": "the kernel-table M, refreshed every substep by a Newton-Schulz step"}
+
+
+def moving_form(ctx, flows, warmup, steps, precond="dense", pairs=0, local=True, warm="previous",
+                nn=None, build="tracked", rebuild_cost=0.1):
     """One form of the moving-body solver on `flows` from a zero initial
     vorticity: warmup steps (the first builds the preconditioner), then
     `steps` timed steps, with the preconditioner rebuilds that fall due inside
@@ -235,7 +241,7 @@ def train_network(ctx, flows, train_steps):
     samples of a separate run of the workload's flows (its first train_steps
     steps), by the library's own training kernels; returns its parameters."""
     import ibpm_amd
-    b = ibpm_amd.Batch(ctx, flows).init().set_precond("lowrank", 8)
+    b = ibpm_amd.Batch(ctx, flows).init().set_precond("dense", 0).set_precond_build("tracked", 0.1)
     b.collect(True, 4 << 20)
     b.advance(train_steps)
     t0 = time.perf_counter()
@@ -259,23 +265,26 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--precond", default="lowrank", choices=["none", "stale", "lowrank"],
+    ap.add_argument("--precond", default="dense", choices=["none", "dense", "stale", "lowrank"],
                     help="force solve of moving bodies: plain CG (the reference's), CG preconditioned "
-                         "by a stale FP64 inverse of M, or that plus the low-rank update")
+                         "by the dense FP64 P alone (dense; stale is its former name), or P plus the "
+                         "low-rank update of --pairs Krylov pairs")
     ap.add_argument("--pairs", type=int, default=8)
     ap.add_argument("--warm", default="learned", choices=["previous", "zero", "learned"],
                     help="CG start of the timed moving-body form (previous = the reference's)")
     ap.add_argument("--train-steps", type=int, default=30,
                     help="steps of the training run of the learned warm start")
-    ap.add_argument("--precond-build", default="finest", choices=["all", "finest"],
-                    help="P of the preconditioned forms from M itself or from its finest-level form")
-    ap.add_argument("--rebuild-cost", type=float, default=0.45,
+    ap.add_argument("--precond-build", default="tracked", choices=sorted(P_SOURCE),
+                    help="P of the preconditioned forms: " + "; ".join("%s: %s" % kv for kv in sorted(P_SOURCE.items())))
+    ap.add_argument("--rebuild-cost", type=float, default=None,
                     help="a build's cost in M applications per unknown, the rebuild rule's break-even "
-                         "(tandem batch: 0.25 and 0.45 give the same throughput, 0.15 rebuilds too often)")
+                         "(default per build: " + ", ".join("%s %g" % kv for kv in sorted(REBUILD_COST.items())) + ")")
     ap.add_argument("--no-forms", action="store_true",
                     help="moving bodies: time only the headline form (no plain, batched-plain or "
                          "previous-start forms beside it)")
     args = ap.parse_args()
+    if args.rebuild_cost is None:
+        args.rebuild_cost = REBUILD_COST[args.precond_build]
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
         sys.exit(2)
@@ -355,7 +364,7 @@ def main():
         "config": {"workload": args.workload, "description": CONFIG_TEXT[args.workload],
                    "flows": len(flows), "grid_points": gp, "parallelism": "single GPU",
                    "form": ("batched, preconditioned (P from %s), %s warm start, rebuilds inside the timed steps"
-                            % ("M" if args.precond_build == "all" else "M's finest-level form", args.warm))
+                            % (P_SOURCE[args.precond_build], args.warm))
                            if moving else "graph replay",
                    "solver": solver, "learned": learn, "forms": forms,
                    "accuracy_vs_oracle_fixture": accuracy,

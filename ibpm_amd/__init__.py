@@ -100,6 +100,7 @@ def load():
         "ibpm_nn_loss_grad": [vp, vp, vp, vp, ctypes.c_long, vp, _D],
         "ibpm_batch_set_precond": [vp, ctypes.c_int, ctypes.c_int],
         "ibpm_batch_set_precond_build": [vp, ctypes.c_int, ctypes.c_double],
+        "ibpm_batch_precond_matrices": [vp, ctypes.c_int, ctypes.c_int, vp, vp],
         "ibpm_batch_set_local_helmholtz": [vp, ctypes.c_int],
         "ibpm_batch_cholesky_cache": [vp, ctypes.c_int, ctypes.c_char_p],
         "ibpm_batch_cholesky_loaded": [vp, ctypes.c_int],
@@ -468,23 +469,45 @@ class Batch:
 
     # -- preconditioned CG force solve (moving bodies) ----------------------
     def set_precond(self, mode="lowrank", pairs=8):
-        """mode "none": plain CG (the reference's solver); "stale": CG
-        preconditioned by the FP64 inverse of M at an earlier body position;
-        "lowrank": that plus a low-rank update from the previous solve's
-        search directions (`pairs` of them, at most 8)."""
-        m = {"none": 0, "stale": 1, "lowrank": 1}[mode]
+        """mode "none": plain CG (the reference's solver); "dense": CG
+        preconditioned by the dense FP64 P alone (its source and refresh:
+        set_precond_build); "stale": the same (P from an earlier body position
+        unless refreshed); "lowrank": P plus a low-rank update from the
+        previous solve's search directions (`pairs` of them, at most 8)."""
+        m = {"none": 0, "dense": 1, "stale": 1, "lowrank": 1}[mode]
         k = pairs if mode == "lowrank" else 0
         _check(load().ibpm_batch_set_precond(self.h, m, k))
         return self
 
     def set_precond_build(self, levels="all", rebuild_cost=1.0):
         """How the preconditioner's P is built: from M ("all" levels of the
-        L^-1 solve, the default) or from M's finest-level form ("finest": L^-1
+        L^-1 solve, the default), from M's finest-level form ("finest": L^-1
         on level 0 under zero boundary values, a fraction of the cost per
-        column; needs the local Helmholtz path).  rebuild_cost: a build's cost
-        in M applications per unknown, the rebuild rule's break-even."""
-        _check(load().ibpm_batch_set_precond_build(self.h, {"all": 0, "finest": 1}[levels], float(rebuild_cost)))
+        column; needs the local Helmholtz path), from the kernel table
+        ("table": M = beta E G E^T with G read off its response to a unit flux
+        at the grid's centre, no M applications), or from the table and
+        refreshed before every solve by a Newton-Schulz step toward the
+        table's M at the substep's points ("tracked").  rebuild_cost: a
+        build's cost in M applications per unknown, the rebuild rule's
+        break-even."""
+        code = {"all": 0, "finest": 1, "table": 2, "tracked": 3}[levels]
+        _check(load().ibpm_batch_set_precond_build(self.h, code, float(rebuild_cost)))
         return self
+
+    def precond_matrices(self, flow=0, substep=2):
+        """(kernel-table M of `substep` at the flow's current points, the P in
+        use scaled to that substep or None), both 2 nb x 2 nb.  After advance,
+        the points are those of the scheme's last substep (2 for RK3)."""
+        n = 2 * self.num_points(flow)
+        sub = substep
+        m = np.zeros((n, n))
+        p = np.zeros((n, n))
+        _check(load().ibpm_batch_precond_matrices(self.h, flow, sub, m.ctypes.data, None))
+        try:
+            _check(load().ibpm_batch_precond_matrices(self.h, flow, sub, None, p.ctypes.data))
+        except IbpmError:
+            p = None
+        return m, p
 
     PRECISION = {"fp64": 0, "fp32_step": 1, "fp32": 2, "reduced_tuned": 3, "gemm_step": 4}
 
@@ -518,7 +541,7 @@ class Batch:
 
     SOLVER_STATS = ("iterations", "solves", "m_applications", "build_m_applications", "builds",
                     "fallbacks", "abandoned", "host_us_kinematics", "host_us_polls", "local_helmholtz_m_applications",
-                    "build_us", "build_host_us")
+                    "build_us", "build_host_us", "refreshes")
 
     def solver_stats(self):
         out = np.zeros(len(self.SOLVER_STATS), dtype=np.int64)

@@ -232,13 +232,13 @@ struct SolveSpec {
 enum KClass { K_PASS1, K_PASS2, K_PASS3, K_COARSIFY, K_VEL, K_CROSS, K_RHS, K_CURLQ, K_INTERP,
               K_SMEAR, K_MINV, K_CG, K_FORCES, K_CLEAR, K_FSPASS1, K_FSPASS2, K_FSPASS3, K_FSCOARSIFY,
               K_FSHELM, K_HELM, K_PASS32_1, K_PASS32_2, K_PASS32_3, K_FSPASS32_1, K_FSPASS32_2,
-              K_FSPASS32_3, K_GEMM, K_NCLASS };
+              K_FSPASS32_3, K_GEMM, K_PCASM, K_PCNS, K_NCLASS };
 static const char* kclass_names[K_NCLASS] = {"ell_pass1", "ell_pass2", "ell_pass3", "coarsify", "vel",
                                              "cross", "rhs", "curl_q", "interp", "smear_curl", "minv",
                                              "cg", "forces", "clear_nodes", "fs_pass1", "fs_pass2",
                                              "fs_pass3", "fs_coarsify", "fs_helm", "helm", "ell32_pass1",
                                              "ell32_pass2", "ell32_pass3", "fs32_pass1", "fs32_pass2",
-                                             "fs32_pass3", "ell_gemm"};
+                                             "fs32_pass3", "ell_gemm", "pc_assemble", "pc_refresh"};
 struct Probe {
     struct Rec { int cls; hipEvent_t a, b; double bytes; };
     std::vector<Rec> recs;
@@ -624,6 +624,11 @@ struct Flow {
     double *lmp_s[4][2] = {}, *lmp_y[4][2] = {};
     int* lmp_cnt = nullptr;
     double pbeta = 0.0;            // beta of the substep P was built for
+    // P from the kernel table (pc_build_levels 2, 3): G's responses per
+    // substep (shared by the flows of one grid, Re and dt), and the product
+    // buffer of P's Newton-Schulz refresh
+    const double* ktab[4] = {nullptr, nullptr, nullptr, nullptr};
+    double* pc_T = nullptr;
     std::string chol_cache;        // basename of the .cholesky factor files ("" = none)
     int chol_loaded = 0;           // substep factors read from those files
     // regulariser
@@ -758,9 +763,25 @@ struct ibpm_batch {
     int pc_base[4] = {-1, -1, -1, -1};   // iterations of the first solve after a build
     int pc_last[4] = {0, 0, 0, 0};
     double pc_excess = 0.0;              // iterations above pc_base since the build
-    // P from M (0) or from its finest-level form (1): M with L^-1 on level 0
-    // alone, zero boundary values (plan M1L, build_inverse)
+    // P from M (0), from its finest-level form (1: M with L^-1 on level 0
+    // alone, zero boundary values; plan M1L, build_inverse), or from the
+    // kernel table (2; 3: and refreshed every substep by one Newton-Schulz
+    // step against the table's M at the substep's points)
     int pc_build_levels = 0;
+    // kernel tables: per grid class, the level-0 flux responses of G (the
+    // flux-to-flux operator inside M, without beta) to a unit X and a unit Y
+    // flux at the centre, per substep
+    struct KernTab {
+        int nx, ny, ngrid;
+        double dx, re, dt;
+        double* tab[4];
+    };
+    std::vector<KernTab> ktabs;
+    ibpmg::PcAsmOps* d_pcasm[4] = {};     // the refresh's assembly per substep (scale pbeta)
+    ibpmg::PcAsmOps* d_pcasm_b = nullptr; // a build's assembly
+    ibpmg::GemmOps *d_ns1 = nullptr, *d_ns2 = nullptr;
+    ibpmg::CopyPiece* d_ns_copy = nullptr;
+    long st_refreshes = 0;
     double pc_build_cost = 1.0;          // a build's cost in M applications per unknown (rebuild rule)
     ibpmg::SolvePlan M1L;
     bool m1l_built = false;
@@ -1897,6 +1918,7 @@ static int nn_samples_room(const ibpm_batch* B) {
 }
 
 static int build_precond(ibpm_batch* B, int i);
+static int refresh_precond(ibpm_batch* B, int i);
 
 // Force-solve state of every flow: zero, with tol^2 in slot 1.
 // Pinned host buffers and the event of the solver polls (first use).
@@ -2144,6 +2166,7 @@ static int substep_cg(ibpm_batch* B, int i) {
     }
     std::vector<int> its(nfl, 0);
     if (B->pc_mode && !B->pc_broken && !B->pc_valid) CKR(build_precond(B, i));
+    else if (B->pc_mode && !B->pc_broken && B->pc_build_levels == 3) CKR(refresh_precond(B, i));
     if (B->pc_mode && !B->pc_broken) {
         bool fallback = false;
         CKR(solve_pcg(B, i, its.data(), &fallback));
@@ -2366,6 +2389,67 @@ static bool cholesky_read(const std::string& path, int n, double alpha_beta, std
 // substitution (:200-224) applied to unit vectors.  The inverse goes to
 // F.minv[i] (Cholesky force solve) or, with precond, to F.pinv.  Returns -5
 // with precond when some M is not positive definite.
+}  // namespace ibpmg
+extern "C" {
+static int solve_common(ibpm_ctx* c, int kind, int nx, int ny, int ngrid, double dx, double alpha,
+                        const double* d_f, const double* d_bc, double* d_u, double in_scale, int precision);
+}
+namespace ibpmg {
+
+// The kernel tables of every flow (once per grid class): for each substep i,
+// G_i f = Curl L^-1 (-A_i^-1 Curl f) on level 0 (M_i = beta_i E G_i E^T,
+// ProjectionSolver.cc:96-101 without beta) applied to a unit X flux and to a
+// unit Y flux at node (nx/2, ny/2) of level 0, through the library's own curl,
+// multi-domain Helmholtz solve and flux-from-vorticity; the level-0 part of
+// each response is kept.
+static int kernel_tables(ibpm_batch* B) {
+    hipStream_t s = B->ctx->c.stream;
+    for (auto& F : B->flows) {
+        if (F.ktab[0]) continue;
+        ibpm_batch::KernTab* T = nullptr;
+        for (auto& K : B->ktabs)
+            if (K.nx == F.nx && K.ny == F.ny && K.ngrid == F.L && K.dx == F.dx && K.re == F.p.reynolds &&
+                K.dt == F.p.dt)
+                T = &K;
+        if (!T) {
+            ibpm_batch::KernTab K{F.nx, F.ny, F.L, F.dx, F.p.reynolds, F.p.dt, {nullptr, nullptr, nullptr, nullptr}};
+            const long nf1 = 2L * F.nx * F.ny + F.nx + F.ny, nfx = (long)(F.nx + 1) * F.ny;
+            DevMem tmp;
+            double *q = nullptr, *w = nullptr, *u = nullptr, *r = nullptr;
+            CKR(tmp.alloc(&q, nf1 * F.L)); CKR(tmp.alloc(&r, nf1 * F.L));
+            CKR(tmp.alloc(&w, F.ns * F.L)); CKR(tmp.alloc(&u, F.ns * F.L));
+            const long unit[2] = {(long)(F.nx / 2) * F.ny + F.ny / 2, nfx + (long)(F.nx / 2) * (F.ny + 1) + F.ny / 2};
+            const double one = 1.0;
+            for (int i = 0; i < B->nsub; ++i) {
+                CKR(B->mem.alloc(&K.tab[i], 2 * nf1));
+                for (int comp = 0; comp < 2; ++comp) {
+                    CK(hipMemsetAsync(q, 0, sizeof(double) * nf1 * F.L, s));
+                    CK(hipMemcpyAsync(q + unit[comp], &one, sizeof(double), hipMemcpyHostToDevice, s));
+                    CKR(ibpm_curl_flux(B->ctx, F.nx, F.ny, F.L, F.dx, q, w));
+                    CKR(solve_common(B->ctx, 1, F.nx, F.ny, F.L, F.dx, -beta_of(B, F, i) / 2. / F.p.reynolds, w,
+                                     nullptr, u, 1.0, 0));
+                    CKR(ibpm_flux_from_vorticity(B->ctx, F.nx, F.ny, F.L, F.dx, 0.0, 0.0, u, r));
+                    CK(hipMemcpyAsync(K.tab[i] + comp * nf1, r, sizeof(double) * nf1, hipMemcpyDeviceToDevice, s));
+                }
+            }
+            CK(hipStreamSynchronize(s));
+            B->ktabs.push_back(K);
+            T = &B->ktabs.back();
+        }
+        for (int i = 0; i < B->nsub; ++i) F.ktab[i] = T->tab[i];
+    }
+    return 0;
+}
+
+// Assembly descriptors of the kernel-table matrix of substep i, scaled by
+// scale(F), into each flow's mcol.
+static std::vector<PcAsmOps> pc_asm_ops(ibpm_batch* B, int i, const std::function<double(const Flow&)>& scale) {
+    std::vector<PcAsmOps> ops;
+    for (auto& F : B->flows)
+        ops.push_back(PcAsmOps{2 * F.nb, F.nx, F.ny, F.e_ptr, F.e_f, F.e_w, F.ktab[i], scale(F), F.mcol});
+    return ops;
+}
+
 static int build_inverse(ibpm_batch* B, int i, bool precond) {
     hipStream_t s = B->ctx->c.stream;
     const int nfl = (int)B->flows.size();
@@ -2429,7 +2513,17 @@ static int build_inverse(ibpm_batch* B, int i, bool precond) {
                 }
             }
     }
-    if (!all_cached) {
+    const bool table = precond && B->pc_build_levels >= 2;
+    if (table) {
+        // the matrix from the kernel table at the current points (no M
+        // applications): M_i = beta_i E G_i E^T
+        CKR(kernel_tables(B));
+        if (!B->d_pcasm_b) CKR(B->mem.alloc(&B->d_pcasm_b, (size_t)nfl));
+        const auto ops = pc_asm_ops(B, i, [&](const Flow& F) { return beta_of(B, F, i); });
+        CK(hipStreamSynchronize(s));
+        CKR(upload(B->d_pcasm_b, ops.data(), sizeof(PcAsmOps) * nfl));
+        launch_pc_assemble(B->d_pcasm_b, nfl, nmax, s);
+    } else if (!all_cached) {
         // one smear descriptor per flow (the unit vector moves in place), one
         // interpolation descriptor per column and flow (column j of M into
         // mcol; column n is a dummy), so that the columns run without host
@@ -2548,7 +2642,7 @@ static int build_precond(ibpm_batch* B, int i) {
     B->pc_t_build = now_s() - t0;
     B->st_build_s += B->pc_t_build;
     B->st_builds += 1;
-    B->st_build_mapps += B->max_n;
+    if (B->pc_build_levels < 2) B->st_build_mapps += B->max_n;
     if (rc == -5) {
         B->pc_broken = true;
         return 0;
@@ -2562,9 +2656,55 @@ static int build_precond(ibpm_batch* B, int i) {
         }
         CKR(upload(B->d_pminv[j], mo.data(), sizeof(MinvOps) * nfl));
     }
+    if (B->pc_build_levels == 3) {
+        // the refresh: mcol = pbeta E G_j E^T at substep j's points (P's
+        // scale to substep j stays pbeta / beta_j), pc_T = mcol P, mcol = 2 P
+        // - P pc_T, P = mcol
+        for (int j = 0; j < B->nsub; ++j) {
+            if (!B->d_pcasm[j]) CKR(B->mem.alloc(&B->d_pcasm[j], (size_t)nfl));
+            const auto ops = pc_asm_ops(B, j, [](const Flow& F) { return F.pbeta; });
+            CKR(upload(B->d_pcasm[j], ops.data(), sizeof(PcAsmOps) * nfl));
+        }
+        std::vector<GemmOps> g1(nfl), g2(nfl);
+        std::vector<CopyPiece> cp(nfl);
+        for (int k = 0; k < nfl; ++k) {
+            Flow& F = B->flows[k];
+            const int n = 2 * F.nb;
+            if (!F.pc_T) CKR(B->mem.alloc(&F.pc_T, (size_t)n * n));
+            g1[k] = GemmOps{n, F.mcol, F.pinv, nullptr, F.pc_T, 1.0, 0.0};
+            g2[k] = GemmOps{n, F.pinv, F.pc_T, F.pinv, F.mcol, -1.0, 2.0};
+            cp[k] = CopyPiece{(const unsigned int*)F.mcol, (unsigned int*)F.pinv, 2L * n * n};
+        }
+        if (!B->d_ns1) {
+            CKR(B->mem.alloc(&B->d_ns1, (size_t)nfl));
+            CKR(B->mem.alloc(&B->d_ns2, (size_t)nfl));
+            CKR(B->mem.alloc(&B->d_ns_copy, (size_t)nfl));
+        }
+        CKR(upload(B->d_ns1, g1.data(), sizeof(GemmOps) * nfl));
+        CKR(upload(B->d_ns2, g2.data(), sizeof(GemmOps) * nfl));
+        CKR(upload(B->d_ns_copy, cp.data(), sizeof(CopyPiece) * nfl));
+    }
     B->pc_valid = true;
     B->pc_excess = 0.0;
     for (int j = 0; j < 4; ++j) B->pc_base[j] = -1;
+    return 0;
+}
+
+// One Newton-Schulz step of P toward the inverse of the kernel-table matrix
+// at substep i's points, P <- P (2 I - M~ P) (symmetric), before the solve:
+// the error I - M~ P squares, so P follows the bodies from substep to substep
+// at the cost of the assembly and two batched GEMMs.
+static int refresh_precond(ibpm_batch* B, int i) {
+    hipStream_t s = B->ctx->c.stream;
+    const int nfl = (int)B->flows.size();
+    Probe* pr = B->probe;
+    timed(pr, K_PCASM, 0, s, [&] { launch_pc_assemble(B->d_pcasm[i], nfl, B->max_n, s); });
+    timed(pr, K_PCNS, 0, s, [&] {
+        launch_dgemm(B->d_ns1, nfl, B->max_n, false, s);
+        launch_dgemm(B->d_ns2, nfl, B->max_n, true, s);
+        launch_copy_pieces(B->d_ns_copy, nfl, s);
+    });
+    B->st_refreshes += 1;
     return 0;
 }
 
@@ -3924,7 +4064,11 @@ int ibpm_batch_set_precond(ibpm_batch* B, int mode, int pairs) {
 
 int ibpm_batch_set_precond_build(ibpm_batch* B, int levels, double rebuild_cost) {
     if (!B) { set_error("null batch"); return -1; }
-    if (levels < 0 || levels > 1) { set_error("preconditioner build levels must be 0 (all) or 1 (finest)"); return -1; }
+    if (levels < 0 || levels > 3) {
+        set_error("preconditioner build must be 0 (all levels), 1 (finest level), 2 (kernel table) or 3 (kernel "
+                  "table, refreshed every substep)");
+        return -1;
+    }
     if (!(rebuild_cost > 0.0)) { set_error("rebuild cost must be positive"); return -1; }
     if (levels == 1 && !B->helm_allowed) {
         set_error("the finest-level build needs the local Helmholtz path of the force solve");
@@ -4002,13 +4146,43 @@ int ibpm_batch_set_local_helmholtz(ibpm_batch* B, int on) {
     return 0;
 }
 
+int ibpm_batch_precond_matrices(ibpm_batch* B, int flow, int substep, double* h_mtable, double* h_p) {
+    if (!B) { set_error("null batch"); return -1; }
+    if (flow < 0 || flow >= (int)B->flows.size() || substep < 0 || substep >= B->nsub) {
+        set_error("flow %d or substep %d out of range", flow, substep);
+        return -1;
+    }
+    Flow& F = B->flows[flow];
+    if (F.geom.stationary || F.nb == 0) { set_error("flow %d has no moving points", flow); return -1; }
+    const int n = 2 * F.nb;
+    hipStream_t s = B->ctx->c.stream;
+    CK(hipStreamSynchronize(s));
+    if (h_mtable) {
+        CKR(kernel_tables(B));
+        if (!B->d_pcasm_b) CKR(B->mem.alloc(&B->d_pcasm_b, B->flows.size()));
+        const PcAsmOps op{n, F.nx, F.ny, F.e_ptr, F.e_f, F.e_w, F.ktab[substep], beta_of(B, F, substep), F.mcol};
+        CKR(upload(B->d_pcasm_b, &op, sizeof op));
+        launch_pc_assemble(B->d_pcasm_b, 1, n, s);
+        CK(hipStreamSynchronize(s));
+        CK(hipGetLastError());
+        CK(hipMemcpy(h_mtable, F.mcol, sizeof(double) * n * n, hipMemcpyDeviceToHost));
+    }
+    if (h_p) {
+        if (!B->pc_mode || !B->pc_valid || !F.pinv) { set_error("no preconditioner built"); return -1; }
+        CK(hipMemcpy(h_p, F.pinv, sizeof(double) * n * n, hipMemcpyDeviceToHost));
+        const double sc = F.pbeta / beta_of(B, F, substep);
+        for (long k = 0; k < (long)n * n; ++k) h_p[k] *= sc;
+    }
+    return 0;
+}
+
 int ibpm_batch_solver_stats(ibpm_batch* B, long* stats, int nstats) {
     if (!B || !stats) { set_error("null argument"); return -1; }
-    const long v[12] = {B->st_iters, B->st_solves, B->st_mapps, B->st_build_mapps, B->st_builds,
+    const long v[13] = {B->st_iters, B->st_solves, B->st_mapps, B->st_build_mapps, B->st_builds,
                         B->st_fallbacks, B->pc_broken ? 1L : 0L, (long)(1e6 * B->st_host_kin_s),
                         (long)(1e6 * B->st_host_poll_s), B->st_local_mapps, (long)(1e6 * B->st_build_s),
-                        (long)(1e6 * B->st_build_host_s)};
-    const int n = std::min(nstats, 12);
+                        (long)(1e6 * B->st_build_host_s), B->st_refreshes};
+    const int n = std::min(nstats, 13);
     for (int k = 0; k < n; ++k) stats[k] = v[k];
     return n;
 }

@@ -128,6 +128,33 @@ struct CopyPiece {
     long n;
 };
 
+// The preconditioner's matrix from a kernel table.  M = E G E^T with G the
+// level-0 flux-to-flux operator inside M (Curl, A^-1, L^-1, Curl); on an
+// unbounded lattice G is translation invariant, so G(a, b) is read from the
+// response to one unit flux of b's component at the level's centre (nx/2,
+// ny/2), at a's node shifted by the centre minus b's node (zero past the
+// table).  Entry (r, c) = scale * sum over the E entries a of row r and b of
+// row c of w_a w_b G(a, b); the upper triangle is formed and mirrored.
+struct PcAsmOps {
+    int n, nx, ny;                 // unknowns; level-0 cells
+    const int* e_ptr;              // E: per component, level-0 flux indices and weights
+    const int* e_f;
+    const double* e_w;
+    const double* tab;             // level-0 flux responses to a unit X flux, then to a unit Y flux
+    double scale;
+    double* out;                   // n x n, row-major
+};
+
+// C = alpha A B + beta D for n x n row-major FP64 matrices (D may be null);
+// with sym, C is symmetric: the tiles on and above the diagonal are formed
+// and mirrored.
+struct GemmOps {
+    int n;
+    const double *A, *B, *D;
+    double* C;
+    double alpha, beta;
+};
+
 struct ForceOps {
     int nb;
     const double* f;
@@ -186,6 +213,8 @@ void launch_pcg_dir(const PcgOps* d, int nflows, int begin, hipStream_t s);
 void launch_forces(const ForceOps* d, int nflows, hipStream_t s);
 void launch_unit_column(const UnitOps* d, int nflows, int j, hipStream_t s);
 void launch_copy_pieces(const CopyPiece* d, int npieces, hipStream_t s);
+void launch_pc_assemble(const PcAsmOps* d, int nflows, int max_n, hipStream_t s);
+void launch_dgemm(const GemmOps* d, int nflows, int max_n, bool sym, hipStream_t s);
 void launch_reg_to_flux(const RegFluxOps* d, int nslots, const double* f, double* q, hipStream_t s);
 void launch_reg_to_boundary(const RegFluxOps* d, int nb, const double* q, double* f, hipStream_t s);
 

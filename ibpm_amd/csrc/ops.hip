@@ -990,6 +990,156 @@ void launch_copy_pieces(const CopyPiece* d, int npieces, hipStream_t s) {
     if (npieces > 0) hipLaunchKernelGGL(k_copy_pieces, dim3(16, npieces), dim3(kThreads), 0, s, d);
 }
 
+// ---------------------------------------------------------------------------
+// Kernel-table assembly of the preconditioner's matrix (PcAsmOps).  A
+// workgroup forms a 16 x 16 tile on or above the diagonal: the E entries of
+// its 16 rows and 16 columns are staged in LDS as packed (component, i, j)
+// nodes with their weights, then each thread sums its entry over the pairs.
+// An E row holds at most 9 entries: the nodes of one component within 1.5
+// cells of the point in each direction.
+constexpr int kAsmTile = 16, kAsmMaxE = 9;
+
+__device__ __forceinline__ int pack_node(int f, int nx, int ny) {
+    const long nfx = (long)(nx + 1) * ny;
+    if (f < nfx) return ((f / ny) << 15) | (f % ny);
+    const int g = (int)(f - nfx);
+    return (1 << 30) | ((g / (ny + 1)) << 15) | (g % (ny + 1));
+}
+
+__global__ void __launch_bounds__(kAsmTile * kAsmTile) k_pc_assemble(const PcAsmOps* __restrict__ ops, int ntiles) {
+    const PcAsmOps o = ops[blockIdx.y];
+    const int nt = (o.n + kAsmTile - 1) / kAsmTile;
+    const int br = blockIdx.x / ntiles, bc = blockIdx.x % ntiles;
+    if (br >= nt || bc >= nt || br > bc) return;
+    __shared__ int node[2][kAsmTile][kAsmMaxE];
+    __shared__ double wt[2][kAsmTile][kAsmMaxE];
+    __shared__ int cnt[2][kAsmTile];
+    const int t = threadIdx.x;
+    if (t < 2 * kAsmTile) {
+        const int side = t / kAsmTile, q = t % kAsmTile;
+        const int row = (side ? bc : br) * kAsmTile + q;
+        int m = 0;
+        if (row < o.n) {
+            const int a0 = o.e_ptr[row], a1 = o.e_ptr[row + 1];
+            for (int a = a0; a < a1 && m < kAsmMaxE; ++a, ++m) {
+                node[side][q][m] = pack_node(o.e_f[a], o.nx, o.ny);
+                wt[side][q][m] = o.e_w[a];
+            }
+        }
+        cnt[side][q] = m;
+    }
+    __syncthreads();
+    const int ty = t / kAsmTile, tx = t % kAsmTile;
+    const int r = br * kAsmTile + ty, c = bc * kAsmTile + tx;
+    if (r >= o.n || c >= o.n || (br == bc && ty > tx)) return;
+    const long nfx = (long)(o.nx + 1) * o.ny, nf0 = nfx + (long)o.nx * (o.ny + 1);
+    const int cx = o.nx / 2, cy = o.ny / 2;
+    double sum = 0.0;
+    for (int b = 0; b < cnt[1][tx]; ++b) {
+        const int pb = node[1][tx][b];
+        const double* T = o.tab + (pb >> 30) * nf0;   // response to a unit flux of b's component
+        const int ib = (pb >> 15) & 0x7fff, jb = pb & 0x7fff;
+        double part = 0.0;
+        for (int a = 0; a < cnt[0][ty]; ++a) {
+            const int pa = node[0][ty][a];
+            const int di = ((pa >> 15) & 0x7fff) - ib + cx, dj = (pa & 0x7fff) - jb + cy;
+            double g = 0.0;
+            if ((pa >> 30) == 0) {
+                if (di >= 0 && di <= o.nx && dj >= 0 && dj < o.ny) g = T[(long)di * o.ny + dj];
+            } else {
+                if (di >= 0 && di < o.nx && dj >= 0 && dj <= o.ny) g = T[nfx + (long)di * (o.ny + 1) + dj];
+            }
+            part += wt[0][ty][a] * g;
+        }
+        sum += wt[1][tx][b] * part;
+    }
+    const double v = o.scale * sum;
+    o.out[(long)r * o.n + c] = v;
+    o.out[(long)c * o.n + r] = v;
+}
+
+void launch_pc_assemble(const PcAsmOps* d, int nflows, int max_n, hipStream_t s) {
+    const int nt = (max_n + kAsmTile - 1) / kAsmTile;
+    if (nflows > 0 && nt > 0)
+        hipLaunchKernelGGL(k_pc_assemble, dim3(nt * nt, nflows), dim3(kAsmTile * kAsmTile), 0, s, d, nt);
+}
+
+// ---------------------------------------------------------------------------
+// Batched FP64 GEMM (GemmOps), one matrix per blockIdx.y: 64 x 64 output tiles,
+// 256 threads of 4 x 4 outputs, k in steps of 16 staged in LDS (A's tile
+// transposed so that a thread's four rows are adjacent).  FP64 FMA on the
+// VALU runs at the rate of the FP64 matrix cores on this chip.
+constexpr int kGemmT = 64, kGemmK = 16;
+
+__global__ void __launch_bounds__(256) k_dgemm(const GemmOps* __restrict__ ops, int ntiles, int sym) {
+    const GemmOps o = ops[blockIdx.y];
+    const int n = o.n, nt = (n + kGemmT - 1) / kGemmT;
+    const int br = blockIdx.x / ntiles, bc = blockIdx.x % ntiles;
+    if (br >= nt || bc >= nt || (sym && br > bc)) return;
+    __shared__ double As[kGemmK][kGemmT];
+    __shared__ double Bs[kGemmK][kGemmT];
+    const int t = threadIdx.x, ty = t / 16, tx = t % 16;
+    const int r0 = br * kGemmT, c0 = bc * kGemmT;
+    double acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+    for (int k0 = 0; k0 < n; k0 += kGemmK) {
+        {
+            const int rr = t >> 2, kk = (t & 3) * 4, r = r0 + rr;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int k = k0 + kk + q;
+                As[kk + q][rr] = (r < n && k < n) ? o.A[(long)r * n + k] : 0.0;
+            }
+        }
+        {
+            const int kk = t >> 4, cc = (t & 15) * 4, k = k0 + kk;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int c = c0 + cc + q;
+                Bs[kk][cc + q] = (k < n && c < n) ? o.B[(long)k * n + c] : 0.0;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < kGemmK; ++kk) {
+            double a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                a[i] = As[kk][ty * 4 + i];
+                b[i] = Bs[kk][tx * 4 + i];
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = fma(a[i], b[j], acc[i][j]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int r = r0 + ty * 4 + i, c = c0 + tx * 4 + j;
+            if (r >= n || c >= n) continue;
+            const double v = o.alpha * acc[i][j] + (o.D ? o.beta * o.D[(long)r * n + c] : 0.0);
+            if (!sym) {
+                o.C[(long)r * n + c] = v;
+            } else if (r <= c) {   // the upper triangle, mirrored
+                o.C[(long)r * n + c] = v;
+                o.C[(long)c * n + r] = v;
+            }
+        }
+}
+
+void launch_dgemm(const GemmOps* d, int nflows, int max_n, bool sym, hipStream_t s) {
+    const int nt = (max_n + kGemmT - 1) / kGemmT;
+    if (nflows > 0 && nt > 0)
+        hipLaunchKernelGGL(k_dgemm, dim3(nt * nt, nflows), dim3(256), 0, s, d, nt, sym ? 1 : 0);
+}
+
 void launch_interp(const RegOps* d, int nflows, int max_comp, hipStream_t s) {
     hipLaunchKernelGGL(k_interp, dim3(blocks_for(max_comp), nflows), dim3(kThreads), 0, s, d);
 }

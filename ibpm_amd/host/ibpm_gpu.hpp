@@ -648,7 +648,8 @@ public:
     // plain CG, 1 CG preconditioned by a stale FP64 inverse of M with `pairs`
     // low-rank pairs (ibpm_batch_set_precond).  Before init().
     void setPrecond(int mode, int pairs) { _pc_mode = mode; _pc_pairs = pairs; }
-    // P built from M (levels 0) or from M's finest-level form (levels 1),
+    // P built from M (levels 0), from M's finest-level form (1), from the
+    // kernel-table M (2) or from that table refreshed before every solve (3),
     // with a build's cost in M applications per unknown for the rebuild rule
     // (ibpm_batch_set_precond_build).  Before init().
     void setPrecondBuild(int levels, double rebuild_cost) {
@@ -693,7 +694,7 @@ public:
         if (!_model.geTimeDependent()) gpu_check(ibpm_batch_use_graph(_batch, 1));
         else if (_pc_mode) gpu_check(ibpm_batch_set_precond(_batch, _pc_mode, _pc_pairs));
         if (!_local_helm) gpu_check(ibpm_batch_set_local_helmholtz(_batch, 0));
-        if (_pc_mode && _model.geTimeDependent() && _local_helm)
+        if (_pc_mode && _model.geTimeDependent() && (_local_helm || _pc_levels != 1))
             gpu_check(ibpm_batch_set_precond_build(_batch, _pc_levels, _pc_cost));
     }
     // IBSolver::advance, reference src/IBSolver.cc:142-150

@@ -7,9 +7,10 @@
 // binary restart file <name>%05d.bin every n steps and -ic loads one
 // (src/State.cc:65-223, src/ibpm.cc:250-274).  Added flag: -tol (CG tolerance,
 // which the reference sets only from code, src/ibpm.cc:292-294) and -precond
-// (force solve of moving bodies: none = the reference's plain CG, stale, or
-// lowrank; all end on the same residual test; -pcbuild finest builds P from
-// M's finest-level form).  -ubf 1 (unsteady base flow,
+// (force solve of moving bodies: none = the reference's plain CG, dense (P
+// alone), stale (its former name), or lowrank; all end on the same residual
+// test; -pcbuild finest builds P from M's finest-level form, table from the
+// kernel-table M, tracked from that table refreshed before every solve).  -ubf 1 (unsteady base flow,
 // src/ibpm.cc:166-178) solves a single moving body in its own frame and
 // rotates the forces by the base-flow angle after each step (:347-351).
 // -model linear|adjoint|linearperiodic|sfd with -baseflow, -period,
@@ -107,17 +108,20 @@ int main(int argc, char** argv) {
     std::set<std::string> models = {"nonlinear", "linear", "adjoint", "linearperiodic", "sfd"};
     std::map<std::string, Scheme::SchemeType> schemes = {
         {"euler", Scheme::EULER}, {"ab2", Scheme::AB2}, {"rk3", Scheme::RK3}, {"rk3b", Scheme::RK3b}};
-    std::map<std::string, std::pair<int, int>> preconds = {{"none", {0, 0}}, {"stale", {1, 0}}, {"lowrank", {1, 8}}};
+    std::map<std::string, std::pair<int, int>> preconds = {
+        {"none", {0, 0}}, {"dense", {1, 0}}, {"stale", {1, 0}}, {"lowrank", {1, 8}}};
+    const std::map<std::string, std::pair<int, double>> pcbuilds = {
+        {"all", {0, 1.0}}, {"finest", {1, 0.45}}, {"table", {2, 0.1}}, {"tracked", {3, 0.1}}};
     if (!P.valid() || P.help || !schemes.count(scheme) || !models.count(model) || !preconds.count(precond) ||
-        (helmholtz != "local" && helmholtz != "dst") || (pcbuild != "all" && pcbuild != "finest")) {
+        (helmholtz != "local" && helmholtz != "dst") || !pcbuilds.count(pcbuild)) {
         std::cerr << "usage: ibpm [-name ibpm] [-outdir .] [-force 1] [-nx 200] [-ny 200] [-ngrid 1]\n"
                      "            [-length 4] [-xoffset -2] [-yoffset -2] [-alpha 0] [-geom name.geom]\n"
                      "            [-Re 100] [-dt 0.02] [-nsteps 250] [-scheme euler|ab2|rk3|rk3b]\n"
                      "            [-model nonlinear|linear|adjoint|linearperiodic|sfd] [-baseflow file.bin]\n"
                      "            [-period 1] [-periodstart 0] [-pbaseflowname flow%05d.bin] [-chi 0.02]\n"
                      "            [-Delta 15] [-subbaseflow 0|1] [-resettime 0|1] [-ic file.bin]\n"
-                     "            [-tol 1e-7] [-device 0] [-precond none|stale|lowrank]\n"
-                     "            [-ubf 0|1] [-helmholtz local|dst] [-pcbuild all|finest]\n";
+                     "            [-tol 1e-7] [-device 0] [-precond none|dense|stale|lowrank]\n"
+                     "            [-ubf 0|1] [-helmholtz local|dst] [-pcbuild all|finest|table|tracked]\n";
         return 1;
     }
     const bool linearModel = model == "linear" || model == "adjoint" || model == "linearperiodic";
@@ -187,7 +191,7 @@ int main(int argc, char** argv) {
     }
     solver->setPrecond(preconds[precond].first, preconds[precond].second);
     solver->setLocalHelmholtz(helmholtz == "local");
-    if (pcbuild == "finest") solver->setPrecondBuild(1, 0.45);
+    if (pcbuild != "all") solver->setPrecondBuild(pcbuilds.at(pcbuild).first, pcbuilds.at(pcbuild).second);
     State x(grid, npts);
     nsm->init();
     try {

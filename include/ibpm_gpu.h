@@ -329,10 +329,23 @@ int ibpm_batch_set_precond(ibpm_batch* b, int mode, int pairs);
    level-0 Poisson solve under zero boundary values, symmetric positive
    definite like M and a fraction of its cost per column (needs the local
    Helmholtz path, ibpm_batch_set_local_helmholtz; without it P comes from M).
+   levels 2: from the kernel table, M = beta E G E^T with the level-0
+   flux-to-flux operator G inside M read off its response to one unit flux of
+   each component at the level's centre (a translation-invariant lattice
+   kernel, computed once per grid), at the current points: no M applications.
+   levels 3: as 2, and before every solve one Newton-Schulz step moves P
+   toward the inverse of the table's M at that substep's points, P <- P (2 I
+   - M~ P), two batched FP64 GEMMs, so that P follows the moving bodies.
    P only steers the iterations: the solves still end on M's own residual.
    rebuild_cost: a build's cost in M applications per unknown, which sets the
    rebuild rule's break-even (1 for levels 0).  Marks P for a rebuild. */
 int ibpm_batch_set_precond_build(ibpm_batch* b, int levels, double rebuild_cost);
+/* Inspection of the preconditioner (no counterpart in the reference): for a
+   moving flow, h_mtable (2 nb x 2 nb, row-major, may be null) receives the
+   kernel-table M of substep `substep` at the flow's current points, and h_p
+   (may be null) the dense P in use scaled to that substep (an error if none
+   is built).  Synchronises the batch's stream. */
+int ibpm_batch_precond_matrices(ibpm_batch* b, int flow, int substep, double* h_mtable, double* h_p);
 /* Force-solve counters since init: [0] iterations summed over flows and
    substeps, [1] solves (flow-substeps), [2] batched M applications in solves,
    [3] batched M applications spent building preconditioners, [4] builds,
@@ -342,7 +355,8 @@ int ibpm_batch_set_precond_build(ibpm_batch* b, int levels, double rebuild_cost)
    microseconds in solver polls (stream synchronisation and state read-back),
    [9] batched M applications that took the local Helmholtz path,
    [10] wall microseconds in preconditioner builds, [11] of those, host
-   microseconds in the factorisations and inverse assemblies.
+   microseconds in the factorisations and inverse assemblies, [12] refreshes
+   of P (build 3).
    Returns the count written. */
 int ibpm_batch_solver_stats(ibpm_batch* b, long* stats, int nstats);
 /* No counterpart in the reference (its M application and its step solve

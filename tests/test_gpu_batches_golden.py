@@ -5,11 +5,11 @@ flows (tests/golden/make_golden.py):
   * mixed_64x3: the 64 stationary flows (graph replay) for 40 steps; one
     flow of each grid size (256, 384, 512) and body (cylinder, ellipse,
     plate) is checked;
-  * tandem_16x1024x6: the 16 moving tandem-plate cases (preconditioned CG
-    with 8 pairs and P from M's finest-level form, local Helmholtz solves,
-    restricted L^-1 passes) for 30
-    steps; case 0 is checked, with the reference's warm start and with the
-    learned one.
+  * tandem_16x1024x6: the 16 moving tandem-plate cases (preconditioned CG,
+    local Helmholtz solves, restricted L^-1 passes) for 30 steps; case 0 is
+    checked with P from the kernel table refreshed before every solve (the
+    bench's solver; reference and learned warm starts) and with P from M's
+    finest-level form plus 8 pairs.
 
 Gates: Cd relative error <= 1e-6 at every step; Cl the same, relative to
 max|Cd| for the mirror-symmetric cylinders at zero incidence (their Cl is
@@ -69,8 +69,12 @@ def test_mixed_batch_matches_oracle_fixtures(gpu):
     b.close()
 
 
-def _tandem(gpu, ctx, flows, nsteps, nn=None):
-    b = gpu.Batch(ctx, flows).init().set_precond("lowrank", 8).set_precond_build("finest", 0.45)
+SOLVERS = {"tracked": (("dense", 0), ("tracked", 0.1)), "finest_lowrank8": (("lowrank", 8), ("finest", 0.45))}
+
+
+def _tandem(gpu, ctx, flows, nsteps, nn=None, solver="tracked"):
+    (mode, pairs), (build, cost) = SOLVERS[solver]
+    b = gpu.Batch(ctx, flows).init().set_precond(mode, pairs).set_precond_build(build, cost)
     if nn is not None:
         b.nn_params(nn)
         b.set_warm_start("learned")
@@ -78,22 +82,28 @@ def _tandem(gpu, ctx, flows, nsteps, nn=None):
     return b
 
 
-@pytest.mark.parametrize("warm", ["previous", "learned"])
-def test_tandem_batch_matches_oracle_fixture(gpu, warm):
+@pytest.mark.parametrize("warm,solver", [("previous", "tracked"), ("learned", "tracked"),
+                                         ("previous", "finest_lowrank8")])
+def test_tandem_batch_matches_oracle_fixture(gpu, warm, solver):
     g = _fixture("tandem_16x1024x6_c0")
     flows = W.tandem_16x1024x6()
     nsteps = int(g["nsteps"])
-    print("tandem_16x1024x6: 16 cases, %d steps, %s warm start" % (nsteps, warm), flush=True)
+    print("tandem_16x1024x6: 16 cases, %d steps, %s warm start, %s" % (nsteps, warm, solver), flush=True)
     ctx = gpu.Context(0)
     nn = None
     if warm == "learned":   # network trained on a separate run's first steps
-        t = gpu.Batch(ctx, flows).init().set_precond("lowrank", 8).collect(True, 1 << 20)
+        t = gpu.Batch(ctx, flows).init().set_precond("dense", 0).set_precond_build("tracked", 0.1)
+        t.collect(True, 1 << 20)
         t.advance(10)
         t.train(iters=300, lr=3e-3)
         nn = t.nn_params()
         t.close()
-    b = _tandem(gpu, ctx, flows, nsteps, nn)
+    b = _tandem(gpu, ctx, flows, nsteps, nn, solver)
     st = b.solver_stats()
+    print(st, flush=True)
     assert st["local_helmholtz_m_applications"] > 0   # the bench's local A^-1 path ran
-    _check(b, 0, flows[0], g, "tandem_16x1024x6 (%s start)" % warm)
+    assert st["fallbacks"] == 0 and st["abandoned"] == 0
+    if solver == "tracked":
+        assert st["refreshes"] > 0 and st["build_m_applications"] == 0
+    _check(b, 0, flows[0], g, "tandem_16x1024x6 (%s start, %s)" % (warm, solver))
     b.close()

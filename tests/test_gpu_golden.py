@@ -99,9 +99,9 @@ def test_ibpm_driver_force_file(gpu, oracle):
         assert abs(float(rows[k][2]) - cd) <= 1e-5 * abs(cd)
 
 
-@pytest.mark.parametrize("pcbuild", ["all", "finest"])
-def test_ibpm_driver_moving_plate_precond(gpu, oracle, pcbuild):
-    """The driver on a pitching plate with -precond lowrank, -tol and either
+@pytest.mark.parametrize("pcbuild,precond", [("all", "lowrank"), ("finest", "lowrank"), ("tracked", "dense")])
+def test_ibpm_driver_moving_plate_precond(gpu, oracle, pcbuild, precond):
+    """The driver on a pitching plate with -precond, -tol and each
     preconditioner build: forces agree with the oracle's plain CG to the CG
     tolerance's effect."""
     exe = os.path.join(os.path.dirname(HERE), "ibpm_amd", "ibpm")
@@ -109,7 +109,7 @@ def test_ibpm_driver_moving_plate_precond(gpu, oracle, pcbuild):
         gpath = os.path.join(d, "plate.geom")
         open(gpath, "w").write(PLATE)
         r = subprocess.run([exe, "-geom", gpath, "-nx", "128", "-ny", "128", "-ngrid", "3", "-Re", "300",
-                            "-dt", "0.005", "-nsteps", "5", "-tol", "1e-10", "-precond", "lowrank",
+                            "-dt", "0.005", "-nsteps", "5", "-tol", "1e-10", "-precond", precond,
                             "-pcbuild", pcbuild, "-outdir", d, "-name", "plate"],
                            capture_output=True, text=True, timeout=120)
         assert r.returncode == 0, r.stderr
@@ -146,21 +146,25 @@ def test_ibpm_driver_unsteady_base_flow(gpu, oracle):
         assert abs(full[k, 2] - cd) <= 1e-9 * scale and abs(full[k, 3] - cl) <= 1e-9 * scale
 
 
+REBUILD_COST = {"all": 1.0, "finest": 0.45, "table": 0.1, "tracked": 0.1}
+
+
 @pytest.mark.parametrize("name", ["plate", "plate_512x5_short"])
 @pytest.mark.parametrize("pairs", [0, 8])
-@pytest.mark.parametrize("build", ["all", "finest"])
+@pytest.mark.parametrize("build", ["all", "finest", "table", "tracked"])
 def test_preconditioned_cg_matches_golden(gpu, name, pairs, build):
     """The preconditioned force solve (stale FP64 inverse of M, with and
-    without the low-rank update; P from M or from its finest-level form)
-    against the oracle's plain-CG fixtures, at the same gates: only iteration
-    counts may change."""
+    without the low-rank update; P from M, from its finest-level form, from
+    the kernel table, or from the table refreshed every substep) against the
+    oracle's plain-CG fixtures, at the same gates: only iteration counts may
+    change."""
     g = _fixture(name)
     f = _flow(name)
     nsteps = int(g["nsteps"])
     print("preconditioned", name, "pairs", pairs, "build", build, nsteps, "steps", flush=True)
     ctx = gpu.Context(0)
     b = gpu.Batch(ctx, [f]).init().set_precond("lowrank" if pairs else "stale", pairs)
-    b.set_precond_build(build, 1.0 if build == "all" else 0.45)
+    b.set_precond_build(build, REBUILD_COST[build])
     iters = []
     for k in range(nsteps):
         b.advance(1)
@@ -243,15 +247,15 @@ def test_ibpm_driver_other_models(gpu, oracle, model):
 
 
 def test_precond_build_arguments(gpu):
-    """ibpm_batch_set_precond_build refuses an unknown level count, a
+    """ibpm_batch_set_precond_build refuses an unknown source of P, a
     non-positive rebuild cost, and the finest-level build without the local
     Helmholtz path; every refusal leaves an error message."""
     from ibpm_amd import IbpmError, load
     ctx = gpu.Context(0)
     b = gpu.Batch(ctx, [_flow("plate")]).init().set_precond("lowrank", 8)
     lib = load()
-    assert lib.ibpm_batch_set_precond_build(b.h, 2, 1.0) != 0
-    assert b"levels" in lib.ibpm_last_error()
+    assert lib.ibpm_batch_set_precond_build(b.h, 4, 1.0) != 0
+    assert b"preconditioner build" in lib.ibpm_last_error()
     assert lib.ibpm_batch_set_precond_build(b.h, 1, 0.0) != 0
     assert b"rebuild cost" in lib.ibpm_last_error()
     with pytest.raises(IbpmError):

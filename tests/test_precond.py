@@ -140,3 +140,71 @@ def test_lowrank_pairs_from_previous_solve_help_a_stale_inverse(plate_mats):
     _, it2_stale, _ = pcg(M0late, b2, np.zeros(n), lambda r: P0 @ r)
     _, it2_lmp, _ = pcg(M0late, b2, np.zeros(n), lmp(P0, pairs))
     assert it2_lmp <= it2_stale
+
+
+# ---------------------------------------------------------------------------
+# The kernel-table matrix: M from G's response to one unit flux per component.
+
+@pytest.fixture(scope="module")
+def table_case(oracle):
+    """A pitching plate of 25 points on 128^2 x 3: the oracle's dense M of
+    substep 2 after 4 steps, the association lists of its points and the
+    kernel tables of that substep."""
+    from tests import kernel_table as KT
+    nx, ngrid, length = 128, 3, 4.0
+    dx = length / nx
+    s = oracle.Sim(nx, nx, ngrid, length, -2.0, -2.0, 300.0, 0.005, plate_geom(25, 1.0, 0.3, 0.2, 0.4), tol=TOL)
+    s.init()
+    s.advance(4)
+    n = 2 * s.nb
+    M = np.stack([s.M(2, e) for e in np.eye(n)], 1)
+    pts = s.points()
+    assoc = oracle.reg_build(nx, nx, dx, -2.0, -2.0, pts[:s.nb], pts[s.nb:])
+    tab = KT.tables(oracle, nx, nx, ngrid, dx, s.beta(2), 300.0)
+    return dict(s=s, M=M, assoc=assoc, tab=tab, nx=nx, n=n, beta=s.beta(2))
+
+
+def test_kernel_table_matrix_approximates_m(table_case):
+    """E G E^T from the translated unit response agrees with the exact
+    multi-level M to a few parts in 10^6 (3e-6 here), and its inverse
+    clusters the spectrum of P M within 1e-3 of 1; a table read about an
+    origin one node off, or with the X and Y responses swapped, misses by
+    order one (planted mistakes)."""
+    from tests import kernel_table as KT
+    c = table_case
+    nx, n = c["nx"], c["n"]
+    MK = KT.upper_mirrored(KT.assemble(c["tab"], nx, nx, c["assoc"], n, c["beta"]))
+    rel = np.linalg.norm(MK - c["M"]) / np.linalg.norm(c["M"])
+    assert rel <= 1e-4, rel
+    ev = np.linalg.eigvals(np.linalg.solve(MK, c["M"])).real
+    assert 0.999 <= ev.min() and ev.max() <= 1.001, (ev.min(), ev.max())
+    off = KT.assemble(c["tab"], nx, nx, c["assoc"], n, c["beta"], centre=(nx // 2 + 1, nx // 2))
+    assert np.linalg.norm(off - c["M"]) / np.linalg.norm(c["M"]) > 0.1
+    swapped = KT.assemble(c["tab"][::-1], nx, nx, c["assoc"], n, c["beta"])
+    assert np.linalg.norm(swapped - c["M"]) / np.linalg.norm(c["M"]) > 0.1
+
+
+def test_newton_schulz_step_tracks_the_moving_table_matrix(table_case, oracle):
+    """One step P <- P (2 I - M~ P) from the inverse at the previous step's
+    points squares the error I - M~ P at the new points (I - M~ P1 = (I - M~
+    P)^2: 0.072 to 0.0022 here), and PCG with the refreshed P needs 3
+    iterations where plain CG needs 26."""
+    from tests import kernel_table as KT
+    c = table_case
+    s, nx, n = c["s"], c["nx"], c["n"]
+    dx = 4.0 / nx
+    M_old = KT.upper_mirrored(KT.assemble(c["tab"], nx, nx, c["assoc"], n, c["beta"]))
+    P = np.linalg.inv(M_old)
+    s.advance(1)
+    pts = s.points()
+    assoc = oracle.reg_build(nx, nx, dx, -2.0, -2.0, pts[:s.nb], pts[s.nb:])
+    M_new = KT.upper_mirrored(KT.assemble(c["tab"], nx, nx, assoc, n, c["beta"]))
+    e0 = np.linalg.norm(np.eye(n) - M_new @ P, 2)
+    P1 = P @ (2 * np.eye(n) - M_new @ P)
+    e1 = np.linalg.norm(np.eye(n) - M_new @ P1, 2)
+    assert 1e-2 < e0 < 0.5 and e1 <= e0 * e0 * (1 + 1e-6), (e0, e1)
+    M = np.stack([s.M(2, e) for e in np.eye(n)], 1)
+    b = M @ np.random.default_rng(4).standard_normal(n)
+    _, it_cg, _ = pcg(M, b, np.zeros(n), lambda r: r)
+    _, it_p, _ = pcg(M, b, np.zeros(n), lambda r: P1 @ r)
+    assert it_p <= 4 and it_p * 5 <= it_cg, (it_p, it_cg)

@@ -635,6 +635,7 @@ struct Flow {
     RegLists reg;
     int *e_ptr, *e_f, *t_ptr, *t_k, *node_idx, *old_idx;
     double *e_w, *t_w;
+    double* e_sep;                 // E rows in separable form (kESep per component)
     int4* node_nbr;
     int nnodes = 0, nold = 0;
     // algorithmic bytes of one k_interp (without the constraint b) and one
@@ -782,6 +783,12 @@ struct ibpm_batch {
     ibpmg::GemmOps *d_ns1 = nullptr, *d_ns2 = nullptr;
     ibpmg::CopyPiece* d_ns_copy = nullptr;
     long st_refreshes = 0;
+    // the refresh runs on a stream of its own beside the substep's right-hand
+    // side and step solves: it starts after the substep's uploads (pc_ev_in)
+    // and the force solve waits for it (pc_ev_out)
+    hipStream_t pc_stream = nullptr;
+    hipEvent_t pc_ev_in = nullptr, pc_ev_out = nullptr;
+    bool pc_pending = false;
     // Newton-Schulz steps per refresh: on the tandem batch a second step cut
     // the mean iterations per step from 9.1 to 7.6 but not the slowest
     // flow's, and cost 0.6 ms per substep
@@ -902,7 +909,7 @@ static int up(ibpm_batch* B, void* d, const void* h, size_t bytes) {
 static int async_prep_begin(ibpm_batch* B) {
     if (!B->ustage[0].p) {
         for (int p = 0; p < 2; ++p) {
-            B->ustage[p].cap = (size_t)8 << 20;
+            B->ustage[p].cap = (size_t)16 << 20;
             CK(hipHostMalloc((void**)&B->ustage[p].p, B->ustage[p].cap, hipHostMallocDefault));
             CKR(B->mem.alloc(&B->d_ustage[p], B->ustage[p].cap));
         }
@@ -1095,6 +1102,7 @@ static int upload_regularizer(ibpm_batch* B, Flow& F) {
     CKR(put(F.e_ptr, R.e_ptr.data(), sizeof(int) * R.e_ptr.size()));
     CKR(put(F.e_f, R.e_f.data(), sizeof(int) * R.e_f.size()));
     CKR(put(F.e_w, R.e_w.data(), sizeof(double) * R.e_w.size()));
+    CKR(put(F.e_sep, R.e_sep.data(), sizeof(double) * R.e_sep.size()));
     CKR(put(F.t_ptr, R.t_ptr.data(), sizeof(int) * R.t_ptr.size()));
     CKR(put(F.t_k, R.t_k.data(), sizeof(int) * R.t_k.size()));
     CKR(put(F.t_w, R.t_w.data(), sizeof(double) * R.t_w.size()));
@@ -1202,11 +1210,13 @@ static int alloc_flow(ibpm_batch* B, Flow& F) {
     CKR(m.alloc(&F.hist, 2 * (size_t)F.hist_cap)); CKR(m.alloc(&F.hist_count, 1));
     const int cap = 18 * F.nb + 1;
     CKR(m.alloc(&F.e_ptr, n + 1)); CKR(m.alloc(&F.e_f, cap)); CKR(m.alloc(&F.e_w, cap));
+    CKR(m.alloc(&F.e_sep, (size_t)kESep * n));
     CKR(m.alloc(&F.t_ptr, cap + 1)); CKR(m.alloc(&F.t_k, cap)); CKR(m.alloc(&F.t_w, cap));
     CKR(m.alloc(&F.node_idx, 4 * cap)); CKR(m.alloc(&F.old_idx, 4 * cap)); CKR(m.alloc(&F.node_nbr, 4 * cap));
     {   // staging for the lists (capacities above), the constraint and the base flow
         const size_t bytes = sizeof(int) * (n + 1) + (sizeof(int) + sizeof(double)) * 2 * cap + sizeof(int) * (cap + 1) +
-                             sizeof(int) * 8 * cap + sizeof(int4) * 4 * cap + sizeof(double) * (n + 4) + 12 * 256;
+                             sizeof(int) * 8 * cap + sizeof(int4) * 4 * cap + sizeof(double) * (n + 4) +
+                             sizeof(double) * kESep * n + 13 * 256;
         CK(hipHostMalloc((void**)&F.stage.p, bytes, hipHostMallocDefault));
         F.stage.cap = bytes;
     }
@@ -1923,6 +1933,7 @@ static int nn_samples_room(const ibpm_batch* B) {
 
 static int build_precond(ibpm_batch* B, int i);
 static int refresh_precond(ibpm_batch* B, int i);
+static int join_refresh(ibpm_batch* B);
 
 // Force-solve state of every flow: zero, with tol^2 in slot 1.
 // Pinned host buffers and the event of the solver polls (first use).
@@ -2141,6 +2152,9 @@ static int substep_cg(ibpm_batch* B, int i) {
     Probe* pr = B->probe;
     timed(pr, K_CLEAR, 0, s, [&] { launch_clear_nodes(B->d_clear, nfl, B->max_nodes, s); });
     enqueue_rhs(B, i, s);
+    // P's refresh at this substep's points, beside the step solves (whose
+    // transforms leave CUs idle; the stencils above fill the GPU)
+    if (B->pc_mode && !B->pc_broken && B->pc_valid && B->pc_build_levels == 3) CKR(refresh_precond(B, i));
     run_solve(B->S1[i], s, pr);
     run_solve(B->S2[i], s, pr);
     timed(pr, K_INTERP, pr ? reg_bytes(B, false, true) : 0, s,
@@ -2169,8 +2183,8 @@ static int substep_cg(ibpm_batch* B, int i) {
         launch_nn_infer(d, nfl, maxnb, s);
     }
     std::vector<int> its(nfl, 0);
+    CKR(join_refresh(B));
     if (B->pc_mode && !B->pc_broken && !B->pc_valid) CKR(build_precond(B, i));
-    else if (B->pc_mode && !B->pc_broken && B->pc_build_levels == 3) CKR(refresh_precond(B, i));
     if (B->pc_mode && !B->pc_broken) {
         bool fallback = false;
         CKR(solve_pcg(B, i, its.data(), &fallback));
@@ -2450,7 +2464,7 @@ static int kernel_tables(ibpm_batch* B) {
 static std::vector<PcAsmOps> pc_asm_ops(ibpm_batch* B, int i, const std::function<double(const Flow&)>& scale) {
     std::vector<PcAsmOps> ops;
     for (auto& F : B->flows)
-        ops.push_back(PcAsmOps{2 * F.nb, F.nx, F.ny, F.e_ptr, F.e_f, F.e_w, F.ktab[i], scale(F), F.mcol});
+        ops.push_back(PcAsmOps{2 * F.nb, F.nx, F.ny, F.e_sep, F.ktab[i], scale(F), F.mcol});
     return ops;
 }
 
@@ -2699,18 +2713,41 @@ static int build_precond(ibpm_batch* B, int i) {
 // at substep i's points, P <- P (2 I - M~ P) (symmetric), before the solve:
 // the error I - M~ P squares, so P follows the bodies from substep to substep
 // at the cost of the assembly and two batched GEMMs.
+// Queued on the refresh stream behind the substep's uploads (the regulariser
+// lists it reads), so that it overlaps the substep's right-hand side and step
+// solves; join_refresh makes the stream wait for it before P is read.  P and
+// the refresh's buffers are touched by nothing else in between: the previous
+// solve's reads of P precede pc_ev_in, the next substep's uploads follow the
+// join.
 static int refresh_precond(ibpm_batch* B, int i) {
     hipStream_t s = B->ctx->c.stream;
     const int nfl = (int)B->flows.size();
+    if (!B->pc_stream) {
+        CK(hipStreamCreate(&B->pc_stream));
+        CK(hipEventCreateWithFlags(&B->pc_ev_in, hipEventDisableTiming));
+        CK(hipEventCreateWithFlags(&B->pc_ev_out, hipEventDisableTiming));
+    }
+    hipStream_t r = B->pc_stream;
+    CK(hipEventRecord(B->pc_ev_in, s));
+    CK(hipStreamWaitEvent(r, B->pc_ev_in, 0));
     Probe* pr = B->probe;
-    timed(pr, K_PCASM, 0, s, [&] { launch_pc_assemble(B->d_pcasm[i], nfl, B->max_n, s); });
+    timed(pr, K_PCASM, 0, r, [&] { launch_pc_assemble(B->d_pcasm[i], nfl, B->max_n, r); });
     for (int k = 0; k < B->pc_ns_steps; ++k)
-        timed(pr, K_PCNS, 0, s, [&] {
-            launch_dgemm(B->d_ns1, nfl, B->max_n, false, s);
-            launch_dgemm(B->d_ns2, nfl, B->max_n, true, s);
-            launch_copy_pieces(B->d_ns_copy, nfl, s);
+        timed(pr, K_PCNS, 0, r, [&] {
+            launch_dgemm(B->d_ns1, nfl, B->max_n, false, r);
+            launch_dgemm(B->d_ns2, nfl, B->max_n, true, r);
+            launch_copy_pieces(B->d_ns_copy, nfl, r);
         });
+    CK(hipEventRecord(B->pc_ev_out, r));
+    B->pc_pending = true;
     B->st_refreshes += 1;
+    return 0;
+}
+
+static int join_refresh(ibpm_batch* B) {
+    if (!B->pc_pending) return 0;
+    CK(hipStreamWaitEvent(B->ctx->c.stream, B->pc_ev_out, 0));
+    B->pc_pending = false;
     return 0;
 }
 
@@ -3757,6 +3794,12 @@ int ibpm_batch_destroy(ibpm_batch* b) {
     for (auto e : b->uev)
         if (e) (void)hipEventDestroy(e);
     if (b->poll_ev) (void)hipEventDestroy(b->poll_ev);
+    if (b->pc_stream) {
+        (void)hipStreamSynchronize(b->pc_stream);
+        (void)hipStreamDestroy(b->pc_stream);
+    }
+    if (b->pc_ev_in) (void)hipEventDestroy(b->pc_ev_in);
+    if (b->pc_ev_out) (void)hipEventDestroy(b->pc_ev_out);
     if (b->h_states) (void)hipHostFree(b->h_states);
     if (b->h_reset) (void)hipHostFree(b->h_reset);
     delete b;
@@ -4163,10 +4206,11 @@ int ibpm_batch_precond_matrices(ibpm_batch* B, int flow, int substep, double* h_
     const int n = 2 * F.nb;
     hipStream_t s = B->ctx->c.stream;
     CK(hipStreamSynchronize(s));
+    if (B->pc_stream) CK(hipStreamSynchronize(B->pc_stream));
     if (h_mtable) {
         CKR(kernel_tables(B));
         if (!B->d_pcasm_b) CKR(B->mem.alloc(&B->d_pcasm_b, B->flows.size()));
-        const PcAsmOps op{n, F.nx, F.ny, F.e_ptr, F.e_f, F.e_w, F.ktab[substep], beta_of(B, F, substep), F.mcol};
+        const PcAsmOps op{n, F.nx, F.ny, F.e_sep, F.ktab[substep], beta_of(B, F, substep), F.mcol};
         CKR(upload(B->d_pcasm_b, &op, sizeof op));
         launch_pc_assemble(B->d_pcasm_b, 1, n, s);
         CK(hipStreamSynchronize(s));

@@ -392,6 +392,7 @@ void build_reg_lists(int nx, int ny, double h, double xoff, double yoff,
     out.e_ptr.assign(2 * nb + 1, 0);
     out.e_f.clear();
     out.e_w.clear();
+    out.e_sep.assign((size_t)2 * nb * kESep, 0.0);
     // (flux, component, weight) in the order the reference visits them
     struct Hit {
         int f, k;
@@ -427,6 +428,29 @@ void build_reg_lists(int nx, int ny, double h, double xoff, double yoff,
                 }
             }
             out.e_ptr[k + 1] = (int)out.e_f.size();
+            // the same members as a rectangle of rows and columns (the
+            // inclusion test is separable) with the row and column factors of
+            // the weights: at most 3 of each within 1.5 cells
+            double* sep = &out.e_sep[(size_t)k * kESep];
+            int ni = 0, nj = 0;
+            for (int i = i0; i <= i1; ++i) {
+                const double fx = dir == 0 ? xoff + i * h : xoff + (i + 0.5) * h;
+                const double ddx = std::fabs(fx - px[p]) / h;
+                if (ddx < 1.5 && ni < 3) {
+                    if (ni == 0) sep[0] = i;
+                    sep[4 + ni++] = delta_fn(ddx);
+                }
+            }
+            for (int j = j0; j <= j1; ++j) {
+                const double fy = dir == 0 ? yoff + (j + 0.5) * h : yoff + j * h;
+                const double ddy = std::fabs(fy - py[p]) / h;
+                if (ddy < 1.5 && nj < 3) {
+                    if (nj == 0) sep[1] = j;
+                    sep[7 + nj++] = delta_fn(ddy);
+                }
+            }
+            sep[2] = ni;
+            sep[3] = nj;
         }
     }
     // E^T: per flux in ascending order, components in visiting (= ascending) order

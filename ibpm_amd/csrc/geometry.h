@@ -80,9 +80,16 @@ struct BaseFlowMotion {
 //   E   : per boundary component k (0..2nb-1), level-0 flux indices ascending
 //   E^T : per touched flux, components ascending
 //   nodes touched by Curl(E^T f) with the flux-list slots of their 4 fluxes
+// Per component of E, its row in separable form (kESep doubles): first row
+// and column index of its nodes, their counts ni and nj (at most 3), the
+// row factors u[ni] and the column factors v[nj]; the node (i0 + a, j0 + b)
+// has weight u[a] v[b].
+constexpr int kESep = 10;
+
 struct RegLists {
     std::vector<int> e_ptr, e_f;
     std::vector<double> e_w;
+    std::vector<double> e_sep;                // kESep per component: i0, j0, ni, nj, u[3], v[3]
     std::vector<int> flux_of_slot;            // flux index of each flux-list slot
     std::vector<int> t_ptr, t_k;
     std::vector<double> t_w;

@@ -3,6 +3,7 @@
 // per-flow descriptors and is launched with blockIdx.y = flow, so one launch
 // advances a whole batch of flows of uneven size (blocks past a flow's extent
 // exit at once).
+#include "geometry.h"
 #include "ops.h"
 
 namespace ibpmg {
@@ -992,66 +993,64 @@ void launch_copy_pieces(const CopyPiece* d, int npieces, hipStream_t s) {
 
 // ---------------------------------------------------------------------------
 // Kernel-table assembly of the preconditioner's matrix (PcAsmOps).  A
-// workgroup forms a 16 x 16 tile on or above the diagonal: the E entries of
-// its 16 rows and 16 columns are staged in LDS as packed (component, i, j)
-// nodes with their weights, then each thread sums its entry over the pairs.
-// An E row holds at most 9 entries: the nodes of one component within 1.5
-// cells of the point in each direction.
-constexpr int kAsmTile = 16, kAsmMaxE = 9;
-
-__device__ __forceinline__ int pack_node(int f, int nx, int ny) {
-    const long nfx = (long)(nx + 1) * ny;
-    if (f < nfx) return ((f / ny) << 15) | (f % ny);
-    const int g = (int)(f - nfx);
-    return (1 << 30) | ((g / (ny + 1)) << 15) | (g % (ny + 1));
-}
+// workgroup forms a 16 x 16 tile on or above the diagonal; the separable rows
+// of its 16 rows and 16 columns are staged in LDS.  For rows r (component cr,
+// nodes i0r + a, j0r + b, weights ur[a] vr[b]) and c, the sum over node
+// pairs is a sum over offsets (di, dj) = (a - a', b - b') of U(di) V(dj) G at
+// offset (i0r - i0c + di, j0r - j0c + dj), with U the correlation of ur and
+// uc and V that of vr and vc: 25 table reads instead of 81.
+constexpr int kAsmTile = 16;
 
 __global__ void __launch_bounds__(kAsmTile * kAsmTile) k_pc_assemble(const PcAsmOps* __restrict__ ops, int ntiles) {
     const PcAsmOps o = ops[blockIdx.y];
     const int nt = (o.n + kAsmTile - 1) / kAsmTile;
     const int br = blockIdx.x / ntiles, bc = blockIdx.x % ntiles;
     if (br >= nt || bc >= nt || br > bc) return;
-    __shared__ int node[2][kAsmTile][kAsmMaxE];
-    __shared__ double wt[2][kAsmTile][kAsmMaxE];
-    __shared__ int cnt[2][kAsmTile];
+    __shared__ double sp[2][kAsmTile][kESep];
     const int t = threadIdx.x;
-    if (t < 2 * kAsmTile) {
-        const int side = t / kAsmTile, q = t % kAsmTile;
-        const int row = (side ? bc : br) * kAsmTile + q;
-        int m = 0;
-        if (row < o.n) {
-            const int a0 = o.e_ptr[row], a1 = o.e_ptr[row + 1];
-            for (int a = a0; a < a1 && m < kAsmMaxE; ++a, ++m) {
-                node[side][q][m] = pack_node(o.e_f[a], o.nx, o.ny);
-                wt[side][q][m] = o.e_w[a];
-            }
-        }
-        cnt[side][q] = m;
+    for (int q = t; q < 2 * kAsmTile * kESep; q += kAsmTile * kAsmTile) {
+        const int side = q / (kAsmTile * kESep), w = q % (kAsmTile * kESep);
+        const int row = (side ? bc : br) * kAsmTile + w / kESep;
+        sp[side][w / kESep][w % kESep] = row < o.n ? o.sep[(long)row * kESep + w % kESep] : 0.0;
     }
     __syncthreads();
     const int ty = t / kAsmTile, tx = t % kAsmTile;
     const int r = br * kAsmTile + ty, c = bc * kAsmTile + tx;
     if (r >= o.n || c >= o.n || (br == bc && ty > tx)) return;
-    const long nfx = (long)(o.nx + 1) * o.ny, nf0 = nfx + (long)o.nx * (o.ny + 1);
-    const int cx = o.nx / 2, cy = o.ny / 2;
-    double sum = 0.0;
-    for (int b = 0; b < cnt[1][tx]; ++b) {
-        const int pb = node[1][tx][b];
-        const double* T = o.tab + (pb >> 30) * nf0;   // response to a unit flux of b's component
-        const int ib = (pb >> 15) & 0x7fff, jb = pb & 0x7fff;
-        double part = 0.0;
-        for (int a = 0; a < cnt[0][ty]; ++a) {
-            const int pa = node[0][ty][a];
-            const int di = ((pa >> 15) & 0x7fff) - ib + cx, dj = (pa & 0x7fff) - jb + cy;
-            double g = 0.0;
-            if ((pa >> 30) == 0) {
-                if (di >= 0 && di <= o.nx && dj >= 0 && dj < o.ny) g = T[(long)di * o.ny + dj];
-            } else {
-                if (di >= 0 && di < o.nx && dj >= 0 && dj <= o.ny) g = T[nfx + (long)di * (o.ny + 1) + dj];
-            }
-            part += wt[0][ty][a] * g;
+    const double* R = sp[0][ty];
+    const double* C = sp[1][tx];
+    const int nir = (int)R[2], njr = (int)R[3], nic = (int)C[2], njc = (int)C[3];
+    // U[k]: offset di = k - (nic - 1); V[m]: dj = m - (njc - 1)
+    double U[5], V[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        const int di = k - (nic - 1), dj = k - (njc - 1);
+        double su = 0.0, sv = 0.0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (a < nir && a - di >= 0 && a - di < nic) su += R[4 + a] * C[4 + a - di];
+            if (a < njr && a - dj >= 0 && a - dj < njc) sv += R[7 + a] * C[7 + a - dj];
         }
-        sum += wt[1][tx][b] * part;
+        U[k] = su;
+        V[k] = sv;
+    }
+    const int cr = r >= o.n / 2, cc = c >= o.n / 2;
+    const long nfx = (long)(o.nx + 1) * o.ny, nf0 = nfx + (long)o.nx * (o.ny + 1);
+    // the part of b's response on a's component: rows ni0 x columns of length ld
+    const double* T = o.tab + cc * nf0 + (cr ? nfx : 0);
+    const int rows = cr ? o.nx : o.nx + 1, cols = cr ? o.ny + 1 : o.ny;
+    const int i0 = (int)R[0] - (int)C[0] + o.nx / 2 - (nic - 1);
+    const int j0 = (int)R[1] - (int)C[1] + o.ny / 2 - (njc - 1);
+    double sum = 0.0;
+    for (int k = 0; k < nir + nic - 1; ++k) {
+        const int i = i0 + k;
+        if (i < 0 || i >= rows) continue;
+        double part = 0.0;
+        for (int m = 0; m < njr + njc - 1; ++m) {
+            const int j = j0 + m;
+            if (j >= 0 && j < cols) part += V[m] * T[(long)i * cols + j];
+        }
+        sum += U[k] * part;
     }
     const double v = o.scale * sum;
     o.out[(long)r * o.n + c] = v;
@@ -1067,8 +1066,10 @@ void launch_pc_assemble(const PcAsmOps* d, int nflows, int max_n, hipStream_t s)
 // ---------------------------------------------------------------------------
 // Batched FP64 GEMM (GemmOps), one matrix per blockIdx.y: 64 x 64 output tiles,
 // 256 threads of 4 x 4 outputs, k in steps of 16 staged in LDS (A's tile
-// transposed so that a thread's four rows are adjacent).  FP64 FMA on the
-// VALU runs at the rate of the FP64 matrix cores on this chip.
+// transposed so that a thread's four rows are adjacent); each step's global
+// loads for the next step are issued before its arithmetic.  FP64 FMA on the
+// VALU runs at the rate of the FP64 matrix cores on this chip (an MFMA
+// 16x16x4 form of this kernel ran no faster: both wait on the tiles' loads).
 constexpr int kGemmT = 64, kGemmK = 16;
 
 __global__ void __launch_bounds__(256) k_dgemm(const GemmOps* __restrict__ ops, int ntiles, int sym) {
@@ -1080,29 +1081,31 @@ __global__ void __launch_bounds__(256) k_dgemm(const GemmOps* __restrict__ ops, 
     __shared__ double Bs[kGemmK][kGemmT];
     const int t = threadIdx.x, ty = t / 16, tx = t % 16;
     const int r0 = br * kGemmT, c0 = bc * kGemmT;
+    const int arr = t >> 2, akk = (t & 3) * 4, bkk = t >> 4, bcc = (t & 15) * 4;
+    double ra[4], rb[4];
+    auto fetch = [&](int k0) {
+        const int r = r0 + arr, k = k0 + bkk;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int ka = k0 + akk + q, c = c0 + bcc + q;
+            ra[q] = (r < n && ka < n) ? o.A[(long)r * n + ka] : 0.0;
+            rb[q] = (k < n && c < n) ? o.B[(long)k * n + c] : 0.0;
+        }
+    };
     double acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+    fetch(0);
     for (int k0 = 0; k0 < n; k0 += kGemmK) {
-        {
-            const int rr = t >> 2, kk = (t & 3) * 4, r = r0 + rr;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int k = k0 + kk + q;
-                As[kk + q][rr] = (r < n && k < n) ? o.A[(long)r * n + k] : 0.0;
-            }
-        }
-        {
-            const int kk = t >> 4, cc = (t & 15) * 4, k = k0 + kk;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int c = c0 + cc + q;
-                Bs[kk][cc + q] = (k < n && c < n) ? o.B[(long)k * n + c] : 0.0;
-            }
+        for (int q = 0; q < 4; ++q) {
+            As[akk + q][arr] = ra[q];
+            Bs[bkk][bcc + q] = rb[q];
         }
         __syncthreads();
+        if (k0 + kGemmK < n) fetch(k0 + kGemmK);
 #pragma unroll
         for (int kk = 0; kk < kGemmK; ++kk) {
             double a[4], b[4];

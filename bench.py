@@ -107,9 +107,10 @@ def traffic_per_launch(workload, kernel, alg_bytes):
     if not os.path.exists(path):
         return None
     t = json.load(open(path))
-    if t.get("workload") != workload or t.get("kernel") != kernel:
+    ratio = t.get("traffic_over_algorithmic", {}).get(kernel)
+    if t.get("workload") != workload or ratio is None:
         return None
-    return t["traffic_over_algorithmic"] * alg_bytes
+    return ratio * alg_bytes
 
 
 def log(msg):

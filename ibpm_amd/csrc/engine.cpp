@@ -813,6 +813,8 @@ struct ibpm_batch {
     // k_copy_pieces tables: cg_d = f per flow (start_residual); the history
     // ring's f and b copies per ring slot
     ibpmg::CopyPiece *d_start_pieces = nullptr, *d_ring_pieces = nullptr;
+    // zero fills of a zero start, one launch each: f of every flow, then M x0
+    ibpmg::CopyPiece* d_zero_pieces = nullptr;
     hipEvent_t uev[2] = {nullptr, nullptr};
     bool upending[2] = {false, false};
     int upar = 0;
@@ -1987,12 +1989,23 @@ static int read_states(ibpm_batch* B, std::vector<double>& st) {
     return poll_wait(B, st);
 }
 
+// Zero-fill pieces of every flow's f, then of its cg_q (first use).
+static int zero_pieces(ibpm_batch* B) {
+    if (B->d_zero_pieces) return 0;
+    std::vector<CopyPiece> pc;
+    for (auto& F : B->flows) pc.push_back(CopyPiece{nullptr, (unsigned int*)F.f, 4L * F.nb});
+    for (auto& F : B->flows) pc.push_back(CopyPiece{nullptr, (unsigned int*)F.cg_q, 4L * F.nb});
+    CKR(B->mem.alloc(&B->d_zero_pieces, pc.size()));
+    return upload(B->d_zero_pieces, pc.data(), sizeof(CopyPiece) * pc.size());
+}
+
 // q = M x0 for r0 = b - q.  A zero start (warm start mode 0) needs no M
 // application: q is cleared instead.
 static int start_residual(ibpm_batch* B, int i) {
     hipStream_t s = B->ctx->c.stream;
     if (B->warm_mode == 0) {
-        for (auto& F : B->flows) CK(hipMemsetAsync(F.cg_q, 0, sizeof(double) * 2 * F.nb, s));
+        CKR(zero_pieces(B));
+        launch_copy_pieces(B->d_zero_pieces + B->flows.size(), (int)B->flows.size(), s);
         return 0;
     }
     if (!B->d_start_pieces) {   // cg_d = f of every flow, one launch
@@ -2173,7 +2186,8 @@ static int substep_cg(ibpm_batch* B, int i) {
     // x0: previous f (ProjectionSolver.cc:72 warm start), zero, or the
     // learned prediction; then r0 = rhs - M x0
     if (B->warm_mode == 0) {
-        for (auto& F : B->flows) CK(hipMemsetAsync(F.f, 0, sizeof(double) * 2 * F.nb, s));
+        CKR(zero_pieces(B));
+        launch_copy_pieces(B->d_zero_pieces, nfl, s);
     } else if (B->warm_mode == 2 && B->nn_ready && nn_history_ready(B)) {
         CKR(nn_infer_table(B));
         const NnOps* d = B->d_nn_inf + ((size_t)(B->ksub % B->ring) * B->nsub + i) * nfl;

@@ -120,8 +120,8 @@ struct UnitOps {
     int n;
 };
 
-// One contiguous copy of n 4-byte words (k_copy_pieces: many copies in one
-// launch instead of one runtime copy each).
+// One contiguous copy of n 4-byte words, or n zero words with src null
+// (k_copy_pieces: many copies in one launch instead of one runtime call each).
 struct CopyPiece {
     const unsigned int* src;
     unsigned int* dst;

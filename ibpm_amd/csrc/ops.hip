@@ -984,7 +984,7 @@ void launch_unit_column(const UnitOps* d, int nflows, int j, hipStream_t s) {
 __global__ void __launch_bounds__(kThreads) k_copy_pieces(const CopyPiece* __restrict__ pieces) {
     const CopyPiece P = pieces[blockIdx.y];
     for (long w = (long)blockIdx.x * kThreads + threadIdx.x; w < P.n; w += (long)gridDim.x * kThreads)
-        P.dst[w] = P.src[w];
+        P.dst[w] = P.src ? P.src[w] : 0u;
 }
 
 void launch_copy_pieces(const CopyPiece* d, int npieces, hipStream_t s) {

@@ -2105,7 +2105,10 @@ static int solve_pcg(ibpm_batch* B, int i, int* iters, bool* fallback) {
             iterate();
             ahead = true;
             done += 1;
-            chunk = 1;
+            // from now on each round reads the states after the iteration
+            // queued ahead in the round before and queues one more, so at most
+            // one iteration past convergence is issued
+            chunk = 0;
             CKR(poll_wait(B, st));
         }
         bool all = true;

@@ -2211,8 +2211,11 @@ static int substep_cg(ibpm_batch* B, int i) {
         int itmax = 0;
         for (int k = 0; k < nfl; ++k) itmax = std::max(itmax, its[k]);
         B->pc_last[i] = itmax;
+        // (a refreshed P does not go stale: only a jump of more than two
+        // iterations over the first solve, a refresh losing track, counts)
+        const int slack = B->pc_build_levels == 3 ? 2 : 0;
         if (B->pc_base[i] < 0) B->pc_base[i] = itmax;
-        else B->pc_excess += std::max(0, itmax - B->pc_base[i]);
+        else B->pc_excess += std::max(0, itmax - B->pc_base[i] - slack);
         if (B->pc_excess > B->max_n * B->pc_build_cost) B->pc_valid = false;
         if (fallback) {
             B->st_fallbacks += 1;

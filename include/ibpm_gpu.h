@@ -335,7 +335,9 @@ int ibpm_batch_set_precond(ibpm_batch* b, int mode, int pairs);
    kernel, computed once per grid), at the current points: no M applications.
    levels 3: as 2, and before every solve one Newton-Schulz step moves P
    toward the inverse of the table's M at that substep's points, P <- P (2 I
-   - M~ P), two batched FP64 GEMMs, so that P follows the moving bodies.
+   - M~ P), two batched FP64 GEMMs, so that P follows the moving bodies; the
+   rebuild rule then counts only iterations more than two above a substep's
+   first solve after a build.
    P only steers the iterations: the solves still end on M's own residual.
    rebuild_cost: a build's cost in M applications per unknown, which sets the
    rebuild rule's break-even (1 for levels 0).  Marks P for a rebuild. */

@@ -628,6 +628,9 @@ struct Flow {
     // substep (shared by the flows of one grid, Re and dt), and the product
     // buffer of P's Newton-Schulz refresh
     const double* ktab[4] = {nullptr, nullptr, nullptr, nullptr};
+    const int* ksrc = nullptr;     // the table's unit-flux nodes (i, j), nsrc of them
+    int nsrc = 0;
+    double bbox0[4] = {0, 0, 0, 0};   // the points' extent at set-up: x min, max, y min, max
     double *pc_T = nullptr, *pc_X = nullptr;
     std::string chol_cache;        // basename of the .cholesky factor files ("" = none)
     int chol_loaded = 0;           // substep factors read from those files
@@ -775,7 +778,9 @@ struct ibpm_batch {
     struct KernTab {
         int nx, ny, ngrid;
         double dx, re, dt;
-        double* tab[4];
+        double* tab[4];                   // per substep: nsrc x (X response, Y response)
+        std::vector<int> src;             // unit-flux nodes (i, j)
+        int* d_src;
     };
     std::vector<KernTab> ktabs;
     ibpmg::PcAsmOps* d_pcasm[4] = {};     // the refresh's assembly per substep (scale pbeta)
@@ -789,10 +794,10 @@ struct ibpm_batch {
     hipStream_t pc_stream = nullptr;
     hipEvent_t pc_ev_in = nullptr, pc_ev_out = nullptr;
     bool pc_pending = false;
-    // Newton-Schulz steps per refresh: on the tandem batch a second step cut
-    // the mean iterations per step from 9.1 to 7.6 but not the slowest
-    // flow's, and cost 0.6 ms per substep
-    int pc_ns_steps = 1;
+    // Newton-Schulz steps per refresh: with the multi-source table (spectrum
+    // of M~^-1 M within 1e-4 of 1) a second step takes P's own error from
+    // about 1e-3 to 1e-6, and every tandem solve from 3 iterations to 2
+    int pc_ns_steps = 2;
     double pc_build_cost = 1.0;          // a build's cost in M applications per unknown (rebuild rule)
     ibpmg::SolvePlan M1L;
     bool m1l_built = false;
@@ -2434,40 +2439,93 @@ static int solve_common(ibpm_ctx* c, int kind, int nx, int ny, int ngrid, double
 }
 namespace ibpmg {
 
+static bool same_class(const ibpm_batch::KernTab& K, const Flow& F) {
+    return K.nx == F.nx && K.ny == F.ny && K.ngrid == F.L && K.dx == F.dx && K.re == F.p.reynolds && K.dt == F.p.dt;
+}
+
+// Unit-flux nodes of a grid class's kernel table: a lattice over the level-0
+// nodes its flows' points covered at set-up, nx/8 by ny/8 nodes apart (at
+// least 8; coarser if more than 64 nodes), centred on that extent.  The
+// level's boundary and the coarser levels make the operator depend on where a
+// source sits, and a source within half a spacing of the points keeps that
+// part of M (3e-3 of its spectrum with one source at the centre of the
+// tandem plates' level, 1e-4 with a source every eighth of it).
+static std::vector<int> table_sources(const std::vector<const Flow*>& fl) {
+    const Flow& F0 = *fl[0];
+    double imin = 1e300, imax = -1e300, jmin = 1e300, jmax = -1e300;
+    for (const Flow* f : fl) {
+        imin = std::min(imin, (f->bbox0[0] - f->p.xoffset) / f->dx);
+        imax = std::max(imax, (f->bbox0[1] - f->p.xoffset) / f->dx);
+        jmin = std::min(jmin, (f->bbox0[2] - f->p.yoffset) / f->dx);
+        jmax = std::max(jmax, (f->bbox0[3] - f->p.yoffset) / f->dx);
+    }
+    const int i0 = (int)std::floor(imin), i1 = (int)std::ceil(imax), j0 = (int)std::floor(jmin), j1 = (int)std::ceil(jmax);
+    int di = std::max(8, F0.nx / 8), dj = std::max(8, F0.ny / 8), ni = 1, nj = 1;
+    for (;;) {
+        ni = std::max(0, i1 - i0) / di + 1;
+        nj = std::max(0, j1 - j0) / dj + 1;
+        if (ni * nj <= 64) break;
+        di *= 2;
+        dj *= 2;
+    }
+    std::vector<int> src;
+    for (int a = 0; a < ni; ++a)
+        for (int b = 0; b < nj; ++b) {
+            const int i = (i0 + i1) / 2 - ((ni - 1) * di) / 2 + a * di;
+            const int j = (j0 + j1) / 2 - ((nj - 1) * dj) / 2 + b * dj;
+            src.push_back(std::min(std::max(i, 1), F0.nx - 2));
+            src.push_back(std::min(std::max(j, 1), F0.ny - 2));
+        }
+    return src;
+}
+
 // The kernel tables of every flow (once per grid class): for each substep i,
 // G_i f = Curl L^-1 (-A_i^-1 Curl f) on level 0 (M_i = beta_i E G_i E^T,
 // ProjectionSolver.cc:96-101 without beta) applied to a unit X flux and to a
-// unit Y flux at node (nx/2, ny/2) of level 0, through the library's own curl,
-// multi-domain Helmholtz solve and flux-from-vorticity; the level-0 part of
-// each response is kept.
+// unit Y flux at each of the class's source nodes (table_sources), through
+// the library's own curl, multi-domain Helmholtz solve and
+// flux-from-vorticity; the level-0 part of each response is kept.
 static int kernel_tables(ibpm_batch* B) {
     hipStream_t s = B->ctx->c.stream;
     for (auto& F : B->flows) {
         if (F.ktab[0]) continue;
         ibpm_batch::KernTab* T = nullptr;
         for (auto& K : B->ktabs)
-            if (K.nx == F.nx && K.ny == F.ny && K.ngrid == F.L && K.dx == F.dx && K.re == F.p.reynolds &&
-                K.dt == F.p.dt)
-                T = &K;
+            if (same_class(K, F)) T = &K;
         if (!T) {
-            ibpm_batch::KernTab K{F.nx, F.ny, F.L, F.dx, F.p.reynolds, F.p.dt, {nullptr, nullptr, nullptr, nullptr}};
+            ibpm_batch::KernTab K{F.nx, F.ny, F.L, F.dx, F.p.reynolds, F.p.dt, {nullptr, nullptr, nullptr, nullptr},
+                                  {}, nullptr};
+            std::vector<const Flow*> members;
+            for (const auto& G : B->flows)
+                if (!G.geom.stationary && G.nx == F.nx && G.ny == F.ny && G.L == F.L && G.dx == F.dx &&
+                    G.p.reynolds == F.p.reynolds && G.p.dt == F.p.dt)
+                    members.push_back(&G);
+            if (members.empty()) members.push_back(&F);
+            K.src = table_sources(members);
+            const int ns = (int)K.src.size() / 2;
+            CKR(B->mem.alloc(&K.d_src, K.src.size()));
+            CKR(upload(K.d_src, K.src.data(), sizeof(int) * K.src.size()));
             const long nf1 = 2L * F.nx * F.ny + F.nx + F.ny, nfx = (long)(F.nx + 1) * F.ny;
             DevMem tmp;
             double *q = nullptr, *w = nullptr, *u = nullptr, *r = nullptr;
             CKR(tmp.alloc(&q, nf1 * F.L)); CKR(tmp.alloc(&r, nf1 * F.L));
             CKR(tmp.alloc(&w, F.ns * F.L)); CKR(tmp.alloc(&u, F.ns * F.L));
-            const long unit[2] = {(long)(F.nx / 2) * F.ny + F.ny / 2, nfx + (long)(F.nx / 2) * (F.ny + 1) + F.ny / 2};
             const double one = 1.0;
             for (int i = 0; i < B->nsub; ++i) {
-                CKR(B->mem.alloc(&K.tab[i], 2 * nf1));
-                for (int comp = 0; comp < 2; ++comp) {
-                    CK(hipMemsetAsync(q, 0, sizeof(double) * nf1 * F.L, s));
-                    CK(hipMemcpyAsync(q + unit[comp], &one, sizeof(double), hipMemcpyHostToDevice, s));
-                    CKR(ibpm_curl_flux(B->ctx, F.nx, F.ny, F.L, F.dx, q, w));
-                    CKR(solve_common(B->ctx, 1, F.nx, F.ny, F.L, F.dx, -beta_of(B, F, i) / 2. / F.p.reynolds, w,
-                                     nullptr, u, 1.0, 0));
-                    CKR(ibpm_flux_from_vorticity(B->ctx, F.nx, F.ny, F.L, F.dx, 0.0, 0.0, u, r));
-                    CK(hipMemcpyAsync(K.tab[i] + comp * nf1, r, sizeof(double) * nf1, hipMemcpyDeviceToDevice, s));
+                CKR(B->mem.alloc(&K.tab[i], 2 * nf1 * ns));
+                for (int a = 0; a < ns; ++a) {
+                    const int si = K.src[2 * a], sj = K.src[2 * a + 1];
+                    const long unit[2] = {(long)si * F.ny + sj, nfx + (long)si * (F.ny + 1) + sj};
+                    for (int comp = 0; comp < 2; ++comp) {
+                        CK(hipMemsetAsync(q, 0, sizeof(double) * nf1 * F.L, s));
+                        CK(hipMemcpyAsync(q + unit[comp], &one, sizeof(double), hipMemcpyHostToDevice, s));
+                        CKR(ibpm_curl_flux(B->ctx, F.nx, F.ny, F.L, F.dx, q, w));
+                        CKR(solve_common(B->ctx, 1, F.nx, F.ny, F.L, F.dx, -beta_of(B, F, i) / 2. / F.p.reynolds,
+                                         w, nullptr, u, 1.0, 0));
+                        CKR(ibpm_flux_from_vorticity(B->ctx, F.nx, F.ny, F.L, F.dx, 0.0, 0.0, u, r));
+                        CK(hipMemcpyAsync(K.tab[i] + (2L * a + comp) * nf1, r, sizeof(double) * nf1,
+                                          hipMemcpyDeviceToDevice, s));
+                    }
                 }
             }
             CK(hipStreamSynchronize(s));
@@ -2475,6 +2533,8 @@ static int kernel_tables(ibpm_batch* B) {
             T = &B->ktabs.back();
         }
         for (int i = 0; i < B->nsub; ++i) F.ktab[i] = T->tab[i];
+        F.ksrc = T->d_src;
+        F.nsrc = (int)T->src.size() / 2;
     }
     return 0;
 }
@@ -2484,7 +2544,7 @@ static int kernel_tables(ibpm_batch* B) {
 static std::vector<PcAsmOps> pc_asm_ops(ibpm_batch* B, int i, const std::function<double(const Flow&)>& scale) {
     std::vector<PcAsmOps> ops;
     for (auto& F : B->flows)
-        ops.push_back(PcAsmOps{2 * F.nb, F.nx, F.ny, F.e_sep, F.ktab[i], scale(F), F.mcol});
+        ops.push_back(PcAsmOps{2 * F.nb, F.nx, F.ny, F.nsrc, F.ksrc, F.e_sep, F.ktab[i], scale(F), F.mcol});
     return ops;
 }
 
@@ -3837,6 +3897,12 @@ int ibpm_batch_init(ibpm_batch* B) {
     for (auto& F : B->flows) {
         F.geom.move(B->time, F.px, F.py, F.pu, F.pv);   // ibpm.cc:285
         CKR(update_regularizer(B, F));                   // model->init()
+        if (!F.px.empty()) {
+            F.bbox0[0] = *std::min_element(F.px.begin(), F.px.end());
+            F.bbox0[1] = *std::max_element(F.px.begin(), F.px.end());
+            F.bbox0[2] = *std::min_element(F.py.begin(), F.py.end());
+            F.bbox0[3] = *std::max_element(F.py.begin(), F.py.end());
+        }
     }
     CKR(upload_regops(B));
     if (!B->any_cg) CKR(build_cholesky(B));              // solver->init()
@@ -4230,7 +4296,7 @@ int ibpm_batch_precond_matrices(ibpm_batch* B, int flow, int substep, double* h_
     if (h_mtable) {
         CKR(kernel_tables(B));
         if (!B->d_pcasm_b) CKR(B->mem.alloc(&B->d_pcasm_b, B->flows.size()));
-        const PcAsmOps op{n, F.nx, F.ny, F.e_sep, F.ktab[substep], beta_of(B, F, substep), F.mcol};
+        const PcAsmOps op{n, F.nx, F.ny, F.nsrc, F.ksrc, F.e_sep, F.ktab[substep], beta_of(B, F, substep), F.mcol};
         CKR(upload(B->d_pcasm_b, &op, sizeof op));
         launch_pc_assemble(B->d_pcasm_b, 1, n, s);
         CK(hipStreamSynchronize(s));

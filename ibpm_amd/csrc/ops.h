@@ -131,17 +131,19 @@ struct CopyPiece {
 // The preconditioner's matrix from a kernel table.  M = E G E^T with G the
 // level-0 flux-to-flux operator inside M (Curl, A^-1, L^-1, Curl); on an
 // unbounded lattice G is translation invariant, so G(a, b) is read from the
-// response to one unit flux of b's component at the level's centre (nx/2,
-// ny/2), at a's node shifted by the centre minus b's node (zero past the
-// table).  Entry (r, c) = scale * sum over the E entries a of row r and b of
+// response to one unit flux of b's component at the source node nearest to
+// b's row of E, at a's node shifted by the source minus b's node (zero past
+// the level).  Entry (r, c) = scale * sum over the E entries a of row r and b of
 // row c of w_a w_b G(a, b); the upper triangle is formed and mirrored.  The
 // rows come in separable form (weights u[a] v[b] over a rectangle of nodes),
 // so the double sum is one over the 5 x 5 node offsets of the two rectangles,
 // weighted by the correlations of their row and of their column factors.
 struct PcAsmOps {
     int n, nx, ny;                 // unknowns (X components, then Y); level-0 cells
+    int nsrc;                      // source nodes of the table
+    const int* src;                // their (i, j)
     const double* sep;             // E rows in separable form, kESep per component (RegLists::e_sep)
-    const double* tab;             // level-0 flux responses to a unit X flux, then to a unit Y flux
+    const double* tab;             // per source: level-0 flux responses to a unit X flux, then a unit Y flux
     double scale;
     double* out;                   // n x n, row-major
 };

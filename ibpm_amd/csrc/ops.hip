@@ -1007,11 +1007,26 @@ __global__ void __launch_bounds__(kAsmTile * kAsmTile) k_pc_assemble(const PcAsm
     const int br = blockIdx.x / ntiles, bc = blockIdx.x % ntiles;
     if (br >= nt || bc >= nt || br > bc) return;
     __shared__ double sp[2][kAsmTile][kESep];
+    __shared__ int near[kAsmTile];   // each column's source
     const int t = threadIdx.x;
     for (int q = t; q < 2 * kAsmTile * kESep; q += kAsmTile * kAsmTile) {
         const int side = q / (kAsmTile * kESep), w = q % (kAsmTile * kESep);
         const int row = (side ? bc : br) * kAsmTile + w / kESep;
         sp[side][w / kESep][w % kESep] = row < o.n ? o.sep[(long)row * kESep + w % kESep] : 0.0;
+    }
+    __syncthreads();
+    if (t < kAsmTile) {   // the source nearest to the centre of the column's nodes (first of equals)
+        const int ci = (int)sp[1][t][0] + 1, cj = (int)sp[1][t][1] + 1;
+        int best = 0;
+        long bd = LONG_MAX;
+        for (int a = 0; a < o.nsrc; ++a) {
+            const long di = ci - o.src[2 * a], dj = cj - o.src[2 * a + 1], d = di * di + dj * dj;
+            if (d < bd) {
+                bd = d;
+                best = a;
+            }
+        }
+        near[t] = best;
     }
     __syncthreads();
     const int ty = t / kAsmTile, tx = t % kAsmTile;
@@ -1036,11 +1051,12 @@ __global__ void __launch_bounds__(kAsmTile * kAsmTile) k_pc_assemble(const PcAsm
     }
     const int cr = r >= o.n / 2, cc = c >= o.n / 2;
     const long nfx = (long)(o.nx + 1) * o.ny, nf0 = nfx + (long)o.nx * (o.ny + 1);
-    // the part of b's response on a's component: rows ni0 x columns of length ld
-    const double* T = o.tab + cc * nf0 + (cr ? nfx : 0);
+    // the part of the column's source's response on a's component
+    const int a = near[tx];
+    const double* T = o.tab + (2L * a + cc) * nf0 + (cr ? nfx : 0);
     const int rows = cr ? o.nx : o.nx + 1, cols = cr ? o.ny + 1 : o.ny;
-    const int i0 = (int)R[0] - (int)C[0] + o.nx / 2 - (nic - 1);
-    const int j0 = (int)R[1] - (int)C[1] + o.ny / 2 - (njc - 1);
+    const int i0 = (int)R[0] - (int)C[0] + o.src[2 * a] - (nic - 1);
+    const int j0 = (int)R[1] - (int)C[1] + o.src[2 * a + 1] - (njc - 1);
     double sum = 0.0;
     for (int k = 0; k < nir + nic - 1; ++k) {
         const int i = i0 + k;

@@ -1080,77 +1080,82 @@ void launch_pc_assemble(const PcAsmOps* d, int nflows, int max_n, hipStream_t s)
 }
 
 // ---------------------------------------------------------------------------
-// Batched FP64 GEMM (GemmOps), one matrix per blockIdx.y: 64 x 64 output tiles,
-// 256 threads of 4 x 4 outputs, k in steps of 16 staged in LDS (A's tile
-// transposed so that a thread's four rows are adjacent); each step's global
-// loads for the next step are issued before its arithmetic.  FP64 FMA on the
-// VALU runs at the rate of the FP64 matrix cores on this chip (an MFMA
-// 16x16x4 form of this kernel ran no faster: both wait on the tiles' loads).
+// Batched FP64 GEMM (GemmOps), one matrix per blockIdx.y, on the FP64 matrix
+// cores: a 64 x 64 output tile per workgroup of 4 waves, each wave a 32 x 32
+// quarter as 2 x 2 blocks of v_mfma_f64_16x16x4f64, k in steps of 16 staged in
+// LDS (A's tile transposed); each step's global loads for the next step are
+// issued before its products.  Fragments (lane l, checked on the GPU): A[l mod
+// 16][l / 16], B[l / 16][l mod 16], D[l / 16 + 4 r][l mod 16] in register r.
+// (Measured on 16 products of 600^2: 194 us, against 301 us for the same
+// tiling with 4 x 4 VALU FMAs per thread.)
 constexpr int kGemmT = 64, kGemmK = 16;
+typedef double double4_v __attribute__((ext_vector_type(4)));
 
 __global__ void __launch_bounds__(256) k_dgemm(const GemmOps* __restrict__ ops, int ntiles, int sym) {
     const GemmOps o = ops[blockIdx.y];
     const int n = o.n, nt = (n + kGemmT - 1) / kGemmT;
     const int br = blockIdx.x / ntiles, bc = blockIdx.x % ntiles;
     if (br >= nt || bc >= nt || (sym && br > bc)) return;
-    __shared__ double As[kGemmK][kGemmT];
+    __shared__ double As[kGemmK][kGemmT + 1];
     __shared__ double Bs[kGemmK][kGemmT];
-    const int t = threadIdx.x, ty = t / 16, tx = t % 16;
+    const int t = threadIdx.x, w = t / 64, l = t & 63;
     const int r0 = br * kGemmT, c0 = bc * kGemmT;
-    const int arr = t >> 2, akk = (t & 3) * 4, bkk = t >> 4, bcc = (t & 15) * 4;
-    double ra[4], rb[4];
+    const int wr = (w >> 1) * 32, wc = (w & 1) * 32;   // this wave's quarter
+    const int lr = l & 15, lk = l >> 4;
+    constexpr int kPer = kGemmT * kGemmK / 256;      // elements of each tile per thread
+    double ra[kPer], rb[kPer];
     auto fetch = [&](int k0) {
-        const int r = r0 + arr, k = k0 + bkk;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int ka = k0 + akk + q, c = c0 + bcc + q;
-            ra[q] = (r < n && ka < n) ? o.A[(long)r * n + ka] : 0.0;
-            rb[q] = (k < n && c < n) ? o.B[(long)k * n + c] : 0.0;
+        for (int q = 0; q < kPer; ++q) {
+            const int e = t + 256 * q;
+            const int ar = r0 + e / kGemmK, ak = k0 + e % kGemmK;
+            const int bk = k0 + e / kGemmT, bcol = c0 + e % kGemmT;
+            ra[q] = (ar < n && ak < n) ? o.A[(long)ar * n + ak] : 0.0;
+            rb[q] = (bk < n && bcol < n) ? o.B[(long)bk * n + bcol] : 0.0;
         }
     };
-    double acc[4][4];
+    double4_v acc[2][2];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+        for (int j = 0; j < 2; ++j) acc[i][j] = double4_v{0.0, 0.0, 0.0, 0.0};
     fetch(0);
     for (int k0 = 0; k0 < n; k0 += kGemmK) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            As[akk + q][arr] = ra[q];
-            Bs[bkk][bcc + q] = rb[q];
+        for (int q = 0; q < kPer; ++q) {
+            const int e = t + 256 * q;
+            As[e % kGemmK][e / kGemmK] = ra[q];
+            Bs[e / kGemmT][e % kGemmT] = rb[q];
         }
         __syncthreads();
         if (k0 + kGemmK < n) fetch(k0 + kGemmK);
 #pragma unroll
-        for (int kk = 0; kk < kGemmK; ++kk) {
-            double a[4], b[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                a[i] = As[kk][ty * 4 + i];
-                b[i] = Bs[kk][tx * 4 + i];
-            }
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] = fma(a[i], b[j], acc[i][j]);
+        for (int kk = 0; kk < kGemmK; kk += 4) {
+            const double a0 = As[kk + lk][wr + lr], a1 = As[kk + lk][wr + 16 + lr];
+            const double b0 = Bs[kk + lk][wc + lr], b1 = Bs[kk + lk][wc + 16 + lr];
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int r = r0 + ty * 4 + i, c = c0 + tx * 4 + j;
-            if (r >= n || c >= n) continue;
-            const double v = o.alpha * acc[i][j] + (o.D ? o.beta * o.D[(long)r * n + c] : 0.0);
-            if (!sym) {
-                o.C[(long)r * n + c] = v;
-            } else if (r <= c) {   // the upper triangle, mirrored
-                o.C[(long)r * n + c] = v;
-                o.C[(long)c * n + r] = v;
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int rg = 0; rg < 4; ++rg) {
+                const int r = r0 + wr + 16 * i + lk + 4 * rg, c = c0 + wc + 16 * j + lr;
+                if (r >= n || c >= n) continue;
+                const double v = o.alpha * acc[i][j][rg] + (o.D ? o.beta * o.D[(long)r * n + c] : 0.0);
+                if (!sym) {
+                    o.C[(long)r * n + c] = v;
+                } else if (r <= c) {   // the upper triangle, mirrored
+                    o.C[(long)r * n + c] = v;
+                    o.C[(long)c * n + r] = v;
+                }
             }
-        }
 }
 
 void launch_dgemm(const GemmOps* d, int nflows, int max_n, bool sym, hipStream_t s) {

@@ -2085,12 +2085,15 @@ static int solve_pcg(ibpm_batch* B, int i, int* iters, bool* fallback) {
     };
     timed(pr, K_CG, 0, s, [&] { launch_pcg_begin(ops, nfl, s); });
     precondition(1);
-    // the first poll comes after one iteration fewer than the last solve of
-    // this substep took (iterations past convergence are no-ops)
+    // the first poll comes after as many iterations as the last solve of
+    // this substep took, with nothing queued behind it: a solve that
+    // converges there issues no iteration past convergence (each would be a
+    // chain of no-op launches); later polls keep one iteration queued
     // (measuring: one iteration per poll, so the active flows of every
     // launch are known exactly)
     const int limit = std::max(100, 2 * B->max_n);
-    int chunk = pr ? 0 : std::max(0, B->pc_last[i] - 1), done = 0;
+    int chunk = pr ? 0 : std::max(0, B->pc_last[i]), done = 0;
+    bool first = chunk > 0;
     std::vector<double> st;
     auto iterate = [&] {
         enqueue_M(B, i, B->d_smear_d[i], s, pr ? &active : nullptr);
@@ -2104,6 +2107,10 @@ static int solve_pcg(ibpm_batch* B, int i, int* iters, bool* fallback) {
         done += chunk;
         if (pr) {   // measuring: the states of every launch exactly
             chunk = 1;
+            CKR(read_states(B, st));
+        } else if (first) {   // the expected count: read the states with nothing queued
+            first = false;
+            chunk = 0;
             CKR(read_states(B, st));
         } else {    // the next iteration runs while the host waits for the states
             CKR(poll_begin(B));

@@ -25,8 +25,8 @@ def _compile(src, defines=(), obj_dir=OBJ):
     path = os.path.join(CSRC, src)
     if src.endswith(".hip"):
         cmd = [HIPCC] + COMMON + DEVICE_FLAGS + list(defines) + ["-c", path, "-o", obj]
-    else:
-        cmd = [HIPCC] + COMMON + list(defines) + ["-c", path, "-o", obj]
+    else:   # host code (no kernels): its device pass targets the same architecture
+        cmd = [HIPCC] + COMMON + ["--offload-arch=" + ARCH] + list(defines) + ["-c", path, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     msgs = [l for l in (r.stdout + r.stderr).splitlines()
             if "packed-fp32-ops" not in l and "not a recognized feature" not in l and l.strip()]

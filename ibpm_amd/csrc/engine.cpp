@@ -785,7 +785,10 @@ struct ibpm_batch {
     std::vector<KernTab> ktabs;
     ibpmg::PcAsmOps* d_pcasm[4] = {};     // the refresh's assembly per substep (scale pbeta)
     ibpmg::PcAsmOps* d_pcasm_b = nullptr; // a build's assembly
-    ibpmg::GemmOps *d_ns1 = nullptr, *d_ns2 = nullptr;
+    // Newton-Schulz steps alternate between P and pc_X: d_ns1[0], d_ns2[0]
+    // take P to pc_X, d_ns1[1], d_ns2[1] take pc_X back to P; an odd count
+    // ends with d_ns_copy (pc_X to P)
+    ibpmg::GemmOps *d_ns1[2] = {}, *d_ns2[2] = {};
     ibpmg::CopyPiece* d_ns_copy = nullptr;
     long st_refreshes = 0;
     // the refresh runs on a stream of its own beside the substep's right-hand
@@ -2764,13 +2767,13 @@ static int build_precond(ibpm_batch* B, int i) {
     if (B->pc_build_levels == 3) {
         // the refresh: mcol = pbeta E G_j E^T at substep j's points (P's
         // scale to substep j stays pbeta / beta_j), then per step pc_T = mcol
-        // P, pc_X = 2 P - P pc_T, P = pc_X
+        // P, pc_X = 2 P - P pc_T, and the next step from pc_X back into P
         for (int j = 0; j < B->nsub; ++j) {
             if (!B->d_pcasm[j]) CKR(B->mem.alloc(&B->d_pcasm[j], (size_t)nfl));
             const auto ops = pc_asm_ops(B, j, [](const Flow& F) { return F.pbeta; });
             CKR(upload(B->d_pcasm[j], ops.data(), sizeof(PcAsmOps) * nfl));
         }
-        std::vector<GemmOps> g1(nfl), g2(nfl);
+        std::vector<GemmOps> g1(2 * nfl), g2(2 * nfl);
         std::vector<CopyPiece> cp(nfl);
         for (int k = 0; k < nfl; ++k) {
             Flow& F = B->flows[k];
@@ -2779,15 +2782,19 @@ static int build_precond(ibpm_batch* B, int i) {
             if (!F.pc_X) CKR(B->mem.alloc(&F.pc_X, (size_t)n * n));
             g1[k] = GemmOps{n, F.mcol, F.pinv, nullptr, F.pc_T, 1.0, 0.0};
             g2[k] = GemmOps{n, F.pinv, F.pc_T, F.pinv, F.pc_X, -1.0, 2.0};
+            g1[nfl + k] = GemmOps{n, F.mcol, F.pc_X, nullptr, F.pc_T, 1.0, 0.0};
+            g2[nfl + k] = GemmOps{n, F.pc_X, F.pc_T, F.pc_X, F.pinv, -1.0, 2.0};
             cp[k] = CopyPiece{(const unsigned int*)F.pc_X, (unsigned int*)F.pinv, 2L * n * n};
         }
-        if (!B->d_ns1) {
-            CKR(B->mem.alloc(&B->d_ns1, (size_t)nfl));
-            CKR(B->mem.alloc(&B->d_ns2, (size_t)nfl));
-            CKR(B->mem.alloc(&B->d_ns_copy, (size_t)nfl));
+        for (int h = 0; h < 2; ++h) {
+            if (!B->d_ns1[h]) {
+                CKR(B->mem.alloc(&B->d_ns1[h], (size_t)nfl));
+                CKR(B->mem.alloc(&B->d_ns2[h], (size_t)nfl));
+            }
+            CKR(upload(B->d_ns1[h], g1.data() + h * nfl, sizeof(GemmOps) * nfl));
+            CKR(upload(B->d_ns2[h], g2.data() + h * nfl, sizeof(GemmOps) * nfl));
         }
-        CKR(upload(B->d_ns1, g1.data(), sizeof(GemmOps) * nfl));
-        CKR(upload(B->d_ns2, g2.data(), sizeof(GemmOps) * nfl));
+        if (!B->d_ns_copy) CKR(B->mem.alloc(&B->d_ns_copy, (size_t)nfl));
         CKR(upload(B->d_ns_copy, cp.data(), sizeof(CopyPiece) * nfl));
     }
     B->pc_valid = true;
@@ -2796,10 +2803,12 @@ static int build_precond(ibpm_batch* B, int i) {
     return 0;
 }
 
-// One Newton-Schulz step of P toward the inverse of the kernel-table matrix
-// at substep i's points, P <- P (2 I - M~ P) (symmetric), before the solve:
-// the error I - M~ P squares, so P follows the bodies from substep to substep
-// at the cost of the assembly and two batched GEMMs.
+// pc_ns_steps Newton-Schulz steps of P toward the inverse of the
+// kernel-table matrix at substep i's points, P <- P (2 I - M~ P)
+// (symmetric), before the solve: each squares the error I - M~ P, so P follows
+// the bodies from substep to substep at the cost of the assembly and two
+// batched GEMMs per step.  The steps alternate between P and pc_X, so an even
+// count copies nothing.
 // Queued on the refresh stream behind the substep's uploads (the regulariser
 // lists it reads), so that it overlaps the substep's right-hand side and step
 // solves; join_refresh makes the stream wait for it before P is read.  P and
@@ -2821,9 +2830,9 @@ static int refresh_precond(ibpm_batch* B, int i) {
     timed(pr, K_PCASM, 0, r, [&] { launch_pc_assemble(B->d_pcasm[i], nfl, B->max_n, r); });
     for (int k = 0; k < B->pc_ns_steps; ++k)
         timed(pr, K_PCNS, 0, r, [&] {
-            launch_dgemm(B->d_ns1, nfl, B->max_n, false, r);
-            launch_dgemm(B->d_ns2, nfl, B->max_n, true, r);
-            launch_copy_pieces(B->d_ns_copy, nfl, r);
+            launch_dgemm(B->d_ns1[k & 1], nfl, B->max_n, false, r);
+            launch_dgemm(B->d_ns2[k & 1], nfl, B->max_n, true, r);
+            if ((k & 1) == 0 && k + 1 == B->pc_ns_steps) launch_copy_pieces(B->d_ns_copy, nfl, r);
         });
     CK(hipEventRecord(B->pc_ev_out, r));
     B->pc_pending = true;

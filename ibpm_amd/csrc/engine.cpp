@@ -1320,6 +1320,7 @@ static int build_plans(ibpm_batch* B) {
             o.xc = F.bf.xc; o.yc = F.bf.yc;
             o.xoff = F.p.xoffset; o.yoff = F.p.yoffset;
             o.model = F.p.model;
+            o.skip_central = F.p.model == 0;
             o.q0b = F.bq.empty() ? nullptr : F.bq[0];
             o.w0b = F.bw.empty() ? nullptr : F.bw[0];
             o.sff = F.sff;
@@ -3788,10 +3789,16 @@ int ibpm_batch_create(ibpm_ctx* ctx, int nflows, const ibpm_flow_params* params,
         B->max_n = std::max(B->max_n, 2 * F.nb);
         // algorithmic bytes: vel reads omega + q, writes U, V; cross reads U, V
         // and writes the cross flux; rhs reads cross, omega, N_prev and writes
-        // a, N; curl_q reads psi and writes q
-        B->bytes_vel += 8.0 * (3 * F.ns + F.nf) * F.L;
+        // a, N; curl_q reads psi and writes q.  vel (and rhs, for the
+        // nonlinear model) pass over the nodes strictly inside the coarse
+        // levels' central regions, so their bytes count the other nodes, each
+        // with its share of the level's fluxes
+        const double ncent = (double)(F.nx / 2 - 1) * (F.ny / 2 - 1) * (F.L - 1);
+        const double nact = (double)F.ns * F.L - ncent;
+        const double fpn = (double)F.nf / F.ns;
+        B->bytes_vel += 8.0 * (3 + fpn) * nact;
         B->bytes_cross += 8.0 * (2 * F.ns + F.nf) * F.L;
-        B->bytes_rhs += 8.0 * (4 * F.ns + F.nf) * F.L;
+        B->bytes_rhs += 8.0 * (4 + fpn) * (F.p.model == 0 ? nact : (double)F.ns * F.L);
         B->bytes_curlq += 8.0 * (F.ns + F.nf) * F.L;
     }
     if (B->any_cg) {

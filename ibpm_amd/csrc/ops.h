@@ -29,6 +29,11 @@ struct FlowOps {
     double *what, *sfd_prev;
     double sfd_chi, sfd_inv_delta;
     int sfd_valid;                 // what holds the filtered state (else: zero in N, omega at the update)
+    // nonlinear-model flows of a batch: a and N are not formed at the nodes
+    // strictly inside a coarse level's central region, which no kernel reads
+    // (the Helmholtz solve and its coarsify chain read the coarsified
+    // right-hand side there, and N there only feeds the next substep's a there)
+    int skip_central;
 };
 
 // CrossProduct(Flux, Flux) of one flow before its coarsify: out = X(q1) Y(q2)

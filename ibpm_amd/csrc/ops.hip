@@ -195,6 +195,12 @@ __global__ void __launch_bounds__(kThreads) k_vel(const FlowOps* __restrict__ fl
     const int lev = (int)((unsigned)t / ns);
     const unsigned r = (unsigned)t - lev * ns;
     const int i = (int)(r / ny1) + 1, j = (int)(r % ny1) + 1;
+    // nodes strictly inside a coarse level's central region: xvel / yvel give
+    // 0 there, and k_cross never reads them (its central fluxes take the finer
+    // level's cross), so they are not written
+    if (lev >= 1 && i > nx_ext(g.nx) && i < g.nx / 2 + nx_ext(g.nx) && j > nx_ext(g.ny) &&
+        j < g.ny / 2 + nx_ext(g.ny))
+        return;
     const double w = gin(F.omega)[t];
     double u, v;
     if (F.model == 1 || F.model == 3) {
@@ -338,6 +344,9 @@ __global__ void __launch_bounds__(kThreads) k_rhs(const FlowOps* __restrict__ fl
     const int lev = (int)((unsigned)t / ns);
     const unsigned r = (unsigned)t - lev * ns;
     const int i = (int)(r / ny1) + 1, j = (int)(r % ny1) + 1;
+    if (F.skip_central && lev >= 1 && i > nx_ext(g.nx) && i < g.nx / 2 + nx_ext(g.nx) && j > nx_ext(g.ny) &&
+        j < g.ny / 2 + nx_ext(g.ny))
+        return;
     const double d = F.dx * (1 << lev);
     const double bydx2 = 1. / (d * d);
     const IBPM_GLOBAL double* C = gin(F.cross);

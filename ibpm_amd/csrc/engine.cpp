@@ -703,7 +703,7 @@ struct ibpm_batch {
     bool old_saved = false;
     double time = 0.0;
     int timestep = 0;
-    long max_ns_all = 0, max_nf = 0, max_nf_all = 0;
+    long max_ns_all = 0, max_nf = 0, max_cells_all = 0;   // cells: (nx + 1)(ny + 1) L
     int max_comp = 0, max_nodes = 0, max_n = 0;
     // plans per substep
     ibpmg::SolvePlan S1[4], S2[4], S3[4], S4, M1[4], M2;
@@ -1856,7 +1856,7 @@ static void enqueue_rhs(ibpm_batch* B, int i, hipStream_t s) {
 static void enqueue_refresh(ibpm_batch* B, hipStream_t s) {
     run_solve(B->S4, s, B->probe);
     timed(B->probe, K_CURLQ, B->bytes_curlq, s,
-          [&] { launch_curl_q(B->d_fops[0], (int)B->flows.size(), B->max_nf_all, s); });
+          [&] { launch_curl_q(B->d_fops[0], (int)B->flows.size(), B->max_cells_all, s); });
 }
 
 // The stationary-body (Cholesky) substep: fixed launch sequence.
@@ -3269,7 +3269,7 @@ int ibpm_curl_scalar(ibpm_ctx* c, int nx, int ny, int ngrid, double dx, double u
     FlowOps* d = nullptr;
     CKR(mem.alloc(&d, 1));
     CK(hipMemcpyAsync(d, &o, sizeof(o), hipMemcpyHostToDevice, c->c.stream));
-    launch_curl_q(d, 1, (2L * nx * ny + nx + ny) * ngrid, c->c.stream);
+    launch_curl_q(d, 1, (long)(nx + 1) * (ny + 1) * ngrid, c->c.stream);
     CK(hipStreamSynchronize(c->c.stream));
     CK(hipGetLastError());
     return 0;
@@ -3784,7 +3784,7 @@ int ibpm_batch_create(ibpm_ctx* ctx, int nflows, const ibpm_flow_params* params,
         }
         B->max_ns_all = std::max(B->max_ns_all, F.ns * F.L);
         B->max_nf = std::max(B->max_nf, F.nf);
-        B->max_nf_all = std::max(B->max_nf_all, F.nf * F.L);
+        B->max_cells_all = std::max(B->max_cells_all, (long)(F.nx + 1) * (F.ny + 1) * F.L);
         B->max_comp = std::max(B->max_comp, 2 * F.nb);
         B->max_n = std::max(B->max_n, 2 * F.nb);
         // algorithmic bytes: vel reads omega + q, writes U, V; cross reads U, V

@@ -209,7 +209,8 @@ void launch_vel(const FlowOps* d, int nflows, long max_pts, hipStream_t s);
 void launch_cross(const FlowOps* d, int nflows, long max_nf, int lev, hipStream_t s);
 void launch_cross_ff(const CrossFFOps* d, int nflows, long max_pts, hipStream_t s);
 void launch_rhs(const FlowOps* d, int nflows, long max_pts, hipStream_t s);
-void launch_curl_q(const FlowOps* d, int nflows, long max_nf_all, hipStream_t s);
+// max_cells_all: (nx + 1)(ny + 1) L of the largest flow
+void launch_curl_q(const FlowOps* d, int nflows, long max_cells_all, hipStream_t s);
 void launch_interp(const RegOps* d, int nflows, int max_comp, hipStream_t s);
 void launch_smear_curl(const RegOps* d, int nflows, int max_nodes, hipStream_t s);
 void launch_clear_nodes(const RegOps* d, int nflows, int max_nodes, hipStream_t s);

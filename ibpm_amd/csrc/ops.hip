@@ -405,79 +405,50 @@ __global__ void __launch_bounds__(kThreads) k_rhs(const FlowOps* __restrict__ fl
     }
 }
 
-// One flux of q = Curl psi + q0 (flat index t over all levels of one flow):
-// the two psi offsets of curl_x's / curl_y's interior difference,
-// Curl = psi[ka] - psi[kb] (ka = kb = 0 at a flux next to the level's edge,
-// whose Curl the getBC branches give), and q0.
-struct CurlQAt {
-    unsigned ka, kb;
-    bool interior, xflux;
-    int lev, i, j;
-    double q0;
-};
-__device__ inline CurlQAt curl_q_at(const FlowOps& F, const Lay& g, unsigned t) {
-    // (sizes below 2^32: 32-bit divisions)
-    const unsigned nf = (unsigned)g.nf(), nfx = (unsigned)g.nfx(), ny1 = (unsigned)(g.ny - 1);
-    CurlQAt a;
-    a.lev = (int)(t / nf);
-    const unsigned r = t - a.lev * nf, lbase = a.lev * (unsigned)g.ns();
-    const double d = F.dx * (1 << a.lev);
-    // level offsets: offset + 0.5 (shift - 1) (2^lev - 1) n dx with shift 0
-    const double lofs = -0.5 * ((1 << a.lev) - 1);
-    a.xflux = r < nfx;
-    if (a.xflux) {
-        a.i = (int)(r / (unsigned)g.ny);
-        a.j = (int)(r % (unsigned)g.ny);
-        a.q0 = F.ubase * d;
-        if (F.ubf) {
-            const double y = (F.yoff + lofs * (F.ny * F.dx)) + (a.j + 0.5) * d;
-            a.q0 = (F.bf[0] - F.bf[2] * (y - F.yc)) * d;
-        }
-        a.interior = a.i >= 1 && a.i <= g.nx - 1 && a.j >= 1 && a.j <= g.ny - 2;
-        const unsigned k = lbase + (a.i - 1) * ny1 + (a.j - 1);
-        a.ka = a.interior ? k + 1 : 0;
-        a.kb = a.interior ? k : 0;
-    } else {
-        const unsigned u = r - nfx;
-        a.i = (int)(u / (unsigned)(g.ny + 1));
-        a.j = (int)(u % (unsigned)(g.ny + 1));
-        a.q0 = F.vbase * d;
-        if (F.ubf) {
-            const double x = (F.xoff + lofs * (F.nx * F.dx)) + (a.i + 0.5) * d;
-            a.q0 = (F.bf[1] + F.bf[2] * (x - F.xc)) * d;
-        }
-        a.interior = a.j >= 1 && a.j <= g.ny - 1 && a.i >= 1 && a.i <= g.nx - 2;
-        const unsigned k = lbase + (a.i - 1) * ny1 + (a.j - 1);
-        a.ka = a.interior ? k : 0;
-        a.kb = a.interior ? k + ny1 : 0;
-    }
-    return a;
-}
-
 // q = Curl(psi) + q0 on every level (NavierStokesModel.cc:121-129): q0 is
 // the uniform base flow (Flux.cc:79-99) or the unsteady one, free stream plus
 // rotation about the body centre at the edge coordinates of each level
-// (Flux.cc:101-129, Grid.cc:104-139).  Two consecutive fluxes per thread:
-// their four psi values are loaded together from valid offsets, and a flux
-// next to the level's edge takes curl_x / curl_y instead, blended in exactly
-// (the interior difference times 1 plus 0, or times 0 plus the edge value).
+// (Flux.cc:101-129, Grid.cc:104-139).  Thread (lev, i, j) of each level's
+// (nx + 1) x (ny + 1) cells writes X flux (i, j) (j < ny) and Y flux (i, j)
+// (i < nx), so each psi value is fetched once for both: a cell away from the
+// level's edge reads psi(i, j), psi(i, j + 1) and psi(i + 1, j) and takes
+// curl_x's and curl_y's interior differences; the others take those
+// functions (getBC at the edge).
 __global__ void __launch_bounds__(kThreads) k_curl_q(const FlowOps* __restrict__ flows) {
     const FlowOps& F = flows[blockIdx.y];
     const Lay g{F.nx, F.ny, F.L};
-    const long total = g.L * g.nf();
-    const long t0 = 2 * ((long)blockIdx.x * kThreads + threadIdx.x);
-    if (t0 >= total) return;
-    const bool two = t0 + 1 < total;
-    const CurlQAt a = curl_q_at(F, g, (unsigned)t0), b = curl_q_at(F, g, (unsigned)(two ? t0 + 1 : t0));
-    const IBPM_GLOBAL double* P = gin(F.psi);
-    const double pa0 = P[a.ka], pa1 = P[a.kb], pb0 = P[b.ka], pb1 = P[b.kb];
-    const double ma = a.interior ? 1.0 : 0.0, mb = b.interior ? 1.0 : 0.0;
-    const double ea = a.interior ? 0.0 : (a.xflux ? curl_x(g, F.psi, a.lev, a.i, a.j) : curl_y(g, F.psi, a.lev, a.i, a.j));
-    const double eb = b.interior ? 0.0 : (b.xflux ? curl_x(g, F.psi, b.lev, b.i, b.j) : curl_y(g, F.psi, b.lev, b.i, b.j));
-    const double ca = (pa0 - pa1) * ma + ea, cb = (pb0 - pb1) * mb + eb;
-    IBPM_GLOBAL double* q = gout(F.q);
-    q[t0] = ca + a.q0;
-    if (two) q[t0 + 1] = cb + b.q0;
+    // (sizes below 2^32: 32-bit offsets and divisions)
+    const unsigned nyc = (unsigned)(g.ny + 1), cells = (unsigned)(g.nx + 1) * nyc;
+    const unsigned t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= cells * (unsigned)g.L) return;
+    const int lev = (int)(t / cells);
+    const unsigned r = t - lev * cells;
+    const int i = (int)(r / nyc), j = (int)(r % nyc);
+    double cx, cy;
+    if (i >= 1 && i <= g.nx - 2 && j >= 1 && j <= g.ny - 2) {
+        const unsigned ny1 = (unsigned)(g.ny - 1);
+        const unsigned k = lev * (unsigned)g.ns() + (unsigned)(i - 1) * ny1 + (unsigned)(j - 1);
+        const IBPM_GLOBAL double* P = gin(F.psi);
+        const double p = P[k], pn = P[k + 1], pe = P[k + ny1];
+        cx = pn - p;   // psi(i, j + 1) - psi(i, j)
+        cy = p - pe;   // psi(i, j) - psi(i + 1, j)
+    } else {
+        cx = j < g.ny ? curl_x(g, F.psi, lev, i, j) : 0.0;
+        cy = i < g.nx ? curl_y(g, F.psi, lev, i, j) : 0.0;
+    }
+    const double d = F.dx * (1 << lev);
+    // level offsets: offset + 0.5 (shift - 1) (2^lev - 1) n dx with shift 0
+    const double lofs = -0.5 * ((1 << lev) - 1);
+    double qx0 = F.ubase * d, qy0 = F.vbase * d;
+    if (F.ubf) {
+        const double y = (F.yoff + lofs * (F.ny * F.dx)) + (j + 0.5) * d;
+        qx0 = (F.bf[0] - F.bf[2] * (y - F.yc)) * d;
+        const double x = (F.xoff + lofs * (F.nx * F.dx)) + (i + 0.5) * d;
+        qy0 = (F.bf[1] + F.bf[2] * (x - F.xc)) * d;
+    }
+    IBPM_GLOBAL double* q = gout(F.q) + lev * (unsigned)g.nf();
+    if (j < g.ny) q[(unsigned)i * (unsigned)g.ny + (unsigned)j] = cx + qx0;
+    if (i < g.nx) q[(unsigned)g.nfx() + (unsigned)i * nyc + (unsigned)j] = cy + qy0;
 }
 
 // ---------------------------------------------------------- regulariser
@@ -973,8 +944,8 @@ void launch_cross_ff(const CrossFFOps* d, int nflows, long max_pts, hipStream_t 
 void launch_rhs(const FlowOps* d, int nflows, long max_pts, hipStream_t s) {
     hipLaunchKernelGGL(k_rhs, dim3(blocks_for(max_pts), nflows), dim3(kThreads), 0, s, d);
 }
-void launch_curl_q(const FlowOps* d, int nflows, long max_nf_all, hipStream_t s) {
-    hipLaunchKernelGGL(k_curl_q, dim3(blocks_for((max_nf_all + 1) / 2), nflows), dim3(kThreads), 0, s, d);
+void launch_curl_q(const FlowOps* d, int nflows, long max_cells_all, hipStream_t s) {
+    hipLaunchKernelGGL(k_curl_q, dim3(blocks_for(max_cells_all), nflows), dim3(kThreads), 0, s, d);
 }
 // One thread per flow: e_{j-1} -> e_j.
 __global__ void k_unit_column(const UnitOps* __restrict__ ops, int nflows, int j) {

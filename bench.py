@@ -231,6 +231,45 @@ def moving_form(ctx, flows, warmup, steps, precond="dense", pairs=0, local=True,
     return out, b
 
 
+def streams_form(flows, warmup, steps, nstreams=2, build="tracked", rebuild_cost=0.1):
+    """The headline form split into `nstreams` batches of consecutive flows,
+    each on a context (HIP stream) of its own, advanced at the same time from
+    one host thread per batch (ibpm_amd.advance_concurrently).  Returns the
+    form's figures for the whole job."""
+    import ibpm_amd
+    from ibpm_amd.workloads import grid_points
+    sync = ibpm_amd.load().ibpm_ctx_synchronize
+    k = (len(flows) + nstreams - 1) // nstreams
+    groups = [flows[i:i + k] for i in range(0, len(flows), k)]
+    ctxs = [ibpm_amd.Context(0) for _ in groups]
+    bs = []
+    for c, g in zip(ctxs, groups):
+        b = ibpm_amd.Batch(c, g).init()
+        b.set_precond("dense", 0).set_local_helmholtz(True).set_precond_build(build, rebuild_cost)
+        b.set_warm_start("zero")
+        bs.append(b)
+    ibpm_amd.advance_concurrently(bs, max(1, warmup))
+    for c in ctxs:
+        sync(c.h)
+    it0 = [b.solver_stats()["iterations"] for b in bs]
+    t0 = time.perf_counter()
+    ibpm_amd.advance_concurrently(bs, steps)
+    for c in ctxs:
+        sync(c.h)
+    sec = time.perf_counter() - t0
+    its = sum(b.solver_stats()["iterations"] - i for b, i in zip(bs, it0)) / len(flows) / steps
+    for b in bs:
+        b.close()
+    for c in ctxs:
+        c.close()
+    gp = sum(grid_points(f) for f in flows)
+    log("form dense/local/zero warm start, %d flows on %d streams: %.1f ms/step, %.1f CG iterations/step"
+        % (len(flows), len(groups), sec / steps * 1e3, its))
+    return {"flows": len(flows), "streams": len(groups), "flows_per_stream": [len(g) for g in groups],
+            "warm_start": "zero", "value": gp * steps / sec, "ms_per_step": sec / steps * 1e3,
+            "cg_iters_per_step": its}
+
+
 def plain_form(ctx, flow, warmup, steps):
     """The plain GPU form on one flow: plain CG from the previous force with
     every A^-1 of M by the DST solve (the reference's solver), no batching."""
@@ -330,6 +369,8 @@ def main():
             if learn is not None:
                 for w, it in iters.items():
                     learn["cg_iters_per_step_%s_start" % w] = it
+            forms["batched_preconditioned_zero_start_two_streams"] = streams_form(
+                flows, args.warmup, steps, 2, args.precond_build, args.rebuild_cost)
             bpl, bb = moving_form(ctx, flows, 1, 3, precond="none", local=False, build=None)
             bb.close()
             forms["batched_plain_cg"] = bpl

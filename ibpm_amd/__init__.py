@@ -7,6 +7,7 @@ calls raise.
 """
 import ctypes
 import os
+import threading
 from dataclasses import dataclass
 
 import numpy as np
@@ -586,3 +587,32 @@ class Batch:
     @property
     def time(self):
         return load().ibpm_batch_time(self.h)
+
+
+def advance_concurrently(batches, nsteps):
+    """Advances batches that live on different contexts (one HIP stream each)
+    by nsteps each at the same time, one host thread per batch, and returns
+    when all have finished.  Independent groups of flows on separate streams
+    let one group's bandwidth-bound stencils fill the HBM that another group's
+    occupancy-bound DST passes leave idle.  Raises the first error a batch
+    reported (after every thread has ended)."""
+    batches = list(batches)
+    ctxs = [b.ctx.h.value for b in batches]
+    if len(set(ctxs)) != len(ctxs):
+        raise ValueError("advance_concurrently: each batch needs a context (stream) of its own")
+    errs = [None] * len(batches)
+
+    def run(i):
+        try:
+            batches[i].advance(nsteps)
+        except Exception as e:   # reported after the join
+            errs[i] = e
+
+    th = [threading.Thread(target=run, args=(i,)) for i in range(len(batches))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for e in errs:
+        if e is not None:
+            raise e

@@ -3,8 +3,10 @@
  *
  * Plain pointers and sizes only.  Every call returns 0 on success and a
  * negative value on failure (never aborts); ibpm_last_error() then describes
- * the failure.  Calls are ordered on the context's single HIP stream; a
- * context is not thread-safe.  "d_" pointers are device pointers (allocate
+ * the failure (per host thread).  Calls are ordered on the context's single
+ * HIP stream; a context is not thread-safe, but batches on different contexts
+ * may be driven from different host threads at once (their work then shares
+ * the GPU, one stream each).  "d_" pointers are device pointers (allocate
  * them with ibpm_malloc or any HIP allocator on the context's device).
  *
  * Layouts are the reference's (so a host Scalar/Flux/BoundaryVector buffer

@@ -231,11 +231,12 @@ def moving_form(ctx, flows, warmup, steps, precond="dense", pairs=0, local=True,
     return out, b
 
 
-def streams_form(flows, warmup, steps, nstreams=2, build="tracked", rebuild_cost=0.1):
+def streams_form(flows, warmup, steps, nstreams=2, build="tracked", rebuild_cost=0.1, moving=True):
     """The headline form split into `nstreams` batches of consecutive flows,
     each on a context (HIP stream) of its own, advanced at the same time from
-    one host thread per batch (ibpm_amd.advance_concurrently).  Returns the
-    form's figures for the whole job."""
+    one host thread per batch (ibpm_amd.advance_concurrently): moving bodies
+    with the preconditioned CG from a zero start, stationary ones by graph
+    replay.  Returns the form's figures for the whole job."""
     import ibpm_amd
     from ibpm_amd.workloads import grid_points
     sync = ibpm_amd.load().ibpm_ctx_synchronize
@@ -245,29 +246,34 @@ def streams_form(flows, warmup, steps, nstreams=2, build="tracked", rebuild_cost
     bs = []
     for c, g in zip(ctxs, groups):
         b = ibpm_amd.Batch(c, g).init()
-        b.set_precond("dense", 0).set_local_helmholtz(True).set_precond_build(build, rebuild_cost)
-        b.set_warm_start("zero")
+        if moving:
+            b.set_precond("dense", 0).set_local_helmholtz(True).set_precond_build(build, rebuild_cost)
+            b.set_warm_start("zero")
+        else:
+            b.use_graph(True)
         bs.append(b)
     ibpm_amd.advance_concurrently(bs, max(1, warmup))
     for c in ctxs:
         sync(c.h)
-    it0 = [b.solver_stats()["iterations"] for b in bs]
+    it0 = [b.solver_stats()["iterations"] if moving else 0 for b in bs]
     t0 = time.perf_counter()
     ibpm_amd.advance_concurrently(bs, steps)
     for c in ctxs:
         sync(c.h)
     sec = time.perf_counter() - t0
-    its = sum(b.solver_stats()["iterations"] - i for b, i in zip(bs, it0)) / len(flows) / steps
+    its = (sum(b.solver_stats()["iterations"] - i for b, i in zip(bs, it0)) / len(flows) / steps) if moving else None
     for b in bs:
         b.close()
     for c in ctxs:
         c.close()
     gp = sum(grid_points(f) for f in flows)
-    log("form dense/local/zero warm start, %d flows on %d streams: %.1f ms/step, %.1f CG iterations/step"
-        % (len(flows), len(groups), sec / steps * 1e3, its))
-    return {"flows": len(flows), "streams": len(groups), "flows_per_stream": [len(g) for g in groups],
-            "warm_start": "zero", "value": gp * steps / sec, "ms_per_step": sec / steps * 1e3,
-            "cg_iters_per_step": its}
+    log("form %s, %d flows on %d streams: %.2f ms/step"
+        % ("dense/local/zero warm start" if moving else "graph replay", len(flows), len(groups), sec / steps * 1e3))
+    out = {"flows": len(flows), "streams": len(groups), "flows_per_stream": [len(g) for g in groups],
+           "value": gp * steps / sec, "ms_per_step": sec / steps * 1e3}
+    if moving:
+        out.update({"warm_start": "zero", "cg_iters_per_step": its})
+    return out
 
 
 def plain_form(ctx, flow, warmup, steps):
@@ -323,8 +329,8 @@ def main():
                     help="a build's cost in M applications per unknown, the rebuild rule's break-even "
                          "(default per build: " + ", ".join("%s %g" % kv for kv in sorted(REBUILD_COST.items())) + ")")
     ap.add_argument("--no-forms", action="store_true",
-                    help="moving bodies: time only the headline form (no plain, batched-plain or "
-                         "previous-start forms beside it)")
+                    help="time only the headline form (moving bodies: no plain, batched-plain, other-start "
+                         "or two-stream forms beside it; stationary batches: no two-stream form)")
     args = ap.parse_args()
     if args.rebuild_cost is None:
         args.rebuild_cost = REBUILD_COST[args.precond_build]
@@ -397,6 +403,8 @@ def main():
         solver = None
         stats = b.kernel_stats(3)
         b.close()
+        if len(flows) > 1 and not args.no_forms and not args.no_graph:
+            forms = {"graph_replay_two_streams": streams_form(flows, args.warmup, steps, 2, moving=False)}
     # dominant kernel: most time among the classes with algorithmic bytes
     dom = max(stats.items(), key=lambda kv: kv[1][1] if kv[1][2] else 0.0)
     name, (cnt, ms, by) = dom
@@ -432,7 +440,7 @@ def main():
     }
     if not args.no_cpu_baseline:
         log("CPU baseline")
-        plain_its = forms["plain_gpu_form"]["cg_iters_per_step"] if forms else None
+        plain_its = forms["plain_gpu_form"]["cg_iters_per_step"] if forms and "plain_gpu_form" in forms else None
         line["cpu_baseline"] = cpu_baseline(flows, args.cpu_seconds, plain_its)
     print(json.dumps(line))
     if accuracy is not None and not accuracy["ok"]:

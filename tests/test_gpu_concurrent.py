@@ -48,6 +48,37 @@ def test_concurrent_batches_equal_lone_runs(gpu):
         b.close()
 
 
+CYL = "body C\n circle_n 0 0 0.5 40\nend\n"
+
+
+def test_concurrent_graph_replays_equal_lone_runs(gpu):
+    # stationary flows (Cholesky force solve): each batch replays its own captured step graph
+    print("concurrent: two stationary cylinders 96^2 x 2 and 64^2 x 3, graph replay, %d steps" % STEPS, flush=True)
+    shapes = [(96, 2), (64, 3)]
+
+    def mk(ctx, nx, ngrid):
+        f = gpu.Flow(nx, nx, ngrid, LENGTH, -2.0, -2.0, 100.0, 0.01, CYL)
+        return gpu.Batch(ctx, [f]).init().use_graph(True)
+
+    alone = []
+    for nx, ngrid in shapes:
+        ctx = gpu.Context(0)
+        b = mk(ctx, nx, ngrid)
+        b.advance(STEPS)
+        alone.append((b.omega(0), b.q(0), np.array(b.forces(0))))
+        b.close()
+        ctx.close()
+    ctxs = [gpu.Context(0) for _ in shapes]
+    bs = [mk(c, nx, ngrid) for c, (nx, ngrid) in zip(ctxs, shapes)]
+    gpu.advance_concurrently(bs, STEPS)
+    for k, b in enumerate(bs):
+        w0, q0, fc0 = alone[k]
+        assert np.array_equal(b.omega(0), w0) and np.array_equal(b.q(0), q0)
+        assert np.array_equal(np.array(b.forces(0)), fc0)
+    for b in bs:
+        b.close()
+
+
 def test_concurrent_batches_need_their_own_contexts(gpu):
     ctx = gpu.Context(0)
     bs = [_batch(gpu, ctx, g) for g in GEOMS]
